@@ -1,0 +1,128 @@
+"""ctypes declaration of the C ABI in include/fastiso.h and the loader of the HIP engine.
+
+The engine library is fastisostasy.jl_amd/lib/libfastiso_hip.so, built in-tree by
+fastisostasy.jl_amd/build.py. There is no fallback: if the library is missing or fails to load,
+load_engine() raises.
+"""
+import ctypes as C
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ENGINE_PATH = os.path.join(PKG_ROOT, "lib", "libfastiso_hip.so")
+
+FI_ABI_VERSION = 1
+
+# status codes
+FI_OK = 0
+FI_ERR_INVALID = -1
+FI_ERR_UNSUPPORTED = -2
+FI_ERR_TIME_RANGE = -3
+FI_ERR_NONFINITE = -4
+FI_ERR_DEVICE = -5
+FI_ERR_STATE = -6
+FI_ERR_DT_MIN = -7
+
+# model enums
+FI_RIGID_LITHOSPHERE, FI_LATERALLY_CONSTANT_LITHOSPHERE, FI_LATERALLY_VARIABLE_LITHOSPHERE = 0, 1, 2
+FI_RIGID_MANTLE, FI_RELAXED_MANTLE, FI_MAXWELL_MANTLE = 0, 1, 2
+FI_INCOMPRESSIBLE_LITHOSPHERE_COLUMN, FI_COMPRESSIBLE_LITHOSPHERE_COLUMN = 0, 1
+FI_LATERALLY_CONSTANT_SEA_SURFACE, FI_LATERALLY_VARIABLE_SEA_SURFACE = 0, 1
+FI_NO_SEALEVEL_LOAD, FI_INTERACTIVE_SEALEVEL_LOAD = 0, 1
+FI_NO_CONTRIBUTION, FI_GOELZER_CONTRIBUTION = 0, 1
+
+# field ids
+FIELDS = {
+    "litho_rigidity": 1, "effective_viscosity": 2, "scaled_pseudodiff_inv": 3, "maskactive": 4,
+    "green_elastic": 5, "green_geoid": 6, "ref_u": 7, "ref_ue": 8, "ref_dz_ss": 9, "ref_z_b": 10,
+    "ref_H_ice": 11,
+    "u": 32, "dudt": 33, "ue": 34, "dz_ss": 35, "z_ss": 36, "z_b": 37, "H_ice": 38, "H_af": 39,
+    "H_water": 40, "maskgrounded": 41, "maskocean": 42, "canom_ice": 43, "canom_seawater": 44,
+    "canom_litho": 45, "canom_mantle": 46, "canom_load": 47, "canom_full": 48,
+}
+
+
+class FiConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("Wx", C.c_double), ("Wy", C.c_double),
+        ("g", C.c_double), ("seconds_per_year", C.c_double), ("rho_ice", C.c_double),
+        ("rho_water", C.c_double), ("rho_seawater", C.c_double), ("r_pole", C.c_double),
+        ("mE", C.c_double),
+        ("litho_poissonratio", C.c_double), ("rho_uppermantle", C.c_double), ("rho_litho", C.c_double),
+        ("lithosphere", C.c_int32), ("mantle", C.c_int32), ("lithosphere_column", C.c_int32),
+        ("sea_surface", C.c_int32), ("sealevel_load", C.c_int32), ("volume_contribution", C.c_int32),
+        ("density_contribution", C.c_int32), ("adjustment_contribution", C.c_int32),
+        ("bsl_z", C.c_double),
+        ("viscous_x_border", C.c_double), ("elastic_x_border", C.c_double), ("ssh_x_border", C.c_double),
+        ("dt_sparse", C.c_double), ("reltol", C.c_double), ("abstol", C.c_double),
+        ("device", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class FiStats(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("dt", C.c_double),
+        ("n_rhs", C.c_int64), ("n_accept", C.c_int64), ("n_reject", C.c_int64),
+        ("count_sparse_updates", C.c_int64), ("n_output", C.c_int64),
+        ("V_af", C.c_double), ("V_den", C.c_double), ("V_pov", C.c_double), ("delta_V", C.c_double),
+        ("z_bsl", C.c_double), ("last_EEst", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+OUTPUT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
+
+_dp = C.POINTER(C.c_double)
+
+
+def declare(lib):
+    """Attach argtypes/restype of every fi_* entry point to a loaded library."""
+    P = C.c_void_p
+    sig = {
+        "fi_abi_version": (C.c_int32, []),
+        "fi_engine_name": (C.c_char_p, []),
+        "fi_default_config": (None, [C.POINTER(FiConfig), C.c_int32, C.c_int32, C.c_double, C.c_double]),
+        "fi_create": (C.c_int, [C.POINTER(FiConfig), C.POINTER(P)]),
+        "fi_destroy": (None, [P]),
+        "fi_last_error": (C.c_char_p, [P]),
+        "fi_set_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
+        "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
+        "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
+        "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
+        "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),
+        "fi_init": (C.c_int, [P, C.c_double, C.c_double]),
+        "fi_rhs": (C.c_int, [P, C.c_double, _dp, _dp]),
+        "fi_rhs_device": (C.c_int, [P, C.c_double, C.c_void_p, C.c_void_p]),
+        "fi_sparse_update": (C.c_int, [P, C.c_double]),
+        "fi_integrator_init": (C.c_int, [P]),
+        "fi_integrate": (C.c_int, [P, C.c_double, OUTPUT_CB, C.c_void_p]),
+        "fi_get_stats": (C.c_int, [P, C.POINTER(FiStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_ENGINE = None
+
+
+def load_engine(path=None):
+    """Load the HIP engine (libfastiso_hip.so). Raises if it is missing: no CPU fallback exists."""
+    global _ENGINE
+    if path is None and _ENGINE is not None:
+        return _ENGINE
+    p = path or ENGINE_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"HIP engine not built: {p} is missing. Build it with "
+            f"`python fastisostasy.jl_amd/build.py` (hipcc --offload-arch=gfx950).")
+    lib = declare(C.CDLL(p))
+    if lib.fi_abi_version() != FI_ABI_VERSION:
+        raise RuntimeError("libfastiso_hip.so ABI version mismatch")
+    if path is None:
+        _ENGINE = lib
+    return lib

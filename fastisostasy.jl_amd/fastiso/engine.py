@@ -1,0 +1,116 @@
+"""Thin object wrapper over one fi_ctx of the HIP engine."""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fastiso engine error {code}: {msg}")
+        self.code = code
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, order="F") if np.ndim(a) == 2
+                             else np.asarray(a, dtype=np.float64).ravel())
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fortran_flat(X):
+    """(nx, ny) array -> column-major flat FP64 vector (element (i, j) at i + j*nx)."""
+    return np.ascontiguousarray(np.asarray(X, dtype=np.float64).ravel(order="F"))
+
+
+def from_flat(v, nx, ny):
+    return np.asarray(v).reshape((nx, ny), order="F")
+
+
+class Engine:
+    """One simulation context on the HIP engine (libfastiso_hip.so)."""
+
+    def __init__(self, cfg, lib=None):
+        self.lib = lib if lib is not None else _abi.load_engine()
+        self.cfg = cfg
+        self.nx, self.ny = cfg.nx, cfg.ny
+        self.N = cfg.nx * cfg.ny
+        h = C.c_void_p()
+        st = self.lib.fi_create(C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise EngineError(st, "fi_create failed")
+        self.h = h
+        self._cb = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            msg = self.lib.fi_last_error(self.h)
+            raise EngineError(st, msg.decode() if msg else "")
+
+    def set_field(self, name, X):
+        v = fortran_flat(X) if np.ndim(X) == 2 else _f64(X, self.N)
+        self._check(self.lib.fi_set_field(self.h, _abi.FIELDS[name], _ptr(v), self.N))
+
+    def get_field(self, name):
+        v = np.empty(self.N, dtype=np.float64)
+        self._check(self.lib.fi_get_field(self.h, _abi.FIELDS[name], _ptr(v), self.N))
+        return from_flat(v, self.nx, self.ny)
+
+    def set_ice_snapshots(self, t, H_list, flat_bc=False):
+        t = _f64(t)
+        H = np.concatenate([fortran_flat(H) for H in H_list])
+        self._check(self.lib.fi_set_ice_snapshots(self.h, _ptr(t), len(t), _ptr(H), int(flat_bc)))
+
+    def set_ice_separable(self, t, a, H_shape, flat_bc=False):
+        t, a, H = _f64(t), _f64(a), fortran_flat(H_shape)
+        self._check(self.lib.fi_set_ice_separable(self.h, _ptr(t), _ptr(a), len(t), _ptr(H), int(flat_bc)))
+
+    def set_output_times(self, t):
+        t = _f64(t)
+        self._check(self.lib.fi_set_output_times(self.h, _ptr(t), len(t)))
+
+    def init(self, t0, t_end):
+        self._check(self.lib.fi_init(self.h, float(t0), float(t_end)))
+
+    def rhs(self, t, u):
+        """update_diagnostics! on host arrays; returns (dudt, u_after_bc)."""
+        uu = fortran_flat(u).copy()
+        du = np.empty(self.N, dtype=np.float64)
+        self._check(self.lib.fi_rhs(self.h, float(t), _ptr(uu), _ptr(du)))
+        return from_flat(du, self.nx, self.ny), from_flat(uu, self.nx, self.ny)
+
+    def sparse_update(self, t):
+        self._check(self.lib.fi_sparse_update(self.h, float(t)))
+
+    def integrator_init(self):
+        self._check(self.lib.fi_integrator_init(self.h))
+
+    def integrate(self, t_target, callback=None):
+        if callback is None:
+            cb = _abi.OUTPUT_CB()
+        else:
+            cb = _abi.OUTPUT_CB(lambda user, t, k: callback(t, k))
+        self._cb = cb
+        self._check(self.lib.fi_integrate(self.h, float(t_target), cb, None))
+
+    def stats(self):
+        s = _abi.FiStats()
+        self._check(self.lib.fi_get_stats(self.h, C.byref(s)))
+        return s.as_dict()

@@ -1,0 +1,88 @@
+"""Seeded synthetic workloads for the benchmark and parity configurations.
+
+disc_benchmark  64x64 homogeneous Earth (eta 1e21 Pa s, 88 km lithosphere, rho_mantle 3300,
+                rho_litho 0), 1000 km x 1000 m ice disc applied over the first year, 0-50 kyr
+                (docs/src/examples/benchmark_analytic.jl:11-43); compared with analytic_solution.
+lv_elva         N x N laterally variable Earth: log10 viscosity a Gaussian random field
+                (correlation length 500 km) mapped to [19, 23], lithosphere thickness a smooth
+                random field (1000 km) mapped to [50, 250] km, 3-6 Gaussian ice domes (0-4000 m)
+                scaled by a sawtooth glacial cycle over 0-120 kyr with knots every 1 kyr; elastic,
+                viscous and sea-surface responses on. N = 350 over a 6300 km square is dx = 18 km.
+"""
+import numpy as np
+
+from .bcs import BoundaryConditions, SeparableIceThickness, TimeInterpolatedIceThickness
+from .domain import PhysicalConstants, RegionalDomain
+from .sealevel import LaterallyVariableSeaSurface, RegionalSeaLevel
+from .simulation import NativeOutput, SolverOptions
+from .solidearth import SolidEarth
+from .utils import uniform_ice_cylinder
+
+
+def gaussian_random_field(domain, seed, corr_length):
+    """White N(0,1) noise filtered by exp(-kappa^2 l^2 / 4) in Fourier space, min-max scaled to
+    [0, 1]."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((domain.nx, domain.ny))
+    kx = 2 * np.pi * np.fft.fftfreq(domain.nx, d=domain.dx)
+    ky = 2 * np.pi * np.fft.fftfreq(domain.ny, d=domain.dy)
+    k2 = kx[:, None] ** 2 + ky[None, :] ** 2
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-k2 * corr_length ** 2 / 4)))
+    return (f - f.min()) / (f.max() - f.min())
+
+
+def ice_domes(domain, seed, inner=0.7, hmax=4000.0):
+    """3-6 Gaussian domes: centres uniform in the inner 70 % of the domain, widths 200-600 km,
+    peaks 1000-4000 m, total clipped to [0, hmax]."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(3, 7))
+    H = np.zeros((domain.nx, domain.ny))
+    for _ in range(n):
+        cx = rng.uniform(-inner * domain.Wx, inner * domain.Wx)
+        cy = rng.uniform(-inner * domain.Wy, inner * domain.Wy)
+        s = rng.uniform(200e3, 600e3)
+        a = rng.uniform(1000.0, 4000.0)
+        H += a * np.exp(-((domain.X - cx) ** 2 + (domain.Y - cy) ** 2) / (2 * s ** 2))
+    return np.clip(H, 0.0, hmax)
+
+
+def sawtooth(t_end=120e3, t_peak=100e3, dt=1e3):
+    """Glacial-cycle amplitude: 0 -> 1 linearly until t_peak, 1 -> 0 until t_end."""
+    t = np.arange(0.0, t_end + dt / 2, dt)
+    a = np.where(t <= t_peak, t / t_peak, (t_end - t) / (t_end - t_peak))
+    return t, a
+
+
+def disc_benchmark(n=64, W=3e6, t_end=50e3, t_out=None):
+    domain = RegionalDomain(W, W, n, n)
+    H1 = uniform_ice_cylinder(domain, 1e6, 1e3)
+    it = TimeInterpolatedIceThickness([0.0, 1.0, t_end], [domain.zeros(), H1, H1], domain)
+    bcs = BoundaryConditions(domain, ice_thickness=it)
+    se = SolidEarth(domain, layer_boundaries=[88e3], layer_viscosities=[1e21], rho_uppermantle=3300.0,
+                    rho_litho=0.0)
+    sl = RegionalSeaLevel()
+    if t_out is None:
+        t_out = np.arange(1e3, t_end + 1, 1e3)
+    nout = NativeOutput(t=t_out, vars=["u"])
+    return dict(domain=domain, bcs=bcs, sealevel=sl, solidearth=se, t_span=(0.0, t_end), nout=nout,
+                opts=SolverOptions(), c=PhysicalConstants())
+
+
+def lv_elva(n=350, W=3.15e6, t_end=120e3, seeds=(1001, 1002, 1003), separable=True, t_out=None):
+    domain = RegionalDomain(W, W, n, n)
+    logeta = 19.0 + 4.0 * gaussian_random_field(domain, seeds[0], 500e3)
+    h = 50e3 + 200e3 * gaussian_random_field(domain, seeds[1], 1000e3)
+    se = SolidEarth(domain, layer_boundaries=[h], layer_viscosities=[10.0 ** logeta])
+    Hd = ice_domes(domain, seeds[2])
+    tk, ak = sawtooth(t_end=max(t_end, 120e3))
+    if separable:
+        it = SeparableIceThickness(tk, ak, Hd)
+    else:
+        it = TimeInterpolatedIceThickness(tk, [a * Hd for a in ak], domain)
+    bcs = BoundaryConditions(domain, ice_thickness=it)
+    sl = RegionalSeaLevel(surface=LaterallyVariableSeaSurface())
+    if t_out is None:
+        t_out = np.arange(1e3, t_end + 1, 1e3)
+    nout = NativeOutput(t=t_out, vars=["u"])
+    return dict(domain=domain, bcs=bcs, sealevel=sl, solidearth=se, t_span=(0.0, t_end), nout=nout,
+                opts=SolverOptions(), c=PhysicalConstants(), logeta=logeta, h=h, H_dome=Hd)

@@ -1,0 +1,233 @@
+/*
+ * fastiso.h -- C ABI of the LV-ELVA deformation engine.
+ *
+ * One ABI, two implementations:
+ *   libfastiso_hip.so  the MI355X (gfx950) engine: hand-written HIP kernels, device-resident state;
+ *   libfastiso_ref.so  the FP64 CPU restatement used as the test oracle (built from oracle/).
+ *
+ * Conventions (all entry points):
+ *   - plain C types only; fields are FP64 host arrays in Julia column-major order (nx, ny),
+ *     x fastest: element (i, j) at i + j*nx (0-based);
+ *   - every function returns an int status (FI_OK = 0, negative = error); no exception crosses
+ *     the ABI; fi_last_error() gives the text of the last error on that context;
+ *   - the library owns its device memory, the caller owns host arrays;
+ *   - one fi_ctx per host thread; calls on one context are not thread-safe.
+ *
+ * Each entry point names the reference function it stands in for (FastIsostasy.jl, file:line).
+ */
+#ifndef FASTISO_H
+#define FASTISO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FI_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------------------------ */
+enum {
+    FI_OK = 0,
+    FI_ERR_INVALID = -1,      /* bad argument / size mismatch / missing required field        */
+    FI_ERR_UNSUPPORTED = -2,  /* model choice not implemented by this engine                   */
+    FI_ERR_TIME_RANGE = -3,   /* ice-thickness interpolation outside its time vector           */
+                              /* (src/interpolations.jl:51-74 throws there)                      */
+    FI_ERR_NONFINITE = -4,    /* non-finite state or error norm that the step control cannot fix */
+    FI_ERR_DEVICE = -5,       /* HIP runtime error or device-side abort                         */
+    FI_ERR_STATE = -6,        /* call out of order (e.g. fi_rhs before fi_init)                  */
+    FI_ERR_DT_MIN = -7        /* adaptive step fell below the minimum step                        */
+};
+
+/* ---- model choices (singleton types of the reference) ------------------------------------ */
+enum { /* src/solidearth.jl:13-36 */
+    FI_RIGID_LITHOSPHERE = 0,
+    FI_LATERALLY_CONSTANT_LITHOSPHERE = 1,
+    FI_LATERALLY_VARIABLE_LITHOSPHERE = 2
+};
+enum { /* src/solidearth.jl:50-82 */
+    FI_RIGID_MANTLE = 0,
+    FI_RELAXED_MANTLE = 1,
+    FI_MAXWELL_MANTLE = 2
+};
+enum { /* src/solidearth.jl:95-109 */
+    FI_INCOMPRESSIBLE_LITHOSPHERE_COLUMN = 0,
+    FI_COMPRESSIBLE_LITHOSPHERE_COLUMN = 1
+};
+enum { /* src/sealevel.jl:114-134 */
+    FI_LATERALLY_CONSTANT_SEA_SURFACE = 0,
+    FI_LATERALLY_VARIABLE_SEA_SURFACE = 1
+};
+enum { /* src/loads.jl:12-26 */
+    FI_NO_SEALEVEL_LOAD = 0,
+    FI_INTERACTIVE_SEALEVEL_LOAD = 1
+};
+enum { /* src/barystatic_sealevel.jl:21-101: the No- and Goelzer- contributions */
+    FI_NO_CONTRIBUTION = 0,
+    FI_GOELZER_CONTRIBUTION = 1
+};
+
+/* ---- configuration (the scalar part of RegionalDomain, PhysicalConstants, SolidEarth,
+ *      RegionalSeaLevel, BoundaryConditions and SolverOptions) ----------------------------- */
+typedef struct fi_config {
+    int32_t nx, ny;              /* RegionalDomain(Wx, Wy, nx, ny)      src/domain.jl:76-83   */
+    double Wx, Wy;               /* half-widths (m)                                            */
+    /* PhysicalConstants                                                src/constants.jl:16-28 */
+    double g;                    /* 9.8                                                        */
+    double seconds_per_year;     /* 31557600 (src/utils.jl:7)                                  */
+    double rho_ice;              /* 910                                                        */
+    double rho_water;            /* 1000                                                       */
+    double rho_seawater;         /* 1023                                                       */
+    double r_pole;               /* 6357e3                                                     */
+    double mE;                   /* 5.972e24                                                   */
+    /* SolidEarth scalars                                               src/solidearth.jl:175-193 */
+    double litho_poissonratio;   /* 0.28                                                       */
+    double rho_uppermantle;      /* 3400                                                       */
+    double rho_litho;            /* 3200                                                       */
+    int32_t lithosphere;         /* FI_*_LITHOSPHERE                                           */
+    int32_t mantle;              /* FI_*_MANTLE                                                */
+    int32_t lithosphere_column;  /* FI_*_LITHOSPHERE_COLUMN                                    */
+    /* RegionalSeaLevel                                                 src/sealevel.jl:150-166 */
+    int32_t sea_surface;         /* FI_*_SEA_SURFACE                                           */
+    int32_t sealevel_load;       /* FI_*_SEALEVEL_LOAD                                         */
+    int32_t volume_contribution;     /* FI_GOELZER_CONTRIBUTION by default                     */
+    int32_t density_contribution;    /* FI_GOELZER_CONTRIBUTION by default                     */
+    int32_t adjustment_contribution; /* FI_NO_CONTRIBUTION by default                          */
+    double bsl_z;                /* ConstantBSL: z = ref.z (src/barystatic_sealevel.jl:215-218) */
+    /* BoundaryConditions: x_border of the three CornerBCs   src/boundary_conditions.jl:308-325 */
+    double viscous_x_border, elastic_x_border, ssh_x_border;
+    /* SolverOptions / DiffEqOptions                                    src/simulation.jl:11-26 */
+    double dt_sparse;            /* dt_sparse_diagnostics, 10 yr                               */
+    double reltol;               /* 1e-5                                                       */
+    double abstol;               /* 1e-6                                                       */
+    int32_t device;              /* HIP device ordinal (ignored by the CPU oracle)             */
+    int32_t reserved;
+} fi_config;
+
+/* ---- fields ------------------------------------------------------------------------------- */
+enum {
+    /* static inputs, set before fi_init (each nx*ny doubles) */
+    FI_LITHO_RIGIDITY = 1,        /* SolidEarth.litho_rigidity          src/solidearth.jl:211  */
+    FI_EFFECTIVE_VISCOSITY = 2,   /* SolidEarth.effective_viscosity     src/solidearth.jl:212-216 */
+    FI_SCALED_PSEUDODIFF_INV = 3, /* SolidEarth.scaled_pseudodiff_inv   src/solidearth.jl:222  */
+    FI_MASK_ACTIVE = 4,           /* SolidEarth.maskactive (0/1)        src/solidearth.jl:183  */
+    FI_GREEN_ELASTIC = 5,         /* elastic Green kernel               src/tools.jl:64-71     */
+    FI_GREEN_GEOID = 6,           /* sea-surface Green kernel           src/sealevel.jl:208-219 */
+    FI_REF_U = 7,                 /* Simulation kwargs u_ref, ue_ref, dz_ss_ref, z_b_ref        */
+    FI_REF_UE = 8,                /*                                    src/simulation.jl:112-116 */
+    FI_REF_DZ_SS = 9,
+    FI_REF_Z_B = 10,
+    FI_REF_H_ICE = 11,            /* reference ice; default: forcing at t0 (src/simulation.jl:133-134) */
+    /* state, readable after fi_init; FI_U and FI_H_ICE are also writable */
+    FI_U = 32,                    /* CurrentState fields                src/state.jl:49-77     */
+    FI_DUDT = 33,
+    FI_UE = 34,
+    FI_DZ_SS = 35,
+    FI_Z_SS = 36,
+    FI_Z_B = 37,
+    FI_H_ICE = 38,                /* writable: ExternallyUpdatedIceThickness (src/boundary_conditions.jl:48,60-62) */
+    FI_H_AF = 39,
+    FI_H_WATER = 40,
+    FI_MASK_GROUNDED = 41,
+    FI_MASK_OCEAN = 42,
+    FI_CANOM_ICE = 43,            /* ColumnAnomalies                    src/state.jl:1-9       */
+    FI_CANOM_SEAWATER = 44,
+    FI_CANOM_LITHO = 45,
+    FI_CANOM_MANTLE = 46,
+    FI_CANOM_LOAD = 47,
+    FI_CANOM_FULL = 48
+};
+
+/* ---- run statistics ------------------------------------------------------------------------ */
+typedef struct fi_stats {
+    double t;                     /* current time (yr)                                          */
+    double dt;                    /* step the controller proposes next (yr)                     */
+    int64_t n_rhs;                /* RHS evaluations (update_diagnostics! calls)                */
+    int64_t n_accept, n_reject;   /* accepted / rejected steps                                  */
+    int64_t count_sparse_updates; /* CurrentState.count_sparse_updates  src/state.jl:76        */
+    int64_t n_output;             /* output callbacks fired (NativeOutput.k - 1)               */
+    double V_af, V_den, V_pov, delta_V, z_bsl;   /* src/state.jl:69-73                         */
+    double last_EEst;             /* error norm of the last attempted step                      */
+} fi_stats;
+
+/* Output callback, fired after an accepted step at which t >= the next output time
+ * (nout_condition/nout_affect!, src/simulation.jl:194-225); k is the 0-based output index.
+ * The callback may call fi_get_field / fi_get_stats / fi_set_field(FI_H_ICE). */
+typedef void (*fi_output_cb)(void* user, double t, int32_t k);
+
+typedef struct fi_ctx fi_ctx;
+
+/* ---- entry points --------------------------------------------------------------------------- */
+
+/* ABI version (FI_ABI_VERSION) and engine name ("hip-gfx950" or "cpu-oracle-fp64"). */
+int32_t fi_abi_version(void);
+const char* fi_engine_name(void);
+
+/* Fill cfg with the reference defaults (PhysicalConstants(), SolidEarth(), RegionalSeaLevel(),
+ * SolverOptions(); src/constants.jl:16-28, src/solidearth.jl:115-121,175-193,
+ * src/sealevel.jl:150-166, src/simulation.jl:11-26) for an nx x ny grid of half-widths Wx, Wy. */
+void fi_default_config(fi_config* cfg, int32_t nx, int32_t ny, double Wx, double Wy);
+
+/* Create a context: RegionalDomain(Wx, Wy, nx, ny) (src/domain.jl:76-169) plus the buffers of
+ * GIATools (src/tools.jl:49-98). Allocates device memory. */
+int fi_create(const fi_config* cfg, fi_ctx** out);
+void fi_destroy(fi_ctx* ctx);
+const char* fi_last_error(const fi_ctx* ctx);
+
+/* Set / get one field (count must be nx*ny). Static inputs go before fi_init. */
+int fi_set_field(fi_ctx* ctx, int32_t field, const double* host, size_t count);
+int fi_get_field(fi_ctx* ctx, int32_t field, double* host, size_t count);
+
+/* TimeInterpolatedIceThickness(t_vec, H_vec) (src/boundary_conditions.jl:25-40) with the
+ * piecewise-linear TimeInterpolation2D (src/interpolations.jl:51-74): H holds nt snapshots of
+ * nx*ny each, t ascending (yr). flat_bc != 0 clamps outside [t_1, t_nt], else FI_ERR_TIME_RANGE. */
+int fi_set_ice_snapshots(fi_ctx* ctx, const double* t, int32_t nt, const double* H, int32_t flat_bc);
+
+/* The same forcing when every snapshot is a(t_k) * H_shape: H(t) = a(t) H_shape with a(t)
+ * piecewise linear over the knots (t_k, a_k). Mathematically identical to fi_set_ice_snapshots
+ * on the snapshots a_k * H_shape; reads one field per RHS instead of two. */
+int fi_set_ice_separable(fi_ctx* ctx, const double* t, const double* a, int32_t nt,
+                         const double* H_shape, int32_t flat_bc);
+
+/* Output times = tstops of the solver and the times of the output callbacks
+ * (tstops = sort(vcat(nout.t, ncout.t)), src/simulation.jl:255-257). */
+int fi_set_output_times(fi_ctx* ctx, const double* t, int32_t n);
+
+/* Finish the Simulation constructor (src/simulation.jl:105-156): the padded Green spectra
+ * (ConvolutionPlan, src/convolutions.jl:41-56,89-92), the reference state (H_ice_ref at t0,
+ * z_ss_ref, H_af_ref, masks, H_water_ref) and CurrentState(domain, ref, z_bsl). */
+int fi_init(fi_ctx* ctx, double t0, double t_end);
+
+/* The ODE right-hand side update_diagnostics!(dudt, u, sim, t) (src/simulation.jl:335-393):
+ * corner BC on u (u is mutated, as the reference mutates the solver's stage vector), load
+ * update, the sparse elastic / sea-surface update when (t - t0)/dt_sparse >= count, and
+ * update_dudt!(MaxwellMantle, LaterallyVariableLithosphere) (src/deformation.jl:91-131).
+ * u and dudt are nx*ny host FP64 arrays; dudt is in m/yr. */
+int fi_rhs(fi_ctx* ctx, double t, double* u, double* dudt);
+
+/* Same on device-resident FP64 arrays (HIP engine: device pointers, stream-ordered on the
+ * context's stream; CPU oracle: host pointers). The form a Julia caller holding a device
+ * array uses per solver stage. */
+int fi_rhs_device(fi_ctx* ctx, double t, void* u, void* dudt);
+
+/* Only the sparse block of update_diagnostics! (src/simulation.jl:359-381): elastic response,
+ * barystatic volumes, sea-surface perturbation, masks, counted as one sparse update. */
+int fi_sparse_update(fi_ctx* ctx, double t);
+
+/* init_integrator / the first half of run! (src/simulation.jl:236-238,293-311): init_problem!
+ * (volumes + one RHS at t0), FSAL initialisation and the initial step size. */
+int fi_integrator_init(fi_ctx* ctx);
+
+/* Advance the adaptive BS3 integration (src/simulation.jl:255-257; OrdinaryDiffEq BS3 with
+ * reltol, abstol) until t == t_target, stopping exactly at the output times and firing cb after
+ * accepted steps that reach one. run!(sim) = fi_integrator_init + fi_integrate(t_end);
+ * step!(integrator, dt, true) (src/simulation.jl:319-322) = fi_integrate(t + dt). */
+int fi_integrate(fi_ctx* ctx, double t_target, fi_output_cb cb, void* user);
+
+int fi_get_stats(fi_ctx* ctx, fi_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTISO_H */

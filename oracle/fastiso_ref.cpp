@@ -1,0 +1,1129 @@
+// =====================================================================================
+// ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of FastIsostasy.jl's LV-ELVA deformation path, exporting the same
+// C ABI as the HIP engine (include/fastiso.h) plus ref_* hooks for the unit checks.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+// as the checker or as the timed CPU baseline; the product never links or calls it.
+//
+// Each routine follows the cited reference file:line (paths relative to the reference
+// repository root) in operation order, including the reference's ordering quirks:
+//   - the sparse elastic / sea-surface update reads the column anomalies of the PREVIOUS
+//     RHS call (src/simulation.jl:355-385);
+//   - apply_bc! mutates the solver's stage vector (src/simulation.jl:341);
+//   - the sparse trigger is evaluated on every RHS call, rejected stages included.
+//
+// What pins each part (tests/test_oracle_*.py):
+//   FFT                 numpy.fft on random data (library routine);
+//   stencils            closed form of X^2 Y^2 (test/test_derivatives.jl:15-26) + exactness of
+//                       the one-sided edge formulas on quadratics;
+//   padded convolution  brute-force direct 2D convolution (test/test_convolution.jl:2-10 uses
+//                       DSP.conv) and delta kernels;
+//   Green tables        closed form of the geoid kernel; GL nodes (closed form n<=5); elastic
+//                       table vs an independent numpy quadrature of the Farrell table;
+//   spectral operator   eigenfunction test: cos modes are scaled by 1/kappa exactly;
+//   BS3 integrator      order test on du/dt = lambda u and the published tableau values;
+//   full trajectory     the analytic disc-load solution (src/analytic_solutions.jl:11-65)
+//                       evaluated independently with scipy.special.j0/j1.
+//   Parity unpinned: the step-size sequence (OrdinaryDiffEq controller internals are not in
+//   the reference; constants chosen below are the published PI-controller defaults).
+// =====================================================================================
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/fastiso.h"
+#include "ref_fft.hpp"
+
+using fiorc::cd;
+using fiorc::FFT;
+
+namespace {
+
+typedef std::vector<double> vec;
+
+// ---------------------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------------------
+
+// nextprod((2,3,5,7), n)  (src/convolutions.jl:17-18)
+int nextfastfft(int n) {
+    for (int m = std::max(n, 1);; ++m) {
+        int r = m;
+        for (int p : {2, 3, 5, 7}) while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+// fftint(N)  (src/derivatives.jl:216-222)
+std::vector<double> fftint(int N) {
+    std::vector<double> f;
+    int N2 = N / 2;
+    for (int k = 0; k <= N2; ++k) f.push_back(k);
+    if (N % 2 == 0) for (int k = N2 - 1; k >= 1; --k) f.push_back(k);
+    else for (int k = N2; k >= 1; --k) f.push_back(k);
+    return f;
+}
+
+// samesize_conv_indices(N, M)  (src/convolutions.jl:196-204); 1-based as in the reference
+void samesize_conv_indices(int N, int M, int& j1, int& j2) {
+    j1 = (N % 2 == 0) ? M : M + 1;
+    j2 = 2 * N - 1 - M;
+}
+
+// get_differential_fourier + DC fix (src/derivatives.jl:201-214, src/domain.jl:151-155)
+vec pseudodiff(int nx, int ny, double Wx, double Wy) {
+    double mux = M_PI / Wx, muy = M_PI / Wy;
+    std::vector<double> fx = fftint(nx), fy = fftint(ny);
+    vec p((size_t)nx * ny);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double X = mux * fx[i], Y = muy * fy[j];
+            p[i + (size_t)j * nx] = std::sqrt(X * X + Y * Y);
+        }
+    // pseudodiff[1,1] = 1e-3 * mean([pseudodiff[1,2], pseudodiff[2,1]])
+    p[0] = 1e-3 * ((p[(size_t)nx] + p[1]) / 2.0);
+    return p;
+}
+
+// Gauss-Legendre nodes/weights on [-1,1], ascending (FastGaussQuadrature.gausslegendre;
+// textbook Newton iteration on the Legendre recurrence).
+void gauss_legendre(int n, vec& x, vec& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int k = 1; k <= n; ++k) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * k - 1.0) * z * p2 - (k - 1.0) * p3) / k;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            double z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) < 1e-16) break;
+        }
+        x[i] = -z;
+        x[n - 1 - i] = z;
+        w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+// get_loadgreen with the linear interpolant (src/material.jl:299-332)
+double loadgreen(double r, const vec& rm, const vec& ge) {
+    if (r < 0.01) return ge[0] / (rm[1] * 1e12);
+    if (r > rm.back()) return 0.0;
+    size_t k = std::upper_bound(rm.begin(), rm.end(), r) - rm.begin();  // rm[k-1] <= r < rm[k]
+    if (k >= rm.size()) return ge.back() / (r * 1e12);
+    double t = (r - rm[k - 1]) / (rm[k] - rm[k - 1]);
+    double g = ge[k - 1] + t * (ge[k] - ge[k - 1]);
+    return g / (r * 1e12);
+}
+
+// get_elastic_green with quadrature2D (src/material.jl:340-364, src/utils.jl:146-186)
+vec elastic_green(int nx, int ny, double Wx, double Wy, const vec& rm, const vec& ge, int nq) {
+    double dx = 2 * Wx / nx, dy = 2 * Wy / ny;
+    int mx = nx / 2, my = ny / 2;
+    vec xq, wq;
+    gauss_legendre(nq, xq, wq);
+    vec G((size_t)nx * ny);
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j) {
+            int p = i - mx - 1, q = j - my - 1;
+            double x1 = p * dx, x2 = (p + 1) * dx, y1 = q * dy, y2 = (q + 1) * dy;
+            double mxl = 2.0 / (x2 - x1), pxl = -1.0 - mxl * x1;
+            double myl = 2.0 / (y2 - y1), pyl = -1.0 - myl * y1;
+            double s = 0.0;
+            for (int a = 0; a < nq; ++a)
+                for (int b = 0; b < nq; ++b) {
+                    double xx = (xq[a] - pxl) / mxl, yy = (xq[b] - pyl) / myl;
+                    s = s + loadgreen(std::sqrt(xx * xx + yy * yy), rm, ge) * wq[a] * wq[b] / mxl / myl;
+                }
+            G[(i - 1) + (size_t)(j - 1) * nx] = s;
+        }
+    return G;
+}
+
+// get_dz_ss_green (src/sealevel.jl:208-219)
+vec geoid_green(int nx, int ny, double Wx, double Wy, double r_pole, double mE) {
+    double dx = 2 * Wx / nx, dy = 2 * Wy / ny;
+    vec G((size_t)nx * ny);
+    auto unb = [&](double R) { return r_pole / (2.0 * mE * std::sin(R / (2.0 * r_pole))); };
+    double gmax = unb(dx / 2);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double X = -Wx + (i + 1) * dx, Y = -Wy + (j + 1) * dy;
+            double R = std::sqrt(X * X + Y * Y);
+            G[i + (size_t)j * nx] = std::min(unb(R), gmax);
+        }
+    return G;
+}
+
+// ---------------------------------------------------------------------------------------
+// Finite differences, CPU rule (src/derivatives.jl:48-57,66-76,114-124,166-191)
+// ---------------------------------------------------------------------------------------
+void dxx(double* du, const double* u, int nx, int ny, double Dx) {
+    double D2 = Dx * Dx;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+        const double* c = u + (size_t)j * nx;
+        double* o = du + (size_t)j * nx;
+        for (int i = 1; i < nx - 1; ++i) o[i] = (c[i + 1] - 2 * c[i] + c[i - 1]) / D2;
+        o[0] = (c[2] - 2 * c[1] + c[0]) / D2;
+        o[nx - 1] = (c[nx - 1] - 2 * c[nx - 2] + c[nx - 3]) / D2;
+    }
+}
+void dyy(double* du, const double* u, int nx, int ny, double Dy) {
+    double D2 = Dy * Dy;
+    auto U = [&](int i, int j) { return u[i + (size_t)j * nx]; };
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+        double* o = du + (size_t)j * nx;
+        if (j == 0) for (int i = 0; i < nx; ++i) o[i] = (U(i, 2) - 2 * U(i, 1) + U(i, 0)) / D2;
+        else if (j == ny - 1)
+            for (int i = 0; i < nx; ++i) o[i] = (U(i, ny - 1) - 2 * U(i, ny - 2) + U(i, ny - 3)) / D2;
+        else for (int i = 0; i < nx; ++i) o[i] = (U(i, j + 1) - 2 * U(i, j) + U(i, j - 1)) / D2;
+    }
+}
+void dx1(double* du, const double* u, int nx, int ny, double Dx) {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+        const double* c = u + (size_t)j * nx;
+        double* o = du + (size_t)j * nx;
+        for (int i = 1; i < nx - 1; ++i) o[i] = (c[i + 1] - c[i - 1]) / (2 * Dx);
+        o[0] = (c[1] - c[0]) / Dx;
+        o[nx - 1] = (c[nx - 1] - c[nx - 2]) / Dx;
+    }
+}
+void dy1(double* du, const double* u, int nx, int ny, double Dy) {
+    auto U = [&](int i, int j) { return u[i + (size_t)j * nx]; };
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+        double* o = du + (size_t)j * nx;
+        if (j == 0) for (int i = 0; i < nx; ++i) o[i] = (U(i, 1) - U(i, 0)) / Dy;
+        else if (j == ny - 1) for (int i = 0; i < nx; ++i) o[i] = (U(i, ny - 1) - U(i, ny - 2)) / Dy;
+        else for (int i = 0; i < nx; ++i) o[i] = (U(i, j + 1) - U(i, j - 1)) / (2 * Dy);
+    }
+}
+// update_second_derivatives!(uxx, uyy, ux, uxy, u1, u2, u3)
+void second_derivatives(double* uxx, double* uyy, double* ux, double* uxy, const double* u1,
+                        const double* u2, const double* u3, int nx, int ny, double Dx, double Dy) {
+    dxx(uxx, u1, nx, ny, Dx);
+    dyy(uyy, u2, nx, ny, Dy);
+    dx1(ux, u3, nx, ny, Dx);
+    dy1(uxy, ux, nx, ny, Dy);
+}
+
+// ---------------------------------------------------------------------------------------
+// 2D FFT on column-major (nx, ny) complex data
+// ---------------------------------------------------------------------------------------
+void fft2d(cd* a, int nx, int ny, const FFT& fx, const FFT& fy, int sign) {
+#pragma omp parallel
+    {
+        std::vector<cd> work(std::max(nx, ny)), col(ny);
+#pragma omp for schedule(static)
+        for (int j = 0; j < ny; ++j) fx.exec(a + (size_t)j * nx, work.data(), sign);
+#pragma omp for schedule(static)
+        for (int i = 0; i < nx; ++i) {
+            for (int j = 0; j < ny; ++j) col[j] = a[i + (size_t)j * nx];
+            fy.exec(col.data(), work.data(), sign);
+            for (int j = 0; j < ny; ++j) a[i + (size_t)j * nx] = col[j];
+        }
+    }
+}
+
+// Padded linear convolution (ConvolutionPlanHelpers / ConvolutionPlan / conv!,
+// src/convolutions.jl:1-15,21-56,81-106). The rfft/irfft pair is restated as the full complex
+// transform of the real padded array followed by the real part, which is the same mathematics.
+struct ConvPlan {
+    int nx = 0, ny = 0, nfx = 0, nfy = 0;
+    FFT fx, fy;
+    void init(int nx_, int ny_) {
+        nx = nx_;
+        ny = ny_;
+        nfx = nextfastfft(2 * nx - 1);
+        nfy = nextfastfft(2 * ny - 1);
+        fx = FFT(nfx);
+        fy = FFT(nfy);
+    }
+    std::vector<cd> spectrum(const double* kernel) const {
+        std::vector<cd> k((size_t)nfx * nfy, cd(0.0, 0.0));
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) k[i + (size_t)j * nfx] = kernel[i + (size_t)j * nx];
+        fft2d(k.data(), nfx, nfy, fx, fy, -1);
+        return k;
+    }
+    // output_cropped = (irfft(rfft(pad(input)) .* kernel_fft))[1:2nx-1, 1:2ny-1]
+    void conv_cropped(const double* input, const std::vector<cd>& khat, vec& cropped) const {
+        std::vector<cd> a((size_t)nfx * nfy, cd(0.0, 0.0));
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) a[i + (size_t)j * nfx] = input[i + (size_t)j * nx];
+        fft2d(a.data(), nfx, nfy, fx, fy, -1);
+        for (size_t k = 0; k < a.size(); ++k) a[k] *= khat[k];
+        fft2d(a.data(), nfx, nfy, fx, fy, +1);
+        double s = 1.0 / ((double)nfx * nfy);
+        int ex = 2 * nx - 1, ey = 2 * ny - 1;
+        cropped.resize((size_t)ex * ey);
+        for (int j = 0; j < ey; ++j)
+            for (int i = 0; i < ex; ++i) cropped[i + (size_t)j * ex] = a[i + (size_t)j * nfx].real() * s;
+    }
+};
+
+// apply_bc!(X, OffsetBC) for a CornerBC (src/boundary_conditions.jl:144-148,199-207,221-243):
+// X .-= sum(W .* X) - x_border with W = 1/4 on the four corners.
+void apply_corner_bc(double* X, int nx, int ny, double x_border) {
+    double a = 0.25 * X[0], b = 0.25 * X[nx - 1];
+    double c = 0.25 * X[(size_t)(ny - 1) * nx], d = 0.25 * X[(nx - 1) + (size_t)(ny - 1) * nx];
+    double off = ((a + b) + (c + d)) - x_border;
+    size_t N = (size_t)nx * ny;
+    for (size_t k = 0; k < N; ++k) X[k] -= off;
+}
+
+// samesize_conv!(output, input, p, h, domain, bc, ExtendedBCSpace) (src/convolutions.jl:134-143)
+void samesize_conv_ext(double* out, const double* input, const ConvPlan& cp, const std::vector<cd>& khat,
+                       int i1, int i2, int j1, int j2, double x_border) {
+    vec crop;
+    cp.conv_cropped(input, khat, crop);
+    int ex = 2 * cp.nx - 1, ey = 2 * cp.ny - 1;
+    apply_corner_bc(crop.data(), ex, ey, x_border);
+    int nx = cp.nx;
+    for (int j = j1; j <= j2; ++j)
+        for (int i = i1; i <= i2; ++i)
+            out[(i - i1) + (size_t)(j - j1) * nx] = crop[(i - 1) + (size_t)(j - 1) * ex];
+}
+
+// height_above_floatation (src/topography.jl:20-23)
+inline double hab(double H, double zb, double zss, double rho_sw_ice) {
+    return std::max(H + std::min(zb - zss, 0.0) * rho_sw_ice, 0.0);
+}
+
+}  // namespace
+
+// =======================================================================================
+// Context
+// =======================================================================================
+struct fi_ctx {
+    fi_config cfg;
+    std::string err;
+    int nx = 0, ny = 0, mx = 0, my = 0;
+    size_t N = 0;
+    double dx = 0, dy = 0, A = 0, rho_sw_ice = 0;
+    int i1 = 0, i2 = 0, j1 = 0, j2 = 0;
+    vec pd;
+    FFT vfx, vfy;  // n x n complex transforms of the viscous operator (src/tools.jl:100-109)
+    ConvPlan cp;
+    std::vector<cd> ge_hat, gg_hat;
+    // static inputs
+    vec D, eta, spinv, maskactive, green_e, green_g, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ice_ref_in;
+    bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false,
+         has_href = false;
+    // ice forcing
+    int ice_mode = 0;  // 0 external, 1 snapshots, 2 separable
+    vec ice_t, ice_a, ice_H;
+    int ice_flat = 0;
+    vec tout;
+    // reference state
+    vec H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
+    std::vector<uint8_t> mg_ref, mo_ref;
+    // current state (src/state.jl:49-77)
+    vec u, ue, dudt, H_ice, H_af, H_water, z_b, dz_ss, z_ss;
+    std::vector<uint8_t> mg, mo;
+    vec ca_ice, ca_sw, ca_sed, ca_litho, ca_mantle, ca_load, ca_full;
+    double V_af = 0, V_pov = 0, V_den = 0, delta_V = 0, z_bsl = 0;
+    int64_t count = 0;
+    double t0 = 0, t_end = 0, t_timer = 0;
+    bool initialized = false;
+    // preallocated buffers (PreAllocated, src/tools.jl:4-16)
+    vec rhs, bxx, byy, bx, bxy, Mxx, Myy, Mxy, buf;
+    std::vector<cd> fftrhs;
+    // integrator
+    bool integ = false;
+    double t = 0, dt = 0, qold = 1e-4, q11 = 1, EEst = 0;
+    vec uprev, k1, k2, k3, k4, tmp;
+    int64_t nrhs = 0, nacc = 0, nrej = 0, kout = 0;
+    int fail = 0;
+};
+
+namespace {
+
+int fail(fi_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+// TimeInterpolation2D interpolate! (src/interpolations.jl:51-74)
+int interpolate_ice(fi_ctx* c, double t, double* out) {
+    const vec& tv = c->ice_t;
+    int nt = (int)tv.size();
+    size_t N = c->N;
+    auto snap = [&](int k, size_t p) {
+        return c->ice_mode == 1 ? c->ice_H[(size_t)k * N + p] : c->ice_a[k] * c->ice_H[p];
+    };
+    if (t < tv.front()) {
+        if (!c->ice_flat) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+        for (size_t p = 0; p < N; ++p) out[p] = snap(0, p);
+    } else if (t > tv.back()) {
+        if (!c->ice_flat) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+        for (size_t p = 0; p < N; ++p) out[p] = snap(nt - 1, p);
+    } else {
+        int i = (int)(std::lower_bound(tv.begin(), tv.end(), t) - tv.begin());  // searchsortedfirst - 1 (0-based)
+        if (i < nt && tv[i] == t) {
+            for (size_t p = 0; p < N; ++p) out[p] = snap(i, p);
+        } else {
+            i -= 1;
+            double f = (t - tv[i]);
+            double dtk = tv[i + 1] - tv[i];
+            for (size_t p = 0; p < N; ++p) {
+                double a = snap(i, p), b = snap(i + 1, p);
+                out[p] = a + (b - a) / dtk * f;
+            }
+        }
+    }
+    return FI_OK;
+}
+
+// watercolumn! (src/loads.jl:69-85)
+void watercolumn(fi_ctx* c, const vec& H, const std::vector<uint8_t>& mg, const vec& zb, const vec& zss, vec& Hw) {
+    double r = c->cfg.rho_ice / c->cfg.rho_seawater;
+    for (size_t p = 0; p < c->N; ++p) {
+        double b = std::max(zss[p] - zb[p], 0.0);
+        Hw[p] = (H[p] <= 1 ? 1.0 : 0.0) * b + (mg[p] ? 0.0 : 1.0) * (H[p] > 1 ? 1.0 : 0.0) * (b - H[p] * r);
+    }
+}
+
+double sum_times_A(fi_ctx* c, const vec& X) {
+    double s = 0.0;
+    for (size_t p = 0; p < c->N; ++p) s += X[p] * c->A;
+    return s;
+}
+
+// update_V_af!/update_V_den!/update_V_pov! (src/sealevel.jl:273-331)
+void update_volumes(fi_ctx* c) {
+    const fi_config& g = c->cfg;
+    if (g.volume_contribution == FI_GOELZER_CONTRIBUTION)
+        c->V_af = sum_times_A(c, c->H_af) * g.rho_ice / g.rho_seawater;
+    else c->V_af = 0.0;
+    if (g.density_contribution == FI_GOELZER_CONTRIBUTION) {
+        double f = g.rho_ice / g.rho_water - g.rho_ice / g.rho_seawater;
+        c->V_den = sum_times_A(c, c->H_ice) * f;
+    } else c->V_den = 0.0;
+    if (g.adjustment_contribution == FI_GOELZER_CONTRIBUTION) {
+        double s = 0.0;
+        for (size_t p = 0; p < c->N; ++p) s += std::max(c->z_ss[p] - c->z_b[p], 0.0) * c->A;
+        c->V_pov = s;
+    } else c->V_pov = 0.0;
+}
+
+void update_Haf(fi_ctx* c) {
+    for (size_t p = 0; p < c->N; ++p) c->H_af[p] = hab(c->H_ice[p], c->z_b[p], c->z_ss[p], c->rho_sw_ice);
+}
+void columnanom_ice(fi_ctx* c) {
+    for (size_t p = 0; p < c->N; ++p) c->ca_ice[p] = c->cfg.rho_ice * (c->H_af[p] - c->H_af_ref[p]);
+}
+
+// The sparse block of update_diagnostics! (src/simulation.jl:359-381)
+int sparse_block(fi_ctx* c) {
+    const fi_config& g = c->cfg;
+    size_t N = c->N;
+    // update_elasticresponse! (src/deformation.jl:175-189)
+    if (g.lithosphere == FI_RIGID_LITHOSPHERE) {
+        std::fill(c->ue.begin(), c->ue.end(), 0.0);
+    } else {
+        for (size_t p = 0; p < N; ++p) c->buf[p] = c->ca_load[p] * 1.0;  // K^2 = 1
+        samesize_conv_ext(c->ue.data(), c->buf.data(), c->cp, c->ge_hat, c->i1, c->i2, c->j1, c->j2,
+                          g.elastic_x_border);
+    }
+    // columnanom_litho! (src/loads.jl:41-44)
+    for (size_t p = 0; p < N; ++p) c->ca_litho[p] = g.rho_litho * (c->ue[p] - c->ue_ref[p]);
+    // internal_update_bsl! -> update_delta_V!, update_bsl!(ConstantBSL) (src/sealevel.jl:242-262)
+    double Vold = c->V_af + c->V_den + c->V_pov;
+    update_volumes(c);
+    double Vnew = c->V_af + c->V_den + c->V_pov;
+    c->delta_V = Vnew - Vold;
+    c->z_bsl = g.bsl_z;
+    // update_dz_ss! (src/sealevel.jl:173-201)
+    if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE) {
+        if (g.lithosphere_column == FI_COMPRESSIBLE_LITHOSPHERE_COLUMN) {
+            for (size_t p = 0; p < N; ++p) c->buf[p] = c->ca_load[p] + c->maskactive[p] * c->ca_mantle[p];
+            for (size_t p = 0; p < N; ++p) c->buf[p] = c->A * c->buf[p];
+        } else {
+            for (size_t p = 0; p < N; ++p) c->buf[p] = c->A * c->ca_full[p];
+        }
+        samesize_conv_ext(c->dz_ss.data(), c->buf.data(), c->cp, c->gg_hat, c->i1, c->i2, c->j1, c->j2,
+                          g.ssh_x_border);
+    }
+    // update_z_ss! (src/sealevel.jl:228-231)
+    for (size_t p = 0; p < N; ++p) c->z_ss[p] = c->z_ss_ref[p] + c->dz_ss[p] + c->z_bsl;
+    update_Haf(c);
+    // update_maskgrounded!, update_maskocean! (src/topography.jl:1-4,25-28)
+    for (size_t p = 0; p < N; ++p) c->mg[p] = c->H_af[p] > 0;
+    for (size_t p = 0; p < N; ++p) c->mo[p] = (c->z_ss[p] - c->z_b[p] > 0) && !c->mg[p];
+    // columnanom_water! (src/loads.jl:56-67)
+    watercolumn(c, c->H_ice, c->mg, c->z_b, c->z_ss, c->H_water);
+    if (g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD) {
+        for (size_t p = 0; p < N; ++p)
+            c->ca_sw[p] = g.rho_seawater * (c->z_ss[p] - c->z_ss_ref[p]) * (c->mo[p] ? 1.0 : 0.0) * c->maskactive[p];
+    } else {
+        std::fill(c->ca_sw.begin(), c->ca_sw.end(), 0.0);
+    }
+    columnanom_ice(c);
+    c->count += 1;
+    return FI_OK;
+}
+
+// update_deformation_rhs! (src/deformation.jl:110-131) then
+// update_dudt!(MaxwellMantle, LaterallyVariableLithosphere) (src/deformation.jl:91-103)
+void update_dudt_lvelva(fi_ctx* c, double* u, double* dudt) {
+    const fi_config& g = c->cfg;
+    int nx = c->nx, ny = c->ny;
+    size_t N = c->N;
+    double nu = g.litho_poissonratio;
+    for (size_t p = 0; p < N; ++p) c->rhs[p] = -g.g * c->ca_full[p];
+    second_derivatives(c->bxx.data(), c->byy.data(), c->bx.data(), c->bxy.data(), u, u, u, nx, ny, c->dx, c->dy);
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < N; ++p) {
+        c->Mxx[p] = -c->D[p] * (nu * c->byy[p] + c->bxx[p]);
+        c->Myy[p] = -c->D[p] * (nu * c->bxx[p] + c->byy[p]);
+        c->Mxy[p] = -c->D[p] * (1 - nu) * c->bxy[p];
+    }
+    second_derivatives(c->bxx.data(), c->byy.data(), c->bx.data(), c->bxy.data(), c->Mxx.data(), c->Myy.data(),
+                       c->Mxy.data(), nx, ny, c->dx, c->dy);
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < N; ++p) c->rhs[p] += c->bxx[p] + (2 * c->bxy[p] + c->byy[p]);
+    // samesize_conv!(rhs, ..., EmptyConvolution) is a no-op (src/convolutions.jl:120-122)
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < N; ++p) c->fftrhs[p] = cd(c->rhs[p] * 1.0 / (2 * c->eta[p]), 0.0);
+    fft2d(c->fftrhs.data(), nx, ny, c->vfx, c->vfy, -1);
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < N; ++p) c->fftrhs[p] *= c->spinv[p];
+    fft2d(c->fftrhs.data(), nx, ny, c->vfx, c->vfy, +1);
+    double s = 1.0 / ((double)nx * ny);
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < N; ++p) dudt[p] = (c->fftrhs[p].real() * s) * g.seconds_per_year;
+    apply_corner_bc(dudt, nx, ny, g.viscous_x_border);
+}
+
+// update_diagnostics!(dudt, u, sim, t) (src/simulation.jl:335-393)
+int update_diagnostics(fi_ctx* c, double* dudt, double* u, double t) {
+    const fi_config& g = c->cfg;
+    size_t N = c->N;
+    c->t_timer = t;
+    c->nrhs += 1;
+    apply_corner_bc(u, c->nx, c->ny, g.viscous_x_border);
+    // update_bedrock! (src/topography.jl:30-34)
+    if (u != c->u.data()) std::memcpy(c->u.data(), u, N * sizeof(double));
+    for (size_t p = 0; p < N; ++p) c->z_b[p] = c->z_b_ref[p] + c->ue[p] + c->u[p];
+    // columnanom_mantle! (src/loads.jl:36-39)
+    for (size_t p = 0; p < N; ++p) c->ca_mantle[p] = g.rho_uppermantle * (c->u[p] - c->u_ref[p]);
+    // apply_bc!(H_ice, t, ice_thickness) (src/boundary_conditions.jl:55-62)
+    if (c->ice_mode != 0) {
+        int st = interpolate_ice(c, t, c->H_ice.data());
+        if (st) return st;
+    }
+    update_Haf(c);
+    columnanom_ice(c);
+    if (((t - c->t0) / g.dt_sparse) >= (double)c->count) {
+        int st = sparse_block(c);
+        if (st) return st;
+    }
+    // columnanom_load!, columnanom_full! (src/loads.jl:96-106)
+    for (size_t p = 0; p < N; ++p) c->ca_load[p] = c->maskactive[p] * (c->ca_ice[p] + c->ca_sw[p] + c->ca_sed[p]);
+    for (size_t p = 0; p < N; ++p)
+        c->ca_full[p] = c->ca_load[p] + c->maskactive[p] * (c->ca_litho[p] + c->ca_mantle[p]);
+    // update_dudt! (src/deformation.jl:20-27,91-103)
+    if (g.mantle == FI_RIGID_MANTLE) {
+        std::fill(dudt, dudt + N, 0.0);
+    } else {
+        update_dudt_lvelva(c, u, dudt);
+    }
+    if (dudt != c->dudt.data()) std::memcpy(c->dudt.data(), dudt, N * sizeof(double));
+    return FI_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// BS3 (Bogacki-Shampine 3(2), FSAL) with a PI step-size controller, restated from the
+// published method (Bogacki & Shampine 1989; Hairer, Norsett & Wanner II.4 initial step;
+// PI control after Gustafsson / Soderlind) as OrdinaryDiffEq's BS3 uses it by default
+// (src/simulation.jl:11-15,255-257). Controller constants: beta1 = 7/30, beta2 = 2/15,
+// gamma = 0.9, qmin = 0.2, qmax = 10, qoldinit = 1e-4 -- parity of the step sequence is
+// unpinned (no reference source holds them).
+// ---------------------------------------------------------------------------------------
+const double A21 = 0.5, A32 = 0.75, A41 = 2.0 / 9.0, A42 = 1.0 / 3.0, A43 = 4.0 / 9.0;
+const double C2 = 0.5, C3 = 0.75;
+const double BT1 = 2.0 / 9.0 - 7.0 / 24.0, BT2 = 1.0 / 3.0 - 0.25, BT3 = 4.0 / 9.0 - 1.0 / 3.0, BT4 = -0.125;
+const double BETA1 = 7.0 / 30.0, BETA2 = 2.0 / 15.0, GAMMA = 0.9, QMIN = 0.2, QMAX = 10.0, QOLDINIT = 1e-4;
+
+double rms(const vec& a) {
+    double s = 0.0;
+    for (double x : a) s += x * x;
+    return std::sqrt(s / (double)a.size());
+}
+
+}  // namespace
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+extern "C" {
+
+int32_t fi_abi_version(void) { return FI_ABI_VERSION; }
+const char* fi_engine_name(void) { return "cpu-oracle-fp64"; }
+
+void fi_default_config(fi_config* c, int32_t nx, int32_t ny, double Wx, double Wy) {
+    std::memset(c, 0, sizeof(*c));
+    c->nx = nx;
+    c->ny = ny;
+    c->Wx = Wx;
+    c->Wy = Wy;
+    c->g = 9.8;
+    c->seconds_per_year = 60.0 * 60.0 * 24.0 * 365.25;
+    c->rho_ice = 0.910e3;
+    c->rho_water = 1e3;
+    c->rho_seawater = 1.023e3;
+    c->r_pole = 6357e3;
+    c->mE = 5.972e24;
+    c->litho_poissonratio = 0.28;
+    c->rho_uppermantle = 3.4e3;
+    c->rho_litho = 3.2e3;
+    c->lithosphere = FI_LATERALLY_VARIABLE_LITHOSPHERE;
+    c->mantle = FI_MAXWELL_MANTLE;
+    c->lithosphere_column = FI_INCOMPRESSIBLE_LITHOSPHERE_COLUMN;
+    c->sea_surface = FI_LATERALLY_CONSTANT_SEA_SURFACE;
+    c->sealevel_load = FI_NO_SEALEVEL_LOAD;
+    c->volume_contribution = FI_GOELZER_CONTRIBUTION;
+    c->density_contribution = FI_GOELZER_CONTRIBUTION;
+    c->adjustment_contribution = FI_NO_CONTRIBUTION;
+    c->bsl_z = 0.0;
+    c->dt_sparse = 10.0;
+    c->reltol = 1e-5;
+    c->abstol = 1e-6;
+}
+
+int fi_create(const fi_config* cfg, fi_ctx** out) {
+    if (!cfg || !out) return FI_ERR_INVALID;
+    *out = nullptr;
+    if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->Wx > 0) || !(cfg->Wy > 0)) return FI_ERR_INVALID;
+    if (cfg->mantle == FI_RELAXED_MANTLE) return FI_ERR_UNSUPPORTED;
+    if (cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE)
+        return FI_ERR_UNSUPPORTED;  // implicit Bueler step (src/deformation.jl:52-89) is not this path
+    fi_ctx* c = new fi_ctx();
+    c->cfg = *cfg;
+    c->nx = cfg->nx;
+    c->ny = cfg->ny;
+    c->N = (size_t)c->nx * c->ny;
+    c->mx = c->nx / 2;
+    c->my = c->ny / 2;
+    c->dx = 2 * cfg->Wx / c->nx;
+    c->dy = 2 * cfg->Wy / c->ny;
+    c->A = c->dx * c->dy;  // (dx*dy) .* K.^2, K = 1 (src/domain.jl:167)
+    c->rho_sw_ice = cfg->rho_seawater / cfg->rho_ice;
+    samesize_conv_indices(c->nx, c->mx, c->i1, c->i2);
+    samesize_conv_indices(c->ny, c->my, c->j1, c->j2);
+    c->pd = pseudodiff(c->nx, c->ny, cfg->Wx, cfg->Wy);
+    c->vfx = FFT(c->nx);
+    c->vfy = FFT(c->ny);
+    c->cp.init(c->nx, c->ny);
+    size_t N = c->N;
+    c->maskactive.assign(N, 1.0);
+    c->u_ref.assign(N, 0.0);
+    c->ue_ref.assign(N, 0.0);
+    c->dz_ss_ref.assign(N, 0.0);
+    c->z_b_ref.assign(N, 1e6);
+    for (vec* v : {&c->rhs, &c->bxx, &c->byy, &c->bx, &c->bxy, &c->Mxx, &c->Myy, &c->Mxy, &c->buf}) v->assign(N, 0.0);
+    c->fftrhs.assign(N, cd(0, 0));
+    *out = c;
+    return FI_OK;
+}
+
+void fi_destroy(fi_ctx* c) { delete c; }
+const char* fi_last_error(const fi_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
+    if (!c || !h || n != c->N) return fail(c, FI_ERR_INVALID, "fi_set_field: size mismatch");
+    vec v(h, h + n);
+    switch (f) {
+        case FI_LITHO_RIGIDITY: c->D = v; c->has_D = true; break;
+        case FI_EFFECTIVE_VISCOSITY: c->eta = v; c->has_eta = true; break;
+        case FI_SCALED_PSEUDODIFF_INV: c->spinv = v; c->has_spinv = true; break;
+        case FI_MASK_ACTIVE: c->maskactive = v; break;
+        case FI_GREEN_ELASTIC: c->green_e = v; c->has_ge = true; break;
+        case FI_GREEN_GEOID: c->green_g = v; c->has_gg = true; break;
+        case FI_REF_U: c->u_ref = v; break;
+        case FI_REF_UE: c->ue_ref = v; break;
+        case FI_REF_DZ_SS: c->dz_ss_ref = v; break;
+        case FI_REF_Z_B: c->z_b_ref = v; break;
+        case FI_REF_H_ICE: c->H_ice_ref_in = v; c->has_href = true; break;
+        case FI_U:
+            if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_field(FI_U) before fi_init");
+            c->u = v;
+            if (c->integ) c->uprev = v;
+            break;
+        case FI_H_ICE:
+            if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_field(FI_H_ICE) before fi_init");
+            c->H_ice = v;
+            break;
+        default: return fail(c, FI_ERR_INVALID, "fi_set_field: field not writable");
+    }
+    return FI_OK;
+}
+
+int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
+    if (!c || !h || n != c->N) return fail(c, FI_ERR_INVALID, "fi_get_field: size mismatch");
+    if (!c->initialized && f >= FI_U) return fail(c, FI_ERR_STATE, "fi_get_field before fi_init");
+    const vec* v = nullptr;
+    const std::vector<uint8_t>* m = nullptr;
+    switch (f) {
+        case FI_LITHO_RIGIDITY: v = &c->D; break;
+        case FI_EFFECTIVE_VISCOSITY: v = &c->eta; break;
+        case FI_SCALED_PSEUDODIFF_INV: v = &c->spinv; break;
+        case FI_MASK_ACTIVE: v = &c->maskactive; break;
+        case FI_GREEN_ELASTIC: v = &c->green_e; break;
+        case FI_GREEN_GEOID: v = &c->green_g; break;
+        case FI_REF_U: v = &c->u_ref; break;
+        case FI_REF_UE: v = &c->ue_ref; break;
+        case FI_REF_DZ_SS: v = &c->dz_ss_ref; break;
+        case FI_REF_Z_B: v = &c->z_b_ref; break;
+        case FI_REF_H_ICE: v = &c->H_ice_ref; break;
+        case FI_U: v = c->integ ? &c->uprev : &c->u; break;
+        case FI_DUDT: v = &c->dudt; break;
+        case FI_UE: v = &c->ue; break;
+        case FI_DZ_SS: v = &c->dz_ss; break;
+        case FI_Z_SS: v = &c->z_ss; break;
+        case FI_Z_B: v = &c->z_b; break;
+        case FI_H_ICE: v = &c->H_ice; break;
+        case FI_H_AF: v = &c->H_af; break;
+        case FI_H_WATER: v = &c->H_water; break;
+        case FI_MASK_GROUNDED: m = &c->mg; break;
+        case FI_MASK_OCEAN: m = &c->mo; break;
+        case FI_CANOM_ICE: v = &c->ca_ice; break;
+        case FI_CANOM_SEAWATER: v = &c->ca_sw; break;
+        case FI_CANOM_LITHO: v = &c->ca_litho; break;
+        case FI_CANOM_MANTLE: v = &c->ca_mantle; break;
+        case FI_CANOM_LOAD: v = &c->ca_load; break;
+        case FI_CANOM_FULL: v = &c->ca_full; break;
+        default: return fail(c, FI_ERR_INVALID, "fi_get_field: unknown field");
+    }
+    if (v) {
+        if (v->size() != n) return fail(c, FI_ERR_STATE, "fi_get_field: field not set");
+        std::memcpy(h, v->data(), n * sizeof(double));
+    } else {
+        for (size_t p = 0; p < n; ++p) h[p] = (*m)[p] ? 1.0 : 0.0;
+    }
+    return FI_OK;
+}
+
+int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H, int32_t flat) {
+    if (!c || !t || !H || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_ice_snapshots: bad args");
+    for (int k = 1; k < nt; ++k)
+        if (!(t[k] > t[k - 1])) return fail(c, FI_ERR_INVALID, "ice times must increase");
+    c->ice_mode = 1;
+    c->ice_t.assign(t, t + nt);
+    c->ice_H.assign(H, H + (size_t)nt * c->N);
+    c->ice_flat = flat;
+    return FI_OK;
+}
+
+int fi_set_ice_separable(fi_ctx* c, const double* t, const double* a, int32_t nt, const double* H, int32_t flat) {
+    if (!c || !t || !a || !H || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_ice_separable: bad args");
+    for (int k = 1; k < nt; ++k)
+        if (!(t[k] > t[k - 1])) return fail(c, FI_ERR_INVALID, "ice times must increase");
+    c->ice_mode = 2;
+    c->ice_t.assign(t, t + nt);
+    c->ice_a.assign(a, a + nt);
+    c->ice_H.assign(H, H + c->N);
+    c->ice_flat = flat;
+    return FI_OK;
+}
+
+int fi_set_output_times(fi_ctx* c, const double* t, int32_t n) {
+    if (!c || (n > 0 && !t) || n < 0) return fail(c, FI_ERR_INVALID, "fi_set_output_times: bad args");
+    c->tout.assign(t, t + n);
+    std::sort(c->tout.begin(), c->tout.end());
+    c->kout = 0;
+    return FI_OK;
+}
+
+// Simulation(...) (src/simulation.jl:105-156)
+int fi_init(fi_ctx* c, double t0, double t_end) {
+    if (!c) return FI_ERR_INVALID;
+    const fi_config& g = c->cfg;
+    size_t N = c->N;
+    if (g.mantle == FI_MAXWELL_MANTLE && (!c->has_D || !c->has_eta || !c->has_spinv))
+        return fail(c, FI_ERR_INVALID, "litho rigidity, effective viscosity and scaled_pseudodiff_inv required");
+    if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge)
+        return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
+    if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
+        return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
+    if (g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD) {
+        double s = 0;
+        for (double m : c->maskactive) s += m;
+        if (s > 0.6 * c->nx * c->ny)
+            return fail(c, FI_ERR_INVALID, "Mask defining regions of active load must not cover more than 60% "
+                                           "of the cells when using an interactive sea level.");
+    }
+    if (c->has_ge) c->ge_hat = c->cp.spectrum(c->green_e.data());
+    if (c->has_gg) c->gg_hat = c->cp.spectrum(c->green_g.data());
+    c->t0 = t0;
+    c->t_end = t_end;
+    c->t_timer = t0;
+    // H_ice_ref = apply_bc!(zeros, t0, ice_thickness)
+    c->H_ice_ref.assign(N, 0.0);
+    if (c->has_href) c->H_ice_ref = c->H_ice_ref_in;
+    else if (c->ice_mode != 0) {
+        int st = interpolate_ice(c, t0, c->H_ice_ref.data());
+        if (st) return st;
+    }
+    c->z_ss_ref.resize(N);
+    for (size_t p = 0; p < N; ++p) c->z_ss_ref[p] = g.bsl_z + c->dz_ss_ref[p];
+    c->mg_ref.resize(N);
+    c->mo_ref.resize(N);
+    c->H_af_ref.resize(N);
+    for (size_t p = 0; p < N; ++p) {
+        double h = hab(c->H_ice_ref[p], c->z_b_ref[p], c->z_ss_ref[p], g.rho_seawater / g.rho_ice);
+        c->mg_ref[p] = h > 0;
+        c->mo_ref[p] = (c->z_ss_ref[p] - c->z_b_ref[p] > 0) && !c->mg_ref[p];
+        c->H_af_ref[p] = h;
+    }
+    c->H_water_ref.resize(N);
+    watercolumn(c, c->H_ice_ref, c->mg_ref, c->z_b_ref, c->z_ss_ref, c->H_water_ref);
+    // CurrentState(domain, ref, z_bsl) (src/state.jl:80-103)
+    c->u = c->u_ref;
+    c->ue = c->ue_ref;
+    c->dudt.assign(N, 0.0);
+    c->H_ice = c->H_ice_ref;
+    c->H_af = c->H_af_ref;
+    c->H_water = c->H_water_ref;
+    for (vec* v : {&c->ca_ice, &c->ca_sw, &c->ca_sed, &c->ca_litho, &c->ca_mantle, &c->ca_load, &c->ca_full})
+        v->assign(N, 0.0);
+    c->z_b = c->z_b_ref;
+    c->dz_ss.assign(N, 0.0);
+    c->z_ss = c->z_ss_ref;
+    c->V_af = c->V_pov = c->V_den = c->delta_V = 0.0;
+    c->z_bsl = g.bsl_z;
+    c->mg = c->mg_ref;
+    c->mo = c->mo_ref;
+    c->count = 0;
+    c->initialized = true;
+    c->integ = false;
+    c->nrhs = c->nacc = c->nrej = 0;
+    c->kout = 0;
+    return FI_OK;
+}
+
+int fi_rhs(fi_ctx* c, double t, double* u, double* dudt) {
+    if (!c || !u || !dudt) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_rhs before fi_init");
+    return update_diagnostics(c, dudt, u, t);
+}
+
+int fi_rhs_device(fi_ctx* c, double t, void* u, void* dudt) {
+    return fi_rhs(c, t, (double*)u, (double*)dudt);
+}
+
+int fi_sparse_update(fi_ctx* c, double t) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_sparse_update before fi_init");
+    c->t_timer = t;
+    return sparse_block(c);
+}
+
+// init_problem! (src/simulation.jl:305-311), FSAL initialisation and the initial step
+int fi_integrator_init(fi_ctx* c) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_integrator_init before fi_init");
+    const fi_config& g = c->cfg;
+    size_t N = c->N;
+    update_volumes(c);
+    int st = update_diagnostics(c, c->dudt.data(), c->u.data(), c->t0);
+    if (st) return st;
+    c->t = c->t0;
+    c->uprev = c->u;
+    for (vec* v : {&c->k1, &c->k2, &c->k3, &c->k4, &c->tmp}) v->assign(N, 0.0);
+    st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t);  // FSAL: fsalfirst = f(u0, t0)
+    if (st) return st;
+    // initial step (Hairer, Norsett & Wanner, Solving ODEs I, II.4)
+    vec sk(N), tq(N);
+    for (size_t p = 0; p < N; ++p) sk[p] = g.abstol + std::fabs(c->uprev[p]) * g.reltol;
+    for (size_t p = 0; p < N; ++p) tq[p] = c->uprev[p] / sk[p];
+    double d0 = rms(tq);
+    for (size_t p = 0; p < N; ++p) tq[p] = c->k1[p] / sk[p];
+    double d1 = rms(tq);
+    double dtmax = c->t_end - c->t0;
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+    dt0 = std::min(dt0, dtmax);
+    for (size_t p = 0; p < N; ++p) c->tmp[p] = c->uprev[p] + dt0 * c->k1[p];
+    st = update_diagnostics(c, c->k2.data(), c->tmp.data(), c->t + dt0);
+    if (st) return st;
+    for (size_t p = 0; p < N; ++p) tq[p] = (c->k2[p] - c->k1[p]) / sk[p];
+    double d2 = rms(tq) / dt0;
+    double mx = std::max(d1, d2);
+    double dt1 = (mx <= 1e-15) ? std::max(1e-6, dt0 * 1e-3) : std::pow(10.0, -(2.0 + std::log10(mx)) / 3.0);
+    c->dt = std::min(std::min(100 * dt0, dt1), dtmax);
+    c->qold = QOLDINIT;
+    c->integ = true;
+    c->kout = 0;
+    return FI_OK;
+}
+
+int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_integrate before fi_integrator_init");
+    const fi_config& g = c->cfg;
+    size_t N = c->N;
+    if (t_target < c->t) return fail(c, FI_ERR_INVALID, "fi_integrate: target before current time");
+    while (c->t < t_target) {
+        // next stop: the first output time after t, or t_target (modify_dt_for_tstops!)
+        double stop = t_target;
+        for (double to : c->tout)
+            if (to > c->t) { stop = std::min(stop, to); break; }
+        double dt = c->dt;
+        bool clipped = false;
+        if (c->t + dt >= stop) { dt = stop - c->t; clipped = true; }
+        if (!(dt > 1e-14 * std::max(1.0, std::fabs(c->t)))) return fail(c, FI_ERR_DT_MIN, "dt below minimum");
+        int st;
+        double t = c->t;
+        for (size_t p = 0; p < N; ++p) c->tmp[p] = c->uprev[p] + dt * A21 * c->k1[p];
+        if ((st = update_diagnostics(c, c->k2.data(), c->tmp.data(), t + C2 * dt))) return st;
+        for (size_t p = 0; p < N; ++p) c->tmp[p] = c->uprev[p] + dt * A32 * c->k2[p];
+        if ((st = update_diagnostics(c, c->k3.data(), c->tmp.data(), t + C3 * dt))) return st;
+        vec unew(N);
+        for (size_t p = 0; p < N; ++p) unew[p] = c->uprev[p] + dt * (A41 * c->k1[p] + A42 * c->k2[p] + A43 * c->k3[p]);
+        if ((st = update_diagnostics(c, c->k4.data(), unew.data(), t + dt))) return st;
+        double s = 0.0;
+        for (size_t p = 0; p < N; ++p) {
+            double ut = dt * (BT1 * c->k1[p] + BT2 * c->k2[p] + BT3 * c->k3[p] + BT4 * c->k4[p]);
+            double sc = g.abstol + std::max(std::fabs(c->uprev[p]), std::fabs(unew[p])) * g.reltol;
+            double e = ut / sc;
+            s += e * e;
+        }
+        double EEst = std::sqrt(s / (double)N);
+        c->EEst = EEst;
+        double q;
+        if (!std::isfinite(EEst)) {
+            c->q11 = 1.0 / QMIN * GAMMA * 10;
+            q = 1.0 / QMIN;
+        } else if (EEst == 0.0) {
+            q = 1.0 / QMAX;
+        } else {
+            double q11 = std::pow(EEst, BETA1);
+            c->q11 = q11;
+            q = q11 / std::pow(c->qold, BETA2);
+            q = std::max(1.0 / QMAX, std::min(1.0 / QMIN, q / GAMMA));
+        }
+        if (std::isfinite(EEst) && EEst <= 1.0) {
+            // accept (step_accept_controller!)
+            c->qold = std::max(EEst, QOLDINIT);
+            c->t = clipped ? stop : t + dt;
+            c->uprev.swap(unew);
+            c->k1.swap(c->k4);
+            c->dt = dt / q;
+            c->nacc += 1;
+            // output callback (nout_condition / nout_affect!) then the FSAL re-evaluation that
+            // a DiscreteCallback's u_modified! triggers
+            if (c->kout < (int64_t)c->tout.size() && c->t >= c->tout[c->kout]) {
+                c->u = c->uprev;
+                if (cb) cb(user, c->t, (int32_t)c->kout);
+                c->kout += 1;
+                if ((st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t))) return st;
+            }
+        } else {
+            // reject (step_reject_controller!)
+            c->dt = dt / std::min(1.0 / QMIN, c->q11 / GAMMA);
+            c->nrej += 1;
+        }
+    }
+    c->u = c->uprev;
+    return FI_OK;
+}
+
+int fi_get_stats(fi_ctx* c, fi_stats* o) {
+    if (!c || !o) return FI_ERR_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->t = c->integ ? c->t : c->t_timer;
+    o->dt = c->dt;
+    o->n_rhs = c->nrhs;
+    o->n_accept = c->nacc;
+    o->n_reject = c->nrej;
+    o->count_sparse_updates = c->count;
+    o->n_output = c->kout;
+    o->V_af = c->V_af;
+    o->V_den = c->V_den;
+    o->V_pov = c->V_pov;
+    o->delta_V = c->delta_V;
+    o->z_bsl = c->z_bsl;
+    o->last_EEst = c->EEst;
+    return FI_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// ref_* hooks for the unit checks
+// ---------------------------------------------------------------------------------------
+void ref_fft(int n, const double* re, const double* im, double* ore, double* oim, int sign) {
+    FFT f(n);
+    std::vector<cd> a(n), w(n);
+    for (int k = 0; k < n; ++k) a[k] = cd(re[k], im[k]);
+    f.exec(a.data(), w.data(), sign);
+    for (int k = 0; k < n; ++k) { ore[k] = a[k].real(); oim[k] = a[k].imag(); }
+}
+
+void ref_derivatives(int nx, int ny, double dx, double dy, const double* u, double* uxx, double* uyy,
+                     double* ux, double* uxy) {
+    second_derivatives(uxx, uyy, ux, uxy, u, u, u, nx, ny, dx, dy);
+}
+
+// conv! -> output_cropped ((2nx-1) x (2ny-1)), src/convolutions.jl:94-106
+void ref_conv_cropped(int nx, int ny, const double* kernel, const double* input, double* out) {
+    ConvPlan cp;
+    cp.init(nx, ny);
+    std::vector<cd> kh = cp.spectrum(kernel);
+    vec crop;
+    cp.conv_cropped(input, kh, crop);
+    std::memcpy(out, crop.data(), crop.size() * sizeof(double));
+}
+
+// samesize_conv! with an extended-space CornerBC (src/convolutions.jl:134-143)
+void ref_samesize_conv(int nx, int ny, const double* kernel, const double* input, double x_border, double* out) {
+    ConvPlan cp;
+    cp.init(nx, ny);
+    std::vector<cd> kh = cp.spectrum(kernel);
+    int i1, i2, j1, j2;
+    samesize_conv_indices(nx, nx / 2, i1, i2);
+    samesize_conv_indices(ny, ny / 2, j1, j2);
+    samesize_conv_ext(out, input, cp, kh, i1, i2, j1, j2, x_border);
+}
+
+int ref_nextfastfft(int n) { return nextfastfft(n); }
+
+void ref_gauss_legendre(int n, double* x, double* w) {
+    vec xx, ww;
+    gauss_legendre(n, xx, ww);
+    for (int k = 0; k < n; ++k) { x[k] = xx[k]; w[k] = ww[k]; }
+}
+
+void ref_pseudodiff(int nx, int ny, double Wx, double Wy, double* out) {
+    vec p = pseudodiff(nx, ny, Wx, Wy);
+    std::memcpy(out, p.data(), p.size() * sizeof(double));
+}
+
+void ref_elastic_green(int nx, int ny, double Wx, double Wy, const double* rm, const double* ge, int nrm,
+                       int nq, double* out) {
+    vec r(rm, rm + nrm), g(ge, ge + nrm);
+    vec G = elastic_green(nx, ny, Wx, Wy, r, g, nq);
+    std::memcpy(out, G.data(), G.size() * sizeof(double));
+}
+
+void ref_geoid_green(int nx, int ny, double Wx, double Wy, double r_pole, double mE, double* out) {
+    vec G = geoid_green(nx, ny, Wx, Wy, r_pole, mE);
+    std::memcpy(out, G.data(), G.size() * sizeof(double));
+}
+
+// SolidEarth(domain; layer_boundaries, layer_viscosities, ...) derived fields
+// (src/solidearth.jl:175-233, src/material.jl:10,46-51,154-178,221-262); nl layers of nx*ny.
+// compressible != 0 applies CompressibleMantle. Outputs D, eta_eff, R, scaled_pseudodiff_inv.
+void ref_solidearth(int nx, int ny, double Wx, double Wy, int nl, const double* lb, const double* lv,
+                    const double* maskactive, double E, double nu, double nu_m, int compressible,
+                    double* D, double* eta, double* R, double* spinv) {
+    size_t N = (size_t)nx * ny;
+    vec kappa = pseudodiff(nx, ny, Wx, Wy);
+    for (size_t p = 0; p < N; ++p) {
+        double h = lb[p];
+        D[p] = (E * h * h * h) / (12 * (1 - nu * nu));
+        eta[p] = lv[(size_t)(nl - 1) * N + p];
+        R[p] = 1.0;
+    }
+    for (int l = 1; l <= nl - 1; ++l) {
+        // channel_thickness = lb[:,:,end-l+1] - lb[:,:,end-l]; visc_ratio = lv[:,:,end-l] ./ eta
+        int a = nl - l, b = nl - l - 1;  // 0-based layer indices of end-l+1 and end-l
+        double sth = 0, svr = 0, cnt = 0;
+        for (size_t p = 0; p < N; ++p) {
+            sth += lb[(size_t)a * N + p] - lb[(size_t)b * N + p];
+            if (maskactive[p] != 0) { svr += lv[(size_t)b * N + p] / eta[p]; cnt += 1; }
+        }
+        double th = sth / (double)N, vr = svr / cnt;
+        for (size_t p = 0; p < N; ++p) {
+            // channel_scaling (src/material.jl:232-262)
+            double k = kappa[p];
+            double C = std::cosh(th * k), S = std::sinh(th * k);
+            double num = 0, den = 0;
+            num += 2 * vr * C * S;
+            num += (1 - vr * vr) * th * th * k * k;
+            num += vr * vr * S * S + C * C;
+            den += (vr + 1 / vr) * C * S;
+            den += (vr - 1 / vr) * th * k;
+            den += S * S + C * C;
+            R[p] *= num / den;
+        }
+    }
+    if (compressible) {
+        double f = (1 + 0.5) / (1 + nu_m);
+        for (size_t p = 0; p < N; ++p) eta[p] *= f;
+    }
+    for (size_t p = 0; p < N; ++p) spinv[p] = 1.0 / (R[p] * kappa[p]);
+}
+
+// analytic_solution(r, t, c, solidearth, H0, R0; n_quad_support) (src/analytic_solutions.jl:11-65);
+// t in seconds, D and eta are the (laterally constant) rigidity and effective viscosity.
+double ref_analytic_solution(double r, double t, double rho_ice, double g, double H0, double R0,
+                             double rho_um, double D, double eta, int nq) {
+    vec xq, wq;
+    gauss_legendre(nq, xq, wq);
+    std::vector<double> support;
+    support.push_back(1e-14);
+    for (int k = 0; k <= 140; ++k) support.push_back(std::pow(10.0, -10.0 + 0.05 * k));
+    support.push_back(1.0);
+    double scaling = rho_ice * g * H0 * R0;
+    double integral = 0.0;
+    for (size_t s = 0; s + 1 < support.size(); ++s) {
+        double x1 = support[s], x2 = support[s + 1];
+        double m = 2.0 / (x2 - x1), p = -1.0 - m * x1;
+        double sum = 0.0;
+        for (int i = 0; i < nq; ++i) {
+            double kappa = (xq[i] - p) / m;
+            double beta = rho_um * g + D * std::pow(kappa, 4);
+            double j0v = ::j0(kappa * r), j1v = ::j1(kappa * R0);
+            double f = std::isinf(t) ? -j0v * j1v / beta
+                                     : (std::exp(-beta * t / (2 * eta * kappa)) - 1) * j0v * j1v / beta;
+            sum = sum + f * wq[i] / m;
+        }
+        integral += sum;
+    }
+    return scaling * integral;
+}
+
+// BS3 + PI controller on the scalar test problem du/dt = lam * u, u(0) = u0, to t_end (no
+// tstops): the same stage, error and controller code path as fi_integrate, used for the
+// order and tableau checks. Returns u(t_end); nsteps receives the accepted step count.
+double ref_bs3_linear(double lam, double u0, double t_end, double reltol, double abstol, double dt_fixed,
+                      int64_t* nsteps) {
+    double t = 0, u = u0, k1 = lam * u0, qold = QOLDINIT, dt = dt_fixed > 0 ? dt_fixed : 1e-3, q11 = 1;
+    int64_t n = 0;
+    while (t < t_end) {
+        double h = std::min(dt, t_end - t);
+        double k2 = lam * (u + h * A21 * k1);
+        double k3 = lam * (u + h * A32 * k2);
+        double un = u + h * (A41 * k1 + A42 * k2 + A43 * k3);
+        double k4 = lam * un;
+        if (dt_fixed > 0) { t += h; u = un; k1 = k4; ++n; continue; }
+        double ut = h * (BT1 * k1 + BT2 * k2 + BT3 * k3 + BT4 * k4);
+        double e = std::fabs(ut / (abstol + std::max(std::fabs(u), std::fabs(un)) * reltol));
+        double q;
+        if (e == 0) q = 1 / QMAX;
+        else {
+            q11 = std::pow(e, BETA1);
+            q = std::max(1 / QMAX, std::min(1 / QMIN, q11 / std::pow(qold, BETA2) / GAMMA));
+        }
+        if (e <= 1) { qold = std::max(e, QOLDINIT); t += h; u = un; k1 = k4; dt = h / q; ++n; }
+        else dt = h / std::min(1 / QMIN, q11 / GAMMA);
+    }
+    if (nsteps) *nsteps = n;
+    return u;
+}
+
+}  // extern "C"

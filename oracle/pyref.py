@@ -1,0 +1,253 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of oracle/lib/libfastiso_ref.so (the FP64 CPU restatement). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfastiso_ref.so")
+
+FIELDS = {
+    "litho_rigidity": 1, "effective_viscosity": 2, "scaled_pseudodiff_inv": 3, "maskactive": 4,
+    "green_elastic": 5, "green_geoid": 6, "ref_u": 7, "ref_ue": 8, "ref_dz_ss": 9, "ref_z_b": 10,
+    "ref_H_ice": 11,
+    "u": 32, "dudt": 33, "ue": 34, "dz_ss": 35, "z_ss": 36, "z_b": 37, "H_ice": 38, "H_af": 39,
+    "H_water": 40, "maskgrounded": 41, "maskocean": 42, "canom_ice": 43, "canom_seawater": 44,
+    "canom_litho": 45, "canom_mantle": 46, "canom_load": 47, "canom_full": 48,
+}
+
+
+class RefStats(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("dt", C.c_double),
+        ("n_rhs", C.c_int64), ("n_accept", C.c_int64), ("n_reject", C.c_int64),
+        ("count_sparse_updates", C.c_int64), ("n_output", C.c_int64),
+        ("V_af", C.c_double), ("V_den", C.c_double), ("V_pov", C.c_double), ("delta_V", C.c_double),
+        ("z_bsl", C.c_double), ("last_EEst", C.c_double),
+    ]
+
+
+CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "fi_abi_version": (C.c_int32, []),
+            "fi_create": (C.c_int, [P, C.POINTER(P)]),
+            "fi_destroy": (None, [P]),
+            "fi_last_error": (C.c_char_p, [P]),
+            "fi_set_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
+            "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
+            "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
+            "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
+            "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),
+            "fi_init": (C.c_int, [P, C.c_double, C.c_double]),
+            "fi_rhs": (C.c_int, [P, C.c_double, _dp, _dp]),
+            "fi_sparse_update": (C.c_int, [P, C.c_double]),
+            "fi_integrator_init": (C.c_int, [P]),
+            "fi_integrate": (C.c_int, [P, C.c_double, CB, C.c_void_p]),
+            "fi_get_stats": (C.c_int, [P, C.POINTER(RefStats)]),
+            "ref_fft": (None, [C.c_int, _dp, _dp, _dp, _dp, C.c_int]),
+            "ref_derivatives": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+            "ref_conv_cropped": (None, [C.c_int, C.c_int, _dp, _dp, _dp]),
+            "ref_samesize_conv": (None, [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp]),
+            "ref_nextfastfft": (C.c_int, [C.c_int]),
+            "ref_gauss_legendre": (None, [C.c_int, _dp, _dp]),
+            "ref_pseudodiff": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp]),
+            "ref_elastic_green": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_int, _dp]),
+            "ref_geoid_green": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+            "ref_solidearth": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
+                                      C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+            "ref_analytic_solution": (C.c_double, [C.c_double] * 9 + [C.c_int]),
+            "ref_bs3_linear": (C.c_double, [C.c_double] * 6 + [C.POINTER(C.c_int64)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _flat(X):
+    return np.ascontiguousarray(np.asarray(X, dtype=np.float64).ravel(order="F"))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _shape(v, nx, ny):
+    return v.reshape((nx, ny), order="F")
+
+
+class RefEngine:
+    """One context of the CPU oracle, fed with the same inputs as the HIP engine."""
+
+    def __init__(self, cfg):
+        self.L = lib()
+        self.nx, self.ny = cfg.nx, cfg.ny
+        self.N = cfg.nx * cfg.ny
+        h = C.c_void_p()
+        st = self.L.fi_create(C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"oracle fi_create failed: {st}")
+        self.h = h
+        self._cb = None
+
+    @classmethod
+    def from_inputs(cls, inputs):
+        e = cls(inputs["cfg"])
+        for name, X in inputs["fields"].items():
+            e.set_field(name, X)
+        ice = inputs["ice"]
+        if ice[0] == "snapshots":
+            t = np.asarray(ice[1], dtype=np.float64)
+            H = np.concatenate([_flat(x) for x in ice[2]])
+            e._ck(e.L.fi_set_ice_snapshots(e.h, _p(t), len(t), _p(H), int(ice[3])))
+        elif ice[0] == "separable":
+            t = np.asarray(ice[1], dtype=np.float64)
+            a = np.asarray(ice[2], dtype=np.float64)
+            H = _flat(ice[3])
+            e._ck(e.L.fi_set_ice_separable(e.h, _p(t), _p(a), len(t), _p(H), int(ice[4])))
+        tout = np.asarray(inputs["tout"], dtype=np.float64)
+        e._ck(e.L.fi_set_output_times(e.h, _p(tout), len(tout)))
+        e._ck(e.L.fi_init(e.h, *inputs["t_span"]))
+        return e
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.fi_destroy(self.h)
+            self.h = None
+
+    def _ck(self, st):
+        if st != 0:
+            raise RuntimeError(f"oracle error {st}: {self.L.fi_last_error(self.h).decode()}")
+
+    def set_field(self, name, X):
+        v = _flat(X)
+        self._ck(self.L.fi_set_field(self.h, FIELDS[name], _p(v), self.N))
+
+    def get_field(self, name):
+        v = np.empty(self.N)
+        self._ck(self.L.fi_get_field(self.h, FIELDS[name], _p(v), self.N))
+        return _shape(v, self.nx, self.ny)
+
+    def rhs(self, t, u):
+        uu = _flat(u).copy()
+        du = np.empty(self.N)
+        self._ck(self.L.fi_rhs(self.h, float(t), _p(uu), _p(du)))
+        return _shape(du, self.nx, self.ny), _shape(uu, self.nx, self.ny)
+
+    def sparse_update(self, t):
+        self._ck(self.L.fi_sparse_update(self.h, float(t)))
+
+    def integrator_init(self):
+        self._ck(self.L.fi_integrator_init(self.h))
+
+    def integrate(self, t_target, callback=None):
+        cb = CB() if callback is None else CB(lambda user, t, k: callback(t, k))
+        self._cb = cb
+        self._ck(self.L.fi_integrate(self.h, float(t_target), cb, None))
+
+    def stats(self):
+        s = RefStats()
+        self._ck(self.L.fi_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+# ---- ref_* hooks ---------------------------------------------------------------------------
+def fft(x, sign=-1):
+    x = np.asarray(x, dtype=np.complex128)
+    n = len(x)
+    re, im = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+    ore, oim = np.empty(n), np.empty(n)
+    lib().ref_fft(n, _p(re), _p(im), _p(ore), _p(oim), sign)
+    return ore + 1j * oim
+
+
+def derivatives(u, dx, dy):
+    nx, ny = u.shape
+    uf = _flat(u)
+    outs = [np.empty(nx * ny) for _ in range(4)]
+    lib().ref_derivatives(nx, ny, dx, dy, _p(uf), *[_p(o) for o in outs])
+    return [_shape(o, nx, ny) for o in outs]
+
+
+def conv_cropped(kernel, inp):
+    nx, ny = kernel.shape
+    out = np.empty((2 * nx - 1) * (2 * ny - 1))
+    lib().ref_conv_cropped(nx, ny, _p(_flat(kernel)), _p(_flat(inp)), _p(out))
+    return _shape(out, 2 * nx - 1, 2 * ny - 1)
+
+
+def samesize_conv(kernel, inp, x_border=0.0):
+    nx, ny = kernel.shape
+    out = np.empty(nx * ny)
+    lib().ref_samesize_conv(nx, ny, _p(_flat(kernel)), _p(_flat(inp)), x_border, _p(out))
+    return _shape(out, nx, ny)
+
+
+def gauss_legendre(n):
+    x, w = np.empty(n), np.empty(n)
+    lib().ref_gauss_legendre(n, _p(x), _p(w))
+    return x, w
+
+
+def pseudodiff(nx, ny, Wx, Wy):
+    out = np.empty(nx * ny)
+    lib().ref_pseudodiff(nx, ny, Wx, Wy, _p(out))
+    return _shape(out, nx, ny)
+
+
+def elastic_green(nx, ny, Wx, Wy, rm, ge, nq=4):
+    out = np.empty(nx * ny)
+    rm = np.ascontiguousarray(rm, dtype=np.float64)
+    ge = np.ascontiguousarray(ge, dtype=np.float64)
+    lib().ref_elastic_green(nx, ny, Wx, Wy, _p(rm), _p(ge), len(rm), nq, _p(out))
+    return _shape(out, nx, ny)
+
+
+def geoid_green(nx, ny, Wx, Wy, r_pole=6357e3, mE=5.972e24):
+    out = np.empty(nx * ny)
+    lib().ref_geoid_green(nx, ny, Wx, Wy, r_pole, mE, _p(out))
+    return _shape(out, nx, ny)
+
+
+def solidearth(nx, ny, Wx, Wy, lb, lv, maskactive=None, E=6.6e10, nu=0.28, nu_m=0.28, compressible=True):
+    nl = len(lb)
+    N = nx * ny
+    lbf = np.concatenate([_flat(np.broadcast_to(np.asarray(x, float), (nx, ny))) for x in lb])
+    lvf = np.concatenate([_flat(np.broadcast_to(np.asarray(x, float), (nx, ny))) for x in lv])
+    m = _flat(np.ones((nx, ny)) if maskactive is None else maskactive)
+    outs = [np.empty(N) for _ in range(4)]
+    lib().ref_solidearth(nx, ny, Wx, Wy, nl, _p(lbf), _p(lvf), _p(m), E, nu, nu_m, int(compressible),
+                         *[_p(o) for o in outs])
+    return [_shape(o, nx, ny) for o in outs]
+
+
+def analytic_solution(r, t_sec, rho_ice, g, H0, R0, rho_um, D, eta, nq=5):
+    return lib().ref_analytic_solution(r, t_sec, rho_ice, g, H0, R0, rho_um, D, eta, nq)
+
+
+def bs3_linear(lam, u0, t_end, reltol=1e-5, abstol=1e-6, dt_fixed=0.0):
+    n = C.c_int64()
+    u = lib().ref_bs3_linear(lam, u0, t_end, reltol, abstol, dt_fixed, C.byref(n))
+    return u, n.value
