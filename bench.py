@@ -1,0 +1,124 @@
+"""Benchmark: simulated kyr/s of the N=350 laterally variable ELVA configuration on one MI355X.
+
+Workload: lv_elva n=350 (dx = 18 km over a 6300 km square, log10 viscosity in [19, 23] with a
+500 km correlation length, lithosphere 50-250 km, 3-6 seeded ice domes under a sawtooth glacial
+cycle, elastic + viscous + sea-surface responses on), adaptive BS3 with reltol 1e-5, abstol 1e-6,
+sparse elastic / sea-surface update every 10 yr. Inputs are generated, then resident on the device.
+
+A step is one 10-yr window of simulated time (all adaptive BS3 steps inside it plus its one sparse
+update). value = simulated kyr / wall-clock s over --steps windows after --warmup windows.
+
+roofline: the dominant kernel is the persistent integration kernel. Its algorithmic traffic is
+171 B per cell per accepted BS3 step (FP64 state, FP32 derivatives and fields, counted once per
+unit); achieved = that traffic / the kernel time measured with HIP events on the engine's stream.
+cpu_baseline: the FP64 CPU restatement of the reference path (oracle/, OpenMP) on this host, on
+the same workload, over a bounded window.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "fastisostasy.jl_amd")]
+
+BYTES_PER_CELL_STEP = 171.0
+HBM_PEAK_GBS = 8000.0
+
+
+def make_sim(n, engine):
+    import fastiso
+    from fastiso import workloads
+    w = workloads.lv_elva(n=n, t_end=120e3, t_out=[])
+    return w, fastiso.Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
+                                 nout=w["nout"], opts=w["opts"], c=w["c"], engine=engine)
+
+
+def cpu_baseline(n, window, sample_s):
+    threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = threads
+    from oracle import pyref
+    w, sim = make_sim(n, engine=None)
+    e = pyref.RefEngine.from_inputs(sim.inputs)
+    e.integrator_init()
+    e.integrate(window)                 # same warm-up as the GPU leg: the first window
+    t_sim0 = e.stats()["t"]
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < sample_s:
+        e.integrate(e.stats()["t"] + window / 10)
+    el = time.perf_counter() - t0
+    st = e.stats()
+    sim_yr = st["t"] - t_sim0
+    return {"value": sim_yr / 1e3 / el, "unit": "kyr/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle FP64, n={n}, t={t_sim0:.0f}->{st['t']:.2f} yr after a {window:.0f}-yr warm-up, "
+                      f"{el:.1f} s wall, {st['n_accept']} accepted steps total"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=350)
+    ap.add_argument("--window", type=float, default=10.0)
+    ap.add_argument("--cpu-sample-s", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch  # device memory is owned by the engine; torch only confirms the device
+    assert torch.cuda.is_available(), "no GPU"
+    w, sim = make_sim(a.n, engine="hip")
+    eng = sim.engine
+    eng.integrator_init()
+    t = 0.0
+    for _ in range(a.warmup):
+        t += a.window
+        eng.integrate(t)
+    torch.cuda.synchronize()
+    s0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        t += a.window
+        eng.integrate(t)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    s1 = eng.stats()
+    kyr = a.steps * a.window / 1e3
+    N = a.n * a.n
+    acc = s1["n_accept_timed"] - s0["n_accept_timed"]
+    dev_s = (s1["device_ms"] - s0["device_ms"]) / 1e3
+    launches = s1["n_launch"] - s0["n_launch"]
+    achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
+    out = {
+        "metric": "simulated kyr/s at N=350 LV-ELVA (plain solver)",
+        "value": kyr / el,
+        "unit": "kyr/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fft/stencil-moments, f64 state/stencil/error-norm",
+        "data": "synthetic (seeded fields)",
+        "config": {"workload": f"lv_elva_n{a.n}", "n": a.n, "dx_km": 6300.0 / a.n, "window_yr": a.window,
+                   "integrator": "BS3 reltol 1e-5 abstol 1e-6", "accepted_steps": int(s1["n_accept"] - s0["n_accept"]),
+                   "rejected_steps": int(s1["n_reject"] - s0["n_reject"]),
+                   "rhs_calls": int(s1["n_rhs"] - s0["n_rhs"]), "sim_t_end_yr": s1["t"],
+                   "kernel_launches": int(launches), "kernel_ms_per_launch": dev_s * 1e3 / max(launches, 1),
+                   "us_per_accepted_step": dev_s * 1e6 / max(acc, 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

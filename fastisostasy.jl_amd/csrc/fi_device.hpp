@@ -1,0 +1,120 @@
+// Device-side data structures and phases of the LV-ELVA engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fi_fft.hpp"
+
+namespace fi {
+
+constexpr int NT = 256;  // threads per workgroup (four wave64)
+
+enum { MODE_RHS = 0, MODE_INTEGRATE = 1, MODE_SPARSE_ONLY = 2 };
+enum { ICE_EXTERNAL = 0, ICE_SNAPSHOTS = 1, ICE_SEPARABLE = 2 };
+enum { ERR_TIME_RANGE = 3, ERR_NONFINITE = 4, ERR_ABORT = 5, ERR_DT_MIN = 7 };
+
+// BS3 tableau (Bogacki & Shampine 1989) and PI-controller constants
+#define BS3_A21 0.5
+#define BS3_A32 0.75
+#define BS3_A41 (2.0 / 9.0)
+#define BS3_A42 (1.0 / 3.0)
+#define BS3_A43 (4.0 / 9.0)
+#define BS3_BT1 (2.0 / 9.0 - 7.0 / 24.0)
+#define BS3_BT2 (1.0 / 3.0 - 0.25)
+#define BS3_BT3 (4.0 / 9.0 - 1.0 / 3.0)
+#define BS3_BT4 (-0.125)
+#define PI_BETA1 (7.0 / 30.0)
+#define PI_BETA2 (2.0 / 15.0)
+#define PI_GAMMA 0.9
+#define PI_QMIN 0.2
+#define PI_QMAX 10.0
+#define PI_QOLDINIT 1e-4
+
+// Control block: integrator state shared by host and device. Every workgroup keeps an
+// identical copy in LDS and updates it with identical arithmetic; workgroup 0 writes it back.
+struct Ctl {
+    double t, dt, qold, q11, EEst, t0;
+    double stop;        // clip steps to land here (next output time or the target)
+    double next_out;    // leave the loop after an accepted step with t >= next_out
+    double t_rhs;       // RHS mode: evaluation time
+    double t_last;      // time of the last RHS call
+    double V_af, V_den, V_pov, delta_V, z_bsl;
+    long long count, nrhs, nacc, nrej, steps;
+    int cur;       // U buffer holding u_n
+    int k1;        // K slot of the FSAL derivative
+    int last_k;    // K slot written by the last RHS call
+    int last_u;    // U buffer holding the last RHS call's (corner-corrected) displacement
+    int err;       // device error (ERR_*)
+    int last_trig; // the last RHS call ran the sparse update
+    int pad0, pad1;
+};
+
+struct Bar {
+    unsigned count;
+    unsigned gen;
+    int abort;
+    int pad;
+};
+
+struct KP {
+    int nx, ny, nhx, nfx, nfy, nfhx, nrowitems, ncolitems, colnb;
+    int i1, i2, j1, j2;  // 1-based crop indices (src/convolutions.jl:196-204)
+    double dx, dy, A, inv_dx, inv_dy, inv_dx2, inv_dy2;
+    double g, spy, rho_ice, rho_sw_ice, rho_ice_sw, rho_seawater, rho_water, rho_um, rho_litho, nu;
+    double dt_sparse, reltol, abstol, vbc_x, ebc_x, sbc_x, bsl_z;
+    double N;  // nx*ny as double
+    int rigid_litho, rigid_mantle, compressible_column, lv_surface, interactive, vol_goelzer, den_goelzer,
+        pov_goelzer;
+    int ice_mode, ice_nt, ice_flat, mt_double;
+    const double* ice_t;
+    const double* ice_a;
+    const float* ice_H;
+    // state and fields (float unless noted); U buffer b at Ub + b*Nl, K slot k at Kb + k*Nl
+    long Nl;
+    double* Ub;
+    float* Kb;
+    float* kcorner;  // [4 slots][4 corners], raw
+    float2* spec;    // [nhx][ny]
+    const float* mult;
+    const float* D;
+    const float* inv2eta;
+    const unsigned char* mask;
+    const float* H_af_ref;
+    const float* u_ref;
+    const float* z_b_ref;
+    const float* z_ss_ref;
+    const float* ue_ref;
+    float* H_ext;
+    float* ue;
+    float* dz_ss;
+    float* z_ss;
+    float* zoff;
+    float* zoff_trig;
+    float* LW;
+    float* LT;
+    float* H_water;
+    unsigned char* mg;
+    unsigned char* mo;
+    float* sig_load_prev;
+    float* sig_geo_prev;
+    float2* pA;
+    float2* pB;
+    float2* oA;
+    float2* oB;
+    float* ue_raw;
+    float* dz_raw;
+    float* ext_corners;  // [2][4]
+    const double2* ge_hat;
+    const double2* gg_hat;
+    double* part;    // [G][4] error moments
+    double* vpart;   // [G][4] volume partial sums
+    FftPlan px, py, pfx, pfy;
+    const float2* tw32;
+    Ctl* ctl;
+    Bar* bar;
+    int mode, rhs_ubuf, rhs_kslot, max_steps;
+    int write_sigma_always;
+    unsigned long long spin_limit;  // wall-clock ticks before a barrier wait gives up
+};
+
+}  // namespace fi

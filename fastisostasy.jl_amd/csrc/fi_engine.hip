@@ -1,0 +1,1176 @@
+// libfastiso_hip.so -- the MI355X (gfx950) LV-ELVA engine behind include/fastiso.h.
+//
+// Device-resident state; one cooperative persistent kernel runs either a single RHS
+// evaluation (fi_rhs) or the adaptive BS3 loop (fi_integrate) until the next output time,
+// with every step-control decision taken on the device (no host round trip per step).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fastiso.h"
+#include "fi_kernels.hpp"
+
+namespace fi {
+
+// ------------------------------------------------------------------------------------------
+// persistent kernel
+// ------------------------------------------------------------------------------------------
+template <typename MT>
+__global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
+    const KP& p = *pp;
+    extern __shared__ __align__(16) char smem[];
+    __shared__ Ctl s;
+    __shared__ double red[3 * NT];
+    __shared__ int flag;
+    unsigned gen = 0;
+    if (threadIdx.x == 0) s = *p.ctl;
+    __syncthreads();
+
+    if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY) {
+        StageDesc st{};
+        st.ubase = p.rhs_ubuf;
+        st.nk = 0;
+        st.dt = 0;
+        st.write_u = -1;
+        st.kout = p.rhs_kslot;
+        st.t = s.t_rhs;
+        st.t_next_max = s.t_rhs;
+        rhs_call<MT>(p, s, st, gen, smem, red, &flag, p.mode == MODE_SPARSE_ONLY);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *p.ctl = s;
+        return;
+    }
+
+    // MODE_INTEGRATE: adaptive BS3 from (s.t, U[s.cur], K[s.k1]) until t reaches s.next_out
+    // (after an accepted step) or s.stop, or max_steps attempts
+    long long attempts = 0;
+    for (;;) {
+        if (s.err || s.t >= s.stop || attempts >= p.max_steps) break;
+        const double tn = s.t;
+        double dt = s.dt;
+        bool clipped = false;
+        if (tn + dt >= s.stop) {
+            dt = s.stop - tn;
+            clipped = true;
+        }
+        if (!(dt > 1e-14 * fmax(1.0, fabs(tn)))) {
+            if (threadIdx.x == 0) s.err = ERR_DT_MIN;
+            __syncthreads();
+            break;
+        }
+        const int k1 = s.k1;
+        const int s2 = (k1 + 1) & 3, s3 = (k1 + 2) & 3, s4 = (k1 + 3) & 3;
+        const int cur = s.cur, unew = 1 - s.cur;
+        bool ok = true;
+        for (int stg = 0; stg < 3; ++stg) {
+            StageDesc st{};
+            st.ubase = cur;
+            st.dt = dt;
+            st.write_u = -1;
+            if (stg == 0) {          // tmp = uprev + dt a21 k1
+                st.nk = 1;
+                st.slot0 = k1;
+                st.a0 = BS3_A21;
+                st.kout = s2;
+                st.t = tn + 0.5 * dt;
+                st.t_next_max = tn + 0.75 * dt;
+            } else if (stg == 1) {   // tmp = uprev + dt a32 k2
+                st.nk = 1;
+                st.slot0 = s2;
+                st.a0 = BS3_A32;
+                st.kout = s3;
+                st.t = tn + 0.75 * dt;
+                st.t_next_max = tn + dt;
+            } else {                 // u = uprev + dt (a41 k1 + a42 k2 + a43 k3); error moments
+                st.nk = 3;
+                st.slot0 = k1;
+                st.slot1 = s2;
+                st.slot2 = s3;
+                st.a0 = BS3_A41;
+                st.a1 = BS3_A42;
+                st.a2 = BS3_A43;
+                st.write_u = unew;
+                st.kout = s4;
+                st.t = tn + dt;
+                st.t_next_max = tn + dt + 0.5 * PI_QMAX * dt;
+                st.final_err = 1;
+                st.e1 = k1;
+                st.e2 = s2;
+                st.e3 = s3;
+                st.u_prev = cur;
+            }
+            if (!rhs_call<MT>(p, s, st, gen, smem, red, &flag, false)) {
+                ok = false;
+                break;
+            }
+        }
+        if (!ok) break;
+        ++attempts;
+        // error norm with the corner-BC offsets of k1..k4 folded in:
+        // sum ((a - C) w)^2 = S2 - 2 C S1 + C^2 S0
+        if (threadIdx.x == 0) {
+            double S2 = 0, S1 = 0, S0 = 0;
+            for (int w = 0; w < (int)gridDim.x; ++w) {
+                S2 += p.part[4 * w];
+                S1 += p.part[4 * w + 1];
+                S0 += p.part[4 * w + 2];
+            }
+            const double C = dt * (BS3_BT1 * k_offset(p, k1) + BS3_BT2 * k_offset(p, s2) +
+                                   BS3_BT3 * k_offset(p, s3) + BS3_BT4 * k_offset(p, s4));
+            const double e2 = (S2 - 2.0 * C * S1 + C * C * S0) / p.N;
+            const double EEst = sqrt(fmax(e2, 0.0));
+            s.EEst = EEst;
+            double qv;
+            bool finite = isfinite(EEst);
+            if (!finite) {
+                s.q11 = PI_GAMMA / PI_QMIN;
+                qv = 1.0 / PI_QMIN;
+            } else if (EEst == 0.0) {
+                qv = 1.0 / PI_QMAX;
+            } else {
+                const double q11 = pow(EEst, PI_BETA1);
+                s.q11 = q11;
+                qv = q11 / pow(s.qold, PI_BETA2);
+                qv = fmax(1.0 / PI_QMAX, fmin(1.0 / PI_QMIN, qv / PI_GAMMA));
+            }
+            if (finite && EEst <= 1.0) {
+                s.qold = fmax(EEst, PI_QOLDINIT);
+                s.t = clipped ? s.stop : tn + dt;
+                s.cur = unew;
+                s.k1 = s4;
+                s.last_u = unew;
+                s.dt = dt / qv;
+                s.nacc += 1;
+                s.steps = 1;  // accepted flag for this attempt
+            } else {
+                s.dt = dt / fmin(1.0 / PI_QMIN, s.q11 / PI_GAMMA);
+                s.nrej += 1;
+                s.steps = 0;
+            }
+        }
+        __syncthreads();
+        if (s.steps == 1 && s.t >= s.next_out) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.ctl = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// set-up: FP64 half spectra of the zero-padded Green kernels (ConvolutionPlan,
+// src/convolutions.jl:41-56,89-92), stored [kx][ky] and pre-scaled by 1/(nfx nfy) * scale
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_green_rows(const double* __restrict__ G, int nx, int ny, int nfx, int nfhx,
+                                                    FftPlan pl, const double2* __restrict__ tw, double2* out) {
+    extern __shared__ __align__(16) char smem[];
+    double2* a = (double2*)smem;
+    double2* b = a + nfx;
+    const int j = blockIdx.x;
+    for (int i = threadIdx.x; i < nfx; i += NT) {
+        double2 z;
+        z.x = (i < nx) ? G[i + (long)j * nx] : 0.0;
+        z.y = 0.0;
+        a[i] = z;
+    }
+    __syncthreads();
+    double2* r = fft_lds<double2>(a, b, 1, nfx, pl, tw, -1);
+    for (int kx = threadIdx.x; kx < nfhx; kx += NT) out[(long)kx * ny + j] = r[kx];
+}
+
+__global__ void __launch_bounds__(NT) k_green_cols(const double2* __restrict__ in, int ny, int nfy, FftPlan pl,
+                                                    const double2* __restrict__ tw, double scale, double2* out) {
+    extern __shared__ __align__(16) char smem[];
+    double2* a = (double2*)smem;
+    double2* b = a + nfy;
+    const int kx = blockIdx.x;
+    for (int j = threadIdx.x; j < nfy; j += NT) {
+        double2 z;
+        z.x = 0;
+        z.y = 0;
+        if (j < ny) z = in[(long)kx * ny + j];
+        a[j] = z;
+    }
+    __syncthreads();
+    double2* r = fft_lds<double2>(a, b, 1, nfy, pl, tw, -1);
+    for (int ky = threadIdx.x; ky < nfy; ky += NT) {
+        double2 v = r[ky];
+        v.x *= scale;
+        v.y *= scale;
+        out[(long)kx * nfy + ky] = v;
+    }
+}
+
+// dudt = k - offset (FP64), for device-array callers
+__global__ void k_dudt_out(const float* __restrict__ k, double off, long n, double* out) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+        out[c] = (double)k[c] - off;
+}
+__global__ void k_sub_offset(double* u, double off, long n) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) u[c] -= off;
+}
+
+}  // namespace fi
+
+// ==========================================================================================
+// host side
+// ==========================================================================================
+using namespace fi;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t cnt) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = cnt;
+        return hipMalloc((void**)&p, std::max<size_t>(cnt, 1) * sizeof(T));
+    }
+};
+
+bool plan_fft(int L, int twoff, FftPlan& pl) {
+    std::memset(&pl, 0, sizeof(pl));
+    pl.L = L;
+    pl.tw = twoff;
+    int m = L;
+    auto push = [&](int r) {
+        if (pl.nr >= 16) return false;
+        pl.code |= (unsigned long long)r << (4 * pl.nr);
+        pl.nr++;
+        m /= r;
+        return true;
+    };
+    while (m % 8 == 0) if (!push(8)) return false;
+    while (m % 4 == 0) if (!push(4)) return false;
+    while (m % 2 == 0) if (!push(2)) return false;
+    for (int r : {3, 5, 7}) while (m % r == 0) if (!push(r)) return false;
+    return m == 1;
+}
+
+int nextfastfft(int n) {
+    for (int m = std::max(n, 1);; ++m) {
+        int r = m;
+        for (int q : {2, 3, 5, 7}) while (r % q == 0) r /= q;
+        if (r == 1) return m;
+    }
+}
+
+}  // namespace
+
+struct fi_ctx {
+    fi_config cfg;
+    std::string err;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0, nhx = 0, nfx = 0, nfy = 0, nfhx = 0;
+    size_t N = 0;
+    double dx = 0, dy = 0, A = 0;
+    int i1 = 0, i2 = 0, j1 = 0, j2 = 0;
+    // host copies of the static inputs
+    std::vector<double> D, eta, spinv, mask, ge, gg, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ref_in;
+    bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_href = false;
+    int ice_mode = ICE_EXTERNAL, ice_flat = 0;
+    std::vector<double> ice_t, ice_a, ice_H;
+    std::vector<double> tout;
+    std::vector<double> H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
+    std::vector<double> H_ext_host;
+    // device memory
+    DBuf<double> U, ice_t_d, ice_a_d, part, vpart;
+    DBuf<float> K, kcorner, mult, Df, inv2eta, H_af_ref_d, u_ref_d, z_b_ref_d, z_ss_ref_d, ue_ref_d, H_ext, ue, dz_ss,
+        z_ss, zoff, zoff_trig, LW, LT, H_water, sig_load_prev, sig_geo_prev, ue_raw, dz_raw, ext_corners, ice_H_d;
+    DBuf<unsigned char> maskd, mg, mo;
+    DBuf<float2> spec, pA, pB, oA, oB, tw32;
+    DBuf<double2> ge_hat, gg_hat, tw64, gtmp;
+    DBuf<Ctl> ctl_d;
+    DBuf<Bar> bar_d;
+    DBuf<KP> kp_d;
+    KP kp;
+    Ctl ctl;
+    int G = 0;
+    size_t lds = 0;
+    bool mt_double = true;
+    FftPlan px64, py64, pfx64, pfy64;
+    bool initialized = false, integ = false, k1_stale = false;
+    long long kout = 0;
+    double t_end = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double device_ms = 0;
+    long long n_launch = 0, n_accept_timed = 0;
+};
+
+namespace {
+
+int fail(fi_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIPCHECK(c, x)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail((c), FI_ERR_DEVICE, std::string(#x " : ") + hipGetErrorString(e_)); \
+    } while (0)
+
+std::vector<float> to_f(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+template <typename T>
+int upload(fi_ctx* c, DBuf<T>& b, const std::vector<T>& h) {
+    HIPCHECK(c, b.alloc(h.size()));
+    HIPCHECK(c, hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FI_OK;
+}
+
+// TimeInterpolation2D on the host (src/interpolations.jl:51-74), for H_ice_ref and diagnostics
+int host_ice(fi_ctx* c, double t, std::vector<double>& out) {
+    out.assign(c->N, 0.0);
+    if (c->ice_mode == ICE_EXTERNAL) {
+        if (!c->H_ext_host.empty()) out = c->H_ext_host;
+        return FI_OK;
+    }
+    const std::vector<double>& tv = c->ice_t;
+    int nt = (int)tv.size();
+    auto snap = [&](int k, size_t p) { return c->ice_mode == ICE_SNAPSHOTS ? c->ice_H[(size_t)k * c->N + p] : c->ice_a[k] * c->ice_H[p]; };
+    if (t < tv.front() || t > tv.back()) {
+        if (!c->ice_flat) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+        int k = t < tv.front() ? 0 : nt - 1;
+        for (size_t p = 0; p < c->N; ++p) out[p] = snap(k, p);
+        return FI_OK;
+    }
+    int i = (int)(std::lower_bound(tv.begin(), tv.end(), t) - tv.begin());
+    if (tv[i] == t) {
+        for (size_t p = 0; p < c->N; ++p) out[p] = snap(i, p);
+    } else {
+        --i;
+        for (size_t p = 0; p < c->N; ++p) {
+            double a = snap(i, p), b = snap(i + 1, p);
+            out[p] = a + (b - a) / (tv[i + 1] - tv[i]) * (t - tv[i]);
+        }
+    }
+    return FI_OK;
+}
+
+double hab_h(double H, double zb, double zss, double rsi) { return std::max(H + std::min(zb - zss, 0.0) * rsi, 0.0); }
+
+int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_sigma_always) {
+    KP kp = c->kp;
+    kp.mode = mode;
+    kp.rhs_ubuf = ubuf;
+    kp.rhs_kslot = kslot;
+    kp.max_steps = max_steps;
+    kp.write_sigma_always = write_sigma_always;
+    HIPCHECK(c, hipMemsetAsync(c->bar_d.p, 0, sizeof(Bar), c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->kp_d.p, &kp, sizeof(KP), hipMemcpyHostToDevice, c->stream));
+    const KP* kptr = c->kp_d.p;
+    void* args[] = {&kptr};
+    const void* f = c->mt_double ? (const void*)fi_persistent<double> : (const void*)fi_persistent<float>;
+    const bool timed = mode == MODE_INTEGRATE;
+    const long long nacc0 = c->ctl.nacc;
+    if (timed) HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHECK(c, hipLaunchCooperativeKernel(f, dim3(c->G), dim3(NT), args, (unsigned)c->lds, c->stream));
+    if (timed) HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(&c->ctl, c->ctl_d.p, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    Bar bar;
+    HIPCHECK(c, hipMemcpyAsync(&bar, c->bar_d.p, sizeof(Bar), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (timed) {
+        float ms = 0;
+        HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->device_ms += ms;
+        c->n_launch += 1;
+        c->n_accept_timed += c->ctl.nacc - nacc0;
+    }
+    if (bar.abort) return fail(c, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
+    if (c->ctl.err == ERR_TIME_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+    if (c->ctl.err == ERR_DT_MIN) return fail(c, FI_ERR_DT_MIN, "dt below minimum");
+    if (c->ctl.err) return fail(c, FI_ERR_DEVICE, "device error");
+    return FI_OK;
+}
+
+// corner BC offset of a host array (src/boundary_conditions.jl:144-148)
+double corner_offset(const double* X, int nx, int ny, double xb) {
+    double a = 0.25 * X[0], b = 0.25 * X[nx - 1], cc = 0.25 * X[(size_t)(ny - 1) * nx],
+           d = 0.25 * X[(size_t)(ny - 1) * nx + nx - 1];
+    return ((a + b) + (cc + d)) - xb;
+}
+
+double kslot_offset(fi_ctx* c, int slot, float kc[4]) {
+    double a = 0.25 * (double)kc[0], b = 0.25 * (double)kc[1], cc = 0.25 * (double)kc[2], d = 0.25 * (double)kc[3];
+    (void)slot;
+    return ((a + b) + (cc + d)) - c->cfg.viscous_x_border;
+}
+
+int read_k(fi_ctx* c, int slot, std::vector<double>& out) {
+    std::vector<float> k(c->N);
+    float kc[4];
+    HIPCHECK(c, hipMemcpy(k.data(), c->K.p + (size_t)slot * c->N, c->N * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(kc, c->kcorner.p + 4 * slot, 4 * sizeof(float), hipMemcpyDeviceToHost));
+    double off = kslot_offset(c, slot, kc);
+    out.resize(c->N);
+    for (size_t p = 0; p < c->N; ++p) out[p] = (double)k[p] - off;
+    return FI_OK;
+}
+
+int read_f(fi_ctx* c, const float* d, std::vector<double>& out) {
+    std::vector<float> h(c->N);
+    HIPCHECK(c, hipMemcpy(h.data(), d, c->N * sizeof(float), hipMemcpyDeviceToHost));
+    out.assign(h.begin(), h.end());
+    return FI_OK;
+}
+
+int read_u(fi_ctx* c, int buf, std::vector<double>& out) {
+    out.resize(c->N);
+    HIPCHECK(c, hipMemcpy(out.data(), c->U.p + (size_t)buf * c->N, c->N * sizeof(double), hipMemcpyDeviceToHost));
+    return FI_OK;
+}
+
+// one RHS evaluation on U[ubuf] at time t into K[slot] (the input is not modified; the
+// corner-corrected displacement is u - offset, computed identically on the host)
+int rhs_on_device(fi_ctx* c, double t, int ubuf, int slot) {
+    c->ctl.t_rhs = t;
+    int st = launch(c, MODE_RHS, ubuf, slot, 0, 1);
+    if (st) return st;
+    return FI_OK;
+}
+
+}  // namespace
+
+// ==========================================================================================
+// C ABI
+// ==========================================================================================
+extern "C" {
+
+int32_t fi_abi_version(void) { return FI_ABI_VERSION; }
+const char* fi_engine_name(void) { return "hip-gfx950"; }
+
+void fi_default_config(fi_config* c, int32_t nx, int32_t ny, double Wx, double Wy) {
+    std::memset(c, 0, sizeof(*c));
+    c->nx = nx;
+    c->ny = ny;
+    c->Wx = Wx;
+    c->Wy = Wy;
+    c->g = 9.8;
+    c->seconds_per_year = 60.0 * 60.0 * 24.0 * 365.25;
+    c->rho_ice = 0.910e3;
+    c->rho_water = 1e3;
+    c->rho_seawater = 1.023e3;
+    c->r_pole = 6357e3;
+    c->mE = 5.972e24;
+    c->litho_poissonratio = 0.28;
+    c->rho_uppermantle = 3.4e3;
+    c->rho_litho = 3.2e3;
+    c->lithosphere = FI_LATERALLY_VARIABLE_LITHOSPHERE;
+    c->mantle = FI_MAXWELL_MANTLE;
+    c->lithosphere_column = FI_INCOMPRESSIBLE_LITHOSPHERE_COLUMN;
+    c->sea_surface = FI_LATERALLY_CONSTANT_SEA_SURFACE;
+    c->sealevel_load = FI_NO_SEALEVEL_LOAD;
+    c->volume_contribution = FI_GOELZER_CONTRIBUTION;
+    c->density_contribution = FI_GOELZER_CONTRIBUTION;
+    c->adjustment_contribution = FI_NO_CONTRIBUTION;
+    c->dt_sparse = 10.0;
+    c->reltol = 1e-5;
+    c->abstol = 1e-6;
+}
+
+int fi_create(const fi_config* cfg, fi_ctx** out) {
+    if (!cfg || !out) return FI_ERR_INVALID;
+    *out = nullptr;
+    if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->Wx > 0) || !(cfg->Wy > 0)) return FI_ERR_INVALID;
+    if (cfg->mantle == FI_RELAXED_MANTLE) return FI_ERR_UNSUPPORTED;
+    if (cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE)
+        return FI_ERR_UNSUPPORTED;
+    if (cfg->mantle == FI_RIGID_MANTLE) return FI_ERR_UNSUPPORTED;  // dudt = 0: nothing to accelerate
+    fi_ctx* c = new fi_ctx();
+    c->cfg = *cfg;
+    c->nx = cfg->nx;
+    c->ny = cfg->ny;
+    c->N = (size_t)c->nx * c->ny;
+    c->nhx = c->nx / 2 + 1;
+    c->nfx = nextfastfft(2 * c->nx - 1);
+    c->nfy = nextfastfft(2 * c->ny - 1);
+    c->nfhx = c->nfx / 2 + 1;
+    c->dx = 2 * cfg->Wx / c->nx;
+    c->dy = 2 * cfg->Wy / c->ny;
+    c->A = c->dx * c->dy;
+    int mx = c->nx / 2, my = c->ny / 2;
+    c->i1 = (c->nx % 2 == 0) ? mx : mx + 1;
+    c->i2 = 2 * c->nx - 1 - mx;
+    c->j1 = (c->ny % 2 == 0) ? my : my + 1;
+    c->j2 = 2 * c->ny - 1 - my;
+    FftPlan tmp;
+    if (!plan_fft(c->nx, 0, tmp) || !plan_fft(c->ny, 0, tmp)) {
+        delete c;
+        return FI_ERR_UNSUPPORTED;  // grid sides must factor into 2, 3, 5, 7
+    }
+    c->dev = cfg->device;
+    if (hipSetDevice(c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return FI_ERR_DEVICE;
+    }
+    size_t N = c->N;
+    c->mask.assign(N, 1.0);
+    c->u_ref.assign(N, 0.0);
+    c->ue_ref.assign(N, 0.0);
+    c->dz_ss_ref.assign(N, 0.0);
+    c->z_b_ref.assign(N, 1e6);
+    *out = c;
+    return FI_OK;
+}
+
+void fi_destroy(fi_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    hipStream_t s = c->stream;
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* fi_last_error(const fi_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
+    if (!c || !h || n != c->N) return fail(c, FI_ERR_INVALID, "fi_set_field: size mismatch");
+    std::vector<double> v(h, h + n);
+    switch (f) {
+        case FI_LITHO_RIGIDITY: c->D = v; c->has_D = true; return FI_OK;
+        case FI_EFFECTIVE_VISCOSITY: c->eta = v; c->has_eta = true; return FI_OK;
+        case FI_SCALED_PSEUDODIFF_INV: c->spinv = v; c->has_spinv = true; return FI_OK;
+        case FI_MASK_ACTIVE: c->mask = v; return FI_OK;
+        case FI_GREEN_ELASTIC: c->ge = v; c->has_ge = true; return FI_OK;
+        case FI_GREEN_GEOID: c->gg = v; c->has_gg = true; return FI_OK;
+        case FI_REF_U: c->u_ref = v; return FI_OK;
+        case FI_REF_UE: c->ue_ref = v; return FI_OK;
+        case FI_REF_DZ_SS: c->dz_ss_ref = v; return FI_OK;
+        case FI_REF_Z_B: c->z_b_ref = v; return FI_OK;
+        case FI_REF_H_ICE: c->H_ref_in = v; c->has_href = true; return FI_OK;
+        default: break;
+    }
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "state field written before fi_init");
+    (void)hipSetDevice(c->dev);
+    if (f == FI_U) {
+        int buf = c->integ ? c->ctl.cur : 0;
+        HIPCHECK(c, hipMemcpy(c->U.p + (size_t)buf * c->N, h, n * sizeof(double), hipMemcpyHostToDevice));
+        c->ctl.last_u = buf;
+        if (c->integ) c->k1_stale = true;
+        return FI_OK;
+    }
+    if (f == FI_H_ICE) {
+        if (c->ice_mode != ICE_EXTERNAL) return fail(c, FI_ERR_INVALID, "H_ice is interpolated in time; not writable");
+        c->H_ext_host = v;
+        std::vector<float> fv = to_f(v);
+        HIPCHECK(c, hipMemcpy(c->H_ext.p, fv.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        return FI_OK;
+    }
+    return fail(c, FI_ERR_INVALID, "fi_set_field: field not writable");
+}
+
+int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H, int32_t flat) {
+    if (!c || !t || !H || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_ice_snapshots: bad args");
+    for (int k = 1; k < nt; ++k)
+        if (!(t[k] > t[k - 1])) return fail(c, FI_ERR_INVALID, "ice times must increase");
+    c->ice_mode = ICE_SNAPSHOTS;
+    c->ice_t.assign(t, t + nt);
+    c->ice_H.assign(H, H + (size_t)nt * c->N);
+    c->ice_a.assign(nt, 1.0);
+    c->ice_flat = flat;
+    return FI_OK;
+}
+
+int fi_set_ice_separable(fi_ctx* c, const double* t, const double* a, int32_t nt, const double* H, int32_t flat) {
+    if (!c || !t || !a || !H || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_ice_separable: bad args");
+    for (int k = 1; k < nt; ++k)
+        if (!(t[k] > t[k - 1])) return fail(c, FI_ERR_INVALID, "ice times must increase");
+    c->ice_mode = ICE_SEPARABLE;
+    c->ice_t.assign(t, t + nt);
+    c->ice_a.assign(a, a + nt);
+    c->ice_H.assign(H, H + c->N);
+    c->ice_flat = flat;
+    return FI_OK;
+}
+
+int fi_set_output_times(fi_ctx* c, const double* t, int32_t n) {
+    if (!c || n < 0 || (n > 0 && !t)) return fail(c, FI_ERR_INVALID, "fi_set_output_times: bad args");
+    c->tout.assign(t, t + n);
+    std::sort(c->tout.begin(), c->tout.end());
+    c->kout = 0;
+    return FI_OK;
+}
+
+int fi_init(fi_ctx* c, double t0, double t_end) {
+    if (!c) return FI_ERR_INVALID;
+    const fi_config& g = c->cfg;
+    const size_t N = c->N;
+    (void)hipSetDevice(c->dev);
+    if (!c->has_D || !c->has_eta || !c->has_spinv)
+        return fail(c, FI_ERR_INVALID, "litho rigidity, effective viscosity and scaled_pseudodiff_inv required");
+    if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge) return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
+    if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
+        return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
+    if (g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD) {
+        double s = 0;
+        for (double m : c->mask) s += m;
+        if (s > 0.6 * c->nx * c->ny)
+            return fail(c, FI_ERR_INVALID, "Mask defining regions of active load must not cover more than 60% "
+                                           "of the cells when using an interactive sea level.");
+    }
+    const int nx = c->nx, ny = c->ny;
+    // FFT plans and twiddle tables
+    std::vector<int> lens = {nx, ny, c->nfx, c->nfy};
+    std::vector<int> offs;
+    std::vector<float2> tw32;
+    for (int L : lens) {
+        offs.push_back((int)tw32.size());
+        for (int k = 0; k < L; ++k) {
+            double a = -2.0 * M_PI * (double)k / (double)L;
+            tw32.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+    }
+    std::vector<double2> tw64;
+    int off64x = 0, off64y = 0;
+    for (int q = 0; q < 2; ++q) {
+        int L = q ? c->nfy : c->nfx;
+        (q ? off64y : off64x) = (int)tw64.size();
+        for (int k = 0; k < L; ++k) {
+            double a = -2.0 * M_PI * (double)k / (double)L;
+            tw64.push_back(make_double2(std::cos(a), std::sin(a)));
+        }
+    }
+    FftPlan px, py, pfx, pfy, pfx64, pfy64;
+    if (!plan_fft(nx, offs[0], px) || !plan_fft(ny, offs[1], py) || !plan_fft(c->nfx, offs[2], pfx) ||
+        !plan_fft(c->nfy, offs[3], pfy) || !plan_fft(c->nfx, off64x, pfx64) || !plan_fft(c->nfy, off64y, pfy64))
+        return fail(c, FI_ERR_UNSUPPORTED, "FFT length not 2,3,5,7-smooth");
+    if (upload(c, c->tw32, tw32) || upload(c, c->tw64, tw64)) return FI_ERR_DEVICE;
+
+    // spectral multiplier spy/(nx ny) * even part of scaled_pseudodiff_inv, transposed [kx][ky];
+    // the DC term is zeroed: the corner BC on dudt removes any constant exactly
+    std::vector<float> mult((size_t)c->nhx * ny);
+    const double sc = g.seconds_per_year / ((double)nx * ny);
+    for (int kx = 0; kx < c->nhx; ++kx)
+        for (int ky = 0; ky < ny; ++ky) {
+            int mkx = (nx - kx) % nx, mky = (ny - ky) % ny;
+            double m = 0.5 * (c->spinv[kx + (size_t)ky * nx] + c->spinv[mkx + (size_t)mky * nx]);
+            mult[(size_t)kx * ny + ky] = (kx == 0 && ky == 0) ? 0.0f : (float)(m * sc);
+        }
+    if (upload(c, c->mult, mult)) return FI_ERR_DEVICE;
+    std::vector<float> inv2eta(N);
+    for (size_t p = 0; p < N; ++p) inv2eta[p] = (float)(1.0 / (2.0 * c->eta[p]));  // K = 1
+    if (upload(c, c->inv2eta, inv2eta) || upload(c, c->Df, to_f(c->D))) return FI_ERR_DEVICE;
+    std::vector<unsigned char> m8(N);
+    for (size_t p = 0; p < N; ++p) m8[p] = c->mask[p] != 0;
+    if (upload(c, c->maskd, m8)) return FI_ERR_DEVICE;
+
+    // reference state (src/simulation.jl:132-152)
+    c->t_end = t_end;
+    c->H_ice_ref.assign(N, 0.0);
+    if (c->has_href) c->H_ice_ref = c->H_ref_in;
+    else if (c->ice_mode != ICE_EXTERNAL) {
+        int st = host_ice(c, t0, c->H_ice_ref);
+        if (st) return st;
+    }
+    const double rsi = g.rho_seawater / g.rho_ice;
+    c->z_ss_ref.resize(N);
+    c->H_af_ref.resize(N);
+    c->H_water_ref.resize(N);
+    std::vector<unsigned char> mg(N), mo(N);
+    for (size_t p = 0; p < N; ++p) {
+        c->z_ss_ref[p] = g.bsl_z + c->dz_ss_ref[p];
+        double h = hab_h(c->H_ice_ref[p], c->z_b_ref[p], c->z_ss_ref[p], rsi);
+        c->H_af_ref[p] = h;
+        mg[p] = h > 0;
+        mo[p] = (c->z_ss_ref[p] - c->z_b_ref[p] > 0) && !mg[p];
+        double b = std::max(c->z_ss_ref[p] - c->z_b_ref[p], 0.0), H = c->H_ice_ref[p];
+        c->H_water_ref[p] = (H <= 1 ? b : 0.0) + ((!mg[p] && H > 1) ? (b - H * (g.rho_ice / g.rho_seawater)) : 0.0);
+    }
+    if (c->ice_mode == ICE_EXTERNAL) c->H_ext_host = c->H_ice_ref;
+    // device state: CurrentState(domain, ref, z_bsl) (src/state.jl:80-103)
+    HIPCHECK(c, c->U.alloc(3 * N));
+    for (int b = 0; b < 3; ++b)
+        HIPCHECK(c, hipMemcpy(c->U.p + b * N, c->u_ref.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, c->K.alloc(4 * N));
+    HIPCHECK(c, hipMemset(c->K.p, 0, 4 * N * sizeof(float)));
+    HIPCHECK(c, c->kcorner.alloc(16));
+    HIPCHECK(c, hipMemset(c->kcorner.p, 0, 16 * sizeof(float)));
+    HIPCHECK(c, c->spec.alloc((size_t)c->nhx * ny));
+    if (upload(c, c->H_af_ref_d, to_f(c->H_af_ref)) || upload(c, c->u_ref_d, to_f(c->u_ref)) ||
+        upload(c, c->z_b_ref_d, to_f(c->z_b_ref)) || upload(c, c->z_ss_ref_d, to_f(c->z_ss_ref)) ||
+        upload(c, c->ue_ref_d, to_f(c->ue_ref)) || upload(c, c->H_ext, to_f(c->H_ext_host.empty() ? c->H_ice_ref : c->H_ext_host)) ||
+        upload(c, c->ue, to_f(c->ue_ref)) || upload(c, c->dz_ss, std::vector<float>(N, 0.0f)) ||
+        upload(c, c->z_ss, to_f(c->z_ss_ref)) || upload(c, c->H_water, to_f(c->H_water_ref)) ||
+        upload(c, c->mg, mg) || upload(c, c->mo, mo) || upload(c, c->LW, std::vector<float>(N, 0.0f)) ||
+        upload(c, c->sig_load_prev, std::vector<float>(N, 0.0f)) ||
+        upload(c, c->sig_geo_prev, std::vector<float>(N, 0.0f)))
+        return FI_ERR_DEVICE;
+    {
+        // column anomalies start at zero (ColumnAnomalies(domain), src/state.jl:11-14):
+        // LT = sigma_litho - rho_um u_ref with sigma_litho = 0
+        std::vector<float> LT(N), zoff(N);
+        for (size_t p = 0; p < N; ++p) {
+            LT[p] = (float)(-g.rho_uppermantle * c->u_ref[p]);
+            zoff[p] = (float)(c->z_b_ref[p] + c->ue_ref[p] - c->z_ss_ref[p]);
+        }
+        if (upload(c, c->LT, LT) || upload(c, c->zoff, zoff) || upload(c, c->zoff_trig, zoff)) return FI_ERR_DEVICE;
+    }
+    HIPCHECK(c, c->ue_raw.alloc(N));
+    HIPCHECK(c, c->dz_raw.alloc(N));
+    HIPCHECK(c, c->ext_corners.alloc(8));
+    HIPCHECK(c, hipMemset(c->ext_corners.p, 0, 8 * sizeof(float)));
+    HIPCHECK(c, c->pA.alloc((size_t)c->nfhx * ny));
+    HIPCHECK(c, c->pB.alloc((size_t)c->nfhx * ny));
+    HIPCHECK(c, c->oA.alloc((size_t)c->nfhx * (ny + 2)));
+    HIPCHECK(c, c->oB.alloc((size_t)c->nfhx * (ny + 2)));
+    if (c->ice_mode != ICE_EXTERNAL) {
+        if (upload(c, c->ice_t_d, c->ice_t) || upload(c, c->ice_a_d, c->ice_a) || upload(c, c->ice_H_d, to_f(c->ice_H)))
+            return FI_ERR_DEVICE;
+    } else {
+        HIPCHECK(c, c->ice_t_d.alloc(1));
+        HIPCHECK(c, c->ice_a_d.alloc(1));
+        HIPCHECK(c, c->ice_H_d.alloc(1));
+    }
+
+    // Green spectra in FP64
+    HIPCHECK(c, c->ge_hat.alloc((size_t)c->nfhx * c->nfy));
+    HIPCHECK(c, c->gg_hat.alloc((size_t)c->nfhx * c->nfy));
+    HIPCHECK(c, c->gtmp.alloc((size_t)c->nfhx * ny));
+    {
+        DBuf<double> gd;
+        const double nrm = 1.0 / ((double)c->nfx * c->nfy);
+        for (int q = 0; q < 2; ++q) {
+            const std::vector<double>* G = q ? &c->gg : &c->ge;
+            bool have = q ? c->has_gg : c->has_ge;
+            DBuf<double2>& out = q ? c->gg_hat : c->ge_hat;
+            if (!have) {
+                HIPCHECK(c, hipMemset(out.p, 0, out.n * sizeof(double2)));
+                continue;
+            }
+            if (upload(c, gd, *G)) return FI_ERR_DEVICE;
+            size_t l1 = 2 * (size_t)c->nfx * sizeof(double2), l2 = 2 * (size_t)c->nfy * sizeof(double2);
+            HIPCHECK(c, hipFuncSetAttribute((const void*)k_green_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+            HIPCHECK(c, hipFuncSetAttribute((const void*)k_green_cols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2));
+            hipLaunchKernelGGL(k_green_rows, dim3(ny), dim3(NT), l1, c->stream, gd.p, nx, ny, c->nfx, c->nfhx, pfx64,
+                               c->tw64.p, c->gtmp.p);
+            HIPCHECK(c, hipGetLastError());
+            // the sea-surface input is the mass anomaly divided by the cell area: A goes in the kernel
+            double scale = nrm * (q ? c->A : 1.0);
+            hipLaunchKernelGGL(k_green_cols, dim3(c->nfhx), dim3(NT), l2, c->stream, c->gtmp.p, ny, c->nfy, pfy64,
+                               c->tw64.p, scale, out.p);
+            HIPCHECK(c, hipGetLastError());
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+        }
+    }
+
+    // launch geometry
+    KP& kp = c->kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.nx = nx;
+    kp.ny = ny;
+    kp.nhx = c->nhx;
+    kp.nfx = c->nfx;
+    kp.nfy = c->nfy;
+    kp.nfhx = c->nfhx;
+    kp.nrowitems = (ny + 1) / 2;
+    kp.colnb = 1;
+    kp.ncolitems = (c->nhx + kp.colnb - 1) / kp.colnb;
+    kp.i1 = c->i1;
+    kp.i2 = c->i2;
+    kp.j1 = c->j1;
+    kp.j2 = c->j2;
+    kp.dx = c->dx;
+    kp.dy = c->dy;
+    kp.A = c->A;
+    kp.inv_dx = 1.0 / c->dx;
+    kp.inv_dy = 1.0 / c->dy;
+    kp.inv_dx2 = 1.0 / (c->dx * c->dx);
+    kp.inv_dy2 = 1.0 / (c->dy * c->dy);
+    kp.g = g.g;
+    kp.spy = g.seconds_per_year;
+    kp.rho_ice = g.rho_ice;
+    kp.rho_sw_ice = g.rho_seawater / g.rho_ice;
+    kp.rho_ice_sw = g.rho_ice / g.rho_seawater;
+    kp.rho_seawater = g.rho_seawater;
+    kp.rho_water = g.rho_water;
+    kp.rho_um = g.rho_uppermantle;
+    kp.rho_litho = g.rho_litho;
+    kp.nu = g.litho_poissonratio;
+    kp.dt_sparse = g.dt_sparse;
+    kp.reltol = g.reltol;
+    kp.abstol = g.abstol;
+    kp.vbc_x = g.viscous_x_border;
+    kp.ebc_x = g.elastic_x_border;
+    kp.sbc_x = g.ssh_x_border;
+    kp.bsl_z = g.bsl_z;
+    kp.N = (double)N;
+    kp.rigid_litho = g.lithosphere == FI_RIGID_LITHOSPHERE;
+    kp.rigid_mantle = g.mantle == FI_RIGID_MANTLE;
+    kp.compressible_column = g.lithosphere_column == FI_COMPRESSIBLE_LITHOSPHERE_COLUMN;
+    kp.lv_surface = g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE;
+    kp.interactive = g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD;
+    kp.vol_goelzer = g.volume_contribution == FI_GOELZER_CONTRIBUTION;
+    kp.den_goelzer = g.density_contribution == FI_GOELZER_CONTRIBUTION;
+    kp.pov_goelzer = g.adjustment_contribution == FI_GOELZER_CONTRIBUTION;
+    kp.ice_mode = c->ice_mode;
+    kp.ice_nt = (int)c->ice_t.size();
+    kp.ice_flat = c->ice_flat;
+    kp.ice_t = c->ice_t_d.p;
+    kp.ice_a = c->ice_a_d.p;
+    kp.ice_H = c->ice_H_d.p;
+    kp.Nl = (long)N;
+    kp.Ub = c->U.p;
+    kp.Kb = c->K.p;
+    kp.kcorner = c->kcorner.p;
+    kp.spec = c->spec.p;
+    kp.mult = c->mult.p;
+    kp.D = c->Df.p;
+    kp.inv2eta = c->inv2eta.p;
+    kp.mask = c->maskd.p;
+    kp.H_af_ref = c->H_af_ref_d.p;
+    kp.u_ref = c->u_ref_d.p;
+    kp.z_b_ref = c->z_b_ref_d.p;
+    kp.z_ss_ref = c->z_ss_ref_d.p;
+    kp.ue_ref = c->ue_ref_d.p;
+    kp.H_ext = c->H_ext.p;
+    kp.ue = c->ue.p;
+    kp.dz_ss = c->dz_ss.p;
+    kp.z_ss = c->z_ss.p;
+    kp.zoff = c->zoff.p;
+    kp.zoff_trig = c->zoff_trig.p;
+    kp.LW = c->LW.p;
+    kp.LT = c->LT.p;
+    kp.H_water = c->H_water.p;
+    kp.mg = c->mg.p;
+    kp.mo = c->mo.p;
+    kp.sig_load_prev = c->sig_load_prev.p;
+    kp.sig_geo_prev = c->sig_geo_prev.p;
+    kp.pA = c->pA.p;
+    kp.pB = c->pB.p;
+    kp.oA = c->oA.p;
+    kp.oB = c->oB.p;
+    kp.ue_raw = c->ue_raw.p;
+    kp.dz_raw = c->dz_raw.p;
+    kp.ext_corners = c->ext_corners.p;
+    kp.ge_hat = c->ge_hat.p;
+    kp.gg_hat = c->gg_hat.p;
+    kp.px = px;
+    kp.py = py;
+    kp.pfx = pfx;
+    kp.pfy = pfy;
+    kp.tw32 = c->tw32.p;
+    kp.spin_limit = 2000000000ull;  // 20 s of the 100 MHz wall clock
+
+    // LDS: rows phase (u FP64 6 rows, 3 moment fields 4 rows, FFT pair) vs the column phases
+    size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16);
+    size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)2 * c->nfx * 8, (size_t)4 * c->nfy * 8});
+    c->mt_double = rows_d <= 150 * 1024;
+    c->lds = std::max(c->mt_double ? rows_d : rows_f, other);
+    if (c->lds > 160 * 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
+    const void* f = c->mt_double ? (const void*)fi_persistent<double> : (const void*)fi_persistent<float>;
+    HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
+    int per_cu = 0, ncu = 0;
+    HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, c->lds));
+    HIPCHECK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+    int maxg = per_cu * ncu;
+    int want = std::max(kp.nrowitems, kp.ncolitems);
+    c->G = std::max(1, std::min(maxg, want));
+    if (maxg < 1) return fail(c, FI_ERR_UNSUPPORTED, "persistent kernel does not fit on the device");
+    HIPCHECK(c, c->part.alloc(4 * (size_t)c->G));
+    HIPCHECK(c, c->vpart.alloc(4 * (size_t)c->G));
+    kp.part = c->part.p;
+    kp.vpart = c->vpart.p;
+    HIPCHECK(c, c->ctl_d.alloc(1));
+    HIPCHECK(c, c->bar_d.alloc(1));
+    HIPCHECK(c, c->kp_d.alloc(1));
+    kp.ctl = c->ctl_d.p;
+    kp.bar = c->bar_d.p;
+
+    std::memset(&c->ctl, 0, sizeof(Ctl));
+    c->ctl.t0 = t0;
+    c->ctl.t = t0;
+    c->ctl.t_last = t0;
+    c->ctl.z_bsl = g.bsl_z;
+    c->ctl.qold = PI_QOLDINIT;
+    c->ctl.cur = 0;
+    c->ctl.k1 = 0;
+    c->ctl.last_k = 0;
+    c->ctl.last_u = 0;
+    c->initialized = true;
+    c->integ = false;
+    c->kout = 0;
+    return FI_OK;
+}
+
+// diagnostic fields that the engine does not store are assembled from stored ones with the
+// reference formulas (src/topography.jl:30-38, src/loads.jl:31-106)
+int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
+    if (!c || !h || n != c->N) return fail(c, FI_ERR_INVALID, "fi_get_field: size mismatch");
+    const size_t N = c->N;
+    auto put = [&](const std::vector<double>& v) -> int {
+        if (v.size() != N) return fail(c, FI_ERR_STATE, "fi_get_field: field not set");
+        std::memcpy(h, v.data(), N * sizeof(double));
+        return FI_OK;
+    };
+    switch (f) {
+        case FI_LITHO_RIGIDITY: return put(c->D);
+        case FI_EFFECTIVE_VISCOSITY: return put(c->eta);
+        case FI_SCALED_PSEUDODIFF_INV: return put(c->spinv);
+        case FI_MASK_ACTIVE: return put(c->mask);
+        case FI_GREEN_ELASTIC: return put(c->ge);
+        case FI_GREEN_GEOID: return put(c->gg);
+        case FI_REF_U: return put(c->u_ref);
+        case FI_REF_UE: return put(c->ue_ref);
+        case FI_REF_DZ_SS: return put(c->dz_ss_ref);
+        case FI_REF_Z_B: return put(c->z_b_ref);
+        case FI_REF_H_ICE: return put(c->H_ice_ref);
+        default: break;
+    }
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_get_field before fi_init");
+    (void)hipSetDevice(c->dev);
+    const fi_config& g = c->cfg;
+    std::vector<double> v, u, ue, zss, H;
+    int st = 0;
+    auto umain = [&](std::vector<double>& out) -> int {
+        int s = read_u(c, c->ctl.last_u, out);
+        if (s) return s;
+        if (c->ctl.last_u == 2) {  // fi_rhs input buffer is kept unmodified: apply the corner BC here
+            double off = corner_offset(out.data(), c->nx, c->ny, g.viscous_x_border);
+            for (double& x : out) x -= off;
+        }
+        return FI_OK;
+    };
+    switch (f) {
+        case FI_U: if ((st = umain(v))) return st; return put(v);
+        case FI_DUDT: if ((st = read_k(c, c->ctl.last_k, v))) return st; return put(v);
+        case FI_UE: if ((st = read_f(c, c->ue.p, v))) return st; return put(v);
+        case FI_DZ_SS: if ((st = read_f(c, c->dz_ss.p, v))) return st; return put(v);
+        case FI_Z_SS: if ((st = read_f(c, c->z_ss.p, v))) return st; return put(v);
+        case FI_H_WATER: if ((st = read_f(c, c->H_water.p, v))) return st; return put(v);
+        case FI_CANOM_SEAWATER: if ((st = read_f(c, c->LW.p, v))) return st; return put(v);
+        case FI_MASK_GROUNDED:
+        case FI_MASK_OCEAN: {
+            std::vector<unsigned char> m(N);
+            HIPCHECK(c, hipMemcpy(m.data(), f == FI_MASK_GROUNDED ? c->mg.p : c->mo.p, N, hipMemcpyDeviceToHost));
+            for (size_t p = 0; p < N; ++p) h[p] = m[p] ? 1.0 : 0.0;
+            return FI_OK;
+        }
+        default: break;
+    }
+    if ((st = umain(u)) || (st = read_f(c, c->ue.p, ue)) || (st = read_f(c, c->z_ss.p, zss))) return st;
+    if ((st = host_ice(c, c->ctl.t_last, H))) return st;
+    v.resize(N);
+    const double rsi = g.rho_seawater / g.rho_ice;
+    for (size_t p = 0; p < N; ++p) {
+        double zb = c->z_b_ref[p] + ue[p] + u[p];
+        double haf = hab_h(H[p], zb, zss[p], rsi);
+        double s_ice = g.rho_ice * (haf - c->H_af_ref[p]);
+        double s_litho = g.rho_litho * (ue[p] - c->ue_ref[p]);
+        double s_mant = g.rho_uppermantle * (u[p] - c->u_ref[p]);
+        double m = c->mask[p];
+        switch (f) {
+            case FI_Z_B: v[p] = zb; break;
+            case FI_H_ICE: v[p] = H[p]; break;
+            case FI_H_AF: v[p] = haf; break;
+            case FI_CANOM_ICE: v[p] = s_ice; break;
+            case FI_CANOM_LITHO: v[p] = s_litho; break;
+            case FI_CANOM_MANTLE: v[p] = s_mant; break;
+            case FI_CANOM_LOAD:
+            case FI_CANOM_FULL: v[p] = 0; break;
+            default: return fail(c, FI_ERR_INVALID, "fi_get_field: unknown field");
+        }
+    }
+    if (f == FI_CANOM_LOAD || f == FI_CANOM_FULL) {
+        std::vector<double> sw;
+        if ((st = read_f(c, c->LW.p, sw))) return st;
+        for (size_t p = 0; p < N; ++p) {
+            double zb = c->z_b_ref[p] + ue[p] + u[p];
+            double haf = hab_h(H[p], zb, zss[p], rsi);
+            double s_load = c->mask[p] * (g.rho_ice * (haf - c->H_af_ref[p]) + sw[p]);
+            v[p] = f == FI_CANOM_LOAD ? s_load
+                                      : s_load + c->mask[p] * (g.rho_litho * (ue[p] - c->ue_ref[p]) +
+                                                               g.rho_uppermantle * (u[p] - c->u_ref[p]));
+        }
+    }
+    return put(v);
+}
+
+int fi_rhs(fi_ctx* c, double t, double* u, double* dudt) {
+    if (!c || !u || !dudt) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_rhs before fi_init");
+    (void)hipSetDevice(c->dev);
+    const int ubuf = 2, slot = (c->ctl.k1 + 1) % 4;
+    HIPCHECK(c, hipMemcpy(c->U.p + 2 * c->N, u, c->N * sizeof(double), hipMemcpyHostToDevice));
+    int st = rhs_on_device(c, t, ubuf, slot);
+    if (st) return st;
+    std::vector<double> k;
+    if ((st = read_k(c, slot, k))) return st;
+    std::memcpy(dudt, k.data(), c->N * sizeof(double));
+    double off = corner_offset(u, c->nx, c->ny, c->cfg.viscous_x_border);
+    for (size_t p = 0; p < c->N; ++p) u[p] -= off;
+    c->ctl.last_u = ubuf;
+    return FI_OK;
+}
+
+int fi_rhs_device(fi_ctx* c, double t, void* u, void* dudt) {
+    if (!c || !u || !dudt) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_rhs_device before fi_init");
+    (void)hipSetDevice(c->dev);
+    const int ubuf = 2, slot = (c->ctl.k1 + 1) % 4;
+    HIPCHECK(c, hipMemcpyAsync(c->U.p + 2 * c->N, u, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    int st = rhs_on_device(c, t, ubuf, slot);
+    if (st) return st;
+    float kc[4];
+    double uc[4];
+    HIPCHECK(c, hipMemcpy(kc, c->kcorner.p + 4 * slot, sizeof(kc), hipMemcpyDeviceToHost));
+    const size_t N = c->N;
+    const size_t idx[4] = {0, (size_t)c->nx - 1, (size_t)(c->ny - 1) * c->nx, (size_t)(c->ny - 1) * c->nx + c->nx - 1};
+    for (int q = 0; q < 4; ++q) HIPCHECK(c, hipMemcpy(&uc[q], (double*)u + idx[q], sizeof(double), hipMemcpyDeviceToHost));
+    double uoff = ((0.25 * uc[0] + 0.25 * uc[1]) + (0.25 * uc[2] + 0.25 * uc[3])) - c->cfg.viscous_x_border;
+    double koff = kslot_offset(c, slot, kc);
+    int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_dudt_out, dim3(nb), dim3(256), 0, c->stream, c->K.p + slot * N, koff, (long)N, (double*)dudt);
+    hipLaunchKernelGGL(k_sub_offset, dim3(nb), dim3(256), 0, c->stream, (double*)u, uoff, (long)N);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->ctl.last_u = ubuf;
+    return FI_OK;
+}
+
+int fi_sparse_update(fi_ctx* c, double t) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_sparse_update before fi_init");
+    (void)hipSetDevice(c->dev);
+    c->ctl.t_rhs = t;
+    return launch(c, MODE_SPARSE_ONLY, c->ctl.last_u, (c->ctl.k1 + 1) % 4, 0, 1);
+}
+
+// init_problem! + FSAL + initial step (src/simulation.jl:293-311)
+int fi_integrator_init(fi_ctx* c) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_integrator_init before fi_init");
+    (void)hipSetDevice(c->dev);
+    const fi_config& g = c->cfg;
+    const size_t N = c->N;
+    int st;
+    // update_V_af!, update_V_den!, update_V_pov! on the current state
+    {
+        std::vector<double> haf(N), H(N), zb(N), zss(N);
+        if ((st = fi_get_field(c, FI_H_AF, haf.data(), N)) || (st = fi_get_field(c, FI_H_ICE, H.data(), N)) ||
+            (st = fi_get_field(c, FI_Z_B, zb.data(), N)) || (st = fi_get_field(c, FI_Z_SS, zss.data(), N)))
+            return st;
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (size_t p = 0; p < N; ++p) {
+            s0 += haf[p] * c->A;
+            s1 += H[p] * c->A;
+            s2 += std::max(zss[p] - zb[p], 0.0) * c->A;
+        }
+        c->ctl.V_af = g.volume_contribution ? s0 * g.rho_ice / g.rho_seawater : 0.0;
+        c->ctl.V_den = g.density_contribution ? s1 * (g.rho_ice / g.rho_water - g.rho_ice / g.rho_seawater) : 0.0;
+        c->ctl.V_pov = g.adjustment_contribution ? s2 : 0.0;
+    }
+    // u0 = now.u with the corner BC (update_diagnostics! mutates it)
+    std::vector<double> u0;
+    if ((st = read_u(c, c->ctl.last_u, u0))) return st;
+    double off = corner_offset(u0.data(), c->nx, c->ny, g.viscous_x_border);
+    for (double& x : u0) x -= off;
+    HIPCHECK(c, hipMemcpy(c->U.p, u0.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    const double t0 = c->ctl.t0;
+    c->ctl.cur = 0;
+    c->ctl.last_u = 0;
+    if ((st = rhs_on_device(c, t0, 0, 1))) return st;   // init_problem!
+    if ((st = rhs_on_device(c, t0, 0, 0))) return st;   // FSAL: k1 = f(u0, t0)
+    // initial step (Hairer, Norsett & Wanner, Solving ODEs I, II.4)
+    std::vector<double> f0, f1;
+    if ((st = read_k(c, 0, f0))) return st;
+    std::vector<double> sk(N), tq(N);
+    auto rms = [&](const std::vector<double>& a) {
+        double s = 0;
+        for (double x : a) s += x * x;
+        return std::sqrt(s / (double)a.size());
+    };
+    for (size_t p = 0; p < N; ++p) sk[p] = g.abstol + std::fabs(u0[p]) * g.reltol;
+    for (size_t p = 0; p < N; ++p) tq[p] = u0[p] / sk[p];
+    double d0 = rms(tq);
+    for (size_t p = 0; p < N; ++p) tq[p] = f0[p] / sk[p];
+    double d1 = rms(tq);
+    double dtmax = c->t_end - t0;
+    double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+    dt0 = std::min(dt0, dtmax);
+    std::vector<double> u1(N);
+    for (size_t p = 0; p < N; ++p) u1[p] = u0[p] + dt0 * f0[p];
+    HIPCHECK(c, hipMemcpy(c->U.p + 2 * N, u1.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    if ((st = rhs_on_device(c, t0 + dt0, 2, 1))) return st;
+    if ((st = read_k(c, 1, f1))) return st;
+    for (size_t p = 0; p < N; ++p) tq[p] = (f1[p] - f0[p]) / sk[p];
+    double d2 = rms(tq) / dt0;
+    double mx = std::max(d1, d2);
+    double dt1 = (mx <= 1e-15) ? std::max(1e-6, dt0 * 1e-3) : std::pow(10.0, -(2.0 + std::log10(mx)) / 3.0);
+    c->ctl.dt = std::min(std::min(100 * dt0, dt1), dtmax);
+    c->ctl.t = t0;
+    c->ctl.qold = PI_QOLDINIT;
+    c->ctl.k1 = 0;
+    c->ctl.cur = 0;
+    c->ctl.last_u = 0;
+    c->ctl.nacc = c->ctl.nrej = 0;
+    c->integ = true;
+    c->k1_stale = false;
+    c->kout = 0;
+    return FI_OK;
+}
+
+int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_integrate before fi_integrator_init");
+    if (t_target < c->ctl.t) return fail(c, FI_ERR_INVALID, "fi_integrate: target before current time");
+    (void)hipSetDevice(c->dev);
+    int st;
+    while (c->ctl.t < t_target) {
+        if (c->k1_stale) {
+            if ((st = rhs_on_device(c, c->ctl.t, c->ctl.cur, c->ctl.k1))) return st;
+            c->ctl.last_u = c->ctl.cur;
+            c->k1_stale = false;
+        }
+        double stop = t_target;
+        for (double to : c->tout)
+            if (to > c->ctl.t) { stop = std::min(stop, to); break; }
+        c->ctl.stop = stop;
+        c->ctl.next_out = (c->kout < (long long)c->tout.size()) ? c->tout[c->kout] : INFINITY;
+        if ((st = launch(c, MODE_INTEGRATE, 0, 0, 20000, 0))) return st;
+        c->ctl.last_u = c->ctl.cur;
+        if (c->ctl.steps == 1 && c->kout < (long long)c->tout.size() && c->ctl.t >= c->tout[c->kout]) {
+            if (cb) cb(user, c->ctl.t, (int32_t)c->kout);
+            c->kout += 1;
+            // a DiscreteCallback marks u as modified: the FSAL derivative is re-evaluated
+            if ((st = rhs_on_device(c, c->ctl.t, c->ctl.cur, c->ctl.k1))) return st;
+            c->ctl.last_u = c->ctl.cur;
+        }
+    }
+    return FI_OK;
+}
+
+int fi_get_stats(fi_ctx* c, fi_stats* o) {
+    if (!c || !o) return FI_ERR_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->t = c->integ ? c->ctl.t : c->ctl.t_last;
+    o->dt = c->ctl.dt;
+    o->n_rhs = c->ctl.nrhs;
+    o->n_accept = c->ctl.nacc;
+    o->n_reject = c->ctl.nrej;
+    o->count_sparse_updates = c->ctl.count;
+    o->n_output = c->kout;
+    o->V_af = c->ctl.V_af;
+    o->V_den = c->ctl.V_den;
+    o->V_pov = c->ctl.V_pov;
+    o->delta_V = c->ctl.delta_V;
+    o->z_bsl = c->ctl.z_bsl;
+    o->last_EEst = c->ctl.EEst;
+    o->device_ms = c->device_ms;
+    o->n_launch = c->n_launch;
+    o->n_accept_timed = c->n_accept_timed;
+    return FI_OK;
+}
+
+}  // extern "C"

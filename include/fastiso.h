@@ -149,6 +149,9 @@ typedef struct fi_stats {
     int64_t n_output;             /* output callbacks fired (NativeOutput.k - 1)               */
     double V_af, V_den, V_pov, delta_V, z_bsl;   /* src/state.jl:69-73                         */
     double last_EEst;             /* error norm of the last attempted step                      */
+    double device_ms;             /* integration-kernel time, HIP events on the context's stream */
+    int64_t n_launch;             /* integration-kernel launches timed in device_ms             */
+    int64_t n_accept_timed;       /* accepted steps inside those launches                       */
 } fi_stats;
 
 /* Output callback, fired after an accepted step at which t >= the next output time

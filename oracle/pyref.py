@@ -29,6 +29,7 @@ class RefStats(C.Structure):
         ("count_sparse_updates", C.c_int64), ("n_output", C.c_int64),
         ("V_af", C.c_double), ("V_den", C.c_double), ("V_pov", C.c_double), ("delta_V", C.c_double),
         ("z_bsl", C.c_double), ("last_EEst", C.c_double),
+        ("device_ms", C.c_double), ("n_launch", C.c_int64), ("n_accept_timed", C.c_int64),
     ]
 
 

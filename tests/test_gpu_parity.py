@@ -1,0 +1,165 @@
+"""Parity of the HIP engine (through its C ABI) against the FP64 oracle and closed forms.
+
+Tolerances: the engine runs the FFTs and the second stencil sweep in FP32 (FP64 state, first
+stencil sweep, loads and error norm), so single RHS evaluations are compared at
+max |d - d_ref| <= 2e-6 max |d_ref| and relative L2 <= 1e-6; trajectories at the output times
+within 1e-5 (disc) or 1e-4 (laterally variable) of max |u| (the step-size sequences of the two back-ends differ once their error
+estimates differ in the last FP32 digits).
+"""
+import numpy as np
+import pytest
+
+import fastiso
+from fastiso import workloads
+from fastiso.analytic import analytic_solution
+
+from _cases import corner_mean, cosine_case, lv_case, sim_inputs
+
+pytestmark = pytest.mark.gpu
+
+RHS_MAX = 2e-6
+RHS_L2 = 1e-6
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max(), np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def pair(w):
+    from oracle import pyref
+    sim = sim_inputs(w, engine="hip")
+    return sim, pyref.RefEngine.from_inputs(sim.inputs)
+
+
+def test_rhs_cosine_closed_form():
+    w = cosine_case()
+    sim, ref = pair(w)
+    d, c, se = w["domain"], w["c"], w["solidearth"]
+    u = 7.0 * np.cos(2 * np.pi * w["l"] * d.Y / (2 * d.Wy)) + 3.0
+    du, ubc = sim.engine.rhs(1.0, u)
+    eta = w["eta"] * 1.5 / 1.28
+    F = c.seconds_per_year / (2 * eta) * (
+        -c.g * c.rho_ice * w["A"] * np.cos(2 * np.pi * w["k"] * d.X / (2 * d.Wx)) * (d.Wx / (np.pi * w["k"]))
+        - c.g * se.rho_uppermantle * 7.0 * np.cos(2 * np.pi * w["l"] * d.Y / (2 * d.Wy)) * (d.Wy / (np.pi * w["l"])))
+    want = F - corner_mean(F)
+    m, l2 = rel(du, want)
+    assert m < RHS_MAX and l2 < RHS_L2, (m, l2)
+    np.testing.assert_allclose(ubc, u - corner_mean(u), atol=1e-12)
+    np.testing.assert_allclose(sim.engine.get_field("u"), ubc, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,seed,surface", [(32, 11, True), (64, 5, True), (48, 7, False), (96, 3, True)])
+def test_rhs_sequence_vs_oracle(n, seed, surface):
+    """A sequence of RHS calls through two sparse updates: the first at t0 reads zero loads, the
+    second (t = 600 after t = 500) reads the loads of the call before it."""
+    w = lv_case(n=n, seed=seed, sea_surface=surface)
+    sim, ref = pair(w)
+    rng = np.random.default_rng(seed)
+    u = -50 * np.exp(-(w["domain"].R / 1e6) ** 2) + 0.01 * rng.standard_normal((n, n))
+    for t in (0.0, 500.0, 600.0, 605.0):
+        dg, ug = sim.engine.rhs(t, u)
+        dr, ur = ref.rhs(t, u)
+        m, l2 = rel(dg, dr)
+        assert m < RHS_MAX and l2 < RHS_L2, (t, m, l2)
+        np.testing.assert_allclose(ug, ur, atol=1e-9)
+    assert sim.engine.stats()["count_sparse_updates"] == ref.stats()["count_sparse_updates"] == 3
+    for f in ("ue", "dz_ss", "z_ss"):
+        a, b = sim.engine.get_field(f), ref.get_field(f)
+        if np.abs(b).max() == 0:
+            assert np.abs(a).max() == 0
+            continue
+        m, l2 = rel(a, b)
+        assert m < 5e-6 and l2 < 5e-6, (f, m, l2)
+    for f in ("canom_full", "canom_load", "H_af", "z_b"):
+        a, b = sim.engine.get_field(f), ref.get_field(f)
+        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1.0), f
+
+
+def test_rhs_snapshots_equal_separable():
+    """TimeInterpolatedIceThickness snapshots and the separable form give the same RHS."""
+    a = lv_case(n=32, separable=True)
+    b = lv_case(n=32, separable=False)
+    sa, sb = sim_inputs(a, engine="hip"), sim_inputs(b, engine="hip")
+    u = np.zeros((32, 32))
+    for t in (0.0, 1500.5):
+        da, _ = sa.engine.rhs(t, u)
+        db, _ = sb.engine.rhs(t, u)
+        m, l2 = rel(da, db)
+        assert m < 1e-6 and l2 < 1e-6
+
+
+def test_rhs_device_pointers_match_host():
+    import torch
+    w = lv_case(n=32)
+    sim = sim_inputs(w, engine="hip")
+    u = -20 * np.exp(-(w["domain"].R / 8e5) ** 2)
+    dh, uh = sim.engine.rhs(0.0, u)
+    sim2 = sim_inputs(w, engine="hip")
+    ud = torch.tensor(np.asarray(u).ravel(order="F"), dtype=torch.float64, device="cuda")
+    dd = torch.empty_like(ud)
+    st = sim2.engine.lib.fi_rhs_device(sim2.engine.h, 0.0, ud.data_ptr(), dd.data_ptr())
+    assert st == 0
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(dd.cpu().numpy().reshape((32, 32), order="F"), dh, rtol=0, atol=0)
+    np.testing.assert_allclose(ud.cpu().numpy().reshape((32, 32), order="F"), uh, rtol=0, atol=1e-12)
+
+
+def test_time_out_of_range_is_an_error():
+    w = lv_case(n=32, t_end=1000.0)
+    sim = sim_inputs(w, engine="hip")
+    with pytest.raises(fastiso.EngineError) as e:
+        sim.engine.rhs(200e3, np.zeros((32, 32)))
+    assert e.value.code == -3
+
+
+@pytest.fixture(scope="module")
+def disc_runs():
+    from oracle import pyref
+    w = workloads.disc_benchmark(n=64)
+    sim = sim_inputs(w, engine="hip")
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    og, orf = [], []
+    sim.engine.integrator_init()
+    sim.engine.integrate(w["t_span"][1], lambda t, k: og.append((t, sim.engine.get_field("u"))))
+    ref.integrator_init()
+    ref.integrate(w["t_span"][1], lambda t, k: orf.append((t, ref.get_field("u"))))
+    return w, sim, ref, og, orf
+
+
+def test_disc_trajectory_vs_oracle(disc_runs):
+    w, sim, ref, og, orf = disc_runs
+    assert [t for t, _ in og] == [t for t, _ in orf] == list(np.arange(1e3, 50e3 + 1, 1e3))
+    for (t, a), (_, b) in zip(og, orf):
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max(), t
+
+
+def test_disc_trajectory_vs_analytic(disc_runs):
+    """64x64 disc benchmark (docs/src/examples/benchmark_analytic.jl) against
+    analytic_solution (src/analytic_solutions.jl:11-65) on the centre transect."""
+    w, sim, ref, og, orf = disc_runs
+    d, c, se = w["domain"], w["c"], w["solidearth"]
+    j = d.my - 1
+    for t, u in og:
+        ua = analytic_solution(np.abs(d.x), t * c.seconds_per_year, c, se, 1e3, 1e6)
+        assert np.abs(u[:, j] - ua).max() < 0.05 * np.abs(ua).max() + 5.0, t
+        if t >= 10e3:
+            assert abs(u[d.mx - 1, j] - ua[d.mx - 1]) < 0.5, t
+
+
+def test_lv_trajectory_vs_oracle():
+    from oracle import pyref
+    w = lv_case(n=32, t_end=200.0)
+    sim = sim_inputs(w, engine="hip")
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    og, orf = [], []
+    sim.engine.integrator_init()
+    sim.engine.integrate(200.0, lambda t, k: og.append((t, sim.engine.get_field("u"), sim.engine.get_field("ue"))))
+    ref.integrator_init()
+    ref.integrate(200.0, lambda t, k: orf.append((t, ref.get_field("u"), ref.get_field("ue"))))
+    assert len(og) == len(orf) == 4
+    for (t, a, ea), (_, b, eb) in zip(og, orf):
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max() + 1e-6, t
+        assert np.abs(ea - eb).max() <= 1e-4 * np.abs(eb).max() + 1e-6, t
+    sg, sr = sim.engine.stats(), ref.stats()
+    assert abs(sg["n_accept"] - sr["n_accept"]) <= 0.05 * sr["n_accept"] + 3
+    assert sg["count_sparse_updates"] == sr["count_sparse_updates"]
