@@ -1,8 +1,24 @@
 """Small seeded cases shared by the oracle pins and the GPU parity tests."""
+import os
+
 import numpy as np
 
 import fastiso
 from fastiso import workloads
+
+
+# FASTISO_TEST_DRYRUN=1 runs the GPU tests on the CPU with the oracle standing in for the HIP
+# engine: it checks the tests' own expectations before GPU time is spent; it proves nothing
+# about the engine.
+DRYRUN = os.environ.get("FASTISO_TEST_DRYRUN") == "1"
+
+
+def hip_sim(w):
+    sim = sim_inputs(w, engine=None if DRYRUN else "hip")
+    if DRYRUN:
+        from oracle import pyref
+        sim.engine = pyref.RefEngine.from_inputs(sim.inputs)
+    return sim
 
 
 def sim_inputs(w, engine=None):
