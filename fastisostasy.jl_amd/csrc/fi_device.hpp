@@ -7,7 +7,7 @@
 
 namespace fi {
 
-constexpr int NT = 256;  // threads per workgroup (four wave64)
+constexpr int NT = 512;  // threads per workgroup (eight wave64: two per SIMD)
 
 enum { MODE_RHS = 0, MODE_INTEGRATE = 1, MODE_SPARSE_ONLY = 2 };
 enum { ICE_EXTERNAL = 0, ICE_SNAPSHOTS = 1, ICE_SEPARABLE = 2 };
@@ -49,12 +49,14 @@ struct Ctl {
     int pad0, pad1;
 };
 
+// grid barrier words: the release generation and the abort flag, followed in memory by one
+// 128-byte arrival slot per workgroup (BAR_SLOT unsigneds apart)
 struct Bar {
-    unsigned count;
-    unsigned gen;
+    unsigned release;
     int abort;
-    int pad;
+    unsigned pad[30];
 };
+constexpr int BAR_SLOT = 32;
 
 struct KP {
     int nx, ny, nhx, nfx, nfy, nfhx, nrowitems, ncolitems, colnb;
@@ -74,7 +76,8 @@ struct KP {
     double* Ub;
     float* Kb;
     float* kcorner;  // [4 slots][4 corners], raw
-    float2* spec;    // [nhx][ny]
+    float2* specb;   // two spectrum buffers [2][nhx][ny]; an RHS call writes buffer (call count & 1)
+    long specN;      // nhx * ny
     const float* mult;
     const float* D;
     const float* inv2eta;
@@ -115,6 +118,9 @@ struct KP {
     int mode, rhs_ubuf, rhs_kslot, max_steps;
     int write_sigma_always;
     unsigned long long spin_limit;  // wall-clock ticks before a barrier wait gives up
+    // optional phase timing (workgroup 0): [tag] work ticks, [8+tag] wait ticks, [16+tag] count
+    unsigned long long* ptime;
 };
+enum { TAG_ROWS = 0, TAG_COLS = 1, TAG_C2R = 2, TAG_SPARSE = 3, TAG_KIN = 4 };
 
 }  // namespace fi

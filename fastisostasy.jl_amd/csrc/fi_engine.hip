@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -23,24 +24,33 @@ namespace fi {
 template <typename MT>
 __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
     const KP& p = *pp;
-    extern __shared__ __align__(16) char smem[];
+    extern __shared__ __align__(16) char smem_all[];
     __shared__ Ctl s;
-    __shared__ double red[3 * NT];
+    __shared__ double red[64];
     __shared__ int flag;
     unsigned gen = 0;
+    unsigned long long tl = wall_clock64();
+    // twiddle tables of the two run-time transform lengths stay in LDS for the whole kernel
+    float2* twx = (float2*)smem_all;
+    float2* twy = twx + p.nx;
+    char* smem = (char*)(twy + p.ny);
+    for (int k = threadIdx.x; k < p.nx; k += NT) twx[k] = p.tw32[p.px.tw + k];
+    for (int k = threadIdx.x; k < p.ny; k += NT) twy[k] = p.tw32[p.py.tw + k];
     if (threadIdx.x == 0) s = *p.ctl;
     __syncthreads();
 
     if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY) {
         StageDesc st{};
         st.ubase = p.rhs_ubuf;
-        st.nk = 0;
+        st.ng = 0;
+        st.kin = -1;
         st.dt = 0;
         st.write_u = -1;
         st.kout = p.rhs_kslot;
+        st.do_c2r = 1;
         st.t = s.t_rhs;
         st.t_next_max = s.t_rhs;
-        rhs_call<MT>(p, s, st, gen, smem, red, &flag, p.mode == MODE_SPARSE_ONLY);
+        rhs_call<MT>(p, s, st, gen, twx, twy, smem, red, &flag, p.mode == MODE_SPARSE_ONLY, tl);
         if (blockIdx.x == 0 && threadIdx.x == 0) *p.ctl = s;
         return;
     }
@@ -71,30 +81,32 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
             st.ubase = cur;
             st.dt = dt;
             st.write_u = -1;
+            st.kin = -1;
             if (stg == 0) {          // tmp = uprev + dt a21 k1
-                st.nk = 1;
-                st.slot0 = k1;
+                st.ng = 1;
+                st.g0 = k1;
                 st.a0 = BS3_A21;
                 st.kout = s2;
                 st.t = tn + 0.5 * dt;
                 st.t_next_max = tn + 0.75 * dt;
-            } else if (stg == 1) {   // tmp = uprev + dt a32 k2
-                st.nk = 1;
-                st.slot0 = s2;
-                st.a0 = BS3_A32;
+            } else if (stg == 1) {   // tmp = uprev + dt a32 k2 (k2 still spectral)
+                st.ng = 0;
+                st.kin = s2;
+                st.ain = BS3_A32;
                 st.kout = s3;
                 st.t = tn + 0.75 * dt;
                 st.t_next_max = tn + dt;
             } else {                 // u = uprev + dt (a41 k1 + a42 k2 + a43 k3); error moments
-                st.nk = 3;
-                st.slot0 = k1;
-                st.slot1 = s2;
-                st.slot2 = s3;
+                st.ng = 2;
+                st.g0 = k1;
                 st.a0 = BS3_A41;
+                st.g1 = s2;
                 st.a1 = BS3_A42;
-                st.a2 = BS3_A43;
+                st.kin = s3;
+                st.ain = BS3_A43;
                 st.write_u = unew;
                 st.kout = s4;
+                st.do_c2r = 1;
                 st.t = tn + dt;
                 st.t_next_max = tn + dt + 0.5 * PI_QMAX * dt;
                 st.final_err = 1;
@@ -103,7 +115,7 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
                 st.e3 = s3;
                 st.u_prev = cur;
             }
-            if (!rhs_call<MT>(p, s, st, gen, smem, red, &flag, false)) {
+            if (!rhs_call<MT>(p, s, st, gen, twx, twy, smem, red, &flag, false, tl)) {
                 ok = false;
                 break;
             }
@@ -175,7 +187,7 @@ __global__ void __launch_bounds__(NT) k_green_rows(const double* __restrict__ G,
         a[i] = z;
     }
     __syncthreads();
-    double2* r = fft_lds<double2>(a, b, 1, nfx, pl, tw, -1);
+    double2* r = fft_lds<double2>(a, b, 1, nfx, pl, tw + pl.tw, -1);
     for (int kx = threadIdx.x; kx < nfhx; kx += NT) out[(long)kx * ny + j] = r[kx];
 }
 
@@ -193,7 +205,7 @@ __global__ void __launch_bounds__(NT) k_green_cols(const double2* __restrict__ i
         a[j] = z;
     }
     __syncthreads();
-    double2* r = fft_lds<double2>(a, b, 1, nfy, pl, tw, -1);
+    double2* r = fft_lds<double2>(a, b, 1, nfy, pl, tw + pl.tw, -1);
     for (int ky = threadIdx.x; ky < nfy; ky += NT) {
         double2 v = r[ky];
         v.x *= scale;
@@ -288,6 +300,7 @@ struct fi_ctx {
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
+    DBuf<unsigned long long> ptime;
     KP kp;
     Ctl ctl;
     int G = 0;
@@ -362,7 +375,7 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     kp.rhs_kslot = kslot;
     kp.max_steps = max_steps;
     kp.write_sigma_always = write_sigma_always;
-    HIPCHECK(c, hipMemsetAsync(c->bar_d.p, 0, sizeof(Bar), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->bar_d.p, 0, sizeof(Bar) * (1 + 2 * (size_t)c->G), c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->kp_d.p, &kp, sizeof(KP), hipMemcpyHostToDevice, c->stream));
     const KP* kptr = c->kp_d.p;
@@ -526,6 +539,21 @@ void fi_destroy(fi_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ptime.p) {
+        // per-phase wall-clock totals of workgroup 0 (100 MHz ticks): work before the barrier and
+        // the barrier wait
+        unsigned long long h[32];
+        if (hipMemcpy(h, c->ptime.p, sizeof(h), hipMemcpyDeviceToHost) == hipSuccess) {
+            const char* names[5] = {"rows", "cols", "c2r", "sparse", "kin-c2r"};
+            for (int t = 0; t < 5; ++t)
+                if (h[16 + t])
+                    fprintf(stderr, "fastiso phase %-8s n=%llu work %.3f us wait %.3f us (per phase)\n", names[t],
+                            h[16 + t], h[t] / 100.0 / h[16 + t], h[8 + t] / 100.0 / h[16 + t]);
+            for (int k = 0; k < 8; ++k)
+                if (h[24 + k] && h[16])
+                    fprintf(stderr, "fastiso rows mark %d %.3f us (per rows phase)\n", k, h[24 + k] / 100.0 / h[16]);
+        }
+    }
     hipStream_t s = c->stream;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -697,7 +725,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     HIPCHECK(c, hipMemset(c->K.p, 0, 4 * N * sizeof(float)));
     HIPCHECK(c, c->kcorner.alloc(16));
     HIPCHECK(c, hipMemset(c->kcorner.p, 0, 16 * sizeof(float)));
-    HIPCHECK(c, c->spec.alloc((size_t)c->nhx * ny));
+    HIPCHECK(c, c->spec.alloc(2 * (size_t)c->nhx * ny));
     if (upload(c, c->H_af_ref_d, to_f(c->H_af_ref)) || upload(c, c->u_ref_d, to_f(c->u_ref)) ||
         upload(c, c->z_b_ref_d, to_f(c->z_b_ref)) || upload(c, c->z_ss_ref_d, to_f(c->z_ss_ref)) ||
         upload(c, c->ue_ref_d, to_f(c->ue_ref)) || upload(c, c->H_ext, to_f(c->H_ext_host.empty() ? c->H_ice_ref : c->H_ext_host)) ||
@@ -824,7 +852,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.Ub = c->U.p;
     kp.Kb = c->K.p;
     kp.kcorner = c->kcorner.p;
-    kp.spec = c->spec.p;
+    kp.specb = c->spec.p;
+    kp.specN = (long)c->nhx * ny;
     kp.mult = c->mult.p;
     kp.D = c->Df.p;
     kp.inv2eta = c->inv2eta.p;
@@ -863,12 +892,15 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.tw32 = c->tw32.p;
     kp.spin_limit = 2000000000ull;  // 20 s of the 100 MHz wall clock
 
-    // LDS: rows phase (u FP64 6 rows, 3 moment fields 4 rows, FFT pair) vs the column phases
-    size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16);
-    size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)2 * c->nfx * 8, (size_t)4 * c->nfy * 8});
-    c->mt_double = rows_d <= 150 * 1024;
-    c->lds = std::max(c->mt_double ? rows_d : rows_f, other);
-    if (c->lds > 160 * 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
+    // LDS: twiddles of nx and ny for the whole kernel, then the largest phase scratch: rows phase
+    // (u FP64 6 rows, 3 moment fields 4 rows, R2C pair, 6 rows of the spectral-stage derivative)
+    // or the padded-grid phases (their twiddles staged in the scratch)
+    const size_t twb = (size_t)(nx + ny) * 8;
+    size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16 + 24), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16 + 24);
+    size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)3 * c->nfx * 8, (size_t)5 * c->nfy * 8});
+    c->mt_double = nx <= 512 && twb + rows_d <= 160 * 1024;   // FP64 moments, register prefetch sized for nx <= 512
+    c->lds = twb + std::max(c->mt_double ? rows_d : rows_f, other);
+    if (c->lds > 160 * 1024 || nx > 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
     const void* f = c->mt_double ? (const void*)fi_persistent<double> : (const void*)fi_persistent<float>;
     HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
     int per_cu = 0, ncu = 0;
@@ -883,10 +915,16 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.part = c->part.p;
     kp.vpart = c->vpart.p;
     HIPCHECK(c, c->ctl_d.alloc(1));
-    HIPCHECK(c, c->bar_d.alloc(1));
+    HIPCHECK(c, c->bar_d.alloc(1 + 2 * (size_t)c->G));  // words + 128-byte arrival and release slots per workgroup
     HIPCHECK(c, c->kp_d.alloc(1));
     kp.ctl = c->ctl_d.p;
     kp.bar = c->bar_d.p;
+    kp.ptime = nullptr;
+    if (getenv("FASTISO_PHASE_TIMING")) {
+        HIPCHECK(c, c->ptime.alloc(32));
+        HIPCHECK(c, hipMemset(c->ptime.p, 0, 32 * sizeof(unsigned long long)));
+        kp.ptime = c->ptime.p;
+    }
 
     std::memset(&c->ctl, 0, sizeof(Ctl));
     c->ctl.t0 = t0;

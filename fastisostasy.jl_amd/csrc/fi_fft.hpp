@@ -169,10 +169,14 @@ __device__ __forceinline__ void fft_pass(const C* in, C* out, int L, int nb, int
     const int m = L / R;
     const int total = nb * m;
     const int tstride = L / (Ns * R);
+    // exact quotients of small non-negative integers by float reciprocal: (x + 0.5) / d has a
+    // fractional part >= 0.5/d, far above the float rounding error for x, d < 2^20
+    const float inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
     for (int b = threadIdx.x; b < total; b += blockDim.x) {
-        const int s = b / m;
+        const int s = (nb == 1) ? 0 : (int)(((float)b + 0.5f) * inv_m);
         const int j = b - s * m;
-        const int k = j % Ns;
+        const int jq = (int)(((float)j + 0.5f) * inv_ns);
+        const int k = j - jq * Ns;
         const C* ip = in + s * seq;
         C v[R];
 #pragma unroll
@@ -186,18 +190,18 @@ __device__ __forceinline__ void fft_pass(const C* in, C* out, int L, int nb, int
             }
         }
         Dft<C, R>::run(v, sign);
-        C* op = out + s * seq + (j / Ns) * Ns * R + k;
+        C* op = out + s * seq + jq * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) op[r * Ns] = v[r];
     }
 }
 
 // Transform nb sequences held in buffer a (seq elements apart); b is scratch of the same size.
-// Unnormalised; sign -1 forward, +1 inverse. Returns the buffer that holds the result.
-// Caller must have synchronised the workgroup before the call; the result is synchronised.
+// tw is this length's twiddle table (in LDS for the run-time transforms). Unnormalised; sign -1
+// forward, +1 inverse. Returns the buffer that holds the result. The caller synchronises the
+// workgroup before the call; the result is synchronised.
 template <typename C>
-__device__ C* fft_lds(C* a, C* b, int nb, int seq, const FftPlan& pl, const C* __restrict__ twtab, int sign) {
-    const C* tw = twtab + pl.tw;
+__device__ __forceinline__ C* fft_lds(C* a, C* b, int nb, int seq, const FftPlan& pl, const C* __restrict__ tw, int sign) {
     const int L = pl.L;
     C* in = a;
     C* out = b;

@@ -1,15 +1,19 @@
 // Phases of the LV-ELVA right-hand side and the persistent integration kernel (gfx950).
 //
 // One RHS evaluation = update_diagnostics!(dudt, u, sim, t) (src/simulation.jl:335-393) runs as
-// three grid-wide phases separated by grid barriers:
-//   rows  : stage displacement (BS3 combination + viscous corner BC, FP64), column loads,
-//           the two-sweep plate stencil (src/deformation.jl:110-131, src/derivatives.jl CPU edge
-//           rules) on LDS rows with +-2 halos, rhs * K/(2 eta), and a length-nx FFT of row pairs
-//           packed as one complex sequence; the half spectrum is stored transposed [kx][j];
+// grid-wide phases separated by grid barriers:
+//   rows  : [inverse real transform of the previous stage's spectrum for the row pairs covering
+//           this block and its +-2-row halo] stage displacement (BS3 combination + viscous corner
+//           BC, FP64), column loads, the two-sweep plate stencil (src/deformation.jl:110-131,
+//           src/derivatives.jl CPU edge rules) on LDS rows, rhs * K/(2 eta), and a length-nx FFT
+//           of the row pair packed as one complex sequence; the half spectrum is stored
+//           transposed [kx][j];
 //   cols  : length-ny FFT of each spectral column, multiplication by spy/(nx ny) / (R kappa),
 //           inverse FFT (src/deformation.jl:95-98);
-//   c2r   : inverse real transform of row pairs -> dudt (m/yr) before the corner BC; the BC
-//           offset (mean of the four raw corners) is published and applied by consumers.
+//   c2r   : standalone inverse real transform (last BS3 stage, with the error moments, and
+//           single-RHS evaluations) -> dudt (m/yr) before the corner BC; the BC offset (mean of
+//           the four raw corners) is published and applied by consumers.
+// A BS3 step therefore costs 7 grid barriers: rows, cols, rows+c2r, cols, rows+c2r, cols, c2r.
 // When the sparse trigger fires (src/simulation.jl:355-381) four more phases compute the elastic
 // and sea-surface convolutions on the zero-padded nfx x nfy grid (both in one pass) and the
 // pointwise sparse diagnostics.
@@ -19,58 +23,90 @@
 namespace fi {
 
 // ------------------------------------------------------------------------------------------
-// grid barrier (all workgroups co-resident: cooperative launch). Bounded wait: a barrier that
-// does not complete within spin_limit wall-clock ticks raises the abort flag and every
+// grid barrier (all workgroups co-resident: cooperative launch). Every workgroup publishes its
+// arrival in its own 128-byte slot with an agent-scope release store; workgroup 0 polls all slots
+// in parallel (one thread per slot) and then publishes the new generation, which the others poll.
+// Depth is independent of the number of workgroups (no serialised atomics on one address).
+// Waits are bounded: after spin_limit wall-clock ticks the abort flag is raised and every
 // workgroup leaves the kernel.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool grid_sync(const KP& p, unsigned& gen, int* s_flag) {
+__device__ __forceinline__ bool grid_sync(const KP& p, unsigned& gen, int* s_flag, int tag,
+                                          unsigned long long& tlast) {
+    Bar* bar = p.bar;
+    unsigned* arrive = (unsigned*)(bar + 1);
+    unsigned* release = arrive + (long)gridDim.x * BAR_SLOT;
+    const unsigned g = gen + 1u;
+    gen = g;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Bar* bar = p.bar;
-        __threadfence();
-        const unsigned g = gen;
-        const unsigned arrived = atomicAdd(&bar->count, 1u);
-        if (arrived == gridDim.x - 1) {
-            atomicExch(&bar->count, 0u);
-            __threadfence();
-            atomicExch(&bar->gen, g + 1u);
-        } else {
+    unsigned long long tin = 0;
+    if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) tin = wall_clock64();
+    if (blockIdx.x == 0) {
+        for (int w = 1 + (int)threadIdx.x; w < (int)gridDim.x; w += NT) {
             const unsigned long long t0 = wall_clock64();
-            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+            while (__hip_atomic_load(&arrive[w * BAR_SLOT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) {
                 if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
                 if (wall_clock64() - t0 > p.spin_limit) {
-                    atomicExch(&bar->abort, 1);
+                    __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();
-        gen = g + 1u;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        // publish the generation into every workgroup's own release slot
+        for (int w = 1 + (int)threadIdx.x; w < (int)gridDim.x; w += NT)
+            __hip_atomic_store(&release[w * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        __hip_atomic_store(&arrive[blockIdx.x * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long t0 = wall_clock64();
+        unsigned* mine = &release[blockIdx.x * BAR_SLOT];
+        while (__hip_atomic_load(mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) {
+            if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (wall_clock64() - t0 > p.spin_limit) {
+                __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
+    if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long tout = wall_clock64();
+        p.ptime[tag] += tin - tlast;
+        p.ptime[8 + tag] += tout - tin;
+        p.ptime[16 + tag] += 1;
+        tlast = tout;
+    }
     return *s_flag == 0;
 }
 
-// Block-wide sum of three doubles per thread (fixed order -> identical in every workgroup).
-__device__ __forceinline__ void block_sum3(double* red, double a, double b, double c, double out[3]) {
-    const int t = threadIdx.x;
-    red[t] = a;
-    red[NT + t] = b;
-    red[2 * NT + t] = c;
-    __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            red[t] += red[t + s];
-            red[NT + t] += red[NT + t + s];
-            red[2 * NT + t] += red[2 * NT + t + s];
-        }
-        __syncthreads();
+// Block-wide sums of M doubles per thread: wave64 butterfly, then the waves in fixed order
+// (identical rounding in every workgroup). red holds at least M * NT/64 doubles.
+template <int M>
+__device__ __forceinline__ void block_sum(double* red, double v[M]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        v[q] = x;
     }
-    out[0] = red[0];
-    out[1] = red[NT];
-    out[2] = red[2 * NT];
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < M; ++q) red[w * M + q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        double s = 0.0;
+        for (int ww = 0; ww < NT / 64; ++ww) s += red[ww * M + q];
+        v[q] = s;
+    }
     __syncthreads();
 }
 
@@ -79,12 +115,15 @@ __device__ __forceinline__ void block_sum3(double* red, double a, double b, doub
 // ------------------------------------------------------------------------------------------
 struct StageDesc {
     int ubase;         // U buffer of the base state
-    int nk;            // number of derivative terms (0, 1 or 3)
-    int slot0, slot1, slot2;
+    int ng;            // derivative terms read from global memory (0..3)
+    int g0, g1, g2;
     double a0, a1, a2;
+    int kin;           // K slot still held as a spectrum (inverse-transformed in the rows phase), or -1
+    double ain;
     double dt;
     int write_u;       // U buffer receiving the corner-corrected stage displacement, or -1
-    int kout;          // K slot receiving this call's derivative
+    int kout;          // K slot of this call's derivative
+    int do_c2r;        // this call ends with the standalone inverse transform
     double t;          // call time
     double t_next_max; // upper bound of the next call's time (for the stale-load store)
     int trig;          // set by rhs_call: this call ran the sparse update
@@ -92,6 +131,15 @@ struct StageDesc {
     int e1, e2, e3;    // K slots of k1, k2, k3 for the error moments
     int u_prev;        // U buffer of u_n for the error moments
     int write_sigma;   // set by rhs_call: store this call's loads for a later sparse update
+    int sb_in;         // spectrum buffer holding the pending derivative (kin)
+    int sb_out;        // spectrum buffer this call writes
+};
+
+struct Acc3 {
+    double a, b, c;
+};
+struct KC4 {
+    float v0, v1, v2, v3;  // (0,0), (nx-1,0), (0,ny-1), (nx-1,ny-1)
 };
 
 struct IceT {
@@ -156,21 +204,36 @@ __device__ __forceinline__ double hab(double H, double zd, double rsi) {
 }
 
 __device__ __forceinline__ const float* kslot(const KP& p, int k) { return p.Kb + (long)k * p.Nl; }
+__device__ __forceinline__ float2* specbuf(const KP& p, int b) { return p.specb + (long)b * p.specN; }
 __device__ __forceinline__ double* ubuf(const KP& p, int b) { return p.Ub + (long)b * p.Nl; }
 
+// sum of the global derivative terms of a stage at cell c
+__device__ __forceinline__ double kterms(const KP& p, const StageDesc& st, long c) {
+    double v = 0.0;
+    if (st.ng > 0) v = st.a0 * (double)kslot(p, st.g0)[c];
+    if (st.ng > 1) v += st.a1 * (double)kslot(p, st.g1)[c];
+    if (st.ng > 2) v += st.a2 * (double)kslot(p, st.g2)[c];
+    return v;
+}
+// stage displacement before the BC when every term is in global memory
 __device__ __forceinline__ double u_raw(const KP& p, const StageDesc& st, long c) {
     const double u0 = ubuf(p, st.ubase)[c];
-    if (st.nk == 0) return u0;
-    double v = st.a0 * (double)kslot(p, st.slot0)[c];
-    if (st.nk == 3) v += st.a1 * (double)kslot(p, st.slot1)[c] + st.a2 * (double)kslot(p, st.slot2)[c];
-    return u0 + st.dt * v;
+    if (st.ng == 0) return u0;
+    return u0 + st.dt * kterms(p, st, c);
 }
 
-// apply_bc!(u, CornerBC) offset of the stage displacement (src/boundary_conditions.jl:144-148)
-__device__ __forceinline__ double u_offset(const KP& p, const StageDesc& st) {
+// apply_bc!(u, CornerBC) offset of the stage displacement (src/boundary_conditions.jl:144-148);
+// kc holds the four corner values of the spectral term when st.kin >= 0
+__device__ __forceinline__ double u_corner(const KP& p, const StageDesc& st, long c, float kcv) {
+    const double u0 = ubuf(p, st.ubase)[c];
+    double v = kterms(p, st, c);
+    if (st.kin >= 0) v += st.ain * (double)kcv;
+    return (st.ng == 0 && st.kin < 0) ? u0 : u0 + st.dt * v;
+}
+__device__ __forceinline__ double u_offset(const KP& p, const StageDesc& st, const KC4 kc) {
     const long n1 = p.nx - 1, n2 = (long)(p.ny - 1) * p.nx, n3 = n2 + p.nx - 1;
-    const double a = 0.25 * u_raw(p, st, 0), b = 0.25 * u_raw(p, st, n1);
-    const double c = 0.25 * u_raw(p, st, n2), d = 0.25 * u_raw(p, st, n3);
+    const double a = 0.25 * u_corner(p, st, 0, kc.v0), b = 0.25 * u_corner(p, st, n1, kc.v1);
+    const double c = 0.25 * u_corner(p, st, n2, kc.v2), d = 0.25 * u_corner(p, st, n3, kc.v3);
     return ((a + b) + (c + d)) - p.vbc_x;
 }
 
@@ -179,6 +242,45 @@ __device__ __forceinline__ double k_offset(const KP& p, int slot) {
     const float* q = p.kcorner + 4 * slot;
     const double a = 0.25 * (double)q[0], b = 0.25 * (double)q[1], c = 0.25 * (double)q[2], d = 0.25 * (double)q[3];
     return ((a + b) + (c + d)) - p.vbc_x;
+}
+
+// The four corner values of the inverse real transform of the spectrum, by direct sums over
+// spectral rows 0 and ny-1: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx})
+// (w = 1 for the DC and Nyquist terms, 2 otherwise). Every workgroup gets the same values.
+__device__ __forceinline__ KC4 spectral_corners(const KP& p, const float2* spec, const float2* twx, double* red) {
+    const int nx = p.nx, ny = p.ny;
+    const bool even = (nx % 2) == 0;
+    double v[4] = {0, 0, 0, 0};
+    for (int kx = threadIdx.x; kx < p.nhx; kx += NT) {
+        const float2 a = spec[(long)kx * ny], b = spec[(long)kx * ny + ny - 1];
+        const bool nyq = even && kx == nx / 2;
+        const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
+        v[0] += w * (double)a.x;
+        v[2] += w * (double)b.x;
+        if (nyq) {
+            v[1] -= (double)a.x;
+            v[3] -= (double)b.x;
+        } else {
+            const float2 t = twx[kx];
+            v[1] += w * ((double)a.x * t.x - (double)a.y * t.y);
+            v[3] += w * ((double)b.x * t.x - (double)b.y * t.y);
+        }
+    }
+    block_sum<4>(red, v);
+    KC4 kc;
+    kc.v0 = (float)v[0];
+    kc.v1 = (float)v[1];
+    kc.v2 = (float)v[2];
+    kc.v3 = (float)v[3];
+    return kc;
+}
+
+__device__ __forceinline__ int corner_index(const KP& p, int i, int j) {
+    if (j == 0 && i == 0) return 0;
+    if (j == 0 && i == p.nx - 1) return 1;
+    if (j == p.ny - 1 && i == 0) return 2;
+    if (j == p.ny - 1 && i == p.nx - 1) return 3;
+    return -1;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -213,78 +315,228 @@ __device__ __forceinline__ double dxy(const V* F, int r0, int j, int i, int nx, 
     return (dxr(j + 1) - dxr(j - 1)) * (0.5 * invy);
 }
 
+// Hermitian-extended pair value Z[k] = Aext[k] + i Bext[k] of spectral rows ja, jb (jb may be a
+// phantom row) for the inverse real transform
+__device__ __forceinline__ float2 c2r_pair_value(const KP& p, const float2* spec, int k, int ja, int jb) {
+    const int nx = p.nx, ny = p.ny;
+    float2 A, B;
+    const int kk = (k < p.nhx) ? k : nx - k;
+    A = spec[(long)kk * ny + ja];
+    if (jb < ny) B = spec[(long)kk * ny + jb];
+    else { B.x = 0; B.y = 0; }
+    if (k >= p.nhx) { A.y = -A.y; B.y = -B.y; }
+    else if (k == 0 || ((nx % 2) == 0 && k == nx / 2)) { A.y = 0; B.y = 0; }
+    float2 Z;
+    Z.x = A.x - B.y;
+    Z.y = A.y + B.x;
+    return Z;
+}
+
+// Sub-phase wall-clock marks of workgroup 0 inside the rows phase, compiled in only with
+// -DFI_SUBPHASE_TIMING (each mark synchronises the workgroup, so timings there are upper bounds).
+#ifdef FI_SUBPHASE_TIMING
+#define FI_SUBMARK(k)                                                                   \
+    do {                                                                                \
+        __syncthreads();                                                                \
+        if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) {                           \
+            const unsigned long long t_ = wall_clock64();                               \
+            p.ptime[24 + (k)] += t_ - tsub;                                             \
+            tsub = t_;                                                                  \
+        }                                                                               \
+    } while (0)
+#else
+#define FI_SUBMARK(k) \
+    do {              \
+    } while (0)
+#endif
+
 // ------------------------------------------------------------------------------------------
 // rows phase
 // ------------------------------------------------------------------------------------------
-template <typename MT>
-__device__ void phase_rows(const KP& p, const StageDesc& st, double uoff, const IceT& it, char* smem) {
+// exact x / d for small non-negative integers (fractional part of (x+0.5)/d >= 0.5/d)
+__device__ __forceinline__ int fdivi(int x, float inv_d) { return (int)(((float)x + 0.5f) * inv_d); }
+
+// Per-cell fields of the load update, prefetched into registers at the start of a row block.
+struct CellF {
+    float H, zo, hr, lt, lw, i2e, m;
+};
+
+template <typename MT, int QD, int QF>
+__device__ __forceinline__ void phase_rows(const KP& p, const StageDesc& st, double uoff, const IceT it, const float2* twx,
+                           char* smem, const KC4 kc) {
     const int nx = p.nx, ny = p.ny;
-    double* su = (double*)smem;
-    MT* sMxx = (MT*)(su + 6 * nx);
+    const float inv_nx = 1.0f / (float)nx;
+    double* su = (double*)smem;                 // 6 rows of the stage displacement
+    MT* sMxx = (MT*)(su + 6 * nx);              // 4 rows each of the plate moments
     MT* sMyy = sMxx + 4 * nx;
     MT* sMxy = sMyy + 4 * nx;
-    float2* sf = (float2*)(sMxy + 4 * nx);
+    float2* sf = (float2*)(sMxy + 4 * nx);      // R2C pair + scratch
     float2* sg = sf + nx;
+    float* skin = (float*)(sg + nx);            // 6 rows of the spectral-stage derivative
+    // the inverse transforms of the halo pairs use the moment region as scratch
+    float2* ca = (float2*)sMxx;
+    const float2* spin = specbuf(p, st.sb_in);
+    float2* spout = specbuf(p, st.sb_out);
+    const int maxpairs = (int)((12 * nx * sizeof(MT)) / (16 * nx));
     const double nu = p.nu;
     const bool wsig = st.write_sigma != 0;
+    const double* U0 = ubuf(p, st.ubase);
     for (int item = blockIdx.x; item < p.nrowitems; item += gridDim.x) {
         const int j0 = 2 * item, je = min(j0 + 2, ny);
         const int jm0 = max(0, min(j0 - 1, ny - 3)), jm1 = min(ny, max(je + 1, 3));
         const int ju0 = max(0, min(jm0 - 1, ny - 3)), ju1 = min(ny, max(jm1 + 1, 3));
-        // stage displacement with the corner BC applied
-        for (int idx = threadIdx.x; idx < (ju1 - ju0) * nx; idx += NT) {
-            const int r = idx / nx, i = idx - r * nx, j = ju0 + r;
-            const long c = i + (long)j * nx;
-            const double u = u_raw(p, st, c) - uoff;
-            su[idx] = u;
-            if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[c] = u;
+        const int totU = (ju1 - ju0) * nx, totM = (jm1 - jm0) * nx, totO = 2 * nx;
+#ifdef FI_SUBPHASE_TIMING
+        unsigned long long tsub = wall_clock64();
+#endif
+        // stage terms held in global memory (base state + derivative slots) -> su
+        for (int base = 0; base < totU; base += 4 * NT) {
+            double uv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = min(base + (int)threadIdx.x + q * NT, totU - 1);
+                const int r = fdivi(idx, inv_nx), i = idx - r * nx;
+                const long c = i + (long)(ju0 + r) * nx;
+                uv[q] = (st.ng == 0) ? U0[c] : U0[c] + st.dt * kterms(p, st, c);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int idx = base + (int)threadIdx.x + q * NT;
+                if (idx < totU) su[idx] = uv[q];
+            }
         }
-        __syncthreads();
-        // plate moments M = -D (...) on rows [jm0, jm1)
-        for (int idx = threadIdx.x; idx < (jm1 - jm0) * nx; idx += NT) {
-            const int r = idx / nx, i = idx - r * nx, j = jm0 + r;
-            const long c = i + (long)j * nx;
-            const double uxx = d2x(su + (j - ju0) * nx, i, nx, p.inv_dx2);
-            const double uyy = d2y(su, ju0, j, i, nx, ny, p.inv_dy2);
-            const double uxy = dxy(su, ju0, j, i, nx, ny, p.inv_dx, p.inv_dy);
-            const double D = (double)p.D[c];
-            sMxx[idx] = (MT)(-D * (nu * uyy + uxx));
-            sMyy[idx] = (MT)(-D * (nu * uxx + uyy));
-            sMxy[idx] = (MT)(-D * (1.0 - nu) * uxy);
-        }
-        __syncthreads();
-        // rhs = -g sigma_full + dxx Mxx + 2 dxy Mxy + dyy Myy, times K/(2 eta)
-        for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
-            const int r = idx / nx, i = idx - r * nx, j = j0 + r;
-            float f = 0.0f;
-            if (j < ny) {
-                const long c = i + (long)j * nx;
-                const double bxx = d2x(sMxx + (j - jm0) * nx, i, nx, p.inv_dx2);
-                const double byy = d2y(sMyy, jm0, j, i, nx, ny, p.inv_dy2);
-                const double bxy = dxy(sMxy, jm0, j, i, nx, ny, p.inv_dx, p.inv_dy);
-                const double u = su[(j - ju0) * nx + i];
-                const double H = ice_at(p, it, c);
-                const double zd = (double)(st.trig ? p.zoff_trig[c] : p.zoff[c]) + u;
-                const double Haf = hab(H, zd, p.rho_sw_ice);
-                const double s_ice = p.rho_ice * (Haf - (double)p.H_af_ref[c]);
-                const double m = p.mask[c] ? 1.0 : 0.0;
-                const double s_load = m * (s_ice + (p.interactive ? (double)p.LW[c] : 0.0));
-                const double s_mant = p.rho_um * u;
-                const double s_full = s_load + m * ((double)p.LT[c] + s_mant);
-                const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
-                f = (float)(rhs * (double)p.inv2eta[c]);
-                if (wsig) {
-                    p.sig_load_prev[c] = (float)s_load;
-                    p.sig_geo_prev[c] = (float)(p.compressible_column
-                                                    ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
-                                                    : s_full);
+        FI_SUBMARK(0);
+        // previous stage's derivative on rows [ju0, ju1): inverse real transforms of row pairs
+        if (st.kin >= 0) {
+            float* Kin = p.Kb + (long)st.kin * p.Nl;
+            const int p0 = ju0 & ~1;
+            const int np = (ju1 - p0 + 1) / 2;
+            for (int q0 = 0; q0 < np; q0 += maxpairs) {
+                const int nb = min(maxpairs, np - q0);
+                for (int b0 = 0; b0 < nb * nx; b0 += 4 * NT) {
+                    float2 zv[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int idx = min(b0 + (int)threadIdx.x + q * NT, nb * nx - 1);
+                        const int b = fdivi(idx, inv_nx), k = idx - b * nx;
+                        const int ja = p0 + 2 * (q0 + b);
+                        zv[q] = c2r_pair_value(p, spin, k, ja, ja + 1);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int idx = b0 + (int)threadIdx.x + q * NT;
+                        if (idx < nb * nx) ca[idx] = zv[q];
+                    }
+                }
+                __syncthreads();
+                FI_SUBMARK(1);
+                float2* res = fft_lds<float2>(ca, ca + nb * nx, nb, nx, p.px, twx, +1);
+                FI_SUBMARK(2);
+                for (int idx = threadIdx.x; idx < 2 * nb * nx; idx += NT) {
+                    const int b = fdivi(idx, 0.5f * inv_nx), rem = idx - b * 2 * nx, r = rem >= nx ? 1 : 0,
+                              i = rem - r * nx;
+                    const int j = p0 + 2 * (q0 + b) + r;
+                    if (j < ju0 || j >= ju1) continue;
+                    float v = r ? res[b * nx + i].y : res[b * nx + i].x;
+                    const int ci = corner_index(p, i, j);
+                    if (ci >= 0) v = ci == 0 ? kc.v0 : ci == 1 ? kc.v1 : ci == 2 ? kc.v2 : kc.v3;
+                    skin[(j - ju0) * nx + i] = v;
+                    if (j >= j0 && j < je) {
+                        Kin[i + (long)j * nx] = v;
+                        if (ci >= 0) p.kcorner[4 * st.kin + ci] = v;
+                    }
                 }
             }
-            if (r == 0) sf[i].x = f;
-            else sf[i].y = f;
+        }
+        FI_SUBMARK(3);
+        // ---- plate rigidity and per-cell fields: loads in flight during the LDS-only work below ----
+        float Dr[QD];
+        CellF fr[QF];
+#pragma unroll
+        for (int q = 0; q < QD; ++q) {
+            const int idx = min((int)threadIdx.x + q * NT, totM - 1);
+            const int r = fdivi(idx, inv_nx), i = idx - r * nx;
+            Dr[q] = p.D[i + (long)(jm0 + r) * nx];
+        }
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            const int r = fdivi(min(idx, totO - 1), inv_nx), i = min(idx, totO - 1) - r * nx;
+            const int j = min(j0 + r, ny - 1);
+            const long c = i + (long)j * nx;
+            fr[q].H = (float)ice_at(p, it, c);
+            fr[q].zo = st.trig ? p.zoff_trig[c] : p.zoff[c];
+            fr[q].hr = p.H_af_ref[c];
+            fr[q].lt = p.LT[c];
+            fr[q].lw = p.interactive ? p.LW[c] : 0.0f;
+            fr[q].i2e = p.inv2eta[c];
+            fr[q].m = p.mask[c] ? 1.0f : 0.0f;
         }
         __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, p.tw32, -1);
+        // finish the stage displacement: spectral term, corner BC; keep u_{n+1} when asked
+        for (int idx = threadIdx.x; idx < totU; idx += NT) {
+            double u = su[idx];
+            if (st.kin >= 0) u += st.dt * (st.ain * (double)skin[idx]);
+            u -= uoff;
+            su[idx] = u;
+            const int r = fdivi(idx, inv_nx);
+            const int j = ju0 + r;
+            if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[idx - r * nx + (long)j * nx] = u;
+        }
+        __syncthreads();
+        FI_SUBMARK(4);
+        // plate moments M = -D (...) on rows [jm0, jm1)
+#pragma unroll
+        for (int q = 0; q < QD; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            if (idx < totM) {
+                const int r = fdivi(idx, inv_nx), i = idx - r * nx, j = jm0 + r;
+                const double D = (double)Dr[q];
+                const double uxx = d2x(su + (j - ju0) * nx, i, nx, p.inv_dx2);
+                const double uyy = d2y(su, ju0, j, i, nx, ny, p.inv_dy2);
+                const double uxy = dxy(su, ju0, j, i, nx, ny, p.inv_dx, p.inv_dy);
+                sMxx[idx] = (MT)(-D * (nu * uyy + uxx));
+                sMyy[idx] = (MT)(-D * (nu * uxx + uyy));
+                sMxy[idx] = (MT)(-D * (1.0 - nu) * uxy);
+            }
+        }
+        __syncthreads();
+        FI_SUBMARK(5);
+        // rhs = -g sigma_full + dxx Mxx + 2 dxy Mxy + dyy Myy, times K/(2 eta)
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            if (idx < totO) {
+                const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+                float f = 0.0f;
+                if (j < ny) {
+                    const long c = i + (long)j * nx;
+                    const double bxx = d2x(sMxx + (j - jm0) * nx, i, nx, p.inv_dx2);
+                    const double byy = d2y(sMyy, jm0, j, i, nx, ny, p.inv_dy2);
+                    const double bxy = dxy(sMxy, jm0, j, i, nx, ny, p.inv_dx, p.inv_dy);
+                    const double u = su[(j - ju0) * nx + i];
+                    const double m = fr[q].m;
+                    const double Haf = hab((double)fr[q].H, (double)fr[q].zo + u, p.rho_sw_ice);
+                    const double s_ice = p.rho_ice * (Haf - (double)fr[q].hr);
+                    const double s_load = m * (s_ice + (double)fr[q].lw);
+                    const double s_mant = p.rho_um * u;
+                    const double s_full = s_load + m * ((double)fr[q].lt + s_mant);
+                    const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
+                    f = (float)(rhs * (double)fr[q].i2e);
+                    if (wsig) {
+                        p.sig_load_prev[c] = (float)s_load;
+                        p.sig_geo_prev[c] = (float)(p.compressible_column
+                                                        ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
+                                                        : s_full);
+                    }
+                }
+                if (r == 0) sf[i].x = f;
+                else sf[i].y = f;
+            }
+        }
+        __syncthreads();
+        FI_SUBMARK(6);
+        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, -1);
         for (int kx = threadIdx.x; kx < p.nhx; kx += NT) {
             const float2 Z = res[kx], Zm = res[kx == 0 ? 0 : nx - kx];
             float2 X0, X1;
@@ -292,17 +544,18 @@ __device__ void phase_rows(const KP& p, const StageDesc& st, double uoff, const 
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            p.spec[(long)kx * ny + j0] = X0;
-            if (j0 + 1 < ny) p.spec[(long)kx * ny + j0 + 1] = X1;
+            spout[(long)kx * ny + j0] = X0;
+            if (j0 + 1 < ny) spout[(long)kx * ny + j0 + 1] = X1;
         }
         __syncthreads();
+        FI_SUBMARK(7);
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // columns phase: FFT_y, * spy/(nx ny)/(R kappa), inverse FFT_y
 // ------------------------------------------------------------------------------------------
-__device__ void phase_cols(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_cols(const KP& p, float2* spec, const float2* twy, char* smem) {
     const int ny = p.ny, nb = p.colnb;
     float2* sa = (float2*)smem;
     float2* sb = sa + nb * ny;
@@ -310,9 +563,9 @@ __device__ void phase_cols(const KP& p, char* smem) {
         const int kx0 = item * nb;
         const int nbb = min(nb, p.nhx - kx0);
         const long base = (long)kx0 * ny;
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) sa[idx] = p.spec[base + idx];
+        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) sa[idx] = spec[base + idx];
         __syncthreads();
-        float2* r = fft_lds<float2>(sa, sb, nbb, ny, p.py, p.tw32, -1);
+        float2* r = fft_lds<float2>(sa, sb, nbb, ny, p.py, twy, -1);
         for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) {
             const float m = p.mult[base + idx];
             r[idx].x *= m;
@@ -320,77 +573,67 @@ __device__ void phase_cols(const KP& p, char* smem) {
         }
         __syncthreads();
         float2* o = (r == sa) ? sb : sa;
-        float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, p.tw32, +1);
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) p.spec[base + idx] = r2[idx];
+        float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
+        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) spec[base + idx] = r2[idx];
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// c2r phase: inverse real transform of row pairs; optional BS3 error moments
+// standalone c2r phase: inverse real transform of row pairs into K[slot]; optional BS3 error
+// moments
 // ------------------------------------------------------------------------------------------
-__device__ void phase_c2r(const KP& p, const StageDesc& st, char* smem, double acc[3]) {
-    const int nx = p.nx, ny = p.ny, nhx = p.nhx;
-    const bool even = (nx % 2) == 0;
+__device__ __forceinline__ Acc3 phase_c2r(const KP& p, const StageDesc& st, const float2* spec, int slot, const float2* twx,
+                          char* smem) {
+    Acc3 acc;
+    acc.a = acc.b = acc.c = 0.0;
+    const int nx = p.nx, ny = p.ny;
     float2* sf = (float2*)smem;
     float2* sg = sf + nx;
-    float* K = p.Kb + (long)st.kout * p.Nl;
+    float* K = p.Kb + (long)slot * p.Nl;
     for (int item = blockIdx.x; item < p.nrowitems; item += gridDim.x) {
         const int j0 = 2 * item;
-        const bool two = j0 + 1 < ny;
-        for (int k = threadIdx.x; k < nx; k += NT) {
-            float2 A, B;
-            if (k < nhx) {
-                A = p.spec[(long)k * ny + j0];
-                if (two) B = p.spec[(long)k * ny + j0 + 1];
-                else { B.x = 0; B.y = 0; }
-                if (k == 0 || (even && k == nx / 2)) { A.y = 0; B.y = 0; }
-            } else {
-                const int kk = nx - k;
-                A = p.spec[(long)kk * ny + j0];
-                if (two) B = p.spec[(long)kk * ny + j0 + 1];
-                else { B.x = 0; B.y = 0; }
-                A.y = -A.y;
-                B.y = -B.y;
-            }
-            float2 Z;
-            Z.x = A.x - B.y;
-            Z.y = A.y + B.x;
-            sf[k] = Z;
-        }
+        for (int k = threadIdx.x; k < nx; k += NT) sf[k] = c2r_pair_value(p, spec, k, j0, j0 + 1);
         __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, p.tw32, +1);
+        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, +1);
         for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
             const int r = idx / nx, i = idx - r * nx, j = j0 + r;
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
             const float v = r ? res[i].y : res[i].x;
             K[c] = v;
-            if ((j == 0 || j == ny - 1) && (i == 0 || i == nx - 1))
-                p.kcorner[4 * st.kout + (j == 0 ? 0 : 2) + (i == 0 ? 0 : 1)] = v;
+            const int ci = corner_index(p, i, j);
+            if (ci >= 0) p.kcorner[4 * slot + ci] = v;
             if (st.final_err) {
-                const double a = st.dt * (BS3_BT1 * (double)kslot(p, st.e1)[c] +
-                                          BS3_BT2 * (double)kslot(p, st.e2)[c] +
-                                          BS3_BT3 * (double)kslot(p, st.e3)[c] + BS3_BT4 * (double)v);
+                const double k1 = kslot(p, st.e1)[c], k2 = kslot(p, st.e2)[c], k3 = kslot(p, st.e3)[c];
                 const double un = ubuf(p, st.u_prev)[c], up = ubuf(p, st.write_u)[c];
+                const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)v);
                 const double w = 1.0 / (p.abstol + fmax(fabs(un), fabs(up)) * p.reltol);
-                acc[0] += (a * w) * (a * w);
-                acc[1] += a * w * w;
-                acc[2] += w * w;
+                acc.a += (a * w) * (a * w);
+                acc.b += a * w * w;
+                acc.c += w * w;
             }
         }
         __syncthreads();
     }
+    return acc;
 }
 
 // ------------------------------------------------------------------------------------------
 // sparse update phases (src/simulation.jl:359-381)
 // ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void stage_twiddles(const KP& p, const FftPlan& pl, float2* dst) {
+    for (int k = threadIdx.x; k < pl.L; k += NT) dst[k] = p.tw32[pl.tw + k];
+    __syncthreads();
+}
+
 // padded rows: the two convolution inputs of row j packed as one complex sequence
-__device__ void phase_sp_rows(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_sp_rows(const KP& p, char* smem) {
     const int nx = p.nx, ny = p.ny, nfx = p.nfx;
-    float2* sf = (float2*)smem;
+    float2* tw = (float2*)smem;
+    float2* sf = tw + nfx;
     float2* sg = sf + nfx;
+    stage_twiddles(p, p.pfx, tw);
     for (int j = blockIdx.x; j < ny; j += gridDim.x) {
         for (int i = threadIdx.x; i < nfx; i += NT) {
             float2 z;
@@ -404,7 +647,7 @@ __device__ void phase_sp_rows(const KP& p, char* smem) {
             sf[i] = z;
         }
         __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, p.tw32, -1);
+        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, -1);
         for (int kx = threadIdx.x; kx < p.nfhx; kx += NT) {
             const float2 Z = res[kx], Zm = res[kx == 0 ? 0 : nfx - kx];
             float2 A, B;
@@ -427,10 +670,12 @@ __device__ __forceinline__ int sp_row_of(const KP& p, int jj) {
 
 // padded columns: FFT_y of both half spectra, multiply by the FP64 Green spectra, inverse FFT_y,
 // keep the rows the crop and the extended-space corner BC need
-__device__ void phase_sp_cols(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_sp_cols(const KP& p, char* smem) {
     const int ny = p.ny, nfy = p.nfy, no = ny + 2;
-    float2* sa = (float2*)smem;
+    float2* tw = (float2*)smem;
+    float2* sa = tw + nfy;
     float2* sb = sa + 2 * nfy;
+    stage_twiddles(p, p.pfy, tw);
     for (int kx = blockIdx.x; kx < p.nfhx; kx += gridDim.x) {
         for (int idx = threadIdx.x; idx < 2 * nfy; idx += NT) {
             const int s = idx / nfy, j = idx - s * nfy;
@@ -441,7 +686,7 @@ __device__ void phase_sp_cols(const KP& p, char* smem) {
             sa[idx] = v;
         }
         __syncthreads();
-        float2* r = fft_lds<float2>(sa, sb, 2, nfy, p.pfy, p.tw32, -1);
+        float2* r = fft_lds<float2>(sa, sb, 2, nfy, p.pfy, tw, -1);
         for (int idx = threadIdx.x; idx < 2 * nfy; idx += NT) {
             const int s = idx / nfy, ky = idx - s * nfy;
             const double2 G = (s ? p.gg_hat : p.ge_hat)[(long)kx * nfy + ky];
@@ -451,7 +696,7 @@ __device__ void phase_sp_cols(const KP& p, char* smem) {
         }
         __syncthreads();
         float2* o = (r == sa) ? sb : sa;
-        float2* r2 = fft_lds<float2>(r, o, 2, nfy, p.pfy, p.tw32, +1);
+        float2* r2 = fft_lds<float2>(r, o, 2, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < 2 * no; idx += NT) {
             const int s = idx / no, jj = idx - s * no;
             (s ? p.oB : p.oA)[(long)kx * no + jj] = r2[s * nfy + sp_row_of(p, jj)];
@@ -461,11 +706,13 @@ __device__ void phase_sp_cols(const KP& p, char* smem) {
 }
 
 // needed padded rows: inverse real transforms, crop [i1:i2] and the extended corners
-__device__ void phase_sp_c2r(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_sp_c2r(const KP& p, char* smem) {
     const int nx = p.nx, nfx = p.nfx, nfhx = p.nfhx, no = p.ny + 2;
     const bool even = (nfx % 2) == 0;
-    float2* sf = (float2*)smem;
+    float2* tw = (float2*)smem;
+    float2* sf = tw + nfx;
     float2* sg = sf + nfx;
+    stage_twiddles(p, p.pfx, tw);
     for (int jj = blockIdx.x; jj < no; jj += gridDim.x) {
         for (int k = threadIdx.x; k < nfx; k += NT) {
             float2 A, B;
@@ -486,7 +733,7 @@ __device__ void phase_sp_c2r(const KP& p, char* smem) {
             sf[k] = Z;
         }
         __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, p.tw32, +1);
+        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, +1);
         if (jj == 0 || jj == no - 1) {
             if (threadIdx.x == 0) {
                 const int q = (jj == 0) ? 0 : 2;
@@ -508,7 +755,9 @@ __device__ void phase_sp_c2r(const KP& p, char* smem) {
 }
 
 // pointwise sparse diagnostics; volume partial sums for internal_update_bsl!
-__device__ void phase_sp_point(const KP& p, const StageDesc& st, double uoff, const IceT& it, double acc[3]) {
+__device__ __forceinline__ Acc3 phase_sp_point(const KP& p, const StageDesc& st, double uoff, const IceT it) {
+    Acc3 acc;
+    acc.a = acc.b = acc.c = 0.0;
     const long N = (long)p.nx * p.ny;
     const float* ec = p.ext_corners;
     const double oe = ((0.25 * ec[0] + 0.25 * ec[1]) + (0.25 * ec[2] + 0.25 * ec[3])) - p.ebc_x;
@@ -519,9 +768,9 @@ __device__ void phase_sp_point(const KP& p, const StageDesc& st, double uoff, co
         const double ue_old = p.ue[c], zss_old = p.z_ss[c];
         const double zb_old = (double)p.z_b_ref[c] + ue_old + u;
         const double haf_pre = hab(H, zb_old - zss_old, p.rho_sw_ice);
-        acc[0] += haf_pre * p.A;
-        acc[1] += H * p.A;
-        acc[2] += fmax(zss_old - zb_old, 0.0) * p.A;
+        acc.a += haf_pre * p.A;
+        acc.b += H * p.A;
+        acc.c += fmax(zss_old - zb_old, 0.0) * p.A;
         const double ue_new = p.rigid_litho ? 0.0 : (double)p.ue_raw[c] - oe;
         const double dz_new = p.lv_surface ? (double)p.dz_raw[c] - og : (double)p.dz_ss[c];
         const double zss_new = (double)p.z_ss_ref[c] + dz_new + p.bsl_z;
@@ -544,39 +793,58 @@ __device__ void phase_sp_point(const KP& p, const StageDesc& st, double uoff, co
         p.zoff[c] = (float)((double)p.z_b_ref[c] + ue_new - zss_new);
         p.zoff_trig[c] = (float)((double)p.z_b_ref[c] + ue_old - zss_new);
     }
+    return acc;
 }
 
 // ------------------------------------------------------------------------------------------
 // one RHS call
 // ------------------------------------------------------------------------------------------
 template <typename MT>
-__device__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, unsigned& gen, char* smem, double* red, int* flag,
-                         bool sparse_only) {
+__device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, unsigned& gen, const float2* twx, const float2* twy,
+                         char* smem, double* red, int* flag, bool sparse_only, unsigned long long& tl) {
+    st.sb_out = (int)(s.nrhs & 1);
+    st.sb_in = 1 - st.sb_out;   // the previous call's output
     const IceT it = ice_time(p, st.t);
     if (it.err) {
         if (threadIdx.x == 0) s.err = ERR_TIME_RANGE;
         __syncthreads();
         return false;
     }
-    const double uoff = u_offset(p, st);
+    KC4 kc;
+    kc.v0 = kc.v1 = kc.v2 = kc.v3 = 0.0f;
     const bool trig = sparse_only || (((st.t - s.t0) / p.dt_sparse) >= (double)s.count);
     st.trig = trig ? 1 : 0;
+    if (trig && st.kin >= 0) {
+        // the sparse update needs this call's displacement everywhere: materialise the pending
+        // derivative first and treat it as a global term
+        StageDesc sk = st;
+        sk.final_err = 0;
+        (void)phase_c2r(p, sk, specbuf(p, st.sb_in), st.kin, twx, smem);
+        if (!grid_sync(p, gen, flag, TAG_KIN, tl)) return false;
+        if (st.ng == 0) { st.g0 = st.kin; st.a0 = st.ain; }
+        else if (st.ng == 1) { st.g1 = st.kin; st.a1 = st.ain; }
+        else { st.g2 = st.kin; st.a2 = st.ain; }
+        st.ng += 1;
+        st.kin = -1;
+    }
+    if (st.kin >= 0) kc = spectral_corners(p, specbuf(p, st.sb_in), twx, red);
+    const double uoff = u_offset(p, st, kc);
     if (trig) {
         phase_sp_rows(p, smem);
-        if (!grid_sync(p, gen, flag)) return false;
+        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
         phase_sp_cols(p, smem);
-        if (!grid_sync(p, gen, flag)) return false;
+        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
         phase_sp_c2r(p, smem);
-        if (!grid_sync(p, gen, flag)) return false;
-        double acc[3] = {0, 0, 0}, tot[3];
-        phase_sp_point(p, st, uoff, it, acc);
-        block_sum3(red, acc[0], acc[1], acc[2], tot);
+        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
+        const Acc3 a3 = phase_sp_point(p, st, uoff, it);
+        double acc[3] = {a3.a, a3.b, a3.c};
+        block_sum<3>(red, acc);
         if (threadIdx.x == 0) {
-            p.vpart[4 * blockIdx.x + 0] = tot[0];
-            p.vpart[4 * blockIdx.x + 1] = tot[1];
-            p.vpart[4 * blockIdx.x + 2] = tot[2];
+            p.vpart[4 * blockIdx.x + 0] = acc[0];
+            p.vpart[4 * blockIdx.x + 1] = acc[1];
+            p.vpart[4 * blockIdx.x + 2] = acc[2];
         }
-        if (!grid_sync(p, gen, flag)) return false;
+        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
         if (threadIdx.x == 0) {
             double v0 = 0, v1 = 0, v2 = 0;
             for (int w = 0; w < (int)gridDim.x; ++w) {
@@ -598,22 +866,24 @@ __device__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, unsigned& gen, char
     // store this call's loads if the next call may trigger (the sparse update reads the
     // column anomalies of the previous call, src/simulation.jl:362-385)
     st.write_sigma = (p.write_sigma_always || (((st.t_next_max - s.t0) / p.dt_sparse) >= (double)s.count)) ? 1 : 0;
-    phase_rows<MT>(p, st, uoff, it, smem);
-    if (!grid_sync(p, gen, flag)) return false;
-    phase_cols(p, smem);
-    if (!grid_sync(p, gen, flag)) return false;
-    double acc[3] = {0, 0, 0};
-    phase_c2r(p, st, smem, acc);
-    if (st.final_err) {
-        double tot[3];
-        block_sum3(red, acc[0], acc[1], acc[2], tot);
-        if (threadIdx.x == 0) {
-            p.part[4 * blockIdx.x + 0] = tot[0];
-            p.part[4 * blockIdx.x + 1] = tot[1];
-            p.part[4 * blockIdx.x + 2] = tot[2];
+    if (sizeof(MT) == 8) phase_rows<MT, 4, 2>(p, st, uoff, it, twx, smem, kc);
+    else phase_rows<MT, 8, 4>(p, st, uoff, it, twx, smem, kc);
+    if (!grid_sync(p, gen, flag, TAG_ROWS, tl)) return false;
+    phase_cols(p, specbuf(p, st.sb_out), twy, smem);
+    if (!grid_sync(p, gen, flag, TAG_COLS, tl)) return false;
+    if (st.do_c2r) {
+        const Acc3 a3 = phase_c2r(p, st, specbuf(p, st.sb_out), st.kout, twx, smem);
+        if (st.final_err) {
+            double acc[3] = {a3.a, a3.b, a3.c};
+            block_sum<3>(red, acc);
+            if (threadIdx.x == 0) {
+                p.part[4 * blockIdx.x + 0] = acc[0];
+                p.part[4 * blockIdx.x + 1] = acc[1];
+                p.part[4 * blockIdx.x + 2] = acc[2];
+            }
         }
+        if (!grid_sync(p, gen, flag, TAG_C2R, tl)) return false;
     }
-    if (!grid_sync(p, gen, flag)) return false;
     if (threadIdx.x == 0) {
         s.nrhs += 1;
         s.t_last = st.t;

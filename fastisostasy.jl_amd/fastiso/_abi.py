@@ -116,7 +116,8 @@ def load_engine(path=None):
     global _ENGINE
     if path is None and _ENGINE is not None:
         return _ENGINE
-    p = path or ENGINE_PATH
+    # FASTISO_HIP_LIB selects another build of the same library (e.g. an instrumented one)
+    p = path or os.environ.get("FASTISO_HIP_LIB") or ENGINE_PATH
     if not os.path.exists(p):
         raise RuntimeError(
             f"HIP engine not built: {p} is missing. Build it with "
