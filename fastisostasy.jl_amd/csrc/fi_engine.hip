@@ -122,17 +122,23 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
         }
         if (!ok) break;
         ++attempts;
+#ifdef FI_SUBPHASE_TIMING
+        unsigned long long te0 = wall_clock64();
+#endif
         // error norm with the corner-BC offsets of k1..k4 folded in:
         // sum ((a - C) w)^2 = S2 - 2 C S1 + C^2 S0
+        // (one round of loads by the whole workgroup, fixed-order sum: identical in every workgroup)
+        const double C = dt * (BS3_BT1 * k_offset(p, k1) + BS3_BT2 * k_offset(p, s2) +
+                               BS3_BT3 * k_offset(p, s3) + BS3_BT4 * k_offset(p, s4));
+        double Sm[3] = {0.0, 0.0, 0.0};
+        for (int w = threadIdx.x; w < (int)gridDim.x; w += NT) {
+            Sm[0] += p.part[4 * w];
+            Sm[1] += p.part[4 * w + 1];
+            Sm[2] += p.part[4 * w + 2];
+        }
+        block_sum<3>(red, Sm);
         if (threadIdx.x == 0) {
-            double S2 = 0, S1 = 0, S0 = 0;
-            for (int w = 0; w < (int)gridDim.x; ++w) {
-                S2 += p.part[4 * w];
-                S1 += p.part[4 * w + 1];
-                S0 += p.part[4 * w + 2];
-            }
-            const double C = dt * (BS3_BT1 * k_offset(p, k1) + BS3_BT2 * k_offset(p, s2) +
-                                   BS3_BT3 * k_offset(p, s3) + BS3_BT4 * k_offset(p, s4));
+            const double S2 = Sm[0], S1 = Sm[1], S0 = Sm[2];
             const double e2 = (S2 - 2.0 * C * S1 + C * C * S0) / p.N;
             const double EEst = sqrt(fmax(e2, 0.0));
             s.EEst = EEst;
@@ -165,6 +171,9 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
             }
         }
         __syncthreads();
+#ifdef FI_SUBPHASE_TIMING
+        if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) p.ptime[22] += wall_clock64() - te0;
+#endif
         if (s.steps == 1 && s.t >= s.next_out) break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *p.ctl = s;
@@ -549,6 +558,8 @@ void fi_destroy(fi_ctx* c) {
                 if (h[16 + t])
                     fprintf(stderr, "fastiso phase %-8s n=%llu work %.3f us wait %.3f us (per phase)\n", names[t],
                             h[16 + t], h[t] / 100.0 / h[16 + t], h[8 + t] / 100.0 / h[16 + t]);
+            if (h[23] && h[16]) fprintf(stderr, "fastiso rows preamble %.3f us (per rows phase)\n", h[23] / 100.0 / h[16]);
+            if (h[22] && h[16]) fprintf(stderr, "fastiso step control %.3f us (per rows phase)\n", h[22] / 100.0 / h[16]);
             for (int k = 0; k < 8; ++k)
                 if (h[24 + k] && h[16])
                     fprintf(stderr, "fastiso rows mark %d %.3f us (per rows phase)\n", k, h[24 + k] / 100.0 / h[16]);

@@ -147,8 +147,12 @@ struct IceT {
     double tt, dtk, a;  // t - t_i0, t_i1 - t_i0, separable amplitude a(t)
 };
 
-// TimeInterpolation2D interpolate! bracket (src/interpolations.jl:51-74)
+// TimeInterpolation2D interpolate! bracket (src/interpolations.jl:51-74): i = first snapshot with
+// t_i >= t; exact hit at i, else linear between i-1 and i; outside [t_0, t_end] an error unless the
+// series is flat-extrapolated. Found by the whole workgroup in one round of loads (thread k tests
+// snapshot k); every thread returns the same bracket.
 __device__ __forceinline__ IceT ice_time(const KP& p, double t) {
+    __shared__ IceT sh_it;
     IceT it;
     it.i0 = it.i1 = 0;
     it.exact = 1;
@@ -159,33 +163,39 @@ __device__ __forceinline__ IceT ice_time(const KP& p, double t) {
     if (p.ice_mode == ICE_EXTERNAL) return it;
     const double* tv = p.ice_t;
     const int nt = p.ice_nt;
-    if (t < tv[0]) {
+    const bool sep = p.ice_mode == ICE_SEPARABLE;
+    const double tfirst = tv[0], tlast = tv[nt - 1];
+    if (t < tfirst || t > tlast) {
         if (!p.ice_flat) it.err = 1;
-        it.i0 = it.i1 = 0;
-    } else if (t > tv[nt - 1]) {
-        if (!p.ice_flat) it.err = 1;
-        it.i0 = it.i1 = nt - 1;
-    } else {
-        int lo = 0, hi = nt - 1;  // first index with tv[i] >= t
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (tv[mid] >= t) hi = mid;
-            else lo = mid + 1;
-        }
-        if (tv[lo] == t) {
-            it.i0 = it.i1 = lo;
-        } else {
-            it.i0 = lo - 1;
-            it.i1 = lo;
-            it.exact = 0;
-            it.tt = t - tv[lo - 1];
-            it.dtk = tv[lo] - tv[lo - 1];
+        it.i0 = it.i1 = (t < tfirst) ? 0 : nt - 1;
+        if (sep) it.a = p.ice_a[it.i0];
+        return it;
+    }
+    for (int k = threadIdx.x; k < nt; k += NT) {
+        const double tk = tv[k];
+        const double tp = k ? tv[k - 1] : 0.0;
+        const double ak = sep ? p.ice_a[k] : 0.0;
+        const double ap = (sep && k) ? p.ice_a[k - 1] : 0.0;
+        if (tk >= t && (k == 0 || tp < t)) {
+            IceT r = it;
+            if (tk == t) {
+                r.i0 = r.i1 = k;
+                r.a = ak;
+            } else {
+                r.i0 = k - 1;
+                r.i1 = k;
+                r.exact = 0;
+                r.tt = t - tp;
+                r.dtk = tk - tp;
+                r.a = ap + (ak - ap) / r.dtk * r.tt;
+            }
+            sh_it = r;
         }
     }
-    if (p.ice_mode == ICE_SEPARABLE) {
-        const double a0 = p.ice_a[it.i0], a1 = p.ice_a[it.i1];
-        it.a = it.exact ? a0 : a0 + (a1 - a0) / it.dtk * it.tt;
-    }
+    __syncthreads();
+    it = sh_it;
+    if (!sep) it.a = 0;
+    __syncthreads();
     return it;
 }
 
@@ -804,12 +814,6 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
                          char* smem, double* red, int* flag, bool sparse_only, unsigned long long& tl) {
     st.sb_out = (int)(s.nrhs & 1);
     st.sb_in = 1 - st.sb_out;   // the previous call's output
-    const IceT it = ice_time(p, st.t);
-    if (it.err) {
-        if (threadIdx.x == 0) s.err = ERR_TIME_RANGE;
-        __syncthreads();
-        return false;
-    }
     KC4 kc;
     kc.v0 = kc.v1 = kc.v2 = kc.v3 = 0.0f;
     const bool trig = sparse_only || (((st.t - s.t0) / p.dt_sparse) >= (double)s.count);
@@ -826,6 +830,12 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
         else { st.g2 = st.kin; st.a2 = st.ain; }
         st.ng += 1;
         st.kin = -1;
+    }
+    const IceT it = ice_time(p, st.t);
+    if (it.err) {
+        if (threadIdx.x == 0) s.err = ERR_TIME_RANGE;
+        __syncthreads();
+        return false;
     }
     if (st.kin >= 0) kc = spectral_corners(p, specbuf(p, st.sb_in), twx, red);
     const double uoff = u_offset(p, st, kc);
@@ -845,13 +855,15 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
             p.vpart[4 * blockIdx.x + 2] = acc[2];
         }
         if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
+        double vs[3] = {0.0, 0.0, 0.0};
+        for (int w = threadIdx.x; w < (int)gridDim.x; w += NT) {
+            vs[0] += p.vpart[4 * w];
+            vs[1] += p.vpart[4 * w + 1];
+            vs[2] += p.vpart[4 * w + 2];
+        }
+        block_sum<3>(red, vs);
         if (threadIdx.x == 0) {
-            double v0 = 0, v1 = 0, v2 = 0;
-            for (int w = 0; w < (int)gridDim.x; ++w) {
-                v0 += p.vpart[4 * w];
-                v1 += p.vpart[4 * w + 1];
-                v2 += p.vpart[4 * w + 2];
-            }
+            const double v0 = vs[0], v1 = vs[1], v2 = vs[2];
             const double Vold = s.V_af + s.V_den + s.V_pov;
             s.V_af = p.vol_goelzer ? v0 * p.rho_ice_sw : 0.0;
             s.V_den = p.den_goelzer ? v1 * (p.rho_ice / p.rho_water - p.rho_ice / p.rho_seawater) : 0.0;
@@ -866,6 +878,10 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
     // store this call's loads if the next call may trigger (the sparse update reads the
     // column anomalies of the previous call, src/simulation.jl:362-385)
     st.write_sigma = (p.write_sigma_always || (((st.t_next_max - s.t0) / p.dt_sparse) >= (double)s.count)) ? 1 : 0;
+#ifdef FI_SUBPHASE_TIMING
+    __syncthreads();
+    if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) p.ptime[23] += wall_clock64() - tl;
+#endif
     if (sizeof(MT) == 8) phase_rows<MT, 4, 2>(p, st, uoff, it, twx, smem, kc);
     else phase_rows<MT, 8, 4>(p, st, uoff, it, twx, smem, kc);
     if (!grid_sync(p, gen, flag, TAG_ROWS, tl)) return false;
