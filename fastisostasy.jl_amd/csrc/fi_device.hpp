@@ -58,6 +58,15 @@ struct Bar {
 };
 constexpr int BAR_SLOT = 32;
 
+// The workgroups that cooperate on one simulation: the whole grid for a single context, one
+// team of the grid for an ensemble member. rank/size take the place of blockIdx.x/gridDim.x in
+// every phase; bar is the team's barrier region (a header Bar, then 128-byte arrival and
+// release slots per workgroup).
+struct Team {
+    int rank, size;
+    Bar* bar;
+};
+
 struct KP {
     int nx, ny, nhx, nfx, nfy, nfhx, nrowitems, ncolitems, colnb;
     int i1, i2, j1, j2;  // 1-based crop indices (src/convolutions.jl:196-204)

@@ -18,45 +18,13 @@
 
 namespace fi {
 
-// ------------------------------------------------------------------------------------------
-// persistent kernel
-// ------------------------------------------------------------------------------------------
+// The adaptive BS3 loop of one simulation on a team: from (s.t, U[s.cur], K[s.k1]) until t
+// reaches s.next_out (after an accepted step) or s.stop, or p.max_steps attempts. Every team
+// member holds the same Ctl copy in LDS and takes the same decisions.
 template <typename MT>
-__global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
-    const KP& p = *pp;
-    extern __shared__ __align__(16) char smem_all[];
-    __shared__ Ctl s;
-    __shared__ double red[64];
-    __shared__ int flag;
-    unsigned gen = 0;
-    unsigned long long tl = wall_clock64();
-    // twiddle tables of the two run-time transform lengths stay in LDS for the whole kernel
-    float2* twx = (float2*)smem_all;
-    float2* twy = twx + p.nx;
-    char* smem = (char*)(twy + p.ny);
-    for (int k = threadIdx.x; k < p.nx; k += NT) twx[k] = p.tw32[p.px.tw + k];
-    for (int k = threadIdx.x; k < p.ny; k += NT) twy[k] = p.tw32[p.py.tw + k];
-    if (threadIdx.x == 0) s = *p.ctl;
-    __syncthreads();
-
-    if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY) {
-        StageDesc st{};
-        st.ubase = p.rhs_ubuf;
-        st.ng = 0;
-        st.kin = -1;
-        st.dt = 0;
-        st.write_u = -1;
-        st.kout = p.rhs_kslot;
-        st.do_c2r = 1;
-        st.t = s.t_rhs;
-        st.t_next_max = s.t_rhs;
-        rhs_call<MT>(p, s, st, gen, twx, twy, smem, red, &flag, p.mode == MODE_SPARSE_ONLY, tl);
-        if (blockIdx.x == 0 && threadIdx.x == 0) *p.ctl = s;
-        return;
-    }
-
-    // MODE_INTEGRATE: adaptive BS3 from (s.t, U[s.cur], K[s.k1]) until t reaches s.next_out
-    // (after an accepted step) or s.stop, or max_steps attempts
+__device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
+                                               const float2* twy, char* smem, double* red, int* flag,
+                                               unsigned long long& tl) {
     long long attempts = 0;
     for (;;) {
         if (s.err || s.t >= s.stop || attempts >= p.max_steps) break;
@@ -115,7 +83,7 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
                 st.e3 = s3;
                 st.u_prev = cur;
             }
-            if (!rhs_call<MT>(p, s, st, gen, twx, twy, smem, red, &flag, false, tl)) {
+            if (!rhs_call<MT>(p, tm, s, st, gen, twx, twy, smem, red, flag, false, tl)) {
                 ok = false;
                 break;
             }
@@ -126,12 +94,12 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
         unsigned long long te0 = wall_clock64();
 #endif
         // error norm with the corner-BC offsets of k1..k4 folded in:
-        // sum ((a - C) w)^2 = S2 - 2 C S1 + C^2 S0
+        // sum ((a - C) w)^2 = S2 - 2 C S1 + C^2 S0, from one partial per row pair
         // (one round of loads by the whole workgroup, fixed-order sum: identical in every workgroup)
         const double C = dt * (BS3_BT1 * k_offset(p, k1) + BS3_BT2 * k_offset(p, s2) +
                                BS3_BT3 * k_offset(p, s3) + BS3_BT4 * k_offset(p, s4));
         double Sm[3] = {0.0, 0.0, 0.0};
-        for (int w = threadIdx.x; w < (int)gridDim.x; w += NT) {
+        for (int w = threadIdx.x; w < p.nrowitems; w += NT) {
             Sm[0] += p.part[4 * w];
             Sm[1] += p.part[4 * w + 1];
             Sm[2] += p.part[4 * w + 2];
@@ -172,11 +140,128 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
         }
         __syncthreads();
 #ifdef FI_SUBPHASE_TIMING
-        if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) p.ptime[22] += wall_clock64() - te0;
+        if (p.ptime && tm.rank == 0 && threadIdx.x == 0) p.ptime[22] += wall_clock64() - te0;
 #endif
         if (s.steps == 1 && s.t >= s.next_out) break;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *p.ctl = s;
+}
+
+// A single RHS evaluation on U[ubuf] at time t into K[kslot] (update_diagnostics!).
+template <typename MT>
+__device__ __forceinline__ bool single_rhs(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
+                                           const float2* twy, char* smem, double* red, int* flag,
+                                           unsigned long long& tl, int ubuf, int kslot, double t, bool sparse_only,
+                                           bool force_sigma) {
+    StageDesc st{};
+    st.ubase = ubuf;
+    st.ng = 0;
+    st.kin = -1;
+    st.dt = 0;
+    st.write_u = -1;
+    st.kout = kslot;
+    st.do_c2r = 1;
+    st.t = t;
+    st.t_next_max = t;
+    st.force_sigma = force_sigma ? 1 : 0;
+    return rhs_call<MT>(p, tm, s, st, gen, twx, twy, smem, red, flag, sparse_only, tl);
+}
+
+__device__ __forceinline__ void load_twiddles(const KP& p, float2* twx, float2* twy) {
+    for (int k = threadIdx.x; k < p.nx; k += NT) twx[k] = p.tw32[p.px.tw + k];
+    for (int k = threadIdx.x; k < p.ny; k += NT) twy[k] = p.tw32[p.py.tw + k];
+}
+
+// ------------------------------------------------------------------------------------------
+// persistent kernel of one context: the whole grid is one team
+// ------------------------------------------------------------------------------------------
+template <typename MT>
+__global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
+    const KP& p = *pp;
+    extern __shared__ __align__(16) char smem_all[];
+    __shared__ Ctl s;
+    __shared__ double red[64];
+    __shared__ int flag;
+    const Team tm{(int)blockIdx.x, (int)gridDim.x, p.bar};
+    unsigned gen = 0;
+    unsigned long long tl = wall_clock64();
+    // twiddle tables of the two run-time transform lengths stay in LDS for the whole kernel
+    float2* twx = (float2*)smem_all;
+    float2* twy = twx + p.nx;
+    char* smem = (char*)(twy + p.ny);
+    load_twiddles(p, twx, twy);
+    if (threadIdx.x == 0) s = *p.ctl;
+    __syncthreads();
+    if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY)
+        (void)single_rhs<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, p.rhs_ubuf, p.rhs_kslot, s.t_rhs,
+                             p.mode == MODE_SPARSE_ONLY, false);
+    else
+        integrate_loop<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
+    if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// persistent ensemble kernel: the grid is split into nteams teams of tsize workgroups (team =
+// blockIdx.x mod nteams, so with nteams a multiple of 8 a team shares one XCD); each team takes
+// members from a work queue (ordered largest first) and integrates each to its own s.stop with
+// team-local barriers. With fsal set, a member that reached s.stop re-evaluates its first-same-as-
+// last derivative there, as after an output callback. Results do not depend on the team size:
+// every cross-workgroup sum runs over fixed per-row-pair partials.
+// ------------------------------------------------------------------------------------------
+struct BatchArgs {
+    const KP* const* kps;   // member parameter blocks (device memory)
+    const int* order;       // queue position -> member
+    int nmem, nteams, tsize, fsal;
+    Bar* bars;              // nteams regions of (1 + 2 tsize) Bar
+    int* queue;             // queue head (zeroed by the host)
+    int* team_member;       // per team: the member it is working on (written by its rank 0)
+    unsigned long long spin_limit;
+};
+
+template <typename MT>
+__global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __restrict__ ba) {
+    extern __shared__ __align__(16) char smem_all[];
+    __shared__ Ctl s;
+    __shared__ double red[64];
+    __shared__ int flag;
+    __shared__ int s_m;
+    const int team = (int)blockIdx.x % ba->nteams;
+    const Team tm{(int)blockIdx.x / ba->nteams, ba->tsize, ba->bars + (long)team * (1 + 2 * ba->tsize)};
+    unsigned gen = 0;
+    unsigned long long tl = wall_clock64();
+    for (;;) {
+        // rank 0 pops the next member; the team barrier publishes it
+        if (tm.rank == 0 && threadIdx.x == 0) {
+            const int q = atomicAdd(ba->queue, 1);
+            __hip_atomic_store(&ba->team_member[team], q < ba->nmem ? ba->order[q] : -1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!team_sync(tm, gen, &flag, ba->spin_limit)) return;
+        if (threadIdx.x == 0)
+            s_m = __hip_atomic_load(&ba->team_member[team], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int m = s_m;
+        if (m < 0) return;
+        const KP& p = *ba->kps[m];
+        float2* twx = (float2*)smem_all;
+        float2* twy = twx + p.nx;
+        char* smem = (char*)(twy + p.ny);
+        load_twiddles(p, twx, twy);
+        if (threadIdx.x == 0) s = *p.ctl;
+        __syncthreads();
+        const bool todo = s.t < s.stop && !s.err;
+        if (todo) {
+            integrate_loop<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
+            if (flag) return;
+            if (ba->fsal && !s.err && s.t >= s.stop) {
+                if (!single_rhs<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, s.cur, s.k1, s.t, false, true)) {
+                    if (flag) return;
+                }
+                if (threadIdx.x == 0) s.last_u = s.cur;
+                __syncthreads();
+            }
+        }
+        if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -313,7 +398,7 @@ struct fi_ctx {
     KP kp;
     Ctl ctl;
     int G = 0;
-    size_t lds = 0;
+    size_t lds = 0, lds_d = 0, lds_f = 0;   // LDS bytes of this context's launch; with FP64 / FP32 moments
     bool mt_double = true;
     FftPlan px64, py64, pfx64, pfy64;
     bool initialized = false, integ = false, k1_stale = false;
@@ -910,7 +995,9 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16 + 24), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16 + 24);
     size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)3 * c->nfx * 8, (size_t)5 * c->nfy * 8});
     c->mt_double = nx <= 512 && twb + rows_d <= 160 * 1024;   // FP64 moments, register prefetch sized for nx <= 512
-    c->lds = twb + std::max(c->mt_double ? rows_d : rows_f, other);
+    c->lds_d = twb + std::max(rows_d, other);
+    c->lds_f = twb + std::max(rows_f, other);
+    c->lds = c->mt_double ? c->lds_d : c->lds_f;
     if (c->lds > 160 * 1024 || nx > 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
     const void* f = c->mt_double ? (const void*)fi_persistent<double> : (const void*)fi_persistent<float>;
     HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
@@ -921,8 +1008,9 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     int want = std::max(kp.nrowitems, kp.ncolitems);
     c->G = std::max(1, std::min(maxg, want));
     if (maxg < 1) return fail(c, FI_ERR_UNSUPPORTED, "persistent kernel does not fit on the device");
-    HIPCHECK(c, c->part.alloc(4 * (size_t)c->G));
-    HIPCHECK(c, c->vpart.alloc(4 * (size_t)c->G));
+    // one error-norm and one volume partial per row pair, whatever the number of workgroups
+    HIPCHECK(c, c->part.alloc(4 * (size_t)kp.nrowitems));
+    HIPCHECK(c, c->vpart.alloc(4 * (size_t)kp.nrowitems));
     kp.part = c->part.p;
     kp.vpart = c->vpart.p;
     HIPCHECK(c, c->ctl_d.alloc(1));
@@ -1219,6 +1307,272 @@ int fi_get_stats(fi_ctx* c, fi_stats* o) {
     o->device_ms = c->device_ms;
     o->n_launch = c->n_launch;
     o->n_accept_timed = c->n_accept_timed;
+    return FI_OK;
+}
+
+}  // extern "C"
+
+// ==========================================================================================
+// ensembles
+// ==========================================================================================
+struct fi_batch {
+    std::vector<fi_ctx*> mem;
+    std::string err;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int tsize_fixed = 0;         // 0: choose by timing
+    int tsize_best = 0;
+    size_t cand_next = 0;        // next candidate team size under trial
+    std::vector<int> cands;
+    std::vector<double> cand_rate;
+    int last_tsize = 0, last_nteams = 0;
+    double device_ms = 0, cell_attempts = 0;
+    long long n_launch = 0;
+    DBuf<const KP*> kps;
+    DBuf<int> order, queue, team_member;
+    DBuf<Bar> bars;
+    DBuf<BatchArgs> args;
+    std::vector<int> order_h;
+};
+
+namespace {
+
+int bfail(fi_batch* b, int code, const std::string& m) {
+    if (b) b->err = m;
+    return code;
+}
+
+#define BHIPCHECK(b, x)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return bfail((b), FI_ERR_DEVICE, std::string(#x " : ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// attempts per member per trial launch while the team size is being chosen; pausing a member on
+// an attempt count leaves its step sequence unchanged
+constexpr int TUNE_ATTEMPTS = 64;
+// attempts per member per launch otherwise (bounds one launch's duration)
+constexpr int LAUNCH_ATTEMPTS = 1 << 20;
+
+// one persistent ensemble launch with teams of ts workgroups; returns the device time
+int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int max_steps, int fsal, double t_target,
+                 float* ms_out, double* work_out) {
+    const int nm = (int)b->mem.size();
+    const int nteams = std::max(1, gmax / ts);
+    std::vector<KP> kph(nm);
+    std::vector<const KP*> ptrs(nm);
+    std::vector<long long> att0(nm);
+    for (int m = 0; m < nm; ++m) {
+        fi_ctx* c = b->mem[m];
+        c->ctl.stop = t_target;
+        c->ctl.next_out = t_target;
+        att0[m] = c->ctl.nacc + c->ctl.nrej;
+        kph[m] = c->kp;
+        kph[m].mode = MODE_INTEGRATE;
+        kph[m].max_steps = max_steps;
+        kph[m].write_sigma_always = 0;
+        ptrs[m] = c->kp_d.p;
+        BHIPCHECK(b, hipMemcpyAsync(c->kp_d.p, &kph[m], sizeof(KP), hipMemcpyHostToDevice, b->stream));
+        BHIPCHECK(b, hipMemcpyAsync(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
+    }
+    const size_t nbar = (size_t)nteams * (1 + 2 * (size_t)ts);
+    if (b->bars.n < nbar) BHIPCHECK(b, b->bars.alloc(nbar));
+    if (b->team_member.n < (size_t)nteams) BHIPCHECK(b, b->team_member.alloc(nteams));
+    BHIPCHECK(b, hipMemsetAsync(b->bars.p, 0, nbar * sizeof(Bar), b->stream));
+    BHIPCHECK(b, hipMemsetAsync(b->queue.p, 0, sizeof(int), b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(b->kps.p, ptrs.data(), nm * sizeof(const KP*), hipMemcpyHostToDevice, b->stream));
+    BatchArgs a;
+    a.kps = b->kps.p;
+    a.order = b->order.p;
+    a.nmem = nm;
+    a.nteams = nteams;
+    a.tsize = ts;
+    a.fsal = fsal;
+    a.bars = b->bars.p;
+    a.queue = b->queue.p;
+    a.team_member = b->team_member.p;
+    a.spin_limit = 2000000000ull;
+    BHIPCHECK(b, hipMemcpyAsync(b->args.p, &a, sizeof(a), hipMemcpyHostToDevice, b->stream));
+    const BatchArgs* ap = b->args.p;
+    void* kargs[] = {&ap};
+    const void* f = mt_double ? (const void*)fi_batch_persistent<double> : (const void*)fi_batch_persistent<float>;
+    BHIPCHECK(b, hipEventRecord(b->ev0, b->stream));
+    BHIPCHECK(b, hipLaunchCooperativeKernel(f, dim3(nteams * ts), dim3(NT), kargs, (unsigned)lds, b->stream));
+    BHIPCHECK(b, hipEventRecord(b->ev1, b->stream));
+    for (int m = 0; m < nm; ++m)
+        BHIPCHECK(b, hipMemcpyAsync(&b->mem[m]->ctl, b->mem[m]->ctl_d.p, sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    std::vector<Bar> hb(nbar);
+    BHIPCHECK(b, hipMemcpyAsync(hb.data(), b->bars.p, nbar * sizeof(Bar), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHECK(b, hipStreamSynchronize(b->stream));
+    float ms = 0;
+    BHIPCHECK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    double work = 0;
+    for (int m = 0; m < nm; ++m) {
+        fi_ctx* c = b->mem[m];
+        work += (double)(c->ctl.nacc + c->ctl.nrej - att0[m]) * (double)c->N;
+        c->ctl.last_u = c->ctl.cur;
+    }
+    b->device_ms += ms;
+    b->cell_attempts += work;
+    b->n_launch += 1;
+    b->last_tsize = ts;
+    b->last_nteams = nteams;
+    *ms_out = ms;
+    *work_out = work;
+    for (int t = 0; t < nteams; ++t)
+        if (hb[(size_t)t * (1 + 2 * (size_t)ts)].abort) return bfail(b, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
+    for (int m = 0; m < nm; ++m) {
+        const int e = b->mem[m]->ctl.err;
+        if (e == ERR_TIME_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation2D out of range.");
+        if (e == ERR_DT_MIN) return bfail(b, FI_ERR_DT_MIN, "member " + std::to_string(m) + ": dt below minimum");
+        if (e) return bfail(b, FI_ERR_DEVICE, "member " + std::to_string(m) + ": device error");
+    }
+    return FI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out) {
+    if (!cfgs || !out || nmem < 1) return FI_ERR_INVALID;
+    *out = nullptr;
+    fi_batch* b = new fi_batch();
+    b->dev = cfgs[0].device;
+    for (int m = 0; m < nmem; ++m) {
+        if (cfgs[m].device != b->dev) {
+            fi_batch_destroy(b);
+            return FI_ERR_INVALID;  // one device per ensemble
+        }
+        fi_ctx* c = nullptr;
+        const int st = fi_create(&cfgs[m], &c);
+        if (st) {
+            fi_batch_destroy(b);
+            return st;
+        }
+        b->mem.push_back(c);
+    }
+    if (hipSetDevice(b->dev) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        b->kps.alloc(nmem) != hipSuccess || b->order.alloc(nmem) != hipSuccess || b->queue.alloc(1) != hipSuccess ||
+        b->args.alloc(1) != hipSuccess) {
+        fi_batch_destroy(b);
+        return FI_ERR_DEVICE;
+    }
+    *out = b;
+    return FI_OK;
+}
+
+fi_ctx* fi_batch_member(fi_batch* b, int32_t m) {
+    if (!b || m < 0 || m >= (int)b->mem.size()) return nullptr;
+    return b->mem[m];
+}
+
+void fi_batch_destroy(fi_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (fi_ctx* c : b->mem) fi_destroy(c);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    hipStream_t s = b->stream;
+    delete b;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* fi_batch_last_error(const fi_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int fi_batch_set_team_size(fi_batch* b, int32_t ts) {
+    if (!b || ts < 0) return FI_ERR_INVALID;
+    b->tsize_fixed = ts;
+    b->tsize_best = 0;
+    b->cand_next = 0;
+    b->cands.clear();
+    b->cand_rate.clear();
+    return FI_OK;
+}
+
+int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
+    if (!b) return FI_ERR_INVALID;
+    const int nm = (int)b->mem.size();
+    bool mt_double = true;
+    size_t lds_d = 0, lds_f = 0;
+    for (int m = 0; m < nm; ++m) {
+        fi_ctx* c = b->mem[m];
+        if (!c->integ) return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": fi_batch_integrate before fi_integrator_init");
+        if (t_target < c->ctl.t) return bfail(b, FI_ERR_INVALID, "member " + std::to_string(m) + ": target before current time");
+        if (c->k1_stale) return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": state written during integration");
+        mt_double = mt_double && c->mt_double;
+        lds_d = std::max(lds_d, c->lds_d);
+        lds_f = std::max(lds_f, c->lds_f);
+    }
+    const size_t lds = mt_double ? lds_d : lds_f;
+    if (lds > 160 * 1024) return bfail(b, FI_ERR_UNSUPPORTED, "members too large for one workgroup's LDS");
+    BHIPCHECK(b, hipSetDevice(b->dev));
+    const void* f = mt_double ? (const void*)fi_batch_persistent<double> : (const void*)fi_batch_persistent<float>;
+    BHIPCHECK(b, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0, ncu = 0;
+    BHIPCHECK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, lds));
+    BHIPCHECK(b, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->dev));
+    const int gmax = per_cu * ncu;
+    if (gmax < 1) return bfail(b, FI_ERR_UNSUPPORTED, "ensemble kernel does not fit on the device");
+    // queue order: largest members first (cost per attempt ~ cells)
+    if ((int)b->order_h.size() != nm) {
+        b->order_h.resize(nm);
+        for (int m = 0; m < nm; ++m) b->order_h[m] = m;
+        std::stable_sort(b->order_h.begin(), b->order_h.end(),
+                         [&](int x, int y) { return b->mem[x]->N > b->mem[y]->N; });
+        BHIPCHECK(b, hipMemcpy(b->order.p, b->order_h.data(), nm * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (b->tsize_fixed == 0 && b->cands.empty()) {
+        for (int ts : {1, 2, 4, 8, 16, 32})
+            if (ts <= gmax) b->cands.push_back(ts);
+        b->cand_rate.assign(b->cands.size(), 0.0);
+    }
+    auto done = [&]() {
+        for (fi_ctx* c : b->mem)
+            if (c->ctl.t < t_target) return false;
+        return true;
+    };
+    while (!done()) {
+        int ts;
+        bool trial = false;
+        if (b->tsize_fixed > 0) ts = std::min(b->tsize_fixed, gmax);
+        else if (b->tsize_best > 0) ts = b->tsize_best;
+        else {
+            ts = b->cands[b->cand_next];
+            trial = true;
+        }
+        float ms = 0;
+        double work = 0;
+        int st = batch_launch(b, mt_double, lds, gmax, ts, trial ? TUNE_ATTEMPTS : LAUNCH_ATTEMPTS, fsal, t_target,
+                              &ms, &work);
+        if (st) return st;
+        if (trial) {
+            b->cand_rate[b->cand_next] = ms > 0 ? work / ms : 0.0;
+            b->cand_next += 1;
+            if (b->cand_next == b->cands.size()) {
+                size_t best = 0;
+                for (size_t k = 1; k < b->cands.size(); ++k)
+                    if (b->cand_rate[k] > b->cand_rate[best]) best = k;
+                b->tsize_best = b->cands[best];
+            }
+        }
+    }
+    return FI_OK;
+}
+
+int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {
+    if (!b || !o) return FI_ERR_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->n_members = (int32_t)b->mem.size();
+    o->team_size = b->last_tsize;
+    o->n_teams = b->last_nteams;
+    o->tuned = (b->tsize_fixed > 0 || b->tsize_best > 0) ? 1 : 0;
+    o->device_ms = b->device_ms;
+    o->n_launch = b->n_launch;
+    o->cell_attempts = b->cell_attempts;
     return FI_OK;
 }
 

@@ -30,22 +30,19 @@ namespace fi {
 // Waits are bounded: after spin_limit wall-clock ticks the abort flag is raised and every
 // workgroup leaves the kernel.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool grid_sync(const KP& p, unsigned& gen, int* s_flag, int tag,
-                                          unsigned long long& tlast) {
-    Bar* bar = p.bar;
+__device__ __forceinline__ bool team_sync(const Team& tm, unsigned& gen, int* s_flag, unsigned long long spin_limit) {
+    Bar* bar = tm.bar;
     unsigned* arrive = (unsigned*)(bar + 1);
-    unsigned* release = arrive + (long)gridDim.x * BAR_SLOT;
+    unsigned* release = arrive + (long)tm.size * BAR_SLOT;
     const unsigned g = gen + 1u;
     gen = g;
     __syncthreads();
-    unsigned long long tin = 0;
-    if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) tin = wall_clock64();
-    if (blockIdx.x == 0) {
-        for (int w = 1 + (int)threadIdx.x; w < (int)gridDim.x; w += NT) {
+    if (tm.rank == 0) {
+        for (int w = 1 + (int)threadIdx.x; w < tm.size; w += NT) {
             const unsigned long long t0 = wall_clock64();
             while (__hip_atomic_load(&arrive[w * BAR_SLOT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) {
                 if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (wall_clock64() - t0 > p.spin_limit) {
+                if (wall_clock64() - t0 > spin_limit) {
                     __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
@@ -55,16 +52,16 @@ __device__ __forceinline__ bool grid_sync(const KP& p, unsigned& gen, int* s_fla
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
         // publish the generation into every workgroup's own release slot
-        for (int w = 1 + (int)threadIdx.x; w < (int)gridDim.x; w += NT)
+        for (int w = 1 + (int)threadIdx.x; w < tm.size; w += NT)
             __hip_atomic_store(&release[w * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         if (threadIdx.x == 0) *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (threadIdx.x == 0) {
-        __hip_atomic_store(&arrive[blockIdx.x * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&arrive[tm.rank * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long t0 = wall_clock64();
-        unsigned* mine = &release[blockIdx.x * BAR_SLOT];
+        unsigned* mine = &release[tm.rank * BAR_SLOT];
         while (__hip_atomic_load(mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) {
             if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-            if (wall_clock64() - t0 > p.spin_limit) {
+            if (wall_clock64() - t0 > spin_limit) {
                 __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
@@ -74,14 +71,25 @@ __device__ __forceinline__ bool grid_sync(const KP& p, unsigned& gen, int* s_fla
         *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) {
+    return *s_flag == 0;
+}
+
+// team_sync with the optional per-phase timing of the team's rank 0 (p.ptime, 100 MHz ticks:
+// [tag] work before the barrier, [8 + tag] barrier wait, [16 + tag] count)
+__device__ __forceinline__ bool grid_sync(const KP& p, const Team& tm, unsigned& gen, int* s_flag, int tag,
+                                          unsigned long long& tlast) {
+    const bool timed = p.ptime && tm.rank == 0 && threadIdx.x == 0;
+    unsigned long long tin = 0;
+    if (timed) tin = wall_clock64();
+    const bool ok = team_sync(tm, gen, s_flag, p.spin_limit);
+    if (timed) {
         const unsigned long long tout = wall_clock64();
         p.ptime[tag] += tin - tlast;
         p.ptime[8 + tag] += tout - tin;
         p.ptime[16 + tag] += 1;
         tlast = tout;
     }
-    return *s_flag == 0;
+    return ok;
 }
 
 // Block-wide sums of M doubles per thread: wave64 butterfly, then the waves in fixed order
@@ -133,6 +141,7 @@ struct StageDesc {
     int write_sigma;   // set by rhs_call: store this call's loads for a later sparse update
     int sb_in;         // spectrum buffer holding the pending derivative (kin)
     int sb_out;        // spectrum buffer this call writes
+    int force_sigma;   // store this call's loads whatever the next call's time
 };
 
 struct Acc3 {
@@ -348,7 +357,7 @@ __device__ __forceinline__ float2 c2r_pair_value(const KP& p, const float2* spec
 #define FI_SUBMARK(k)                                                                   \
     do {                                                                                \
         __syncthreads();                                                                \
-        if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) {                           \
+        if (p.ptime && tm.rank == 0 && threadIdx.x == 0) {                              \
             const unsigned long long t_ = wall_clock64();                               \
             p.ptime[24 + (k)] += t_ - tsub;                                             \
             tsub = t_;                                                                  \
@@ -372,8 +381,8 @@ struct CellF {
 };
 
 template <typename MT, int QD, int QF>
-__device__ __forceinline__ void phase_rows(const KP& p, const StageDesc& st, double uoff, const IceT it, const float2* twx,
-                           char* smem, const KC4 kc) {
+__device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const StageDesc& st, double uoff, const IceT it,
+                           const float2* twx, char* smem, const KC4 kc) {
     const int nx = p.nx, ny = p.ny;
     const float inv_nx = 1.0f / (float)nx;
     double* su = (double*)smem;                 // 6 rows of the stage displacement
@@ -391,7 +400,7 @@ __device__ __forceinline__ void phase_rows(const KP& p, const StageDesc& st, dou
     const double nu = p.nu;
     const bool wsig = st.write_sigma != 0;
     const double* U0 = ubuf(p, st.ubase);
-    for (int item = blockIdx.x; item < p.nrowitems; item += gridDim.x) {
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item, je = min(j0 + 2, ny);
         const int jm0 = max(0, min(j0 - 1, ny - 3)), jm1 = min(ny, max(je + 1, 3));
         const int ju0 = max(0, min(jm0 - 1, ny - 3)), ju1 = min(ny, max(jm1 + 1, 3));
@@ -565,11 +574,11 @@ __device__ __forceinline__ void phase_rows(const KP& p, const StageDesc& st, dou
 // ------------------------------------------------------------------------------------------
 // columns phase: FFT_y, * spy/(nx ny)/(R kappa), inverse FFT_y
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void phase_cols(const KP& p, float2* spec, const float2* twy, char* smem) {
+__device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* spec, const float2* twy, char* smem) {
     const int ny = p.ny, nb = p.colnb;
     float2* sa = (float2*)smem;
     float2* sb = sa + nb * ny;
-    for (int item = blockIdx.x; item < p.ncolitems; item += gridDim.x) {
+    for (int item = tm.rank; item < p.ncolitems; item += tm.size) {
         const int kx0 = item * nb;
         const int nbb = min(nb, p.nhx - kx0);
         const long base = (long)kx0 * ny;
@@ -591,23 +600,23 @@ __device__ __forceinline__ void phase_cols(const KP& p, float2* spec, const floa
 
 // ------------------------------------------------------------------------------------------
 // standalone c2r phase: inverse real transform of row pairs into K[slot]; optional BS3 error
-// moments
+// moments, one partial per row pair in p.part (so the step-size decision does not depend on how
+// many workgroups share the work)
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ Acc3 phase_c2r(const KP& p, const StageDesc& st, const float2* spec, int slot, const float2* twx,
-                          char* smem) {
-    Acc3 acc;
-    acc.a = acc.b = acc.c = 0.0;
+__device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const StageDesc& st, const float2* spec, int slot,
+                                          const float2* twx, char* smem, double* red) {
     const int nx = p.nx, ny = p.ny;
     float2* sf = (float2*)smem;
     float2* sg = sf + nx;
     float* K = p.Kb + (long)slot * p.Nl;
-    for (int item = blockIdx.x; item < p.nrowitems; item += gridDim.x) {
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item;
         for (int k = threadIdx.x; k < nx; k += NT) sf[k] = c2r_pair_value(p, spec, k, j0, j0 + 1);
         __syncthreads();
         float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, +1);
+        double acc[3] = {0.0, 0.0, 0.0};
         for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
-            const int r = idx / nx, i = idx - r * nx, j = j0 + r;
+            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
             const float v = r ? res[i].y : res[i].x;
@@ -619,14 +628,21 @@ __device__ __forceinline__ Acc3 phase_c2r(const KP& p, const StageDesc& st, cons
                 const double un = ubuf(p, st.u_prev)[c], up = ubuf(p, st.write_u)[c];
                 const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)v);
                 const double w = 1.0 / (p.abstol + fmax(fabs(un), fabs(up)) * p.reltol);
-                acc.a += (a * w) * (a * w);
-                acc.b += a * w * w;
-                acc.c += w * w;
+                acc[0] += (a * w) * (a * w);
+                acc[1] += a * w * w;
+                acc[2] += w * w;
+            }
+        }
+        if (st.final_err) {
+            block_sum<3>(red, acc);
+            if (threadIdx.x == 0) {
+                p.part[4 * item + 0] = acc[0];
+                p.part[4 * item + 1] = acc[1];
+                p.part[4 * item + 2] = acc[2];
             }
         }
         __syncthreads();
     }
-    return acc;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -638,13 +654,13 @@ __device__ __forceinline__ void stage_twiddles(const KP& p, const FftPlan& pl, f
 }
 
 // padded rows: the two convolution inputs of row j packed as one complex sequence
-__device__ __forceinline__ void phase_sp_rows(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_sp_rows(const KP& p, const Team& tm, char* smem) {
     const int nx = p.nx, ny = p.ny, nfx = p.nfx;
     float2* tw = (float2*)smem;
     float2* sf = tw + nfx;
     float2* sg = sf + nfx;
     stage_twiddles(p, p.pfx, tw);
-    for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+    for (int j = tm.rank; j < ny; j += tm.size) {
         for (int i = threadIdx.x; i < nfx; i += NT) {
             float2 z;
             z.x = 0;
@@ -680,13 +696,13 @@ __device__ __forceinline__ int sp_row_of(const KP& p, int jj) {
 
 // padded columns: FFT_y of both half spectra, multiply by the FP64 Green spectra, inverse FFT_y,
 // keep the rows the crop and the extended-space corner BC need
-__device__ __forceinline__ void phase_sp_cols(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_sp_cols(const KP& p, const Team& tm, char* smem) {
     const int ny = p.ny, nfy = p.nfy, no = ny + 2;
     float2* tw = (float2*)smem;
     float2* sa = tw + nfy;
     float2* sb = sa + 2 * nfy;
     stage_twiddles(p, p.pfy, tw);
-    for (int kx = blockIdx.x; kx < p.nfhx; kx += gridDim.x) {
+    for (int kx = tm.rank; kx < p.nfhx; kx += tm.size) {
         for (int idx = threadIdx.x; idx < 2 * nfy; idx += NT) {
             const int s = idx / nfy, j = idx - s * nfy;
             float2 v;
@@ -716,14 +732,14 @@ __device__ __forceinline__ void phase_sp_cols(const KP& p, char* smem) {
 }
 
 // needed padded rows: inverse real transforms, crop [i1:i2] and the extended corners
-__device__ __forceinline__ void phase_sp_c2r(const KP& p, char* smem) {
+__device__ __forceinline__ void phase_sp_c2r(const KP& p, const Team& tm, char* smem) {
     const int nx = p.nx, nfx = p.nfx, nfhx = p.nfhx, no = p.ny + 2;
     const bool even = (nfx % 2) == 0;
     float2* tw = (float2*)smem;
     float2* sf = tw + nfx;
     float2* sg = sf + nfx;
     stage_twiddles(p, p.pfx, tw);
-    for (int jj = blockIdx.x; jj < no; jj += gridDim.x) {
+    for (int jj = tm.rank; jj < no; jj += tm.size) {
         for (int k = threadIdx.x; k < nfx; k += NT) {
             float2 A, B;
             if (k < nfhx) {
@@ -765,22 +781,26 @@ __device__ __forceinline__ void phase_sp_c2r(const KP& p, char* smem) {
 }
 
 // pointwise sparse diagnostics; volume partial sums for internal_update_bsl!
-__device__ __forceinline__ Acc3 phase_sp_point(const KP& p, const StageDesc& st, double uoff, const IceT it) {
-    Acc3 acc;
-    acc.a = acc.b = acc.c = 0.0;
-    const long N = (long)p.nx * p.ny;
+// (one partial per row pair in p.vpart)
+__device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                               const IceT it, double* red) {
     const float* ec = p.ext_corners;
     const double oe = ((0.25 * ec[0] + 0.25 * ec[1]) + (0.25 * ec[2] + 0.25 * ec[3])) - p.ebc_x;
     const double og = ((0.25 * ec[4] + 0.25 * ec[5]) + (0.25 * ec[6] + 0.25 * ec[7])) - p.sbc_x;
-    for (long c = (long)blockIdx.x * NT + threadIdx.x; c < N; c += (long)gridDim.x * NT) {
+    const int nx = p.nx;
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+      double acc[3] = {0.0, 0.0, 0.0};
+      const int ncell = (min(2 * item + 2, p.ny) - 2 * item) * nx;
+      for (int q = threadIdx.x; q < ncell; q += NT) {
+        const long c = (long)2 * item * nx + q;
         const double u = u_raw(p, st, c) - uoff;
         const double H = ice_at(p, it, c);
         const double ue_old = p.ue[c], zss_old = p.z_ss[c];
         const double zb_old = (double)p.z_b_ref[c] + ue_old + u;
         const double haf_pre = hab(H, zb_old - zss_old, p.rho_sw_ice);
-        acc.a += haf_pre * p.A;
-        acc.b += H * p.A;
-        acc.c += fmax(zss_old - zb_old, 0.0) * p.A;
+        acc[0] += haf_pre * p.A;
+        acc[1] += H * p.A;
+        acc[2] += fmax(zss_old - zb_old, 0.0) * p.A;
         const double ue_new = p.rigid_litho ? 0.0 : (double)p.ue_raw[c] - oe;
         const double dz_new = p.lv_surface ? (double)p.dz_raw[c] - og : (double)p.dz_ss[c];
         const double zss_new = (double)p.z_ss_ref[c] + dz_new + p.bsl_z;
@@ -802,16 +822,23 @@ __device__ __forceinline__ Acc3 phase_sp_point(const KP& p, const StageDesc& st,
         p.LT[c] = (float)(ca_litho - p.rho_um * (double)p.u_ref[c]);
         p.zoff[c] = (float)((double)p.z_b_ref[c] + ue_new - zss_new);
         p.zoff_trig[c] = (float)((double)p.z_b_ref[c] + ue_old - zss_new);
+      }
+      block_sum<3>(red, acc);
+      if (threadIdx.x == 0) {
+          p.vpart[4 * item + 0] = acc[0];
+          p.vpart[4 * item + 1] = acc[1];
+          p.vpart[4 * item + 2] = acc[2];
+      }
     }
-    return acc;
 }
 
 // ------------------------------------------------------------------------------------------
 // one RHS call
 // ------------------------------------------------------------------------------------------
 template <typename MT>
-__device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, unsigned& gen, const float2* twx, const float2* twy,
-                         char* smem, double* red, int* flag, bool sparse_only, unsigned long long& tl) {
+__device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, StageDesc& st, unsigned& gen,
+                                         const float2* twx, const float2* twy, char* smem, double* red, int* flag,
+                                         bool sparse_only, unsigned long long& tl) {
     st.sb_out = (int)(s.nrhs & 1);
     st.sb_in = 1 - st.sb_out;   // the previous call's output
     KC4 kc;
@@ -823,8 +850,8 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
         // derivative first and treat it as a global term
         StageDesc sk = st;
         sk.final_err = 0;
-        (void)phase_c2r(p, sk, specbuf(p, st.sb_in), st.kin, twx, smem);
-        if (!grid_sync(p, gen, flag, TAG_KIN, tl)) return false;
+        phase_c2r(p, tm, sk, specbuf(p, st.sb_in), st.kin, twx, smem, red);
+        if (!grid_sync(p, tm, gen, flag, TAG_KIN, tl)) return false;
         if (st.ng == 0) { st.g0 = st.kin; st.a0 = st.ain; }
         else if (st.ng == 1) { st.g1 = st.kin; st.a1 = st.ain; }
         else { st.g2 = st.kin; st.a2 = st.ain; }
@@ -840,23 +867,16 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
     if (st.kin >= 0) kc = spectral_corners(p, specbuf(p, st.sb_in), twx, red);
     const double uoff = u_offset(p, st, kc);
     if (trig) {
-        phase_sp_rows(p, smem);
-        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
-        phase_sp_cols(p, smem);
-        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
-        phase_sp_c2r(p, smem);
-        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
-        const Acc3 a3 = phase_sp_point(p, st, uoff, it);
-        double acc[3] = {a3.a, a3.b, a3.c};
-        block_sum<3>(red, acc);
-        if (threadIdx.x == 0) {
-            p.vpart[4 * blockIdx.x + 0] = acc[0];
-            p.vpart[4 * blockIdx.x + 1] = acc[1];
-            p.vpart[4 * blockIdx.x + 2] = acc[2];
-        }
-        if (!grid_sync(p, gen, flag, TAG_SPARSE, tl)) return false;
+        phase_sp_rows(p, tm, smem);
+        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+        phase_sp_cols(p, tm, smem);
+        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+        phase_sp_c2r(p, tm, smem);
+        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+        phase_sp_point(p, tm, st, uoff, it, red);
+        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
         double vs[3] = {0.0, 0.0, 0.0};
-        for (int w = threadIdx.x; w < (int)gridDim.x; w += NT) {
+        for (int w = threadIdx.x; w < p.nrowitems; w += NT) {
             vs[0] += p.vpart[4 * w];
             vs[1] += p.vpart[4 * w + 1];
             vs[2] += p.vpart[4 * w + 2];
@@ -877,28 +897,20 @@ __device__ __forceinline__ bool rhs_call(const KP& p, Ctl& s, StageDesc& st, uns
     }
     // store this call's loads if the next call may trigger (the sparse update reads the
     // column anomalies of the previous call, src/simulation.jl:362-385)
-    st.write_sigma = (p.write_sigma_always || (((st.t_next_max - s.t0) / p.dt_sparse) >= (double)s.count)) ? 1 : 0;
+    st.write_sigma = (st.force_sigma || p.write_sigma_always ||
+                      (((st.t_next_max - s.t0) / p.dt_sparse) >= (double)s.count)) ? 1 : 0;
 #ifdef FI_SUBPHASE_TIMING
     __syncthreads();
-    if (p.ptime && blockIdx.x == 0 && threadIdx.x == 0) p.ptime[23] += wall_clock64() - tl;
+    if (p.ptime && tm.rank == 0 && threadIdx.x == 0) p.ptime[23] += wall_clock64() - tl;
 #endif
-    if (sizeof(MT) == 8) phase_rows<MT, 4, 2>(p, st, uoff, it, twx, smem, kc);
-    else phase_rows<MT, 8, 4>(p, st, uoff, it, twx, smem, kc);
-    if (!grid_sync(p, gen, flag, TAG_ROWS, tl)) return false;
-    phase_cols(p, specbuf(p, st.sb_out), twy, smem);
-    if (!grid_sync(p, gen, flag, TAG_COLS, tl)) return false;
+    if (sizeof(MT) == 8) phase_rows<MT, 4, 2>(p, tm, st, uoff, it, twx, smem, kc);
+    else phase_rows<MT, 8, 4>(p, tm, st, uoff, it, twx, smem, kc);
+    if (!grid_sync(p, tm, gen, flag, TAG_ROWS, tl)) return false;
+    phase_cols(p, tm, specbuf(p, st.sb_out), twy, smem);
+    if (!grid_sync(p, tm, gen, flag, TAG_COLS, tl)) return false;
     if (st.do_c2r) {
-        const Acc3 a3 = phase_c2r(p, st, specbuf(p, st.sb_out), st.kout, twx, smem);
-        if (st.final_err) {
-            double acc[3] = {a3.a, a3.b, a3.c};
-            block_sum<3>(red, acc);
-            if (threadIdx.x == 0) {
-                p.part[4 * blockIdx.x + 0] = acc[0];
-                p.part[4 * blockIdx.x + 1] = acc[1];
-                p.part[4 * blockIdx.x + 2] = acc[2];
-            }
-        }
-        if (!grid_sync(p, gen, flag, TAG_C2R, tl)) return false;
+        phase_c2r(p, tm, st, specbuf(p, st.sb_out), st.kout, twx, smem, red);
+        if (!grid_sync(p, tm, gen, flag, TAG_C2R, tl)) return false;
     }
     if (threadIdx.x == 0) {
         s.nrhs += 1;

@@ -25,3 +25,4 @@ from .engine import Engine, EngineError
 run_ = run
 step_ = step
 update_diagnostics_ = update_diagnostics
+from .ensemble import Ensemble

@@ -10,7 +10,7 @@ import os
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ENGINE_PATH = os.path.join(PKG_ROOT, "lib", "libfastiso_hip.so")
 
-FI_ABI_VERSION = 1
+FI_ABI_VERSION = 2
 
 # status codes
 FI_OK = 0
@@ -73,6 +73,16 @@ class FiStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class FiBatchInfo(C.Structure):
+    _fields_ = [
+        ("n_members", C.c_int32), ("team_size", C.c_int32), ("n_teams", C.c_int32), ("tuned", C.c_int32),
+        ("device_ms", C.c_double), ("n_launch", C.c_int64), ("cell_attempts", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 OUTPUT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
 
 _dp = C.POINTER(C.c_double)
@@ -100,6 +110,13 @@ def declare(lib):
         "fi_integrator_init": (C.c_int, [P]),
         "fi_integrate": (C.c_int, [P, C.c_double, OUTPUT_CB, C.c_void_p]),
         "fi_get_stats": (C.c_int, [P, C.POINTER(FiStats)]),
+        "fi_create_batch": (C.c_int, [C.POINTER(FiConfig), C.c_int32, C.POINTER(P)]),
+        "fi_batch_member": (P, [P, C.c_int32]),
+        "fi_batch_destroy": (None, [P]),
+        "fi_batch_last_error": (C.c_char_p, [P]),
+        "fi_batch_set_team_size": (C.c_int, [P, C.c_int32]),
+        "fi_batch_integrate": (C.c_int, [P, C.c_double, C.c_int32]),
+        "fi_batch_get_info": (C.c_int, [P, C.POINTER(FiBatchInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

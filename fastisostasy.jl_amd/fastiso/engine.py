@@ -48,9 +48,22 @@ class Engine:
         self.h = h
         self._cb = None
 
+    @classmethod
+    def member(cls, lib, cfg, handle):
+        """Wrap a context owned by an ensemble (fi_batch_member); closing it is the ensemble's job."""
+        e = cls.__new__(cls)
+        e.lib, e.cfg = lib, cfg
+        e.nx, e.ny = cfg.nx, cfg.ny
+        e.N = cfg.nx * cfg.ny
+        e.h = C.c_void_p(handle)
+        e._cb = None
+        e._owned = False
+        return e
+
     def close(self):
         if getattr(self, "h", None):
-            self.lib.fi_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.fi_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -86,3 +86,19 @@ def lv_elva(n=350, W=3.15e6, t_end=120e3, seeds=(1001, 1002, 1003), separable=Tr
     nout = NativeOutput(t=t_out, vars=["u"])
     return dict(domain=domain, bcs=bcs, sealevel=sl, solidearth=se, t_span=(0.0, t_end), nout=nout,
                 opts=SolverOptions(), c=PhysicalConstants(), logeta=logeta, h=h, H_dome=Hd)
+
+
+def ensemble(nmem=256, W=3e6, t_end=20e3, sizes=(64, 96, 128, 192, 256), member_seed=2000, eta_seed=3000,
+             litho_seed=4000, dome_seed=1003, separable=True):
+    """Uneven ensemble: member m has side n_m drawn uniformly from sizes (seed member_seed + m), its
+    own log10 viscosity and lithosphere fields as in lv_elva (seeds eta_seed + m, litho_seed + m),
+    and the shared dome field (dome_seed) evaluated on its grid. All members span the same
+    2W x 2W square."""
+    out = []
+    for m in range(nmem):
+        n = int(np.random.default_rng(member_seed + m).choice(sizes))
+        w = lv_elva(n=n, W=W, t_end=t_end, seeds=(eta_seed + m, litho_seed + m, dome_seed), separable=separable,
+                    t_out=[])
+        w["member"] = m
+        out.append(w)
+    return out

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FI_ABI_VERSION 1
+#define FI_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------------------------ */
 enum {
@@ -229,6 +229,43 @@ int fi_integrator_init(fi_ctx* ctx);
 int fi_integrate(fi_ctx* ctx, double t_target, fi_output_cb cb, void* user);
 
 int fi_get_stats(fi_ctx* ctx, fi_stats* out);
+
+/* ---- ensembles of independent simulations (uneven grids) ----------------------------------
+ * The reference runs an ensemble as one deep copy of the Simulation per member, integrated in a
+ * threaded loop (ext/FastIsostasyEnsembleKalmanProcessesExt.jl:103-122). Here one persistent
+ * launch advances every member: the device is split into teams of workgroups and each team
+ * integrates members taken from a queue, largest first. */
+typedef struct fi_batch fi_batch;
+
+typedef struct fi_batch_info {
+    int32_t n_members;
+    int32_t team_size;     /* workgroups per team of the last launch */
+    int32_t n_teams;
+    int32_t tuned;         /* 1 once the team size is fixed (given, or chosen by timing) */
+    double device_ms;      /* kernel time of all ensemble launches (HIP events) */
+    int64_t n_launch;
+    double cell_attempts;  /* sum over members of BS3 attempts x grid cells, all launches */
+} fi_batch_info;
+
+/* Creates nmem member contexts, member m as fi_create(&cfgs[m]). Members are set up, initialised
+ * and read through fi_batch_member(m) with the single-context calls (fi_set_field, fi_set_ice_*,
+ * fi_init, fi_integrator_init, fi_get_field, fi_get_stats); they are owned by the batch. */
+int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out);
+fi_ctx* fi_batch_member(fi_batch* b, int32_t m);
+void fi_batch_destroy(fi_batch* b);
+const char* fi_batch_last_error(const fi_batch* b);
+
+/* Workgroups per member team. 0 (the default) times the candidate sizes on the first launches of
+ * fi_batch_integrate and keeps the fastest. Results do not depend on the team size. */
+int fi_batch_set_team_size(fi_batch* b, int32_t team_size);
+
+/* Advances every member (after fi_integrator_init) to t_target with its own adaptive BS3 steps.
+ * Member output times are not used. fsal = 1 then re-evaluates each member's first-same-as-last
+ * derivative at t_target, as fi_integrate does after an output callback (src/simulation.jl:
+ * 194-225): a member then matches a single context integrated with t_target among its output
+ * times. */
+int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal);
+int fi_batch_get_info(const fi_batch* b, fi_batch_info* out);
 
 #ifdef __cplusplus
 }

@@ -1127,3 +1127,93 @@ double ref_bs3_linear(double lam, double u0, double t_end, double reltol, double
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Ensembles: the reference integrates one deep copy of the Simulation per member in a threaded
+// loop (ext/FastIsostasyEnsembleKalmanProcessesExt.jl:103-122); restated here as a loop over
+// independent member contexts (OpenMP inside each member's RHS).
+// ------------------------------------------------------------------------------------------
+struct fi_batch {
+    std::vector<fi_ctx*> mem;
+    std::string err;
+    int64_t n_launch = 0;
+    double cell_attempts = 0;
+};
+
+extern "C" {
+
+int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out) {
+    if (!cfgs || !out || nmem < 1) return FI_ERR_INVALID;
+    *out = nullptr;
+    fi_batch* b = new fi_batch();
+    for (int m = 0; m < nmem; ++m) {
+        fi_ctx* c = nullptr;
+        int st = fi_create(&cfgs[m], &c);
+        if (st) {
+            fi_batch_destroy(b);
+            return st;
+        }
+        b->mem.push_back(c);
+    }
+    *out = b;
+    return FI_OK;
+}
+
+fi_ctx* fi_batch_member(fi_batch* b, int32_t m) {
+    if (!b || m < 0 || m >= (int32_t)b->mem.size()) return nullptr;
+    return b->mem[m];
+}
+
+void fi_batch_destroy(fi_batch* b) {
+    if (!b) return;
+    for (fi_ctx* c : b->mem) fi_destroy(c);
+    delete b;
+}
+
+const char* fi_batch_last_error(const fi_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int fi_batch_set_team_size(fi_batch* b, int32_t ts) { return (b && ts >= 0) ? FI_OK : FI_ERR_INVALID; }
+
+int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
+    if (!b) return FI_ERR_INVALID;
+    for (size_t m = 0; m < b->mem.size(); ++m) {
+        fi_ctx* c = b->mem[m];
+        if (!c->integ) {
+            b->err = "member " + std::to_string(m) + ": fi_batch_integrate before fi_integrator_init";
+            return FI_ERR_STATE;
+        }
+        if (c->t >= t_target) continue;
+        const int64_t a0 = c->nacc + c->nrej;
+        std::vector<double> tout;
+        tout.swap(c->tout);   // member output times are not used
+        const int64_t kout = c->kout;
+        int st = fi_integrate(c, t_target, nullptr, nullptr);
+        c->tout.swap(tout);
+        c->kout = kout;
+        if (!st && fsal) {
+            st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t);
+            c->u = c->uprev;
+        }
+        if (st) {
+            b->err = "member " + std::to_string(m) + ": " + c->err;
+            return st;
+        }
+        b->cell_attempts += (double)(c->nacc + c->nrej - a0) * (double)c->N;
+    }
+    b->n_launch += 1;
+    return FI_OK;
+}
+
+int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {
+    if (!b || !o) return FI_ERR_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->n_members = (int32_t)b->mem.size();
+    o->team_size = 0;
+    o->n_teams = 0;
+    o->tuned = 1;
+    o->n_launch = b->n_launch;
+    o->cell_attempts = b->cell_attempts;
+    return FI_OK;
+}
+
+}  // extern "C"

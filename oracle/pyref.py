@@ -65,6 +65,11 @@ def lib():
             "fi_integrator_init": (C.c_int, [P]),
             "fi_integrate": (C.c_int, [P, C.c_double, CB, C.c_void_p]),
             "fi_get_stats": (C.c_int, [P, C.POINTER(RefStats)]),
+            "fi_create_batch": (C.c_int, [P, C.c_int32, C.POINTER(P)]),
+            "fi_batch_member": (P, [P, C.c_int32]),
+            "fi_batch_destroy": (None, [P]),
+            "fi_batch_last_error": (C.c_char_p, [P]),
+            "fi_batch_integrate": (C.c_int, [P, C.c_double, C.c_int32]),
             "ref_fft": (None, [C.c_int, _dp, _dp, _dp, _dp, C.c_int]),
             "ref_derivatives": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]),
             "ref_conv_cropped": (None, [C.c_int, C.c_int, _dp, _dp, _dp]),
@@ -114,8 +119,8 @@ class RefEngine:
         self._cb = None
 
     @classmethod
-    def from_inputs(cls, inputs):
-        e = cls(inputs["cfg"])
+    def from_inputs(cls, inputs, handle=None):
+        e = cls(inputs["cfg"]) if handle is None else cls._member(inputs["cfg"], handle)
         for name, X in inputs["fields"].items():
             e.set_field(name, X)
         ice = inputs["ice"]
@@ -133,9 +138,21 @@ class RefEngine:
         e._ck(e.L.fi_init(e.h, *inputs["t_span"]))
         return e
 
+    @classmethod
+    def _member(cls, cfg, handle):
+        e = cls.__new__(cls)
+        e.L = lib()
+        e.nx, e.ny = cfg.nx, cfg.ny
+        e.N = cfg.nx * cfg.ny
+        e.h = C.c_void_p(handle)
+        e._cb = None
+        e._owned = False
+        return e
+
     def __del__(self):
         if getattr(self, "h", None):
-            self.L.fi_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.L.fi_destroy(self.h)
             self.h = None
 
     def _ck(self, st):
@@ -172,6 +189,45 @@ class RefEngine:
         s = RefStats()
         self._ck(self.L.fi_get_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+class RefEnsemble:
+    """fi_create_batch of the oracle: its members integrated one after the other."""
+
+    def __init__(self, inputs_list):
+        self.L = lib()
+        n = len(inputs_list)
+        cfg_t = type(inputs_list[0]["cfg"])
+        cfgs = (cfg_t * n)(*[inp["cfg"] for inp in inputs_list])
+        h = C.c_void_p()
+        st = self.L.fi_create_batch(C.cast(cfgs, C.c_void_p), n, C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"oracle fi_create_batch failed: {st}")
+        self.h = h
+        self.members = [RefEngine.from_inputs(inp, handle=self.L.fi_batch_member(h, m))
+                        for m, inp in enumerate(inputs_list)]
+
+    def __getitem__(self, m):
+        return self.members[m]
+
+    def __len__(self):
+        return len(self.members)
+
+    def integrator_init(self):
+        for e in self.members:
+            e.integrator_init()
+
+    def integrate(self, t_target, fsal=True):
+        st = self.L.fi_batch_integrate(self.h, float(t_target), int(bool(fsal)))
+        if st != 0:
+            raise RuntimeError(f"oracle error {st}: {self.L.fi_batch_last_error(self.h).decode()}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            for e in self.members:
+                e.h = None
+            self.L.fi_batch_destroy(self.h)
+            self.h = None
 
 
 # ---- ref_* hooks ---------------------------------------------------------------------------

@@ -45,14 +45,16 @@ def test_oracle_exports_all_symbols(ref):
 def test_struct_layout_matches_header(tmp_path):
     prog = tmp_path / "layout.c"
     prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "fastiso.h"\n'
-                    'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(fi_config), sizeof(fi_stats),'
-                    ' offsetof(fi_config, dt_sparse), offsetof(fi_config, device), offsetof(fi_stats, last_EEst));'
+                    'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(fi_config), sizeof(fi_stats),'
+                    ' offsetof(fi_config, dt_sparse), offsetof(fi_config, device), offsetof(fi_stats, last_EEst),'
+                    ' sizeof(fi_batch_info), offsetof(fi_batch_info, cell_attempts));'
                     'return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)])
     out = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     assert out == [ctypes.sizeof(_abi.FiConfig), ctypes.sizeof(_abi.FiStats), _abi.FiConfig.dt_sparse.offset,
-                   _abi.FiConfig.device.offset, _abi.FiStats.last_EEst.offset]
+                   _abi.FiConfig.device.offset, _abi.FiStats.last_EEst.offset, ctypes.sizeof(_abi.FiBatchInfo),
+                   _abi.FiBatchInfo.cell_attempts.offset]
 
 
 def test_default_config_matches_reference_defaults(ref):

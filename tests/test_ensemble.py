@@ -1,0 +1,155 @@
+"""Uneven ensembles (fi_create_batch / fi_batch_integrate).
+
+The ensemble engine splits the device into teams of workgroups that integrate members taken from
+a queue. Every cross-workgroup sum runs over fixed per-row-pair partials, so a member's trajectory
+cannot depend on the team size or on which team ran it: members are compared bit for bit with
+single contexts (fi_integrate with the same output times) and across team sizes, and within the
+trajectory tolerance with the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from fastiso import workloads
+
+from _cases import DRYRUN, sim_inputs
+
+T_OUT = (40.0, 100.0)
+
+
+def small_ensemble(nmem=6, t_end=100.0):
+    ws = workloads.ensemble(nmem=nmem, sizes=(32, 48, 64), t_end=t_end)
+    return [sim_inputs(w).inputs for w in ws]
+
+
+def with_tout(inp, tout):
+    d = dict(inp)
+    d["tout"] = list(tout)
+    return d
+
+
+# ---- CPU: workload generator and the oracle's ensemble ----------------------------------------
+def test_ensemble_workload_is_seeded_and_uneven():
+    ws = workloads.ensemble(nmem=40, t_end=20e3)
+    sizes = [w["domain"].nx for w in ws]
+    assert set(sizes) <= {64, 96, 128, 192, 256} and len(set(sizes)) >= 4
+    again = workloads.ensemble(nmem=3, t_end=20e3)
+    for a, b in zip(ws, again):
+        assert a["domain"].nx == b["domain"].nx
+        np.testing.assert_array_equal(a["logeta"], b["logeta"])
+    # one shared dome field: members of the same size carry the same load, fields differ
+    by_n = {}
+    for w in ws:
+        by_n.setdefault(w["domain"].nx, []).append(w)
+    pairs = [v for v in by_n.values() if len(v) >= 2]
+    assert pairs
+    a, b = pairs[0][:2]
+    np.testing.assert_array_equal(a["H_dome"], b["H_dome"])
+    assert not np.array_equal(a["logeta"], b["logeta"])
+
+
+def test_oracle_ensemble_equals_single_contexts(ref):
+    inputs = small_ensemble(nmem=4)
+    ens = ref.RefEnsemble(inputs)
+    ens.integrator_init()
+    for t in T_OUT:
+        ens.integrate(t)
+    for m, inp in enumerate(inputs):
+        e = ref.RefEngine.from_inputs(with_tout(inp, T_OUT))
+        e.integrator_init()
+        e.integrate(T_OUT[-1], lambda t, k: None)
+        np.testing.assert_array_equal(e.get_field("u"), ens[m].get_field("u"))
+        assert e.stats()["n_accept"] == ens[m].stats()["n_accept"]
+
+
+# ---- GPU --------------------------------------------------------------------------------------
+def hip_ensemble(inputs, team_size):
+    if DRYRUN:
+        from oracle import pyref
+        return pyref.RefEnsemble(inputs)
+    import fastiso
+    return fastiso.Ensemble(inputs, team_size=team_size)
+
+
+def hip_single(inp):
+    if DRYRUN:
+        from oracle import pyref
+        return pyref.RefEngine.from_inputs(inp)
+    from fastiso.engine import Engine
+    from fastiso.simulation import push_inputs
+    return push_inputs(Engine(inp["cfg"]), inp)
+
+
+def run_ensemble(inputs, team_size):
+    ens = hip_ensemble(inputs, team_size)
+    ens.integrator_init()
+    for t in T_OUT:
+        ens.integrate(t)
+    out = [(ens[m].get_field("u"), ens[m].get_field("ue"), ens[m].stats()) for m in range(len(inputs))]
+    info = None if DRYRUN else ens.info()
+    return out, info
+
+
+@pytest.mark.gpu
+def test_ensemble_members_equal_single_contexts_bitwise():
+    inputs = small_ensemble()
+    res, info = run_ensemble(inputs, team_size=4)
+    for m, inp in enumerate(inputs):
+        e = hip_single(with_tout(inp, T_OUT))
+        e.integrator_init()
+        e.integrate(T_OUT[-1], lambda t, k: None)
+        u, ue, st = res[m]
+        np.testing.assert_array_equal(u, e.get_field("u"), err_msg=f"member {m}")
+        np.testing.assert_array_equal(ue, e.get_field("ue"), err_msg=f"member {m}")
+        s1 = e.stats()
+        assert (st["n_accept"], st["n_reject"], st["n_rhs"]) == (s1["n_accept"], s1["n_reject"], s1["n_rhs"])
+        assert st["t"] == T_OUT[-1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("team_size", [1, 3, 0])
+def test_ensemble_independent_of_team_size(team_size):
+    inputs = small_ensemble()
+    ref_res, _ = run_ensemble(inputs, team_size=8)
+    res, info = run_ensemble(inputs, team_size=team_size)
+    for m in range(len(inputs)):
+        np.testing.assert_array_equal(res[m][0], ref_res[m][0], err_msg=f"member {m}")
+    if info is not None:
+        assert info["n_members"] == len(inputs) and info["n_launch"] >= 1
+        if team_size:
+            assert info["tuned"] == 1 and info["team_size"] == team_size
+        # with team_size 0 this short run may end before every candidate size has been timed
+
+
+def scaled_reltol(inp, factor):
+    import ctypes
+    cfg = type(inp["cfg"])()
+    ctypes.memmove(ctypes.byref(cfg), ctypes.byref(inp["cfg"]), ctypes.sizeof(cfg))
+    cfg.reltol *= factor
+    d = dict(inp)
+    d["cfg"] = cfg
+    return d
+
+
+@pytest.mark.gpu
+def test_ensemble_vs_oracle():
+    """Early in the glacial cycle |u| is a few cm and two valid BS3 integrations differ at the
+    level of the step-size tolerance: the bound is three times the oracle's own change when its
+    reltol moves from 1e-5 to 0.9e-5, plus 1e-4 of max |u|."""
+    from oracle import pyref
+    inputs = small_ensemble()
+    res, _ = run_ensemble(inputs, team_size=0)
+    ref = pyref.RefEnsemble(inputs)
+    ref.integrator_init()
+    alt = pyref.RefEnsemble([scaled_reltol(inp, 0.9) for inp in inputs])
+    alt.integrator_init()
+    for t in T_OUT:
+        ref.integrate(t)
+        alt.integrate(t)
+    for m in range(len(inputs)):
+        b, eb = ref[m].get_field("u"), ref[m].get_field("ue")
+        tol_u = 3 * np.abs(b - alt[m].get_field("u")).max() + 1e-4 * np.abs(b).max() + 1e-6
+        tol_e = 3 * np.abs(eb - alt[m].get_field("ue")).max() + 1e-4 * np.abs(eb).max() + 1e-6
+        u, ue, st = res[m]
+        assert np.abs(u - b).max() <= tol_u, (m, np.abs(u - b).max(), tol_u)
+        assert np.abs(ue - eb).max() <= tol_e, (m, np.abs(ue - eb).max(), tol_e)
+        assert st["count_sparse_updates"] == ref[m].stats()["count_sparse_updates"]
