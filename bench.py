@@ -13,6 +13,11 @@ roofline: the dominant kernel is the persistent integration kernel. Its algorith
 unit); achieved = that traffic / the kernel time measured with HIP events on the engine's stream.
 cpu_baseline: the FP64 CPU restatement of the reference path (oracle/, OpenMP) on this host, on
 the same workload, over a bounded window.
+
+Other workloads (--workload): lv_elva_hr is the N=1024 high-resolution instance (W = 3000 km,
+dx = 5.86 km, field seeds 5001/5002); ensemble is the uneven ensemble of 256 members with sides
+drawn from {64, 96, 128, 192, 256}, advanced together by fi_batch_integrate (value = member-kyr/s,
+the sum over members of simulated kyr per wall-clock second).
 """
 import argparse
 import json
@@ -27,19 +32,82 @@ BYTES_PER_CELL_STEP = 171.0
 HBM_PEAK_GBS = 8000.0
 
 
-def make_sim(n, engine):
+def make_sim(n, engine, hr=False):
     import fastiso
     from fastiso import workloads
-    w = workloads.lv_elva(n=n, t_end=120e3, t_out=[])
+    if hr:
+        w = workloads.lv_elva(n=n, W=3e6, t_end=120e3, seeds=(5001, 5002, 1003), t_out=[])
+    else:
+        w = workloads.lv_elva(n=n, t_end=120e3, t_out=[])
     return w, fastiso.Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
                                  nout=w["nout"], opts=w["opts"], c=w["c"], engine=engine)
 
 
-def cpu_baseline(n, window, sample_s):
+def ensemble_inputs(nmem):
+    import fastiso
+    from fastiso import workloads
+    ws = workloads.ensemble(nmem=nmem, t_end=20e3)
+    return [fastiso.Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
+                               nout=w["nout"], opts=w["opts"], c=w["c"], engine=None).inputs for w in ws]
+
+
+def bench_ensemble(a):
+    import fastiso
+    t0 = time.perf_counter()
+    inputs = ensemble_inputs(a.members)
+    ens = fastiso.Ensemble(inputs, team_size=a.team_size)
+    ens.integrator_init()
+    setup_s = time.perf_counter() - t0
+    print(f"ensemble of {len(inputs)} members set up in {setup_s:.1f} s", file=sys.stderr, flush=True)
+    cells = [inp["cfg"].nx * inp["cfg"].ny for inp in inputs]
+    t = 0.0
+    for k in range(a.warmup):
+        t += a.window
+        ens.integrate(t, fsal=False)
+        print(f"warm-up window {k + 1}/{a.warmup} done: {ens.info()}", file=sys.stderr, flush=True)
+    acc0 = [ens[m].stats()["n_accept"] for m in range(len(inputs))]
+    i0 = ens.info()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        t += a.window
+        ens.integrate(t, fsal=False)
+        print(f"window {k + 1}/{a.steps} done", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    i1 = ens.info()
+    acc = [ens[m].stats()["n_accept"] - acc0[m] for m in range(len(inputs))]
+    cell_steps = float(sum(c * n for c, n in zip(cells, acc)))
+    dev_s = (i1["device_ms"] - i0["device_ms"]) / 1e3
+    member_kyr = len(inputs) * a.steps * a.window / 1e3
+    achieved = BYTES_PER_CELL_STEP * cell_steps / dev_s / 1e9 if dev_s > 0 else None
+    sizes = sorted(set(inp["cfg"].nx for inp in inputs))
+    return {
+        "metric": "simulated member-kyr/s, 256-member uneven ensemble (plain solver)",
+        "value": member_kyr / el,
+        "unit": "member-kyr/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fft/stencil-moments, f64 state/stencil/error-norm",
+        "data": "synthetic (seeded fields)",
+        "config": {"workload": "ensemble", "members": len(inputs), "sizes": sizes, "total_cells": int(sum(cells)),
+                   "window_yr": a.window, "integrator": "BS3 reltol 1e-5 abstol 1e-6",
+                   "accepted_steps_total": int(sum(acc)), "cell_steps": cell_steps,
+                   "team_size": i1["team_size"], "n_teams": i1["n_teams"], "autotuned": a.team_size == 0,
+                   "kernel_launches": int(i1["n_launch"] - i0["n_launch"]), "setup_s": setup_s},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+    }
+
+
+def cpu_baseline(n, window, sample_s, hr=False):
     threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
     os.environ["OMP_NUM_THREADS"] = threads
     from oracle import pyref
-    w, sim = make_sim(n, engine=None)
+    w, sim = make_sim(n, engine=None, hr=hr)
     e = pyref.RefEngine.from_inputs(sim.inputs)
     e.integrator_init()
     e.integrate(window)                 # same warm-up as the GPU leg: the first window
@@ -64,14 +132,23 @@ def main():
     ap.add_argument("--window", type=float, default=10.0)
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble"], default="lv_elva")
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--team-size", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "ensemble":
+        print(json.dumps(bench_ensemble(a)), flush=True)
+        return
+    hr = a.workload == "lv_elva_hr"
+    if hr and a.n == 350:
+        a.n = 1024
     # the engine owns its device memory and stream; every fi_integrate call returns after its
     # stream has synchronised, so wall-clock brackets below time completed device work
-    w, sim = make_sim(a.n, engine="hip")
+    w, sim = make_sim(a.n, engine="hip", hr=hr)
     eng = sim.engine
     eng.integrator_init()
     t = 0.0
@@ -92,7 +169,7 @@ def main():
     launches = s1["n_launch"] - s0["n_launch"]
     achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
     out = {
-        "metric": "simulated kyr/s at N=350 LV-ELVA (plain solver)",
+        "metric": f"simulated kyr/s at N={a.n} LV-ELVA (plain solver)",
         "value": kyr / el,
         "unit": "kyr/s",
         "n_gpus": 1,
@@ -104,7 +181,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 fft/stencil-moments, f64 state/stencil/error-norm",
         "data": "synthetic (seeded fields)",
-        "config": {"workload": f"lv_elva_n{a.n}", "n": a.n, "dx_km": 6300.0 / a.n, "window_yr": a.window,
+        "config": {"workload": f"{a.workload}_n{a.n}", "n": a.n, "dx_km": 2 * w["domain"].Wx / a.n / 1e3,
+                   "window_yr": a.window,
                    "integrator": "BS3 reltol 1e-5 abstol 1e-6", "accepted_steps": int(s1["n_accept"] - s0["n_accept"]),
                    "rejected_steps": int(s1["n_reject"] - s0["n_reject"]),
                    "rhs_calls": int(s1["n_rhs"] - s0["n_rhs"]), "sim_t_end_yr": s1["t"],
@@ -114,7 +192,7 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s)
+        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr)
     print(json.dumps(out), flush=True)
 
 
