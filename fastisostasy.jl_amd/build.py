@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "fi_engine.hip")
+SRCS = [os.path.join(HERE, "csrc", "fi_engine.hip"), os.path.join(HERE, "csrc", "fno.hip")]
 OUT = os.path.join(HERE, "lib", "libfastiso_hip.so")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -14,7 +14,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
 def build(verbose=True):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", OUT + ".tmp", SRC]
+    cmd = [hipcc] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", OUT + ".tmp"] + SRCS
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

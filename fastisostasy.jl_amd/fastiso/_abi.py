@@ -125,6 +125,40 @@ def declare(lib):
     return lib
 
 
+class FnoConfig(C.Structure):
+    _fields_ = [
+        ("in_channels", C.c_int32), ("width", C.c_int32), ("modes_x", C.c_int32), ("modes_y", C.c_int32),
+        ("layers", C.c_int32), ("proj_hidden", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+def declare_surrogate(lib):
+    """argtypes/restype of the fi_surrogate_* entry points (include/fastiso_surrogate.h)."""
+    P = C.c_void_p
+    _fp = C.POINTER(C.c_float)
+    sig = {
+        "fi_fno_default_config": (None, [C.POINTER(FnoConfig)]),
+        "fi_surrogate_create": (C.c_int, [C.POINTER(FnoConfig), C.POINTER(P)]),
+        "fi_surrogate_destroy": (None, [P]),
+        "fi_surrogate_last_error": (C.c_char_p, [P]),
+        "fi_surrogate_num_params": (C.c_int64, [P]),
+        "fi_surrogate_get_params": (C.c_int, [P, _fp, C.c_int64]),
+        "fi_surrogate_set_params": (C.c_int, [P, _fp, C.c_int64]),
+        "fi_surrogate_set_grid": (C.c_int, [P, C.c_int32, C.c_int32]),
+        "fi_surrogate_forward": (C.c_int, [P, _fp, _fp]),
+        "fi_surrogate_loss_grad": (C.c_int, [P, _fp, _fp, _dp, _fp]),
+        "fi_surrogate_set_data": (C.c_int, [P, C.c_int32, _fp, _fp]),
+        "fi_surrogate_train": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64, _dp]),
+        "fi_surrogate_device_ms": (C.c_double, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
 _ENGINE = None
 
 
@@ -139,7 +173,7 @@ def load_engine(path=None):
         raise RuntimeError(
             f"HIP engine not built: {p} is missing. Build it with "
             f"`python fastisostasy.jl_amd/build.py` (hipcc --offload-arch=gfx950).")
-    lib = declare(C.CDLL(p))
+    lib = declare_surrogate(declare(C.CDLL(p)))
     if lib.fi_abi_version() != FI_ABI_VERSION:
         raise RuntimeError("libfastiso_hip.so ABI version mismatch")
     if path is None:
