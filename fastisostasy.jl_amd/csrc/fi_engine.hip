@@ -1288,6 +1288,27 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
     return FI_OK;
 }
 
+int fi_set_time(fi_ctx* c, double t) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_set_time before fi_integrator_init");
+    if (!(t >= c->ctl.t)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the current time");
+    (void)hipSetDevice(c->dev);
+    const long long before = c->ctl.count;
+    const long long due = (long long)std::floor((t - c->ctl.t0) / c->cfg.dt_sparse);
+    // store the loads of (t, u) with a call that cannot trigger the sparse update
+    c->ctl.count = std::max(before, due + 1);
+    int st = rhs_on_device(c, t, c->ctl.cur, (c->ctl.k1 + 1) % 4);
+    if (st) return st;
+    // the sparse update at t with those loads; the next one is due one interval later
+    c->ctl.t_rhs = t;
+    if ((st = launch(c, MODE_SPARSE_ONLY, c->ctl.cur, (c->ctl.k1 + 1) % 4, 0, 1))) return st;
+    c->ctl.count = std::max(before, due + 1);
+    c->ctl.t = t;
+    c->ctl.last_u = c->ctl.cur;
+    c->k1_stale = true;
+    return FI_OK;
+}
+
 int fi_get_stats(fi_ctx* c, fi_stats* o) {
     if (!c || !o) return FI_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));
@@ -1356,8 +1377,8 @@ constexpr int TUNE_ATTEMPTS = 64;
 constexpr int LAUNCH_ATTEMPTS = 1 << 20;
 
 // one persistent ensemble launch with teams of ts workgroups; returns the device time
-int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int max_steps, int fsal, double t_target,
-                 float* ms_out, double* work_out) {
+int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int max_steps, int fsal,
+                 const std::vector<double>& t_target, float* ms_out, double* work_out) {
     const int nm = (int)b->mem.size();
     const int nteams = std::max(1, gmax / ts);
     std::vector<KP> kph(nm);
@@ -1365,8 +1386,8 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     std::vector<long long> att0(nm);
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
-        c->ctl.stop = t_target;
-        c->ctl.next_out = t_target;
+        c->ctl.stop = t_target[m];
+        c->ctl.next_out = t_target[m];
         att0[m] = c->ctl.nacc + c->ctl.nrej;
         kph[m] = c->kp;
         kph[m].mode = MODE_INTEGRATE;
@@ -1495,14 +1516,27 @@ int fi_batch_set_team_size(fi_batch* b, int32_t ts) {
 
 int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
     if (!b) return FI_ERR_INVALID;
+    std::vector<double> tt(b->mem.size(), t_target);
+    return fi_batch_integrate_each(b, tt.data(), fsal);
+}
+
+int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) {
+    if (!b || !t_targets) return FI_ERR_INVALID;
     const int nm = (int)b->mem.size();
+    const std::vector<double> t_target(t_targets, t_targets + nm);
     bool mt_double = true;
     size_t lds_d = 0, lds_f = 0;
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
         if (!c->integ) return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": fi_batch_integrate before fi_integrator_init");
-        if (t_target < c->ctl.t) return bfail(b, FI_ERR_INVALID, "member " + std::to_string(m) + ": target before current time");
-        if (c->k1_stale) return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": state written during integration");
+        if (t_target[m] < c->ctl.t) return bfail(b, FI_ERR_INVALID, "member " + std::to_string(m) + ": target before current time");
+        if (c->k1_stale) {
+            // state written since the last step: re-evaluate the FSAL derivative as fi_integrate does
+            const int st = rhs_on_device(c, c->ctl.t, c->ctl.cur, c->ctl.k1);
+            if (st) return bfail(b, st, "member " + std::to_string(m) + ": " + c->err);
+            c->ctl.last_u = c->ctl.cur;
+            c->k1_stale = false;
+        }
         mt_double = mt_double && c->mt_double;
         lds_d = std::max(lds_d, c->lds_d);
         lds_f = std::max(lds_f, c->lds_f);
@@ -1531,8 +1565,8 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
         b->cand_rate.assign(b->cands.size(), 0.0);
     }
     auto done = [&]() {
-        for (fi_ctx* c : b->mem)
-            if (c->ctl.t < t_target) return false;
+        for (int m = 0; m < nm; ++m)
+            if (b->mem[m]->ctl.t < t_target[m]) return false;
         return true;
     };
     while (!done()) {

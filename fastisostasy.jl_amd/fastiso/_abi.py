@@ -110,12 +110,14 @@ def declare(lib):
         "fi_integrator_init": (C.c_int, [P]),
         "fi_integrate": (C.c_int, [P, C.c_double, OUTPUT_CB, C.c_void_p]),
         "fi_get_stats": (C.c_int, [P, C.POINTER(FiStats)]),
+        "fi_set_time": (C.c_int, [P, C.c_double]),
         "fi_create_batch": (C.c_int, [C.POINTER(FiConfig), C.c_int32, C.POINTER(P)]),
         "fi_batch_member": (P, [P, C.c_int32]),
         "fi_batch_destroy": (None, [P]),
         "fi_batch_last_error": (C.c_char_p, [P]),
         "fi_batch_set_team_size": (C.c_int, [P, C.c_int32]),
         "fi_batch_integrate": (C.c_int, [P, C.c_double, C.c_int32]),
+        "fi_batch_integrate_each": (C.c_int, [P, _dp, C.c_int32]),
         "fi_batch_get_info": (C.c_int, [P, C.POINTER(FiBatchInfo)]),
     }
     for name, (res, args) in sig.items():

@@ -123,6 +123,10 @@ class Engine:
         self._cb = cb
         self._check(self.lib.fi_integrate(self.h, float(t_target), cb, None))
 
+    def set_time(self, t):
+        """Move the integrator time to t without integrating (after set_field("u", ...))."""
+        self._check(self.lib.fi_set_time(self.h, float(t)))
+
     def stats(self):
         s = _abi.FiStats()
         self._check(self.lib.fi_get_stats(self.h, C.byref(s)))

@@ -54,6 +54,13 @@ class Ensemble:
         there, as after an output callback."""
         self._check(self.lib.fi_batch_integrate(self.h, float(t_target), int(bool(fsal))))
 
+    def integrate_each(self, t_targets, fsal=True):
+        """Advance member m to t_targets[m]."""
+        import numpy as np
+        tt = np.ascontiguousarray(t_targets, dtype=np.float64)
+        self._check(self.lib.fi_batch_integrate_each(self.h, tt.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     int(bool(fsal))))
+
     def info(self):
         s = _abi.FiBatchInfo()
         self._check(self.lib.fi_batch_get_info(self.h, C.byref(s)))

@@ -120,3 +120,243 @@ class Surrogate:
         self.set_params(d["theta"])
         if d["scales"].size:
             self.scales = tuple(float(v) for v in d["scales"])
+
+
+# ---- the LV-ELVA macro step -------------------------------------------------------------------
+# Fixed physical scales of the network's inputs and output.
+U_SCALE = 500.0          # m, displacement
+S_SCALE = 1.0e6          # kg m^-2, load anomaly (about 1100 m of ice)
+D_SCALE = 2.0            # m per macro step, correction of the extrapolated increment
+LOGETA_MID, LOGETA_HALF = 21.0, 2.0
+H_MID, H_HALF = 150e3, 100e3
+
+
+class MacroStep:
+    """Inputs and scaling of the surrogate for one simulation, and the spin-up step of its
+    training states.
+
+    The network input is [u, load anomaly, log10 viscosity, lithosphere thickness, previous
+    increment] (scaled), the load anomaly being the surface load at the step midpoint plus the
+    lithospheric column anomaly rho_litho (ue - ue_ref) of the current elastic displacement
+    (src/loads.jl:41-44), which is of the size of the disequilibrium that drives the viscous flow.
+    The increment over a macro step of macro_dt years is the previous increment (linear
+    extrapolation in time) plus the network's correction:
+        du = du_prev + D_SCALE * network(input)
+
+    baseline() (used only to spin up training states) blends exact steps of laterally
+    homogeneous Earths: for each (eta_j, D_l) of a lattice
+    spanning the simulation's effective viscosity and rigidity, mode by mode
+        du_k = (u_eq,k - u_k)(1 - exp(-beta_k dt / (2 eta_j kappa_k))),  u_eq,k = -g sigma_k / beta_k,
+        beta_k = rho_um g + D_l kappa_k^4
+    (the LV-ELVA operator of src/deformation.jl:91-131 with constant fields), and every cell takes
+    the bilinear blend in (log10 eta, log10 D) of its own values; the result is corner-corrected
+    like the plain solver's displacement."""
+
+    def __init__(self, domain, solidearth, c, load_fn, macro_dt=500.0, n_eta=10, n_D=3):
+        self.domain, self.c, self.macro_dt = domain, c, float(macro_dt)
+        self.load_fn = load_fn
+        eta = np.asarray(solidearth.effective_viscosity, dtype=np.float64)
+        D = np.asarray(solidearth.litho_rigidity, dtype=np.float64)
+        self.logeta = np.log10(eta)
+        self.h = None
+        self.rho_um = solidearth.rho_uppermantle
+        self.rho_litho = solidearth.rho_litho
+        nx, ny = domain.nx, domain.ny
+        kx = np.pi / domain.Wx * np.fft.fftfreq(nx, d=1.0 / nx)
+        ky = np.pi / domain.Wy * np.fft.rfftfreq(ny, d=1.0 / ny)
+        self.kappa = np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2)
+        logD = np.log10(np.maximum(D, 1.0))
+        self.le = np.linspace(self.logeta.min(), self.logeta.max(), n_eta) if np.ptp(self.logeta) > 0 else \
+            np.array([self.logeta.min()])
+        self.lD = np.linspace(logD.min(), logD.max(), n_D) if np.ptp(logD) > 0 else np.array([logD.min()])
+        self.weights = [self._lin_weights(self.logeta, self.le), self._lin_weights(logD, self.lD)]
+
+    @staticmethod
+    def _lin_weights(v, grid):
+        """Per cell: (index of the lower lattice point, weight of the upper one)."""
+        if grid.size == 1:
+            return np.zeros(v.shape, dtype=int), np.zeros(v.shape)
+        i = np.clip(np.searchsorted(grid, v, side="right") - 1, 0, grid.size - 2)
+        w = np.clip((v - grid[i]) / (grid[i + 1] - grid[i]), 0.0, 1.0)
+        return i, w
+
+    def load(self, t, ue):
+        """Load anomaly driving the step from t: surface load at the midpoint + rho_litho ue."""
+        return self.load_fn(t + 0.5 * self.macro_dt) + self.rho_litho * ue
+
+    def features(self, u, ue, du_prev, t, h_field):
+        sig = self.load(t, ue)
+        return stack_inputs([u / U_SCALE, sig / S_SCALE, (self.logeta - LOGETA_MID) / LOGETA_HALF,
+                             (h_field - H_MID) / H_HALF, du_prev / D_SCALE])
+
+    def _homogeneous(self, U, S, logeta, logD, dt):
+        beta = self.rho_um * self.c.g + 10.0 ** logD * self.kappa ** 4
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ueq = -self.c.g * S / beta
+            rate = beta / (2 * 10.0 ** logeta * self.kappa) * self.c.seconds_per_year
+            fac = -np.expm1(-rate * dt)
+        dU = (ueq - U) * fac
+        dU[0, 0] = 0.0
+        return np.fft.irfft2(dU, s=self.kappa.shape[:1] + (self.domain.ny,))
+
+    def baseline(self, u, ue, t):
+        """Blended homogeneous-Earth increment over [t, t + macro_dt] with the midpoint load."""
+        sig = self.load(t, ue)
+        U, S = np.fft.rfft2(u), np.fft.rfft2(sig)
+        (ie, we), (iD, wD) = self.weights
+        du = np.zeros_like(u)
+        for a in range(self.le.size):
+            for b in range(self.lD.size):
+                wa = np.where(ie == a, 1 - we, 0.0) + np.where(ie == a - 1, we, 0.0) if self.le.size > 1 else 1.0
+                wb = np.where(iD == b, 1 - wD, 0.0) + np.where(iD == b - 1, wD, 0.0) if self.lD.size > 1 else 1.0
+                w = wa * wb
+                if np.ndim(w) and not np.any(w):
+                    continue
+                du += w * self._homogeneous(U, S, self.le[a], self.lD[b], self.macro_dt)
+        return du - 0.25 * (du[0, 0] + du[-1, 0] + du[0, -1] + du[-1, -1])
+
+    def equilibrium(self, t):
+        """Equilibrium displacement under the load at t with the mean rigidity."""
+        S = np.fft.rfft2(self.load_fn(t))
+        beta = self.rho_um * self.c.g + 10.0 ** self.lD.mean() * self.kappa ** 4
+        U = -self.c.g * S / beta
+        U[0, 0] = 0.0
+        u = np.fft.irfft2(U, s=self.kappa.shape[:1] + (self.domain.ny,))
+        return u - 0.25 * (u[0, 0] + u[-1, 0] + u[0, -1] + u[-1, -1])
+
+
+def lv_macro(w, macro_dt=500.0):
+    """MacroStep of an lv_elva workload (separable dome load: sigma = rho_ice a(t) H_dome)."""
+    c = w["c"]
+    it = w["bcs"].ice_thickness
+    tk, ak = np.asarray(it.t_vec), np.asarray(it.a_vec)
+    Hd = np.asarray(it.H_shape)
+
+    def load(t):
+        return c.rho_ice * np.interp(t, tk, ak) * Hd
+
+    m = MacroStep(w["domain"], w["solidearth"], c, load, macro_dt)
+    m.h = np.asarray(w["h"])
+    return m
+
+
+def predict(sur, macro, u, ue, du_prev, t):
+    """Surrogate increment of u over [t, t + macro_dt] (sur None: extrapolation alone)."""
+    if sur is None:
+        return du_prev.copy()
+    x = macro.features(u, ue, du_prev, t, macro.h)
+    r = sur.forward(x).astype(np.float64)
+    nx, ny = u.shape
+    return du_prev + D_SCALE * r.reshape((nx, ny), order="F")
+
+
+class SurrogateRunner:
+    """Surrogate-with-fallback integration of a HIP Simulation in macro steps.
+
+    Every check_every-th macro step (the first included) is integrated by the plain solver and
+    compared with the surrogate's prediction of the same step; if max |du_sur - du_plain| > tol the
+    plain solver takes the following check_every - 1 macro steps as well. Other steps apply the
+    surrogate's increment (fi_set_field(u) + fi_set_time). The first step of a run has no previous
+    increment and is always a plain one."""
+
+    def __init__(self, sim, sur, macro, check_every=10, tol=1.0):
+        self.sim, self.sur, self.macro = sim, sur, macro
+        self.check_every, self.tol = int(check_every), float(tol)
+        self.log = []
+
+    def run(self, t_end, t0=None, progress=None):
+        eng = self.sim.engine
+        dt = self.macro.macro_dt
+        t = eng.stats()["t"] if t0 is None else t0
+        k = 0
+        plain_left = 0
+        n_plain = n_sur = n_fallback = 0
+        du_prev = None
+        while t < t_end - 1e-9:
+            t1 = min(t + dt, t_end)
+            u = eng.get_field("u")
+            ue = eng.get_field("ue")
+            if du_prev is None or plain_left > 0 or t1 - t < dt:
+                eng.integrate(t1)
+                plain_left = max(plain_left - 1, 0)
+                n_plain += 1
+                kind, dev = "plain", None
+                du_prev = eng.get_field("u") - u
+            elif k % self.check_every == 0:
+                du = predict(self.sur, self.macro, u, ue, du_prev, t)
+                eng.integrate(t1)
+                n_plain += 1
+                du_prev = eng.get_field("u") - u
+                dev = float(np.abs(du_prev - du).max())
+                kind = "check"
+                if dev > self.tol:
+                    plain_left = self.check_every - 1
+                    n_fallback += 1
+                    kind = "check-fallback"
+            else:
+                du = predict(self.sur, self.macro, u, ue, du_prev, t)
+                eng.set_field("u", u + du)
+                eng.set_time(t1)
+                n_sur += 1
+                du_prev = du
+                kind, dev = "surrogate", None
+            self.log.append((t, kind, dev))
+            if progress:
+                progress(t1, kind, dev)
+            t = t1
+            k += 1
+        return {"plain_steps": n_plain, "surrogate_steps": n_sur, "fallbacks": n_fallback}
+
+
+def generate_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=3.15e6, t_range=(5e3, 115e3),
+                   team_size=0, spin=6, progress=None):
+    """Training pairs of the plain solver: member m gets C1-style fields and domes (seeds seed0 + m,
+    seed0 + 100000 + m, seed0 + 200000 + m) and a random time t_m of the glacial cycle. Its state at
+    t_m is spun up by spin baseline macro steps from a random fraction of the equilibrium, then the
+    ensemble engine integrates every member steps_per_member macro steps; step k >= 1 gives a pair
+    (input with the increment of step k - 1, scaled correction du_k - du_{k-1}). Returns
+    (X, Y, meta): X (pairs, cells, 5), Y (pairs, cells)."""
+    from . import workloads
+    from .ensemble import Ensemble
+    from .simulation import Simulation
+    rng = np.random.default_rng(seed0)
+    ws, macros, inputs, starts = [], [], [], []
+    for m in range(n_members):
+        w = workloads.lv_elva(n=n, W=W, t_end=120e3, seeds=(seed0 + m, seed0 + 100000 + m, seed0 + 200000 + m),
+                              t_out=[])
+        ws.append(w)
+        macros.append(lv_macro(w, macro_dt))
+        inputs.append(Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
+                                 nout=w["nout"], opts=w["opts"], c=w["c"], engine=None).inputs)
+        starts.append((float(rng.uniform(*t_range)), float(rng.uniform(0.5, 1.0))))
+    ens = Ensemble(inputs, team_size=team_size)
+    ens.integrator_init()
+    for m, (ts, f) in enumerate(starts):
+        t = ts - spin * macro_dt
+        u = f * macros[m].equilibrium(t)
+        ens[m].set_field("u", u)
+        ens[m].set_time(t)
+        for _ in range(spin):
+            u = u + macros[m].baseline(u, ens[m].get_field("ue"), t)
+            t += macro_dt
+            ens[m].set_field("u", u)
+            ens[m].set_time(t)
+    X, Y, meta = [], [], []
+    states = [(ens[m].get_field("u"), ens[m].get_field("ue")) for m in range(n_members)]
+    prev = [None] * n_members
+    for s in range(steps_per_member):
+        ens.integrate_each([starts[m][0] + (s + 1) * macro_dt for m in range(n_members)], fsal=False)
+        for m in range(n_members):
+            t_m = starts[m][0] + s * macro_dt
+            (u0, ue0), u1 = states[m], ens[m].get_field("u")
+            du = u1 - u0
+            if prev[m] is not None:
+                X.append(macros[m].features(u0, ue0, prev[m], t_m, macros[m].h))
+                Y.append(cells((du - prev[m]) / D_SCALE).astype(np.float32))
+                meta.append((m, t_m, float(np.abs(du).max()), float(np.abs(du - prev[m]).max())))
+            prev[m] = du
+            states[m] = (u1, ens[m].get_field("ue"))
+        if progress:
+            progress(s, ens.info())
+    ens.close()
+    return np.array(X), np.array(Y), meta

@@ -1,0 +1,101 @@
+"""Train the LV-ELVA macro-step surrogate on plain-solver trajectories.
+
+    python -m fastiso.train_surrogate --out fno.npz [--members 64 --steps-per-member 3 ...]
+
+1. Training pairs: generate_pairs() integrates C1-style members (seeds from --seed) with the
+   ensemble engine, macro step by macro step, from random phases of the glacial cycle.
+2. Training: Adam on the device (fi_surrogate_train) on the scaled corrections of the
+   extrapolated increment.
+3. Evaluation on held-out members: max and RMS error of the increment with extrapolation alone
+   and with the network, printed with the weights' path.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import surrogate as S
+
+
+def evaluate(sur, X, Y):
+    """Errors (m) of extrapolation alone and of extrapolation + network on scaled pairs (X, Y)."""
+    out = []
+    for x, y in zip(X, Y):
+        r = sur.forward(x) if sur is not None else 0.0 * y
+        e_net = S.D_SCALE * (y - r)
+        e_base = S.D_SCALE * y
+        out.append((float(np.abs(e_base).max()), float(np.sqrt(np.mean(e_base ** 2))),
+                    float(np.abs(e_net).max()), float(np.sqrt(np.mean(e_net ** 2)))))
+    return np.array(out)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--n", type=int, default=350)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--steps-per-member", type=int, default=5)
+    ap.add_argument("--val-members", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=10000)
+    ap.add_argument("--macro-dt", type=float, default=500.0)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--width", type=int, default=32)
+    ap.add_argument("--modes", type=int, default=16)
+    ap.add_argument("--layers", type=int, default=4)
+    a = ap.parse_args(argv)
+
+    def log(*msg):
+        print(*msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    X, Y, meta = S.generate_pairs(a.members, a.seed, a.steps_per_member, a.macro_dt, n=a.n,
+                                  progress=lambda s, i: log(f"train pairs: macro step {s + 1} done "
+                                                            f"({time.perf_counter() - t0:.0f} s) {i}"))
+    Xv, Yv, meta_v = S.generate_pairs(a.val_members, a.seed + 5000, a.steps_per_member, a.macro_dt, n=a.n,
+                                      progress=lambda s, i: log(f"validation pairs done "
+                                                                f"({time.perf_counter() - t0:.0f} s)"))
+    gen_s = time.perf_counter() - t0
+    log(f"{len(X)} training and {len(Xv)} validation pairs in {gen_s:.0f} s")
+    sur = S.Surrogate(width=a.width, modes_x=a.modes, modes_y=a.modes, layers=a.layers, seed=a.seed,
+                      in_channels=X.shape[2])
+    sur.set_grid(a.n, a.n)
+    sur.set_data(X, Y)
+    base_v = evaluate(None, Xv, Yv)
+    t1 = time.perf_counter()
+    losses = []
+    chunk = 200
+    done = 0
+    while done < a.train_steps:
+        k = min(chunk, a.train_steps - done)
+        # one cosine schedule over all steps, run in chunks so progress is printed
+        frac0, frac1 = done / a.train_steps, (done + k) / a.train_steps
+        lr0 = 1e-5 + 0.5 * (a.lr - 1e-5) * (1 + np.cos(np.pi * frac0))
+        lr1 = 1e-5 + 0.5 * (a.lr - 1e-5) * (1 + np.cos(np.pi * frac1))
+        losses += list(sur.train(k, batch=a.batch, lr=lr0, lr_min=lr1, seed=a.seed + done))
+        done += k
+        ev = evaluate(sur, Xv, Yv)
+        log(f"step {done}: train loss {np.mean(losses[-k:]):.4g}  val max err {ev[:, 2].max():.3f} m "
+            f"(extrapolation {base_v[:, 0].max():.3f})  val rms {ev[:, 3].mean():.4f} m (extrapolation {base_v[:, 1].mean():.4f})"
+            f"  {time.perf_counter() - t1:.0f} s")
+    train_s = time.perf_counter() - t1
+    ev = evaluate(sur, Xv, Yv)
+    sur.scales = (S.U_SCALE, S.S_SCALE, S.D_SCALE, a.macro_dt)
+    sur.save(a.out)
+    summary = {
+        "pairs": len(X), "val_pairs": len(Xv), "n": a.n, "macro_dt": a.macro_dt, "generation_s": gen_s,
+        "train_steps": a.train_steps, "batch": a.batch, "train_s": train_s,
+        "surrogate_device_ms": sur.device_ms(),
+        "val_extrapolation_max_m": base_v[:, 0].tolist(), "val_extrapolation_rms_m": base_v[:, 1].tolist(),
+        "val_network_max_m": ev[:, 2].tolist(), "val_network_rms_m": ev[:, 3].tolist(),
+        "val_increment_max_m": [m[2] for m in meta_v],
+        "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:])),
+    }
+    print(json.dumps(summary), flush=True)
+
+
+if __name__ == "__main__":
+    main()

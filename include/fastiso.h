@@ -230,6 +230,14 @@ int fi_integrate(fi_ctx* ctx, double t_target, fi_output_cb cb, void* user);
 
 int fi_get_stats(fi_ctx* ctx, fi_stats* out);
 
+/* Moves the integrator's time forward to t without integrating (after fi_integrator_init), for
+ * drivers that advance the displacement themselves (the surrogate macro step:
+ * fi_set_field(FI_U) then fi_set_time). The column loads are stored at (t, u) and the sparse
+ * update (elastic response, sea surface, barystatic sea level; src/simulation.jl:355-381) runs
+ * with them at once; the next one is due one dt_sparse later. The next fi_integrate re-evaluates
+ * the first-same-as-last derivative. No counterpart in the reference. */
+int fi_set_time(fi_ctx* ctx, double t);
+
 /* ---- ensembles of independent simulations (uneven grids) ----------------------------------
  * The reference runs an ensemble as one deep copy of the Simulation per member, integrated in a
  * threaded loop (ext/FastIsostasyEnsembleKalmanProcessesExt.jl:103-122). Here one persistent
@@ -265,6 +273,9 @@ int fi_batch_set_team_size(fi_batch* b, int32_t team_size);
  * 194-225): a member then matches a single context integrated with t_target among its output
  * times. */
 int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal);
+
+/* As fi_batch_integrate with one target time per member (t_targets[m] for member m). */
+int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal);
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* out);
 
 #ifdef __cplusplus

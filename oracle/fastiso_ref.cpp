@@ -338,6 +338,7 @@ struct fi_ctx {
     vec ca_ice, ca_sw, ca_sed, ca_litho, ca_mantle, ca_load, ca_full;
     double V_af = 0, V_pov = 0, V_den = 0, delta_V = 0, z_bsl = 0;
     int64_t count = 0;
+    bool k1_stale = false;
     double t0 = 0, t_end = 0, t_timer = 0;
     bool initialized = false;
     // preallocated buffers (PreAllocated, src/tools.jl:4-16)
@@ -663,7 +664,10 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_U:
             if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_field(FI_U) before fi_init");
             c->u = v;
-            if (c->integ) c->uprev = v;
+            if (c->integ) {
+                c->uprev = v;
+                c->k1_stale = true;   // the next fi_integrate re-evaluates the FSAL derivative
+            }
             break;
         case FI_H_ICE:
             if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_field(FI_H_ICE) before fi_init");
@@ -868,6 +872,7 @@ int fi_integrator_init(fi_ctx* c) {
     c->dt = std::min(std::min(100 * dt0, dt1), dtmax);
     c->qold = QOLDINIT;
     c->integ = true;
+    c->k1_stale = false;
     c->kout = 0;
     return FI_OK;
 }
@@ -878,6 +883,11 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
     const fi_config& g = c->cfg;
     size_t N = c->N;
     if (t_target < c->t) return fail(c, FI_ERR_INVALID, "fi_integrate: target before current time");
+    if (c->k1_stale) {
+        int st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t);
+        if (st) return st;
+        c->k1_stale = false;
+    }
     while (c->t < t_target) {
         // next stop: the first output time after t, or t_target (modify_dt_for_tstops!)
         double stop = t_target;
@@ -940,6 +950,24 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
         }
     }
     c->u = c->uprev;
+    return FI_OK;
+}
+
+int fi_set_time(fi_ctx* c, double t) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_set_time before fi_integrator_init");
+    if (!(t >= c->t)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the current time");
+    const int64_t before = c->count;
+    const int64_t due = (int64_t)std::floor((t - c->t0) / c->cfg.dt_sparse);
+    c->count = std::max(before, due + 1);
+    vec u = c->uprev, d(c->N);
+    int st = update_diagnostics(c, d.data(), u.data(), t);
+    if (st) return st;
+    if ((st = sparse_block(c))) return st;
+    c->count = std::max(before, due + 1);
+    c->t = t;
+    c->u = c->uprev;
+    c->k1_stale = true;
     return FI_OK;
 }
 
@@ -1176,8 +1204,15 @@ int fi_batch_set_team_size(fi_batch* b, int32_t ts) { return (b && ts >= 0) ? FI
 
 int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
     if (!b) return FI_ERR_INVALID;
+    std::vector<double> tt(b->mem.size(), t_target);
+    return fi_batch_integrate_each(b, tt.data(), fsal);
+}
+
+int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) {
+    if (!b || !t_targets) return FI_ERR_INVALID;
     for (size_t m = 0; m < b->mem.size(); ++m) {
         fi_ctx* c = b->mem[m];
+        const double t_target = t_targets[m];
         if (!c->integ) {
             b->err = "member " + std::to_string(m) + ": fi_batch_integrate before fi_integrator_init";
             return FI_ERR_STATE;

@@ -65,11 +65,13 @@ def lib():
             "fi_integrator_init": (C.c_int, [P]),
             "fi_integrate": (C.c_int, [P, C.c_double, CB, C.c_void_p]),
             "fi_get_stats": (C.c_int, [P, C.POINTER(RefStats)]),
+            "fi_set_time": (C.c_int, [P, C.c_double]),
             "fi_create_batch": (C.c_int, [P, C.c_int32, C.POINTER(P)]),
             "fi_batch_member": (P, [P, C.c_int32]),
             "fi_batch_destroy": (None, [P]),
             "fi_batch_last_error": (C.c_char_p, [P]),
             "fi_batch_integrate": (C.c_int, [P, C.c_double, C.c_int32]),
+            "fi_batch_integrate_each": (C.c_int, [P, _dp, C.c_int32]),
             "ref_fft": (None, [C.c_int, _dp, _dp, _dp, _dp, C.c_int]),
             "ref_derivatives": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]),
             "ref_conv_cropped": (None, [C.c_int, C.c_int, _dp, _dp, _dp]),
@@ -185,6 +187,9 @@ class RefEngine:
         self._cb = cb
         self._ck(self.L.fi_integrate(self.h, float(t_target), cb, None))
 
+    def set_time(self, t):
+        self._ck(self.L.fi_set_time(self.h, float(t)))
+
     def stats(self):
         s = RefStats()
         self._ck(self.L.fi_get_stats(self.h, C.byref(s)))
@@ -219,6 +224,18 @@ class RefEnsemble:
 
     def integrate(self, t_target, fsal=True):
         st = self.L.fi_batch_integrate(self.h, float(t_target), int(bool(fsal)))
+        if st != 0:
+            raise RuntimeError(f"oracle error {st}: {self.L.fi_batch_last_error(self.h).decode()}")
+
+    def info(self):
+        return {"n_members": len(self.members)}
+
+    def close(self):
+        self.__del__()
+
+    def integrate_each(self, t_targets, fsal=True):
+        tt = np.ascontiguousarray(t_targets, dtype=np.float64)
+        st = self.L.fi_batch_integrate_each(self.h, _p(tt), int(bool(fsal)))
         if st != 0:
             raise RuntimeError(f"oracle error {st}: {self.L.fi_batch_last_error(self.h).decode()}")
 

@@ -198,3 +198,27 @@ def test_lv_trajectory_vs_oracle():
     sg, sr = sim.engine.stats(), ref.stats()
     assert abs(sg["n_accept"] - sr["n_accept"]) <= 0.05 * sr["n_accept"] + 3
     assert sg["count_sparse_updates"] == sr["count_sparse_updates"]
+
+
+def test_set_time_jump_vs_oracle():
+    """fi_set_field(u) + fi_set_time (the surrogate's macro step) then integration, both engines."""
+    from oracle import pyref
+    w = lv_case(n=32, t_end=20e3)
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    d = w["domain"]
+    u1 = -30.0 * np.exp(-(d.R / 8e5) ** 2)
+    outs = []
+    for e in (sim.engine, ref):
+        e.integrator_init()
+        e.integrate(100.0)
+        e.set_field("u", u1)
+        e.set_time(5000.0)
+        ue_jump = e.get_field("ue")
+        e.integrate(5100.0)
+        outs.append((e.get_field("u"), ue_jump, e.stats()))
+    (a, ea, sa), (b, eb, sb) = outs
+    assert sa["t"] == sb["t"] == 5100.0
+    assert sa["count_sparse_updates"] == sb["count_sparse_updates"]
+    assert np.abs(ea).max() > 0 and np.abs(ea - eb).max() <= 1e-5 * np.abs(eb).max()
+    assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max() + 1e-6
