@@ -17,7 +17,9 @@ the same workload, over a bounded window.
 Other workloads (--workload): lv_elva_hr is the N=1024 high-resolution instance (W = 3000 km,
 dx = 5.86 km, field seeds 5001/5002); ensemble is the uneven ensemble of 256 members with sides
 drawn from {64, 96, 128, 192, 256}, advanced together by fi_batch_integrate (value = member-kyr/s,
-the sum over members of simulated kyr per wall-clock second).
+the sum over members of simulated kyr per wall-clock second); surrogate is the N=350 workload run
+in 500-yr macro steps by the trained surrogate with a plain-solver check every 10 macro steps and
+fallback above 1 m (--validate adds the plain run of the same window for the error).
 """
 import argparse
 import json
@@ -123,6 +125,83 @@ def cpu_baseline(n, window, sample_s, hr=False):
                       f"{el:.1f} s wall, {st['n_accept']} accepted steps total"}
 
 
+WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "data", "fno_lv_elva_n350.npz")
+
+
+def bench_surrogate(a):
+    """Surrogate-with-fallback over [0, window_kyr] of the N=350 workload: 500-yr macro steps,
+    plain-solver check every 10th (the first included), fallback at max |du| > 1 m. With
+    --validate the plain solver integrates the same window in a second simulation and the
+    displacement difference is reported every 5 kyr."""
+    import numpy as np
+    from fastiso import surrogate as S
+    w, sim = make_sim(a.n, engine="hip")
+    d = np.load(a.weights, allow_pickle=False)
+    cfgv = [int(v) for v in d["cfg"]]
+    sur = S.Surrogate(in_channels=cfgv[0], width=cfgv[1], modes_x=cfgv[2], modes_y=cfgv[3], layers=cfgv[4],
+                      proj_hidden=cfgv[5])
+    sur.set_grid(a.n, a.n)
+    sur.set_params(d["theta"])
+    macro = S.lv_macro(w, 500.0)
+    eng = sim.engine
+    eng.integrator_init()
+    runner = S.SurrogateRunner(sim, sur, macro, check_every=10, tol=1.0)
+    t_end = a.window_kyr * 1e3
+    marks = {}
+    plain_walls = []
+    last = [time.perf_counter()]
+
+    def prog(t1, kind, dev):
+        now = time.perf_counter()
+        if not kind.startswith("surrogate"):
+            plain_walls.append(now - last[0])
+        last[0] = now
+        if abs(t1 / 5e3 - round(t1 / 5e3)) < 1e-9:
+            marks[t1] = eng.get_field("u")
+            print(f"surrogate run at {t1:.0f} yr: {kind} dev={dev}", file=sys.stderr, flush=True)
+
+    s0 = eng.stats()
+    t0 = time.perf_counter()
+    res = runner.run(t_end, t0=0.0, progress=prog)
+    el = time.perf_counter() - t0
+    s1 = eng.stats()
+    devs = [dv for _, kind, dv in runner.log if dv is not None]
+    plain_kyr_s = 0.5 / float(np.median(plain_walls)) if plain_walls else None
+    out = {
+        "metric": f"simulated kyr/s at N={a.n} LV-ELVA (surrogate with plain-solver fallback)",
+        "value": t_end / 1e3 / el,
+        "unit": "kyr/s",
+        "n_gpus": 1,
+        "steps": len(runner.log),
+        "warmup": 0,
+        "ms_per_step": el / max(len(runner.log), 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 mfma (fp32 accumulate) network; f32/f64 plain solver",
+        "data": "synthetic (seeded fields); network trained on plain-solver trajectories",
+        "config": {"workload": f"surrogate_lv_elva_n{a.n}", "n": a.n, "window_kyr": a.window_kyr,
+                   "macro_step_yr": 500.0, "check_every": 10, "tolerance_m": 1.0, **res,
+                   "max_check_deviation_m": max(devs) if devs else None,
+                   "plain_kyr_per_s_same_run": plain_kyr_s,
+                   "speedup_vs_plain": (t_end / 1e3 / el) / plain_kyr_s if plain_kyr_s else None,
+                   "surrogate_device_ms": sur.device_ms(),
+                   "plain_device_ms": s1["device_ms"] - s0["device_ms"], "wall_s": el},
+    }
+    if a.validate:
+        w2, sim2 = make_sim(a.n, engine="hip")
+        e2 = sim2.engine
+        e2.integrator_init()
+        errs = {}
+        for tk in sorted(marks):
+            e2.integrate(tk)
+            errs[f"{tk:.0f}"] = float(np.abs(e2.get_field("u") - marks[tk]).max())
+            print(f"plain reference at {tk:.0f} yr: max |u_sur - u_plain| = {errs[f'{tk:.0f}']:.3f} m",
+                  file=sys.stderr, flush=True)
+        out["config"]["max_abs_u_error_vs_plain_m"] = errs
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,7 +211,10 @@ def main():
     ap.add_argument("--window", type=float, default=10.0)
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble"], default="lv_elva")
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble", "surrogate"], default="lv_elva")
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--window-kyr", type=float, default=20.0)
+    ap.add_argument("--validate", action="store_true")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--team-size", type=int, default=0)
     a = ap.parse_args()
@@ -142,6 +224,9 @@ def main():
         sys.exit(2)
     if a.workload == "ensemble":
         print(json.dumps(bench_ensemble(a)), flush=True)
+        return
+    if a.workload == "surrogate":
+        print(json.dumps(bench_surrogate(a)), flush=True)
         return
     hr = a.workload == "lv_elva_hr"
     if hr and a.n == 350:

@@ -216,16 +216,16 @@ __global__ void k_fill(float* p, long n, float v) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Adam with bias correction on the mean gradient g / nb
+// Adam with bias correction on the mean gradient g / nb, decoupled weight decay wd
 __global__ void k_adam(float* p, float* m, float* v, const float* __restrict__ g, long n, float lr, float b1, float b2,
-                       float eps, float c1, float c2, float inv_nb) {
+                       float eps, float c1, float c2, float inv_nb, float wd) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const float gi = g[i] * inv_nb;
         const float mi = b1 * m[i] + (1.f - b1) * gi;
         const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
-        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] -= lr * ((mi / c1) / (sqrtf(vi / c2) + eps) + wd * p[i]);
     }
 }
 
@@ -687,7 +687,7 @@ int fi_surrogate_set_data(fi_surrogate* s, int32_t n, const float* x, const floa
     return 0;
 }
 
-int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, float lr_min, uint64_t seed,
+int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, float lr_min, float wd, uint64_t seed,
                        double* loss_out) {
     if (!s || steps < 0 || batch < 1) return S_ERR_INVALID;
     if (s->nsamp < 1) return sfail(s, S_ERR_STATE, "no training set");
@@ -715,7 +715,7 @@ int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, 
         const float lr_t = (float)(lr_min + 0.5 * (lr - lr_min) * (1.0 + std::cos(M_PI * frac)));
         const float c1 = (float)(1.0 - std::pow(0.9, t)), c2 = (float)(1.0 - std::pow(0.999, t));
         hipLaunchKernelGGL(k_adam, dim3(ew_grid(s->np)), dim3(256), 0, s->stream, s->par.p, s->am.p, s->av.p, s->grad.p,
-                           s->np, lr_t, 0.9f, 0.999f, 1e-8f, c1, c2, 1.f / (float)batch);
+                           s->np, lr_t, 0.9f, 0.999f, 1e-8f, c1, c2, 1.f / (float)batch, wd);
         SCHECK(s, hipGetLastError());
     }
     int st;

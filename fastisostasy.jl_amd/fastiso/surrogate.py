@@ -22,6 +22,31 @@ def _p(a):
     return a.ctypes.data_as(_fp)
 
 
+def augment(X, Y, nx, ny):
+    """The 8 symmetries of the square grid (flips of x and y, and the transpose), applied to every
+    channel: the LV-ELVA operator on a square domain is invariant under them."""
+    if nx != ny:
+        return X, Y
+    Xs, Ys = [X], [Y]
+    Xg = X.reshape(X.shape[0], ny, nx, X.shape[2])   # [sample, j, i, channel]
+    Yg = Y.reshape(Y.shape[0], ny, nx)
+    for tr in (False, True):
+        for fx in (False, True):
+            for fy in (False, True):
+                if not (tr or fx or fy):
+                    continue
+                a, b = Xg, Yg
+                if tr:
+                    a, b = a.transpose(0, 2, 1, 3), b.transpose(0, 2, 1)
+                if fx:
+                    a, b = a[:, :, ::-1], b[:, :, ::-1]
+                if fy:
+                    a, b = a[:, ::-1], b[:, ::-1]
+                Xs.append(np.ascontiguousarray(a).reshape(X.shape))
+                Ys.append(np.ascontiguousarray(b).reshape(Y.shape))
+    return np.concatenate(Xs), np.concatenate(Ys)
+
+
 def cells(F):
     """(nx, ny) field -> flat cell order i + j nx."""
     return np.asarray(F).ravel(order="F")
@@ -102,10 +127,10 @@ class Surrogate:
         X, Y = _f32(X), _f32(Y)
         self._check(self.lib.fi_surrogate_set_data(self.h, X.shape[0], _p(X), _p(Y)))
 
-    def train(self, steps, batch=4, lr=1e-3, lr_min=1e-5, seed=1):
+    def train(self, steps, batch=4, lr=1e-3, lr_min=1e-5, seed=1, wd=0.0):
         losses = np.empty(max(steps, 1), dtype=np.float64)
-        self._check(self.lib.fi_surrogate_train(self.h, int(steps), int(batch), float(lr), float(lr_min), int(seed),
-                                                losses.ctypes.data_as(C.POINTER(C.c_double))))
+        self._check(self.lib.fi_surrogate_train(self.h, int(steps), int(batch), float(lr), float(lr_min), float(wd),
+                                                int(seed), losses.ctypes.data_as(C.POINTER(C.c_double))))
         return losses[:steps]
 
     def device_ms(self):
@@ -259,9 +284,11 @@ class SurrogateRunner:
     surrogate's increment (fi_set_field(u) + fi_set_time). The first step of a run has no previous
     increment and is always a plain one."""
 
-    def __init__(self, sim, sur, macro, check_every=10, tol=1.0):
+    def __init__(self, sim, sur, macro, check_every=10, tol=1.0, select=True):
         self.sim, self.sur, self.macro = sim, sur, macro
         self.check_every, self.tol = int(check_every), float(tol)
+        self.select = select and sur is not None
+        self.use_net = sur is not None
         self.log = []
 
     def run(self, t_end, t0=None, progress=None):
@@ -283,18 +310,24 @@ class SurrogateRunner:
                 kind, dev = "plain", None
                 du_prev = eng.get_field("u") - u
             elif k % self.check_every == 0:
-                du = predict(self.sur, self.macro, u, ue, du_prev, t)
+                du_x = du_prev.copy()
+                du = predict(self.sur, self.macro, u, ue, du_prev, t) if self.sur is not None else du_x
                 eng.integrate(t1)
                 n_plain += 1
                 du_prev = eng.get_field("u") - u
-                dev = float(np.abs(du_prev - du).max())
-                kind = "check"
+                dev_x = float(np.abs(du_prev - du_x).max())
+                dev_n = float(np.abs(du_prev - du).max())
+                if self.select:
+                    # the predictor of the coming period: the one closer to the plain solver here
+                    self.use_net = dev_n <= dev_x
+                dev = dev_n if self.use_net else dev_x
+                kind = "check" + ("-net" if self.use_net else "-extrapolation")
                 if dev > self.tol:
                     plain_left = self.check_every - 1
                     n_fallback += 1
-                    kind = "check-fallback"
+                    kind += "-fallback"
             else:
-                du = predict(self.sur, self.macro, u, ue, du_prev, t)
+                du = predict(self.sur if self.use_net else None, self.macro, u, ue, du_prev, t)
                 eng.set_field("u", u + du)
                 eng.set_time(t1)
                 n_sur += 1
@@ -305,16 +338,19 @@ class SurrogateRunner:
                 progress(t1, kind, dev)
             t = t1
             k += 1
-        return {"plain_steps": n_plain, "surrogate_steps": n_sur, "fallbacks": n_fallback}
+        return {"plain_steps": n_plain, "surrogate_steps": n_sur, "fallbacks": n_fallback,
+                "network_periods": sum(1 for _, kind, _ in self.log if kind.startswith("check-net")),
+                "extrapolation_periods": sum(1 for _, kind, _ in self.log if kind.startswith("check-extrapolation"))}
 
 
 def generate_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=3.15e6, t_range=(5e3, 115e3),
-                   team_size=0, spin=6, progress=None):
+                   team_size=0, spin=6, skip=1, progress=None):
     """Training pairs of the plain solver: member m gets C1-style fields and domes (seeds seed0 + m,
     seed0 + 100000 + m, seed0 + 200000 + m) and a random time t_m of the glacial cycle. Its state at
     t_m is spun up by spin baseline macro steps from a random fraction of the equilibrium, then the
-    ensemble engine integrates every member steps_per_member macro steps; step k >= 1 gives a pair
-    (input with the increment of step k - 1, scaled correction du_k - du_{k-1}). Returns
+    ensemble engine integrates every member steps_per_member macro steps; step k >= skip gives a
+    pair (input with the increment of step k - 1, scaled correction du_k - du_{k-1}): the first
+    steps after the spin-up carry its transient. Returns
     (X, Y, meta): X (pairs, cells, 5), Y (pairs, cells)."""
     from . import workloads
     from .ensemble import Ensemble
@@ -350,7 +386,7 @@ def generate_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=
             t_m = starts[m][0] + s * macro_dt
             (u0, ue0), u1 = states[m], ens[m].get_field("u")
             du = u1 - u0
-            if prev[m] is not None:
+            if prev[m] is not None and s >= skip:
                 X.append(macros[m].features(u0, ue0, prev[m], t_m, macros[m].h))
                 Y.append(cells((du - prev[m]) / D_SCALE).astype(np.float32))
                 meta.append((m, t_m, float(np.abs(du).max()), float(np.abs(du - prev[m]).max())))

@@ -46,16 +46,18 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=32)
     ap.add_argument("--modes", type=int, default=16)
     ap.add_argument("--layers", type=int, default=4)
+    ap.add_argument("--wd", type=float, default=1e-4)
+    ap.add_argument("--skip", type=int, default=2)
     a = ap.parse_args(argv)
 
     def log(*msg):
         print(*msg, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    X, Y, meta = S.generate_pairs(a.members, a.seed, a.steps_per_member, a.macro_dt, n=a.n,
+    X, Y, meta = S.generate_pairs(a.members, a.seed, a.steps_per_member, a.macro_dt, n=a.n, skip=a.skip,
                                   progress=lambda s, i: log(f"train pairs: macro step {s + 1} done "
                                                             f"({time.perf_counter() - t0:.0f} s) {i}"))
-    Xv, Yv, meta_v = S.generate_pairs(a.val_members, a.seed + 5000, a.steps_per_member, a.macro_dt, n=a.n,
+    Xv, Yv, meta_v = S.generate_pairs(a.val_members, a.seed + 5000, a.steps_per_member, a.macro_dt, n=a.n, skip=a.skip,
                                       progress=lambda s, i: log(f"validation pairs done "
                                                                 f"({time.perf_counter() - t0:.0f} s)"))
     gen_s = time.perf_counter() - t0
@@ -63,10 +65,14 @@ def main(argv=None):
     sur = S.Surrogate(width=a.width, modes_x=a.modes, modes_y=a.modes, layers=a.layers, seed=a.seed,
                       in_channels=X.shape[2])
     sur.set_grid(a.n, a.n)
-    sur.set_data(X, Y)
+    Xa, Ya = S.augment(X, Y, a.n, a.n)
+    log(f"{len(Xa)} training samples after the 8 grid symmetries")
+    sur.set_data(Xa, Ya)
+    del Xa, Ya
     base_v = evaluate(None, Xv, Yv)
     t1 = time.perf_counter()
     losses = []
+    best = None
     chunk = 200
     done = 0
     while done < a.train_steps:
@@ -75,13 +81,18 @@ def main(argv=None):
         frac0, frac1 = done / a.train_steps, (done + k) / a.train_steps
         lr0 = 1e-5 + 0.5 * (a.lr - 1e-5) * (1 + np.cos(np.pi * frac0))
         lr1 = 1e-5 + 0.5 * (a.lr - 1e-5) * (1 + np.cos(np.pi * frac1))
-        losses += list(sur.train(k, batch=a.batch, lr=lr0, lr_min=lr1, seed=a.seed + done))
+        losses += list(sur.train(k, batch=a.batch, lr=lr0, lr_min=lr1, seed=a.seed + done, wd=a.wd))
         done += k
         ev = evaluate(sur, Xv, Yv)
+        score = float(np.mean(ev[:, 2]))   # mean over pairs of the max error
+        if best is None or score < best[0]:
+            best = (score, done, sur.get_params())
         log(f"step {done}: train loss {np.mean(losses[-k:]):.4g}  val max err {ev[:, 2].max():.3f} m "
             f"(extrapolation {base_v[:, 0].max():.3f})  val rms {ev[:, 3].mean():.4f} m (extrapolation {base_v[:, 1].mean():.4f})"
             f"  {time.perf_counter() - t1:.0f} s")
     train_s = time.perf_counter() - t1
+    sur.set_params(best[2])   # early stopping on the held-out members
+    log(f"kept the parameters of step {best[1]}")
     ev = evaluate(sur, Xv, Yv)
     sur.scales = (S.U_SCALE, S.S_SCALE, S.D_SCALE, a.macro_dt)
     sur.save(a.out)
@@ -91,7 +102,8 @@ def main(argv=None):
         "surrogate_device_ms": sur.device_ms(),
         "val_extrapolation_max_m": base_v[:, 0].tolist(), "val_extrapolation_rms_m": base_v[:, 1].tolist(),
         "val_network_max_m": ev[:, 2].tolist(), "val_network_rms_m": ev[:, 3].tolist(),
-        "val_increment_max_m": [m[2] for m in meta_v],
+        "val_increment_max_m": [m[2] for m in meta_v], "best_step": best[1],
+        "val_pass_1m_extrapolation": int(np.sum(base_v[:, 0] <= 1.0)), "val_pass_1m_network": int(np.sum(ev[:, 2] <= 1.0)),
         "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:])),
     }
     print(json.dumps(summary), flush=True)

@@ -67,11 +67,11 @@ int fi_surrogate_loss_grad(fi_surrogate* s, const float* x, const float* target,
  * in_channels, y: nsamples * cells). Replaces any previous set. */
 int fi_surrogate_set_data(fi_surrogate* s, int32_t nsamples, const float* x, const float* y);
 
-/* steps Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on mini-batches of batch samples drawn
- * with a seeded generator; learning rate lr (cosine decay to lr_min over the steps). loss_out
- * (optional, steps entries) receives each step's mean mini-batch loss. */
-int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, float lr_min, uint64_t seed,
-                       double* loss_out);
+/* steps Adam steps (beta1 0.9, beta2 0.999, eps 1e-8, decoupled weight decay wd) on mini-batches
+ * of batch samples drawn with a seeded generator; learning rate lr (cosine decay to lr_min over
+ * the steps). loss_out (optional, steps entries) receives each step's mean mini-batch loss. */
+int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, float lr_min, float wd,
+                       uint64_t seed, double* loss_out);
 
 /* Device time of all kernels launched by the surrogate so far (HIP events), in ms. */
 double fi_surrogate_device_ms(const fi_surrogate* s);

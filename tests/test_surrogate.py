@@ -80,6 +80,6 @@ def test_training_fits_a_smoothing_operator():
         X.append(f.reshape(nx * ny, 4))
         Y.append((low + 0.5 * f[..., 1]).reshape(nx * ny))
     s.set_data(np.array(X), np.array(Y))
-    losses = s.train(300, batch=4, lr=3e-3, lr_min=1e-4, seed=3)
+    losses = s.train(300, batch=4, lr=3e-3, lr_min=1e-4, seed=3, wd=0.0)
     assert np.all(np.isfinite(losses))
     assert losses[-20:].mean() < 0.1 * losses[:5].mean(), (losses[:5].mean(), losses[-20:].mean())
