@@ -992,7 +992,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     // (u FP64 6 rows, 3 moment fields 4 rows, R2C pair, 6 rows of the spectral-stage derivative)
     // or the padded-grid phases (their twiddles staged in the scratch)
     const size_t twb = (size_t)(nx + ny) * 8;
-    size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16 + 24), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16 + 24);
+    // FP64-moment rows phase also stages the rigidity (4 rows) and 7 per-cell fields (2 rows)
+    size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16 + 24 + 16 + 56), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16 + 24);
     size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)3 * c->nfx * 8, (size_t)5 * c->nfy * 8});
     c->mt_double = nx <= 512 && twb + rows_d <= 160 * 1024;   // FP64 moments, register prefetch sized for nx <= 512
     c->lds_d = twb + std::max(rows_d, other);

@@ -156,58 +156,6 @@ struct IceT {
     double tt, dtk, a;  // t - t_i0, t_i1 - t_i0, separable amplitude a(t)
 };
 
-// TimeInterpolation2D interpolate! bracket (src/interpolations.jl:51-74): i = first snapshot with
-// t_i >= t; exact hit at i, else linear between i-1 and i; outside [t_0, t_end] an error unless the
-// series is flat-extrapolated. Found by the whole workgroup in one round of loads (thread k tests
-// snapshot k); every thread returns the same bracket.
-__device__ __forceinline__ IceT ice_time(const KP& p, double t) {
-    __shared__ IceT sh_it;
-    IceT it;
-    it.i0 = it.i1 = 0;
-    it.exact = 1;
-    it.err = 0;
-    it.tt = 0;
-    it.dtk = 1;
-    it.a = 0;
-    if (p.ice_mode == ICE_EXTERNAL) return it;
-    const double* tv = p.ice_t;
-    const int nt = p.ice_nt;
-    const bool sep = p.ice_mode == ICE_SEPARABLE;
-    const double tfirst = tv[0], tlast = tv[nt - 1];
-    if (t < tfirst || t > tlast) {
-        if (!p.ice_flat) it.err = 1;
-        it.i0 = it.i1 = (t < tfirst) ? 0 : nt - 1;
-        if (sep) it.a = p.ice_a[it.i0];
-        return it;
-    }
-    for (int k = threadIdx.x; k < nt; k += NT) {
-        const double tk = tv[k];
-        const double tp = k ? tv[k - 1] : 0.0;
-        const double ak = sep ? p.ice_a[k] : 0.0;
-        const double ap = (sep && k) ? p.ice_a[k - 1] : 0.0;
-        if (tk >= t && (k == 0 || tp < t)) {
-            IceT r = it;
-            if (tk == t) {
-                r.i0 = r.i1 = k;
-                r.a = ak;
-            } else {
-                r.i0 = k - 1;
-                r.i1 = k;
-                r.exact = 0;
-                r.tt = t - tp;
-                r.dtk = tk - tp;
-                r.a = ap + (ak - ap) / r.dtk * r.tt;
-            }
-            sh_it = r;
-        }
-    }
-    __syncthreads();
-    it = sh_it;
-    if (!sep) it.a = 0;
-    __syncthreads();
-    return it;
-}
-
 __device__ __forceinline__ double ice_at(const KP& p, const IceT& it, long c) {
     if (p.ice_mode == ICE_EXTERNAL) return (double)p.H_ext[c];
     if (p.ice_mode == ICE_SEPARABLE) return it.a * (double)p.ice_H[c];
@@ -241,21 +189,6 @@ __device__ __forceinline__ double u_raw(const KP& p, const StageDesc& st, long c
     return u0 + st.dt * kterms(p, st, c);
 }
 
-// apply_bc!(u, CornerBC) offset of the stage displacement (src/boundary_conditions.jl:144-148);
-// kc holds the four corner values of the spectral term when st.kin >= 0
-__device__ __forceinline__ double u_corner(const KP& p, const StageDesc& st, long c, float kcv) {
-    const double u0 = ubuf(p, st.ubase)[c];
-    double v = kterms(p, st, c);
-    if (st.kin >= 0) v += st.ain * (double)kcv;
-    return (st.ng == 0 && st.kin < 0) ? u0 : u0 + st.dt * v;
-}
-__device__ __forceinline__ double u_offset(const KP& p, const StageDesc& st, const KC4 kc) {
-    const long n1 = p.nx - 1, n2 = (long)(p.ny - 1) * p.nx, n3 = n2 + p.nx - 1;
-    const double a = 0.25 * u_corner(p, st, 0, kc.v0), b = 0.25 * u_corner(p, st, n1, kc.v1);
-    const double c = 0.25 * u_corner(p, st, n2, kc.v2), d = 0.25 * u_corner(p, st, n3, kc.v3);
-    return ((a + b) + (c + d)) - p.vbc_x;
-}
-
 // corner-BC offset of a raw derivative slot
 __device__ __forceinline__ double k_offset(const KP& p, int slot) {
     const float* q = p.kcorner + 4 * slot;
@@ -263,35 +196,146 @@ __device__ __forceinline__ double k_offset(const KP& p, int slot) {
     return ((a + b) + (c + d)) - p.vbc_x;
 }
 
-// The four corner values of the inverse real transform of the spectrum, by direct sums over
-// spectral rows 0 and ny-1: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx})
-// (w = 1 for the DC and Nyquist terms, 2 otherwise). Every workgroup gets the same values.
-__device__ __forceinline__ KC4 spectral_corners(const KP& p, const float2* spec, const float2* twx, double* red) {
-    const int nx = p.nx, ny = p.ny;
-    const bool even = (nx % 2) == 0;
-    double v[4] = {0, 0, 0, 0};
-    for (int kx = threadIdx.x; kx < p.nhx; kx += NT) {
-        const float2 a = spec[(long)kx * ny], b = spec[(long)kx * ny + ny - 1];
-        const bool nyq = even && kx == nx / 2;
-        const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
-        v[0] += w * (double)a.x;
-        v[2] += w * (double)b.x;
-        if (nyq) {
-            v[1] -= (double)a.x;
-            v[3] -= (double)b.x;
-        } else {
-            const float2 t = twx[kx];
-            v[1] += w * ((double)a.x * t.x - (double)a.y * t.y);
-            v[3] += w * ((double)b.x * t.x - (double)b.y * t.y);
+// The three inputs every RHS call needs before its rows phase, gathered in one round of global
+// loads and one block reduction (every thread returns the same values):
+//  * the TimeInterpolation2D bracket of st.t (src/interpolations.jl:51-74): i = first snapshot
+//    with t_i >= t; exact hit at i, else linear between i-1 and i; outside [t_0, t_end] an error
+//    unless the series is flat-extrapolated. Thread k tests snapshot k.
+//  * the four corner values of the pending spectral derivative, by direct sums over spectral rows
+//    0 and ny-1: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx})
+//    (w = 1 for the DC and Nyquist terms, 2 otherwise);
+//  * the apply_bc!(u, CornerBC) offset of the stage displacement (src/boundary_conditions.jl:
+//    144-148): the mean of its four corners minus the border value.
+__device__ __forceinline__ void stage_preamble(const KP& p, const StageDesc& st, const float2* spec, const float2* twx,
+                                               double* red, IceT& it_out, KC4& kc_out, double& uoff_out) {
+    __shared__ IceT sh_it;
+    __shared__ double sh_uc[4], sh_u0[4];
+    const int tid = threadIdx.x;
+    const bool ice_search = p.ice_mode != ICE_EXTERNAL;
+    const bool sep = p.ice_mode == ICE_SEPARABLE;
+    const int nt = p.ice_nt;
+    const double t = st.t;
+    // ---- loads ----
+    double tfirst = 0, tlast = 0, tk = 0, tp = 0, ak = 0, ap = 0;
+    if (ice_search) {
+        tfirst = p.ice_t[0];
+        tlast = p.ice_t[nt - 1];
+        if (tid < nt) {
+            tk = p.ice_t[tid];
+            tp = tid ? p.ice_t[tid - 1] : 0.0;
+            if (sep) {
+                ak = p.ice_a[tid];
+                ap = tid ? p.ice_a[tid - 1] : 0.0;
+            }
         }
     }
-    block_sum<4>(red, v);
+    const int nx = p.nx, ny = p.ny;
+    const bool spec_on = st.kin >= 0;
+    float2 sa = make_float2(0.f, 0.f), sb = make_float2(0.f, 0.f), tw = make_float2(1.f, 0.f);
+    if (spec_on && tid < p.nhx) {
+        sa = spec[(long)tid * ny];
+        sb = spec[(long)tid * ny + ny - 1];
+        tw = twx[tid];
+    }
+    if (tid < 4) {
+        // corner tid: base displacement and the sum of the global derivative terms
+        const long n1 = nx - 1, n2 = (long)(ny - 1) * nx;
+        const long c = tid == 0 ? 0 : tid == 1 ? n1 : tid == 2 ? n2 : n2 + n1;
+        sh_u0[tid] = ubuf(p, st.ubase)[c];
+        sh_uc[tid] = kterms(p, st, c);
+    }
+    // ---- ice bracket ----
+    IceT it;
+    it.i0 = it.i1 = 0;
+    it.exact = 1;
+    it.err = 0;
+    it.tt = 0;
+    it.dtk = 1;
+    it.a = 0;
+    bool in_range = true;
+    if (ice_search && (t < tfirst || t > tlast)) {
+        in_range = false;
+        if (!p.ice_flat) it.err = 1;
+        it.i0 = it.i1 = (t < tfirst) ? 0 : nt - 1;
+    }
+    if (ice_search && in_range) {
+        for (int k = tid; k < nt; k += NT) {
+            if (k >= NT) {
+                tk = p.ice_t[k];
+                tp = p.ice_t[k - 1];
+                if (sep) {
+                    ak = p.ice_a[k];
+                    ap = p.ice_a[k - 1];
+                }
+            }
+            if (tk >= t && (k == 0 || tp < t)) {
+                IceT r = it;
+                if (tk == t) {
+                    r.i0 = r.i1 = k;
+                    r.a = ak;
+                } else {
+                    r.i0 = k - 1;
+                    r.i1 = k;
+                    r.exact = 0;
+                    r.tt = t - tp;
+                    r.dtk = tk - tp;
+                    r.a = ap + (ak - ap) / r.dtk * r.tt;
+                }
+                sh_it = r;
+            }
+        }
+    }
+    // ---- spectral corners: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx}) ----
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (spec_on) {
+        const bool even = (nx % 2) == 0;
+        for (int kx = tid; kx < p.nhx; kx += NT) {
+            float2 a = sa, b = sb, tw_k = tw;
+            if (kx >= NT) {
+                a = spec[(long)kx * ny];
+                b = spec[(long)kx * ny + ny - 1];
+                tw_k = twx[kx];
+            }
+            const bool nyq = even && kx == nx / 2;
+            const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
+            v[0] += w * (double)a.x;
+            v[2] += w * (double)b.x;
+            if (nyq) {
+                v[1] -= (double)a.x;
+                v[3] -= (double)b.x;
+            } else {
+                v[1] += w * ((double)a.x * tw_k.x - (double)a.y * tw_k.y);
+                v[3] += w * ((double)b.x * tw_k.x - (double)b.y * tw_k.y);
+            }
+        }
+        block_sum<4>(red, v);
+    } else {
+        __syncthreads();
+    }
     KC4 kc;
     kc.v0 = (float)v[0];
     kc.v1 = (float)v[1];
     kc.v2 = (float)v[2];
     kc.v3 = (float)v[3];
-    return kc;
+    if (ice_search && in_range) {
+        it = sh_it;
+        if (!sep) it.a = 0;
+    } else if (ice_search && sep) {
+        it.a = p.ice_a[it.i0];
+    }
+    // corner-BC offset: u0 + dt (sum of global terms + ain kc) at the four corners
+    const float kcv[4] = {kc.v0, kc.v1, kc.v2, kc.v3};
+    double q[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double vv = sh_uc[c];
+        if (spec_on) vv += st.ain * (double)kcv[c];
+        q[c] = 0.25 * ((st.ng == 0 && !spec_on) ? sh_u0[c] : sh_u0[c] + st.dt * vv);
+    }
+    uoff_out = ((q[0] + q[1]) + (q[2] + q[3])) - p.vbc_x;
+    it_out = it;
+    kc_out = kc;
+    __syncthreads();
 }
 
 __device__ __forceinline__ int corner_index(const KP& p, int i, int j) {
@@ -552,6 +596,209 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                 if (r == 0) sf[i].x = f;
                 else sf[i].y = f;
             }
+        }
+        __syncthreads();
+        FI_SUBMARK(6);
+        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, -1);
+        for (int kx = threadIdx.x; kx < p.nhx; kx += NT) {
+            const float2 Z = res[kx], Zm = res[kx == 0 ? 0 : nx - kx];
+            float2 X0, X1;
+            X0.x = 0.5f * (Z.x + Zm.x);
+            X0.y = 0.5f * (Z.y - Zm.y);
+            X1.x = 0.5f * (Z.y + Zm.y);
+            X1.y = -0.5f * (Z.x - Zm.x);
+            spout[(long)kx * ny + j0] = X0;
+            if (j0 + 1 < ny) spout[(long)kx * ny + j0 + 1] = X1;
+        }
+        __syncthreads();
+        FI_SUBMARK(7);
+    }
+}
+
+// Rows phase for grids up to 512 wide (FP64 moments): every global input of a row item -- the
+// stage terms, the pending spectral derivative's halo pairs, the rigidity and the per-cell fields
+// -- is loaded in one round at the start of the item and staged in LDS, so the rest of the item
+// (inverse transforms, stencils, loads, forward transform) is LDS work.
+__device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                               const IceT it, const float2* twx, char* smem, const KC4 kc) {
+    constexpr int QU = 6, QC = 4, QD = 4, QF = 2;   // loads per thread: (6|4|4|2) x NT >= (6|4|4|2) x 512
+    const int nx = p.nx, ny = p.ny;
+    const float inv_nx = 1.0f / (float)nx;
+    double* su = (double*)smem;                 // 6 rows of the stage displacement
+    double* sMxx = su + 6 * nx;                 // 4 rows each of the plate moments
+    double* sMyy = sMxx + 4 * nx;
+    double* sMxy = sMyy + 4 * nx;
+    float2* sf = (float2*)(sMxy + 4 * nx);      // R2C pair + scratch
+    float2* sg = sf + nx;
+    float* skin = (float*)(sg + nx);            // 6 rows of the spectral-stage derivative
+    float* sD = skin + 6 * nx;                  // 4 rows of the rigidity
+    float* sH = sD + 4 * nx;                    // 2 rows each of the per-cell fields
+    float* sZo = sH + 2 * nx;
+    float* sHr = sZo + 2 * nx;
+    float* sLt = sHr + 2 * nx;
+    float* sLw = sLt + 2 * nx;
+    float* sI2e = sLw + 2 * nx;
+    float* sM = sI2e + 2 * nx;
+    float2* ca = (float2*)sMxx;                 // inverse-transform scratch (moment region)
+    const float2* spin = specbuf(p, st.sb_in);
+    float2* spout = specbuf(p, st.sb_out);
+    const double nu = p.nu;
+    const bool wsig = st.write_sigma != 0;
+    const double* U0 = ubuf(p, st.ubase);
+    const float* zoff = st.trig ? p.zoff_trig : p.zoff;
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+        const int j0 = 2 * item, je = min(j0 + 2, ny);
+        const int jm0 = max(0, min(j0 - 1, ny - 3)), jm1 = min(ny, max(je + 1, 3));
+        const int ju0 = max(0, min(jm0 - 1, ny - 3)), ju1 = min(ny, max(jm1 + 1, 3));
+        const int totU = (ju1 - ju0) * nx, totM = (jm1 - jm0) * nx, totO = 2 * nx;
+        const int p0 = ju0 & ~1;
+        const int np = st.kin >= 0 ? (ju1 - p0 + 1) / 2 : 0;   // halo pairs (<= 4)
+        const int totC = np * nx;
+#ifdef FI_SUBPHASE_TIMING
+        unsigned long long tsub = wall_clock64();
+#endif
+        // ---- one round of global loads ----
+        double uv[QU];
+        float2 zv[QC];
+        float dv[QD];
+        float fH[QF], fZo[QF], fHr[QF], fLt[QF], fLw[QF], fI2e[QF], fM[QF];
+#pragma unroll
+        for (int q = 0; q < QU; ++q) {
+            const int idx = min((int)threadIdx.x + q * NT, totU - 1);
+            const int r = fdivi(idx, inv_nx), i = idx - r * nx;
+            const long c = i + (long)(ju0 + r) * nx;
+            uv[q] = (st.ng == 0) ? U0[c] : U0[c] + st.dt * kterms(p, st, c);
+        }
+#pragma unroll
+        for (int q = 0; q < QC; ++q) {
+            const int idx = min((int)threadIdx.x + q * NT, max(totC - 1, 0));
+            const int b = fdivi(idx, inv_nx), k = idx - b * nx;
+            const int ja = p0 + 2 * b;
+            zv[q] = (totC > 0) ? c2r_pair_value(p, spin, k, ja, ja + 1) : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < QD; ++q) {
+            const int idx = min((int)threadIdx.x + q * NT, totM - 1);
+            const int r = fdivi(idx, inv_nx), i = idx - r * nx;
+            dv[q] = p.D[i + (long)(jm0 + r) * nx];
+        }
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int idx = min((int)threadIdx.x + q * NT, totO - 1);
+            const int r = fdivi(idx, inv_nx), i = idx - r * nx;
+            const long c = i + (long)min(j0 + r, ny - 1) * nx;
+            fH[q] = (float)ice_at(p, it, c);
+            fZo[q] = zoff[c];
+            fHr[q] = p.H_af_ref[c];
+            fLt[q] = p.LT[c];
+            fLw[q] = p.interactive ? p.LW[c] : 0.0f;
+            fI2e[q] = p.inv2eta[c];
+            fM[q] = p.mask[c] ? 1.0f : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < QU; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            if (idx < totU) su[idx] = uv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QC; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            if (idx < totC) ca[idx] = zv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QD; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            if (idx < totM) sD[idx] = dv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int idx = (int)threadIdx.x + q * NT;
+            if (idx < totO) {
+                sH[idx] = fH[q];
+                sZo[idx] = fZo[q];
+                sHr[idx] = fHr[q];
+                sLt[idx] = fLt[q];
+                sLw[idx] = fLw[q];
+                sI2e[idx] = fI2e[q];
+                sM[idx] = fM[q];
+            }
+        }
+        __syncthreads();
+        FI_SUBMARK(0);
+        // previous stage's derivative on rows [ju0, ju1): inverse real transforms of the halo pairs
+        if (np > 0) {
+            float* Kin = p.Kb + (long)st.kin * p.Nl;
+            float2* res = fft_lds<float2>(ca, ca + totC, np, nx, p.px, twx, +1);
+            FI_SUBMARK(2);
+            for (int idx = threadIdx.x; idx < 2 * totC; idx += NT) {
+                const int b = fdivi(idx, 0.5f * inv_nx), rem = idx - b * 2 * nx, r = rem >= nx ? 1 : 0,
+                          i = rem - r * nx;
+                const int j = p0 + 2 * b + r;
+                if (j < ju0 || j >= ju1) continue;
+                float v = r ? res[b * nx + i].y : res[b * nx + i].x;
+                const int ci = corner_index(p, i, j);
+                if (ci >= 0) v = ci == 0 ? kc.v0 : ci == 1 ? kc.v1 : ci == 2 ? kc.v2 : kc.v3;
+                skin[(j - ju0) * nx + i] = v;
+                if (j >= j0 && j < je) {
+                    Kin[i + (long)j * nx] = v;
+                    if (ci >= 0) p.kcorner[4 * st.kin + ci] = v;
+                }
+            }
+            __syncthreads();
+        }
+        FI_SUBMARK(3);
+        // finish the stage displacement: spectral term, corner BC; keep u_{n+1} when asked
+        for (int idx = threadIdx.x; idx < totU; idx += NT) {
+            double u = su[idx];
+            if (np > 0) u += st.dt * (st.ain * (double)skin[idx]);
+            u -= uoff;
+            su[idx] = u;
+            const int r = fdivi(idx, inv_nx);
+            const int j = ju0 + r;
+            if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[idx - r * nx + (long)j * nx] = u;
+        }
+        __syncthreads();
+        FI_SUBMARK(4);
+        // plate moments M = -D (...) on rows [jm0, jm1)
+        for (int idx = threadIdx.x; idx < totM; idx += NT) {
+            const int r = fdivi(idx, inv_nx), i = idx - r * nx, j = jm0 + r;
+            const double D = (double)sD[idx];
+            const double uxx = d2x(su + (j - ju0) * nx, i, nx, p.inv_dx2);
+            const double uyy = d2y(su, ju0, j, i, nx, ny, p.inv_dy2);
+            const double uxy = dxy(su, ju0, j, i, nx, ny, p.inv_dx, p.inv_dy);
+            sMxx[idx] = -D * (nu * uyy + uxx);
+            sMyy[idx] = -D * (nu * uxx + uyy);
+            sMxy[idx] = -D * (1.0 - nu) * uxy;
+        }
+        __syncthreads();
+        FI_SUBMARK(5);
+        // rhs = -g sigma_full + dxx Mxx + 2 dxy Mxy + dyy Myy, times K/(2 eta)
+        for (int idx = threadIdx.x; idx < totO; idx += NT) {
+            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+            float f = 0.0f;
+            if (j < ny) {
+                const long c = i + (long)j * nx;
+                const double bxx = d2x(sMxx + (j - jm0) * nx, i, nx, p.inv_dx2);
+                const double byy = d2y(sMyy, jm0, j, i, nx, ny, p.inv_dy2);
+                const double bxy = dxy(sMxy, jm0, j, i, nx, ny, p.inv_dx, p.inv_dy);
+                const double u = su[(j - ju0) * nx + i];
+                const double m = sM[idx];
+                const double Haf = hab((double)sH[idx], (double)sZo[idx] + u, p.rho_sw_ice);
+                const double s_ice = p.rho_ice * (Haf - (double)sHr[idx]);
+                const double s_load = m * (s_ice + (double)sLw[idx]);
+                const double s_mant = p.rho_um * u;
+                const double s_full = s_load + m * ((double)sLt[idx] + s_mant);
+                const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
+                f = (float)(rhs * (double)sI2e[idx]);
+                if (wsig) {
+                    p.sig_load_prev[c] = (float)s_load;
+                    p.sig_geo_prev[c] = (float)(p.compressible_column
+                                                    ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
+                                                    : s_full);
+                }
+            }
+            if (r == 0) sf[i].x = f;
+            else sf[i].y = f;
         }
         __syncthreads();
         FI_SUBMARK(6);
@@ -858,14 +1105,14 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
         st.ng += 1;
         st.kin = -1;
     }
-    const IceT it = ice_time(p, st.t);
+    IceT it;
+    double uoff;
+    stage_preamble(p, st, specbuf(p, st.sb_in), twx, red, it, kc, uoff);
     if (it.err) {
         if (threadIdx.x == 0) s.err = ERR_TIME_RANGE;
         __syncthreads();
         return false;
     }
-    if (st.kin >= 0) kc = spectral_corners(p, specbuf(p, st.sb_in), twx, red);
-    const double uoff = u_offset(p, st, kc);
     if (trig) {
         phase_sp_rows(p, tm, smem);
         if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
@@ -903,7 +1150,7 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
     __syncthreads();
     if (p.ptime && tm.rank == 0 && threadIdx.x == 0) p.ptime[23] += wall_clock64() - tl;
 #endif
-    if (sizeof(MT) == 8) phase_rows<MT, 4, 2>(p, tm, st, uoff, it, twx, smem, kc);
+    if (sizeof(MT) == 8) phase_rows_lds(p, tm, st, uoff, it, twx, smem, kc);
     else phase_rows<MT, 8, 4>(p, tm, st, uoff, it, twx, smem, kc);
     if (!grid_sync(p, tm, gen, flag, TAG_ROWS, tl)) return false;
     phase_cols(p, tm, specbuf(p, st.sb_out), twy, smem);
