@@ -1292,18 +1292,17 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
 int fi_set_time(fi_ctx* c, double t) {
     if (!c) return FI_ERR_INVALID;
     if (!c->integ) return fail(c, FI_ERR_STATE, "fi_set_time before fi_integrator_init");
-    if (!(t >= c->ctl.t)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the current time");
+    if (!(t >= c->ctl.t0)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the start time");
     (void)hipSetDevice(c->dev);
-    const long long before = c->ctl.count;
     const long long due = (long long)std::floor((t - c->ctl.t0) / c->cfg.dt_sparse);
     // store the loads of (t, u) with a call that cannot trigger the sparse update
-    c->ctl.count = std::max(before, due + 1);
+    c->ctl.count = due + 1;
     int st = rhs_on_device(c, t, c->ctl.cur, (c->ctl.k1 + 1) % 4);
     if (st) return st;
     // the sparse update at t with those loads; the next one is due one interval later
     c->ctl.t_rhs = t;
     if ((st = launch(c, MODE_SPARSE_ONLY, c->ctl.cur, (c->ctl.k1 + 1) % 4, 0, 1))) return st;
-    c->ctl.count = std::max(before, due + 1);
+    c->ctl.count = due + 1;
     c->ctl.t = t;
     c->ctl.last_u = c->ctl.cur;
     c->k1_stale = true;

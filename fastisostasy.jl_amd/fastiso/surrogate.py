@@ -276,17 +276,23 @@ def predict(sur, macro, u, ue, du_prev, t):
 
 
 class SurrogateRunner:
-    """Surrogate-with-fallback integration of a HIP Simulation in macro steps.
+    """Surrogate-with-fallback integration of a HIP Simulation in macro steps of macro_dt years.
 
-    Every check_every-th macro step (the first included) is integrated by the plain solver and
-    compared with the surrogate's prediction of the same step; if max |du_sur - du_plain| > tol the
-    plain solver takes the following check_every - 1 macro steps as well. Other steps apply the
-    surrogate's increment (fi_set_field(u) + fi_set_time). The first step of a run has no previous
-    increment and is always a plain one."""
+    The first `warm` macro steps are plain (they give the first increments to extrapolate, past the
+    initial transient). Without the network a step extrapolates the increments in time: constant
+    (order 1) or changing linearly (order 2, du_k = 2 du_{k-1} - du_{k-2}); the network corrects the
+    order-1 extrapolation. Then the run proceeds in periods of check_every macro steps: the
+    surrogate takes the first check_every - 1 (fi_set_field(u) + fi_set_time), the plain solver
+    takes the last one and is compared with the surrogate's prediction of that step. If
+    max |du_plain - du_sur| <= tol the period is kept; otherwise the run rolls back to the start
+    of the period and the plain solver integrates the whole period (the fallback). At each check the
+    predictor of the next period is the one -- extrapolation alone or extrapolation + network --
+    that was closer to the plain solver."""
 
-    def __init__(self, sim, sur, macro, check_every=10, tol=1.0, select=True):
+    def __init__(self, sim, sur, macro, check_every=10, tol=1.0, select=True, warm=2, order=2):
         self.sim, self.sur, self.macro = sim, sur, macro
-        self.check_every, self.tol = int(check_every), float(tol)
+        self.check_every, self.tol, self.warm = int(check_every), float(tol), int(warm)
+        self.order = int(order)
         self.select = select and sur is not None
         self.use_net = sur is not None
         self.log = []
@@ -295,50 +301,74 @@ class SurrogateRunner:
         eng = self.sim.engine
         dt = self.macro.macro_dt
         t = eng.stats()["t"] if t0 is None else t0
-        k = 0
-        plain_left = 0
-        n_plain = n_sur = n_fallback = 0
-        du_prev = None
-        while t < t_end - 1e-9:
-            t1 = min(t + dt, t_end)
-            u = eng.get_field("u")
-            ue = eng.get_field("ue")
-            if du_prev is None or plain_left > 0 or t1 - t < dt:
-                eng.integrate(t1)
-                plain_left = max(plain_left - 1, 0)
-                n_plain += 1
-                kind, dev = "plain", None
-                du_prev = eng.get_field("u") - u
-            elif k % self.check_every == 0:
-                du_x = du_prev.copy()
-                du = predict(self.sur, self.macro, u, ue, du_prev, t) if self.sur is not None else du_x
-                eng.integrate(t1)
-                n_plain += 1
-                du_prev = eng.get_field("u") - u
-                dev_x = float(np.abs(du_prev - du_x).max())
-                dev_n = float(np.abs(du_prev - du).max())
-                if self.select:
-                    # the predictor of the coming period: the one closer to the plain solver here
-                    self.use_net = dev_n <= dev_x
-                dev = dev_n if self.use_net else dev_x
-                kind = "check" + ("-net" if self.use_net else "-extrapolation")
-                if dev > self.tol:
-                    plain_left = self.check_every - 1
-                    n_fallback += 1
-                    kind += "-fallback"
-            else:
-                du = predict(self.sur if self.use_net else None, self.macro, u, ue, du_prev, t)
-                eng.set_field("u", u + du)
-                eng.set_time(t1)
-                n_sur += 1
-                du_prev = du
-                kind, dev = "surrogate", None
-            self.log.append((t, kind, dev))
+        n_plain = n_sur = n_fallback = n_wasted = 0
+        hist = []
+
+        def note(t1, kind, dev):
+            self.log.append((t1, kind, dev))
             if progress:
                 progress(t1, kind, dev)
-            t = t1
-            k += 1
+
+        def plain_to(t1):
+            u0 = eng.get_field("u")
+            eng.integrate(t1)
+            return eng.get_field("u") - u0
+
+        while t < t_end - 1e-9:
+            t1 = min(t + dt, t_end)
+            remaining = int(round((t_end - t) / dt))
+            if len(hist) < self.warm or remaining < self.check_every or t1 - t < dt:
+                hist.append(plain_to(t1))
+                n_plain += 1
+                note(t1, "plain", None)
+                t = t1
+                continue
+            # a surrogate period
+            u_ck, t_ck = eng.get_field("u"), t
+            inc, tp = [hist[-2], hist[-1]], t
+
+            def extrapolated():
+                # constant increment (order 1) or linearly changing increment (order 2)
+                return 2.0 * inc[-1] - inc[-2] if self.order == 2 else inc[-1].copy()
+
+            for _ in range(self.check_every - 1):
+                u, ue = eng.get_field("u"), eng.get_field("ue")
+                du = predict(self.sur, self.macro, u, ue, inc[-1], tp) if self.use_net else extrapolated()
+                eng.set_field("u", u + du)
+                eng.set_time(tp + dt)
+                inc = [inc[-1], du]
+                tp += dt
+                note(tp, "surrogate", None)
+            u, ue = eng.get_field("u"), eng.get_field("ue")
+            du_x = extrapolated()
+            du_n = predict(self.sur, self.macro, u, ue, inc[-1], tp) if self.sur is not None else du_x
+            du_p = plain_to(tp + dt)
+            dev_x, dev_n = float(np.abs(du_p - du_x).max()), float(np.abs(du_p - du_n).max())
+            dev = dev_n if self.use_net else dev_x
+            used = "net" if self.use_net else "extrapolation"
+            if self.select:
+                self.use_net = dev_n <= dev_x
+            if dev <= self.tol:
+                n_sur += self.check_every - 1
+                n_plain += 1
+                hist.append(du_p)
+                note(tp + dt, f"check-{used}", dev)
+            else:
+                # fallback: roll back to the start of the period and integrate it with the plain solver
+                n_fallback += 1
+                n_wasted += self.check_every - 1
+                note(tp + dt, f"check-{used}-fallback", dev)
+                eng.set_field("u", u_ck)
+                eng.set_time(t_ck)
+                tq = t_ck
+                for _ in range(self.check_every):
+                    hist.append(plain_to(tq + dt))
+                    tq += dt
+                    n_plain += 1
+                    note(tq, "plain", None)
+            t = tp + dt
         return {"plain_steps": n_plain, "surrogate_steps": n_sur, "fallbacks": n_fallback,
+                "discarded_surrogate_steps": n_wasted,
                 "network_periods": sum(1 for _, kind, _ in self.log if kind.startswith("check-net")),
                 "extrapolation_periods": sum(1 for _, kind, _ in self.log if kind.startswith("check-extrapolation"))}
 

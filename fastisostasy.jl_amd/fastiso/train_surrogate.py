@@ -84,7 +84,7 @@ def main(argv=None):
         losses += list(sur.train(k, batch=a.batch, lr=lr0, lr_min=lr1, seed=a.seed + done, wd=a.wd))
         done += k
         ev = evaluate(sur, Xv, Yv)
-        score = float(np.mean(ev[:, 2]))   # mean over pairs of the max error
+        score = float(np.mean(ev[:, 3]))   # mean over held-out pairs of the RMS error
         if best is None or score < best[0]:
             best = (score, done, sur.get_params())
         log(f"step {done}: train loss {np.mean(losses[-k:]):.4g}  val max err {ev[:, 2].max():.3f} m "

@@ -230,8 +230,8 @@ int fi_integrate(fi_ctx* ctx, double t_target, fi_output_cb cb, void* user);
 
 int fi_get_stats(fi_ctx* ctx, fi_stats* out);
 
-/* Moves the integrator's time forward to t without integrating (after fi_integrator_init), for
- * drivers that advance the displacement themselves (the surrogate macro step:
+/* Sets the integrator's time to t >= t0 without integrating (after fi_integrator_init), for
+ * drivers that set the displacement themselves (the surrogate macro step and its rollback:
  * fi_set_field(FI_U) then fi_set_time). The column loads are stored at (t, u) and the sparse
  * update (elastic response, sea surface, barystatic sea level; src/simulation.jl:355-381) runs
  * with them at once; the next one is due one dt_sparse later. The next fi_integrate re-evaluates

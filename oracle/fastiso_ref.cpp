@@ -956,15 +956,14 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
 int fi_set_time(fi_ctx* c, double t) {
     if (!c) return FI_ERR_INVALID;
     if (!c->integ) return fail(c, FI_ERR_STATE, "fi_set_time before fi_integrator_init");
-    if (!(t >= c->t)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the current time");
-    const int64_t before = c->count;
+    if (!(t >= c->t0)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the start time");
     const int64_t due = (int64_t)std::floor((t - c->t0) / c->cfg.dt_sparse);
-    c->count = std::max(before, due + 1);
+    c->count = due + 1;
     vec u = c->uprev, d(c->N);
     int st = update_diagnostics(c, d.data(), u.data(), t);
     if (st) return st;
     if ((st = sparse_block(c))) return st;
-    c->count = std::max(before, due + 1);
+    c->count = due + 1;
     c->t = t;
     c->u = c->uprev;
     c->k1_stale = true;
