@@ -125,7 +125,7 @@ def cpu_baseline(n, window, sample_s, hr=False):
                       f"{el:.1f} s wall, {st['n_accept']} accepted steps total"}
 
 
-WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "data", "fno_lv_elva_n350.npz")
+WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "fastiso", "data", "fno_lv_elva_n350.npz")
 
 
 def bench_surrogate(a):
@@ -203,6 +203,35 @@ def bench_surrogate(a):
     return out
 
 
+def bench_fno_train(a):
+    """Device training throughput of the surrogate network at N=350 (width 32, 16x16 modes, 4
+    layers, 5 input channels) on a synthetic set of 64 samples: Adam steps of batch 8."""
+    import numpy as np
+    from fastiso import surrogate as S
+    n = a.n
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((64, n * n, 5)).astype(np.float32)
+    Y = rng.standard_normal((64, n * n)).astype(np.float32)
+    sur = S.Surrogate(in_channels=5)
+    sur.set_grid(n, n)
+    sur.set_data(X, Y)
+    sur.train(a.warmup, batch=8, lr=1e-4, seed=1)
+    ms0 = sur.device_ms()
+    t0 = time.perf_counter()
+    sur.train(a.steps, batch=8, lr=1e-4, seed=2)
+    el = time.perf_counter() - t0
+    dev = (sur.device_ms() - ms0) / 1e3
+    samples = a.steps * 8
+    return {
+        "metric": f"surrogate training samples/s at N={n} (forward + backward + Adam)",
+        "value": samples / el, "unit": "samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 mfma, fp32 accumulate", "data": "synthetic",
+        "config": {"workload": f"fno_train_n{n}", "n": n, "batch": 8, "params": sur.n_params,
+                   "device_s": dev, "ms_per_sample_device": dev / samples * 1e3},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,7 +241,8 @@ def main():
     ap.add_argument("--window", type=float, default=10.0)
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble", "surrogate"], default="lv_elva")
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble", "surrogate", "fno_train"],
+                    default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=20.0)
     ap.add_argument("--validate", action="store_true")
@@ -228,6 +258,9 @@ def main():
         return
     if a.workload == "surrogate":
         print(json.dumps(bench_surrogate(a)), flush=True)
+        return
+    if a.workload == "fno_train":
+        print(json.dumps(bench_fno_train(a)), flush=True)
         return
     hr = a.workload == "lv_elva_hr"
     if hr and a.n == 350:

@@ -829,10 +829,37 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* 
         const int kx0 = item * nb;
         const int nbb = min(nb, p.nhx - kx0);
         const long base = (long)kx0 * ny;
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) sa[idx] = spec[base + idx];
+        const int tot = nbb * ny;
+        if (tot <= 2 * NT) {
+            // the multiplier is loaded with the column, in the same round
+            float mv[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int idx = min((int)threadIdx.x + q * NT, tot - 1);
+                sa[idx] = spec[base + idx];
+                mv[q] = p.mult[base + idx];
+            }
+            __syncthreads();
+            float2* r = fft_lds<float2>(sa, sb, nbb, ny, p.py, twy, -1);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int idx = (int)threadIdx.x + q * NT;
+                if (idx < tot) {
+                    r[idx].x *= mv[q];
+                    r[idx].y *= mv[q];
+                }
+            }
+            __syncthreads();
+            float2* o = (r == sa) ? sb : sa;
+            float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
+            for (int idx = threadIdx.x; idx < tot; idx += NT) spec[base + idx] = r2[idx];
+            __syncthreads();
+            continue;
+        }
+        for (int idx = threadIdx.x; idx < tot; idx += NT) sa[idx] = spec[base + idx];
         __syncthreads();
         float2* r = fft_lds<float2>(sa, sb, nbb, ny, p.py, twy, -1);
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) {
+        for (int idx = threadIdx.x; idx < tot; idx += NT) {
             const float m = p.mult[base + idx];
             r[idx].x *= m;
             r[idx].y *= m;
@@ -858,11 +885,28 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
     float* K = p.Kb + (long)slot * p.Nl;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item;
+        const bool pre = st.final_err && 2 * nx <= 2 * NT;
+        // error-norm inputs of the pair's cells, loaded with the spectrum (one round)
+        float e1v[2] = {0.f, 0.f}, e2v[2] = {0.f, 0.f}, e3v[2] = {0.f, 0.f};
+        double unv[2] = {0.0, 0.0}, upv[2] = {0.0, 0.0};
+        if (pre) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int idx = min((int)threadIdx.x + q * NT, 2 * nx - 1);
+                const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = min(j0 + r, ny - 1);
+                const long c = i + (long)j * nx;
+                e1v[q] = kslot(p, st.e1)[c];
+                e2v[q] = kslot(p, st.e2)[c];
+                e3v[q] = kslot(p, st.e3)[c];
+                unv[q] = ubuf(p, st.u_prev)[c];
+                upv[q] = ubuf(p, st.write_u)[c];
+            }
+        }
         for (int k = threadIdx.x; k < nx; k += NT) sf[k] = c2r_pair_value(p, spec, k, j0, j0 + 1);
         __syncthreads();
         float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, +1);
         double acc[3] = {0.0, 0.0, 0.0};
-        for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
+        for (int idx = threadIdx.x, q = 0; idx < 2 * nx; idx += NT, ++q) {
             const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
@@ -871,8 +915,21 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
             const int ci = corner_index(p, i, j);
             if (ci >= 0) p.kcorner[4 * slot + ci] = v;
             if (st.final_err) {
-                const double k1 = kslot(p, st.e1)[c], k2 = kslot(p, st.e2)[c], k3 = kslot(p, st.e3)[c];
-                const double un = ubuf(p, st.u_prev)[c], up = ubuf(p, st.write_u)[c];
+                double k1, k2, k3, un, up;
+                if (pre) {
+                    const int qq = q & 1;   // q <= 1 when pre
+                    k1 = qq ? e1v[1] : e1v[0];
+                    k2 = qq ? e2v[1] : e2v[0];
+                    k3 = qq ? e3v[1] : e3v[0];
+                    un = qq ? unv[1] : unv[0];
+                    up = qq ? upv[1] : upv[0];
+                } else {
+                    k1 = kslot(p, st.e1)[c];
+                    k2 = kslot(p, st.e2)[c];
+                    k3 = kslot(p, st.e3)[c];
+                    un = ubuf(p, st.u_prev)[c];
+                    up = ubuf(p, st.write_u)[c];
+                }
                 const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)v);
                 const double w = 1.0 / (p.abstol + fmax(fabs(un), fabs(up)) * p.reltol);
                 acc[0] += (a * w) * (a * w);

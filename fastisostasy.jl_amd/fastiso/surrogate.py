@@ -317,7 +317,8 @@ class SurrogateRunner:
         while t < t_end - 1e-9:
             t1 = min(t + dt, t_end)
             remaining = int(round((t_end - t) / dt))
-            if len(hist) < self.warm or remaining < self.check_every or t1 - t < dt:
+            period = min(self.check_every, remaining)   # the last period may be shorter
+            if len(hist) < self.warm or period < 2 or t1 - t < dt:
                 hist.append(plain_to(t1))
                 n_plain += 1
                 note(t1, "plain", None)
@@ -331,7 +332,7 @@ class SurrogateRunner:
                 # constant increment (order 1) or linearly changing increment (order 2)
                 return 2.0 * inc[-1] - inc[-2] if self.order == 2 else inc[-1].copy()
 
-            for _ in range(self.check_every - 1):
+            for _ in range(period - 1):
                 u, ue = eng.get_field("u"), eng.get_field("ue")
                 du = predict(self.sur, self.macro, u, ue, inc[-1], tp) if self.use_net else extrapolated()
                 eng.set_field("u", u + du)
@@ -349,19 +350,19 @@ class SurrogateRunner:
             if self.select:
                 self.use_net = dev_n <= dev_x
             if dev <= self.tol:
-                n_sur += self.check_every - 1
+                n_sur += period - 1
                 n_plain += 1
                 hist.append(du_p)
                 note(tp + dt, f"check-{used}", dev)
             else:
                 # fallback: roll back to the start of the period and integrate it with the plain solver
                 n_fallback += 1
-                n_wasted += self.check_every - 1
+                n_wasted += period - 1
                 note(tp + dt, f"check-{used}-fallback", dev)
                 eng.set_field("u", u_ck)
                 eng.set_time(t_ck)
                 tq = t_ck
-                for _ in range(self.check_every):
+                for _ in range(period):
                     hist.append(plain_to(tq + dt))
                     tq += dt
                     n_plain += 1

@@ -13,7 +13,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "data", "fno_lv_elva_n350.npz")
+WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "fastiso", "data", "fno_lv_elva_n350.npz")
 
 
 def test_network_beats_extrapolation_on_held_out_members():
