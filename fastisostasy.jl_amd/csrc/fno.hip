@@ -102,16 +102,21 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
     }
 }
 
-// dst(m, n) = beta dst(m, n) + sum_b part[b][m][n] (fixed order)
-__global__ void k_splitk_reduce(const float* __restrict__ part, int nsplit, int M, int N, float* dst, long sm, long sn,
-                                float beta) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= M * N) return;
-    const int m = idx / N, n = idx - m * N;
+// dst(m, n) = beta dst(m, n) + sum_b part[b][m][n]: one wave per output, lane l sums the slices
+// b = l, l + 64, ... and the wave adds the lanes by a butterfly (fixed order: deterministic)
+__global__ void __launch_bounds__(256) k_splitk_reduce(const float* __restrict__ part, int nsplit, int M, int N,
+                                                       float* dst, long sm, long sn, float beta) {
+    const int out = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (out >= M * N) return;
     float s = 0.f;
-    for (int b = 0; b < nsplit; ++b) s += part[(long)b * M * N + idx];
-    float* d = dst + (long)m * sm + (long)n * sn;
-    *d = beta * *d + s;
+    for (int b = lane; b < nsplit; b += 64) s += part[(long)b * M * N + out];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        const int m = out / N, n = out - m * N;
+        float* d = dst + (long)m * sm + (long)n * sn;
+        *d = beta * *d + s;
+    }
 }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
@@ -315,7 +320,7 @@ int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, lo
 // product with a long K (parameter gradients): split-K partials, then a fixed-order reduction
 int gemm_splitk(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, long sBk, long sBn, float* C,
                 long sCm, long sCn, int M, int N, int K, float beta) {
-    const int kchunk = 2048;
+    const int kchunk = 256;   // ~480 slices over the 122 500 cells of an N=350 grid
     const int nsplit = (K + kchunk - 1) / kchunk;
     const long need = (long)nsplit * M * N;
     if (need > s->part_cap) {
@@ -326,7 +331,7 @@ int gemm_splitk(fi_surrogate* s, const float* A, long sAm, long sAk, const float
     dim3 grid((M + 63) / 64, (N + 31) / 32, nsplit);
     hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
     SCHECK(s, hipGetLastError());
-    hipLaunchKernelGGL(k_splitk_reduce, dim3((M * N + 255) / 256), dim3(256), 0, s->stream, s->part.p, nsplit, M, N, C,
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((M * N + 3) / 4), dim3(256), 0, s->stream, s->part.p, nsplit, M, N, C,
                        sCm, sCn, beta);
     SCHECK(s, hipGetLastError());
     return 0;
