@@ -32,6 +32,9 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "fastisostasy.jl_amd")]
 
 BYTES_PER_CELL_STEP = 171.0
 HBM_PEAK_GBS = 8000.0
+# measured HBM-side traffic of the persistent kernel at N=350 (rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, calibrated on the copy probe); scaled to this run's accepted steps per launch
+TRAFFIC_N350 = os.path.join(ROOT, "profiles", "fi_persistent_lv_elva_n350_traffic.json")
 
 
 def make_sim(n, engine, hr=False):
@@ -287,6 +290,10 @@ def main():
     dev_s = (s1["device_ms"] - s0["device_ms"]) / 1e3
     launches = s1["n_launch"] - s0["n_launch"]
     achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
+    traffic = None
+    if a.n == 350 and not hr and os.path.exists(TRAFFIC_N350) and launches > 0:
+        with open(TRAFFIC_N350) as f:
+            traffic = json.load(f)["bytes_per_accepted_step"] * acc / launches
     out = {
         "metric": f"simulated kyr/s at N={a.n} LV-ELVA (plain solver)",
         "value": kyr / el,
@@ -308,7 +315,7 @@ def main():
                    "kernel_launches": int(launches), "kernel_ms_per_launch": dev_s * 1e3 / max(launches, 1),
                    "us_per_accepted_step": dev_s * 1e6 / max(acc, 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr)
