@@ -1309,6 +1309,102 @@ int fi_set_time(fi_ctx* c, double t) {
     return FI_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Checkpoint blob: header, host control block and flags, then the dynamic device buffers in a
+// fixed order. The same routine measures, writes or reads it (mode 0/1/2).
+struct CkHeader {
+    uint32_t magic, version;
+    int32_t nx, ny, ice_mode, reserved;
+};
+constexpr uint32_t CK_MAGIC = 0x46494b31u;   // "FIK1"
+
+template <typename T>
+int ck_dev(fi_ctx* c, int mode, DBuf<T>& b, size_t n, unsigned char* blob, size_t& off) {
+    const size_t bytes = n * sizeof(T);
+    if (mode == 1) HIPCHECK(c, hipMemcpy(blob + off, b.p, bytes, hipMemcpyDeviceToHost));
+    if (mode == 2) HIPCHECK(c, hipMemcpy(b.p, blob + off, bytes, hipMemcpyHostToDevice));
+    off += bytes;
+    return FI_OK;
+}
+
+template <typename T>
+void ck_host(int mode, T& v, unsigned char* blob, size_t& off) {
+    if (mode == 1) std::memcpy(blob + off, &v, sizeof(T));
+    if (mode == 2) std::memcpy(&v, blob + off, sizeof(T));
+    off += sizeof(T);
+}
+
+int ck_walk(fi_ctx* c, int mode, unsigned char* blob, size_t& off) {
+    const size_t N = c->N;
+    CkHeader h{CK_MAGIC, 1, c->nx, c->ny, c->ice_mode, 0};
+    CkHeader hr = h;
+    ck_host(mode, hr, blob, off);
+    if (mode == 2 && (hr.magic != h.magic || hr.version != h.version || hr.nx != h.nx || hr.ny != h.ny ||
+                      hr.ice_mode != h.ice_mode))
+        return fail(c, FI_ERR_INVALID, "checkpoint of another grid, forcing or library");
+    ck_host(mode, c->ctl, blob, off);
+    int32_t flags[2] = {c->integ ? 1 : 0, c->k1_stale ? 1 : 0};
+    ck_host(mode, flags, blob, off);
+    ck_host(mode, c->kout, blob, off);
+    if (mode == 2) {
+        c->integ = flags[0] != 0;
+        c->k1_stale = flags[1] != 0;
+    }
+    int st;
+    if ((st = ck_dev(c, mode, c->U, 3 * N, blob, off)) || (st = ck_dev(c, mode, c->K, 4 * N, blob, off)) ||
+        (st = ck_dev(c, mode, c->kcorner, 16, blob, off)) || (st = ck_dev(c, mode, c->ue, N, blob, off)) ||
+        (st = ck_dev(c, mode, c->dz_ss, N, blob, off)) || (st = ck_dev(c, mode, c->z_ss, N, blob, off)) ||
+        (st = ck_dev(c, mode, c->zoff, N, blob, off)) || (st = ck_dev(c, mode, c->zoff_trig, N, blob, off)) ||
+        (st = ck_dev(c, mode, c->LW, N, blob, off)) || (st = ck_dev(c, mode, c->LT, N, blob, off)) ||
+        (st = ck_dev(c, mode, c->H_water, N, blob, off)) || (st = ck_dev(c, mode, c->mg, N, blob, off)) ||
+        (st = ck_dev(c, mode, c->mo, N, blob, off)) || (st = ck_dev(c, mode, c->sig_load_prev, N, blob, off)) ||
+        (st = ck_dev(c, mode, c->sig_geo_prev, N, blob, off)) || (st = ck_dev(c, mode, c->H_ext, N, blob, off)))
+        return st;
+    if (c->ice_mode == ICE_EXTERNAL) {
+        if (mode == 2) c->H_ext_host.assign(N, 0.0);
+        for (size_t p = 0; p < N; ++p) ck_host(mode, c->H_ext_host[p], blob, off);
+    }
+    return FI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_checkpoint_size(fi_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_checkpoint_size before fi_init");
+    size_t off = 0;
+    int st = ck_walk(c, 0, nullptr, off);
+    *bytes = off;
+    return st;
+}
+
+int fi_checkpoint_save(fi_ctx* c, void* buf, size_t bytes) {
+    size_t need = 0;
+    int st = fi_checkpoint_size(c, &need);
+    if (st) return st;
+    if (!buf || bytes != need) return fail(c, FI_ERR_INVALID, "fi_checkpoint_save: buffer size");
+    (void)hipSetDevice(c->dev);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    size_t off = 0;
+    return ck_walk(c, 1, (unsigned char*)buf, off);
+}
+
+int fi_checkpoint_load(fi_ctx* c, const void* buf, size_t bytes) {
+    size_t need = 0;
+    int st = fi_checkpoint_size(c, &need);
+    if (st) return st;
+    if (!buf || bytes != need) return fail(c, FI_ERR_INVALID, "fi_checkpoint_load: buffer size");
+    (void)hipSetDevice(c->dev);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    size_t off = 0;
+    return ck_walk(c, 2, (unsigned char*)const_cast<void*>(buf), off);
+}
+
 int fi_get_stats(fi_ctx* c, fi_stats* o) {
     if (!c || !o) return FI_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));

@@ -111,6 +111,9 @@ def declare(lib):
         "fi_integrate": (C.c_int, [P, C.c_double, OUTPUT_CB, C.c_void_p]),
         "fi_get_stats": (C.c_int, [P, C.POINTER(FiStats)]),
         "fi_set_time": (C.c_int, [P, C.c_double]),
+        "fi_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "fi_checkpoint_save": (C.c_int, [P, C.c_void_p, C.c_size_t]),
+        "fi_checkpoint_load": (C.c_int, [P, C.c_void_p, C.c_size_t]),
         "fi_create_batch": (C.c_int, [C.POINTER(FiConfig), C.c_int32, C.POINTER(P)]),
         "fi_batch_member": (P, [P, C.c_int32]),
         "fi_batch_destroy": (None, [P]),

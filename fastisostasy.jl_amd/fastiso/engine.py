@@ -123,6 +123,19 @@ class Engine:
         self._cb = cb
         self._check(self.lib.fi_integrate(self.h, float(t_target), cb, None))
 
+    def checkpoint(self):
+        """The dynamic state as bytes (fi_checkpoint_save)."""
+        n = C.c_size_t()
+        self._check(self.lib.fi_checkpoint_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._check(self.lib.fi_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, blob):
+        """Restore a checkpoint() of this library and configuration (fi_checkpoint_load)."""
+        buf = C.create_string_buffer(blob, len(blob))
+        self._check(self.lib.fi_checkpoint_load(self.h, buf, len(blob)))
+
     def set_time(self, t):
         """Move the integrator time to t without integrating (after set_field("u", ...))."""
         self._check(self.lib.fi_set_time(self.h, float(t)))

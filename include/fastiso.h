@@ -238,6 +238,18 @@ int fi_get_stats(fi_ctx* ctx, fi_stats* out);
  * the first-same-as-last derivative. No counterpart in the reference. */
 int fi_set_time(fi_ctx* ctx, double t);
 
+/* Checkpoints of a context's dynamic state: the displacement, the first-same-as-last derivative,
+ * the step-size controller, the sparse-update state (elastic displacement, sea surface, masks,
+ * stored loads, BSL volumes, counters) and the output counter. fi_checkpoint_size gives the byte
+ * size, fi_checkpoint_save writes the state into a host buffer, fi_checkpoint_load restores it
+ * into a context of the same library, created and initialised (fi_init) with the same
+ * configuration and static fields. Restoring and integrating continues bit for bit like the
+ * run that was saved. The reference keeps this state in its Simulation object
+ * (src/state.jl:49-104) and has no checkpoint file of its own. */
+int fi_checkpoint_size(fi_ctx* ctx, size_t* bytes);
+int fi_checkpoint_save(fi_ctx* ctx, void* buf, size_t bytes);
+int fi_checkpoint_load(fi_ctx* ctx, const void* buf, size_t bytes);
+
 /* ---- ensembles of independent simulations (uneven grids) ----------------------------------
  * The reference runs an ensemble as one deep copy of the Simulation per member, integrated in a
  * threaded loop (ext/FastIsostasyEnsembleKalmanProcessesExt.jl:103-122). Here one persistent

@@ -970,6 +970,71 @@ int fi_set_time(fi_ctx* c, double t) {
     return FI_OK;
 }
 
+// Checkpoints: the dynamic state of the restatement (the fields of sim.now, src/state.jl:49-104,
+// the loads of the last call and the integrator), walked in a fixed order to measure, write or
+// read a host blob (mode 0/1/2).
+static int ck_walk(fi_ctx* c, int mode, unsigned char* blob, size_t& off) {
+    auto raw = [&](void* p, size_t n) {
+        if (mode == 1) std::memcpy(blob + off, p, n);
+        if (mode == 2) std::memcpy(p, blob + off, n);
+        off += n;
+    };
+    auto sc = [&](auto& v) { raw(&v, sizeof(v)); };
+    auto vv = [&](vec& v) {
+        if (mode == 2) v.resize(c->N);
+        raw(v.data(), c->N * sizeof(double));
+    };
+    auto vb = [&](std::vector<uint8_t>& v) {
+        if (mode == 2) v.resize(c->N);
+        raw(v.data(), c->N);
+    };
+    int32_t hdr[4] = {0x46494b52, c->nx, c->ny, c->ice_mode}, hr[4] = {hdr[0], hdr[1], hdr[2], hdr[3]};
+    raw(hr, sizeof(hr));
+    if (mode == 2 && std::memcmp(hr, hdr, sizeof(hdr)) != 0)
+        return fail(c, FI_ERR_INVALID, "checkpoint of another grid, forcing or library");
+    vv(c->u); vv(c->ue); vv(c->dudt); vv(c->H_ice); vv(c->H_af); vv(c->H_water); vv(c->z_b); vv(c->dz_ss);
+    vv(c->z_ss); vb(c->mg); vb(c->mo);
+    vv(c->ca_ice); vv(c->ca_sw); vv(c->ca_sed); vv(c->ca_litho); vv(c->ca_mantle); vv(c->ca_load); vv(c->ca_full);
+    sc(c->V_af); sc(c->V_pov); sc(c->V_den); sc(c->delta_V); sc(c->z_bsl); sc(c->count); sc(c->k1_stale);
+    sc(c->t_timer); sc(c->integ); sc(c->t); sc(c->dt); sc(c->qold); sc(c->q11); sc(c->EEst);
+    vv(c->uprev); vv(c->k1);
+    sc(c->nrhs); sc(c->nacc); sc(c->nrej); sc(c->kout);
+    if (mode == 2) {   // stage work vectors of fi_integrate (sized by fi_integrator_init otherwise)
+        c->k2.resize(c->N);
+        c->k3.resize(c->N);
+        c->k4.resize(c->N);
+        c->tmp.resize(c->N);
+    }
+    return FI_OK;
+}
+
+int fi_checkpoint_size(fi_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_checkpoint_size before fi_init");
+    size_t off = 0;
+    int st = ck_walk(c, 0, nullptr, off);
+    *bytes = off;
+    return st;
+}
+
+int fi_checkpoint_save(fi_ctx* c, void* buf, size_t bytes) {
+    size_t need = 0;
+    int st = fi_checkpoint_size(c, &need);
+    if (st) return st;
+    if (!buf || bytes != need) return fail(c, FI_ERR_INVALID, "fi_checkpoint_save: buffer size");
+    size_t off = 0;
+    return ck_walk(c, 1, (unsigned char*)buf, off);
+}
+
+int fi_checkpoint_load(fi_ctx* c, const void* buf, size_t bytes) {
+    size_t need = 0;
+    int st = fi_checkpoint_size(c, &need);
+    if (st) return st;
+    if (!buf || bytes != need) return fail(c, FI_ERR_INVALID, "fi_checkpoint_load: buffer size");
+    size_t off = 0;
+    return ck_walk(c, 2, (unsigned char*)const_cast<void*>(buf), off);
+}
+
 int fi_get_stats(fi_ctx* c, fi_stats* o) {
     if (!c || !o) return FI_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));

@@ -66,6 +66,9 @@ def lib():
             "fi_integrate": (C.c_int, [P, C.c_double, CB, C.c_void_p]),
             "fi_get_stats": (C.c_int, [P, C.POINTER(RefStats)]),
             "fi_set_time": (C.c_int, [P, C.c_double]),
+            "fi_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+            "fi_checkpoint_save": (C.c_int, [P, C.c_void_p, C.c_size_t]),
+            "fi_checkpoint_load": (C.c_int, [P, C.c_void_p, C.c_size_t]),
             "fi_create_batch": (C.c_int, [P, C.c_int32, C.POINTER(P)]),
             "fi_batch_member": (P, [P, C.c_int32]),
             "fi_batch_destroy": (None, [P]),
@@ -189,6 +192,17 @@ class RefEngine:
 
     def set_time(self, t):
         self._ck(self.L.fi_set_time(self.h, float(t)))
+
+    def checkpoint(self):
+        n = C.c_size_t()
+        self._ck(self.L.fi_checkpoint_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._ck(self.L.fi_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, blob):
+        buf = C.create_string_buffer(blob, len(blob))
+        self._ck(self.L.fi_checkpoint_load(self.h, buf, len(blob)))
 
     def stats(self):
         s = RefStats()

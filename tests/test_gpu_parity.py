@@ -222,3 +222,59 @@ def test_set_time_jump_vs_oracle():
     assert sa["count_sparse_updates"] == sb["count_sparse_updates"]
     assert np.abs(ea).max() > 0 and np.abs(ea - eb).max() <= 1e-5 * np.abs(eb).max()
     assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max() + 1e-6
+
+
+def test_interactive_sealevel_rhs_vs_oracle():
+    """InteractiveSealevelLoad (src/loads.jl:56-61): a marine margin, the active-load mask on 40 % of
+    the cells; the sea-water column anomaly enters the load after each sparse update."""
+    import fastiso
+    from oracle import pyref
+    n = 48
+    w = lv_case(n=n, seed=9, interactive=True)
+    d = w["domain"]
+    r = d.R / d.R.max()
+    w["z_b_ref"] = 400.0 - 1400.0 * r                       # land inside, ocean outside
+    se = w["solidearth"]
+    mask = r < 0.55
+    assert 0.2 < mask.mean() < 0.6
+    w["solidearth"] = fastiso.SolidEarth(d, layer_boundaries=[w["h"]], layer_viscosities=[10.0 ** w["logeta"]],
+                                         maskactive=mask)
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    rng = np.random.default_rng(4)
+    u = -80 * np.exp(-(d.R / 1e6) ** 2) + 0.01 * rng.standard_normal((n, n))
+    for t in (0.0, 500.0, 600.0, 605.0):
+        dg, _ = sim.engine.rhs(t, u)
+        dr, _ = ref.rhs(t, u)
+        m, l2 = rel(dg, dr)
+        assert m < RHS_MAX and l2 < RHS_L2, (t, m, l2)
+    sw_g, sw_r = sim.engine.get_field("canom_seawater"), ref.get_field("canom_seawater")
+    assert np.abs(sw_r).max() > 0
+    assert np.abs(sw_g - sw_r).max() <= 1e-5 * np.abs(sw_r).max()
+    for f in ("maskocean", "maskgrounded"):
+        np.testing.assert_array_equal(sim.engine.get_field(f), ref.get_field(f))
+
+
+def test_checkpoint_restart_is_bitwise():
+    """fi_checkpoint_save at t1 (after sparse updates), then integrate to t2; a fresh context that
+    restores the checkpoint and integrates to t2 ends bit for bit equal."""
+    w = lv_case(n=32, t_end=400.0)
+    sim = hip_sim(w)
+    e = sim.engine
+    e.integrator_init()
+    e.integrate(150.0)
+    blob = e.checkpoint()
+    e.integrate(400.0)
+    ua, uea, sa = e.get_field("u"), e.get_field("ue"), e.stats()
+    sim2 = hip_sim(w)
+    e2 = sim2.engine
+    e2.restore(blob)
+    e2.integrate(400.0)
+    np.testing.assert_array_equal(e2.get_field("u"), ua)
+    np.testing.assert_array_equal(e2.get_field("ue"), uea)
+    sb = e2.stats()
+    for k in ("t", "n_accept", "n_reject", "count_sparse_updates", "n_rhs"):
+        assert sa[k] == sb[k], k
+    # rollback on the same context
+    e2.restore(blob)
+    assert e2.stats()["t"] == 150.0

@@ -87,3 +87,25 @@ def test_disc_benchmark_mass_sign(c0_run):
     w, e, outs = c0_run
     u = outs[-1][1]
     assert u.min() < -250 and u.min() > -300       # ~ rho_i/rho_m * 1 km depression
+
+
+def test_oracle_checkpoint_restart_is_bitwise(ref):
+    from _cases import lv_case, sim_inputs
+    w = lv_case(n=24, t_end=300.0)
+    inp = sim_inputs(w).inputs
+    e = ref.RefEngine.from_inputs(inp)
+    e.integrator_init()
+    e.integrate(120.0)
+    blob = e.checkpoint()
+    e.integrate(300.0)
+    e2 = ref.RefEngine.from_inputs(inp)
+    e2.restore(blob)
+    e2.integrate(300.0)
+    np.testing.assert_array_equal(e2.get_field("u"), e.get_field("u"))
+    np.testing.assert_array_equal(e2.get_field("ue"), e.get_field("ue"))
+    assert e2.stats()["n_accept"] == e.stats()["n_accept"]
+    bad = bytearray(blob)
+    bad[0] ^= 0xFF
+    import pytest
+    with pytest.raises(RuntimeError):
+        e2.restore(bytes(bad))
