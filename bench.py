@@ -238,10 +238,10 @@ def bench_fno_train(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed windows (default 200; 3 for lv_elva_hr)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed windows (default 5; 1 for lv_elva_hr)")
     ap.add_argument("--n", type=int, default=350)
-    ap.add_argument("--window", type=float, default=10.0)
+    ap.add_argument("--window", type=float, default=None, help="window in yr (default 10; 5 for lv_elva_hr)")
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble", "surrogate", "fno_train"],
@@ -252,6 +252,13 @@ def main():
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--team-size", type=int, default=0)
     a = ap.parse_args()
+    hr_defaults = a.workload == "lv_elva_hr"
+    if a.steps is None:
+        a.steps = 3 if hr_defaults else 200
+    if a.warmup is None:
+        a.warmup = 1 if hr_defaults else 5
+    if a.window is None:
+        a.window = 5.0 if hr_defaults else 10.0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
@@ -279,9 +286,11 @@ def main():
         eng.integrate(t)
     s0 = eng.stats()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for k in range(a.steps):
         t += a.window
         eng.integrate(t)
+        if hr:   # windows of the N=1024 grid take tens of seconds each
+            print(f"window {k + 1}/{a.steps} done", file=sys.stderr, flush=True)
     el = time.perf_counter() - t0
     s1 = eng.stats()
     kyr = a.steps * a.window / 1e3

@@ -174,9 +174,14 @@ __device__ __forceinline__ void load_twiddles(const KP& p, float2* twx, float2* 
 // ------------------------------------------------------------------------------------------
 // persistent kernel of one context: the whole grid is one team
 // ------------------------------------------------------------------------------------------
+// The parameter block is not written while a kernel runs. Reading it through the constant
+// address space keeps its fields in scalar registers across stores, and lets the compiler treat
+// the pointers it holds as global memory (global rather than flat loads).
+using ConstKP = __attribute__((address_space(4))) KP;
+
 template <typename MT>
 __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
-    const KP& p = *pp;
+    const KP& p = *(const KP*)(const ConstKP*)(unsigned long long)pp;
     extern __shared__ __align__(16) char smem_all[];
     __shared__ Ctl s;
     __shared__ double red[64];
@@ -217,6 +222,36 @@ struct BatchArgs {
     unsigned long long spin_limit;
 };
 
+// one member of the ensemble kernel. The parameter block is passed as a restrict pointer: nothing
+// the integration writes aliases it, so its fields are read with scalar loads (as in the kernel of
+// one context) instead of being reloaded into vector registers after every store.
+template <typename MT>
+__device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, const Team& tm, Ctl& s, unsigned& gen,
+                                             char* smem_all, double* red, int* flag, unsigned long long& tl,
+                                             int fsal) {
+    const KP& p = *(const KP*)pp;
+    float2* twx = (float2*)smem_all;
+    float2* twy = twx + p.nx;
+    char* smem = (char*)(twy + p.ny);
+    load_twiddles(p, twx, twy);
+    if (threadIdx.x == 0) s = *p.ctl;
+    __syncthreads();
+    const bool todo = s.t < s.stop && !s.err;
+    if (todo) {
+        integrate_loop<MT>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
+        if (*flag) return false;
+        if (fsal && !s.err && s.t >= s.stop) {
+            if (!single_rhs<MT>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
+                if (*flag) return false;
+            }
+            if (threadIdx.x == 0) s.last_u = s.cur;
+            __syncthreads();
+        }
+    }
+    if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
+    return true;
+}
+
 template <typename MT>
 __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __restrict__ ba) {
     extern __shared__ __align__(16) char smem_all[];
@@ -239,28 +274,13 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __res
         if (threadIdx.x == 0)
             s_m = __hip_atomic_load(&ba->team_member[team], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const int m = s_m;
+        const int m = __builtin_amdgcn_readfirstlane(s_m);
         if (m < 0) return;
-        const KP& p = *ba->kps[m];
-        float2* twx = (float2*)smem_all;
-        float2* twy = twx + p.nx;
-        char* smem = (char*)(twy + p.ny);
-        load_twiddles(p, twx, twy);
-        if (threadIdx.x == 0) s = *p.ctl;
-        __syncthreads();
-        const bool todo = s.t < s.stop && !s.err;
-        if (todo) {
-            integrate_loop<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
-            if (flag) return;
-            if (ba->fsal && !s.err && s.t >= s.stop) {
-                if (!single_rhs<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, s.cur, s.k1, s.t, false, true)) {
-                    if (flag) return;
-                }
-                if (threadIdx.x == 0) s.last_u = s.cur;
-                __syncthreads();
-            }
-        }
-        if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
+        // the member table is read as global memory (a load through a generic pointer counts as
+        // divergent), so the member's parameter block is a uniform, constant pointer
+        const unsigned long long pk =
+            ((const __attribute__((address_space(1))) unsigned long long*)(unsigned long long)ba->kps)[m];
+        if (!batch_member<MT>((const ConstKP*)pk, tm, s, gen, smem_all, red, &flag, tl, ba->fsal)) return;
     }
 }
 

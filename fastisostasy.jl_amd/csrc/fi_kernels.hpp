@@ -747,58 +747,150 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             __syncthreads();
         }
         FI_SUBMARK(3);
-        // finish the stage displacement: spectral term, corner BC; keep u_{n+1} when asked
-        for (int idx = threadIdx.x; idx < totU; idx += NT) {
-            double u = su[idx];
-            if (np > 0) u += st.dt * (st.ain * (double)skin[idx]);
-            u -= uoff;
-            su[idx] = u;
-            const int r = fdivi(idx, inv_nx);
-            const int j = ju0 + r;
-            if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[idx - r * nx + (long)j * nx] = u;
-        }
-        __syncthreads();
-        FI_SUBMARK(4);
-        // plate moments M = -D (...) on rows [jm0, jm1)
-        for (int idx = threadIdx.x; idx < totM; idx += NT) {
-            const int r = fdivi(idx, inv_nx), i = idx - r * nx, j = jm0 + r;
-            const double D = (double)sD[idx];
-            const double uxx = d2x(su + (j - ju0) * nx, i, nx, p.inv_dx2);
-            const double uyy = d2y(su, ju0, j, i, nx, ny, p.inv_dy2);
-            const double uxy = dxy(su, ju0, j, i, nx, ny, p.inv_dx, p.inv_dy);
-            sMxx[idx] = -D * (nu * uyy + uxx);
-            sMyy[idx] = -D * (nu * uxx + uyy);
-            sMxy[idx] = -D * (1.0 - nu) * uxy;
-        }
-        __syncthreads();
-        FI_SUBMARK(5);
-        // rhs = -g sigma_full + dxx Mxx + 2 dxy Mxy + dyy Myy, times K/(2 eta)
-        for (int idx = threadIdx.x; idx < totO; idx += NT) {
-            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
-            float f = 0.0f;
-            if (j < ny) {
-                const long c = i + (long)j * nx;
-                const double bxx = d2x(sMxx + (j - jm0) * nx, i, nx, p.inv_dx2);
-                const double byy = d2y(sMyy, jm0, j, i, nx, ny, p.inv_dy2);
-                const double bxy = dxy(sMxy, jm0, j, i, nx, ny, p.inv_dx, p.inv_dy);
-                const double u = su[(j - ju0) * nx + i];
-                const double m = sM[idx];
-                const double Haf = hab((double)sH[idx], (double)sZo[idx] + u, p.rho_sw_ice);
-                const double s_ice = p.rho_ice * (Haf - (double)sHr[idx]);
-                const double s_load = m * (s_ice + (double)sLw[idx]);
-                const double s_mant = p.rho_um * u;
-                const double s_full = s_load + m * ((double)sLt[idx] + s_mant);
-                const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
-                f = (float)(rhs * (double)sI2e[idx]);
-                if (wsig) {
-                    p.sig_load_prev[c] = (float)s_load;
-                    p.sig_geo_prev[c] = (float)(p.compressible_column
-                                                    ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
-                                                    : s_full);
+        if (j0 >= 2 && j0 + 3 <= ny - 1) {
+            // interior item: rows j0-2..j0+3 of u, moments on rows j0-1..j0+2, all offsets fixed.
+            // Thread i owns column i: finished u on a 6-row x 3-column window in registers, its
+            // moments to LDS, then its two rhs values.
+            int i = threadIdx.x;
+            asm volatile("" : "+v"(i));   // keep column-derived addresses inside the item loop
+            double uo0 = 0.0, uo1 = 0.0;
+            if (i < nx) {
+                const int a = min(max(i - 1, 0), nx - 3);   // window columns a..a+2 (CPU edge rules)
+                const int ci = i - a;                       // 0 at i = 0, 2 at i = nx-1, else 1
+                // sliding window over rows j0-2..j0+3: centre value, x-first-derivative and
+                // x-second-difference of each row (CPU edge rules); moment row j - 1 once row j is in
+                double cv0 = 0.0, cv1 = 0.0, dd0 = 0.0, dd1 = 0.0, xx1 = 0.0;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    const int b0 = r * nx + a;
+                    double x0 = su[b0], x1 = su[b0 + 1], x2 = su[b0 + 2];
+                    if (np > 0) {
+                        x0 += st.dt * (st.ain * (double)skin[b0]);
+                        x1 += st.dt * (st.ain * (double)skin[b0 + 1]);
+                        x2 += st.dt * (st.ain * (double)skin[b0 + 2]);
+                    }
+                    x0 -= uoff;
+                    x1 -= uoff;
+                    x2 -= uoff;
+                    const double cv2 = ci == 0 ? x0 : (ci == 1 ? x1 : x2);
+                    const double dd2 = ci == 1 ? (x2 - x0) * (0.5 * p.inv_dx)
+                                               : (ci == 0 ? (x1 - x0) * p.inv_dx : (x2 - x1) * p.inv_dx);
+                    const double xx2 = (x2 - 2.0 * x1 + x0) * p.inv_dx2;
+                    if (r == 2) uo0 = cv2;
+                    if (r == 3) uo1 = cv2;
+                    if (r >= 2) {
+                        const int m = r - 2;   // moment row j0 - 1 + m sits on window row r - 1
+                        const double D = (double)sD[m * nx + i];
+                        const double uxx = xx1;
+                        const double uyy = (cv2 - 2.0 * cv1 + cv0) * p.inv_dy2;
+                        const double uxy = (dd2 - dd0) * (0.5 * p.inv_dy);
+                        sMxx[m * nx + i] = -D * (nu * uyy + uxx);
+                        sMyy[m * nx + i] = -D * (nu * uxx + uyy);
+                        sMxy[m * nx + i] = -D * (1.0 - nu) * uxy;
+                    }
+                    cv0 = cv1; cv1 = cv2;
+                    dd0 = dd1; dd1 = dd2;
+                    xx1 = xx2;
+                }
+                if (st.write_u >= 0) {
+                    double* un = ubuf(p, st.write_u);
+                    un[i + (long)j0 * nx] = uo0;
+                    un[i + (long)(j0 + 1) * nx] = uo1;
                 }
             }
-            if (r == 0) sf[i].x = f;
-            else sf[i].y = f;
+            __syncthreads();
+            FI_SUBMARK(5);
+            if (i < nx) {
+                const int a = min(max(i - 1, 0), nx - 3);
+                const int ci = i - a;
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int j = j0 + r, m = r + 1;   // moment row of output row j
+                    const long c = i + (long)j * nx;
+                    auto mx = [&](const double* M, int mm, int cc) { return M[mm * nx + a + cc]; };
+                    auto d1m = [&](int mm) {
+                        return ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
+                                       : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
+                                                  : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                    };
+                    const double bxx = (mx(sMxx, m, 2) - 2.0 * mx(sMxx, m, 1) + mx(sMxx, m, 0)) * p.inv_dx2;
+                    const double byy = (sMyy[(m + 1) * nx + i] - 2.0 * sMyy[m * nx + i] + sMyy[(m - 1) * nx + i]) *
+                                       p.inv_dy2;
+                    const double bxy = (d1m(m + 1) - d1m(m - 1)) * (0.5 * p.inv_dy);
+                    const int idx = r * nx + i;
+                    const double u = r ? uo1 : uo0;
+                    const double msk = sM[idx];
+                    const double Haf = hab((double)sH[idx], (double)sZo[idx] + u, p.rho_sw_ice);
+                    const double s_ice = p.rho_ice * (Haf - (double)sHr[idx]);
+                    const double s_load = msk * (s_ice + (double)sLw[idx]);
+                    const double s_mant = p.rho_um * u;
+                    const double s_full = s_load + msk * ((double)sLt[idx] + s_mant);
+                    const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
+                    const float f = (float)(rhs * (double)sI2e[idx]);
+                    if (wsig) {
+                        p.sig_load_prev[c] = (float)s_load;
+                        p.sig_geo_prev[c] = (float)(p.compressible_column
+                                                        ? s_load + msk * (s_mant - p.rho_um * (double)p.u_ref[c])
+                                                        : s_full);
+                    }
+                    if (r == 0) sf[i].x = f;
+                    else sf[i].y = f;
+                }
+            }
+        } else {
+            // finish the stage displacement: spectral term, corner BC; keep u_{n+1} when asked
+            for (int idx = threadIdx.x; idx < totU; idx += NT) {
+                double u = su[idx];
+                if (np > 0) u += st.dt * (st.ain * (double)skin[idx]);
+                u -= uoff;
+                su[idx] = u;
+                const int r = fdivi(idx, inv_nx);
+                const int j = ju0 + r;
+                if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[idx - r * nx + (long)j * nx] = u;
+            }
+            __syncthreads();
+            FI_SUBMARK(4);
+            // plate moments M = -D (...) on rows [jm0, jm1)
+            for (int idx = threadIdx.x; idx < totM; idx += NT) {
+                const int r = fdivi(idx, inv_nx), i = idx - r * nx, j = jm0 + r;
+                const double D = (double)sD[idx];
+                const double uxx = d2x(su + (j - ju0) * nx, i, nx, p.inv_dx2);
+                const double uyy = d2y(su, ju0, j, i, nx, ny, p.inv_dy2);
+                const double uxy = dxy(su, ju0, j, i, nx, ny, p.inv_dx, p.inv_dy);
+                sMxx[idx] = -D * (nu * uyy + uxx);
+                sMyy[idx] = -D * (nu * uxx + uyy);
+                sMxy[idx] = -D * (1.0 - nu) * uxy;
+            }
+            __syncthreads();
+            FI_SUBMARK(5);
+            // rhs = -g sigma_full + dxx Mxx + 2 dxy Mxy + dyy Myy, times K/(2 eta)
+            for (int idx = threadIdx.x; idx < totO; idx += NT) {
+                const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+                float f = 0.0f;
+                if (j < ny) {
+                    const long c = i + (long)j * nx;
+                    const double bxx = d2x(sMxx + (j - jm0) * nx, i, nx, p.inv_dx2);
+                    const double byy = d2y(sMyy, jm0, j, i, nx, ny, p.inv_dy2);
+                    const double bxy = dxy(sMxy, jm0, j, i, nx, ny, p.inv_dx, p.inv_dy);
+                    const double u = su[(j - ju0) * nx + i];
+                    const double m = sM[idx];
+                    const double Haf = hab((double)sH[idx], (double)sZo[idx] + u, p.rho_sw_ice);
+                    const double s_ice = p.rho_ice * (Haf - (double)sHr[idx]);
+                    const double s_load = m * (s_ice + (double)sLw[idx]);
+                    const double s_mant = p.rho_um * u;
+                    const double s_full = s_load + m * ((double)sLt[idx] + s_mant);
+                    const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
+                    f = (float)(rhs * (double)sI2e[idx]);
+                    if (wsig) {
+                        p.sig_load_prev[c] = (float)s_load;
+                        p.sig_geo_prev[c] = (float)(p.compressible_column
+                                                        ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
+                                                        : s_full);
+                    }
+                }
+                if (r == 0) sf[i].x = f;
+                else sf[i].y = f;
+            }
         }
         __syncthreads();
         FI_SUBMARK(6);
