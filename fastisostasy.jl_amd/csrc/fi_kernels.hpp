@@ -18,6 +18,8 @@
 // and sea-surface convolutions on the zero-padded nfx x nfy grid (both in one pass) and the
 // pointwise sparse diagnostics.
 #pragma once
+#include <type_traits>
+
 #include "fi_device.hpp"
 
 namespace fi {
@@ -747,21 +749,26 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             __syncthreads();
         }
         FI_SUBMARK(3);
-        if (j0 >= 2 && j0 + 3 <= ny - 1) {
-            // interior item: rows j0-2..j0+3 of u, moments on rows j0-1..j0+2, all offsets fixed.
-            // Thread i owns column i: finished u on a 6-row x 3-column window in registers, its
-            // moments to LDS, then its two rhs values.
+        // Row-pair kinds with fixed row offsets (relative to j0: first u row JU, count NU; first
+        // moment row JM, count NM; NO output rows; TOP: row 0 is the first u and moment row; BOT:
+        // row ny-1 is the last one, and output row LO when LO >= 0). Thread i owns column i: it
+        // finishes u on an NU-row x 3-column window in registers, writes its moments to LDS, and
+        // after one workgroup barrier evaluates its output rows. The row rules are the CPU edge
+        // rules of d2y/dxy: one-sided second difference = central one at the neighbouring row,
+        // one-sided first difference at the grid edge.
+        auto window = [&](auto JU_, auto NU_, auto JM_, auto NM_, auto NO_, auto TOP_, auto BOT_, auto LO_) {
+            constexpr int JU = decltype(JU_)::value, NU = decltype(NU_)::value, JM = decltype(JM_)::value,
+                          NM = decltype(NM_)::value, NO = decltype(NO_)::value, LO = decltype(LO_)::value;
+            constexpr bool TOP = decltype(TOP_)::value, BOT = decltype(BOT_)::value;
             int i = threadIdx.x;
             asm volatile("" : "+v"(i));   // keep column-derived addresses inside the item loop
-            double uo0 = 0.0, uo1 = 0.0;
+            double uo[NO];
+            const int a = min(max(i - 1, 0), nx - 3);   // window columns a..a+2 (d2x/d1x edge rules)
+            const int ci = i - a;                       // 0 at i = 0, 2 at i = nx-1, else 1
             if (i < nx) {
-                const int a = min(max(i - 1, 0), nx - 3);   // window columns a..a+2 (CPU edge rules)
-                const int ci = i - a;                       // 0 at i = 0, 2 at i = nx-1, else 1
-                // sliding window over rows j0-2..j0+3: centre value, x-first-derivative and
-                // x-second-difference of each row (CPU edge rules); moment row j - 1 once row j is in
-                double cv0 = 0.0, cv1 = 0.0, dd0 = 0.0, dd1 = 0.0, xx1 = 0.0;
+                double cv[NU], dd[NU], xx[NU];   // centre value, x-first-derivative, x-second-difference
 #pragma unroll
-                for (int r = 0; r < 6; ++r) {
+                for (int r = 0; r < NU; ++r) {
                     const int b0 = r * nx + a;
                     double x0 = su[b0], x1 = su[b0 + 1], x2 = su[b0 + 2];
                     if (np > 0) {
@@ -772,53 +779,58 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     x0 -= uoff;
                     x1 -= uoff;
                     x2 -= uoff;
-                    const double cv2 = ci == 0 ? x0 : (ci == 1 ? x1 : x2);
-                    const double dd2 = ci == 1 ? (x2 - x0) * (0.5 * p.inv_dx)
-                                               : (ci == 0 ? (x1 - x0) * p.inv_dx : (x2 - x1) * p.inv_dx);
-                    const double xx2 = (x2 - 2.0 * x1 + x0) * p.inv_dx2;
-                    if (r == 2) uo0 = cv2;
-                    if (r == 3) uo1 = cv2;
-                    if (r >= 2) {
-                        const int m = r - 2;   // moment row j0 - 1 + m sits on window row r - 1
-                        const double D = (double)sD[m * nx + i];
-                        const double uxx = xx1;
-                        const double uyy = (cv2 - 2.0 * cv1 + cv0) * p.inv_dy2;
-                        const double uxy = (dd2 - dd0) * (0.5 * p.inv_dy);
-                        sMxx[m * nx + i] = -D * (nu * uyy + uxx);
-                        sMyy[m * nx + i] = -D * (nu * uxx + uyy);
-                        sMxy[m * nx + i] = -D * (1.0 - nu) * uxy;
-                    }
-                    cv0 = cv1; cv1 = cv2;
-                    dd0 = dd1; dd1 = dd2;
-                    xx1 = xx2;
+                    cv[r] = ci == 0 ? x0 : (ci == 1 ? x1 : x2);
+                    dd[r] = ci == 1 ? (x2 - x0) * (0.5 * p.inv_dx)
+                                    : (ci == 0 ? (x1 - x0) * p.inv_dx : (x2 - x1) * p.inv_dx);
+                    xx[r] = (x2 - 2.0 * x1 + x0) * p.inv_dx2;
                 }
+#pragma unroll
+                for (int o = 0; o < NO; ++o) uo[o] = cv[o - JU];
                 if (st.write_u >= 0) {
                     double* un = ubuf(p, st.write_u);
-                    un[i + (long)j0 * nx] = uo0;
-                    un[i + (long)(j0 + 1) * nx] = uo1;
+#pragma unroll
+                    for (int o = 0; o < NO; ++o) un[i + (long)(j0 + o) * nx] = uo[o];
+                }
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    const int ur = JM + m - JU;   // window row of this moment row
+                    const bool first = TOP && m == 0, last = BOT && m == NM - 1;
+                    const int c = first ? ur + 1 : (last ? ur - 1 : ur);
+                    const int lo = first ? ur : ur - 1, hi = last ? ur : ur + 1;
+                    const double D = (double)sD[m * nx + i];
+                    const double uxx = xx[ur];
+                    const double uyy = (cv[c + 1] - 2.0 * cv[c] + cv[c - 1]) * p.inv_dy2;
+                    const double uxy = (first || last) ? (dd[hi] - dd[lo]) * p.inv_dy
+                                                       : (dd[hi] - dd[lo]) * (0.5 * p.inv_dy);
+                    sMxx[m * nx + i] = -D * (nu * uyy + uxx);
+                    sMyy[m * nx + i] = -D * (nu * uxx + uyy);
+                    sMxy[m * nx + i] = -D * (1.0 - nu) * uxy;
                 }
             }
             __syncthreads();
             FI_SUBMARK(5);
             if (i < nx) {
-                const int a = min(max(i - 1, 0), nx - 3);
-                const int ci = i - a;
+                auto mx = [&](const double* M, int mm, int cc) { return M[mm * nx + a + cc]; };
+                auto d1m = [&](int mm) {
+                    return ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
+                                   : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
+                                              : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                };
+                float f[2] = {0.0f, 0.0f};
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    const int j = j0 + r, m = r + 1;   // moment row of output row j
-                    const long c = i + (long)j * nx;
-                    auto mx = [&](const double* M, int mm, int cc) { return M[mm * nx + a + cc]; };
-                    auto d1m = [&](int mm) {
-                        return ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
-                                       : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
-                                                  : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
-                    };
-                    const double bxx = (mx(sMxx, m, 2) - 2.0 * mx(sMxx, m, 1) + mx(sMxx, m, 0)) * p.inv_dx2;
-                    const double byy = (sMyy[(m + 1) * nx + i] - 2.0 * sMyy[m * nx + i] + sMyy[(m - 1) * nx + i]) *
-                                       p.inv_dy2;
-                    const double bxy = (d1m(m + 1) - d1m(m - 1)) * (0.5 * p.inv_dy);
-                    const int idx = r * nx + i;
-                    const double u = r ? uo1 : uo0;
+                for (int o = 0; o < NO; ++o) {
+                    const int mr = o - JM;   // moment row of output row j0 + o
+                    const bool first = TOP && o == 0, last = o == LO;
+                    const int c = first ? mr + 1 : (last ? mr - 1 : mr);
+                    const int lo = first ? mr : mr - 1, hi = last ? mr : mr + 1;
+                    const double bxx = (mx(sMxx, mr, 2) - 2.0 * mx(sMxx, mr, 1) + mx(sMxx, mr, 0)) * p.inv_dx2;
+                    const double byy =
+                        (sMyy[(c + 1) * nx + i] - 2.0 * sMyy[c * nx + i] + sMyy[(c - 1) * nx + i]) * p.inv_dy2;
+                    const double bxy = (first || last) ? (d1m(hi) - d1m(lo)) * p.inv_dy
+                                                       : (d1m(hi) - d1m(lo)) * (0.5 * p.inv_dy);
+                    const int idx = o * nx + i;
+                    const long cc = i + (long)(j0 + o) * nx;
+                    const double u = uo[o];
                     const double msk = sM[idx];
                     const double Haf = hab((double)sH[idx], (double)sZo[idx] + u, p.rho_sw_ice);
                     const double s_ice = p.rho_ice * (Haf - (double)sHr[idx]);
@@ -826,17 +838,42 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const double s_mant = p.rho_um * u;
                     const double s_full = s_load + msk * ((double)sLt[idx] + s_mant);
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
-                    const float f = (float)(rhs * (double)sI2e[idx]);
+                    f[o] = (float)(rhs * (double)sI2e[idx]);
                     if (wsig) {
-                        p.sig_load_prev[c] = (float)s_load;
-                        p.sig_geo_prev[c] = (float)(p.compressible_column
-                                                        ? s_load + msk * (s_mant - p.rho_um * (double)p.u_ref[c])
-                                                        : s_full);
+                        p.sig_load_prev[cc] = (float)s_load;
+                        p.sig_geo_prev[cc] = (float)(p.compressible_column
+                                                         ? s_load + msk * (s_mant - p.rho_um * (double)p.u_ref[cc])
+                                                         : s_full);
                     }
-                    if (r == 0) sf[i].x = f;
-                    else sf[i].y = f;
                 }
+                sf[i] = make_float2(f[0], f[1]);
             }
+        };
+        using std::integral_constant;
+#define FI_WIN(JU, NU, JM, NM, NO, TOP, BOT, LO)                                                          \
+    window(integral_constant<int, JU>{}, integral_constant<int, NU>{}, integral_constant<int, JM>{},      \
+           integral_constant<int, NM>{}, integral_constant<int, NO>{}, integral_constant<bool, TOP>{},     \
+           integral_constant<bool, BOT>{}, integral_constant<int, LO>{})
+        // the kind of this row pair; each is used only where the row ranges computed above are
+        // exactly its own (ny >= 6), anything else takes the general path
+        const int rj = ju0 - j0, rm = jm0 - j0, nu_ = ju1 - ju0, nm_ = jm1 - jm0, no_ = je - j0;
+        const int kind = (ny < 6 || nx < 3) ? -1
+                         : (rj == -2 && nu_ == 6 && rm == -1 && nm_ == 4 && no_ == 2 && j0 >= 2 && j0 + 3 <= ny - 1) ? 0
+                         : (j0 == 0 && rj == 0 && nu_ == 4 && rm == 0 && nm_ == 3 && no_ == 2)                     ? 1
+                         : (j0 == ny - 2 && rj == -2 && nu_ == 4 && rm == -1 && nm_ == 3 && no_ == 2)              ? 2
+                         : (j0 == ny - 3 && rj == -2 && nu_ == 5 && rm == -1 && nm_ == 4 && no_ == 2)              ? 3
+                         : (j0 == ny - 1 && rj == -3 && nu_ == 4 && rm == -2 && nm_ == 3 && no_ == 1)              ? 4
+                                                                                                                  : -1;
+        if (kind == 0) {
+            FI_WIN(-2, 6, -1, 4, 2, false, false, -1);
+        } else if (kind == 1) {
+            FI_WIN(0, 4, 0, 3, 2, true, false, -1);
+        } else if (kind == 2) {
+            FI_WIN(-2, 4, -1, 3, 2, false, true, 1);
+        } else if (kind == 3) {
+            FI_WIN(-2, 5, -1, 4, 2, false, true, -1);
+        } else if (kind == 4) {
+            FI_WIN(-3, 4, -2, 3, 1, false, true, 0);
         } else {
             // finish the stage displacement: spectral term, corner BC; keep u_{n+1} when asked
             for (int idx = threadIdx.x; idx < totU; idx += NT) {
@@ -892,6 +929,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                 else sf[i].y = f;
             }
         }
+#undef FI_WIN
         __syncthreads();
         FI_SUBMARK(6);
         float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, -1);

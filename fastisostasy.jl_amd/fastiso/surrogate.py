@@ -327,6 +327,7 @@ class SurrogateRunner:
             # a surrogate period
             u_ck, t_ck = eng.get_field("u"), t
             inc, tp = [hist[-2], hist[-1]], t
+            du_start = hist[-1]
 
             def extrapolated():
                 # constant increment (order 1) or linearly changing increment (order 2)
@@ -352,6 +353,12 @@ class SurrogateRunner:
             if dev <= self.tol:
                 n_sur += period - 1
                 n_plain += 1
+                # seed the next period with the plain check increment and the slope of the plain
+                # increments across the period (from its first plain increment to the check, period
+                # steps apart). The slope between the last surrogate step and the check would carry
+                # the check's one-off relaxation of the surrogate's error, amplified period-fold.
+                slope = (du_p - du_start) / period
+                hist.append(du_p - slope)
                 hist.append(du_p)
                 note(tp + dt, f"check-{used}", dev)
             else:

@@ -48,7 +48,9 @@ def test_rhs_cosine_closed_form():
     np.testing.assert_allclose(sim.engine.get_field("u"), ubc, atol=1e-12)
 
 
-@pytest.mark.parametrize("n,seed,surface", [(32, 11, True), (64, 5, True), (48, 7, False), (96, 3, True)])
+# odd sizes end with a one-row item and a three-row moment window at the bottom edge
+@pytest.mark.parametrize("n,seed,surface", [(32, 11, True), (64, 5, True), (48, 7, False), (96, 3, True),
+                                            (45, 13, True), (35, 17, False)])
 def test_rhs_sequence_vs_oracle(n, seed, surface):
     """A sequence of RHS calls through two sparse updates: the first at t0 reads zero loads, the
     second (t = 600 after t = 500) reads the loads of the call before it."""
