@@ -11,7 +11,7 @@ constexpr int NT = 512;  // threads per workgroup (eight wave64: two per SIMD)
 
 enum { MODE_RHS = 0, MODE_INTEGRATE = 1, MODE_SPARSE_ONLY = 2 };
 enum { ICE_EXTERNAL = 0, ICE_SNAPSHOTS = 1, ICE_SEPARABLE = 2 };
-enum { ERR_TIME_RANGE = 3, ERR_NONFINITE = 4, ERR_ABORT = 5, ERR_DT_MIN = 7 };
+enum { ERR_TIME_RANGE = 3, ERR_NONFINITE = 4, ERR_ABORT = 5, ERR_DT_MIN = 7, ERR_BSL_RANGE = 8 };
 
 // BS3 tableau (Bogacki & Shampine 1989) and PI-controller constants
 #define BS3_A21 0.5
@@ -39,6 +39,7 @@ struct Ctl {
     double t_rhs;       // RHS mode: evaluation time
     double t_last;      // time of the last RHS call
     double V_af, V_den, V_pov, delta_V, z_bsl;
+    double A_bsl;       // current ocean surface of the BSL rule (m^2)
     long long count, nrhs, nacc, nrej, steps;
     int cur;       // U buffer holding u_n
     int k1;        // K slot of the FSAL derivative
@@ -77,6 +78,12 @@ struct KP {
     int rigid_litho, rigid_mantle, compressible_column, lv_surface, interactive, vol_goelzer, den_goelzer,
         pov_goelzer;
     int ice_mode, ice_nt, ice_flat, mt_double;
+    // barystatic sea level: rule (FI_BSL_*), internal update flag, reference ocean surface and the
+    // rule's table (z -> A for the piecewise-constant rule, t -> z for the imposed one)
+    int bsl_rule, bsl_internal, bsl_n, bsl_flat;
+    double bsl_A_ref;
+    const double* bsl_x;
+    const double* bsl_y;
     const double* ice_t;
     const double* ice_a;
     const float* ice_H;

@@ -401,11 +401,15 @@ struct fi_ctx {
     bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_href = false;
     int ice_mode = ICE_EXTERNAL, ice_flat = 0;
     std::vector<double> ice_t, ice_a, ice_H;
+    // barystatic sea-level rule (fi_set_bsl); A_OCEAN_PD (src/barystatic_sealevel.jl:1) by default
+    int bsl_rule = FI_BSL_CONSTANT, bsl_update = FI_BSL_INTERNAL_UPDATE, bsl_flat = 0;
+    double bsl_A_ref = 3.625e14;
+    std::vector<double> bsl_x, bsl_y;
     std::vector<double> tout;
     std::vector<double> H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
     std::vector<double> H_ext_host;
     // device memory
-    DBuf<double> U, ice_t_d, ice_a_d, part, vpart;
+    DBuf<double> U, ice_t_d, ice_a_d, part, vpart, bsl_x_d, bsl_y_d;
     DBuf<float> K, kcorner, mult, Df, inv2eta, H_af_ref_d, u_ref_d, z_b_ref_d, z_ss_ref_d, ue_ref_d, H_ext, ue, dz_ss,
         z_ss, zoff, zoff_trig, LW, LT, H_water, sig_load_prev, sig_geo_prev, ue_raw, dz_raw, ext_corners, ice_H_d;
     DBuf<unsigned char> maskd, mg, mo;
@@ -513,6 +517,7 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     }
     if (bar.abort) return fail(c, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
     if (c->ctl.err == ERR_TIME_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+    if (c->ctl.err == ERR_BSL_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation0D out of range.");
     if (c->ctl.err == ERR_DT_MIN) return fail(c, FI_ERR_DT_MIN, "dt below minimum");
     if (c->ctl.err) return fail(c, FI_ERR_DEVICE, "device error");
     return FI_OK;
@@ -739,6 +744,29 @@ int fi_set_ice_separable(fi_ctx* c, const double* t, const double* a, int32_t nt
     return FI_OK;
 }
 
+int fi_set_bsl(fi_ctx* c, int32_t rule, int32_t update, double A_ref, const double* x, const double* y,
+               int32_t n, int32_t flat) {
+    if (!c) return FI_ERR_INVALID;
+    if (rule < FI_BSL_CONSTANT || rule > FI_BSL_IMPOSED) return fail(c, FI_ERR_INVALID, "fi_set_bsl: unknown rule");
+    if (update != FI_BSL_INTERNAL_UPDATE && update != FI_BSL_EXTERNAL_UPDATE)
+        return fail(c, FI_ERR_INVALID, "fi_set_bsl: unknown update");
+    if ((rule == FI_BSL_CONSTANT_OCEAN_SURFACE || rule == FI_BSL_PIECEWISE_CONSTANT) && !(A_ref > 0))
+        return fail(c, FI_ERR_INVALID, "fi_set_bsl: the reference ocean surface must be positive");
+    const bool table = rule == FI_BSL_PIECEWISE_CONSTANT || rule == FI_BSL_IMPOSED;
+    if (table) {
+        if (!x || !y || n < 1) return fail(c, FI_ERR_INVALID, "fi_set_bsl: the rule needs a table");
+        for (int k = 1; k < n; ++k)
+            if (!(x[k] > x[k - 1])) return fail(c, FI_ERR_INVALID, "fi_set_bsl: table knots must increase");
+    }
+    c->bsl_rule = rule;
+    c->bsl_update = update;
+    c->bsl_A_ref = A_ref > 0 ? A_ref : 3.625e14;
+    c->bsl_flat = flat;
+    c->bsl_x.assign(table ? x : nullptr, table ? x + n : nullptr);
+    c->bsl_y.assign(table ? y : nullptr, table ? y + n : nullptr);
+    return FI_OK;
+}
+
 int fi_set_output_times(fi_ctx* c, const double* t, int32_t n) {
     if (!c || n < 0 || (n > 0 && !t)) return fail(c, FI_ERR_INVALID, "fi_set_output_times: bad args");
     c->tout.assign(t, t + n);
@@ -869,6 +897,12 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     HIPCHECK(c, c->pB.alloc((size_t)c->nfhx * ny));
     HIPCHECK(c, c->oA.alloc((size_t)c->nfhx * (ny + 2)));
     HIPCHECK(c, c->oB.alloc((size_t)c->nfhx * (ny + 2)));
+    if (!c->bsl_x.empty()) {
+        if (upload(c, c->bsl_x_d, c->bsl_x) || upload(c, c->bsl_y_d, c->bsl_y)) return FI_ERR_DEVICE;
+    } else {
+        HIPCHECK(c, c->bsl_x_d.alloc(1));
+        HIPCHECK(c, c->bsl_y_d.alloc(1));
+    }
     if (c->ice_mode != ICE_EXTERNAL) {
         if (upload(c, c->ice_t_d, c->ice_t) || upload(c, c->ice_a_d, c->ice_a) || upload(c, c->ice_H_d, to_f(c->ice_H)))
             return FI_ERR_DEVICE;
@@ -958,6 +992,13 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.vol_goelzer = g.volume_contribution == FI_GOELZER_CONTRIBUTION;
     kp.den_goelzer = g.density_contribution == FI_GOELZER_CONTRIBUTION;
     kp.pov_goelzer = g.adjustment_contribution == FI_GOELZER_CONTRIBUTION;
+    kp.bsl_rule = c->bsl_rule;
+    kp.bsl_internal = c->bsl_update == FI_BSL_INTERNAL_UPDATE;
+    kp.bsl_n = (int)c->bsl_x.size();
+    kp.bsl_flat = c->bsl_flat;
+    kp.bsl_A_ref = c->bsl_A_ref;
+    kp.bsl_x = c->bsl_x_d.p;
+    kp.bsl_y = c->bsl_y_d.p;
     kp.ice_mode = c->ice_mode;
     kp.ice_nt = (int)c->ice_t.size();
     kp.ice_flat = c->ice_flat;
@@ -1051,6 +1092,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     c->ctl.t = t0;
     c->ctl.t_last = t0;
     c->ctl.z_bsl = g.bsl_z;
+    c->ctl.A_bsl = c->bsl_A_ref;
     c->ctl.qold = PI_QOLDINIT;
     c->ctl.cur = 0;
     c->ctl.k1 = 0;
@@ -1440,6 +1482,7 @@ int fi_get_stats(fi_ctx* c, fi_stats* o) {
     o->V_pov = c->ctl.V_pov;
     o->delta_V = c->ctl.delta_V;
     o->z_bsl = c->ctl.z_bsl;
+    o->A_ocean = c->ctl.A_bsl;
     o->last_EEst = c->ctl.EEst;
     o->device_ms = c->device_ms;
     o->n_launch = c->n_launch;
@@ -1562,6 +1605,7 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     for (int m = 0; m < nm; ++m) {
         const int e = b->mem[m]->ctl.err;
         if (e == ERR_TIME_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation2D out of range.");
+        if (e == ERR_BSL_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation0D out of range.");
         if (e == ERR_DT_MIN) return bfail(b, FI_ERR_DT_MIN, "member " + std::to_string(m) + ": dt below minimum");
         if (e) return bfail(b, FI_ERR_DEVICE, "member " + std::to_string(m) + ": device error");
     }

@@ -1214,30 +1214,115 @@ __device__ __forceinline__ void phase_sp_c2r(const KP& p, const Team& tm, char* 
     }
 }
 
-// pointwise sparse diagnostics; volume partial sums for internal_update_bsl!
-// (one partial per row pair in p.vpart)
+// volume contributions of the state before a sparse update, for update_delta_V!
+// (src/sealevel.jl:254-331): H_af with the current bedrock and the old sea surface, H_ice, and the
+// ocean column max(z_ss - z_b, 0), each times the cell area; one partial per row pair in p.vpart
+__device__ __forceinline__ void phase_sp_volumes(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                                 const IceT it, double* red) {
+    const int nx = p.nx;
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        const int ncell = (min(2 * item + 2, p.ny) - 2 * item) * nx;
+        for (int q = threadIdx.x; q < ncell; q += NT) {
+            const long c = (long)2 * item * nx + q;
+            const double u = u_raw(p, st, c) - uoff;
+            const double H = ice_at(p, it, c);
+            const double zss_old = p.z_ss[c];
+            const double zb_old = (double)p.z_b_ref[c] + (double)p.ue[c] + u;
+            acc[0] += hab(H, zb_old - zss_old, p.rho_sw_ice) * p.A;
+            acc[1] += H * p.A;
+            acc[2] += fmax(zss_old - zb_old, 0.0) * p.A;
+        }
+        block_sum<3>(red, acc);
+        if (threadIdx.x == 0) {
+            p.vpart[4 * item + 0] = acc[0];
+            p.vpart[4 * item + 1] = acc[1];
+            p.vpart[4 * item + 2] = acc[2];
+        }
+    }
+}
+
+// TimeInterpolation0D (src/interpolations.jl:28-49): linear between ascending knots, the knot
+// value on an exact hit; outside the knots the end value, flagged in *err unless flat
+__device__ __forceinline__ double interp0d(const double* x, const double* y, int n, int flat, double t, int* err) {
+    if (t < x[0]) {
+        if (!flat) *err = 1;
+        return y[0];
+    }
+    if (t > x[n - 1]) {
+        if (!flat) *err = 1;
+        return y[n - 1];
+    }
+    int lo = 0, hi = n - 1;   // first knot with x >= t (searchsortedfirst)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] >= t) hi = mid;
+        else lo = mid + 1;
+    }
+    if (x[lo] == t) return y[lo];
+    const int i = lo - 1;
+    return y[i] + (y[i + 1] - y[i]) / (x[i + 1] - x[i]) * (t - x[i]);
+}
+
+// Sum the volume partials (fixed order, identical in every workgroup), update V_af, V_den, V_pov
+// and delta_V (update_delta_V!, src/sealevel.jl:254-262) and, with an internal update, the
+// barystatic sea level and ocean surface (update_bsl!(bsl, -delta_V, t),
+// src/barystatic_sealevel.jl:215-234); count the sparse update.
+__device__ __forceinline__ void volumes_and_bsl(const KP& p, Ctl& s, double t, double* red) {
+    double vs[3] = {0.0, 0.0, 0.0};
+    for (int w = threadIdx.x; w < p.nrowitems; w += NT) {
+        vs[0] += p.vpart[4 * w];
+        vs[1] += p.vpart[4 * w + 1];
+        vs[2] += p.vpart[4 * w + 2];
+    }
+    block_sum<3>(red, vs);
+    if (threadIdx.x == 0) {
+        const double Vold = s.V_af + s.V_den + s.V_pov;
+        s.V_af = p.vol_goelzer ? vs[0] * p.rho_ice_sw : 0.0;
+        s.V_den = p.den_goelzer ? vs[1] * (p.rho_ice / p.rho_water - p.rho_ice / p.rho_seawater) : 0.0;
+        s.V_pov = p.pov_goelzer ? vs[2] : 0.0;
+        {
+            s.delta_V = (s.V_af + s.V_den + s.V_pov) - Vold;
+            if (p.bsl_internal) {
+                const double dV = -s.delta_V;
+                int e = 0;
+                if (p.bsl_rule == 1) {          // ConstantOceanSurfaceBSL
+                    s.A_bsl = p.bsl_A_ref;
+                    s.z_bsl += dV / s.A_bsl;
+                } else if (p.bsl_rule == 2) {   // PiecewiseConstantBSL
+                    s.z_bsl += dV / s.A_bsl;
+                    s.A_bsl = interp0d(p.bsl_x, p.bsl_y, p.bsl_n, p.bsl_flat, s.z_bsl, &e);
+                } else if (p.bsl_rule == 3) {   // ImposedBSL
+                    s.z_bsl = interp0d(p.bsl_x, p.bsl_y, p.bsl_n, p.bsl_flat, t, &e);
+                } else {                        // ConstantBSL
+                    s.z_bsl = p.bsl_z;
+                }
+                if (e) s.err = ERR_BSL_RANGE;
+            }
+            s.count += 1;
+        }
+    }
+    __syncthreads();
+}
+
+// pointwise sparse diagnostics with the updated barystatic sea level z_bsl
 __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, const StageDesc& st, double uoff,
-                                               const IceT it, double* red) {
+                                               const IceT it, double z_bsl) {
     const float* ec = p.ext_corners;
     const double oe = ((0.25 * ec[0] + 0.25 * ec[1]) + (0.25 * ec[2] + 0.25 * ec[3])) - p.ebc_x;
     const double og = ((0.25 * ec[4] + 0.25 * ec[5]) + (0.25 * ec[6] + 0.25 * ec[7])) - p.sbc_x;
     const int nx = p.nx;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
-      double acc[3] = {0.0, 0.0, 0.0};
       const int ncell = (min(2 * item + 2, p.ny) - 2 * item) * nx;
       for (int q = threadIdx.x; q < ncell; q += NT) {
         const long c = (long)2 * item * nx + q;
         const double u = u_raw(p, st, c) - uoff;
         const double H = ice_at(p, it, c);
-        const double ue_old = p.ue[c], zss_old = p.z_ss[c];
+        const double ue_old = p.ue[c];
         const double zb_old = (double)p.z_b_ref[c] + ue_old + u;
-        const double haf_pre = hab(H, zb_old - zss_old, p.rho_sw_ice);
-        acc[0] += haf_pre * p.A;
-        acc[1] += H * p.A;
-        acc[2] += fmax(zss_old - zb_old, 0.0) * p.A;
         const double ue_new = p.rigid_litho ? 0.0 : (double)p.ue_raw[c] - oe;
         const double dz_new = p.lv_surface ? (double)p.dz_raw[c] - og : (double)p.dz_ss[c];
-        const double zss_new = (double)p.z_ss_ref[c] + dz_new + p.bsl_z;
+        const double zss_new = (double)p.z_ss_ref[c] + dz_new + z_bsl;
         const double haf = hab(H, zb_old - zss_new, p.rho_sw_ice);
         const bool mgc = haf > 0;
         const bool moc = (zss_new - zb_old > 0) && !mgc;
@@ -1256,12 +1341,6 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
         p.LT[c] = (float)(ca_litho - p.rho_um * (double)p.u_ref[c]);
         p.zoff[c] = (float)((double)p.z_b_ref[c] + ue_new - zss_new);
         p.zoff_trig[c] = (float)((double)p.z_b_ref[c] + ue_old - zss_new);
-      }
-      block_sum<3>(red, acc);
-      if (threadIdx.x == 0) {
-          p.vpart[4 * item + 0] = acc[0];
-          p.vpart[4 * item + 1] = acc[1];
-          p.vpart[4 * item + 2] = acc[2];
       }
     }
 }
@@ -1306,27 +1385,14 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
         phase_sp_cols(p, tm, smem);
         if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
         phase_sp_c2r(p, tm, smem);
+        // the volume partials read only the state before the update: they follow the last
+        // convolution phase without a barrier of their own (the BSL update needs all of them)
+        phase_sp_volumes(p, tm, st, uoff, it, red);
         if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
-        phase_sp_point(p, tm, st, uoff, it, red);
+        volumes_and_bsl(p, s, st.t, red);
+        if (s.err) return false;   // every workgroup sees the same error
+        phase_sp_point(p, tm, st, uoff, it, s.z_bsl);
         if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
-        double vs[3] = {0.0, 0.0, 0.0};
-        for (int w = threadIdx.x; w < p.nrowitems; w += NT) {
-            vs[0] += p.vpart[4 * w];
-            vs[1] += p.vpart[4 * w + 1];
-            vs[2] += p.vpart[4 * w + 2];
-        }
-        block_sum<3>(red, vs);
-        if (threadIdx.x == 0) {
-            const double v0 = vs[0], v1 = vs[1], v2 = vs[2];
-            const double Vold = s.V_af + s.V_den + s.V_pov;
-            s.V_af = p.vol_goelzer ? v0 * p.rho_ice_sw : 0.0;
-            s.V_den = p.den_goelzer ? v1 * (p.rho_ice / p.rho_water - p.rho_ice / p.rho_seawater) : 0.0;
-            s.V_pov = p.pov_goelzer ? v2 : 0.0;
-            s.delta_V = (s.V_af + s.V_den + s.V_pov) - Vold;
-            s.z_bsl = p.bsl_z;
-            s.count += 1;
-        }
-        __syncthreads();
         if (sparse_only) return true;
     }
     // store this call's loads if the next call may trigger (the sparse update reads the

@@ -7,10 +7,12 @@ all time stepping runs in the HIP library libfastiso_hip.so through its C ABI.
 from .domain import PhysicalConstants, RegionalDomain, SECONDS_PER_YEAR, meshgrid, nextfastfft
 from .bcs import (BoundaryConditions, CornerBC, ExtendedBCSpace, ExternallyUpdatedIceThickness,
                   RegularBCSpace, SeparableIceThickness, TimeInterpolatedIceThickness)
-from .sealevel import (ConstantBSL, GoelzerAdjustmentContribution, GoelzerDensityContribution,
-                       GoelzerVolumeContribution, InteractiveSealevelLoad, LaterallyConstantSeaSurface,
+from .sealevel import (A_OCEAN_PD, ConstantBSL, ConstantOceanSurfaceBSL, ExternalBSLUpdate,
+                       GoelzerAdjustmentContribution, GoelzerDensityContribution, GoelzerVolumeContribution,
+                       ImposedBSL, InteractiveSealevelLoad, InternalBSLUpdate, LaterallyConstantSeaSurface,
                        LaterallyVariableSeaSurface, NoAdjustmentContribution, NoDensityContribution,
-                       NoSealevelLoad, NoVolumeContribution, RegionalSeaLevel)
+                       NoSealevelLoad, NoVolumeContribution, PiecewiseConstantBSL, ReferenceBSL,
+                       RegionalSeaLevel, interpolate0d, update_bsl)
 from .solidearth import (CompressibleLithosphereColumn, CompressibleMantle, IncompressibleLithosphereColumn,
                          IncompressibleMantle, LaterallyConstantLithosphere, LaterallyVariableLithosphere,
                          MaxwellMantle, NoCalibration, RigidLithosphere, RigidMantle, SeakonCalibration,

@@ -10,7 +10,7 @@ import os
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ENGINE_PATH = os.path.join(PKG_ROOT, "lib", "libfastiso_hip.so")
 
-FI_ABI_VERSION = 2
+FI_ABI_VERSION = 3
 
 # status codes
 FI_OK = 0
@@ -67,6 +67,7 @@ class FiStats(C.Structure):
         ("V_af", C.c_double), ("V_den", C.c_double), ("V_pov", C.c_double), ("delta_V", C.c_double),
         ("z_bsl", C.c_double), ("last_EEst", C.c_double),
         ("device_ms", C.c_double), ("n_launch", C.c_int64), ("n_accept_timed", C.c_int64),
+        ("A_ocean", C.c_double),
     ]
 
     def as_dict(self):
@@ -102,6 +103,7 @@ def declare(lib):
         "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
         "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
+        "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),
         "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),
         "fi_init": (C.c_int, [P, C.c_double, C.c_double]),
         "fi_rhs": (C.c_int, [P, C.c_double, _dp, _dp]),

@@ -95,6 +95,15 @@ class Engine:
         t, a, H = _f64(t), _f64(a), fortran_flat(H_shape)
         self._check(self.lib.fi_set_ice_separable(self.h, _ptr(t), _ptr(a), len(t), _ptr(H), int(flat_bc)))
 
+    def set_bsl(self, rule, update, A_ref, x=None, y=None, flat_bc=False):
+        """fi_set_bsl: the BSL rule (0 constant, 1 constant ocean surface, 2 piecewise constant,
+        3 imposed), the update mechanism (0 internal, 1 external) and the rule's table."""
+        xs = _f64(x if x is not None else [])
+        ys = _f64(y if y is not None else [])
+        self._check(self.lib.fi_set_bsl(self.h, int(rule), int(update), float(A_ref),
+                                        _ptr(xs) if len(xs) else None, _ptr(ys) if len(ys) else None,
+                                        len(xs), int(flat_bc)))
+
     def set_output_times(self, t):
         t = _f64(t)
         self._check(self.lib.fi_set_output_times(self.h, _ptr(t), len(t)))

@@ -15,7 +15,7 @@ from .domain import PhysicalConstants
 from .engine import Engine
 from .greens import elastic_green, geoid_green
 from .sealevel import (GoelzerAdjustmentContribution, GoelzerDensityContribution, GoelzerVolumeContribution,
-                       InteractiveSealevelLoad, LaterallyVariableSeaSurface)
+                       InteractiveSealevelLoad, LaterallyVariableSeaSurface, bsl_inputs)
 from .solidearth import (CompressibleLithosphereColumn, LaterallyConstantLithosphere, MaxwellMantle,
                          RigidLithosphere, RigidMantle)
 
@@ -128,7 +128,7 @@ class Simulation:
         else:
             raise TypeError("unknown ice thickness forcing")
         self.inputs = {"cfg": cfg, "fields": fields, "ice": ice, "t_span": self.t_span,
-                       "tout": sorted(self.nout.t)}
+                       "tout": sorted(self.nout.t), "bsl": bsl_inputs(sealevel.bsl, sealevel.update_bsl)}
         self.engine = None
         if engine == "hip":
             self.engine = Engine(cfg)
@@ -151,6 +151,8 @@ def push_inputs(eng, inputs):
         eng.set_ice_snapshots(ice[1], ice[2], ice[3])
     elif ice[0] == "separable":
         eng.set_ice_separable(ice[1], ice[2], ice[3], ice[4])
+    if "bsl" in inputs:
+        eng.set_bsl(*inputs["bsl"])
     eng.set_output_times(inputs["tout"])
     eng.init(*inputs["t_span"])
     return eng

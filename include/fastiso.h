@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FI_ABI_VERSION 2
+#define FI_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------------------------ */
 enum {
@@ -152,6 +152,7 @@ typedef struct fi_stats {
     double device_ms;             /* integration-kernel time, HIP events on the context's stream */
     int64_t n_launch;             /* integration-kernel launches timed in device_ms             */
     int64_t n_accept_timed;       /* accepted steps inside those launches                       */
+    double A_ocean;               /* ocean surface of the BSL rule (bsl.A, m^2)                 */
 } fi_stats;
 
 /* Output callback, fired after an accepted step at which t >= the next output time
@@ -192,6 +193,29 @@ int fi_set_ice_snapshots(fi_ctx* ctx, const double* t, int32_t nt, const double*
  * on the snapshots a_k * H_shape; reads one field per RHS instead of two. */
 int fi_set_ice_separable(fi_ctx* ctx, const double* t, const double* a, int32_t nt,
                          const double* H_shape, int32_t flat_bc);
+
+/* Barystatic sea level: the rule of RegionalSeaLevel.bsl (update_bsl!, src/barystatic_sealevel.jl:
+ * 215-234) and the mechanism of RegionalSeaLevel.update_bsl (internal_update_bsl!,
+ * src/sealevel.jl:242-262), set before fi_init. At every sparse update the engine sums the volume
+ * contributions of the state before the update, delta_V = V_new - V_old, and then, with an
+ * internal update, applies update_bsl!(bsl, -delta_V, t):
+ *   FI_BSL_CONSTANT                 z = cfg.bsl_z (the default)
+ *   FI_BSL_CONSTANT_OCEAN_SURFACE   A = A_ref; z += -delta_V / A
+ *   FI_BSL_PIECEWISE_CONSTANT       z += -delta_V / A; A = interp(z) over the table (x = z, y = A)
+ *   FI_BSL_IMPOSED                  z = interp(t) over the table (x = t, y = z)
+ * The ocean surface starts at A_ref and the BSL at cfg.bsl_z. interp is TimeInterpolation0D
+ * (src/interpolations.jl:28-49): linear between ascending knots x[0..n-1]; outside them the end
+ * value if flat_bc != 0, else FI_ERR_TIME_RANGE from the integration. FI_BSL_EXTERNAL_UPDATE
+ * computes delta_V and leaves z unchanged. */
+enum {
+    FI_BSL_CONSTANT = 0,
+    FI_BSL_CONSTANT_OCEAN_SURFACE = 1,
+    FI_BSL_PIECEWISE_CONSTANT = 2,
+    FI_BSL_IMPOSED = 3
+};
+enum { FI_BSL_INTERNAL_UPDATE = 0, FI_BSL_EXTERNAL_UPDATE = 1 };
+int fi_set_bsl(fi_ctx* ctx, int32_t rule, int32_t update, double A_ref, const double* x, const double* y,
+               int32_t n, int32_t flat_bc);
 
 /* Output times = tstops of the solver and the times of the output callbacks
  * (tstops = sort(vcat(nout.t, ncout.t)), src/simulation.jl:255-257). */

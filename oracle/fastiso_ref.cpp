@@ -328,6 +328,10 @@ struct fi_ctx {
     int ice_mode = 0;  // 0 external, 1 snapshots, 2 separable
     vec ice_t, ice_a, ice_H;
     int ice_flat = 0;
+    // barystatic sea-level rule (fi_set_bsl); A_OCEAN_PD (src/barystatic_sealevel.jl:1) by default
+    int bsl_rule = FI_BSL_CONSTANT, bsl_update = FI_BSL_INTERNAL_UPDATE, bsl_flat = 0;
+    double bsl_A_ref = 3.625e14;
+    vec bsl_x, bsl_y;
     vec tout;
     // reference state
     vec H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
@@ -336,7 +340,7 @@ struct fi_ctx {
     vec u, ue, dudt, H_ice, H_af, H_water, z_b, dz_ss, z_ss;
     std::vector<uint8_t> mg, mo;
     vec ca_ice, ca_sw, ca_sed, ca_litho, ca_mantle, ca_load, ca_full;
-    double V_af = 0, V_pov = 0, V_den = 0, delta_V = 0, z_bsl = 0;
+    double V_af = 0, V_pov = 0, V_den = 0, delta_V = 0, z_bsl = 0, A_bsl = 0;
     int64_t count = 0;
     bool k1_stale = false;
     double t0 = 0, t_end = 0, t_timer = 0;
@@ -429,6 +433,24 @@ void columnanom_ice(fi_ctx* c) {
     for (size_t p = 0; p < c->N; ++p) c->ca_ice[p] = c->cfg.rho_ice * (c->H_af[p] - c->H_af_ref[p]);
 }
 
+// interpolate(t, ::TimeInterpolation0D) (src/interpolations.jl:28-49): the knot value on an
+// exact hit (searchsortedfirst), else linear between the knots around t; outside them the end
+// value when flat, else an error
+double interpolate0d(const vec& x, const vec& y, int flat, double t, bool* err) {
+    const size_t n = x.size();
+    if (t < x.front()) {
+        if (!flat) *err = true;
+        return y.front();
+    }
+    if (t > x.back()) {
+        if (!flat) *err = true;
+        return y.back();
+    }
+    const size_t i = (size_t)(std::lower_bound(x.begin(), x.end(), t) - x.begin());
+    if (i < n && x[i] == t) return y[i];
+    return y[i - 1] + (y[i] - y[i - 1]) / (x[i] - x[i - 1]) * (t - x[i - 1]);
+}
+
 // The sparse block of update_diagnostics! (src/simulation.jl:359-381)
 int sparse_block(fi_ctx* c) {
     const fi_config& g = c->cfg;
@@ -443,12 +465,32 @@ int sparse_block(fi_ctx* c) {
     }
     // columnanom_litho! (src/loads.jl:41-44)
     for (size_t p = 0; p < N; ++p) c->ca_litho[p] = g.rho_litho * (c->ue[p] - c->ue_ref[p]);
-    // internal_update_bsl! -> update_delta_V!, update_bsl!(ConstantBSL) (src/sealevel.jl:242-262)
+    // internal_update_bsl! / external: update_delta_V! (src/sealevel.jl:242-262), then with an
+    // internal update update_bsl!(bsl, -delta_V, t) (src/barystatic_sealevel.jl:215-234)
     double Vold = c->V_af + c->V_den + c->V_pov;
     update_volumes(c);
     double Vnew = c->V_af + c->V_den + c->V_pov;
     c->delta_V = Vnew - Vold;
-    c->z_bsl = g.bsl_z;
+    if (c->bsl_update == FI_BSL_INTERNAL_UPDATE) {
+        const double dV = -c->delta_V;
+        bool err = false;
+        switch (c->bsl_rule) {
+            case FI_BSL_CONSTANT_OCEAN_SURFACE:
+                c->A_bsl = c->bsl_A_ref;
+                c->z_bsl += dV / c->A_bsl;
+                break;
+            case FI_BSL_PIECEWISE_CONSTANT:
+                c->z_bsl += dV / c->A_bsl;
+                c->A_bsl = interpolate0d(c->bsl_x, c->bsl_y, c->bsl_flat, c->z_bsl, &err);
+                break;
+            case FI_BSL_IMPOSED:
+                c->z_bsl = interpolate0d(c->bsl_x, c->bsl_y, c->bsl_flat, c->t_timer, &err);
+                break;
+            default:
+                c->z_bsl = g.bsl_z;
+        }
+        if (err) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation0D out of range.");
+    }
     // update_dz_ss! (src/sealevel.jl:173-201)
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE) {
         if (g.lithosphere_column == FI_COMPRESSIBLE_LITHOSPHERE_COLUMN) {
@@ -746,6 +788,29 @@ int fi_set_ice_separable(fi_ctx* c, const double* t, const double* a, int32_t nt
     return FI_OK;
 }
 
+int fi_set_bsl(fi_ctx* c, int32_t rule, int32_t update, double A_ref, const double* x, const double* y,
+               int32_t n, int32_t flat) {
+    if (!c) return FI_ERR_INVALID;
+    if (rule < FI_BSL_CONSTANT || rule > FI_BSL_IMPOSED) return fail(c, FI_ERR_INVALID, "fi_set_bsl: unknown rule");
+    if (update != FI_BSL_INTERNAL_UPDATE && update != FI_BSL_EXTERNAL_UPDATE)
+        return fail(c, FI_ERR_INVALID, "fi_set_bsl: unknown update");
+    if ((rule == FI_BSL_CONSTANT_OCEAN_SURFACE || rule == FI_BSL_PIECEWISE_CONSTANT) && !(A_ref > 0))
+        return fail(c, FI_ERR_INVALID, "fi_set_bsl: the reference ocean surface must be positive");
+    const bool table = rule == FI_BSL_PIECEWISE_CONSTANT || rule == FI_BSL_IMPOSED;
+    if (table) {
+        if (!x || !y || n < 1) return fail(c, FI_ERR_INVALID, "fi_set_bsl: the rule needs a table");
+        for (int k = 1; k < n; ++k)
+            if (!(x[k] > x[k - 1])) return fail(c, FI_ERR_INVALID, "fi_set_bsl: table knots must increase");
+    }
+    c->bsl_rule = rule;
+    c->bsl_update = update;
+    c->bsl_A_ref = A_ref > 0 ? A_ref : 3.625e14;
+    c->bsl_flat = flat;
+    c->bsl_x = table ? vec(x, x + n) : vec();
+    c->bsl_y = table ? vec(y, y + n) : vec();
+    return FI_OK;
+}
+
 int fi_set_output_times(fi_ctx* c, const double* t, int32_t n) {
     if (!c || (n > 0 && !t) || n < 0) return fail(c, FI_ERR_INVALID, "fi_set_output_times: bad args");
     c->tout.assign(t, t + n);
@@ -811,6 +876,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     c->z_ss = c->z_ss_ref;
     c->V_af = c->V_pov = c->V_den = c->delta_V = 0.0;
     c->z_bsl = g.bsl_z;
+    c->A_bsl = c->bsl_A_ref;
     c->mg = c->mg_ref;
     c->mo = c->mo_ref;
     c->count = 0;
@@ -995,7 +1061,7 @@ static int ck_walk(fi_ctx* c, int mode, unsigned char* blob, size_t& off) {
     vv(c->u); vv(c->ue); vv(c->dudt); vv(c->H_ice); vv(c->H_af); vv(c->H_water); vv(c->z_b); vv(c->dz_ss);
     vv(c->z_ss); vb(c->mg); vb(c->mo);
     vv(c->ca_ice); vv(c->ca_sw); vv(c->ca_sed); vv(c->ca_litho); vv(c->ca_mantle); vv(c->ca_load); vv(c->ca_full);
-    sc(c->V_af); sc(c->V_pov); sc(c->V_den); sc(c->delta_V); sc(c->z_bsl); sc(c->count); sc(c->k1_stale);
+    sc(c->V_af); sc(c->V_pov); sc(c->V_den); sc(c->delta_V); sc(c->z_bsl); sc(c->A_bsl); sc(c->count); sc(c->k1_stale);
     sc(c->t_timer); sc(c->integ); sc(c->t); sc(c->dt); sc(c->qold); sc(c->q11); sc(c->EEst);
     vv(c->uprev); vv(c->k1);
     sc(c->nrhs); sc(c->nacc); sc(c->nrej); sc(c->kout);
@@ -1050,6 +1116,7 @@ int fi_get_stats(fi_ctx* c, fi_stats* o) {
     o->V_pov = c->V_pov;
     o->delta_V = c->delta_V;
     o->z_bsl = c->z_bsl;
+    o->A_ocean = c->A_bsl;
     o->last_EEst = c->EEst;
     return FI_OK;
 }

@@ -30,6 +30,7 @@ class RefStats(C.Structure):
         ("V_af", C.c_double), ("V_den", C.c_double), ("V_pov", C.c_double), ("delta_V", C.c_double),
         ("z_bsl", C.c_double), ("last_EEst", C.c_double),
         ("device_ms", C.c_double), ("n_launch", C.c_int64), ("n_accept_timed", C.c_int64),
+        ("A_ocean", C.c_double),
     ]
 
 
@@ -58,6 +59,7 @@ def lib():
             "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
             "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
             "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
+            "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),
             "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),
             "fi_init": (C.c_int, [P, C.c_double, C.c_double]),
             "fi_rhs": (C.c_int, [P, C.c_double, _dp, _dp]),
@@ -138,6 +140,8 @@ class RefEngine:
             a = np.asarray(ice[2], dtype=np.float64)
             H = _flat(ice[3])
             e._ck(e.L.fi_set_ice_separable(e.h, _p(t), _p(a), len(t), _p(H), int(ice[4])))
+        if "bsl" in inputs:
+            e.set_bsl(*inputs["bsl"])
         tout = np.asarray(inputs["tout"], dtype=np.float64)
         e._ck(e.L.fi_set_output_times(e.h, _p(tout), len(tout)))
         e._ck(e.L.fi_init(e.h, *inputs["t_span"]))
@@ -163,6 +167,12 @@ class RefEngine:
     def _ck(self, st):
         if st != 0:
             raise RuntimeError(f"oracle error {st}: {self.L.fi_last_error(self.h).decode()}")
+
+    def set_bsl(self, rule, update, A_ref, x=None, y=None, flat_bc=False):
+        xs = np.asarray(x if x is not None else [], dtype=np.float64)
+        ys = np.asarray(y if y is not None else [], dtype=np.float64)
+        self._ck(self.L.fi_set_bsl(self.h, int(rule), int(update), float(A_ref), _p(xs) if len(xs) else None,
+                                   _p(ys) if len(ys) else None, len(xs), int(flat_bc)))
 
     def set_field(self, name, X):
         v = _flat(X)
