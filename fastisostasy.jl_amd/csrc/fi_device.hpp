@@ -96,7 +96,9 @@ struct KP {
     long specN;      // nhx * ny
     const float* mult;
     const float* D;
-    const float* inv2eta;
+    const float* inv2eta;   // K / (2 eta)
+    const float* Kinv;      // 1/K per cell, null without map distortion
+    const float* K2;        // K^2 per cell, null without map distortion
     const unsigned char* mask;
     const float* H_af_ref;
     const float* u_ref;

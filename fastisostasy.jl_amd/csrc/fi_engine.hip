@@ -21,7 +21,7 @@ namespace fi {
 // The adaptive BS3 loop of one simulation on a team: from (s.t, U[s.cur], K[s.k1]) until t
 // reaches s.next_out (after an accepted step) or s.stop, or p.max_steps attempts. Every team
 // member holds the same Ctl copy in LDS and takes the same decisions.
-template <typename MT>
+template <typename MT, bool DIST>
 __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
                                                const float2* twy, char* smem, double* red, int* flag,
                                                unsigned long long& tl) {
@@ -83,7 +83,7 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
                 st.e3 = s3;
                 st.u_prev = cur;
             }
-            if (!rhs_call<MT>(p, tm, s, st, gen, twx, twy, smem, red, flag, false, tl)) {
+            if (!rhs_call<MT, DIST>(p, tm, s, st, gen, twx, twy, smem, red, flag, false, tl)) {
                 ok = false;
                 break;
             }
@@ -147,7 +147,7 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
 }
 
 // A single RHS evaluation on U[ubuf] at time t into K[kslot] (update_diagnostics!).
-template <typename MT>
+template <typename MT, bool DIST>
 __device__ __forceinline__ bool single_rhs(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
                                            const float2* twy, char* smem, double* red, int* flag,
                                            unsigned long long& tl, int ubuf, int kslot, double t, bool sparse_only,
@@ -163,7 +163,7 @@ __device__ __forceinline__ bool single_rhs(const KP& p, const Team& tm, Ctl& s, 
     st.t = t;
     st.t_next_max = t;
     st.force_sigma = force_sigma ? 1 : 0;
-    return rhs_call<MT>(p, tm, s, st, gen, twx, twy, smem, red, flag, sparse_only, tl);
+    return rhs_call<MT, DIST>(p, tm, s, st, gen, twx, twy, smem, red, flag, sparse_only, tl);
 }
 
 __device__ __forceinline__ void load_twiddles(const KP& p, float2* twx, float2* twy) {
@@ -179,7 +179,7 @@ __device__ __forceinline__ void load_twiddles(const KP& p, float2* twx, float2* 
 // the pointers it holds as global memory (global rather than flat loads).
 using ConstKP = __attribute__((address_space(4))) KP;
 
-template <typename MT>
+template <typename MT, bool DIST>
 __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
     const KP& p = *(const KP*)(const ConstKP*)(unsigned long long)pp;
     extern __shared__ __align__(16) char smem_all[];
@@ -197,10 +197,10 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
     if (threadIdx.x == 0) s = *p.ctl;
     __syncthreads();
     if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY)
-        (void)single_rhs<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, p.rhs_ubuf, p.rhs_kslot, s.t_rhs,
+        (void)single_rhs<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, p.rhs_ubuf, p.rhs_kslot, s.t_rhs,
                              p.mode == MODE_SPARSE_ONLY, false);
     else
-        integrate_loop<MT>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
+        integrate_loop<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
     if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
 }
 
@@ -225,7 +225,7 @@ struct BatchArgs {
 // one member of the ensemble kernel. The parameter block is passed as a restrict pointer: nothing
 // the integration writes aliases it, so its fields are read with scalar loads (as in the kernel of
 // one context) instead of being reloaded into vector registers after every store.
-template <typename MT>
+template <typename MT, bool DIST>
 __device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, const Team& tm, Ctl& s, unsigned& gen,
                                              char* smem_all, double* red, int* flag, unsigned long long& tl,
                                              int fsal) {
@@ -238,10 +238,10 @@ __device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, con
     __syncthreads();
     const bool todo = s.t < s.stop && !s.err;
     if (todo) {
-        integrate_loop<MT>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
+        integrate_loop<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
         if (*flag) return false;
         if (fsal && !s.err && s.t >= s.stop) {
-            if (!single_rhs<MT>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
+            if (!single_rhs<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
                 if (*flag) return false;
             }
             if (threadIdx.x == 0) s.last_u = s.cur;
@@ -252,7 +252,7 @@ __device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, con
     return true;
 }
 
-template <typename MT>
+template <typename MT, bool DIST>
 __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __restrict__ ba) {
     extern __shared__ __align__(16) char smem_all[];
     __shared__ Ctl s;
@@ -280,8 +280,19 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __res
         // divergent), so the member's parameter block is a uniform, constant pointer
         const unsigned long long pk =
             ((const __attribute__((address_space(1))) unsigned long long*)(unsigned long long)ba->kps)[m];
-        if (!batch_member<MT>((const ConstKP*)pk, tm, s, gen, smem_all, red, &flag, tl, ba->fsal)) return;
+        if (!batch_member<MT, DIST>((const ConstKP*)pk, tm, s, gen, smem_all, red, &flag, tl, ba->fsal)) return;
     }
+}
+
+// kernel instances: FP64 or FP32 moments; with or without per-cell grid steps (map distortion,
+// FP64-moment path only)
+inline const void* persistent_kernel(bool mt_double, bool dist) {
+    if (!mt_double) return (const void*)fi_persistent<float, false>;
+    return dist ? (const void*)fi_persistent<double, true> : (const void*)fi_persistent<double, false>;
+}
+inline const void* batch_kernel(bool mt_double, bool dist) {
+    if (!mt_double) return (const void*)fi_batch_persistent<float, false>;
+    return dist ? (const void*)fi_batch_persistent<double, true> : (const void*)fi_batch_persistent<double, false>;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -405,11 +416,15 @@ struct fi_ctx {
     int bsl_rule = FI_BSL_CONSTANT, bsl_update = FI_BSL_INTERNAL_UPDATE, bsl_flat = 0;
     double bsl_A_ref = 3.625e14;
     std::vector<double> bsl_x, bsl_y;
+    // map scale factor K (FI_DISTORTION); has_dist when any K != 1
+    std::vector<double> dist;
+    bool has_dist = false;
     std::vector<double> tout;
     std::vector<double> H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
     std::vector<double> H_ext_host;
     // device memory
     DBuf<double> U, ice_t_d, ice_a_d, part, vpart, bsl_x_d, bsl_y_d;
+    DBuf<float> kinv_d, k2_d;   // 1/K and K^2 per cell (distortion only)
     DBuf<float> K, kcorner, mult, Df, inv2eta, H_af_ref_d, u_ref_d, z_b_ref_d, z_ss_ref_d, ue_ref_d, H_ext, ue, dz_ss,
         z_ss, zoff, zoff_trig, LW, LT, H_water, sig_load_prev, sig_geo_prev, ue_raw, dz_raw, ext_corners, ice_H_d;
     DBuf<unsigned char> maskd, mg, mo;
@@ -423,6 +438,7 @@ struct fi_ctx {
     Ctl ctl;
     int G = 0;
     size_t lds = 0, lds_d = 0, lds_f = 0;   // LDS bytes of this context's launch; with FP64 / FP32 moments
+    size_t lds_dist = 0;                     // FP64 moments with per-cell grid steps
     bool mt_double = true;
     FftPlan px64, py64, pfx64, pfy64;
     bool initialized = false, integ = false, k1_stale = false;
@@ -498,7 +514,7 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     HIPCHECK(c, hipMemcpyAsync(c->kp_d.p, &kp, sizeof(KP), hipMemcpyHostToDevice, c->stream));
     const KP* kptr = c->kp_d.p;
     void* args[] = {&kptr};
-    const void* f = c->mt_double ? (const void*)fi_persistent<double> : (const void*)fi_persistent<float>;
+    const void* f = persistent_kernel(c->mt_double, c->has_dist);
     const bool timed = mode == MODE_INTEGRATE;
     const long long nacc0 = c->ctl.nacc;
     if (timed) HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -699,6 +715,12 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_REF_DZ_SS: c->dz_ss_ref = v; return FI_OK;
         case FI_REF_Z_B: c->z_b_ref = v; return FI_OK;
         case FI_REF_H_ICE: c->H_ref_in = v; c->has_href = true; return FI_OK;
+        case FI_DISTORTION:
+            for (double k : v)
+                if (!(k > 0) || !std::isfinite(k)) return fail(c, FI_ERR_INVALID, "distortion K must be positive");
+            c->dist = v;
+            c->has_dist = std::any_of(v.begin(), v.end(), [](double k) { return k != 1.0; });
+            return FI_OK;
         default: break;
     }
     if (!c->initialized) return fail(c, FI_ERR_STATE, "state field written before fi_init");
@@ -831,9 +853,18 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
             mult[(size_t)kx * ny + ky] = (kx == 0 && ky == 0) ? 0.0f : (float)(m * sc);
         }
     if (upload(c, c->mult, mult)) return FI_ERR_DEVICE;
+    // rhs K / (2 eta) (src/deformation.jl:95)
     std::vector<float> inv2eta(N);
-    for (size_t p = 0; p < N; ++p) inv2eta[p] = (float)(1.0 / (2.0 * c->eta[p]));  // K = 1
+    for (size_t p = 0; p < N; ++p) inv2eta[p] = (float)((c->has_dist ? c->dist[p] : 1.0) * (1.0 / (2.0 * c->eta[p])));
     if (upload(c, c->inv2eta, inv2eta) || upload(c, c->Df, to_f(c->D))) return FI_ERR_DEVICE;
+    if (c->has_dist) {
+        std::vector<float> kinv(N), k2(N);
+        for (size_t p = 0; p < N; ++p) {
+            kinv[p] = (float)(1.0 / c->dist[p]);
+            k2[p] = (float)(c->dist[p] * c->dist[p]);
+        }
+        if (upload(c, c->kinv_d, kinv) || upload(c, c->k2_d, k2)) return FI_ERR_DEVICE;
+    }
     std::vector<unsigned char> m8(N);
     for (size_t p = 0; p < N; ++p) m8[p] = c->mask[p] != 0;
     if (upload(c, c->maskd, m8)) return FI_ERR_DEVICE;
@@ -1014,6 +1045,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.mult = c->mult.p;
     kp.D = c->Df.p;
     kp.inv2eta = c->inv2eta.p;
+    kp.Kinv = c->has_dist ? c->kinv_d.p : nullptr;
+    kp.K2 = c->has_dist ? c->k2_d.p : nullptr;
     kp.mask = c->maskd.p;
     kp.H_af_ref = c->H_af_ref_d.p;
     kp.u_ref = c->u_ref_d.p;
@@ -1056,12 +1089,15 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     // FP64-moment rows phase also stages the rigidity (4 rows) and 7 per-cell fields (2 rows)
     size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16 + 24 + 16 + 56), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16 + 24);
     size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)3 * c->nfx * 8, (size_t)5 * c->nfy * 8});
-    c->mt_double = nx <= 512 && twb + rows_d <= 160 * 1024;   // FP64 moments, register prefetch sized for nx <= 512
-    c->lds_d = twb + std::max(rows_d, other);
+    c->mt_double = nx <= 512 && twb + rows_d + (c->has_dist ? (size_t)nx * 24 : 0) <= 160 * 1024;   // FP64 moments, register prefetch sized for nx <= 512
+    c->lds_dist = twb + std::max(rows_d + (size_t)nx * 24, other);   // also 6 rows of 1/K
+    c->lds_d = c->has_dist ? c->lds_dist : twb + std::max(rows_d, other);
     c->lds_f = twb + std::max(rows_f, other);
     c->lds = c->mt_double ? c->lds_d : c->lds_f;
     if (c->lds > 160 * 1024 || nx > 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
-    const void* f = c->mt_double ? (const void*)fi_persistent<double> : (const void*)fi_persistent<float>;
+    if (c->has_dist && (!c->mt_double || ny < 6 || nx < 3))
+        return fail(c, FI_ERR_UNSUPPORTED, "map distortion needs the FP64-moment rows phase (3 <= nx <= 512, ny >= 6)");
+    const void* f = persistent_kernel(c->mt_double, c->has_dist);
     HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
     int per_cu = 0, ncu = 0;
     HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, c->lds));
@@ -1126,6 +1162,7 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_REF_DZ_SS: return put(c->dz_ss_ref);
         case FI_REF_Z_B: return put(c->z_b_ref);
         case FI_REF_H_ICE: return put(c->H_ice_ref);
+        case FI_DISTORTION: return put(c->has_dist ? c->dist : std::vector<double>(c->N, 1.0));
         default: break;
     }
     if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_get_field before fi_init");
@@ -1263,9 +1300,10 @@ int fi_integrator_init(fi_ctx* c) {
             return st;
         double s0 = 0, s1 = 0, s2 = 0;
         for (size_t p = 0; p < N; ++p) {
-            s0 += haf[p] * c->A;
-            s1 += H[p] * c->A;
-            s2 += std::max(zss[p] - zb[p], 0.0) * c->A;
+            const double a = c->has_dist ? c->A * (c->dist[p] * c->dist[p]) : c->A;   // dx dy K^2
+            s0 += haf[p] * a;
+            s1 += H[p] * a;
+            s2 += std::max(zss[p] - zb[p], 0.0) * a;
         }
         c->ctl.V_af = g.volume_contribution ? s0 * g.rho_ice / g.rho_seawater : 0.0;
         c->ctl.V_den = g.density_contribution ? s1 * (g.rho_ice / g.rho_water - g.rho_ice / g.rho_seawater) : 0.0;
@@ -1507,6 +1545,7 @@ struct fi_batch {
     std::vector<int> cands;
     std::vector<double> cand_rate;
     int last_tsize = 0, last_nteams = 0;
+    bool dist = false;           // the current launch uses the kernel with per-cell grid steps
     double device_ms = 0, cell_attempts = 0;
     long long n_launch = 0;
     DBuf<const KP*> kps;
@@ -1576,7 +1615,7 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     BHIPCHECK(b, hipMemcpyAsync(b->args.p, &a, sizeof(a), hipMemcpyHostToDevice, b->stream));
     const BatchArgs* ap = b->args.p;
     void* kargs[] = {&ap};
-    const void* f = mt_double ? (const void*)fi_batch_persistent<double> : (const void*)fi_batch_persistent<float>;
+    const void* f = batch_kernel(mt_double, b->dist);
     BHIPCHECK(b, hipEventRecord(b->ev0, b->stream));
     BHIPCHECK(b, hipLaunchCooperativeKernel(f, dim3(nteams * ts), dim3(NT), kargs, (unsigned)lds, b->stream));
     BHIPCHECK(b, hipEventRecord(b->ev1, b->stream));
@@ -1684,8 +1723,8 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
     if (!b || !t_targets) return FI_ERR_INVALID;
     const int nm = (int)b->mem.size();
     const std::vector<double> t_target(t_targets, t_targets + nm);
-    bool mt_double = true;
-    size_t lds_d = 0, lds_f = 0;
+    bool mt_double = true, dist = false;
+    size_t lds_d = 0, lds_dd = 0, lds_f = 0;
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
         if (!c->integ) return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": fi_batch_integrate before fi_integrator_init");
@@ -1699,12 +1738,18 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
         }
         mt_double = mt_double && c->mt_double;
         lds_d = std::max(lds_d, c->lds_d);
+        lds_dd = std::max(lds_dd, c->lds_dist);
         lds_f = std::max(lds_f, c->lds_f);
+        dist = dist || c->has_dist;
     }
-    const size_t lds = mt_double ? lds_d : lds_f;
+    if (dist && !mt_double)
+        return bfail(b, FI_ERR_UNSUPPORTED, "map distortion needs every member on the FP64-moment path");
+    // with a distorted member every member runs the distortion kernel (K = 1 where unset)
+    const size_t lds = !mt_double ? lds_f : dist ? lds_dd : lds_d;
     if (lds > 160 * 1024) return bfail(b, FI_ERR_UNSUPPORTED, "members too large for one workgroup's LDS");
+    b->dist = dist;
     BHIPCHECK(b, hipSetDevice(b->dev));
-    const void* f = mt_double ? (const void*)fi_batch_persistent<double> : (const void*)fi_batch_persistent<float>;
+    const void* f = batch_kernel(mt_double, dist);
     BHIPCHECK(b, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0, ncu = 0;
     BHIPCHECK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, lds));

@@ -620,7 +620,10 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
 // Rows phase for grids up to 512 wide (FP64 moments): every global input of a row item -- the
 // stage terms, the pending spectral derivative's halo pairs, the rigidity and the per-cell fields
 // -- is loaded in one round at the start of the item and staged in LDS, so the rest of the item
-// (inverse transforms, stencils, loads, forward transform) is LDS work.
+// (inverse transforms, stencils, loads, forward transform) is LDS work. DIST: per-cell grid steps
+// K dx, K dy (map distortion); 1/K is staged with the displacement rows (K = 1 where p.Kinv is
+// null, so one ensemble launch can mix distorted and undistorted members).
+template <bool DIST>
 __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, const StageDesc& st, double uoff,
                                                const IceT it, const float2* twx, char* smem, const KC4 kc) {
     constexpr int QU = 6, QC = 4, QD = 4, QF = 2;   // loads per thread: (6|4|4|2) x NT >= (6|4|4|2) x 512
@@ -641,6 +644,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
     float* sLw = sLt + 2 * nx;
     float* sI2e = sLw + 2 * nx;
     float* sM = sI2e + 2 * nx;
+    float* sKi = sM + 2 * nx;                   // 6 rows of 1/K (map distortion only)
     float2* ca = (float2*)sMxx;                 // inverse-transform scratch (moment region)
     const float2* spin = specbuf(p, st.sb_in);
     float2* spout = specbuf(p, st.sb_out);
@@ -661,6 +665,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
 #endif
         // ---- one round of global loads ----
         double uv[QU];
+        float kv[QU];
         float2 zv[QC];
         float dv[QD];
         float fH[QF], fZo[QF], fHr[QF], fLt[QF], fLw[QF], fI2e[QF], fM[QF];
@@ -670,6 +675,14 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             const int r = fdivi(idx, inv_nx), i = idx - r * nx;
             const long c = i + (long)(ju0 + r) * nx;
             uv[q] = (st.ng == 0) ? U0[c] : U0[c] + st.dt * kterms(p, st, c);
+        }
+        if (DIST) {
+#pragma unroll
+            for (int q = 0; q < QU; ++q) {
+                const int idx = min((int)threadIdx.x + q * NT, totU - 1);
+                const int r = fdivi(idx, inv_nx), i = idx - r * nx;
+                kv[q] = p.Kinv ? p.Kinv[i + (long)(ju0 + r) * nx] : 1.0f;
+            }
         }
 #pragma unroll
         for (int q = 0; q < QC; ++q) {
@@ -701,6 +714,13 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
         for (int q = 0; q < QU; ++q) {
             const int idx = (int)threadIdx.x + q * NT;
             if (idx < totU) su[idx] = uv[q];
+        }
+        if (DIST) {
+#pragma unroll
+            for (int q = 0; q < QU; ++q) {
+                const int idx = (int)threadIdx.x + q * NT;
+                if (idx < totU) sKi[idx] = kv[q];
+            }
         }
 #pragma unroll
         for (int q = 0; q < QC; ++q) {
@@ -760,6 +780,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             constexpr int JU = decltype(JU_)::value, NU = decltype(NU_)::value, JM = decltype(JM_)::value,
                           NM = decltype(NM_)::value, NO = decltype(NO_)::value, LO = decltype(LO_)::value;
             constexpr bool TOP = decltype(TOP_)::value, BOT = decltype(BOT_)::value;
+            constexpr bool dist = DIST;
             int i = threadIdx.x;
             asm volatile("" : "+v"(i));   // keep column-derived addresses inside the item loop
             double uo[NO];
@@ -783,6 +804,11 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     dd[r] = ci == 1 ? (x2 - x0) * (0.5 * p.inv_dx)
                                     : (ci == 0 ? (x1 - x0) * p.inv_dx : (x2 - x1) * p.inv_dx);
                     xx[r] = (x2 - 2.0 * x1 + x0) * p.inv_dx2;
+                    if (dist) {   // Dx = K dx at this row's cell
+                        const double k = sKi[r * nx + i];
+                        dd[r] *= k;
+                        xx[r] *= k * k;
+                    }
                 }
 #pragma unroll
                 for (int o = 0; o < NO; ++o) uo[o] = cv[o - JU];
@@ -799,9 +825,14 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const int lo = first ? ur : ur - 1, hi = last ? ur : ur + 1;
                     const double D = (double)sD[m * nx + i];
                     const double uxx = xx[ur];
-                    const double uyy = (cv[c + 1] - 2.0 * cv[c] + cv[c - 1]) * p.inv_dy2;
-                    const double uxy = (first || last) ? (dd[hi] - dd[lo]) * p.inv_dy
-                                                       : (dd[hi] - dd[lo]) * (0.5 * p.inv_dy);
+                    double uyy = (cv[c + 1] - 2.0 * cv[c] + cv[c - 1]) * p.inv_dy2;
+                    double uxy = (first || last) ? (dd[hi] - dd[lo]) * p.inv_dy
+                                                 : (dd[hi] - dd[lo]) * (0.5 * p.inv_dy);
+                    if (dist) {   // Dy = K dy at the moment's cell
+                        const double k = sKi[ur * nx + i];
+                        uyy *= k * k;
+                        uxy *= k;
+                    }
                     sMxx[m * nx + i] = -D * (nu * uyy + uxx);
                     sMyy[m * nx + i] = -D * (nu * uxx + uyy);
                     sMxy[m * nx + i] = -D * (1.0 - nu) * uxy;
@@ -811,10 +842,13 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             FI_SUBMARK(5);
             if (i < nx) {
                 auto mx = [&](const double* M, int mm, int cc) { return M[mm * nx + a + cc]; };
+                // 1/K of window row wr at this column (1 without map distortion)
+                auto kw = [&](int wr) { return dist ? (double)sKi[wr * nx + i] : 1.0; };
                 auto d1m = [&](int mm) {
-                    return ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
-                                   : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
-                                              : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                    const double d = ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
+                                             : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
+                                                        : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                    return dist ? d * kw(JM + mm - JU) : d;
                 };
                 float f[2] = {0.0f, 0.0f};
 #pragma unroll
@@ -823,11 +857,17 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const bool first = TOP && o == 0, last = o == LO;
                     const int c = first ? mr + 1 : (last ? mr - 1 : mr);
                     const int lo = first ? mr : mr - 1, hi = last ? mr : mr + 1;
-                    const double bxx = (mx(sMxx, mr, 2) - 2.0 * mx(sMxx, mr, 1) + mx(sMxx, mr, 0)) * p.inv_dx2;
-                    const double byy =
+                    double bxx = (mx(sMxx, mr, 2) - 2.0 * mx(sMxx, mr, 1) + mx(sMxx, mr, 0)) * p.inv_dx2;
+                    double byy =
                         (sMyy[(c + 1) * nx + i] - 2.0 * sMyy[c * nx + i] + sMyy[(c - 1) * nx + i]) * p.inv_dy2;
-                    const double bxy = (first || last) ? (d1m(hi) - d1m(lo)) * p.inv_dy
-                                                       : (d1m(hi) - d1m(lo)) * (0.5 * p.inv_dy);
+                    double bxy = (first || last) ? (d1m(hi) - d1m(lo)) * p.inv_dy
+                                                 : (d1m(hi) - d1m(lo)) * (0.5 * p.inv_dy);
+                    if (dist) {   // Dx, Dy at the output cell
+                        const double k = kw(o - JU);
+                        bxx *= k * k;
+                        byy *= k * k;
+                        bxy *= k;
+                    }
                     const int idx = o * nx + i;
                     const long cc = i + (long)(j0 + o) * nx;
                     const double u = uo[o];
@@ -1101,8 +1141,11 @@ __device__ __forceinline__ void phase_sp_rows(const KP& p, const Team& tm, char*
             z.y = 0;
             if (i < nx) {
                 const long c = i + (long)j * nx;
-                if (!p.rigid_litho) z.x = p.sig_load_prev[c];  // sigma_load * K^2, K = 1
-                if (p.lv_surface) z.y = p.sig_geo_prev[c];     // mass anomaly / A (A folded in the kernel)
+                // sigma_load K^2 (src/deformation.jl:177); mass anomaly / (dx dy) = sigma K^2
+                // (src/sealevel.jl:193-201; the constant dx dy is folded in the Green spectrum)
+                const float k2 = p.K2 ? p.K2[c] : 1.0f;
+                if (!p.rigid_litho) z.x = p.sig_load_prev[c] * k2;
+                if (p.lv_surface) z.y = p.sig_geo_prev[c] * k2;
             }
             sf[i] = z;
         }
@@ -1229,9 +1272,10 @@ __device__ __forceinline__ void phase_sp_volumes(const KP& p, const Team& tm, co
             const double H = ice_at(p, it, c);
             const double zss_old = p.z_ss[c];
             const double zb_old = (double)p.z_b_ref[c] + (double)p.ue[c] + u;
-            acc[0] += hab(H, zb_old - zss_old, p.rho_sw_ice) * p.A;
-            acc[1] += H * p.A;
-            acc[2] += fmax(zss_old - zb_old, 0.0) * p.A;
+            const double a = p.K2 ? p.A * (double)p.K2[c] : p.A;   // cell area dx dy K^2
+            acc[0] += hab(H, zb_old - zss_old, p.rho_sw_ice) * a;
+            acc[1] += H * a;
+            acc[2] += fmax(zss_old - zb_old, 0.0) * a;
         }
         block_sum<3>(red, acc);
         if (threadIdx.x == 0) {
@@ -1348,7 +1392,7 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
 // ------------------------------------------------------------------------------------------
 // one RHS call
 // ------------------------------------------------------------------------------------------
-template <typename MT>
+template <typename MT, bool DIST>
 __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, StageDesc& st, unsigned& gen,
                                          const float2* twx, const float2* twy, char* smem, double* red, int* flag,
                                          bool sparse_only, unsigned long long& tl) {
@@ -1403,7 +1447,7 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
     __syncthreads();
     if (p.ptime && tm.rank == 0 && threadIdx.x == 0) p.ptime[23] += wall_clock64() - tl;
 #endif
-    if (sizeof(MT) == 8) phase_rows_lds(p, tm, st, uoff, it, twx, smem, kc);
+    if constexpr (sizeof(MT) == 8) phase_rows_lds<DIST>(p, tm, st, uoff, it, twx, smem, kc);
     else phase_rows<MT, 8, 4>(p, tm, st, uoff, it, twx, smem, kc);
     if (!grid_sync(p, tm, gen, flag, TAG_ROWS, tl)) return false;
     phase_cols(p, tm, specbuf(p, st.sb_out), twy, smem);

@@ -71,9 +71,14 @@ def nextfastfft(n):
 class RegionalDomain:
     """RegionalDomain(W, n) or RegionalDomain(Wx, Wy, nx, ny) (src/domain.jl:35-169).
 
-    Cell-centred grid x_i = -Wx + i*dx (i = 1..nx), dx = 2Wx/nx, no map distortion (K = 1)."""
+    Cell-centred grid x_i = -Wx + i*dx (i = 1..nx), dx = 2Wx/nx.
 
-    def __init__(self, *args, correct_distortion=False):
+    Map distortion: the reference computes the scale factor K from the latitudes of a Proj
+    stereographic projection (src/domain.jl:123-147). Proj is not available here, so a distorted
+    domain takes K directly (K=array of shape (nx, ny)); correct_distortion=True without K is
+    refused. K gives Dx = K dx, Dy = K dy and the cell area dx dy K^2 (src/domain.jl:167)."""
+
+    def __init__(self, *args, correct_distortion=False, K=None):
         if len(args) == 2:
             W, n = args
             Wx = Wy = float(W)
@@ -82,8 +87,9 @@ class RegionalDomain:
             Wx, Wy, nx, ny = float(args[0]), float(args[1]), int(args[2]), int(args[3])
         else:
             raise TypeError("RegionalDomain(W, n) or RegionalDomain(Wx, Wy, nx, ny)")
-        if correct_distortion:
-            raise NotImplementedError("map distortion (K != 1) is not supported by this engine")
+        if correct_distortion and K is None:
+            raise NotImplementedError("correct_distortion needs the Proj latitudes of the reference; "
+                                      "pass the scale factor as K=...")
         self.Wx, self.Wy, self.nx, self.ny = Wx, Wy, nx, ny
         self.mx, self.my = nx // 2, ny // 2
         self.dx, self.dy = 2 * Wx / nx, 2 * Wy / ny
@@ -91,7 +97,9 @@ class RegionalDomain:
         self.y = -Wy + self.dy * np.arange(1, ny + 1)
         self.X, self.Y = meshgrid(self.x, self.y)
         self.R = np.sqrt(self.X ** 2 + self.Y ** 2)
-        self.K = np.ones((nx, ny))
+        self.K = np.ones((nx, ny)) if K is None else np.asarray(K, dtype=np.float64).reshape(nx, ny).copy()
+        if not np.all(self.K > 0):
+            raise ValueError("the scale factor K must be positive")
         self.Dx = self.K * self.dx
         self.Dy = self.K * self.dy
         self.A = (self.dx * self.dy) * self.K ** 2
@@ -101,7 +109,7 @@ class RegionalDomain:
         self.i1, self.i2 = samesize_conv_indices(nx, self.mx)
         self.j1, self.j2 = samesize_conv_indices(ny, self.my)
         self.convo_offset = 0
-        self.correct_distortion = False
+        self.correct_distortion = bool(np.any(self.K != 1.0))
 
     def zeros(self):
         return np.zeros((self.nx, self.ny))

@@ -110,6 +110,8 @@ class Simulation:
             "scaled_pseudodiff_inv": solidearth.scaled_pseudodiff_inv,
             "maskactive": solidearth.maskactive.astype(np.float64),
         }
+        if np.any(domain.K != 1.0):
+            fields["distortion"] = domain.K
         # GIATools (src/tools.jl:49-98): Green kernels of the elastic and sea-surface responses
         if not isinstance(solidearth.lithosphere, RigidLithosphere):
             fields["green_elastic"] = elastic_green(domain, quad_precision=quad_precision)

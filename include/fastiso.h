@@ -119,6 +119,11 @@ enum {
     FI_REF_DZ_SS = 9,
     FI_REF_Z_B = 10,
     FI_REF_H_ICE = 11,            /* reference ice; default: forcing at t0 (src/simulation.jl:133-134) */
+    FI_DISTORTION = 12,           /* RegionalDomain.K, the map scale factor (src/domain.jl:140-147, 167):
+                                     grid steps K dx, K dy and cell area dx dy K^2 per cell, the RHS
+                                     factor K / (2 eta) (src/deformation.jl:95) and K^2 on the
+                                     convolution inputs (src/deformation.jl:177); default 1. The HIP
+                                     engine supports it on its FP64-moment path (nx <= 512, ny >= 6) */
     /* state, readable after fi_init; FI_U and FI_H_ICE are also writable */
     FI_U = 32,                    /* CurrentState fields                src/state.jl:49-77     */
     FI_DUDT = 33,

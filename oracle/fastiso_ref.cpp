@@ -169,52 +169,56 @@ vec geoid_green(int nx, int ny, double Wx, double Wy, double r_pole, double mE) 
 // ---------------------------------------------------------------------------------------
 // Finite differences, CPU rule (src/derivatives.jl:48-57,66-76,114-124,166-191)
 // ---------------------------------------------------------------------------------------
-void dxx(double* du, const double* u, int nx, int ny, double Dx) {
-    double D2 = Dx * Dx;
+// Dx, Dy: per-cell grid steps K dx, K dy (RegionalDomain.Dx/.Dy, src/domain.jl:167); each
+// difference is divided by the step at its output cell
+void dxx(double* du, const double* u, int nx, int ny, const double* Dx) {
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         const double* c = u + (size_t)j * nx;
+        const double* h = Dx + (size_t)j * nx;
         double* o = du + (size_t)j * nx;
-        for (int i = 1; i < nx - 1; ++i) o[i] = (c[i + 1] - 2 * c[i] + c[i - 1]) / D2;
-        o[0] = (c[2] - 2 * c[1] + c[0]) / D2;
-        o[nx - 1] = (c[nx - 1] - 2 * c[nx - 2] + c[nx - 3]) / D2;
+        for (int i = 1; i < nx - 1; ++i) o[i] = (c[i + 1] - 2 * c[i] + c[i - 1]) / (h[i] * h[i]);
+        o[0] = (c[2] - 2 * c[1] + c[0]) / (h[0] * h[0]);
+        o[nx - 1] = (c[nx - 1] - 2 * c[nx - 2] + c[nx - 3]) / (h[nx - 1] * h[nx - 1]);
     }
 }
-void dyy(double* du, const double* u, int nx, int ny, double Dy) {
-    double D2 = Dy * Dy;
+void dyy(double* du, const double* u, int nx, int ny, const double* Dy) {
     auto U = [&](int i, int j) { return u[i + (size_t)j * nx]; };
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         double* o = du + (size_t)j * nx;
-        if (j == 0) for (int i = 0; i < nx; ++i) o[i] = (U(i, 2) - 2 * U(i, 1) + U(i, 0)) / D2;
+        const double* h = Dy + (size_t)j * nx;
+        if (j == 0) for (int i = 0; i < nx; ++i) o[i] = (U(i, 2) - 2 * U(i, 1) + U(i, 0)) / (h[i] * h[i]);
         else if (j == ny - 1)
-            for (int i = 0; i < nx; ++i) o[i] = (U(i, ny - 1) - 2 * U(i, ny - 2) + U(i, ny - 3)) / D2;
-        else for (int i = 0; i < nx; ++i) o[i] = (U(i, j + 1) - 2 * U(i, j) + U(i, j - 1)) / D2;
+            for (int i = 0; i < nx; ++i) o[i] = (U(i, ny - 1) - 2 * U(i, ny - 2) + U(i, ny - 3)) / (h[i] * h[i]);
+        else for (int i = 0; i < nx; ++i) o[i] = (U(i, j + 1) - 2 * U(i, j) + U(i, j - 1)) / (h[i] * h[i]);
     }
 }
-void dx1(double* du, const double* u, int nx, int ny, double Dx) {
+void dx1(double* du, const double* u, int nx, int ny, const double* Dx) {
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         const double* c = u + (size_t)j * nx;
+        const double* h = Dx + (size_t)j * nx;
         double* o = du + (size_t)j * nx;
-        for (int i = 1; i < nx - 1; ++i) o[i] = (c[i + 1] - c[i - 1]) / (2 * Dx);
-        o[0] = (c[1] - c[0]) / Dx;
-        o[nx - 1] = (c[nx - 1] - c[nx - 2]) / Dx;
+        for (int i = 1; i < nx - 1; ++i) o[i] = (c[i + 1] - c[i - 1]) / (2 * h[i]);
+        o[0] = (c[1] - c[0]) / h[0];
+        o[nx - 1] = (c[nx - 1] - c[nx - 2]) / h[nx - 1];
     }
 }
-void dy1(double* du, const double* u, int nx, int ny, double Dy) {
+void dy1(double* du, const double* u, int nx, int ny, const double* Dy) {
     auto U = [&](int i, int j) { return u[i + (size_t)j * nx]; };
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < ny; ++j) {
         double* o = du + (size_t)j * nx;
-        if (j == 0) for (int i = 0; i < nx; ++i) o[i] = (U(i, 1) - U(i, 0)) / Dy;
-        else if (j == ny - 1) for (int i = 0; i < nx; ++i) o[i] = (U(i, ny - 1) - U(i, ny - 2)) / Dy;
-        else for (int i = 0; i < nx; ++i) o[i] = (U(i, j + 1) - U(i, j - 1)) / (2 * Dy);
+        const double* h = Dy + (size_t)j * nx;
+        if (j == 0) for (int i = 0; i < nx; ++i) o[i] = (U(i, 1) - U(i, 0)) / h[i];
+        else if (j == ny - 1) for (int i = 0; i < nx; ++i) o[i] = (U(i, ny - 1) - U(i, ny - 2)) / h[i];
+        else for (int i = 0; i < nx; ++i) o[i] = (U(i, j + 1) - U(i, j - 1)) / (2 * h[i]);
     }
 }
 // update_second_derivatives!(uxx, uyy, ux, uxy, u1, u2, u3)
 void second_derivatives(double* uxx, double* uyy, double* ux, double* uxy, const double* u1,
-                        const double* u2, const double* u3, int nx, int ny, double Dx, double Dy) {
+                        const double* u2, const double* u3, int nx, int ny, const double* Dx, const double* Dy) {
     dxx(uxx, u1, nx, ny, Dx);
     dyy(uyy, u2, nx, ny, Dy);
     dx1(ux, u3, nx, ny, Dx);
@@ -315,6 +319,9 @@ struct fi_ctx {
     int nx = 0, ny = 0, mx = 0, my = 0;
     size_t N = 0;
     double dx = 0, dy = 0, A = 0, rho_sw_ice = 0;
+    // RegionalDomain.K and what derives from it (src/domain.jl:140-147,167): Dx = K dx, Dy = K dy,
+    // cell area dx dy K^2
+    vec K, Dx, Dy, Acell;
     int i1 = 0, i2 = 0, j1 = 0, j2 = 0;
     vec pd;
     FFT vfx, vfy;  // n x n complex transforms of the viscous operator (src/tools.jl:100-109)
@@ -405,7 +412,7 @@ void watercolumn(fi_ctx* c, const vec& H, const std::vector<uint8_t>& mg, const 
 
 double sum_times_A(fi_ctx* c, const vec& X) {
     double s = 0.0;
-    for (size_t p = 0; p < c->N; ++p) s += X[p] * c->A;
+    for (size_t p = 0; p < c->N; ++p) s += X[p] * c->Acell[p];
     return s;
 }
 
@@ -421,7 +428,7 @@ void update_volumes(fi_ctx* c) {
     } else c->V_den = 0.0;
     if (g.adjustment_contribution == FI_GOELZER_CONTRIBUTION) {
         double s = 0.0;
-        for (size_t p = 0; p < c->N; ++p) s += std::max(c->z_ss[p] - c->z_b[p], 0.0) * c->A;
+        for (size_t p = 0; p < c->N; ++p) s += std::max(c->z_ss[p] - c->z_b[p], 0.0) * c->Acell[p];
         c->V_pov = s;
     } else c->V_pov = 0.0;
 }
@@ -431,6 +438,18 @@ void update_Haf(fi_ctx* c) {
 }
 void columnanom_ice(fi_ctx* c) {
     for (size_t p = 0; p < c->N; ++p) c->ca_ice[p] = c->cfg.rho_ice * (c->H_af[p] - c->H_af_ref[p]);
+}
+
+void set_distortion(fi_ctx* c, const vec& K) {
+    c->K = K;
+    c->Dx.resize(c->N);
+    c->Dy.resize(c->N);
+    c->Acell.resize(c->N);
+    for (size_t p = 0; p < c->N; ++p) {
+        c->Dx[p] = K[p] * c->dx;
+        c->Dy[p] = K[p] * c->dy;
+        c->Acell[p] = (c->dx * c->dy) * (K[p] * K[p]);
+    }
 }
 
 // interpolate(t, ::TimeInterpolation0D) (src/interpolations.jl:28-49): the knot value on an
@@ -459,7 +478,7 @@ int sparse_block(fi_ctx* c) {
     if (g.lithosphere == FI_RIGID_LITHOSPHERE) {
         std::fill(c->ue.begin(), c->ue.end(), 0.0);
     } else {
-        for (size_t p = 0; p < N; ++p) c->buf[p] = c->ca_load[p] * 1.0;  // K^2 = 1
+        for (size_t p = 0; p < N; ++p) c->buf[p] = c->ca_load[p] * (c->K[p] * c->K[p]);
         samesize_conv_ext(c->ue.data(), c->buf.data(), c->cp, c->ge_hat, c->i1, c->i2, c->j1, c->j2,
                           g.elastic_x_border);
     }
@@ -495,9 +514,9 @@ int sparse_block(fi_ctx* c) {
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE) {
         if (g.lithosphere_column == FI_COMPRESSIBLE_LITHOSPHERE_COLUMN) {
             for (size_t p = 0; p < N; ++p) c->buf[p] = c->ca_load[p] + c->maskactive[p] * c->ca_mantle[p];
-            for (size_t p = 0; p < N; ++p) c->buf[p] = c->A * c->buf[p];
+            for (size_t p = 0; p < N; ++p) c->buf[p] = c->Acell[p] * c->buf[p];
         } else {
-            for (size_t p = 0; p < N; ++p) c->buf[p] = c->A * c->ca_full[p];
+            for (size_t p = 0; p < N; ++p) c->buf[p] = c->Acell[p] * c->ca_full[p];
         }
         samesize_conv_ext(c->dz_ss.data(), c->buf.data(), c->cp, c->gg_hat, c->i1, c->i2, c->j1, c->j2,
                           g.ssh_x_border);
@@ -529,7 +548,8 @@ void update_dudt_lvelva(fi_ctx* c, double* u, double* dudt) {
     size_t N = c->N;
     double nu = g.litho_poissonratio;
     for (size_t p = 0; p < N; ++p) c->rhs[p] = -g.g * c->ca_full[p];
-    second_derivatives(c->bxx.data(), c->byy.data(), c->bx.data(), c->bxy.data(), u, u, u, nx, ny, c->dx, c->dy);
+    second_derivatives(c->bxx.data(), c->byy.data(), c->bx.data(), c->bxy.data(), u, u, u, nx, ny, c->Dx.data(),
+                       c->Dy.data());
 #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) {
         c->Mxx[p] = -c->D[p] * (nu * c->byy[p] + c->bxx[p]);
@@ -537,12 +557,12 @@ void update_dudt_lvelva(fi_ctx* c, double* u, double* dudt) {
         c->Mxy[p] = -c->D[p] * (1 - nu) * c->bxy[p];
     }
     second_derivatives(c->bxx.data(), c->byy.data(), c->bx.data(), c->bxy.data(), c->Mxx.data(), c->Myy.data(),
-                       c->Mxy.data(), nx, ny, c->dx, c->dy);
+                       c->Mxy.data(), nx, ny, c->Dx.data(), c->Dy.data());
 #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) c->rhs[p] += c->bxx[p] + (2 * c->bxy[p] + c->byy[p]);
     // samesize_conv!(rhs, ..., EmptyConvolution) is a no-op (src/convolutions.jl:120-122)
 #pragma omp parallel for schedule(static)
-    for (size_t p = 0; p < N; ++p) c->fftrhs[p] = cd(c->rhs[p] * 1.0 / (2 * c->eta[p]), 0.0);
+    for (size_t p = 0; p < N; ++p) c->fftrhs[p] = cd(c->rhs[p] * c->K[p] / (2 * c->eta[p]), 0.0);
     fft2d(c->fftrhs.data(), nx, ny, c->vfx, c->vfy, -1);
 #pragma omp parallel for schedule(static)
     for (size_t p = 0; p < N; ++p) c->fftrhs[p] *= c->spinv[p];
@@ -665,7 +685,7 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     c->my = c->ny / 2;
     c->dx = 2 * cfg->Wx / c->nx;
     c->dy = 2 * cfg->Wy / c->ny;
-    c->A = c->dx * c->dy;  // (dx*dy) .* K.^2, K = 1 (src/domain.jl:167)
+    c->A = c->dx * c->dy;
     c->rho_sw_ice = cfg->rho_seawater / cfg->rho_ice;
     samesize_conv_indices(c->nx, c->mx, c->i1, c->i2);
     samesize_conv_indices(c->ny, c->my, c->j1, c->j2);
@@ -674,6 +694,7 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     c->vfy = FFT(c->ny);
     c->cp.init(c->nx, c->ny);
     size_t N = c->N;
+    set_distortion(c, vec(N, 1.0));
     c->maskactive.assign(N, 1.0);
     c->u_ref.assign(N, 0.0);
     c->ue_ref.assign(N, 0.0);
@@ -703,6 +724,11 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_REF_DZ_SS: c->dz_ss_ref = v; break;
         case FI_REF_Z_B: c->z_b_ref = v; break;
         case FI_REF_H_ICE: c->H_ice_ref_in = v; c->has_href = true; break;
+        case FI_DISTORTION:
+            for (double k : v)
+                if (!(k > 0) || !std::isfinite(k)) return fail(c, FI_ERR_INVALID, "distortion K must be positive");
+            set_distortion(c, v);
+            break;
         case FI_U:
             if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_field(FI_U) before fi_init");
             c->u = v;
@@ -737,6 +763,7 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_REF_DZ_SS: v = &c->dz_ss_ref; break;
         case FI_REF_Z_B: v = &c->z_b_ref; break;
         case FI_REF_H_ICE: v = &c->H_ice_ref; break;
+        case FI_DISTORTION: v = &c->K; break;
         case FI_U: v = c->integ ? &c->uprev : &c->u; break;
         case FI_DUDT: v = &c->dudt; break;
         case FI_UE: v = &c->ue; break;
@@ -1134,7 +1161,19 @@ void ref_fft(int n, const double* re, const double* im, double* ore, double* oim
 
 void ref_derivatives(int nx, int ny, double dx, double dy, const double* u, double* uxx, double* uyy,
                      double* ux, double* uxy) {
-    second_derivatives(uxx, uyy, ux, uxy, u, u, u, nx, ny, dx, dy);
+    const vec Dx((size_t)nx * ny, dx), Dy((size_t)nx * ny, dy);
+    second_derivatives(uxx, uyy, ux, uxy, u, u, u, nx, ny, Dx.data(), Dy.data());
+}
+
+// the same with per-cell steps K dx, K dy
+void ref_derivatives_k(int nx, int ny, double dx, double dy, const double* K, const double* u, double* uxx,
+                       double* uyy, double* ux, double* uxy) {
+    vec Dx((size_t)nx * ny), Dy((size_t)nx * ny);
+    for (size_t p = 0; p < Dx.size(); ++p) {
+        Dx[p] = K[p] * dx;
+        Dy[p] = K[p] * dy;
+    }
+    second_derivatives(uxx, uyy, ux, uxy, u, u, u, nx, ny, Dx.data(), Dy.data());
 }
 
 // conv! -> output_cropped ((2nx-1) x (2ny-1)), src/convolutions.jl:94-106

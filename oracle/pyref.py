@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfastiso_ref.so")
 FIELDS = {
     "litho_rigidity": 1, "effective_viscosity": 2, "scaled_pseudodiff_inv": 3, "maskactive": 4,
     "green_elastic": 5, "green_geoid": 6, "ref_u": 7, "ref_ue": 8, "ref_dz_ss": 9, "ref_z_b": 10,
-    "ref_H_ice": 11,
+    "ref_H_ice": 11, "distortion": 12,
     "u": 32, "dudt": 33, "ue": 34, "dz_ss": 35, "z_ss": 36, "z_b": 37, "H_ice": 38, "H_af": 39,
     "H_water": 40, "maskgrounded": 41, "maskocean": 42, "canom_ice": 43, "canom_seawater": 44,
     "canom_litho": 45, "canom_mantle": 46, "canom_load": 47, "canom_full": 48,
@@ -79,6 +79,7 @@ def lib():
             "fi_batch_integrate_each": (C.c_int, [P, _dp, C.c_int32]),
             "ref_fft": (None, [C.c_int, _dp, _dp, _dp, _dp, C.c_int]),
             "ref_derivatives": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+            "ref_derivatives_k": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
             "ref_conv_cropped": (None, [C.c_int, C.c_int, _dp, _dp, _dp]),
             "ref_samesize_conv": (None, [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp]),
             "ref_nextfastfft": (C.c_int, [C.c_int]),
@@ -286,6 +287,15 @@ def derivatives(u, dx, dy):
     uf = _flat(u)
     outs = [np.empty(nx * ny) for _ in range(4)]
     lib().ref_derivatives(nx, ny, dx, dy, _p(uf), *[_p(o) for o in outs])
+    return [_shape(o, nx, ny) for o in outs]
+
+
+def derivatives_k(u, dx, dy, K):
+    """update_second_derivatives! with per-cell steps K dx, K dy: (uxx, uyy, ux, uxy)."""
+    nx, ny = u.shape
+    uf, kf = _flat(u), _flat(K)
+    outs = [np.empty(nx * ny) for _ in range(4)]
+    lib().ref_derivatives_k(nx, ny, dx, dy, _p(kf), _p(uf), *[_p(o) for o in outs])
     return [_shape(o, nx, ny) for o in outs]
 
 

@@ -153,3 +153,21 @@ def test_ensemble_vs_oracle():
         assert np.abs(u - b).max() <= tol_u, (m, np.abs(u - b).max(), tol_u)
         assert np.abs(ue - eb).max() <= tol_e, (m, np.abs(ue - eb).max(), tol_e)
         assert st["count_sparse_updates"] == ref[m].stats()["count_sparse_updates"]
+
+
+@pytest.mark.gpu
+def test_ensemble_mixing_distorted_members_equals_single_contexts_bitwise():
+    """With one distorted member the whole launch runs the kernel with per-cell grid steps; the
+    undistorted members take K = 1 there, and every member still equals its own single context."""
+    inputs = small_ensemble(nmem=4)
+    d = inputs[1]["cfg"]
+    nx, ny = d.nx, d.ny
+    x = np.linspace(-1.0, 1.0, nx)[:, None]
+    y = np.linspace(-1.0, 1.0, ny)[None, :]
+    inputs[1] = dict(inputs[1], fields=dict(inputs[1]["fields"], distortion=0.96 + 0.05 * (x ** 2 + y ** 2)))
+    res, _ = run_ensemble(inputs, team_size=4)
+    for m, inp in enumerate(inputs):
+        e = hip_single(with_tout(inp, T_OUT))
+        e.integrator_init()
+        e.integrate(T_OUT[-1], lambda t, k: None)
+        np.testing.assert_array_equal(res[m][0], e.get_field("u"), err_msg=f"member {m}")
