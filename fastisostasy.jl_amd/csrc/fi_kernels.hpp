@@ -418,6 +418,22 @@ __device__ __forceinline__ float2 c2r_pair_value(const KP& p, const float2* spec
 // ------------------------------------------------------------------------------------------
 // rows phase
 // ------------------------------------------------------------------------------------------
+// Stores of data that other workgroups read after the next grid barrier (spectra, derivative
+// slots, the stage displacement, partial sums). Relaxed agent-scope atomic stores write through
+// the L2 (sc1), so the barrier's release has no dirty lines of theirs to write back; the
+// barrier's release/acquire pair orders them as before. Vector stores only.
+__device__ __forceinline__ void st_shared(float* a, float v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_shared(double* a, double v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_shared(float2* a, float2 v) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &v, sizeof(b));
+    __hip_atomic_store((unsigned long long*)a, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // exact x / d for small non-negative integers (fractional part of (x+0.5)/d >= 0.5/d)
 __device__ __forceinline__ int fdivi(int x, float inv_d) { return (int)(((float)x + 0.5f) * inv_d); }
 
@@ -507,8 +523,8 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                     if (ci >= 0) v = ci == 0 ? kc.v0 : ci == 1 ? kc.v1 : ci == 2 ? kc.v2 : kc.v3;
                     skin[(j - ju0) * nx + i] = v;
                     if (j >= j0 && j < je) {
-                        Kin[i + (long)j * nx] = v;
-                        if (ci >= 0) p.kcorner[4 * st.kin + ci] = v;
+                        st_shared(&Kin[i + (long)j * nx], v);
+                        if (ci >= 0) st_shared(&p.kcorner[4 * st.kin + ci], v);
                     }
                 }
             }
@@ -546,7 +562,7 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             su[idx] = u;
             const int r = fdivi(idx, inv_nx);
             const int j = ju0 + r;
-            if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[idx - r * nx + (long)j * nx] = u;
+            if (st.write_u >= 0 && j >= j0 && j < je) st_shared(&ubuf(p, st.write_u)[idx - r * nx + (long)j * nx], u);
         }
         __syncthreads();
         FI_SUBMARK(4);
@@ -589,10 +605,10 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
                     f = (float)(rhs * (double)fr[q].i2e);
                     if (wsig) {
-                        p.sig_load_prev[c] = (float)s_load;
-                        p.sig_geo_prev[c] = (float)(p.compressible_column
+                        st_shared(&p.sig_load_prev[c], (float)s_load);
+                        st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
                                                         ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
-                                                        : s_full);
+                                                        : s_full));
                     }
                 }
                 if (r == 0) sf[i].x = f;
@@ -609,8 +625,8 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            spout[(long)kx * ny + j0] = X0;
-            if (j0 + 1 < ny) spout[(long)kx * ny + j0 + 1] = X1;
+            st_shared(&spout[(long)kx * ny + j0], X0);
+            if (j0 + 1 < ny) st_shared(&spout[(long)kx * ny + j0 + 1], X1);
         }
         __syncthreads();
         FI_SUBMARK(7);
@@ -762,8 +778,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                 if (ci >= 0) v = ci == 0 ? kc.v0 : ci == 1 ? kc.v1 : ci == 2 ? kc.v2 : kc.v3;
                 skin[(j - ju0) * nx + i] = v;
                 if (j >= j0 && j < je) {
-                    Kin[i + (long)j * nx] = v;
-                    if (ci >= 0) p.kcorner[4 * st.kin + ci] = v;
+                    st_shared(&Kin[i + (long)j * nx], v);
+                    if (ci >= 0) st_shared(&p.kcorner[4 * st.kin + ci], v);
                 }
             }
             __syncthreads();
@@ -815,7 +831,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                 if (st.write_u >= 0) {
                     double* un = ubuf(p, st.write_u);
 #pragma unroll
-                    for (int o = 0; o < NO; ++o) un[i + (long)(j0 + o) * nx] = uo[o];
+                    for (int o = 0; o < NO; ++o) st_shared(&un[i + (long)(j0 + o) * nx], uo[o]);
                 }
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
@@ -880,10 +896,10 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
                     f[o] = (float)(rhs * (double)sI2e[idx]);
                     if (wsig) {
-                        p.sig_load_prev[cc] = (float)s_load;
-                        p.sig_geo_prev[cc] = (float)(p.compressible_column
+                        st_shared(&p.sig_load_prev[cc], (float)s_load);
+                        st_shared(&p.sig_geo_prev[cc], (float)(p.compressible_column
                                                          ? s_load + msk * (s_mant - p.rho_um * (double)p.u_ref[cc])
-                                                         : s_full);
+                                                         : s_full));
                     }
                 }
                 sf[i] = make_float2(f[0], f[1]);
@@ -923,7 +939,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                 su[idx] = u;
                 const int r = fdivi(idx, inv_nx);
                 const int j = ju0 + r;
-                if (st.write_u >= 0 && j >= j0 && j < je) ubuf(p, st.write_u)[idx - r * nx + (long)j * nx] = u;
+                if (st.write_u >= 0 && j >= j0 && j < je) st_shared(&ubuf(p, st.write_u)[idx - r * nx + (long)j * nx], u);
             }
             __syncthreads();
             FI_SUBMARK(4);
@@ -959,10 +975,10 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
                     f = (float)(rhs * (double)sI2e[idx]);
                     if (wsig) {
-                        p.sig_load_prev[c] = (float)s_load;
-                        p.sig_geo_prev[c] = (float)(p.compressible_column
+                        st_shared(&p.sig_load_prev[c], (float)s_load);
+                        st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
                                                         ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
-                                                        : s_full);
+                                                        : s_full));
                     }
                 }
                 if (r == 0) sf[i].x = f;
@@ -980,8 +996,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            spout[(long)kx * ny + j0] = X0;
-            if (j0 + 1 < ny) spout[(long)kx * ny + j0 + 1] = X1;
+            st_shared(&spout[(long)kx * ny + j0], X0);
+            if (j0 + 1 < ny) st_shared(&spout[(long)kx * ny + j0 + 1], X1);
         }
         __syncthreads();
         FI_SUBMARK(7);
@@ -1022,7 +1038,7 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* 
             __syncthreads();
             float2* o = (r == sa) ? sb : sa;
             float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
-            for (int idx = threadIdx.x; idx < tot; idx += NT) spec[base + idx] = r2[idx];
+            for (int idx = threadIdx.x; idx < tot; idx += NT) st_shared(&spec[base + idx], r2[idx]);
             __syncthreads();
             continue;
         }
@@ -1037,7 +1053,7 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* 
         __syncthreads();
         float2* o = (r == sa) ? sb : sa;
         float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) spec[base + idx] = r2[idx];
+        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) st_shared(&spec[base + idx], r2[idx]);
         __syncthreads();
     }
 }
@@ -1081,9 +1097,9 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
             const float v = r ? res[i].y : res[i].x;
-            K[c] = v;
+            st_shared(&K[c], v);
             const int ci = corner_index(p, i, j);
-            if (ci >= 0) p.kcorner[4 * slot + ci] = v;
+            if (ci >= 0) st_shared(&p.kcorner[4 * slot + ci], v);
             if (st.final_err) {
                 double k1, k2, k3, un, up;
                 if (pre) {
@@ -1110,9 +1126,9 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
         if (st.final_err) {
             block_sum<3>(red, acc);
             if (threadIdx.x == 0) {
-                p.part[4 * item + 0] = acc[0];
-                p.part[4 * item + 1] = acc[1];
-                p.part[4 * item + 2] = acc[2];
+                st_shared(&p.part[4 * item + 0], acc[0]);
+                st_shared(&p.part[4 * item + 1], acc[1]);
+                st_shared(&p.part[4 * item + 2], acc[2]);
             }
         }
         __syncthreads();
