@@ -38,6 +38,12 @@ __device__ __forceinline__ bool team_sync(const Team& tm, unsigned& gen, int* s_
     unsigned* release = arrive + (long)tm.size * BAR_SLOT;
     const unsigned g = gen + 1u;
     gen = g;
+    // Every store that another workgroup reads after this barrier is a write-through agent-scope
+    // store (st_shared). Waiting until each wave's stores have completed therefore publishes
+    // them, and the arrival and release flags need no L2 writeback (release fence) of their own.
+    // The acquire side keeps its L2 invalidate: the data are then read with plain loads.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tm.rank == 0) {
         for (int w = 1 + (int)threadIdx.x; w < tm.size; w += NT) {
@@ -55,10 +61,10 @@ __device__ __forceinline__ bool team_sync(const Team& tm, unsigned& gen, int* s_
         __syncthreads();
         // publish the generation into every workgroup's own release slot
         for (int w = 1 + (int)threadIdx.x; w < tm.size; w += NT)
-            __hip_atomic_store(&release[w * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&release[w * BAR_SLOT], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (threadIdx.x == 0) *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (threadIdx.x == 0) {
-        __hip_atomic_store(&arrive[tm.rank * BAR_SLOT], g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&arrive[tm.rank * BAR_SLOT], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long t0 = wall_clock64();
         unsigned* mine = &release[tm.rank * BAR_SLOT];
         while (__hip_atomic_load(mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g) {
@@ -426,6 +432,9 @@ __device__ __forceinline__ void st_shared(float* a, float v) {
     __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_shared(double* a, double v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_shared(unsigned char* a, unsigned char v) {
     __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_shared(float2* a, float2 v) {
@@ -1174,8 +1183,8 @@ __device__ __forceinline__ void phase_sp_rows(const KP& p, const Team& tm, char*
             A.y = 0.5f * (Z.y - Zm.y);
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
-            p.pA[(long)kx * ny + j] = A;
-            p.pB[(long)kx * ny + j] = B;
+            st_shared(&p.pA[(long)kx * ny + j], A);
+            st_shared(&p.pB[(long)kx * ny + j], B);
         }
         __syncthreads();
     }
@@ -1218,7 +1227,7 @@ __device__ __forceinline__ void phase_sp_cols(const KP& p, const Team& tm, char*
         float2* r2 = fft_lds<float2>(r, o, 2, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < 2 * no; idx += NT) {
             const int s = idx / no, jj = idx - s * no;
-            (s ? p.oB : p.oA)[(long)kx * no + jj] = r2[s * nfy + sp_row_of(p, jj)];
+            st_shared(&(s ? p.oB : p.oA)[(long)kx * no + jj], r2[s * nfy + sp_row_of(p, jj)]);
         }
         __syncthreads();
     }
@@ -1256,17 +1265,17 @@ __device__ __forceinline__ void phase_sp_c2r(const KP& p, const Team& tm, char* 
         if (jj == 0 || jj == no - 1) {
             if (threadIdx.x == 0) {
                 const int q = (jj == 0) ? 0 : 2;
-                p.ext_corners[q] = res[0].x;
-                p.ext_corners[q + 1] = res[2 * nx - 2].x;
-                p.ext_corners[4 + q] = res[0].y;
-                p.ext_corners[4 + q + 1] = res[2 * nx - 2].y;
+                st_shared(&p.ext_corners[q], res[0].x);
+                st_shared(&p.ext_corners[q + 1], res[2 * nx - 2].x);
+                st_shared(&p.ext_corners[4 + q], res[0].y);
+                st_shared(&p.ext_corners[4 + q + 1], res[2 * nx - 2].y);
             }
         } else {
             const int j = jj - 1;
             for (int ii = threadIdx.x; ii < nx; ii += NT) {
                 const float2 v = res[p.i1 - 1 + ii];
-                p.ue_raw[ii + (long)j * nx] = v.x;
-                p.dz_raw[ii + (long)j * nx] = v.y;
+                st_shared(&p.ue_raw[ii + (long)j * nx], v.x);
+                st_shared(&p.dz_raw[ii + (long)j * nx], v.y);
             }
         }
         __syncthreads();
@@ -1295,9 +1304,9 @@ __device__ __forceinline__ void phase_sp_volumes(const KP& p, const Team& tm, co
         }
         block_sum<3>(red, acc);
         if (threadIdx.x == 0) {
-            p.vpart[4 * item + 0] = acc[0];
-            p.vpart[4 * item + 1] = acc[1];
-            p.vpart[4 * item + 2] = acc[2];
+            st_shared(&p.vpart[4 * item + 0], acc[0]);
+            st_shared(&p.vpart[4 * item + 1], acc[1]);
+            st_shared(&p.vpart[4 * item + 2], acc[2]);
         }
     }
 }
@@ -1391,16 +1400,16 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
         const double m = p.mask[c] ? 1.0 : 0.0;
         const double sw = p.interactive ? p.rho_seawater * (zss_new - (double)p.z_ss_ref[c]) * (moc ? 1.0 : 0.0) * m : 0.0;
         const double ca_litho = p.rho_litho * (ue_new - (double)p.ue_ref[c]);
-        p.ue[c] = (float)ue_new;
-        p.dz_ss[c] = (float)dz_new;
-        p.z_ss[c] = (float)zss_new;
-        p.mg[c] = mgc;
-        p.mo[c] = moc;
-        p.H_water[c] = (float)hw;
-        p.LW[c] = (float)sw;
-        p.LT[c] = (float)(ca_litho - p.rho_um * (double)p.u_ref[c]);
-        p.zoff[c] = (float)((double)p.z_b_ref[c] + ue_new - zss_new);
-        p.zoff_trig[c] = (float)((double)p.z_b_ref[c] + ue_old - zss_new);
+        st_shared(&p.ue[c], (float)ue_new);
+        st_shared(&p.dz_ss[c], (float)dz_new);
+        st_shared(&p.z_ss[c], (float)zss_new);
+        st_shared(&p.mg[c], (unsigned char)mgc);
+        st_shared(&p.mo[c], (unsigned char)moc);
+        st_shared(&p.H_water[c], (float)hw);
+        st_shared(&p.LW[c], (float)sw);
+        st_shared(&p.LT[c], (float)(ca_litho - p.rho_um * (double)p.u_ref[c]));
+        st_shared(&p.zoff[c], (float)((double)p.z_b_ref[c] + ue_new - zss_new));
+        st_shared(&p.zoff_trig[c], (float)((double)p.z_b_ref[c] + ue_old - zss_new));
       }
     }
 }
