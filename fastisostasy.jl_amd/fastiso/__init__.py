@@ -15,9 +15,10 @@ from .sealevel import (A_OCEAN_PD, ConstantBSL, ConstantOceanSurfaceBSL, Externa
                        RegionalSeaLevel, interpolate0d, update_bsl)
 from .solidearth import (CompressibleLithosphereColumn, CompressibleMantle, IncompressibleLithosphereColumn,
                          IncompressibleMantle, LaterallyConstantLithosphere, LaterallyVariableLithosphere,
-                         MaxwellMantle, NoCalibration, RigidLithosphere, RigidMantle, SeakonCalibration,
-                         SolidEarth)
-from .greens import elastic_green, gauss_legendre, geoid_green, load_farrell_table
+                         MaxwellMantle, NoCalibration, RelaxedMantle, RigidLithosphere, RigidMantle,
+                         SeakonCalibration, SolidEarth, get_flexural_lengthscale, get_relaxation_time,
+                         get_relaxation_time_stronger, get_relaxation_time_weaker)
+from .greens import elastic_green, gauss_legendre, geoid_green, green_viscous, load_farrell_table
 from .simulation import (DiffEqOptions, Integrator, NativeOutput, Simulation, SolverOptions, init_integrator,
                          push_inputs, run, step, update_diagnostics)
 from .utils import uniform_ice_cylinder

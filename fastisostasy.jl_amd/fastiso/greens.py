@@ -63,3 +63,15 @@ def geoid_green(domain, c):
     """get_dz_ss_green (src/sealevel.jl:208-215)."""
     g = unbounded_dz_ssgreen(domain.R, c)
     return np.minimum(g, unbounded_dz_ssgreen(domain.dx / 2, c))
+
+
+def green_viscous(domain, rho, D):
+    """green_viscous(domain, rho, D) (src/material.jl:370-380): the ELRA response to a point load,
+    -L^2 / (2 pi D) kei(r / L) dx dy with the flexural length L = (D / (9.81 rho))^(1/4)
+    (src/material.jl:396-399, g fixed at 9.81 there) and r = max(R, 1). besselkei(x) =
+    Im K0(x e^{i pi/4}) is the Kelvin function kei (scipy.special.kei)."""
+    from scipy.special import kei
+    from .solidearth import get_flexural_lengthscale
+    L = get_flexural_lengthscale(D, rho, 9.81)
+    R = np.maximum(domain.R, 1.0)
+    return -(L ** 2 / (2 * np.pi * D) * kei(R / L)) * (domain.dx * domain.dy)

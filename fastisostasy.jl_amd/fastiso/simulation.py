@@ -13,11 +13,11 @@ from . import _abi
 from .bcs import ExternallyUpdatedIceThickness, SeparableIceThickness, TimeInterpolatedIceThickness
 from .domain import PhysicalConstants
 from .engine import Engine
-from .greens import elastic_green, geoid_green
+from .greens import elastic_green, geoid_green, green_viscous
 from .sealevel import (GoelzerAdjustmentContribution, GoelzerDensityContribution, GoelzerVolumeContribution,
                        InteractiveSealevelLoad, LaterallyVariableSeaSurface, bsl_inputs)
 from .solidearth import (CompressibleLithosphereColumn, LaterallyConstantLithosphere, MaxwellMantle,
-                         RigidLithosphere, RigidMantle)
+                         RelaxedMantle, RigidLithosphere, RigidMantle)
 
 
 @dataclass
@@ -117,6 +117,11 @@ class Simulation:
             fields["green_elastic"] = elastic_green(domain, quad_precision=quad_precision)
         if isinstance(sealevel.surface, LaterallyVariableSeaSurface):
             fields["green_geoid"] = geoid_green(domain, self.c)
+        if isinstance(solidearth.mantle, RelaxedMantle):
+            # ELRA (src/tools.jl:55-60): the viscous Green kernel of the mean rigidity, and tau
+            fields["green_viscous"] = green_viscous(domain, solidearth.rho_uppermantle,
+                                                    float(np.mean(solidearth.litho_rigidity)))
+            fields["tau"] = np.broadcast_to(np.asarray(solidearth.tau, dtype=np.float64), (domain.nx, domain.ny))
         for name, val in (("ref_u", u_ref), ("ref_ue", ue_ref), ("ref_dz_ss", dz_ss_ref), ("ref_z_b", z_b_ref)):
             if val is not None:
                 fields[name] = np.broadcast_to(np.asarray(val, dtype=np.float64), (domain.nx, domain.ny))

@@ -36,6 +36,27 @@ def get_rigidity(t, E, nu):
     return (E * t ** 3) / (12 * (1 - nu ** 2))
 
 
+def get_relaxation_time(eta, m, p):
+    """Relaxation time (yr) of the relaxed mantle from a viscosity map, 10^(m log10(eta) - p)
+    (src/material.jl:406)."""
+    return 10.0 ** (np.log10(eta) * m - p)
+
+
+def get_relaxation_time_weaker(eta):
+    """src/material.jl:413 (Van Calcar et al., weaker mantle)."""
+    return get_relaxation_time(eta, 0.35, 4.63)
+
+
+def get_relaxation_time_stronger(eta):
+    """src/material.jl:420 (Van Calcar et al., stronger mantle)."""
+    return get_relaxation_time(eta, 0.20, 1.41)
+
+
+def get_flexural_lengthscale(litho_rigidity, rho_uppermantle, g):
+    """(D / (rho g))^(1/4) (src/material.jl:396-399)."""
+    return (litho_rigidity / (rho_uppermantle * g)) ** 0.25
+
+
 def channel_scaling(kappa, channel_thickness, visc_ratio):
     """src/material.jl:232-262, in FP64."""
     C = np.cosh(channel_thickness * kappa)

@@ -88,6 +88,35 @@ def lv_elva(n=350, W=3.15e6, t_end=120e3, seeds=(1001, 1002, 1003), separable=Tr
                 opts=SolverOptions(), c=PhysicalConstants(), logeta=logeta, h=h, H_dome=Hd)
 
 
+def elra(n=64, W=3e6, t_end=50e3, tau=3e3, lithosphere="rigid", t_out=None, interactive=False):
+    """ELRA (docs/src/examples/alternative_models.jl:10-33,44-70): a 1000 m ice cylinder of
+    radius 1000 km switched on over the first year, RelaxedMantle, 0-50 kyr. tau="weak" is the
+    2D relaxation time get_relaxation_time_weaker of a viscosity 10^(21 - g) with g a Gaussian of
+    peak 1 and width W/4 at the centre; lithosphere "rigid" (the example) or "constant" (88 km,
+    elastic response on); interactive adds the laterally variable, interactive sea level."""
+    from .sealevel import InteractiveSealevelLoad
+    from .solidearth import (LaterallyConstantLithosphere, RelaxedMantle, RigidLithosphere,
+                             get_relaxation_time_weaker)
+    domain = RegionalDomain(W, W, n, n)
+    H1 = uniform_ice_cylinder(domain, 1e6, 1e3)
+    it = TimeInterpolatedIceThickness([0.0, 1.0, t_end], [domain.zeros(), H1, H1], domain)
+    bcs = BoundaryConditions(domain, ice_thickness=it)
+    if isinstance(tau, str):
+        s2 = (W / 4) ** 2
+        g = np.exp(-0.5 * (domain.X ** 2 + domain.Y ** 2) / s2)
+        tau = get_relaxation_time_weaker(10.0 ** (21.0 - g / g.max()))
+    litho = RigidLithosphere() if lithosphere == "rigid" else LaterallyConstantLithosphere()
+    se = SolidEarth(domain, tau=tau, lithosphere=litho, mantle=RelaxedMantle())
+    sl = RegionalSeaLevel()
+    if interactive:
+        sl = RegionalSeaLevel(surface=LaterallyVariableSeaSurface(), load=InteractiveSealevelLoad())
+    if t_out is None:
+        t_out = [1e2, 3e2, 1e3, 3e3, 1e4, 2e4, 5e4]
+    nout = NativeOutput(t=[x for x in t_out if x <= t_end], vars=["u"])
+    return dict(domain=domain, bcs=bcs, sealevel=sl, solidearth=se, t_span=(0.0, t_end), nout=nout,
+                opts=SolverOptions(), c=PhysicalConstants(), H1=H1, tau=tau)
+
+
 def ensemble(nmem=256, W=3e6, t_end=20e3, sizes=(64, 96, 128, 192, 256), member_seed=2000, eta_seed=3000,
              litho_seed=4000, dome_seed=1003, separable=True):
     """Uneven ensemble: member m has side n_m drawn uniformly from sizes (seed member_seed + m), its

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FI_ABI_VERSION 3
+#define FI_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------------------------ */
 enum {
@@ -86,7 +86,10 @@ typedef struct fi_config {
     double rho_uppermantle;      /* 3400                                                       */
     double rho_litho;            /* 3200                                                       */
     int32_t lithosphere;         /* FI_*_LITHOSPHERE                                           */
-    int32_t mantle;              /* FI_*_MANTLE                                                */
+    int32_t mantle;              /* FI_*_MANTLE: FI_MAXWELL_MANTLE needs the laterally variable
+                                    lithosphere (LV-ELVA, src/deformation.jl:91-103);
+                                    FI_RELAXED_MANTLE a rigid or laterally constant one (ELRA,
+                                    src/deformation.jl:29-45)                                  */
     int32_t lithosphere_column;  /* FI_*_LITHOSPHERE_COLUMN                                    */
     /* RegionalSeaLevel                                                 src/sealevel.jl:150-166 */
     int32_t sea_surface;         /* FI_*_SEA_SURFACE                                           */
@@ -124,6 +127,11 @@ enum {
                                      factor K / (2 eta) (src/deformation.jl:95) and K^2 on the
                                      convolution inputs (src/deformation.jl:177); default 1. The HIP
                                      engine supports it on its FP64-moment path (nx <= 512, ny >= 6) */
+    FI_GREEN_VISCOUS = 13,        /* ELRA Green kernel (green_viscous, src/material.jl:375-380, built
+                                     from mean(litho_rigidity) and rho_uppermantle in src/tools.jl:
+                                     55-60); required with FI_RELAXED_MANTLE                   */
+    FI_TAU = 14,                  /* SolidEarth.tau, relaxation time (yr) of the relaxed mantle
+                                     (src/solidearth.jl:167,189); required with FI_RELAXED_MANTLE */
     /* state, readable after fi_init; FI_U and FI_H_ICE are also writable */
     FI_U = 32,                    /* CurrentState fields                src/state.jl:49-77     */
     FI_DUDT = 33,
@@ -141,7 +149,10 @@ enum {
     FI_CANOM_LITHO = 45,
     FI_CANOM_MANTLE = 46,
     FI_CANOM_LOAD = 47,
-    FI_CANOM_FULL = 48
+    FI_CANOM_FULL = 48,
+    FI_U_EQ = 49                  /* CurrentState.u_eq, equilibrium displacement of the relaxed
+                                     mantle (src/state.jl:60, src/deformation.jl:37-40); the
+                                     reference state u_ref until the first RHS evaluation      */
 };
 
 /* ---- run statistics ------------------------------------------------------------------------ */

@@ -326,11 +326,11 @@ struct fi_ctx {
     vec pd;
     FFT vfx, vfy;  // n x n complex transforms of the viscous operator (src/tools.jl:100-109)
     ConvPlan cp;
-    std::vector<cd> ge_hat, gg_hat;
+    std::vector<cd> ge_hat, gg_hat, gv_hat;
     // static inputs
-    vec D, eta, spinv, maskactive, green_e, green_g, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ice_ref_in;
-    bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false,
-         has_href = false;
+    vec D, eta, spinv, maskactive, green_e, green_g, green_v, tau, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ice_ref_in;
+    bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_gv = false,
+         has_tau = false, has_href = false;
     // ice forcing
     int ice_mode = 0;  // 0 external, 1 snapshots, 2 separable
     vec ice_t, ice_a, ice_H;
@@ -344,7 +344,7 @@ struct fi_ctx {
     vec H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
     std::vector<uint8_t> mg_ref, mo_ref;
     // current state (src/state.jl:49-77)
-    vec u, ue, dudt, H_ice, H_af, H_water, z_b, dz_ss, z_ss;
+    vec u, ue, dudt, H_ice, H_af, H_water, z_b, dz_ss, z_ss, u_eq;
     std::vector<uint8_t> mg, mo;
     vec ca_ice, ca_sw, ca_sed, ca_litho, ca_mantle, ca_load, ca_full;
     double V_af = 0, V_pov = 0, V_den = 0, delta_V = 0, z_bsl = 0, A_bsl = 0;
@@ -573,6 +573,25 @@ void update_dudt_lvelva(fi_ctx* c, double* u, double* dudt) {
     apply_corner_bc(dudt, nx, ny, g.viscous_x_border);
 }
 
+// update_dudt!(RelaxedMantle, lithosphere) (src/deformation.jl:29-45): the equilibrium displacement
+// of the load, u_eq = samesize_conv(-(sigma_load + sigma_litho) g K^2, green_viscous) in the
+// regular BC space (crop to the domain, then the corner BC, src/convolutions.jl:145-154), and
+// dudt = (u_eq - u) / tau. The update_deformation_rhs! call at the top of that method only fills
+// scratch buffers that this method overwrites, so it has no effect on the result.
+void update_dudt_elra(fi_ctx* c, double* dudt) {
+    const fi_config& g = c->cfg;
+    size_t N = c->N;
+    for (size_t p = 0; p < N; ++p) c->buf[p] = -(c->ca_load[p] + c->ca_litho[p]) * g.g * (c->K[p] * c->K[p]);
+    vec crop;
+    c->cp.conv_cropped(c->buf.data(), c->gv_hat, crop);
+    const int nx = c->nx, ex = 2 * nx - 1;
+    for (int j = c->j1; j <= c->j2; ++j)
+        for (int i = c->i1; i <= c->i2; ++i)
+            c->u_eq[(i - c->i1) + (size_t)(j - c->j1) * nx] = crop[(i - 1) + (size_t)(j - 1) * ex];
+    apply_corner_bc(c->u_eq.data(), nx, c->ny, g.viscous_x_border);
+    for (size_t p = 0; p < N; ++p) dudt[p] = 1.0 / c->tau[p] * (c->u_eq[p] - c->u[p]);
+}
+
 // update_diagnostics!(dudt, u, sim, t) (src/simulation.jl:335-393)
 int update_diagnostics(fi_ctx* c, double* dudt, double* u, double t) {
     const fi_config& g = c->cfg;
@@ -603,6 +622,8 @@ int update_diagnostics(fi_ctx* c, double* dudt, double* u, double t) {
     // update_dudt! (src/deformation.jl:20-27,91-103)
     if (g.mantle == FI_RIGID_MANTLE) {
         std::fill(dudt, dudt + N, 0.0);
+    } else if (g.mantle == FI_RELAXED_MANTLE) {
+        update_dudt_elra(c, dudt);
     } else {
         update_dudt_lvelva(c, u, dudt);
     }
@@ -673,7 +694,10 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     if (!cfg || !out) return FI_ERR_INVALID;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->Wx > 0) || !(cfg->Wy > 0)) return FI_ERR_INVALID;
-    if (cfg->mantle == FI_RELAXED_MANTLE) return FI_ERR_UNSUPPORTED;
+    // RelaxedMantle with the laterally variable lithosphere errors in the reference
+    // (src/deformation.jl:47-50)
+    if (cfg->mantle == FI_RELAXED_MANTLE && cfg->lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE)
+        return FI_ERR_UNSUPPORTED;
     if (cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE)
         return FI_ERR_UNSUPPORTED;  // implicit Bueler step (src/deformation.jl:52-89) is not this path
     fi_ctx* c = new fi_ctx();
@@ -719,6 +743,13 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_MASK_ACTIVE: c->maskactive = v; break;
         case FI_GREEN_ELASTIC: c->green_e = v; c->has_ge = true; break;
         case FI_GREEN_GEOID: c->green_g = v; c->has_gg = true; break;
+        case FI_GREEN_VISCOUS: c->green_v = v; c->has_gv = true; break;
+        case FI_TAU:
+            for (double x : v)
+                if (!(x > 0) || !std::isfinite(x)) return fail(c, FI_ERR_INVALID, "tau must be positive");
+            c->tau = v;
+            c->has_tau = true;
+            break;
         case FI_REF_U: c->u_ref = v; break;
         case FI_REF_UE: c->ue_ref = v; break;
         case FI_REF_DZ_SS: c->dz_ss_ref = v; break;
@@ -758,6 +789,8 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_MASK_ACTIVE: v = &c->maskactive; break;
         case FI_GREEN_ELASTIC: v = &c->green_e; break;
         case FI_GREEN_GEOID: v = &c->green_g; break;
+        case FI_GREEN_VISCOUS: v = &c->green_v; break;
+        case FI_TAU: v = &c->tau; break;
         case FI_REF_U: v = &c->u_ref; break;
         case FI_REF_UE: v = &c->ue_ref; break;
         case FI_REF_DZ_SS: v = &c->dz_ss_ref; break;
@@ -781,6 +814,7 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_CANOM_MANTLE: v = &c->ca_mantle; break;
         case FI_CANOM_LOAD: v = &c->ca_load; break;
         case FI_CANOM_FULL: v = &c->ca_full; break;
+        case FI_U_EQ: v = &c->u_eq; break;
         default: return fail(c, FI_ERR_INVALID, "fi_get_field: unknown field");
     }
     if (v) {
@@ -853,6 +887,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     size_t N = c->N;
     if (g.mantle == FI_MAXWELL_MANTLE && (!c->has_D || !c->has_eta || !c->has_spinv))
         return fail(c, FI_ERR_INVALID, "litho rigidity, effective viscosity and scaled_pseudodiff_inv required");
+    if (g.mantle == FI_RELAXED_MANTLE && (!c->has_gv || !c->has_tau))
+        return fail(c, FI_ERR_INVALID, "viscous Green kernel and tau required");
     if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge)
         return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
@@ -866,6 +902,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     }
     if (c->has_ge) c->ge_hat = c->cp.spectrum(c->green_e.data());
     if (c->has_gg) c->gg_hat = c->cp.spectrum(c->green_g.data());
+    if (g.mantle == FI_RELAXED_MANTLE) c->gv_hat = c->cp.spectrum(c->green_v.data());
     c->t0 = t0;
     c->t_end = t_end;
     c->t_timer = t0;
@@ -892,6 +929,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     // CurrentState(domain, ref, z_bsl) (src/state.jl:80-103)
     c->u = c->u_ref;
     c->ue = c->ue_ref;
+    c->u_eq = c->u_ref;
     c->dudt.assign(N, 0.0);
     c->H_ice = c->H_ice_ref;
     c->H_af = c->H_af_ref;
@@ -1086,7 +1124,7 @@ static int ck_walk(fi_ctx* c, int mode, unsigned char* blob, size_t& off) {
     if (mode == 2 && std::memcmp(hr, hdr, sizeof(hdr)) != 0)
         return fail(c, FI_ERR_INVALID, "checkpoint of another grid, forcing or library");
     vv(c->u); vv(c->ue); vv(c->dudt); vv(c->H_ice); vv(c->H_af); vv(c->H_water); vv(c->z_b); vv(c->dz_ss);
-    vv(c->z_ss); vb(c->mg); vb(c->mo);
+    vv(c->z_ss); vb(c->mg); vb(c->mo); vv(c->u_eq);
     vv(c->ca_ice); vv(c->ca_sw); vv(c->ca_sed); vv(c->ca_litho); vv(c->ca_mantle); vv(c->ca_load); vv(c->ca_full);
     sc(c->V_af); sc(c->V_pov); sc(c->V_den); sc(c->delta_V); sc(c->z_bsl); sc(c->A_bsl); sc(c->count); sc(c->k1_stale);
     sc(c->t_timer); sc(c->integ); sc(c->t); sc(c->dt); sc(c->qold); sc(c->q11); sc(c->EEst);

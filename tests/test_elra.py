@@ -1,0 +1,186 @@
+"""ELRA: RelaxedMantle with a rigid or laterally constant lithosphere (src/deformation.jl:29-45).
+
+u_eq = samesize_conv(-(sigma_load + sigma_litho) g K^2, green_viscous) in the regular BC space
+(crop to the domain, then the corner BC; src/convolutions.jl:145-154) and dudt = (u_eq - u) / tau.
+
+Pins of the oracle:
+* green_viscous (src/material.jl:370-380): scipy's kei against the reference's own definition
+  Im K0(x e^{i pi/4}) evaluated with a second routine (the complex modified Bessel function), and
+  the plane integral of the kernel, -L^2/(2 pi D) 2 pi L^2 int_0^inf s kei(s) ds = L^4 / D =
+  1 / (9.81 rho): a uniform load q then sinks by q / (9.81 rho), local isostasy.
+* u_eq against a brute-force direct convolution of the oracle's own load fields, cropped and
+  corner-corrected in the test; dudt against (u_eq - u) / tau.
+* The relaxation of a rigid-lithosphere model whose load ramps up over the first year:
+  u(t) = U (1 - tau e^{-t/tau} (e^{1/tau} - 1)) per cell, with U the full-load u_eq and tau per
+  cell (the load does not depend on u when the ice stays grounded).
+* The relaxation-time maps (src/material.jl:406-425): 524 and 616 yr at 1e21 Pa s, the values the
+  reference quotes beside them.
+
+The HIP engine is compared with the oracle for single RHS evaluations with the elastic response
+and the interactive sea level on, and for integrations.
+"""
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import fastiso
+from fastiso import workloads
+
+from _cases import hip_sim, sim_inputs
+
+
+def test_relaxation_time_maps():
+    w = fastiso.get_relaxation_time_weaker(1e21)
+    s = fastiso.get_relaxation_time_stronger(1e21)
+    assert 524.0 <= w < 525.0 and 616.0 <= s < 617.0
+    eta = np.array([1e19, 1e22])
+    np.testing.assert_allclose(fastiso.get_relaxation_time(eta, 0.35, 4.63), 10 ** (np.log10(eta) * 0.35 - 4.63))
+
+
+def test_green_viscous_kelvin_function_and_isostatic_mass():
+    x = np.linspace(1e-3, 25.0, 400)
+    np.testing.assert_allclose(sp.kei(x), np.imag(sp.kv(0, x * np.exp(1j * np.pi / 4))), rtol=1e-9, atol=1e-12)
+    d = fastiso.RegionalDomain(1.2e6, 1.2e6, 160, 160)       # dx = 15 km, well below L ~ 100 km
+    rho, D = 3.4e3, fastiso.solidearth.get_rigidity(88e3, 6.6e10, 0.28)
+    G = fastiso.green_viscous(d, rho, D)
+    L = fastiso.get_flexural_lengthscale(D, rho, 9.81)
+    assert 0.9e5 < L < 1.2e5
+    assert abs(G.sum() * 9.81 * rho - 1.0) < 1e-2
+    # r = max(R, 1) at the centre cell
+    assert G[d.mx - 1, d.my - 1] == pytest.approx(-(L ** 2) / (2 * np.pi * D) * sp.kei(1.0 / L) * d.dx * d.dy)
+
+
+def _direct_conv(k, x):
+    n1, n2 = k.shape
+    out = np.zeros((2 * n1 - 1, 2 * n2 - 1))
+    for i in range(n1):
+        for j in range(n2):
+            out[i:i + n1, j:j + n2] += k[i, j] * x
+    return out
+
+
+def _crop(full, n):
+    i1 = n // 2 if n % 2 == 0 else n // 2 + 1     # 1-based (src/convolutions.jl:196-204)
+    i2 = 2 * n - 1 - n // 2
+    return full[i1 - 1:i2, i1 - 1:i2]
+
+
+def elra_case(n=24, lithosphere="constant", interactive=True, tau="weak"):
+    w = workloads.elra(n=n, W=3e6, t_end=20e3, tau=tau, lithosphere=lithosphere, interactive=interactive)
+    if interactive:
+        d = w["domain"]
+        r = d.R / d.R.max()
+        w["z_b_ref"] = 600.0 - 1500.0 * r
+        w["solidearth"] = fastiso.SolidEarth(d, tau=w["tau"], lithosphere=w["solidearth"].lithosphere,
+                                             mantle=fastiso.RelaxedMantle(), maskactive=r < 0.55)
+    return w
+
+
+@pytest.mark.parametrize("lithosphere,interactive", [("rigid", False), ("constant", True)])
+def test_oracle_elra_rhs_vs_direct_convolution(lithosphere, interactive):
+    from oracle import pyref
+    w = elra_case(n=24, lithosphere=lithosphere, interactive=interactive)
+    inp = sim_inputs(w).inputs
+    ref = pyref.RefEngine.from_inputs(inp)
+    d = w["domain"]
+    rng = np.random.default_rng(4)
+    Gv = inp["fields"]["green_viscous"]
+    tau = inp["fields"]["tau"]
+    for t in (0.5, 5000.0, 5010.0, 5015.0):
+        u = -30 * np.exp(-(d.R / 1.2e6) ** 2) + rng.standard_normal((d.nx, d.ny))
+        du, ubc = ref.rhs(t, u)
+        np.testing.assert_allclose(ubc, u - ((u[0, 0] + u[-1, 0] + u[0, -1] + u[-1, -1]) / 4), atol=1e-12)
+        q = -(ref.get_field("canom_load") + ref.get_field("canom_litho")) * 9.8
+        ueq = _crop(_direct_conv(Gv, q), d.nx)
+        ueq = ueq - (ueq[0, 0] + ueq[-1, 0] + ueq[0, -1] + ueq[-1, -1]) / 4
+        got = ref.get_field("u_eq")
+        assert np.abs(got - ueq).max() <= 1e-10 * np.abs(ueq).max(), t
+        np.testing.assert_allclose(du, (ueq - ubc) / tau, rtol=0, atol=1e-10 * np.abs(du).max())
+    if interactive:
+        assert np.abs(ref.get_field("ue")).max() > 0 and np.abs(ref.get_field("canom_seawater")).max() > 0
+
+
+def relaxation_closed_form(U, tau, t):
+    """u(t) / U for du/dt = (a(t) U - u) / tau, a(t) = min(t, 1), u(0) = 0, t >= 1."""
+    return 1.0 - tau * np.expm1(1.0 / tau) * np.exp(-t / tau)
+
+
+def test_oracle_elra_relaxation_closed_form():
+    from oracle import pyref
+    w = elra_case(n=32, lithosphere="rigid", interactive=False, tau="weak")
+    inp = sim_inputs(w).inputs
+    ref = pyref.RefEngine.from_inputs(inp)
+    tau = inp["fields"]["tau"]
+    ref.rhs(1.0, np.zeros(tau.shape))
+    U = ref.get_field("u_eq")                         # full-load equilibrium
+    assert U.min() < -100.0
+    ref = pyref.RefEngine.from_inputs(inp)
+    ref.integrator_init()
+    for t in (500.0, 1000.0, 3000.0):
+        ref.integrate(t)
+        u = ref.get_field("u")
+        want = U * relaxation_closed_form(U, tau, t)
+        assert np.abs(u - want).max() <= 2e-4 * np.abs(U).max(), t
+
+
+def test_oracle_elra_configuration_errors():
+    from oracle import pyref
+    w = elra_case(n=16, lithosphere="rigid", interactive=False)
+    inp = sim_inputs(w).inputs
+    f = dict(inp["fields"])
+    f.pop("tau")
+    with pytest.raises(RuntimeError, match="tau"):
+        pyref.RefEngine.from_inputs(dict(inp, fields=f))
+    bad = dict(inp, fields=dict(inp["fields"], tau=-np.ones((16, 16))))
+    with pytest.raises(RuntimeError, match="positive"):
+        pyref.RefEngine.from_inputs(bad)
+    cfg = sim_inputs(w).inputs["cfg"]
+    cfg.lithosphere = fastiso._abi.FI_LATERALLY_VARIABLE_LITHOSPHERE
+    with pytest.raises(RuntimeError, match="fi_create"):
+        pyref.RefEngine(cfg)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lithosphere,interactive,n", [("rigid", False, 48), ("constant", True, 48),
+                                                       ("constant", True, 45)])
+def test_gpu_elra_rhs_vs_oracle(lithosphere, interactive, n):
+    from oracle import pyref
+    w = elra_case(n=n, lithosphere=lithosphere, interactive=interactive)
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    d = w["domain"]
+    rng = np.random.default_rng(6)
+    for t in (0.5, 5000.0, 5010.0, 5015.0):
+        u = -30 * np.exp(-(d.R / 1.2e6) ** 2) + 0.1 * rng.standard_normal((d.nx, d.ny))
+        dg, _ = sim.engine.rhs(t, u)
+        dr, _ = ref.rhs(t, u)
+        m = np.abs(dg - dr).max() / np.abs(dr).max()
+        assert m < 1e-5, (t, m)
+        a, b = sim.engine.get_field("u_eq"), ref.get_field("u_eq")
+        assert np.abs(a - b).max() <= 2e-6 * np.abs(b).max(), t
+    for f in ("ue", "dz_ss", "canom_load"):
+        a, b = sim.engine.get_field(f), ref.get_field(f)
+        assert np.abs(a - b).max() <= 5e-6 * max(np.abs(b).max(), 1e-30), f
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lithosphere,interactive", [("rigid", False), ("constant", True)])
+def test_gpu_elra_integrate_vs_oracle(lithosphere, interactive):
+    from oracle import pyref
+    w = elra_case(n=48, lithosphere=lithosphere, interactive=interactive)
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    sim.engine.integrator_init()
+    ref.integrator_init()
+    for t in (300.0, 1000.0, 3000.0):
+        sim.engine.integrate(t)
+        ref.integrate(t)
+        a, b = sim.engine.get_field("u"), ref.get_field("u")
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), t
+    if not interactive:
+        tau = sim.inputs["fields"]["tau"]
+        ref2 = pyref.RefEngine.from_inputs(sim.inputs)
+        ref2.rhs(1.0, np.zeros(tau.shape))
+        U = ref2.get_field("u_eq")
+        want = U * relaxation_closed_form(U, tau, 3000.0)
+        assert np.abs(sim.engine.get_field("u") - want).max() <= 2e-4 * np.abs(U).max()
