@@ -14,7 +14,12 @@ unit); achieved = that traffic / the kernel time measured with HIP events on the
 cpu_baseline: the FP64 CPU restatement of the reference path (oracle/, OpenMP) on this host, on
 the same workload, over a bounded window.
 
-Other workloads (--workload): lv_elva_hr is the N=1024 high-resolution instance (W = 3000 km,
+Other workloads (--workload): elra is the relaxed-mantle model (ELRA) on the same N=350 grid:
+laterally constant 88 km lithosphere with the elastic response, the 2D relaxation time of the
+ELRA example (weak mantle, a Gaussian low-viscosity anomaly), a 1000 m ice cylinder of radius
+1000 km switched on over the first year, laterally variable sea surface; a step is a 100-yr window
+and the roofline counts ELRA_BYTES_PER_CELL_RHS per cell per RHS call. lv_elva_hr is the N=1024
+high-resolution instance (W = 3000 km,
 dx = 5.86 km, field seeds 5001/5002); ensemble is the uneven ensemble of 256 members with sides
 drawn from {64, 96, 128, 192, 256}, advanced together by fi_batch_integrate (value = member-kyr/s,
 the sum over members of simulated kyr per wall-clock second); surrogate is the N=350 workload run
@@ -31,16 +36,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "fastisostasy.jl_amd")]
 
 BYTES_PER_CELL_STEP = 171.0
+# ELRA, per cell per RHS call, counted once per unit: rows phase reads the base state (8) and on
+# average 5/3 FP32 derivative slots (6.7), mask, bedrock offset, reference H_af, litho anomaly,
+# u_ref and the two ice snapshots (25), writes the padded half spectrum (8), the loads for the
+# sparse update (8) and on the last stage u_{n+1} (8/3); the column phase reads and writes the
+# padded spectrum (16); the last phase reads it (8), the stage displacement (8) and 1/tau (4),
+# writes dudt and u_eq (8) and on the last stage reads k1..k3 and u_n for the error norm (20/3)
+ELRA_BYTES_PER_CELL_RHS = 8 + 20 / 3 + 25 + 8 + 8 + 8 / 3 + 16 + 8 + 8 + 4 + 8 + 20 / 3
 HBM_PEAK_GBS = 8000.0
 # measured HBM-side traffic of the persistent kernel at N=350 (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, calibrated on the copy probe); scaled to this run's accepted steps per launch
 TRAFFIC_N350 = os.path.join(ROOT, "profiles", "fi_persistent_lv_elva_n350_traffic.json")
 
 
-def make_sim(n, engine, hr=False):
+def make_sim(n, engine, hr=False, elra=False):
     import fastiso
     from fastiso import workloads
-    if hr:
+    if elra:
+        w = workloads.elra(n=n, W=3.15e6, t_end=50e3, tau="weak", lithosphere="constant", t_out=[])
+        w["sealevel"] = fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface())
+    elif hr:
         w = workloads.lv_elva(n=n, W=3e6, t_end=120e3, seeds=(5001, 5002, 1003), t_out=[])
     else:
         w = workloads.lv_elva(n=n, t_end=120e3, t_out=[])
@@ -108,11 +123,11 @@ def bench_ensemble(a):
     }
 
 
-def cpu_baseline(n, window, sample_s, hr=False):
+def cpu_baseline(n, window, sample_s, hr=False, elra=False):
     threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
     os.environ["OMP_NUM_THREADS"] = threads
     from oracle import pyref
-    w, sim = make_sim(n, engine=None, hr=hr)
+    w, sim = make_sim(n, engine=None, hr=hr, elra=elra)
     e = pyref.RefEngine.from_inputs(sim.inputs)
     e.integrator_init()
     e.integrate(window)                 # same warm-up as the GPU leg: the first window
@@ -244,7 +259,7 @@ def main():
     ap.add_argument("--window", type=float, default=None, help="window in yr (default 10; 5 for lv_elva_hr)")
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "ensemble", "surrogate", "fno_train"],
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "ensemble", "surrogate", "fno_train"],
                     default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=20.0)
@@ -253,12 +268,13 @@ def main():
     ap.add_argument("--team-size", type=int, default=0)
     a = ap.parse_args()
     hr_defaults = a.workload == "lv_elva_hr"
+    elra = a.workload == "elra"
     if a.steps is None:
-        a.steps = 3 if hr_defaults else 200
+        a.steps = 3 if hr_defaults else 50 if elra else 200
     if a.warmup is None:
-        a.warmup = 1 if hr_defaults else 5
+        a.warmup = 1 if hr_defaults else 2 if elra else 5
     if a.window is None:
-        a.window = 5.0 if hr_defaults else 10.0
+        a.window = 5.0 if hr_defaults else 100.0 if elra else 10.0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
@@ -277,7 +293,7 @@ def main():
         a.n = 1024
     # the engine owns its device memory and stream; every fi_integrate call returns after its
     # stream has synchronised, so wall-clock brackets below time completed device work
-    w, sim = make_sim(a.n, engine="hip", hr=hr)
+    w, sim = make_sim(a.n, engine="hip", hr=hr, elra=elra)
     eng = sim.engine
     eng.integrator_init()
     t = 0.0
@@ -298,13 +314,17 @@ def main():
     acc = s1["n_accept_timed"] - s0["n_accept_timed"]
     dev_s = (s1["device_ms"] - s0["device_ms"]) / 1e3
     launches = s1["n_launch"] - s0["n_launch"]
-    achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
+    rhs = s1["n_rhs"] - s0["n_rhs"]
+    if elra:
+        achieved = ELRA_BYTES_PER_CELL_RHS * N * rhs / dev_s / 1e9 if dev_s > 0 else None
+    else:
+        achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
     traffic = None
-    if a.n == 350 and not hr and os.path.exists(TRAFFIC_N350) and launches > 0:
+    if a.n == 350 and not hr and not elra and os.path.exists(TRAFFIC_N350) and launches > 0:
         with open(TRAFFIC_N350) as f:
             traffic = json.load(f)["bytes_per_accepted_step"] * acc / launches
     out = {
-        "metric": f"simulated kyr/s at N={a.n} LV-ELVA (plain solver)",
+        "metric": f"simulated kyr/s at N={a.n} {'ELRA' if elra else 'LV-ELVA'} (plain solver)",
         "value": kyr / el,
         "unit": "kyr/s",
         "n_gpus": 1,
@@ -327,7 +347,7 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic},
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr)
+        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr, elra)
     print(json.dumps(out), flush=True)
 
 

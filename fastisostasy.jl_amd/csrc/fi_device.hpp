@@ -138,6 +138,12 @@ struct KP {
     unsigned long long spin_limit;  // wall-clock ticks before a barrier wait gives up
     // optional phase timing (workgroup 0): [tag] work ticks, [8+tag] wait ticks, [16+tag] count
     unsigned long long* ptime;
+    // relaxed mantle (ELRA): viscous Green half spectrum [nfhx][nfy] (FP64, pre-scaled by
+    // 1/(nfx nfy)), 1/tau per cell (1/yr) and the equilibrium displacement u_eq
+    int elra;
+    const double2* gv_hat;
+    const float* inv_tau;
+    float* ueq;
 };
 enum { TAG_ROWS = 0, TAG_COLS = 1, TAG_C2R = 2, TAG_SPARSE = 3, TAG_KIN = 4 };
 

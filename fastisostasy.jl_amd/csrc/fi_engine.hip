@@ -21,7 +21,7 @@ namespace fi {
 // The adaptive BS3 loop of one simulation on a team: from (s.t, U[s.cur], K[s.k1]) until t
 // reaches s.next_out (after an accepted step) or s.stop, or p.max_steps attempts. Every team
 // member holds the same Ctl copy in LDS and takes the same decisions.
-template <typename MT, bool DIST>
+template <typename MT, bool DIST, bool ELRA>
 __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
                                                const float2* twy, char* smem, double* red, int* flag,
                                                unsigned long long& tl) {
@@ -61,6 +61,12 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
                 st.ng = 0;
                 st.kin = s2;
                 st.ain = BS3_A32;
+                if constexpr (ELRA) {   // the ELRA phases leave every derivative in its K slot
+                    st.ng = 1;
+                    st.g0 = s2;
+                    st.a0 = BS3_A32;
+                    st.kin = -1;
+                }
                 st.kout = s3;
                 st.t = tn + 0.75 * dt;
                 st.t_next_max = tn + dt;
@@ -72,6 +78,12 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
                 st.a1 = BS3_A42;
                 st.kin = s3;
                 st.ain = BS3_A43;
+                if constexpr (ELRA) {
+                    st.ng = 3;
+                    st.g2 = s3;
+                    st.a2 = BS3_A43;
+                    st.kin = -1;
+                }
                 st.write_u = unew;
                 st.kout = s4;
                 st.do_c2r = 1;
@@ -83,7 +95,10 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
                 st.e3 = s3;
                 st.u_prev = cur;
             }
-            if (!rhs_call<MT, DIST>(p, tm, s, st, gen, twx, twy, smem, red, flag, false, tl)) {
+            bool good;
+            if constexpr (ELRA) good = elra_call(p, tm, s, st, gen, twx, smem, red, flag, false, tl);
+            else good = rhs_call<MT, DIST>(p, tm, s, st, gen, twx, twy, smem, red, flag, false, tl);
+            if (!good) {
                 ok = false;
                 break;
             }
@@ -95,9 +110,11 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
 #endif
         // error norm with the corner-BC offsets of k1..k4 folded in:
         // sum ((a - C) w)^2 = S2 - 2 C S1 + C^2 S0, from one partial per row pair
-        // (one round of loads by the whole workgroup, fixed-order sum: identical in every workgroup)
-        const double C = dt * (BS3_BT1 * k_offset(p, k1) + BS3_BT2 * k_offset(p, s2) +
-                               BS3_BT3 * k_offset(p, s3) + BS3_BT4 * k_offset(p, s4));
+        // (one round of loads by the whole workgroup, fixed-order sum: identical in every workgroup);
+        // ELRA derivatives carry no BC offset
+        const double C = ELRA ? 0.0
+                              : dt * (BS3_BT1 * k_offset(p, k1) + BS3_BT2 * k_offset(p, s2) +
+                                      BS3_BT3 * k_offset(p, s3) + BS3_BT4 * k_offset(p, s4));
         double Sm[3] = {0.0, 0.0, 0.0};
         for (int w = threadIdx.x; w < p.nrowitems; w += NT) {
             Sm[0] += p.part[4 * w];
@@ -147,7 +164,7 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
 }
 
 // A single RHS evaluation on U[ubuf] at time t into K[kslot] (update_diagnostics!).
-template <typename MT, bool DIST>
+template <typename MT, bool DIST, bool ELRA>
 __device__ __forceinline__ bool single_rhs(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
                                            const float2* twy, char* smem, double* red, int* flag,
                                            unsigned long long& tl, int ubuf, int kslot, double t, bool sparse_only,
@@ -163,7 +180,8 @@ __device__ __forceinline__ bool single_rhs(const KP& p, const Team& tm, Ctl& s, 
     st.t = t;
     st.t_next_max = t;
     st.force_sigma = force_sigma ? 1 : 0;
-    return rhs_call<MT, DIST>(p, tm, s, st, gen, twx, twy, smem, red, flag, sparse_only, tl);
+    if constexpr (ELRA) return elra_call(p, tm, s, st, gen, twx, smem, red, flag, sparse_only, tl);
+    else return rhs_call<MT, DIST>(p, tm, s, st, gen, twx, twy, smem, red, flag, sparse_only, tl);
 }
 
 __device__ __forceinline__ void load_twiddles(const KP& p, float2* twx, float2* twy) {
@@ -189,7 +207,6 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
     const Team tm{(int)blockIdx.x, (int)gridDim.x, p.bar};
     unsigned gen = 0;
     unsigned long long tl = wall_clock64();
-    // twiddle tables of the two run-time transform lengths stay in LDS for the whole kernel
     float2* twx = (float2*)smem_all;
     float2* twy = twx + p.nx;
     char* smem = (char*)(twy + p.ny);
@@ -197,10 +214,36 @@ __global__ void __launch_bounds__(NT) fi_persistent(const KP* __restrict__ pp) {
     if (threadIdx.x == 0) s = *p.ctl;
     __syncthreads();
     if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY)
-        (void)single_rhs<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, p.rhs_ubuf, p.rhs_kslot, s.t_rhs,
-                             p.mode == MODE_SPARSE_ONLY, false);
+        (void)single_rhs<MT, DIST, false>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, p.rhs_ubuf, p.rhs_kslot,
+                                         s.t_rhs, p.mode == MODE_SPARSE_ONLY, false);
     else
-        integrate_loop<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
+        integrate_loop<MT, DIST, false>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
+    if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
+}
+
+// the relaxed mantle (ELRA): per-cell K^2 is read at run time, so one instance serves both grids.
+// (The body repeats fi_persistent's: sharing it through a common inline function changed the
+// register allocation of the LV-ELVA instances.)
+__global__ void __launch_bounds__(NT) fi_persistent_elra(const KP* __restrict__ pp) {
+    const KP& p = *(const KP*)(const ConstKP*)(unsigned long long)pp;
+    extern __shared__ __align__(16) char smem_all[];
+    __shared__ Ctl s;
+    __shared__ double red[64];
+    __shared__ int flag;
+    const Team tm{(int)blockIdx.x, (int)gridDim.x, p.bar};
+    unsigned gen = 0;
+    unsigned long long tl = wall_clock64();
+    float2* twx = (float2*)smem_all;
+    float2* twy = twx + p.nx;
+    char* smem = (char*)(twy + p.ny);
+    load_twiddles(p, twx, twy);
+    if (threadIdx.x == 0) s = *p.ctl;
+    __syncthreads();
+    if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY)
+        (void)single_rhs<double, false, true>(p, tm, s, gen, twx, twy, smem, red, &flag, tl, p.rhs_ubuf, p.rhs_kslot,
+                                              s.t_rhs, p.mode == MODE_SPARSE_ONLY, false);
+    else
+        integrate_loop<double, false, true>(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
     if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
 }
 
@@ -238,10 +281,10 @@ __device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, con
     __syncthreads();
     const bool todo = s.t < s.stop && !s.err;
     if (todo) {
-        integrate_loop<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
+        integrate_loop<MT, DIST, false>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
         if (*flag) return false;
         if (fsal && !s.err && s.t >= s.stop) {
-            if (!single_rhs<MT, DIST>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
+            if (!single_rhs<MT, DIST, false>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
                 if (*flag) return false;
             }
             if (threadIdx.x == 0) s.last_u = s.cur;
@@ -286,7 +329,8 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __res
 
 // kernel instances: FP64 or FP32 moments; with or without per-cell grid steps (map distortion,
 // FP64-moment path only)
-inline const void* persistent_kernel(bool mt_double, bool dist) {
+inline const void* persistent_kernel(bool mt_double, bool dist, bool elra) {
+    if (elra) return (const void*)fi_persistent_elra;
     if (!mt_double) return (const void*)fi_persistent<float, false>;
     return dist ? (const void*)fi_persistent<double, true> : (const void*)fi_persistent<double, false>;
 }
@@ -408,8 +452,10 @@ struct fi_ctx {
     double dx = 0, dy = 0, A = 0;
     int i1 = 0, i2 = 0, j1 = 0, j2 = 0;
     // host copies of the static inputs
-    std::vector<double> D, eta, spinv, mask, ge, gg, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ref_in;
-    bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_href = false;
+    std::vector<double> D, eta, spinv, mask, ge, gg, gv, tau, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ref_in;
+    bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_gv = false,
+         has_tau = false, has_href = false;
+    bool elra = false;   // RelaxedMantle (ELRA) instead of the LV-ELVA Maxwell mantle
     int ice_mode = ICE_EXTERNAL, ice_flat = 0;
     std::vector<double> ice_t, ice_a, ice_H;
     // barystatic sea-level rule (fi_set_bsl); A_OCEAN_PD (src/barystatic_sealevel.jl:1) by default
@@ -429,7 +475,8 @@ struct fi_ctx {
         z_ss, zoff, zoff_trig, LW, LT, H_water, sig_load_prev, sig_geo_prev, ue_raw, dz_raw, ext_corners, ice_H_d;
     DBuf<unsigned char> maskd, mg, mo;
     DBuf<float2> spec, pA, pB, oA, oB, tw32;
-    DBuf<double2> ge_hat, gg_hat, tw64, gtmp;
+    DBuf<double2> ge_hat, gg_hat, gv_hat, tw64, gtmp;
+    DBuf<float> inv_tau_d, ueq;
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
@@ -514,7 +561,7 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     HIPCHECK(c, hipMemcpyAsync(c->kp_d.p, &kp, sizeof(KP), hipMemcpyHostToDevice, c->stream));
     const KP* kptr = c->kp_d.p;
     void* args[] = {&kptr};
-    const void* f = persistent_kernel(c->mt_double, c->has_dist);
+    const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra);
     const bool timed = mode == MODE_INTEGRATE;
     const long long nacc0 = c->ctl.nacc;
     if (timed) HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -547,6 +594,7 @@ double corner_offset(const double* X, int nx, int ny, double xb) {
 }
 
 double kslot_offset(fi_ctx* c, int slot, float kc[4]) {
+    if (c->elra) return 0.0;   // ELRA derivatives are stored whole (no BC on dudt)
     double a = 0.25 * (double)kc[0], b = 0.25 * (double)kc[1], cc = 0.25 * (double)kc[2], d = 0.25 * (double)kc[3];
     (void)slot;
     return ((a + b) + (cc + d)) - c->cfg.viscous_x_border;
@@ -628,12 +676,16 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     if (!cfg || !out) return FI_ERR_INVALID;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->Wx > 0) || !(cfg->Wy > 0)) return FI_ERR_INVALID;
-    if (cfg->mantle == FI_RELAXED_MANTLE) return FI_ERR_UNSUPPORTED;
+    // RelaxedMantle with the laterally variable lithosphere errors in the reference
+    // (src/deformation.jl:47-50)
+    if (cfg->mantle == FI_RELAXED_MANTLE && cfg->lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE)
+        return FI_ERR_UNSUPPORTED;
     if (cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE)
         return FI_ERR_UNSUPPORTED;
     if (cfg->mantle == FI_RIGID_MANTLE) return FI_ERR_UNSUPPORTED;  // dudt = 0: nothing to accelerate
     fi_ctx* c = new fi_ctx();
     c->cfg = *cfg;
+    c->elra = cfg->mantle == FI_RELAXED_MANTLE;
     c->nx = cfg->nx;
     c->ny = cfg->ny;
     c->N = (size_t)c->nx * c->ny;
@@ -710,6 +762,13 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_MASK_ACTIVE: c->mask = v; return FI_OK;
         case FI_GREEN_ELASTIC: c->ge = v; c->has_ge = true; return FI_OK;
         case FI_GREEN_GEOID: c->gg = v; c->has_gg = true; return FI_OK;
+        case FI_GREEN_VISCOUS: c->gv = v; c->has_gv = true; return FI_OK;
+        case FI_TAU:
+            for (double x : v)
+                if (!(x > 0) || !std::isfinite(x)) return fail(c, FI_ERR_INVALID, "tau must be positive");
+            c->tau = v;
+            c->has_tau = true;
+            return FI_OK;
         case FI_REF_U: c->u_ref = v; return FI_OK;
         case FI_REF_UE: c->ue_ref = v; return FI_OK;
         case FI_REF_DZ_SS: c->dz_ss_ref = v; return FI_OK;
@@ -802,8 +861,9 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     const fi_config& g = c->cfg;
     const size_t N = c->N;
     (void)hipSetDevice(c->dev);
-    if (!c->has_D || !c->has_eta || !c->has_spinv)
+    if (!c->elra && (!c->has_D || !c->has_eta || !c->has_spinv))
         return fail(c, FI_ERR_INVALID, "litho rigidity, effective viscosity and scaled_pseudodiff_inv required");
+    if (c->elra && (!c->has_gv || !c->has_tau)) return fail(c, FI_ERR_INVALID, "viscous Green kernel and tau required");
     if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge) return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
         return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
@@ -844,19 +904,28 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
 
     // spectral multiplier spy/(nx ny) * even part of scaled_pseudodiff_inv, transposed [kx][ky];
     // the DC term is zeroed: the corner BC on dudt removes any constant exactly
-    std::vector<float> mult((size_t)c->nhx * ny);
-    const double sc = g.seconds_per_year / ((double)nx * ny);
-    for (int kx = 0; kx < c->nhx; ++kx)
-        for (int ky = 0; ky < ny; ++ky) {
-            int mkx = (nx - kx) % nx, mky = (ny - ky) % ny;
-            double m = 0.5 * (c->spinv[kx + (size_t)ky * nx] + c->spinv[mkx + (size_t)mky * nx]);
-            mult[(size_t)kx * ny + ky] = (kx == 0 && ky == 0) ? 0.0f : (float)(m * sc);
-        }
-    if (upload(c, c->mult, mult)) return FI_ERR_DEVICE;
-    // rhs K / (2 eta) (src/deformation.jl:95)
-    std::vector<float> inv2eta(N);
-    for (size_t p = 0; p < N; ++p) inv2eta[p] = (float)((c->has_dist ? c->dist[p] : 1.0) * (1.0 / (2.0 * c->eta[p])));
-    if (upload(c, c->inv2eta, inv2eta) || upload(c, c->Df, to_f(c->D))) return FI_ERR_DEVICE;
+    if (!c->elra) {
+        std::vector<float> mult((size_t)c->nhx * ny);
+        const double sc = g.seconds_per_year / ((double)nx * ny);
+        for (int kx = 0; kx < c->nhx; ++kx)
+            for (int ky = 0; ky < ny; ++ky) {
+                int mkx = (nx - kx) % nx, mky = (ny - ky) % ny;
+                double m = 0.5 * (c->spinv[kx + (size_t)ky * nx] + c->spinv[mkx + (size_t)mky * nx]);
+                mult[(size_t)kx * ny + ky] = (kx == 0 && ky == 0) ? 0.0f : (float)(m * sc);
+            }
+        if (upload(c, c->mult, mult)) return FI_ERR_DEVICE;
+        // rhs K / (2 eta) (src/deformation.jl:95)
+        std::vector<float> inv2eta(N);
+        for (size_t p = 0; p < N; ++p)
+            inv2eta[p] = (float)((c->has_dist ? c->dist[p] : 1.0) * (1.0 / (2.0 * c->eta[p])));
+        if (upload(c, c->inv2eta, inv2eta) || upload(c, c->Df, to_f(c->D))) return FI_ERR_DEVICE;
+    } else {
+        // ELRA: 1/tau per cell and the equilibrium displacement, u_ref until the first RHS
+        // (CurrentState, src/state.jl:87)
+        std::vector<float> itau(N);
+        for (size_t p = 0; p < N; ++p) itau[p] = (float)(1.0 / c->tau[p]);
+        if (upload(c, c->inv_tau_d, itau) || upload(c, c->ueq, to_f(c->u_ref))) return FI_ERR_DEVICE;
+    }
     if (c->has_dist) {
         std::vector<float> kinv(N), k2(N);
         for (size_t p = 0; p < N; ++p) {
@@ -946,14 +1015,16 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     // Green spectra in FP64
     HIPCHECK(c, c->ge_hat.alloc((size_t)c->nfhx * c->nfy));
     HIPCHECK(c, c->gg_hat.alloc((size_t)c->nfhx * c->nfy));
+    HIPCHECK(c, c->gv_hat.alloc(c->elra ? (size_t)c->nfhx * c->nfy : 1));
     HIPCHECK(c, c->gtmp.alloc((size_t)c->nfhx * ny));
     {
         DBuf<double> gd;
         const double nrm = 1.0 / ((double)c->nfx * c->nfy);
-        for (int q = 0; q < 2; ++q) {
-            const std::vector<double>* G = q ? &c->gg : &c->ge;
-            bool have = q ? c->has_gg : c->has_ge;
-            DBuf<double2>& out = q ? c->gg_hat : c->ge_hat;
+        for (int q = 0; q < 3; ++q) {
+            if (q == 2 && !c->elra) break;
+            const std::vector<double>* G = q == 2 ? &c->gv : q ? &c->gg : &c->ge;
+            bool have = q == 2 ? c->has_gv : q ? c->has_gg : c->has_ge;
+            DBuf<double2>& out = q == 2 ? c->gv_hat : q ? c->gg_hat : c->ge_hat;
             if (!have) {
                 HIPCHECK(c, hipMemset(out.p, 0, out.n * sizeof(double2)));
                 continue;
@@ -966,7 +1037,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
                                c->tw64.p, c->gtmp.p);
             HIPCHECK(c, hipGetLastError());
             // the sea-surface input is the mass anomaly divided by the cell area: A goes in the kernel
-            double scale = nrm * (q ? c->A : 1.0);
+            double scale = nrm * (q == 1 ? c->A : 1.0);
             hipLaunchKernelGGL(k_green_cols, dim3(c->nfhx), dim3(NT), l2, c->stream, c->gtmp.p, ny, c->nfy, pfy64,
                                c->tw64.p, scale, out.p);
             HIPCHECK(c, hipGetLastError());
@@ -1075,6 +1146,10 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.ext_corners = c->ext_corners.p;
     kp.ge_hat = c->ge_hat.p;
     kp.gg_hat = c->gg_hat.p;
+    kp.elra = c->elra ? 1 : 0;
+    kp.gv_hat = c->gv_hat.p;
+    kp.inv_tau = c->inv_tau_d.p;
+    kp.ueq = c->ueq.p;
     kp.px = px;
     kp.py = py;
     kp.pfx = pfx;
@@ -1094,16 +1169,17 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     c->lds_d = c->has_dist ? c->lds_dist : twb + std::max(rows_d, other);
     c->lds_f = twb + std::max(rows_f, other);
     c->lds = c->mt_double ? c->lds_d : c->lds_f;
+    if (c->elra) c->lds = twb + other;   // ELRA runs only the padded-grid phases
     if (c->lds > 160 * 1024 || nx > 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
-    if (c->has_dist && (!c->mt_double || ny < 6 || nx < 3))
+    if (!c->elra && c->has_dist && (!c->mt_double || ny < 6 || nx < 3))
         return fail(c, FI_ERR_UNSUPPORTED, "map distortion needs the FP64-moment rows phase (3 <= nx <= 512, ny >= 6)");
-    const void* f = persistent_kernel(c->mt_double, c->has_dist);
+    const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra);
     HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
     int per_cu = 0, ncu = 0;
     HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, c->lds));
     HIPCHECK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
     int maxg = per_cu * ncu;
-    int want = std::max(kp.nrowitems, kp.ncolitems);
+    int want = std::max(kp.nrowitems, c->elra ? (c->nfhx + 1) / 2 : kp.ncolitems);
     c->G = std::max(1, std::min(maxg, want));
     if (maxg < 1) return fail(c, FI_ERR_UNSUPPORTED, "persistent kernel does not fit on the device");
     // one error-norm and one volume partial per row pair, whatever the number of workgroups
@@ -1157,6 +1233,8 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_MASK_ACTIVE: return put(c->mask);
         case FI_GREEN_ELASTIC: return put(c->ge);
         case FI_GREEN_GEOID: return put(c->gg);
+        case FI_GREEN_VISCOUS: return put(c->gv);
+        case FI_TAU: return put(c->tau);
         case FI_REF_U: return put(c->u_ref);
         case FI_REF_UE: return put(c->ue_ref);
         case FI_REF_DZ_SS: return put(c->dz_ss_ref);
@@ -1186,6 +1264,10 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_DZ_SS: if ((st = read_f(c, c->dz_ss.p, v))) return st; return put(v);
         case FI_Z_SS: if ((st = read_f(c, c->z_ss.p, v))) return st; return put(v);
         case FI_H_WATER: if ((st = read_f(c, c->H_water.p, v))) return st; return put(v);
+        case FI_U_EQ:
+            if (!c->elra) return put(c->u_ref);   // written only by the relaxed mantle's RHS
+            if ((st = read_f(c, c->ueq.p, v))) return st;
+            return put(v);
         case FI_CANOM_SEAWATER: if ((st = read_f(c, c->LW.p, v))) return st; return put(v);
         case FI_MASK_GROUNDED:
         case FI_MASK_OCEAN: {
@@ -1463,6 +1545,7 @@ int ck_walk(fi_ctx* c, int mode, unsigned char* blob, size_t& off) {
         (st = ck_dev(c, mode, c->mo, N, blob, off)) || (st = ck_dev(c, mode, c->sig_load_prev, N, blob, off)) ||
         (st = ck_dev(c, mode, c->sig_geo_prev, N, blob, off)) || (st = ck_dev(c, mode, c->H_ext, N, blob, off)))
         return st;
+    if (c->elra && (st = ck_dev(c, mode, c->ueq, N, blob, off))) return st;
     if (c->ice_mode == ICE_EXTERNAL) {
         if (mode == 2) c->H_ext_host.assign(N, 0.0);
         for (size_t p = 0; p < N; ++p) ck_host(mode, c->H_ext_host[p], blob, off);
@@ -1664,6 +1747,10 @@ int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out) {
         if (cfgs[m].device != b->dev) {
             fi_batch_destroy(b);
             return FI_ERR_INVALID;  // one device per ensemble
+        }
+        if (cfgs[m].mantle == FI_RELAXED_MANTLE) {
+            fi_batch_destroy(b);
+            return FI_ERR_UNSUPPORTED;  // the ensemble kernel runs the LV-ELVA RHS only
         }
         fi_ctx* c = nullptr;
         const int st = fi_create(&cfgs[m], &c);

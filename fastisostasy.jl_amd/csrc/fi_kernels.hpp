@@ -1414,6 +1414,266 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
     }
 }
 
+// the sparse update of an RHS call (src/simulation.jl:355-381): elastic and sea-surface
+// convolutions, volumes and barystatic sea level, pointwise diagnostics; false when the call stops.
+// With SKIP_EMPTY a rigid lithosphere under a laterally constant sea surface (both convolution
+// inputs unused) skips the convolution phases.
+template <bool SKIP_EMPTY>
+__device__ __forceinline__ bool sparse_phases(const KP& p, const Team& tm, Ctl& s, const StageDesc& st, unsigned& gen,
+                                              char* smem, double* red, int* flag, unsigned long long& tl,
+                                              double uoff, const IceT it) {
+    const bool conv = !SKIP_EMPTY || !p.rigid_litho || p.lv_surface;
+    if (conv) {
+        phase_sp_rows(p, tm, smem);
+        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+        phase_sp_cols(p, tm, smem);
+        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+        phase_sp_c2r(p, tm, smem);
+    }
+    // the volume partials read only the state before the update: they follow the last
+    // convolution phase without a barrier of their own (the BSL update needs all of them)
+    phase_sp_volumes(p, tm, st, uoff, it, red);
+    if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+    volumes_and_bsl(p, s, st.t, red);
+    if (s.err) return false;   // every workgroup sees the same error
+    phase_sp_point(p, tm, st, uoff, it, s.z_bsl);
+    return grid_sync(p, tm, gen, flag, TAG_SPARSE, tl);
+}
+
+// ------------------------------------------------------------------------------------------
+// relaxed mantle (ELRA, src/deformation.jl:29-45): u_eq = samesize_conv(-(sigma_load +
+// sigma_litho) g K^2, green_viscous) in the regular BC space (crop to the domain, then the corner
+// BC; src/convolutions.jl:145-154) and dudt = (u_eq - u) / tau, on the zero-padded nfx x nfy grid:
+//   elra_rows : stage displacement (BS3 combination, corner BC) and loads of a row pair, the two
+//               load rows packed as one complex sequence, length-nfx FFT, half spectra stored
+//               transposed [kx][j] in pA;
+//   elra_cols : length-nfy FFTs of two spectral columns, multiplication by the FP64 Green
+//               spectrum, inverse FFTs; the ny rows of the crop [j1:j2] are kept in oA [kx][j];
+//   elra_c2r  : the corner-BC offset of the cropped u_eq by four direct sums over the spectral
+//               rows 0 and ny-1 (each workgroup computes it), inverse real transforms of row
+//               pairs, crop [i1:i2], u_eq and dudt into the K slot (no BC on dudt, so no corner
+//               offset is left for consumers); BS3 error moments on the last stage.
+// update_deformation_rhs!, which the reference calls first in this method, only fills scratch
+// buffers that the method does not read afterwards.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                                const IceT it, char* smem) {
+    const int nx = p.nx, ny = p.ny, nfx = p.nfx;
+    float2* tw = (float2*)smem;
+    float2* sf = tw + nfx;
+    float2* sg = sf + nfx;
+    stage_twiddles(p, p.pfx, tw);
+    const bool wsig = st.write_sigma != 0;
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+        const int j0 = 2 * item;
+        for (int i = threadIdx.x; i < nfx; i += NT) {
+            float2 z = make_float2(0.f, 0.f);
+            if (i < nx) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int j = j0 + r;
+                    if (j >= ny) continue;
+                    const long c = i + (long)j * nx;
+                    const double u = u_raw(p, st, c) - uoff;
+                    if (st.write_u >= 0) st_shared(&ubuf(p, st.write_u)[c], u);
+                    const double m = p.mask[c] ? 1.0 : 0.0;
+                    const double zo = st.trig ? p.zoff_trig[c] : p.zoff[c];
+                    const double Haf = hab(ice_at(p, it, c), zo + u, p.rho_sw_ice);
+                    const double s_ice = p.rho_ice * (Haf - (double)p.H_af_ref[c]);
+                    const double s_load = m * (s_ice + (p.interactive ? (double)p.LW[c] : 0.0));
+                    const double lt = p.LT[c];   // sigma_litho - rho_um u_ref
+                    const double uref = p.u_ref[c];
+                    const double k2 = p.K2 ? (double)p.K2[c] : 1.0;
+                    const float x = (float)(-(s_load + (lt + p.rho_um * uref)) * p.g * k2);
+                    if (r == 0) z.x = x;
+                    else z.y = x;
+                    if (wsig) {
+                        const double s_mant = p.rho_um * u;
+                        const double s_full = s_load + m * (lt + s_mant);
+                        st_shared(&p.sig_load_prev[c], (float)s_load);
+                        st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
+                                                        ? s_load + m * (s_mant - p.rho_um * uref)
+                                                        : s_full));
+                    }
+                }
+            }
+            sf[i] = z;
+        }
+        __syncthreads();
+        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, -1);
+        for (int kx = threadIdx.x; kx < p.nfhx; kx += NT) {
+            const float2 Z = res[kx], Zm = res[kx == 0 ? 0 : nfx - kx];
+            float2 A, B;
+            A.x = 0.5f * (Z.x + Zm.x);
+            A.y = 0.5f * (Z.y - Zm.y);
+            B.x = 0.5f * (Z.y + Zm.y);
+            B.y = -0.5f * (Z.x - Zm.x);
+            st_shared(&p.pA[(long)kx * ny + j0], A);
+            if (j0 + 1 < ny) st_shared(&p.pA[(long)kx * ny + j0 + 1], B);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, char* smem) {
+    const int ny = p.ny, nfy = p.nfy, nfhx = p.nfhx;
+    float2* tw = (float2*)smem;
+    float2* sa = tw + nfy;
+    float2* sb = sa + 2 * nfy;
+    stage_twiddles(p, p.pfy, tw);
+    const int nitems = (nfhx + 1) / 2;
+    for (int item = tm.rank; item < nitems; item += tm.size) {
+        const int kx0 = 2 * item, nb = min(2, nfhx - kx0);
+        for (int idx = threadIdx.x; idx < nb * nfy; idx += NT) {
+            const int s = idx >= nfy ? 1 : 0, j = idx - s * nfy;
+            float2 v = make_float2(0.f, 0.f);
+            if (j < ny) v = p.pA[(long)(kx0 + s) * ny + j];
+            sa[idx] = v;
+        }
+        __syncthreads();
+        float2* r = fft_lds<float2>(sa, sb, nb, nfy, p.pfy, tw, -1);
+        for (int idx = threadIdx.x; idx < nb * nfy; idx += NT) {
+            const int s = idx >= nfy ? 1 : 0, ky = idx - s * nfy;
+            const double2 G = p.gv_hat[(long)(kx0 + s) * nfy + ky];
+            const double xr = r[idx].x, xi = r[idx].y;
+            r[idx].x = (float)(xr * G.x - xi * G.y);
+            r[idx].y = (float)(xr * G.y + xi * G.x);
+        }
+        __syncthreads();
+        float2* o = (r == sa) ? sb : sa;
+        float2* r2 = fft_lds<float2>(r, o, nb, nfy, p.pfy, tw, +1);
+        for (int idx = threadIdx.x; idx < nb * ny; idx += NT) {
+            const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
+            st_shared(&p.oA[(long)(kx0 + s) * ny + j], r2[s * nfy + (p.j1 - 1) + j]);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                               char* smem, double* red) {
+    const int nx = p.nx, ny = p.ny, nfx = p.nfx, nfhx = p.nfhx;
+    const bool even = (nfx % 2) == 0;
+    float2* tw = (float2*)smem;
+    float2* sf = tw + nfx;
+    float2* sg = sf + nfx;
+    stage_twiddles(p, p.pfx, tw);
+    // corners of the cropped u_eq before its BC: x(m, j) = sum_k w_k Re(X_k(j) e^{2 pi i k m / nfx})
+    // at m = i1-1 and i1-1+nx-1 of the kept rows j = 0 and ny-1 (w = 1 for the DC and Nyquist
+    // terms, whose imaginary parts the real transform ignores; 2 otherwise)
+    const int m0 = p.i1 - 1, m1 = p.i1 - 1 + nx - 1;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < nfhx; k += NT) {
+        float2 a = p.oA[(long)k * ny], b = p.oA[(long)k * ny + ny - 1];
+        const bool dcn = k == 0 || (even && k == nfx / 2);
+        if (dcn) {
+            a.y = 0.f;
+            b.y = 0.f;
+        }
+        const double w = dcn ? 1.0 : 2.0;
+        double s0, c0, s1, c1;
+        sincospi(2.0 * (double)(((long)k * m0) % nfx) / (double)nfx, &s0, &c0);
+        sincospi(2.0 * (double)(((long)k * m1) % nfx) / (double)nfx, &s1, &c1);
+        v[0] += w * ((double)a.x * c0 - (double)a.y * s0);
+        v[1] += w * ((double)a.x * c1 - (double)a.y * s1);
+        v[2] += w * ((double)b.x * c0 - (double)b.y * s0);
+        v[3] += w * ((double)b.x * c1 - (double)b.y * s1);
+    }
+    block_sum<4>(red, v);
+    const double off = ((0.25 * v[0] + 0.25 * v[1]) + (0.25 * v[2] + 0.25 * v[3])) - p.vbc_x;
+    float* K = p.Kb + (long)st.kout * p.Nl;
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+        const int j0 = 2 * item;
+        const bool two = j0 + 1 < ny;
+        for (int k = threadIdx.x; k < nfx; k += NT) {
+            const bool cj = k >= nfhx;
+            const int kk = cj ? nfx - k : k;
+            float2 A = p.oA[(long)kk * ny + j0];
+            float2 B = two ? p.oA[(long)kk * ny + j0 + 1] : make_float2(0.f, 0.f);
+            if (k == 0 || (even && k == nfx / 2)) {
+                A.y = 0.f;
+                B.y = 0.f;
+            }
+            if (cj) {
+                A.y = -A.y;
+                B.y = -B.y;
+            }
+            sf[k] = make_float2(A.x - B.y, A.y + B.x);
+        }
+        __syncthreads();
+        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, +1);
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
+            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+            if (j >= ny) continue;
+            const long c = i + (long)j * nx;
+            const float2 z = res[m0 + i];
+            const double ueq = (double)(r ? z.y : z.x) - off;
+            // the stage displacement: stored by the rows phase when this call keeps it
+            const double u = st.write_u >= 0 ? ubuf(p, st.write_u)[c] : u_raw(p, st, c) - uoff;
+            const float dv = (float)((ueq - u) * (double)p.inv_tau[c]);
+            st_shared(&K[c], dv);
+            st_shared(&p.ueq[c], (float)ueq);
+            if (st.final_err) {
+                const double k1 = kslot(p, st.e1)[c], k2 = kslot(p, st.e2)[c], k3 = kslot(p, st.e3)[c];
+                const double un = ubuf(p, st.u_prev)[c];
+                const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)dv);
+                const double wgt = 1.0 / (p.abstol + fmax(fabs(un), fabs(u)) * p.reltol);
+                acc[0] += (a * wgt) * (a * wgt);
+                acc[1] += a * wgt * wgt;
+                acc[2] += wgt * wgt;
+            }
+        }
+        if (st.final_err) {
+            block_sum<3>(red, acc);
+            if (threadIdx.x == 0) {
+                st_shared(&p.part[4 * item + 0], acc[0]);
+                st_shared(&p.part[4 * item + 1], acc[1]);
+                st_shared(&p.part[4 * item + 2], acc[2]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// one ELRA RHS call: preamble, the sparse update when due, then the three convolution phases
+__device__ __forceinline__ bool elra_call(const KP& p, const Team& tm, Ctl& s, StageDesc& st, unsigned& gen,
+                                          const float2* twx, char* smem, double* red, int* flag, bool sparse_only,
+                                          unsigned long long& tl) {
+    KC4 kc;
+    const bool trig = sparse_only || (((st.t - s.t0) / p.dt_sparse) >= (double)s.count);
+    st.trig = trig ? 1 : 0;
+    st.kin = -1;   // every derivative term is held in global memory
+    IceT it;
+    double uoff;
+    stage_preamble(p, st, specbuf(p, 0), twx, red, it, kc, uoff);
+    if (it.err) {
+        if (threadIdx.x == 0) s.err = ERR_TIME_RANGE;
+        __syncthreads();
+        return false;
+    }
+    if (trig) {
+        if (!sparse_phases<true>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
+        if (sparse_only) return true;
+    }
+    st.write_sigma = (st.force_sigma || p.write_sigma_always ||
+                      (((st.t_next_max - s.t0) / p.dt_sparse) >= (double)s.count)) ? 1 : 0;
+    phase_elra_rows(p, tm, st, uoff, it, smem);
+    if (!grid_sync(p, tm, gen, flag, TAG_ROWS, tl)) return false;
+    phase_elra_cols(p, tm, smem);
+    if (!grid_sync(p, tm, gen, flag, TAG_COLS, tl)) return false;
+    phase_elra_c2r(p, tm, st, uoff, smem, red);
+    if (!grid_sync(p, tm, gen, flag, TAG_C2R, tl)) return false;
+    if (threadIdx.x == 0) {
+        s.nrhs += 1;
+        s.t_last = st.t;
+        s.last_k = st.kout;
+        s.last_trig = trig ? 1 : 0;
+    }
+    __syncthreads();
+    return true;
+}
+
 // ------------------------------------------------------------------------------------------
 // one RHS call
 // ------------------------------------------------------------------------------------------
@@ -1449,19 +1709,7 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
         return false;
     }
     if (trig) {
-        phase_sp_rows(p, tm, smem);
-        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
-        phase_sp_cols(p, tm, smem);
-        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
-        phase_sp_c2r(p, tm, smem);
-        // the volume partials read only the state before the update: they follow the last
-        // convolution phase without a barrier of their own (the BSL update needs all of them)
-        phase_sp_volumes(p, tm, st, uoff, it, red);
-        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
-        volumes_and_bsl(p, s, st.t, red);
-        if (s.err) return false;   // every workgroup sees the same error
-        phase_sp_point(p, tm, st, uoff, it, s.z_bsl);
-        if (!grid_sync(p, tm, gen, flag, TAG_SPARSE, tl)) return false;
+        if (!sparse_phases<false>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
         if (sparse_only) return true;
     }
     // store this call's loads if the next call may trigger (the sparse update reads the

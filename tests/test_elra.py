@@ -172,11 +172,19 @@ def test_gpu_elra_integrate_vs_oracle(lithosphere, interactive):
     ref = pyref.RefEngine.from_inputs(sim.inputs)
     sim.engine.integrator_init()
     ref.integrator_init()
-    for t in (300.0, 1000.0, 3000.0):
+    # Tolerances: the oracle against itself at reltol 1e-5 and 0.7e-5 differs by 2.4e-4, 1.9e-5
+    # and 3e-7 (relative) at these times; the relaxation damps the step-sequence differences.
+    errs = []
+    for t, tol in ((300.0, 5e-4), (1000.0, 1e-4), (3000.0, 2e-5)):
         sim.engine.integrate(t)
         ref.integrate(t)
         a, b = sim.engine.get_field("u"), ref.get_field("u")
-        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), t
+        errs.append((t, np.abs(a - b).max() / np.abs(b).max(), tol))
+    print("relative u error vs oracle:", errs)
+    for t, e, tol in errs:
+        assert e <= tol, (t, e)
+    sg, sr = sim.engine.stats(), ref.stats()
+    assert abs(sg["n_accept"] - sr["n_accept"]) <= 0.05 * sr["n_accept"] + 3
     if not interactive:
         tau = sim.inputs["fields"]["tau"]
         ref2 = pyref.RefEngine.from_inputs(sim.inputs)
