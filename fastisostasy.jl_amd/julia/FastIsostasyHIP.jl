@@ -42,8 +42,9 @@ end
 
 const FI = (litho_rigidity = 1, effective_viscosity = 2, scaled_pseudodiff_inv = 3, maskactive = 4,
             green_elastic = 5, green_geoid = 6, ref_u = 7, ref_ue = 8, ref_dz_ss = 9, ref_z_b = 10,
-            ref_H_ice = 11, u = 32, dudt = 33, ue = 34, dz_ss = 35, z_ss = 36, z_b = 37, H_ice = 38,
-            H_af = 39, H_water = 40, maskgrounded = 41, maskocean = 42)
+            ref_H_ice = 11, distortion = 12, green_viscous = 13, tau = 14, u = 32, dudt = 33, ue = 34,
+            dz_ss = 35, z_ss = 36, z_b = 37, H_ice = 38, H_af = 39, H_water = 40, maskgrounded = 41,
+            maskocean = 42, u_eq = 49)
 
 check(ctx, st) = st == 0 || error("fastiso HIP engine error $st: " *
     unsafe_string(ccall((:fi_last_error, libfi), Cstring, (Ptr{Cvoid},), ctx)))
@@ -71,6 +72,9 @@ function HipSimulation(sim)
     cfg.rho_water, cfg.rho_seawater, cfg.r_pole, cfg.mE = c.rho_water, c.rho_seawater, c.r_pole, c.mE
     cfg.litho_poissonratio, cfg.rho_uppermantle, cfg.rho_litho =
         se.litho_poissonratio, se.rho_uppermantle, se.rho_litho
+    cfg.lithosphere = se.lithosphere isa RigidLithosphere ? 0 :
+        se.lithosphere isa LaterallyConstantLithosphere ? 1 : 2
+    cfg.mantle = se.mantle isa RigidMantle ? 0 : se.mantle isa RelaxedMantle ? 1 : 2
     cfg.sea_surface = sl.surface isa LaterallyVariableSeaSurface ? 1 : 0
     cfg.sealevel_load = sl.load isa InteractiveSealevelLoad ? 1 : 0
     cfg.lithosphere_column = se.lithosphere_column isa CompressibleLithosphereColumn ? 1 : 0
@@ -90,6 +94,11 @@ function HipSimulation(sim)
     setfield!(ctx, FI.maskactive, se.maskactive)
     setfield!(ctx, FI.green_elastic, sim.tools.elastic_convo.kernel)
     setfield!(ctx, FI.green_geoid, sim.tools.dz_ss_convo.kernel)
+    if se.mantle isa RelaxedMantle   # ELRA: the viscous Green kernel of GIATools and tau
+        setfield!(ctx, FI.green_viscous, sim.tools.viscous_convo.kernel)
+        setfield!(ctx, FI.tau, se.tau .* ones(d.nx, d.ny))
+    end
+    d.correct_distortion && setfield!(ctx, FI.distortion, d.K)
     setfield!(ctx, FI.ref_u, sim.ref.u)
     setfield!(ctx, FI.ref_ue, sim.ref.ue)
     setfield!(ctx, FI.ref_z_b, sim.ref.z_b)
