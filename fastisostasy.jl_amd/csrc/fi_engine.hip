@@ -391,6 +391,23 @@ __global__ void k_dudt_out(const float* __restrict__ k, double off, long n, doub
 __global__ void k_sub_offset(double* u, double off, long n) {
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) u[c] -= off;
 }
+// thinplate_horizontal_displacement!(u_x, u_y, u + ue, h, domain) (src/deformation.jl:141-164):
+// -h/2 times the first derivative along x (axis 0) or y (axis 1) with the CPU edge rules
+// (src/derivatives.jl:171-191) and the per-cell step K dx or K dy. A constant offset of u (its
+// pending corner BC) does not change the derivative.
+__global__ void k_thinplate(const double* __restrict__ u, const float* __restrict__ ue, const float* __restrict__ h,
+                            const float* __restrict__ kinv, int nx, int ny, double step, int axis, double* out) {
+    const long N = (long)nx * ny;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (long)gridDim.x * blockDim.x) {
+        const long j = c / nx, i = c - j * nx;
+        const long stride = axis == 0 ? 1 : nx;
+        const long pos = axis == 0 ? i : j, len = axis == 0 ? nx : ny;
+        const long a = pos == 0 ? c : c - stride, b = pos == len - 1 ? c : c + stride;
+        const double span = (pos == 0 || pos == len - 1) ? 1.0 : 2.0;
+        const double d = ((u[b] + (double)ue[b]) - (u[a] + (double)ue[a])) / (span * step);
+        out[c] = -(double)h[c] / 2 * (kinv ? d * (double)kinv[c] : d);
+    }
+}
 
 }  // namespace fi
 
@@ -452,7 +469,7 @@ struct fi_ctx {
     double dx = 0, dy = 0, A = 0;
     int i1 = 0, i2 = 0, j1 = 0, j2 = 0;
     // host copies of the static inputs
-    std::vector<double> D, eta, spinv, mask, ge, gg, gv, tau, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ref_in;
+    std::vector<double> D, eta, spinv, mask, ge, gg, gv, tau, litho_h, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ref_in;
     bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_gv = false,
          has_tau = false, has_href = false;
     bool elra = false;   // RelaxedMantle (ELRA) instead of the LV-ELVA Maxwell mantle
@@ -476,7 +493,8 @@ struct fi_ctx {
     DBuf<unsigned char> maskd, mg, mo;
     DBuf<float2> spec, pA, pB, oA, oB, tw32;
     DBuf<double2> ge_hat, gg_hat, gv_hat, tw64, gtmp;
-    DBuf<float> inv_tau_d, ueq;
+    DBuf<float> inv_tau_d, ueq, litho_h_d;
+    DBuf<double> uxy_d;   // horizontal displacement output (FI_U_X / FI_U_Y)
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
@@ -763,6 +781,7 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_GREEN_ELASTIC: c->ge = v; c->has_ge = true; return FI_OK;
         case FI_GREEN_GEOID: c->gg = v; c->has_gg = true; return FI_OK;
         case FI_GREEN_VISCOUS: c->gv = v; c->has_gv = true; return FI_OK;
+        case FI_LITHO_THICKNESS: c->litho_h = v; c->litho_h_d.n = 0; return FI_OK;
         case FI_TAU:
             for (double x : v)
                 if (!(x > 0) || !std::isfinite(x)) return fail(c, FI_ERR_INVALID, "tau must be positive");
@@ -1235,6 +1254,7 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_GREEN_GEOID: return put(c->gg);
         case FI_GREEN_VISCOUS: return put(c->gv);
         case FI_TAU: return put(c->tau);
+        case FI_LITHO_THICKNESS: return put(c->litho_h);
         case FI_REF_U: return put(c->u_ref);
         case FI_REF_UE: return put(c->ue_ref);
         case FI_REF_DZ_SS: return put(c->dz_ss_ref);
@@ -1264,6 +1284,21 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_DZ_SS: if ((st = read_f(c, c->dz_ss.p, v))) return st; return put(v);
         case FI_Z_SS: if ((st = read_f(c, c->z_ss.p, v))) return st; return put(v);
         case FI_H_WATER: if ((st = read_f(c, c->H_water.p, v))) return st; return put(v);
+        case FI_U_X:
+        case FI_U_Y: {
+            if (c->litho_h.size() != N) return fail(c, FI_ERR_STATE, "fi_get_field: litho thickness not set");
+            if (c->litho_h_d.n != N && upload(c, c->litho_h_d, to_f(c->litho_h))) return FI_ERR_DEVICE;
+            if (c->uxy_d.n != N) HIPCHECK(c, c->uxy_d.alloc(N));
+            const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
+            const bool ax = f == FI_U_X;
+            hipLaunchKernelGGL(k_thinplate, dim3(nb), dim3(256), 0, c->stream, c->U.p + (size_t)c->ctl.last_u * N,
+                               c->ue.p, c->litho_h_d.p, c->has_dist ? c->kinv_d.p : nullptr, c->nx, c->ny,
+                               ax ? c->dx : c->dy, ax ? 0 : 1, c->uxy_d.p);
+            HIPCHECK(c, hipGetLastError());
+            HIPCHECK(c, hipMemcpyAsync(h, c->uxy_d.p, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+            return FI_OK;
+        }
         case FI_U_EQ:
             if (!c->elra) return put(c->u_ref);   // written only by the relaxed mantle's RHS
             if ((st = read_f(c, c->ueq.p, v))) return st;

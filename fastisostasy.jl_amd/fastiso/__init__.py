@@ -22,6 +22,7 @@ from .greens import elastic_green, gauss_legendre, geoid_green, green_viscous, l
 from .simulation import (DiffEqOptions, Integrator, NativeOutput, Simulation, SolverOptions, init_integrator,
                          push_inputs, run, step, update_diagnostics)
 from .utils import uniform_ice_cylinder
+from .io import AsymetricOutputCrop, NetcdfOutput, PaddedOutputCrop, crop, read_nc
 from .engine import Engine, EngineError
 
 # Julia-style aliases of the mutating entry points

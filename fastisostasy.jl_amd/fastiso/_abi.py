@@ -34,10 +34,10 @@ FI_NO_CONTRIBUTION, FI_GOELZER_CONTRIBUTION = 0, 1
 FIELDS = {
     "litho_rigidity": 1, "effective_viscosity": 2, "scaled_pseudodiff_inv": 3, "maskactive": 4,
     "green_elastic": 5, "green_geoid": 6, "ref_u": 7, "ref_ue": 8, "ref_dz_ss": 9, "ref_z_b": 10,
-    "ref_H_ice": 11, "distortion": 12, "green_viscous": 13, "tau": 14,
+    "ref_H_ice": 11, "distortion": 12, "green_viscous": 13, "tau": 14, "litho_thickness": 15,
     "u": 32, "dudt": 33, "ue": 34, "dz_ss": 35, "z_ss": 36, "z_b": 37, "H_ice": 38, "H_af": 39,
     "H_water": 40, "maskgrounded": 41, "maskocean": 42, "canom_ice": 43, "canom_seawater": 44,
-    "canom_litho": 45, "canom_mantle": 46, "canom_load": 47, "canom_full": 48, "u_eq": 49,
+    "canom_litho": 45, "canom_mantle": 46, "canom_load": 47, "canom_full": 48, "u_eq": 49, "u_x": 50, "u_y": 51,
 }
 
 

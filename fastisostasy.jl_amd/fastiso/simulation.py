@@ -95,12 +95,14 @@ class Simulation:
     (config, fields, forcing), which is what the CPU-side tests inspect."""
 
     def __init__(self, domain, bcs, sealevel, solidearth, t_span, opts=None, u_ref=None, ue_ref=None,
-                 dz_ss_ref=None, z_b_ref=None, nout=None, c=None, engine="hip", device=0, quad_precision=4):
+                 dz_ss_ref=None, z_b_ref=None, nout=None, ncout=None, c=None, engine="hip", device=0,
+                 quad_precision=4):
         self.domain, self.bcs, self.sealevel, self.solidearth = domain, bcs, sealevel, solidearth
         self.t_span = (float(t_span[0]), float(t_span[1]))
         self.opts = opts or SolverOptions()
         self.c = c or PhysicalConstants()
         self.nout = nout or NativeOutput()
+        self.ncout = ncout
         if isinstance(solidearth.mantle, MaxwellMantle) and not hasattr(solidearth, "litho_rigidity"):
             raise ValueError("solidearth incomplete")
         cfg = build_config(domain, self.c, solidearth, sealevel, bcs, self.opts, device)
@@ -109,6 +111,7 @@ class Simulation:
             "effective_viscosity": solidearth.effective_viscosity,
             "scaled_pseudodiff_inv": solidearth.scaled_pseudodiff_inv,
             "maskactive": solidearth.maskactive.astype(np.float64),
+            "litho_thickness": solidearth.litho_thickness,
         }
         if np.any(domain.K != 1.0):
             fields["distortion"] = domain.K
@@ -135,7 +138,8 @@ class Simulation:
         else:
             raise TypeError("unknown ice thickness forcing")
         self.inputs = {"cfg": cfg, "fields": fields, "ice": ice, "t_span": self.t_span,
-                       "tout": sorted(self.nout.t), "bsl": bsl_inputs(sealevel.bsl, sealevel.update_bsl)}
+                       "tout": sorted(set(self.nout.t) | set(self.ncout.t if self.ncout else ())),
+                       "bsl": bsl_inputs(sealevel.bsl, sealevel.update_bsl)}
         self.engine = None
         if engine == "hip":
             self.engine = Engine(cfg)
@@ -186,17 +190,26 @@ def update_diagnostics(dudt, u, sim, t):
 
 
 def _output_callback(sim):
-    nout = sim.nout
+    """nout_affect! / nc_affect! under their conditions (src/simulation.jl:190-225): the engine
+    stops at the union of both output time lists; each output takes the times it asked for."""
+    nout, ncout = sim.nout, sim.ncout
     t_start = time.perf_counter()
 
     def cb(t, k):
-        for v in nout.vars:
-            nout.vals[v].append(sim.engine.get_field(v))
-        nout.t_saved.append(t)
-        nout.k += 1
-        nout.computation_time = time.perf_counter() - t_start
-        if sim.opts.verbose:
-            print(f"Saving native output at simulation year {t}...")
+        if nout.k <= len(nout.t) and t >= nout.t[nout.k - 1]:
+            for v in nout.vars:
+                nout.vals[v].append(sim.engine.get_field(v))
+            nout.t_saved.append(t)
+            nout.k += 1
+            nout.computation_time = time.perf_counter() - t_start
+            if sim.opts.verbose:
+                print(f"Saving native output at simulation year {t}...")
+        if ncout is not None and ncout.k <= len(ncout.t) and t >= ncout.t[ncout.k - 1]:
+            if ncout.active:
+                if sim.opts.verbose:
+                    print(f"Saving nc output at index {ncout.k}, sim year {t}...")
+                ncout.write(sim.engine)
+            ncout.k += 1
     return cb
 
 

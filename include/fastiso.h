@@ -132,6 +132,8 @@ enum {
                                      55-60); required with FI_RELAXED_MANTLE                   */
     FI_TAU = 14,                  /* SolidEarth.tau, relaxation time (yr) of the relaxed mantle
                                      (src/solidearth.jl:167,189); required with FI_RELAXED_MANTLE */
+    FI_LITHO_THICKNESS = 15,      /* SolidEarth.litho_thickness (m), src/solidearth.jl:210; needed
+                                     only to read FI_U_X / FI_U_Y                              */
     /* state, readable after fi_init; FI_U and FI_H_ICE are also writable */
     FI_U = 32,                    /* CurrentState fields                src/state.jl:49-77     */
     FI_DUDT = 33,
@@ -150,9 +152,15 @@ enum {
     FI_CANOM_MANTLE = 46,
     FI_CANOM_LOAD = 47,
     FI_CANOM_FULL = 48,
-    FI_U_EQ = 49                  /* CurrentState.u_eq, equilibrium displacement of the relaxed
+    FI_U_EQ = 49,                 /* CurrentState.u_eq, equilibrium displacement of the relaxed
                                      mantle (src/state.jl:60, src/deformation.jl:37-40); the
                                      reference state u_ref until the first RHS evaluation      */
+    FI_U_X = 50,                  /* CurrentState.u_x, u_y: thin-plate horizontal displacement of
+                                     u + ue, -h/2 times its x / y derivative (CPU edge rules),
+                                     as the output callbacks compute it (src/simulation.jl:
+                                     204-207,218-221; src/deformation.jl:141-164); needs
+                                     FI_LITHO_THICKNESS                                        */
+    FI_U_Y = 51
 };
 
 /* ---- run statistics ------------------------------------------------------------------------ */

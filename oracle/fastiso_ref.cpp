@@ -328,7 +328,8 @@ struct fi_ctx {
     ConvPlan cp;
     std::vector<cd> ge_hat, gg_hat, gv_hat;
     // static inputs
-    vec D, eta, spinv, maskactive, green_e, green_g, green_v, tau, u_ref, ue_ref, dz_ss_ref, z_b_ref, H_ice_ref_in;
+    vec D, eta, spinv, maskactive, green_e, green_g, green_v, tau, litho_h, u_ref, ue_ref, dz_ss_ref, z_b_ref,
+        H_ice_ref_in;
     bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_gv = false,
          has_tau = false, has_href = false;
     // ice forcing
@@ -744,6 +745,7 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         case FI_GREEN_ELASTIC: c->green_e = v; c->has_ge = true; break;
         case FI_GREEN_GEOID: c->green_g = v; c->has_gg = true; break;
         case FI_GREEN_VISCOUS: c->green_v = v; c->has_gv = true; break;
+        case FI_LITHO_THICKNESS: c->litho_h = v; break;
         case FI_TAU:
             for (double x : v)
                 if (!(x > 0) || !std::isfinite(x)) return fail(c, FI_ERR_INVALID, "tau must be positive");
@@ -780,6 +782,18 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
 int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
     if (!c || !h || n != c->N) return fail(c, FI_ERR_INVALID, "fi_get_field: size mismatch");
     if (!c->initialized && f >= FI_U) return fail(c, FI_ERR_STATE, "fi_get_field before fi_init");
+    if (f == FI_U_X || f == FI_U_Y) {
+        // thinplate_horizontal_displacement!(u_x, u_y, u + ue, litho_thickness, domain)
+        // (src/deformation.jl:141-164) as the output callbacks call it (src/simulation.jl:218-221)
+        if (c->litho_h.size() != c->N) return fail(c, FI_ERR_STATE, "fi_get_field: litho thickness not set");
+        const vec& u = c->integ ? c->uprev : c->u;
+        vec w(c->N);
+        for (size_t p = 0; p < c->N; ++p) w[p] = u[p] + c->ue[p];
+        if (f == FI_U_X) dx1(h, w.data(), c->nx, c->ny, c->Dx.data());
+        else dy1(h, w.data(), c->nx, c->ny, c->Dy.data());
+        for (size_t p = 0; p < c->N; ++p) h[p] *= -c->litho_h[p] / 2;
+        return FI_OK;
+    }
     const vec* v = nullptr;
     const std::vector<uint8_t>* m = nullptr;
     switch (f) {
@@ -791,6 +805,7 @@ int fi_get_field(fi_ctx* c, int32_t f, double* h, size_t n) {
         case FI_GREEN_GEOID: v = &c->green_g; break;
         case FI_GREEN_VISCOUS: v = &c->green_v; break;
         case FI_TAU: v = &c->tau; break;
+        case FI_LITHO_THICKNESS: v = &c->litho_h; break;
         case FI_REF_U: v = &c->u_ref; break;
         case FI_REF_UE: v = &c->ue_ref; break;
         case FI_REF_DZ_SS: v = &c->dz_ss_ref; break;
