@@ -144,6 +144,13 @@ struct KP {
     const double2* gv_hat;
     const float* inv_tau;
     float* ueq;
+    // implicit step of a Maxwell mantle under a rigid or laterally constant lithosphere: the
+    // even parts of (a - 1) and b of u <- a u + b F on the n x n spectrum, [nhx][ny], scaled by
+    // 1/(nx ny); dt_min (yr) is the integrator step
+    int implicit;
+    const float* imp_am1;
+    const float* imp_b;
+    double dt_min;
 };
 enum { TAG_ROWS = 0, TAG_COLS = 1, TAG_C2R = 2, TAG_SPARSE = 3, TAG_KIN = 4 };
 

@@ -247,6 +247,68 @@ __global__ void __launch_bounds__(NT) fi_persistent_elra(const KP* __restrict__ 
     if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
 }
 
+// Euler with dt = dt_min for the implicit step (src/simulation.jl:244-250): each step replaces
+// U[s.cur] by one implicit step at the new time (update_diagnostics! overwrites u and leaves the
+// derivative at zero, so u = uprev + dt * 0 first). Steps land on s.stop; the loop returns after
+// the step that reaches s.next_out (the host then fires the output and the FSAL call).
+__device__ __forceinline__ void implicit_loop(const KP& p, const Team& tm, Ctl& s, unsigned& gen, const float2* twx,
+                                              const float2* twy, char* smem, double* red, int* flag,
+                                              unsigned long long& tl) {
+    long long attempts = 0;
+    for (;;) {
+        if (s.err || s.t >= s.stop || attempts >= p.max_steps) break;
+        const bool clip = s.t + p.dt_min >= s.stop;
+        const double tn = clip ? s.stop : s.t + p.dt_min;
+        StageDesc st{};
+        st.ubase = s.cur;
+        st.write_u = s.cur;
+        st.kin = -1;
+        st.kout = s.k1;
+        st.t = tn;
+        st.t_next_max = tn + p.dt_min;
+        if (!implicit_call(p, tm, s, st, gen, twx, twy, smem, red, flag, false, tl)) break;
+        ++attempts;
+        if (threadIdx.x == 0) {
+            s.t = tn;
+            s.nacc += 1;
+            s.last_u = s.cur;
+            s.steps = 1;
+        }
+        __syncthreads();
+        if (s.t >= s.next_out) break;
+    }
+}
+
+__global__ void __launch_bounds__(NT) fi_persistent_implicit(const KP* __restrict__ pp) {
+    const KP& p = *(const KP*)(const ConstKP*)(unsigned long long)pp;
+    extern __shared__ __align__(16) char smem_all[];
+    __shared__ Ctl s;
+    __shared__ double red[64];
+    __shared__ int flag;
+    const Team tm{(int)blockIdx.x, (int)gridDim.x, p.bar};
+    unsigned gen = 0;
+    unsigned long long tl = wall_clock64();
+    float2* twx = (float2*)smem_all;
+    float2* twy = twx + p.nx;
+    char* smem = (char*)(twy + p.ny);
+    load_twiddles(p, twx, twy);
+    if (threadIdx.x == 0) s = *p.ctl;
+    __syncthreads();
+    if (p.mode == MODE_RHS || p.mode == MODE_SPARSE_ONLY) {
+        StageDesc st{};
+        st.ubase = p.rhs_ubuf;
+        st.write_u = p.rhs_ubuf;
+        st.kin = -1;
+        st.kout = p.rhs_kslot;
+        st.t = s.t_rhs;
+        st.t_next_max = s.t_rhs;
+        (void)implicit_call(p, tm, s, st, gen, twx, twy, smem, red, &flag, p.mode == MODE_SPARSE_ONLY, tl);
+    } else {
+        implicit_loop(p, tm, s, gen, twx, twy, smem, red, &flag, tl);
+    }
+    if (tm.rank == 0 && threadIdx.x == 0) *p.ctl = s;
+}
+
 // ------------------------------------------------------------------------------------------
 // persistent ensemble kernel: the grid is split into nteams teams of tsize workgroups (team =
 // blockIdx.x mod nteams, so with nteams a multiple of 8 a team shares one XCD); each team takes
@@ -329,7 +391,8 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __res
 
 // kernel instances: FP64 or FP32 moments; with or without per-cell grid steps (map distortion,
 // FP64-moment path only)
-inline const void* persistent_kernel(bool mt_double, bool dist, bool elra) {
+inline const void* persistent_kernel(bool mt_double, bool dist, bool elra, bool implicit = false) {
+    if (implicit) return (const void*)fi_persistent_implicit;
     if (elra) return (const void*)fi_persistent_elra;
     if (!mt_double) return (const void*)fi_persistent<float, false>;
     return dist ? (const void*)fi_persistent<double, true> : (const void*)fi_persistent<double, false>;
@@ -473,6 +536,7 @@ struct fi_ctx {
     bool has_D = false, has_eta = false, has_spinv = false, has_ge = false, has_gg = false, has_gv = false,
          has_tau = false, has_href = false;
     bool elra = false;   // RelaxedMantle (ELRA) instead of the LV-ELVA Maxwell mantle
+    bool implicit = false;   // Maxwell mantle under a rigid or laterally constant lithosphere
     int ice_mode = ICE_EXTERNAL, ice_flat = 0;
     std::vector<double> ice_t, ice_a, ice_H;
     // barystatic sea-level rule (fi_set_bsl); A_OCEAN_PD (src/barystatic_sealevel.jl:1) by default
@@ -493,7 +557,7 @@ struct fi_ctx {
     DBuf<unsigned char> maskd, mg, mo;
     DBuf<float2> spec, pA, pB, oA, oB, tw32;
     DBuf<double2> ge_hat, gg_hat, gv_hat, tw64, gtmp;
-    DBuf<float> inv_tau_d, ueq, litho_h_d;
+    DBuf<float> inv_tau_d, ueq, litho_h_d, imp_am1, imp_b;
     DBuf<double> uxy_d;   // horizontal displacement output (FI_U_X / FI_U_Y)
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
@@ -579,7 +643,7 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     HIPCHECK(c, hipMemcpyAsync(c->kp_d.p, &kp, sizeof(KP), hipMemcpyHostToDevice, c->stream));
     const KP* kptr = c->kp_d.p;
     void* args[] = {&kptr};
-    const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra);
+    const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra, c->implicit);
     const bool timed = mode == MODE_INTEGRATE;
     const long long nacc0 = c->ctl.nacc;
     if (timed) HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -602,6 +666,21 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     if (c->ctl.err == ERR_DT_MIN) return fail(c, FI_ERR_DT_MIN, "dt below minimum");
     if (c->ctl.err) return fail(c, FI_ERR_DEVICE, "device error");
     return FI_OK;
+}
+
+// the kappa table with its DC fix (get_differential_fourier, src/derivatives.jl:201-222;
+// src/domain.jl:151-155): sqrt((pi/Wx f_i)^2 + (pi/Wy f_j)^2) with f the FFT integer frequencies
+// 0..N/2 then back down; the DC entry is 1e-3 times the mean of its two neighbours
+std::vector<double> pseudodiff_table(int nx, int ny, double Wx, double Wy) {
+    auto fint = [](int n, int k) { return (double)std::min(k, n - k); };
+    std::vector<double> out((size_t)nx * ny);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const double X = M_PI / Wx * fint(nx, i), Y = M_PI / Wy * fint(ny, j);
+            out[i + (size_t)j * nx] = std::sqrt(X * X + Y * Y);
+        }
+    out[0] = 1e-3 * ((out[(size_t)nx] + out[1]) / 2.0);
+    return out;
 }
 
 // corner BC offset of a host array (src/boundary_conditions.jl:144-148)
@@ -698,12 +777,11 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     // (src/deformation.jl:47-50)
     if (cfg->mantle == FI_RELAXED_MANTLE && cfg->lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE)
         return FI_ERR_UNSUPPORTED;
-    if (cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE)
-        return FI_ERR_UNSUPPORTED;
     if (cfg->mantle == FI_RIGID_MANTLE) return FI_ERR_UNSUPPORTED;  // dudt = 0: nothing to accelerate
     fi_ctx* c = new fi_ctx();
     c->cfg = *cfg;
     c->elra = cfg->mantle == FI_RELAXED_MANTLE;
+    c->implicit = cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE;
     c->nx = cfg->nx;
     c->ny = cfg->ny;
     c->N = (size_t)c->nx * c->ny;
@@ -883,6 +961,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     if (!c->elra && (!c->has_D || !c->has_eta || !c->has_spinv))
         return fail(c, FI_ERR_INVALID, "litho rigidity, effective viscosity and scaled_pseudodiff_inv required");
     if (c->elra && (!c->has_gv || !c->has_tau)) return fail(c, FI_ERR_INVALID, "viscous Green kernel and tau required");
+    if (c->implicit && !(g.dt_min > 0))
+        return fail(c, FI_ERR_INVALID, "the implicit step needs dt_min (DiffEqOptions.dt_min with Euler)");
     if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge) return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
         return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
@@ -923,7 +1003,32 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
 
     // spectral multiplier spy/(nx ny) * even part of scaled_pseudodiff_inv, transposed [kx][ky];
     // the DC term is zeroed: the corner BC on dudt removes any constant exactly
-    if (!c->elra) {
+    if (c->implicit) {
+        // u <- a u + b F per Fourier index (src/deformation.jl:58-79): nabla = 2 eta kappa R =
+        // 2 eta / scaled_pseudodiff_inv, beta = rho_um g + D kappa^4, dt = dt_min in seconds;
+        // stored as the even parts of a - 1 and b (the real part of the inverse transform of a
+        // real field), transposed [kx][ky] and scaled by 1/(nx ny)
+        const std::vector<double> kap = pseudodiff_table(nx, ny, c->cfg.Wx, c->cfg.Wy);
+        const double dt = g.dt_min * g.seconds_per_year, sc = 1.0 / ((double)nx * ny);
+        auto coef = [&](size_t p, double& am1, double& b) {
+            const double k2 = kap[p] * kap[p];
+            const double nabla = 2.0 * c->eta[p] / c->spinv[p];
+            const double beta = g.rho_uppermantle * g.g + c->D[p] * k2 * k2;
+            am1 = -dt * beta / (nabla + 0.5 * dt * beta);
+            b = dt / (nabla + 0.5 * dt * beta);
+        };
+        std::vector<float> A((size_t)c->nhx * ny), B((size_t)c->nhx * ny);
+        for (int kx = 0; kx < c->nhx; ++kx)
+            for (int ky = 0; ky < ny; ++ky) {
+                const int mkx = (nx - kx) % nx, mky = (ny - ky) % ny;
+                double a0, b0, a1, b1;
+                coef(kx + (size_t)ky * nx, a0, b0);
+                coef(mkx + (size_t)mky * nx, a1, b1);
+                A[(size_t)kx * ny + ky] = (float)(0.5 * (a0 + a1) * sc);
+                B[(size_t)kx * ny + ky] = (float)(0.5 * (b0 + b1) * sc);
+            }
+        if (upload(c, c->imp_am1, A) || upload(c, c->imp_b, B)) return FI_ERR_DEVICE;
+    } else if (!c->elra) {
         std::vector<float> mult((size_t)c->nhx * ny);
         const double sc = g.seconds_per_year / ((double)nx * ny);
         for (int kx = 0; kx < c->nhx; ++kx)
@@ -1166,6 +1271,10 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.ge_hat = c->ge_hat.p;
     kp.gg_hat = c->gg_hat.p;
     kp.elra = c->elra ? 1 : 0;
+    kp.implicit = c->implicit ? 1 : 0;
+    kp.imp_am1 = c->imp_am1.p;
+    kp.imp_b = c->imp_b.p;
+    kp.dt_min = g.dt_min;
     kp.gv_hat = c->gv_hat.p;
     kp.inv_tau = c->inv_tau_d.p;
     kp.ueq = c->ueq.p;
@@ -1189,16 +1298,17 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     c->lds_f = twb + std::max(rows_f, other);
     c->lds = c->mt_double ? c->lds_d : c->lds_f;
     if (c->elra) c->lds = twb + other;   // ELRA runs only the padded-grid phases
+    if (c->implicit) c->lds = twb + std::max({other, (size_t)32 * nx, (size_t)32 * ny});
     if (c->lds > 160 * 1024 || nx > 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
-    if (!c->elra && c->has_dist && (!c->mt_double || ny < 6 || nx < 3))
+    if (!c->elra && !c->implicit && c->has_dist && (!c->mt_double || ny < 6 || nx < 3))
         return fail(c, FI_ERR_UNSUPPORTED, "map distortion needs the FP64-moment rows phase (3 <= nx <= 512, ny >= 6)");
-    const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra);
+    const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra, c->implicit);
     HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
     int per_cu = 0, ncu = 0;
     HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, c->lds));
     HIPCHECK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
     int maxg = per_cu * ncu;
-    int want = std::max(kp.nrowitems, c->elra ? (c->nfhx + 1) / 2 : kp.ncolitems);
+    int want = std::max(kp.nrowitems, c->elra ? (c->nfhx + 1) / 2 : c->implicit ? c->nhx : kp.ncolitems);
     c->G = std::max(1, std::min(maxg, want));
     if (maxg < 1) return fail(c, FI_ERR_UNSUPPORTED, "persistent kernel does not fit on the device");
     // one error-norm and one volume partial per row pair, whatever the number of workgroups
@@ -1359,6 +1469,11 @@ int fi_rhs(fi_ctx* c, double t, double* u, double* dudt) {
     HIPCHECK(c, hipMemcpy(c->U.p + 2 * c->N, u, c->N * sizeof(double), hipMemcpyHostToDevice));
     int st = rhs_on_device(c, t, ubuf, slot);
     if (st) return st;
+    if (c->implicit) {   // u is replaced by the implicit step; dudt is not written
+        HIPCHECK(c, hipMemcpy(u, c->U.p + 2 * c->N, c->N * sizeof(double), hipMemcpyDeviceToHost));
+        c->ctl.last_u = ubuf;
+        return FI_OK;
+    }
     std::vector<double> k;
     if ((st = read_k(c, slot, k))) return st;
     std::memcpy(dudt, k.data(), c->N * sizeof(double));
@@ -1376,6 +1491,12 @@ int fi_rhs_device(fi_ctx* c, double t, void* u, void* dudt) {
     HIPCHECK(c, hipMemcpyAsync(c->U.p + 2 * c->N, u, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     int st = rhs_on_device(c, t, ubuf, slot);
     if (st) return st;
+    if (c->implicit) {   // u is replaced by the implicit step; dudt is not written
+        HIPCHECK(c, hipMemcpyAsync(u, c->U.p + 2 * c->N, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->ctl.last_u = ubuf;
+        return FI_OK;
+    }
     float kc[4];
     double uc[4];
     HIPCHECK(c, hipMemcpy(kc, c->kcorner.p + 4 * slot, sizeof(kc), hipMemcpyDeviceToHost));
@@ -1437,6 +1558,18 @@ int fi_integrator_init(fi_ctx* c) {
     c->ctl.last_u = 0;
     if ((st = rhs_on_device(c, t0, 0, 1))) return st;   // init_problem!
     if ((st = rhs_on_device(c, t0, 0, 0))) return st;   // FSAL: k1 = f(u0, t0)
+    if (c->implicit) {
+        // Euler with dt = dt_min (src/simulation.jl:244-250): both calls above replaced U[0] by an
+        // implicit step; the derivative slots stay zero
+        c->ctl.dt = g.dt_min;
+        c->ctl.t = t0;
+        c->ctl.k1 = 0;
+        c->ctl.nacc = c->ctl.nrej = 0;
+        c->integ = true;
+        c->k1_stale = false;
+        c->kout = 0;
+        return FI_OK;
+    }
     // initial step (Hairer, Norsett & Wanner, Solving ODEs I, II.4)
     std::vector<double> f0, f1;
     if ((st = read_k(c, 0, f0))) return st;

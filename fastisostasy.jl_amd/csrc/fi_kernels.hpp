@@ -1675,6 +1675,198 @@ __device__ __forceinline__ bool elra_call(const KP& p, const Team& tm, Ctl& s, S
 }
 
 // ------------------------------------------------------------------------------------------
+// implicit step of a Maxwell mantle under a rigid or laterally constant lithosphere
+// (src/deformation.jl:52-89): u <- Re IFFT[a FFT(u) + b FFT(F)] on the n x n grid with
+// a = (nabla - dt/2 beta) / (nabla + dt/2 beta), b = dt / (nabla + dt/2 beta) and
+// F = -(sigma_load + sigma_litho) g K^2, then the corner BC. It is evaluated as the increment
+// u + Re IFFT[(a - 1) FFT(u) + b FFT(F)], so the FP32 transforms only carry the change of u:
+//   imp_rows : stage displacement and loads of a row pair; u and F rows packed as two complex
+//              sequences, length-nx FFTs; half spectra of u in spectrum buffer 0, of F in 1;
+//   imp_cols : length-ny FFTs of both columns, (a-1) U + b F, inverse FFT into buffer 0;
+//   imp_c2r  : corner offset of the new u from its FP64 corners and direct sums over spectral
+//              rows 0 and ny-1; inverse real transforms of row pairs; u + increment - offset.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void phase_imp_rows(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                               const IceT it, const float2* twx, char* smem) {
+    const int nx = p.nx, ny = p.ny;
+    float2* sa = (float2*)smem;   // [2][nx]: u rows, F rows
+    float2* sb = sa + 2 * nx;
+    const bool wsig = st.write_sigma != 0;
+    const double* U0 = ubuf(p, st.ubase);
+    float2* specU = specbuf(p, 0);
+    float2* specF = specbuf(p, 1);
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+        const int j0 = 2 * item;
+        for (int i = threadIdx.x; i < nx; i += NT) {
+            float2 zu = make_float2(0.f, 0.f), zf = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int j = j0 + r;
+                if (j >= ny) continue;
+                const long c = i + (long)j * nx;
+                const double u = U0[c] - uoff;
+                const double m = p.mask[c] ? 1.0 : 0.0;
+                const double zo = st.trig ? p.zoff_trig[c] : p.zoff[c];
+                const double Haf = hab(ice_at(p, it, c), zo + u, p.rho_sw_ice);
+                const double s_ice = p.rho_ice * (Haf - (double)p.H_af_ref[c]);
+                const double s_load = m * (s_ice + (p.interactive ? (double)p.LW[c] : 0.0));
+                const double lt = p.LT[c];   // sigma_litho - rho_um u_ref
+                const double uref = p.u_ref[c];
+                const double k2 = p.K2 ? (double)p.K2[c] : 1.0;
+                const float f = (float)(-(s_load + (lt + p.rho_um * uref)) * p.g * k2);
+                if (r == 0) {
+                    zu.x = (float)u;
+                    zf.x = f;
+                } else {
+                    zu.y = (float)u;
+                    zf.y = f;
+                }
+                if (wsig) {
+                    const double s_mant = p.rho_um * u;
+                    const double s_full = s_load + m * (lt + s_mant);
+                    st_shared(&p.sig_load_prev[c], (float)s_load);
+                    st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
+                                                    ? s_load + m * (s_mant - p.rho_um * uref)
+                                                    : s_full));
+                }
+            }
+            sa[i] = zu;
+            sa[nx + i] = zf;
+        }
+        __syncthreads();
+        float2* res = fft_lds<float2>(sa, sb, 2, nx, p.px, twx, -1);
+        for (int idx = threadIdx.x; idx < 2 * p.nhx; idx += NT) {
+            const int s = idx >= p.nhx ? 1 : 0, kx = idx - s * p.nhx;
+            const float2 Z = res[s * nx + kx], Zm = res[s * nx + (kx == 0 ? 0 : nx - kx)];
+            float2 A, B;
+            A.x = 0.5f * (Z.x + Zm.x);
+            A.y = 0.5f * (Z.y - Zm.y);
+            B.x = 0.5f * (Z.y + Zm.y);
+            B.y = -0.5f * (Z.x - Zm.x);
+            float2* spec = s ? specF : specU;
+            st_shared(&spec[(long)kx * ny + j0], A);
+            if (j0 + 1 < ny) st_shared(&spec[(long)kx * ny + j0 + 1], B);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void phase_imp_cols(const KP& p, const Team& tm, const float2* twy, char* smem) {
+    const int ny = p.ny;
+    float2* sa = (float2*)smem;   // [2][ny]: U column, F column
+    float2* sb = sa + 2 * ny;
+    float2* specU = specbuf(p, 0);
+    const float2* specF = specbuf(p, 1);
+    for (int kx = tm.rank; kx < p.nhx; kx += tm.size) {
+        for (int idx = threadIdx.x; idx < 2 * ny; idx += NT) {
+            const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
+            sa[idx] = (s ? specF : specU)[(long)kx * ny + j];
+        }
+        __syncthreads();
+        float2* r = fft_lds<float2>(sa, sb, 2, ny, p.py, twy, -1);
+        float2* o = (r == sa) ? sb : sa;
+        for (int ky = threadIdx.x; ky < ny; ky += NT) {
+            const float am1 = p.imp_am1[(long)kx * ny + ky], b = p.imp_b[(long)kx * ny + ky];
+            const float2 U = r[ky], F = r[ny + ky];
+            o[ky] = make_float2(am1 * U.x + b * F.x, am1 * U.y + b * F.y);
+        }
+        __syncthreads();
+        float2* r2 = fft_lds<float2>(o, r, 1, ny, p.py, twy, +1);
+        for (int j = threadIdx.x; j < ny; j += NT) st_shared(&specU[(long)kx * ny + j], r2[j]);
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void phase_imp_c2r(const KP& p, const Team& tm, const StageDesc& st, double uoff,
+                                              const double* ucorner, const float2* twx, char* smem, double* red) {
+    const int nx = p.nx, ny = p.ny;
+    const bool even = (nx % 2) == 0;
+    const float2* spec = specbuf(p, 0);
+    const double* U0 = ubuf(p, st.ubase);
+    double* Uw = ubuf(p, st.write_u);
+    float2* sf = (float2*)smem;
+    float2* sg = sf + nx;
+    // increment at the four corners: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx})
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int kx = threadIdx.x; kx < p.nhx; kx += NT) {
+        const float2 a = spec[(long)kx * ny], b = spec[(long)kx * ny + ny - 1], tw = twx[kx];
+        const bool nyq = even && kx == nx / 2;
+        const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
+        v[0] += w * (double)a.x;
+        v[2] += w * (double)b.x;
+        if (nyq) {
+            v[1] -= (double)a.x;
+            v[3] -= (double)b.x;
+        } else {
+            v[1] += w * ((double)a.x * tw.x - (double)a.y * tw.y);
+            v[3] += w * ((double)b.x * tw.x - (double)b.y * tw.y);
+        }
+    }
+    block_sum<4>(red, v);
+    // corners of the new u: the stage's corners (read before this phase overwrites u) + increment
+    const double off = ((0.25 * (ucorner[0] + v[0]) + 0.25 * (ucorner[1] + v[1])) +
+                        (0.25 * (ucorner[2] + v[2]) + 0.25 * (ucorner[3] + v[3]))) - p.vbc_x;
+    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
+        const int j0 = 2 * item;
+        for (int k = threadIdx.x; k < nx; k += NT) sf[k] = c2r_pair_value(p, spec, k, j0, j0 + 1);
+        __syncthreads();
+        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, +1);
+        // each cell is read and rewritten by the same thread; no other workgroup reads this row
+        // pair in this phase
+        for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
+            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+            if (j >= ny) continue;
+            const long c = i + (long)j * nx;
+            st_shared(&Uw[c], (U0[c] - uoff) + (double)(r ? res[i].y : res[i].x) - off);
+        }
+        __syncthreads();
+    }
+}
+
+// one implicit-step call: preamble, the sparse update when due, the three spectral phases; u is
+// replaced in U[st.write_u] (st.write_u == st.ubase: in place)
+__device__ __forceinline__ bool implicit_call(const KP& p, const Team& tm, Ctl& s, StageDesc& st, unsigned& gen,
+                                              const float2* twx, const float2* twy, char* smem, double* red,
+                                              int* flag, bool sparse_only, unsigned long long& tl) {
+    KC4 kc;
+    const bool trig = sparse_only || (((st.t - s.t0) / p.dt_sparse) >= (double)s.count);
+    st.trig = trig ? 1 : 0;
+    st.kin = -1;
+    st.ng = 0;
+    IceT it;
+    double uoff;
+    stage_preamble(p, st, specbuf(p, 0), twx, red, it, kc, uoff);
+    if (it.err) {
+        if (threadIdx.x == 0) s.err = ERR_TIME_RANGE;
+        __syncthreads();
+        return false;
+    }
+    if (trig) {
+        if (!sparse_phases<true>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
+        if (sparse_only) return true;
+    }
+    // the stage's corners (with their BC), before the last phase overwrites u in place
+    const double* U0 = ubuf(p, st.ubase);
+    const long n1 = p.nx - 1, n2 = (long)(p.ny - 1) * p.nx;
+    const double ucorner[4] = {U0[0] - uoff, U0[n1] - uoff, U0[n2] - uoff, U0[n2 + n1] - uoff};
+    st.write_sigma = (st.force_sigma || p.write_sigma_always ||
+                      (((st.t_next_max - s.t0) / p.dt_sparse) >= (double)s.count)) ? 1 : 0;
+    phase_imp_rows(p, tm, st, uoff, it, twx, smem);
+    if (!grid_sync(p, tm, gen, flag, TAG_ROWS, tl)) return false;
+    phase_imp_cols(p, tm, twy, smem);
+    if (!grid_sync(p, tm, gen, flag, TAG_COLS, tl)) return false;
+    phase_imp_c2r(p, tm, st, uoff, ucorner, twx, smem, red);
+    if (!grid_sync(p, tm, gen, flag, TAG_C2R, tl)) return false;
+    if (threadIdx.x == 0) {
+        s.nrhs += 1;
+        s.t_last = st.t;
+        s.last_trig = trig ? 1 : 0;
+    }
+    __syncthreads();
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
 // one RHS call
 // ------------------------------------------------------------------------------------------
 template <typename MT, bool DIST>

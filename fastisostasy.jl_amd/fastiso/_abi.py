@@ -55,7 +55,7 @@ class FiConfig(C.Structure):
         ("bsl_z", C.c_double),
         ("viscous_x_border", C.c_double), ("elastic_x_border", C.c_double), ("ssh_x_border", C.c_double),
         ("dt_sparse", C.c_double), ("reltol", C.c_double), ("abstol", C.c_double),
-        ("device", C.c_int32), ("reserved", C.c_int32),
+        ("device", C.c_int32), ("reserved", C.c_int32), ("dt_min", C.c_double),
     ]
 
 

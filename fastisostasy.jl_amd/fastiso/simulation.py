@@ -83,6 +83,11 @@ def build_config(domain, c, solidearth, sealevel, bcs, opts, device=0):
     cfg.dt_sparse = opts.dt_sparse_diagnostics
     cfg.reltol = opts.diffeq.reltol
     cfg.abstol = opts.diffeq.abstol
+    # dt_min is only compatible with Euler (src/simulation.jl:244-253); it drives the implicit step
+    if opts.diffeq.dt_min is not None:
+        if str(opts.diffeq.alg).lower() != "euler":
+            raise ValueError("The `dt_min` option is only compatible with the Euler algorithm.")
+        cfg.dt_min = float(opts.diffeq.dt_min)
     cfg.device = device
     return cfg
 

@@ -29,9 +29,9 @@ mutable struct FiConfig
     bsl_z::Float64
     viscous_x_border::Float64; elastic_x_border::Float64; ssh_x_border::Float64
     dt_sparse::Float64; reltol::Float64; abstol::Float64
-    device::Int32; reserved::Int32
+    device::Int32; reserved::Int32; dt_min::Float64
 end
-FiConfig() = FiConfig(zeros(Int32, 2)..., zeros(12)..., zeros(Int32, 8)..., zeros(7)..., Int32(0), Int32(0))
+FiConfig() = FiConfig(zeros(Int32, 2)..., zeros(12)..., zeros(Int32, 8)..., zeros(7)..., Int32(0), Int32(0), 0.0)
 
 struct FiStats
     t::Float64; dt::Float64
@@ -84,6 +84,7 @@ function HipSimulation(sim)
     cfg.ssh_x_border = bcs.sea_surface_perturbation.x_border
     cfg.dt_sparse = sim.opts.dt_sparse_diagnostics
     cfg.reltol = sim.opts.diffeq.reltol
+    sim.opts.diffeq.dt_min isa Real && (cfg.dt_min = sim.opts.diffeq.dt_min)
     ctxr = Ref{Ptr{Cvoid}}(C_NULL)
     st = ccall((:fi_create, libfi), Cint, (Ref{FiConfig}, Ref{Ptr{Cvoid}}), cfg, ctxr)
     st == 0 || error("fi_create failed: $st")

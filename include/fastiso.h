@@ -86,10 +86,11 @@ typedef struct fi_config {
     double rho_uppermantle;      /* 3400                                                       */
     double rho_litho;            /* 3200                                                       */
     int32_t lithosphere;         /* FI_*_LITHOSPHERE                                           */
-    int32_t mantle;              /* FI_*_MANTLE: FI_MAXWELL_MANTLE needs the laterally variable
-                                    lithosphere (LV-ELVA, src/deformation.jl:91-103);
-                                    FI_RELAXED_MANTLE a rigid or laterally constant one (ELRA,
-                                    src/deformation.jl:29-45)                                  */
+    int32_t mantle;              /* FI_*_MANTLE: FI_MAXWELL_MANTLE with the laterally variable
+                                    lithosphere is LV-ELVA (src/deformation.jl:91-103), with a
+                                    rigid or laterally constant one the implicit step (dt_min,
+                                    src/deformation.jl:52-89); FI_RELAXED_MANTLE needs a rigid or
+                                    laterally constant one (ELRA, src/deformation.jl:29-45)    */
     int32_t lithosphere_column;  /* FI_*_LITHOSPHERE_COLUMN                                    */
     /* RegionalSeaLevel                                                 src/sealevel.jl:150-166 */
     int32_t sea_surface;         /* FI_*_SEA_SURFACE                                           */
@@ -106,6 +107,10 @@ typedef struct fi_config {
     double abstol;               /* 1e-6                                                       */
     int32_t device;              /* HIP device ordinal (ignored by the CPU oracle)             */
     int32_t reserved;
+    double dt_min;               /* DiffEqOptions.dt_min (yr) with alg = Euler: required by, and
+                                    only used by, the implicit step of a Maxwell mantle under a
+                                    rigid or laterally constant lithosphere (src/deformation.jl:
+                                    52-89, src/simulation.jl:244-253); 0 otherwise             */
 } fi_config;
 
 /* ---- fields ------------------------------------------------------------------------------- */

@@ -303,6 +303,11 @@ void samesize_conv_ext(double* out, const double* input, const ConvPlan& cp, con
             out[(i - i1) + (size_t)(j - j1) * nx] = crop[(i - 1) + (size_t)(j - 1) * ex];
 }
 
+// MaxwellMantle under a rigid or laterally constant lithosphere: the implicit step
+inline bool implicit_step(const fi_config& g) {
+    return g.mantle == FI_MAXWELL_MANTLE && g.lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE;
+}
+
 // height_above_floatation (src/topography.jl:20-23)
 inline double hab(double H, double zb, double zss, double rho_sw_ice) {
     return std::max(H + std::min(zb - zss, 0.0) * rho_sw_ice, 0.0);
@@ -593,6 +598,35 @@ void update_dudt_elra(fi_ctx* c, double* dudt) {
     for (size_t p = 0; p < N; ++p) dudt[p] = 1.0 / c->tau[p] * (c->u_eq[p] - c->u[p]);
 }
 
+// update_dudt!(MaxwellMantle, lithosphere) for a rigid or laterally constant lithosphere
+// (src/deformation.jl:52-89): one implicit (trapezoidal) step of dt_min on the n x n spectrum,
+// u <- Re IFFT[((nabla - dt/2 beta) FFT(u) + dt FFT(F)) / (nabla + dt/2 beta)] with
+// nabla = 2 eta kappa R = 2 eta / scaled_pseudodiff_inv, beta = rho_um g + D kappa^4 and
+// F = -(sigma_load + sigma_litho) g K^2, all taken elementwise on the Fourier layout; then the
+// corner BC. u is overwritten (and becomes the state); dudt is not written.
+void update_u_implicit(fi_ctx* c, double* u) {
+    const fi_config& g = c->cfg;
+    const int nx = c->nx, ny = c->ny;
+    size_t N = c->N;
+    const double dt = g.dt_min * g.seconds_per_year;
+    std::vector<cd> F(N), U(N);
+    for (size_t p = 0; p < N; ++p) F[p] = cd(-(c->ca_load[p] + c->ca_litho[p]) * g.g * (c->K[p] * c->K[p]), 0.0);
+    for (size_t p = 0; p < N; ++p) U[p] = cd(u[p], 0.0);
+    fft2d(F.data(), nx, ny, c->vfx, c->vfy, -1);
+    fft2d(U.data(), nx, ny, c->vfx, c->vfy, -1);
+    for (size_t p = 0; p < N; ++p) {
+        const double kap = c->pd[p];
+        const double nabla = 2.0 * c->eta[p] / c->spinv[p];
+        const double beta = g.rho_uppermantle * g.g + c->D[p] * (kap * kap) * (kap * kap);
+        U[p] = ((nabla - (dt / 2) * beta) * U[p] + dt * F[p]) / (nabla + (dt / 2) * beta);
+    }
+    fft2d(U.data(), nx, ny, c->vfx, c->vfy, +1);
+    const double s = 1.0 / ((double)nx * ny);
+    for (size_t p = 0; p < N; ++p) u[p] = U[p].real() * s;
+    apply_corner_bc(u, nx, ny, g.viscous_x_border);
+    std::memcpy(c->u.data(), u, N * sizeof(double));
+}
+
 // update_diagnostics!(dudt, u, sim, t) (src/simulation.jl:335-393)
 int update_diagnostics(fi_ctx* c, double* dudt, double* u, double t) {
     const fi_config& g = c->cfg;
@@ -625,6 +659,8 @@ int update_diagnostics(fi_ctx* c, double* dudt, double* u, double t) {
         std::fill(dudt, dudt + N, 0.0);
     } else if (g.mantle == FI_RELAXED_MANTLE) {
         update_dudt_elra(c, dudt);
+    } else if (implicit_step(g)) {
+        update_u_implicit(c, u);
     } else {
         update_dudt_lvelva(c, u, dudt);
     }
@@ -699,8 +735,6 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     // (src/deformation.jl:47-50)
     if (cfg->mantle == FI_RELAXED_MANTLE && cfg->lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE)
         return FI_ERR_UNSUPPORTED;
-    if (cfg->mantle == FI_MAXWELL_MANTLE && cfg->lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE)
-        return FI_ERR_UNSUPPORTED;  // implicit Bueler step (src/deformation.jl:52-89) is not this path
     fi_ctx* c = new fi_ctx();
     c->cfg = *cfg;
     c->nx = cfg->nx;
@@ -904,6 +938,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
         return fail(c, FI_ERR_INVALID, "litho rigidity, effective viscosity and scaled_pseudodiff_inv required");
     if (g.mantle == FI_RELAXED_MANTLE && (!c->has_gv || !c->has_tau))
         return fail(c, FI_ERR_INVALID, "viscous Green kernel and tau required");
+    if (implicit_step(g) && !(g.dt_min > 0))
+        return fail(c, FI_ERR_INVALID, "the implicit step needs dt_min (DiffEqOptions.dt_min with Euler)");
     if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge)
         return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
@@ -998,6 +1034,15 @@ int fi_integrator_init(fi_ctx* c) {
     for (vec* v : {&c->k1, &c->k2, &c->k3, &c->k4, &c->tmp}) v->assign(N, 0.0);
     st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t);  // FSAL: fsalfirst = f(u0, t0)
     if (st) return st;
+    if (implicit_step(g)) {
+        // Euler with dt = dt_min (src/simulation.jl:244-250): both calls above advanced u (the
+        // implicit step writes u and leaves the derivative at zero)
+        c->dt = g.dt_min;
+        c->integ = true;
+        c->k1_stale = false;
+        c->kout = 0;
+        return FI_OK;
+    }
     // initial step (Hairer, Norsett & Wanner, Solving ODEs I, II.4)
     vec sk(N), tq(N);
     for (size_t p = 0; p < N; ++p) sk[p] = g.abstol + std::fabs(c->uprev[p]) * g.reltol;
@@ -1039,6 +1084,22 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
         double stop = t_target;
         for (double to : c->tout)
             if (to > c->t) { stop = std::min(stop, to); break; }
+        if (implicit_step(g)) {
+            // Euler step: u = uprev + dt * 0, then f(u, t + dt) replaces u by one implicit step of
+            // dt_min; an output callback is followed by the FSAL re-evaluation, one more step
+            const bool clip = c->t + g.dt_min >= stop;
+            c->t = clip ? stop : c->t + g.dt_min;
+            int st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t);
+            if (st) return st;
+            c->nacc += 1;
+            if (c->kout < (int64_t)c->tout.size() && c->t >= c->tout[c->kout]) {
+                c->u = c->uprev;
+                if (cb) cb(user, c->t, (int32_t)c->kout);
+                c->kout += 1;
+                if ((st = update_diagnostics(c, c->k1.data(), c->uprev.data(), c->t))) return st;
+            }
+            continue;
+        }
         double dt = c->dt;
         bool clipped = false;
         if (c->t + dt >= stop) { dt = stop - c->t; clipped = true; }
