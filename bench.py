@@ -14,7 +14,9 @@ unit); achieved = that traffic / the kernel time measured with HIP events on the
 cpu_baseline: the FP64 CPU restatement of the reference path (oracle/, OpenMP) on this host, on
 the same workload, over a bounded window.
 
-Other workloads (--workload): elra is the relaxed-mantle model (ELRA) on the same N=350 grid:
+Other workloads (--workload): implicit is the disc benchmark of the reference's docs on the
+N=350 grid with the implicit step (RigidLithosphere, Euler with dt_min = 100 yr, 1-kyr windows;
+the roofline counts IMPLICIT_BYTES_PER_CELL_STEP per cell per step). elra is the relaxed-mantle model (ELRA) on the same N=350 grid:
 laterally constant 88 km lithosphere with the elastic response, the 2D relaxation time of the
 ELRA example (weak mantle, a Gaussian low-viscosity anomaly), a 1000 m ice cylinder of radius
 1000 km switched on over the first year, laterally variable sea surface; a step is a 100-yr window
@@ -43,16 +45,27 @@ BYTES_PER_CELL_STEP = 171.0
 # padded spectrum (16); the last phase reads it (8), the stage displacement (8) and 1/tau (4),
 # writes dudt and u_eq (8) and on the last stage reads k1..k3 and u_n for the error norm (20/3)
 ELRA_BYTES_PER_CELL_RHS = 8 + 20 / 3 + 25 + 8 + 8 + 8 / 3 + 16 + 8 + 8 + 4 + 8 + 20 / 3
+# implicit step, per cell per step: rows phase reads u (8) and the per-cell fields (mask,
+# bedrock offset, two ice snapshots, reference H_af, litho anomaly, u_ref: 25), writes the u and
+# load half spectra (16) and the loads for the sparse update (8); the column phase reads both
+# spectra (16) and writes one (8); the last phase reads it (8) and u (8) and writes u (8)
+IMPLICIT_BYTES_PER_CELL_STEP = 8 + 25 + 16 + 8 + 16 + 8 + 8 + 8 + 8
 HBM_PEAK_GBS = 8000.0
 # measured HBM-side traffic of the persistent kernel at N=350 (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, calibrated on the copy probe); scaled to this run's accepted steps per launch
 TRAFFIC_N350 = os.path.join(ROOT, "profiles", "fi_persistent_lv_elva_n350_traffic.json")
 
 
-def make_sim(n, engine, hr=False, elra=False):
+def make_sim(n, engine, hr=False, elra=False, implicit=False):
     import fastiso
     from fastiso import workloads
-    if elra:
+    if implicit:
+        w = workloads.disc_benchmark(n=n, t_end=1e6, t_out=[])
+        w["solidearth"] = fastiso.SolidEarth(w["domain"], lithosphere=fastiso.RigidLithosphere(),
+                                             layer_boundaries=[88e3], layer_viscosities=[1e21],
+                                             rho_uppermantle=3300.0, rho_litho=0.0)
+        w["opts"] = fastiso.SolverOptions(diffeq=fastiso.DiffEqOptions(alg="Euler", dt_min=100.0))
+    elif elra:
         w = workloads.elra(n=n, W=3.15e6, t_end=50e3, tau="weak", lithosphere="constant", t_out=[])
         w["sealevel"] = fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface())
     elif hr:
@@ -123,11 +136,11 @@ def bench_ensemble(a):
     }
 
 
-def cpu_baseline(n, window, sample_s, hr=False, elra=False):
+def cpu_baseline(n, window, sample_s, hr=False, elra=False, implicit=False):
     threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
     os.environ["OMP_NUM_THREADS"] = threads
     from oracle import pyref
-    w, sim = make_sim(n, engine=None, hr=hr, elra=elra)
+    w, sim = make_sim(n, engine=None, hr=hr, elra=elra, implicit=implicit)
     e = pyref.RefEngine.from_inputs(sim.inputs)
     e.integrator_init()
     e.integrate(window)                 # same warm-up as the GPU leg: the first window
@@ -259,7 +272,7 @@ def main():
     ap.add_argument("--window", type=float, default=None, help="window in yr (default 10; 5 for lv_elva_hr)")
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "ensemble", "surrogate", "fno_train"],
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "surrogate", "fno_train"],
                     default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=20.0)
@@ -269,12 +282,13 @@ def main():
     a = ap.parse_args()
     hr_defaults = a.workload == "lv_elva_hr"
     elra = a.workload == "elra"
+    implicit = a.workload == "implicit"
     if a.steps is None:
-        a.steps = 3 if hr_defaults else 50 if elra else 200
+        a.steps = 3 if hr_defaults else 50 if (elra or implicit) else 200
     if a.warmup is None:
-        a.warmup = 1 if hr_defaults else 2 if elra else 5
+        a.warmup = 1 if hr_defaults else 2 if (elra or implicit) else 5
     if a.window is None:
-        a.window = 5.0 if hr_defaults else 100.0 if elra else 10.0
+        a.window = 5.0 if hr_defaults else 100.0 if elra else 1000.0 if implicit else 10.0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
@@ -293,7 +307,7 @@ def main():
         a.n = 1024
     # the engine owns its device memory and stream; every fi_integrate call returns after its
     # stream has synchronised, so wall-clock brackets below time completed device work
-    w, sim = make_sim(a.n, engine="hip", hr=hr, elra=elra)
+    w, sim = make_sim(a.n, engine="hip", hr=hr, elra=elra, implicit=implicit)
     eng = sim.engine
     eng.integrator_init()
     t = 0.0
@@ -317,14 +331,17 @@ def main():
     rhs = s1["n_rhs"] - s0["n_rhs"]
     if elra:
         achieved = ELRA_BYTES_PER_CELL_RHS * N * rhs / dev_s / 1e9 if dev_s > 0 else None
+    elif implicit:
+        achieved = IMPLICIT_BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
     else:
         achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
     traffic = None
-    if a.n == 350 and not hr and not elra and os.path.exists(TRAFFIC_N350) and launches > 0:
+    if a.n == 350 and not (hr or elra or implicit) and os.path.exists(TRAFFIC_N350) and launches > 0:
         with open(TRAFFIC_N350) as f:
             traffic = json.load(f)["bytes_per_accepted_step"] * acc / launches
     out = {
-        "metric": f"simulated kyr/s at N={a.n} {'ELRA' if elra else 'LV-ELVA'} (plain solver)",
+        "metric": f"simulated kyr/s at N={a.n} " + ("ELRA (plain solver)" if elra else "implicit LC step (Euler, dt_min 100 yr)"
+                                                     if implicit else "LV-ELVA (plain solver)"),
         "value": kyr / el,
         "unit": "kyr/s",
         "n_gpus": 1,
@@ -338,7 +355,7 @@ def main():
         "data": "synthetic (seeded fields)",
         "config": {"workload": f"{a.workload}_n{a.n}", "n": a.n, "dx_km": 2 * w["domain"].Wx / a.n / 1e3,
                    "window_yr": a.window,
-                   "integrator": "BS3 reltol 1e-5 abstol 1e-6", "accepted_steps": int(s1["n_accept"] - s0["n_accept"]),
+                   "integrator": "Euler dt_min 100 yr" if implicit else "BS3 reltol 1e-5 abstol 1e-6", "accepted_steps": int(s1["n_accept"] - s0["n_accept"]),
                    "rejected_steps": int(s1["n_reject"] - s0["n_reject"]),
                    "rhs_calls": int(s1["n_rhs"] - s0["n_rhs"]), "sim_t_end_yr": s1["t"],
                    "kernel_launches": int(launches), "kernel_ms_per_launch": dev_s * 1e3 / max(launches, 1),
@@ -347,7 +364,7 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic},
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr, elra)
+        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr, elra, implicit)
     print(json.dumps(out), flush=True)
 
 
