@@ -1916,7 +1916,7 @@ int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out) {
             fi_batch_destroy(b);
             return FI_ERR_INVALID;  // one device per ensemble
         }
-        if (cfgs[m].mantle == FI_RELAXED_MANTLE) {
+        if (cfgs[m].mantle != FI_MAXWELL_MANTLE || cfgs[m].lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE) {
             fi_batch_destroy(b);
             return FI_ERR_UNSUPPORTED;  // the ensemble kernel runs the LV-ELVA RHS only
         }
