@@ -192,3 +192,27 @@ def test_gpu_elra_integrate_vs_oracle(lithosphere, interactive):
         U = ref2.get_field("u_eq")
         want = U * relaxation_closed_form(U, tau, 3000.0)
         assert np.abs(sim.engine.get_field("u") - want).max() <= 2e-4 * np.abs(U).max()
+
+
+@pytest.mark.gpu
+def test_gpu_elra_with_distortion_vs_oracle():
+    """Map distortion under ELRA: K^2 on the viscous-convolution input and the cell areas of the
+    sea-level diagnostics (any grid: ELRA has no plate stencils)."""
+    from oracle import pyref
+    w = elra_case(n=40, lithosphere="constant", interactive=True)
+    d0 = w["domain"]
+    r = d0.R / d0.R.max()
+    d = fastiso.RegionalDomain(d0.Wx, d0.Wy, d0.nx, d0.ny, K=0.95 + 0.1 * r ** 2)
+    w["domain"] = d
+    w["bcs"] = fastiso.BoundaryConditions(d, ice_thickness=w["bcs"].ice_thickness)
+    w["solidearth"] = fastiso.SolidEarth(d, tau=w["tau"], lithosphere=fastiso.LaterallyConstantLithosphere(),
+                                         mantle=fastiso.RelaxedMantle(), maskactive=r < 0.55)
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    u = -30 * np.exp(-(d.R / 1.2e6) ** 2)
+    for t in (0.5, 5000.0, 5010.0):
+        dg, _ = sim.engine.rhs(t, u)
+        dr, _ = ref.rhs(t, u)
+        assert np.abs(dg - dr).max() <= 1e-5 * np.abs(dr).max(), t
+    a, b = sim.engine.get_field("u_eq"), ref.get_field("u_eq")
+    assert np.abs(a - b).max() <= 2e-6 * np.abs(b).max()

@@ -120,3 +120,33 @@ def test_gpu_implicit_step_vs_oracle(lithosphere):
     for (t, a), (_, b) in zip(og, orf):
         assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max(), t
     assert sim2.engine.stats()["n_accept"] == ref2.stats()["n_accept"]
+
+
+@pytest.mark.gpu
+def test_gpu_implicit_step_with_sea_level_and_distortion_vs_oracle():
+    """Laterally constant lithosphere with the elastic response, interactive sea level on a
+    marine margin and map distortion (K^2 on the load of the implicit step)."""
+    from oracle import pyref
+    w = disc_implicit_case(n=40, t_end=20e3, lithosphere="constant")
+    d0 = w["domain"]
+    r = d0.R / d0.R.max()
+    d = fastiso.RegionalDomain(d0.Wx, d0.Wy, d0.nx, d0.ny, K=0.95 + 0.1 * r ** 2)
+    w["domain"] = d
+    H1 = fastiso.uniform_ice_cylinder(d, 1e6, 1e3)
+    w["bcs"] = fastiso.BoundaryConditions(
+        d, ice_thickness=fastiso.TimeInterpolatedIceThickness([0.0, 1.0, 20e3], [d.zeros(), H1, H1]))
+    w["solidearth"] = fastiso.SolidEarth(d, lithosphere=fastiso.LaterallyConstantLithosphere(),
+                                         layer_boundaries=[88e3], layer_viscosities=[1e21], maskactive=r < 0.55)
+    w["sealevel"] = fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface(),
+                                             load=fastiso.InteractiveSealevelLoad())
+    w["z_b_ref"] = 500.0 - 1500.0 * r
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    sim.engine.integrator_init()
+    ref.integrator_init()
+    for t in (500.0, 2000.0):
+        sim.engine.integrate(t)
+        ref.integrate(t)
+        a, b = sim.engine.get_field("u"), ref.get_field("u")
+        assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max(), t
+    assert np.abs(ref.get_field("canom_seawater")).max() > 0
