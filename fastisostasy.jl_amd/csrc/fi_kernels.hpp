@@ -1417,8 +1417,11 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
 // the sparse update of an RHS call (src/simulation.jl:355-381): elastic and sea-surface
 // convolutions, volumes and barystatic sea level, pointwise diagnostics; false when the call stops.
 // With SKIP_EMPTY a rigid lithosphere under a laterally constant sea surface (both convolution
-// inputs unused) skips the convolution phases.
-template <bool SKIP_EMPTY>
+// inputs unused) skips the convolution phases. With SAME_CELL_NEXT the phase that follows reads
+// the pointwise diagnostics only at the cells of this workgroup's own row pairs (the ELRA and
+// implicit-step rows phases): the last grid barrier becomes a workgroup barrier after this
+// workgroup's stores have completed.
+template <bool SKIP_EMPTY, bool SAME_CELL_NEXT = false>
 __device__ __forceinline__ bool sparse_phases(const KP& p, const Team& tm, Ctl& s, const StageDesc& st, unsigned& gen,
                                               char* smem, double* red, int* flag, unsigned long long& tl,
                                               double uoff, const IceT it) {
@@ -1437,6 +1440,12 @@ __device__ __forceinline__ bool sparse_phases(const KP& p, const Team& tm, Ctl& 
     volumes_and_bsl(p, s, st.t, red);
     if (s.err) return false;   // every workgroup sees the same error
     phase_sp_point(p, tm, st, uoff, it, s.z_bsl);
+    if constexpr (SAME_CELL_NEXT) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        return true;
+    }
     return grid_sync(p, tm, gen, flag, TAG_SPARSE, tl);
 }
 
@@ -1653,7 +1662,7 @@ __device__ __forceinline__ bool elra_call(const KP& p, const Team& tm, Ctl& s, S
         return false;
     }
     if (trig) {
-        if (!sparse_phases<true>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
+        if (!sparse_phases<true, true>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
         if (sparse_only) return true;
     }
     st.write_sigma = (st.force_sigma || p.write_sigma_always ||
@@ -1842,7 +1851,7 @@ __device__ __forceinline__ bool implicit_call(const KP& p, const Team& tm, Ctl& 
         return false;
     }
     if (trig) {
-        if (!sparse_phases<true>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
+        if (!sparse_phases<true, true>(p, tm, s, st, gen, smem, red, flag, tl, uoff, it)) return false;
         if (sparse_only) return true;
     }
     // the stage's corners (with their BC), before the last phase overwrites u in place
