@@ -216,3 +216,23 @@ def test_gpu_elra_with_distortion_vs_oracle():
         assert np.abs(dg - dr).max() <= 1e-5 * np.abs(dr).max(), t
     a, b = sim.engine.get_field("u_eq"), ref.get_field("u_eq")
     assert np.abs(a - b).max() <= 2e-6 * np.abs(b).max()
+
+
+@pytest.mark.gpu
+def test_gpu_elra_checkpoint_restart_is_bitwise():
+    """The checkpoint of a relaxed-mantle context holds u_eq too: a restored context integrates
+    to the same state bit for bit."""
+    w = elra_case(n=32, lithosphere="constant", interactive=True)
+    sim = hip_sim(w)
+    e = sim.engine
+    e.integrator_init()
+    e.integrate(300.0)
+    blob = e.checkpoint()
+    e.integrate(900.0)
+    ua, qa, sa = e.get_field("u"), e.get_field("u_eq"), e.stats()
+    e2 = hip_sim(w).engine
+    e2.restore(blob)
+    e2.integrate(900.0)
+    np.testing.assert_array_equal(e2.get_field("u"), ua)
+    np.testing.assert_array_equal(e2.get_field("u_eq"), qa)
+    assert e2.stats()["n_accept"] == sa["n_accept"]

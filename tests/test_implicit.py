@@ -150,3 +150,19 @@ def test_gpu_implicit_step_with_sea_level_and_distortion_vs_oracle():
         a, b = sim.engine.get_field("u"), ref.get_field("u")
         assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max(), t
     assert np.abs(ref.get_field("canom_seawater")).max() > 0
+
+
+@pytest.mark.gpu
+def test_gpu_implicit_checkpoint_restart_is_bitwise():
+    w = disc_implicit_case(n=32, t_end=20e3)
+    e = hip_sim(w).engine
+    e.integrator_init()
+    e.integrate(3000.0)
+    blob = e.checkpoint()
+    e.integrate(7000.0)
+    ua, sa = e.get_field("u"), e.stats()
+    e2 = hip_sim(w).engine
+    e2.restore(blob)
+    e2.integrate(7000.0)
+    np.testing.assert_array_equal(e2.get_field("u"), ua)
+    assert e2.stats()["n_accept"] == sa["n_accept"]
