@@ -236,3 +236,27 @@ def test_gpu_elra_checkpoint_restart_is_bitwise():
     np.testing.assert_array_equal(e2.get_field("u"), ua)
     np.testing.assert_array_equal(e2.get_field("u_eq"), qa)
     assert e2.stats()["n_accept"] == sa["n_accept"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(40, 30), (35, 28)])
+def test_gpu_elra_non_square_grid_vs_oracle(nx, ny):
+    """Rectangular and odd grids: separate padded lengths and crop offsets along x and y."""
+    from oracle import pyref
+    d = fastiso.RegionalDomain(3e6, 3e6 * ny / nx, nx, ny)
+    H1 = fastiso.uniform_ice_cylinder(d, 1e6, 1e3)
+    it = fastiso.TimeInterpolatedIceThickness([0.0, 1.0, 20e3], [d.zeros(), H1, H1])
+    tau = fastiso.get_relaxation_time_weaker(10.0 ** (21.0 - np.exp(-0.5 * (d.R / 1.5e6) ** 2)))
+    se = fastiso.SolidEarth(d, tau=tau, lithosphere=fastiso.LaterallyConstantLithosphere(),
+                            mantle=fastiso.RelaxedMantle())
+    w = dict(domain=d, bcs=fastiso.BoundaryConditions(d, ice_thickness=it),
+             sealevel=fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface()), solidearth=se,
+             t_span=(0.0, 20e3), nout=fastiso.NativeOutput(), opts=fastiso.SolverOptions(),
+             c=fastiso.PhysicalConstants())
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    u = -30 * np.exp(-(d.R / 1.2e6) ** 2)
+    for t in (0.5, 5000.0, 5010.0):
+        dg, _ = sim.engine.rhs(t, u)
+        dr, _ = ref.rhs(t, u)
+        assert np.abs(dg - dr).max() <= 1e-5 * np.abs(dr).max(), t

@@ -166,3 +166,28 @@ def test_gpu_implicit_checkpoint_restart_is_bitwise():
     e2.integrate(7000.0)
     np.testing.assert_array_equal(e2.get_field("u"), ua)
     assert e2.stats()["n_accept"] == sa["n_accept"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(40, 30), (35, 28)])
+def test_gpu_implicit_non_square_grid_vs_oracle(nx, ny):
+    from oracle import pyref
+    d = fastiso.RegionalDomain(3e6, 3e6 * ny / nx, nx, ny)
+    H1 = fastiso.uniform_ice_cylinder(d, 1e6, 1e3)
+    it = fastiso.TimeInterpolatedIceThickness([0.0, 1.0, 20e3], [d.zeros(), H1, H1])
+    se = fastiso.SolidEarth(d, lithosphere=fastiso.LaterallyConstantLithosphere(), layer_boundaries=[88e3],
+                            layer_viscosities=[1e21])
+    w = dict(domain=d, bcs=fastiso.BoundaryConditions(d, ice_thickness=it),
+             sealevel=fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface()), solidearth=se,
+             t_span=(0.0, 20e3), nout=fastiso.NativeOutput(t=[1000.0, 2000.0]),
+             opts=fastiso.SolverOptions(diffeq=fastiso.DiffEqOptions(alg="Euler", dt_min=100.0)),
+             c=fastiso.PhysicalConstants())
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    sim.engine.integrator_init()
+    ref.integrator_init()
+    for t in (1000.0, 2000.0):
+        sim.engine.integrate(t)
+        ref.integrate(t)
+        a, b = sim.engine.get_field("u"), ref.get_field("u")
+        assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max(), t
