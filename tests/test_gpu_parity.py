@@ -280,3 +280,36 @@ def test_checkpoint_restart_is_bitwise():
     # rollback on the same context
     e2.restore(blob)
     assert e2.stats()["t"] == 150.0
+
+
+@pytest.mark.gpu
+def test_n350_workload_fields_vs_oracle():
+    """The N=350 workload itself (the bench configuration) on the device and on the FP64 oracle
+    over 0-20 yr (two sparse updates): per-field max and relative-L2 errors of u, ue, dz_ss, z_ss,
+    H_af and the full column anomaly, printed and bounded. This early in the glacial cycle |u| is
+    below 1 mm, so the step control works at its absolute tolerance (1e-6 m): the oracle against
+    itself at reltol 1e-5 and 0.9e-5 differs by 2.6e-5 m in u, 1.3e-6 m in ue and dz_ss and 0.09
+    kg/m^2 in the column anomaly at 20 yr. The bounds are absolute, a few times those."""
+    from oracle import pyref
+    w = workloads.lv_elva(n=350, t_end=120e3, t_out=[10.0, 20.0])
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    fields = ("u", "ue", "dz_ss", "z_ss", "H_af", "canom_full")
+    og, orf = [], []
+    sim.engine.integrator_init()
+    sim.engine.integrate(20.0, lambda t, k: og.append((t, {f: sim.engine.get_field(f) for f in fields})))
+    ref.integrator_init()
+    ref.integrate(20.0, lambda t, k: orf.append((t, {f: ref.get_field(f) for f in fields})))
+    assert [t for t, _ in og] == [t for t, _ in orf] == [10.0, 20.0]
+    atol = {"u": 1e-4, "ue": 5e-6, "dz_ss": 5e-6, "z_ss": 5e-6, "H_af": 1e-5, "canom_full": 0.5}
+    for (t, a), (_, b) in zip(og, orf):
+        for f in fields:
+            den = max(np.abs(b[f]).max(), 1e-30)
+            eabs = np.abs(a[f] - b[f]).max()
+            el2 = np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-30)
+            print(f"t={t:5.1f} {f:10s} max|b|={den:10.4g} abs max {eabs:9.2e} rel max {eabs / den:9.2e} "
+                  f"rel L2 {el2:9.2e}")
+            assert eabs <= atol[f], (t, f, eabs)
+    sg, sr = sim.engine.stats(), ref.stats()
+    print("accepted steps", sg["n_accept"], sr["n_accept"])
+    assert abs(sg["n_accept"] - sr["n_accept"]) <= 0.05 * sr["n_accept"] + 3
