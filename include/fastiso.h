@@ -288,7 +288,8 @@ int fi_get_stats(fi_ctx* ctx, fi_stats* out);
  * fi_set_field(FI_U) then fi_set_time). The column loads are stored at (t, u) and the sparse
  * update (elastic response, sea surface, barystatic sea level; src/simulation.jl:355-381) runs
  * with them at once; the next one is due one dt_sparse later. The next fi_integrate re-evaluates
- * the first-same-as-last derivative. No counterpart in the reference. */
+ * the first-same-as-last derivative. No counterpart in the reference. FI_ERR_UNSUPPORTED under the
+ * implicit step, whose every call advances u. */
 int fi_set_time(fi_ctx* ctx, double t);
 
 /* Checkpoints of a context's dynamic state: the displacement, the first-same-as-last derivative,

@@ -1163,6 +1163,8 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
 int fi_set_time(fi_ctx* c, double t) {
     if (!c) return FI_ERR_INVALID;
     if (!c->integ) return fail(c, FI_ERR_STATE, "fi_set_time before fi_integrator_init");
+    // its load-storing call would advance u under the implicit step
+    if (implicit_step(c->cfg)) return fail(c, FI_ERR_UNSUPPORTED, "fi_set_time: not with the implicit step");
     if (!(t >= c->t0)) return fail(c, FI_ERR_INVALID, "fi_set_time: t before the start time");
     const int64_t due = (int64_t)std::floor((t - c->t0) / c->cfg.dt_sparse);
     c->count = due + 1;

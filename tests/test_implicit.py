@@ -191,3 +191,11 @@ def test_gpu_implicit_non_square_grid_vs_oracle(nx, ny):
         ref.integrate(t)
         a, b = sim.engine.get_field("u"), ref.get_field("u")
         assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max(), t
+
+
+def test_oracle_set_time_is_refused_under_the_implicit_step():
+    from oracle import pyref
+    ref = pyref.RefEngine.from_inputs(sim_inputs(cosine_implicit_case()).inputs)
+    ref.integrator_init()
+    with pytest.raises(RuntimeError, match="implicit"):
+        ref.set_time(500.0)
