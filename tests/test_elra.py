@@ -260,3 +260,25 @@ def test_gpu_elra_non_square_grid_vs_oracle(nx, ny):
         dg, _ = sim.engine.rhs(t, u)
         dr, _ = ref.rhs(t, u)
         assert np.abs(dg - dr).max() <= 1e-5 * np.abs(dr).max(), t
+
+
+@pytest.mark.gpu
+def test_gpu_elra_bench_workload_n350_vs_oracle():
+    """The N=350 ELRA bench workload (6300 km square, laterally constant lithosphere, weak 2D tau,
+    laterally variable sea surface) over its first 300 yr on both back-ends."""
+    from oracle import pyref
+    w = workloads.elra(n=350, W=3.15e6, t_end=50e3, tau="weak", lithosphere="constant", t_out=[100.0, 300.0])
+    w["sealevel"] = fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface())
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    og, orf = [], []
+    sim.engine.integrator_init()
+    ref.integrator_init()
+    sim.engine.integrate(300.0, lambda t, k: og.append((t, sim.engine.get_field("u"), sim.engine.get_field("ue"))))
+    ref.integrate(300.0, lambda t, k: orf.append((t, ref.get_field("u"), ref.get_field("ue"))))
+    assert [t for t, *_ in og] == [t for t, *_ in orf] == [100.0, 300.0]
+    for (t, a, ea), (_, b, eb) in zip(og, orf):
+        e1 = np.abs(a - b).max() / np.abs(b).max()
+        e2 = np.abs(ea - eb).max() / np.abs(eb).max()
+        print(f"t={t} u max-rel {e1:.2e} ue max-rel {e2:.2e}")
+        assert e1 <= 5e-4 and e2 <= 5e-4, (t, e1, e2)

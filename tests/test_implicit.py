@@ -199,3 +199,24 @@ def test_oracle_set_time_is_refused_under_the_implicit_step():
     ref.integrator_init()
     with pytest.raises(RuntimeError, match="implicit"):
         ref.set_time(500.0)
+
+
+@pytest.mark.gpu
+def test_gpu_implicit_bench_workload_n350_vs_oracle():
+    """The N=350 implicit-step bench workload (the docs' disc on a 350^2 grid, Euler with
+    dt_min = 100 yr) over 2 kyr: fixed steps, so both back-ends take the same 20 steps."""
+    from oracle import pyref
+    w = disc_implicit_case(n=350, t_end=1e6)
+    w["nout"] = fastiso.NativeOutput(t=[1000.0, 2000.0])
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    og, orf = [], []
+    sim.engine.integrator_init()
+    ref.integrator_init()
+    sim.engine.integrate(2000.0, lambda t, k: og.append((t, sim.engine.get_field("u"))))
+    ref.integrate(2000.0, lambda t, k: orf.append((t, ref.get_field("u"))))
+    for (t, a), (_, b) in zip(og, orf):
+        e = np.abs(a - b).max() / np.abs(b).max()
+        print(f"t={t} u max-rel {e:.2e}")
+        assert e <= 2e-5, (t, e)
+    assert sim.engine.stats()["n_accept"] == ref.stats()["n_accept"]
