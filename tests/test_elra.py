@@ -265,7 +265,9 @@ def test_gpu_elra_non_square_grid_vs_oracle(nx, ny):
 @pytest.mark.gpu
 def test_gpu_elra_bench_workload_n350_vs_oracle():
     """The N=350 ELRA bench workload (6300 km square, laterally constant lithosphere, weak 2D tau,
-    laterally variable sea surface) over its first 300 yr on both back-ends."""
+    laterally variable sea surface) over its first 300 yr on both back-ends. Tolerances: the
+    oracle against itself at reltol 0.7e-5 and 1.3e-5 differs by up to 7.4e-4 (100 yr, in the
+    transient after the load ramp) and 1.5e-4 (300 yr) of max |u|."""
     from oracle import pyref
     w = workloads.elra(n=350, W=3.15e6, t_end=50e3, tau="weak", lithosphere="constant", t_out=[100.0, 300.0])
     w["sealevel"] = fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface())
@@ -281,4 +283,4 @@ def test_gpu_elra_bench_workload_n350_vs_oracle():
         e1 = np.abs(a - b).max() / np.abs(b).max()
         e2 = np.abs(ea - eb).max() / np.abs(eb).max()
         print(f"t={t} u max-rel {e1:.2e} ue max-rel {e2:.2e}")
-        assert e1 <= 5e-4 and e2 <= 5e-4, (t, e1, e2)
+        assert e1 <= (3e-3 if t < 200 else 6e-4) and e2 <= 5e-4, (t, e1, e2)
