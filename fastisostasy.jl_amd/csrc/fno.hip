@@ -34,7 +34,9 @@ constexpr int GEMM_THREADS = 256;   // 4 waves; block tile 64 (M) x 32 (N)
 
 // C[b] = alpha A[b] B[b] + beta C[b] with element (m, k) of A[b] at A + b sAb + m sAm + k sAk
 // (likewise B, C). With kchunk > 0, batch b is the K-slice [b kchunk, (b+1) kchunk) of one product
-// and writes its partial into C + b sCb (split-K; reduced by k_splitk_reduce).
+// and writes its partial into C + b sCb (split-K; reduced by k_splitk_reduce). With nslice > 0 as
+// well, grid batch z is slice (z mod nslice) of product (z / nslice) (A and B batch strides apply
+// to the product), its partial at C + z sCb (batched split-K; reduced by k_splitk_reduce_batched).
 struct Gemm {
     const float* A;
     const float* B;
@@ -42,6 +44,7 @@ struct Gemm {
     long sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn;
     int M, N, K, kchunk;
     float alpha, beta;
+    int nslice;
 };
 
 // Wave w of the block owns rows m0 + 16 w .. +15 and both 16-column halves of the block's 32
@@ -57,7 +60,13 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
     int kbeg = 0, kend = g.K;
     const float* A = g.A + (long)b * g.sAb;
     const float* B = g.B + (long)b * g.sBb;
-    if (g.kchunk > 0) {
+    if (g.nslice > 0) {
+        const int pb = b / g.nslice, sl = b - pb * g.nslice;
+        kbeg = sl * g.kchunk;
+        kend = min(g.K, kbeg + g.kchunk);
+        A = g.A + (long)pb * g.sAb;
+        B = g.B + (long)pb * g.sBb;
+    } else if (g.kchunk > 0) {
         kbeg = b * g.kchunk;
         kend = min(g.K, kbeg + g.kchunk);
         A = g.A;
@@ -116,6 +125,28 @@ __global__ void __launch_bounds__(256) k_splitk_reduce(const float* __restrict__
         const int m = out / N, n = out - m * N;
         float* d = dst + (long)m * sm + (long)n * sn;
         *d = beta * *d + s;
+    }
+}
+
+// dst[p](m, n) = beta dst[p](m, n) + sum_s part[p][s][m][n] for products p: one wave per output,
+// lanes over the slices, fixed-order butterfly (deterministic)
+__global__ void __launch_bounds__(256) k_splitk_reduce_batched(const float* __restrict__ part, int nslice, int M,
+                                                               int N, int nprod, float* dst, long sp, long sm,
+                                                               long sn, float beta) {
+    const long out = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const long MN = (long)M * N;
+    if (out >= MN * nprod) return;
+    const long pb = out / MN, mn = out - pb * MN;
+    const float* pp = part + pb * nslice * MN + mn;
+    float s = 0.f;
+    for (int sl = lane; sl < nslice; sl += 64) s += pp[(long)sl * MN];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        const int m = (int)(mn / N), n = (int)(mn - (long)m * N);
+        float* d = dst + pb * sp + (long)m * sm + (long)n * sn;
+        *d = (beta != 0.f) ? beta * *d + s : s;
     }
 }
 
@@ -307,10 +338,34 @@ int sfail(fi_surrogate* s, int code, const std::string& m) {
 
 constexpr int S_ERR_INVALID = -1, S_ERR_STATE = -6;
 
-// plain product
+// plain product. A batch of few output tiles over a long K (the y transforms per retained kx: 16
+// products of one 64 x 32 tile over K = ny) is split along K so that the launch fills the device,
+// then reduced in a fixed order.
 int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, long sBk, long sBn, float* C, long sCm,
          long sCn, int M, int N, int K, float beta, int batch = 1, long sAb = 0, long sBb = 0, long sCb = 0) {
-    Gemm g{A, B, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta};
+    const long tiles = (long)((M + 63) / 64) * ((N + 31) / 32);
+    if (batch > 1 && tiles * batch < 128 && K >= 128) {
+        const int steps = (K + 31) / 32;
+        const int want = (int)std::max<long>(1, 256 / (tiles * batch));
+        const int nslice0 = std::min(steps, want);
+        const int kchunk = 32 * ((steps + nslice0 - 1) / nslice0);
+        const int nslice = (K + kchunk - 1) / kchunk;
+        const long MN = (long)M * N, need = MN * nslice * batch;
+        if (need > s->part_cap) {
+            SCHECK(s, s->part.alloc(need));
+            s->part_cap = need;
+        }
+        Gemm g{A, B, s->part.p, sAb, sAm, sAk, sBb, sBk, sBn, MN, N, 1, M, N, K, kchunk, 1.f, 0.f, nslice};
+        dim3 grid((M + 63) / 64, (N + 31) / 32, batch * nslice);
+        hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        SCHECK(s, hipGetLastError());
+        const long outs = MN * batch;
+        hipLaunchKernelGGL(k_splitk_reduce_batched, dim3((unsigned)((outs + 3) / 4)), dim3(256), 0, s->stream, s->part.p,
+                           nslice, M, N, batch, C, sCb, sCm, sCn, beta);
+        SCHECK(s, hipGetLastError());
+        return 0;
+    }
+    Gemm g{A, B, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta, 0};
     dim3 grid((M + 63) / 64, (N + 31) / 32, batch);
     hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
     SCHECK(s, hipGetLastError());
@@ -327,7 +382,7 @@ int gemm_splitk(fi_surrogate* s, const float* A, long sAm, long sAk, const float
         SCHECK(s, s->part.alloc(need));
         s->part_cap = need;
     }
-    Gemm g{A, B, s->part.p, 0, sAm, sAk, 0, sBk, sBn, (long)M * N, N, 1, M, N, K, kchunk, 1.f, 0.f};
+    Gemm g{A, B, s->part.p, 0, sAm, sAk, 0, sBk, sBn, (long)M * N, N, 1, M, N, K, kchunk, 1.f, 0.f, 0};
     dim3 grid((M + 63) / 64, (N + 31) / 32, nsplit);
     hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
     SCHECK(s, hipGetLastError());
