@@ -128,22 +128,18 @@ __global__ void __launch_bounds__(256) k_splitk_reduce(const float* __restrict__
     }
 }
 
-// dst[p](m, n) = beta dst[p](m, n) + sum_s part[p][s][m][n] for products p: one wave per output,
-// lanes over the slices, fixed-order butterfly (deterministic)
+// dst[p](m, n) = beta dst[p](m, n) + sum_s part[p][s][m][n] for products p: one thread per output
+// (consecutive threads read consecutive partials), slices summed in order (deterministic)
 __global__ void __launch_bounds__(256) k_splitk_reduce_batched(const float* __restrict__ part, int nslice, int M,
                                                                int N, int nprod, float* dst, long sp, long sm,
                                                                long sn, float beta) {
-    const long out = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
     const long MN = (long)M * N;
-    if (out >= MN * nprod) return;
-    const long pb = out / MN, mn = out - pb * MN;
-    const float* pp = part + pb * nslice * MN + mn;
-    float s = 0.f;
-    for (int sl = lane; sl < nslice; sl += 64) s += pp[(long)sl * MN];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) {
+    for (long out = (long)blockIdx.x * blockDim.x + threadIdx.x; out < MN * nprod;
+         out += (long)gridDim.x * blockDim.x) {
+        const long pb = out / MN, mn = out - pb * MN;
+        const float* pp = part + pb * nslice * MN + mn;
+        float s = 0.f;
+        for (int sl = 0; sl < nslice; ++sl) s += pp[(long)sl * MN];
         const int m = (int)(mn / N), n = (int)(mn - (long)m * N);
         float* d = dst + pb * sp + (long)m * sm + (long)n * sn;
         *d = (beta != 0.f) ? beta * *d + s : s;
@@ -360,8 +356,8 @@ int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, lo
         hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
         SCHECK(s, hipGetLastError());
         const long outs = MN * batch;
-        hipLaunchKernelGGL(k_splitk_reduce_batched, dim3((unsigned)((outs + 3) / 4)), dim3(256), 0, s->stream, s->part.p,
-                           nslice, M, N, batch, C, sCb, sCm, sCn, beta);
+        hipLaunchKernelGGL(k_splitk_reduce_batched, dim3((unsigned)std::min<long>((outs + 255) / 256, 4096)), dim3(256),
+                           0, s->stream, s->part.p, nslice, M, N, batch, C, sCb, sCm, sCn, beta);
         SCHECK(s, hipGetLastError());
         return 0;
     }
