@@ -335,14 +335,15 @@ int sfail(fi_surrogate* s, int code, const std::string& m) {
 constexpr int S_ERR_INVALID = -1, S_ERR_STATE = -6;
 
 // plain product. A batch of few output tiles over a long K (the y transforms per retained kx: 16
-// products of one 64 x 32 tile over K = ny) is split along K so that the launch fills the device,
+// products of one 64 x 32 tile over K = ny; the x transforms per row: ny products over K = nx) is
+// split along K so that the launch has about 1024 workgroups (the K loop is load-latency bound),
 // then reduced in a fixed order.
 int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, long sBk, long sBn, float* C, long sCm,
          long sCn, int M, int N, int K, float beta, int batch = 1, long sAb = 0, long sBb = 0, long sCb = 0) {
     const long tiles = (long)((M + 63) / 64) * ((N + 31) / 32);
-    if (batch > 1 && tiles * batch < 128 && K >= 128) {
+    if (batch > 1 && tiles * batch < 1024 && K >= 128) {
         const int steps = (K + 31) / 32;
-        const int want = (int)std::max<long>(1, 256 / (tiles * batch));
+        const int want = (int)std::max<long>(1, 1024 / (tiles * batch));
         const int nslice0 = std::min(steps, want);
         const int kchunk = 32 * ((steps + nslice0 - 1) / nslice0);
         const int nslice = (K + kchunk - 1) / kchunk;
