@@ -372,7 +372,7 @@ int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, lo
 // product with a long K (parameter gradients): split-K partials, then a fixed-order reduction
 int gemm_splitk(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, long sBk, long sBn, float* C,
                 long sCm, long sCn, int M, int N, int K, float beta) {
-    const int kchunk = 256;   // ~480 slices over the 122 500 cells of an N=350 grid
+    const int kchunk = 128;   // ~960 slices over the 122 500 cells of an N=350 grid
     const int nsplit = (K + kchunk - 1) / kchunk;
     const long need = (long)nsplit * M * N;
     if (need > s->part_cap) {
