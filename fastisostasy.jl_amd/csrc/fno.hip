@@ -47,6 +47,27 @@ struct Gemm {
     int nslice;
 };
 
+// The 8 K values k .. k+7 of one operand row or column: two 16-byte loads when the operand is
+// contiguous in K, the row is in range, all 8 values are inside the K range and the address is
+// 16-byte aligned; element loads (zero outside the ranges) otherwise.
+__device__ __forceinline__ void load8(const float* p, long sk, int k, int kend, bool ok, float (&v)[8]) {
+    if (ok && sk == 1 && k + 8 <= kend && ((reinterpret_cast<unsigned long long>(p + k) & 15ull) == 0)) {
+        const float4 lo = *reinterpret_cast<const float4*>(p + k);
+        const float4 hi = *reinterpret_cast<const float4*>(p + k + 4);
+        v[0] = lo.x;
+        v[1] = lo.y;
+        v[2] = lo.z;
+        v[3] = lo.w;
+        v[4] = hi.x;
+        v[5] = hi.y;
+        v[6] = hi.z;
+        v[7] = hi.w;
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (ok && k + e < kend) ? p[(long)(k + e) * sk] : 0.f;
+}
+
 // Wave w of the block owns rows m0 + 16 w .. +15 and both 16-column halves of the block's 32
 // columns: two 16x16x32 MFMAs per 32-wide K step. Lane l feeds row/column (l mod 16) with the 8
 // K values 8 (l / 16) .. +7 (the same K assignment for A and B), and receives rows
@@ -81,16 +102,16 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
     const float* B1 = B + (long)bn1 * g.sBn;
     for (int k0 = kbeg; k0 < kend; k0 += 32) {
         bf16x8 a, x0, x1;
+        float av[8], v0[8], v1[8];
+        const int kk = k0 + 8 * kg;
+        load8(Ar, g.sAk, kk, kend, aok, av);
+        load8(B0, g.sBk, kk, kend, b0ok, v0);
+        load8(B1, g.sBk, kk, kend, b1ok, v1);
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int k = k0 + 8 * kg + e;
-            const bool kok = k < kend;
-            const float av = (aok && kok) ? Ar[(long)k * g.sAk] : 0.f;
-            const float v0 = (b0ok && kok) ? B0[(long)k * g.sBk] : 0.f;
-            const float v1 = (b1ok && kok) ? B1[(long)k * g.sBk] : 0.f;
-            a[e] = (__bf16)av;
-            x0[e] = (__bf16)v0;
-            x1[e] = (__bf16)v1;
+            a[e] = (__bf16)av[e];
+            x0[e] = (__bf16)v0[e];
+            x1[e] = (__bf16)v1[e];
         }
         acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x0, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x1, acc1, 0, 0, 0);
