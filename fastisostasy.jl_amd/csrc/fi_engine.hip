@@ -330,7 +330,7 @@ struct BatchArgs {
 // one member of the ensemble kernel. The parameter block is passed as a restrict pointer: nothing
 // the integration writes aliases it, so its fields are read with scalar loads (as in the kernel of
 // one context) instead of being reloaded into vector registers after every store.
-template <typename MT, bool DIST>
+template <typename MT, bool DIST, bool ELRA = false>
 __device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, const Team& tm, Ctl& s, unsigned& gen,
                                              char* smem_all, double* red, int* flag, unsigned long long& tl,
                                              int fsal) {
@@ -343,10 +343,10 @@ __device__ __forceinline__ bool batch_member(const ConstKP* __restrict__ pp, con
     __syncthreads();
     const bool todo = s.t < s.stop && !s.err;
     if (todo) {
-        integrate_loop<MT, DIST, false>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
+        integrate_loop<MT, DIST, ELRA>(p, tm, s, gen, twx, twy, smem, red, flag, tl);
         if (*flag) return false;
         if (fsal && !s.err && s.t >= s.stop) {
-            if (!single_rhs<MT, DIST, false>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
+            if (!single_rhs<MT, DIST, ELRA>(p, tm, s, gen, twx, twy, smem, red, flag, tl, s.cur, s.k1, s.t, false, true)) {
                 if (*flag) return false;
             }
             if (threadIdx.x == 0) s.last_u = s.cur;
@@ -389,6 +389,38 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __res
     }
 }
 
+// the ensemble kernel for relaxed-mantle (ELRA) members: the same queue and teams as
+// fi_batch_persistent (its body is repeated, as for the single-context kernels, so that the
+// LV-ELVA instances keep their code), each member integrated by the ELRA loop
+__global__ void __launch_bounds__(NT) fi_batch_persistent_elra(const BatchArgs* __restrict__ ba) {
+    extern __shared__ __align__(16) char smem_all[];
+    __shared__ Ctl s;
+    __shared__ double red[64];
+    __shared__ int flag;
+    __shared__ int s_m;
+    const int team = (int)blockIdx.x % ba->nteams;
+    const Team tm{(int)blockIdx.x / ba->nteams, ba->tsize, ba->bars + (long)team * (1 + 2 * ba->tsize)};
+    unsigned gen = 0;
+    unsigned long long tl = wall_clock64();
+    for (;;) {
+        if (tm.rank == 0 && threadIdx.x == 0) {
+            const int q = atomicAdd(ba->queue, 1);
+            __hip_atomic_store(&ba->team_member[team], q < ba->nmem ? ba->order[q] : -1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!team_sync(tm, gen, &flag, ba->spin_limit)) return;
+        if (threadIdx.x == 0)
+            s_m = __hip_atomic_load(&ba->team_member[team], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int m = __builtin_amdgcn_readfirstlane(s_m);
+        if (m < 0) return;
+        const unsigned long long pk =
+            ((const __attribute__((address_space(1))) unsigned long long*)(unsigned long long)ba->kps)[m];
+        if (!batch_member<double, false, true>((const ConstKP*)pk, tm, s, gen, smem_all, red, &flag, tl, ba->fsal))
+            return;
+    }
+}
+
 // kernel instances: FP64 or FP32 moments; with or without per-cell grid steps (map distortion,
 // FP64-moment path only)
 inline const void* persistent_kernel(bool mt_double, bool dist, bool elra, bool implicit = false) {
@@ -397,7 +429,8 @@ inline const void* persistent_kernel(bool mt_double, bool dist, bool elra, bool 
     if (!mt_double) return (const void*)fi_persistent<float, false>;
     return dist ? (const void*)fi_persistent<double, true> : (const void*)fi_persistent<double, false>;
 }
-inline const void* batch_kernel(bool mt_double, bool dist) {
+inline const void* batch_kernel(bool mt_double, bool dist, bool elra = false) {
+    if (elra) return (const void*)fi_batch_persistent_elra;
     if (!mt_double) return (const void*)fi_batch_persistent<float, false>;
     return dist ? (const void*)fi_batch_persistent<double, true> : (const void*)fi_batch_persistent<double, false>;
 }
@@ -1799,6 +1832,7 @@ struct fi_batch {
     std::vector<double> cand_rate;
     int last_tsize = 0, last_nteams = 0;
     bool dist = false;           // the current launch uses the kernel with per-cell grid steps
+    bool elra = false;           // the members are relaxed-mantle (ELRA) contexts
     double device_ms = 0, cell_attempts = 0;
     long long n_launch = 0;
     DBuf<const KP*> kps;
@@ -1868,7 +1902,7 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     BHIPCHECK(b, hipMemcpyAsync(b->args.p, &a, sizeof(a), hipMemcpyHostToDevice, b->stream));
     const BatchArgs* ap = b->args.p;
     void* kargs[] = {&ap};
-    const void* f = batch_kernel(mt_double, b->dist);
+    const void* f = batch_kernel(mt_double, b->dist, b->elra);
     BHIPCHECK(b, hipEventRecord(b->ev0, b->stream));
     BHIPCHECK(b, hipLaunchCooperativeKernel(f, dim3(nteams * ts), dim3(NT), kargs, (unsigned)lds, b->stream));
     BHIPCHECK(b, hipEventRecord(b->ev1, b->stream));
@@ -1918,9 +1952,13 @@ int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out) {
             fi_batch_destroy(b);
             return FI_ERR_INVALID;  // one device per ensemble
         }
-        if (cfgs[m].mantle != FI_MAXWELL_MANTLE || cfgs[m].lithosphere != FI_LATERALLY_VARIABLE_LITHOSPHERE) {
+        // one kernel per ensemble: every member LV-ELVA, or every member ELRA
+        const bool lv = cfgs[m].mantle == FI_MAXWELL_MANTLE && cfgs[m].lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE;
+        const bool elra = cfgs[m].mantle == FI_RELAXED_MANTLE;
+        const bool elra0 = cfgs[0].mantle == FI_RELAXED_MANTLE;
+        if (!(lv || elra) || elra != elra0) {
             fi_batch_destroy(b);
-            return FI_ERR_UNSUPPORTED;  // the ensemble kernel runs the LV-ELVA RHS only
+            return FI_ERR_UNSUPPORTED;
         }
         fi_ctx* c = nullptr;
         const int st = fi_create(&cfgs[m], &c);
@@ -1999,14 +2037,21 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
         lds_f = std::max(lds_f, c->lds_f);
         dist = dist || c->has_dist;
     }
-    if (dist && !mt_double)
+    const bool elra = b->mem[0]->elra;
+    if (dist && !mt_double && !elra)
         return bfail(b, FI_ERR_UNSUPPORTED, "map distortion needs every member on the FP64-moment path");
-    // with a distorted member every member runs the distortion kernel (K = 1 where unset)
-    const size_t lds = !mt_double ? lds_f : dist ? lds_dd : lds_d;
+    // with a distorted member every member runs the distortion kernel (K = 1 where unset); ELRA
+    // members read K^2 at run time and need the padded-grid phases' LDS only
+    size_t lds = !mt_double ? lds_f : dist ? lds_dd : lds_d;
+    if (elra) {
+        lds = 0;
+        for (int m = 0; m < nm; ++m) lds = std::max(lds, b->mem[m]->lds);
+    }
+    b->elra = elra;
     if (lds > 160 * 1024) return bfail(b, FI_ERR_UNSUPPORTED, "members too large for one workgroup's LDS");
     b->dist = dist;
     BHIPCHECK(b, hipSetDevice(b->dev));
-    const void* f = batch_kernel(mt_double, dist);
+    const void* f = batch_kernel(mt_double, dist, elra);
     BHIPCHECK(b, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0, ncu = 0;
     BHIPCHECK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, lds));

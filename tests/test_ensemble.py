@@ -171,3 +171,63 @@ def test_ensemble_mixing_distorted_members_equals_single_contexts_bitwise():
         e.integrator_init()
         e.integrate(T_OUT[-1], lambda t, k: None)
         np.testing.assert_array_equal(res[m][0], e.get_field("u"), err_msg=f"member {m}")
+
+
+# ---- relaxed-mantle (ELRA) ensembles ------------------------------------------------------------
+def elra_ensemble(sizes=(32, 40, 48, 36), t_end=2000.0):
+    import fastiso
+    out = []
+    for m, n in enumerate(sizes):
+        w = workloads.elra(n=n, W=3e6, t_end=20e3, tau=2000.0 + 500.0 * m, lithosphere="constant",
+                           t_out=[])
+        w["sealevel"] = fastiso.RegionalSeaLevel(surface=fastiso.LaterallyVariableSeaSurface())
+        out.append(sim_inputs(w).inputs)
+    return out
+
+
+def test_oracle_elra_ensemble_equals_single_contexts():
+    from oracle import pyref
+    inputs = elra_ensemble()
+    ens = pyref.RefEnsemble(inputs)
+    ens.integrator_init()
+    ens.integrate(600.0)
+    for m, inp in enumerate(inputs):
+        e = pyref.RefEngine.from_inputs(with_tout(inp, [600.0]))
+        e.integrator_init()
+        e.integrate(600.0, lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("team_size", [2, 0])
+def test_gpu_elra_ensemble_members_equal_single_contexts_bitwise(team_size):
+    """Relaxed-mantle members (uneven sides, their own tau) in one ensemble kernel: each equals its
+    single-context run bit for bit, whatever the team size; and the oracle's ensemble to 1e-4."""
+    from oracle import pyref
+    inputs = elra_ensemble()
+    ens = hip_ensemble(inputs, team_size)
+    ens.integrator_init()
+    ens.integrate(300.0)
+    ens.integrate(600.0)
+    ref = pyref.RefEnsemble(inputs)
+    ref.integrator_init()
+    ref.integrate(300.0)
+    ref.integrate(600.0)
+    for m, inp in enumerate(inputs):
+        e = hip_single(with_tout(inp, [300.0, 600.0]))
+        e.integrator_init()
+        e.integrate(600.0, lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m}")
+        np.testing.assert_array_equal(ens[m].get_field("u_eq"), e.get_field("u_eq"), err_msg=f"member {m}")
+        a, b = ens[m].get_field("u"), ref[m].get_field("u")
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), m
+
+
+@pytest.mark.gpu
+def test_gpu_ensemble_refuses_mixed_rheologies():
+    if DRYRUN:
+        pytest.skip("the oracle's ensemble runs any member")
+    import fastiso
+    mixed = [elra_ensemble(sizes=(32,))[0], small_ensemble(nmem=1)[0]]
+    with pytest.raises(fastiso.EngineError):
+        fastiso.Ensemble(mixed)
