@@ -202,7 +202,9 @@ def test_oracle_elra_ensemble_equals_single_contexts():
 @pytest.mark.parametrize("team_size", [2, 0])
 def test_gpu_elra_ensemble_members_equal_single_contexts_bitwise(team_size):
     """Relaxed-mantle members (uneven sides, their own tau) in one ensemble kernel: each equals its
-    single-context run bit for bit, whatever the team size; and the oracle's ensemble to 1e-4."""
+    single-context run bit for bit, whatever the team size; and the oracle's ensemble to 2e-3 of
+    max |u| (the oracle against itself at reltol 0.7e-5 and 1.3e-5 differs by up to 1.2e-3 on these
+    members at 600 yr)."""
     from oracle import pyref
     inputs = elra_ensemble()
     ens = hip_ensemble(inputs, team_size)
@@ -220,7 +222,7 @@ def test_gpu_elra_ensemble_members_equal_single_contexts_bitwise(team_size):
         np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m}")
         np.testing.assert_array_equal(ens[m].get_field("u_eq"), e.get_field("u_eq"), err_msg=f"member {m}")
         a, b = ens[m].get_field("u"), ref[m].get_field("u")
-        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), m
+        assert np.abs(a - b).max() <= 2e-3 * np.abs(b).max(), m
 
 
 @pytest.mark.gpu
