@@ -14,7 +14,8 @@ unscented transform is exact:
 Step-by-step parity with EnsembleKalmanProcesses itself is unpinned.
 
 The driver then recovers known upper-mantle viscosities from synthetic displacement data of the
-disc benchmark: with the oracle's ensemble on the CPU and with the HIP ensemble on the GPU.
+disc benchmark, and a known ELRA relaxation time from an ELRA ensemble: with the oracle's
+ensemble on the CPU and with the HIP ensemble on the GPU.
 """
 import numpy as np
 import pytest
@@ -122,4 +123,57 @@ def test_gpu_ensemble_inversion_recovers_two_viscosities():
     prob.solve()
     p, Gx, err = prob.extract_inversion()
     assert np.abs(p - np.array([21.0, 20.6])).max() < 0.03, p
+    assert prob.error[-1] < 1e-2 * prob.error[0], prob.error
+
+
+class _LogTauReduction(inversion.ParameterReduction):
+    """log10 of a uniform ELRA relaxation time (the parameter of the reference's ELRA examples)."""
+
+    nparams = 1
+
+    def __init__(self, n=32):
+        self.n = n
+
+    def reconstruct(self, params):
+        w = workloads.elra(n=self.n, W=3e6, t_end=20e3, tau=10.0 ** float(params[0]), lithosphere="rigid",
+                           t_out=[])
+        return fastiso.Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
+                                  nout=w["nout"], opts=w["opts"], c=w["c"], engine=None).inputs
+
+
+def _tau_problem(make_batch, truth=np.log10(3000.0), t=(500.0, 1000.0, 2000.0)):
+    red = _LogTauReduction()
+    ens = make_batch([red.reconstruct([truth])])
+    ens.integrator_init()
+    Y = []
+    for tk in t:
+        ens.integrate(tk)
+        Y.append(ens[0].get_field("u"))
+    close = getattr(ens, "close", None)
+    if close:
+        close()
+    d = fastiso.RegionalDomain(3e6, 3e6, 32, 32)
+    data = inversion.InversionData(list(t), Y, d.R < 1.5e6)
+    prior = inversion.GaussianPrior([np.log10(2000.0)], [0.3])
+    cfg = inversion.InversionConfig("unscented", N_iter=4, alpha_reg=1.0, update_freq=1, scale_obscov=1e-6)
+    return inversion.inversion_problem(cfg, data, red, prior, make_batch=make_batch)
+
+
+def test_oracle_elra_ensemble_inversion_recovers_tau():
+    from oracle import pyref
+    prob = _tau_problem(pyref.RefEnsemble).solve()
+    p, _, _ = prob.extract_inversion()
+    assert abs(p[0] - np.log10(3000.0)) < 0.01, p
+
+
+@pytest.mark.gpu
+def test_gpu_elra_ensemble_inversion_recovers_tau():
+    if DRYRUN:
+        from oracle import pyref
+        make = pyref.RefEnsemble
+    else:
+        make = fastiso.Ensemble
+    prob = _tau_problem(make).solve()
+    p, _, _ = prob.extract_inversion()
+    assert abs(p[0] - np.log10(3000.0)) < 0.01, p
     assert prob.error[-1] < 1e-2 * prob.error[0], prob.error
