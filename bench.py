@@ -24,7 +24,9 @@ and the roofline counts ELRA_BYTES_PER_CELL_RHS per cell per RHS call. lv_elva_h
 high-resolution instance (W = 3000 km,
 dx = 5.86 km, field seeds 5001/5002); ensemble is the uneven ensemble of 256 members with sides
 drawn from {64, 96, 128, 192, 256}, advanced together by fi_batch_integrate (value = member-kyr/s,
-the sum over members of simulated kyr per wall-clock second); surrogate is the N=350 workload run
+the sum over members of simulated kyr per wall-clock second); elra_ensemble is 256 ELRA members
+with the same sides, each with its own relaxation-time anomaly (workloads.elra_ensemble), in 1-kyr
+windows; surrogate is the N=350 workload run
 in 500-yr macro steps by the trained surrogate with a plain-solver check every 10 macro steps and
 fallback above 1 m (--validate adds the plain run of the same window for the error).
 """
@@ -76,10 +78,10 @@ def make_sim(n, engine, hr=False, elra=False, implicit=False):
                                  nout=w["nout"], opts=w["opts"], c=w["c"], engine=engine)
 
 
-def ensemble_inputs(nmem):
+def ensemble_inputs(nmem, elra=False):
     import fastiso
     from fastiso import workloads
-    ws = workloads.ensemble(nmem=nmem, t_end=20e3)
+    ws = workloads.elra_ensemble(nmem=nmem) if elra else workloads.ensemble(nmem=nmem, t_end=20e3)
     return [fastiso.Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
                                nout=w["nout"], opts=w["opts"], c=w["c"], engine=None).inputs for w in ws]
 
@@ -87,7 +89,8 @@ def ensemble_inputs(nmem):
 def bench_ensemble(a):
     import fastiso
     t0 = time.perf_counter()
-    inputs = ensemble_inputs(a.members)
+    elra = a.workload == "elra_ensemble"
+    inputs = ensemble_inputs(a.members, elra)
     ens = fastiso.Ensemble(inputs, team_size=a.team_size)
     ens.integrator_init()
     setup_s = time.perf_counter() - t0
@@ -99,6 +102,7 @@ def bench_ensemble(a):
         ens.integrate(t, fsal=False)
         print(f"warm-up window {k + 1}/{a.warmup} done: {ens.info()}", file=sys.stderr, flush=True)
     acc0 = [ens[m].stats()["n_accept"] for m in range(len(inputs))]
+    rhs0 = [ens[m].stats()["n_rhs"] for m in range(len(inputs))]
     i0 = ens.info()
     t0 = time.perf_counter()
     for k in range(a.steps):
@@ -108,13 +112,19 @@ def bench_ensemble(a):
     el = time.perf_counter() - t0
     i1 = ens.info()
     acc = [ens[m].stats()["n_accept"] - acc0[m] for m in range(len(inputs))]
+    rhs = [ens[m].stats()["n_rhs"] - rhs0[m] for m in range(len(inputs))]
     cell_steps = float(sum(c * n for c, n in zip(cells, acc)))
     dev_s = (i1["device_ms"] - i0["device_ms"]) / 1e3
     member_kyr = len(inputs) * a.steps * a.window / 1e3
-    achieved = BYTES_PER_CELL_STEP * cell_steps / dev_s / 1e9 if dev_s > 0 else None
+    if elra:
+        alg_bytes = ELRA_BYTES_PER_CELL_RHS * float(sum(c * n for c, n in zip(cells, rhs)))
+    else:
+        alg_bytes = BYTES_PER_CELL_STEP * cell_steps
+    achieved = alg_bytes / dev_s / 1e9 if dev_s > 0 else None
     sizes = sorted(set(inp["cfg"].nx for inp in inputs))
+    kind = "ELRA" if elra else "LV-ELVA"
     return {
-        "metric": "simulated member-kyr/s, 256-member uneven ensemble (plain solver)",
+        "metric": f"simulated member-kyr/s, {len(inputs)}-member uneven {kind} ensemble (plain solver)",
         "value": member_kyr / el,
         "unit": "member-kyr/s",
         "n_gpus": 1,
@@ -126,9 +136,9 @@ def bench_ensemble(a):
         "vs_baseline": None,
         "dtype": "f32 fft/stencil-moments, f64 state/stencil/error-norm",
         "data": "synthetic (seeded fields)",
-        "config": {"workload": "ensemble", "members": len(inputs), "sizes": sizes, "total_cells": int(sum(cells)),
+        "config": {"workload": a.workload, "members": len(inputs), "sizes": sizes, "total_cells": int(sum(cells)),
                    "window_yr": a.window, "integrator": "BS3 reltol 1e-5 abstol 1e-6",
-                   "accepted_steps_total": int(sum(acc)), "cell_steps": cell_steps,
+                   "accepted_steps_total": int(sum(acc)), "rhs_calls_total": int(sum(rhs)), "cell_steps": cell_steps,
                    "team_size": i1["team_size"], "n_teams": i1["n_teams"], "autotuned": a.team_size == 0,
                    "kernel_launches": int(i1["n_launch"] - i0["n_launch"]), "setup_s": setup_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -272,7 +282,8 @@ def main():
     ap.add_argument("--window", type=float, default=None, help="window in yr (default 10; 5 for lv_elva_hr)")
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "surrogate", "fno_train"],
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "elra_ensemble", "surrogate",
+                                               "fno_train"],
                     default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=20.0)
@@ -283,17 +294,18 @@ def main():
     hr_defaults = a.workload == "lv_elva_hr"
     elra = a.workload == "elra"
     implicit = a.workload == "implicit"
+    elra_ens = a.workload == "elra_ensemble"
     if a.steps is None:
-        a.steps = 3 if hr_defaults else 50 if (elra or implicit) else 200
+        a.steps = 3 if hr_defaults else 10 if elra_ens else 50 if (elra or implicit) else 200
     if a.warmup is None:
-        a.warmup = 1 if hr_defaults else 2 if (elra or implicit) else 5
+        a.warmup = 1 if hr_defaults else 8 if elra_ens else 2 if (elra or implicit) else 5
     if a.window is None:
-        a.window = 5.0 if hr_defaults else 100.0 if elra else 1000.0 if implicit else 10.0
+        a.window = 5.0 if hr_defaults else 100.0 if elra else 1000.0 if (implicit or elra_ens) else 10.0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    if a.workload == "ensemble":
+    if a.workload in ("ensemble", "elra_ensemble"):
         print(json.dumps(bench_ensemble(a)), flush=True)
         return
     if a.workload == "surrogate":

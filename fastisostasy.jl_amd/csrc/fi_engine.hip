@@ -1830,6 +1830,9 @@ struct fi_batch {
     size_t cand_next = 0;        // next candidate team size under trial
     std::vector<int> cands;
     std::vector<double> cand_rate;
+    // trials time whole windows: set once a window's first (capped) trial launch ended with most
+    // members at their window end, i.e. the windows are shorter than TUNE_ATTEMPTS
+    bool tune_window = false;
     int last_tsize = 0, last_nteams = 0;
     bool dist = false;           // the current launch uses the kernel with per-cell grid steps
     bool elra = false;           // the members are relaxed-mantle (ELRA) contexts
@@ -1858,19 +1861,23 @@ int bfail(fi_batch* b, int code, const std::string& m) {
 // attempts per member per trial launch while the team size is being chosen; pausing a member on
 // an attempt count leaves its step sequence unchanged
 constexpr int TUNE_ATTEMPTS = 64;
+// share of a trial's capped work (cells x attempts) that must run for the trial to be timed
+constexpr double TUNE_CAPPED_SHARE = 0.9;
 // attempts per member per launch otherwise (bounds one launch's duration)
 constexpr int LAUNCH_ATTEMPTS = 1 << 20;
 
 // one persistent ensemble launch with teams of ts workgroups; returns the device time
 int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int max_steps, int fsal,
-                 const std::vector<double>& t_target, float* ms_out, double* work_out) {
+                 const std::vector<double>& t_target, float* ms_out, double* work_out, double* cap_share_out) {
     const int nm = (int)b->mem.size();
     const int nteams = std::max(1, gmax / ts);
     std::vector<KP> kph(nm);
     std::vector<const KP*> ptrs(nm);
     std::vector<long long> att0(nm);
+    double cap_work = 0;  // cells x attempts if every member still short of its target used the cap
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
+        if (c->ctl.t < t_target[m]) cap_work += (double)max_steps * (double)c->N;
         c->ctl.stop = t_target[m];
         c->ctl.next_out = t_target[m];
         att0[m] = c->ctl.nacc + c->ctl.nrej;
@@ -1926,6 +1933,7 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     b->last_nteams = nteams;
     *ms_out = ms;
     *work_out = work;
+    *cap_share_out = cap_work > 0 ? work / cap_work : 0.0;
     for (int t = 0; t < nteams; ++t)
         if (hb[(size_t)t * (1 + 2 * (size_t)ts)].abort) return bfail(b, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
     for (int m = 0; m < nm; ++m) {
@@ -2003,6 +2011,7 @@ int fi_batch_set_team_size(fi_batch* b, int32_t ts) {
     b->tsize_fixed = ts;
     b->tsize_best = 0;
     b->cand_next = 0;
+    b->tune_window = false;
     b->cands.clear();
     b->cand_rate.clear();
     return FI_OK;
@@ -2076,6 +2085,7 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
             if (b->mem[m]->ctl.t < t_target[m]) return false;
         return true;
     };
+    bool first_launch = true;
     while (!done()) {
         int ts;
         bool trial = false;
@@ -2086,11 +2096,23 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
             trial = true;
         }
         float ms = 0;
-        double work = 0;
-        int st = batch_launch(b, mt_double, lds, gmax, ts, trial ? TUNE_ATTEMPTS : LAUNCH_ATTEMPTS, fsal, t_target,
-                              &ms, &work);
+        double work = 0, cap_share = 0;
+        const int cap = trial && !b->tune_window ? TUNE_ATTEMPTS : LAUNCH_ATTEMPTS;
+        int st = batch_launch(b, mt_double, lds, gmax, ts, cap, fsal, t_target, &ms, &work, &cap_share);
         if (st) return st;
-        if (trial) {
+        const bool window_start = first_launch;
+        const bool whole_window = first_launch && done();
+        first_launch = false;
+        // A capped trial in which members reached their window end timed the window's tail: its
+        // rate is not comparable with the other candidates'. One that straddled a window end is
+        // run again at the next launch; when even a window's first launch falls short, the windows
+        // are shorter than the cap and every candidate is timed over whole windows instead.
+        bool timed = trial && (b->tune_window ? whole_window : cap_share >= TUNE_CAPPED_SHARE);
+        if (trial && !b->tune_window && !timed && window_start) {
+            b->tune_window = true;
+            b->cand_next = 0;
+        }
+        if (timed) {
             b->cand_rate[b->cand_next] = ms > 0 ? work / ms : 0.0;
             b->cand_next += 1;
             if (b->cand_next == b->cands.size()) {

@@ -131,3 +131,25 @@ def ensemble(nmem=256, W=3e6, t_end=20e3, sizes=(64, 96, 128, 192, 256), member_
         w["member"] = m
         out.append(w)
     return out
+
+
+def elra_ensemble(nmem=256, W=3e6, t_end=50e3, sizes=(64, 96, 128, 192, 256), member_seed=2000):
+    """Uneven ensemble of ELRA members: member m has side n_m drawn uniformly from sizes (seed
+    member_seed + m) and the 2D relaxation time get_relaxation_time_weaker of a viscosity
+    10^(21 - a_m g), g the Gaussian of elra(tau="weak") and a_m uniform in [0.5, 1.5] from the
+    same seed; the ice cylinder of elra, a laterally constant 88 km lithosphere with the elastic
+    response, and the laterally variable sea surface."""
+    from .solidearth import get_relaxation_time_weaker
+    out = []
+    for m in range(nmem):
+        rng = np.random.default_rng(member_seed + m)
+        n = int(rng.choice(sizes))
+        amp = float(rng.uniform(0.5, 1.5))
+        d = RegionalDomain(W, W, n, n)
+        g = np.exp(-0.5 * (d.X ** 2 + d.Y ** 2) / (W / 4) ** 2)
+        tau = get_relaxation_time_weaker(10.0 ** (21.0 - amp * g / g.max()))
+        w = elra(n=n, W=W, t_end=t_end, tau=tau, lithosphere="constant", t_out=[])
+        w["sealevel"] = RegionalSeaLevel(surface=LaterallyVariableSeaSurface())
+        w["member"] = m
+        out.append(w)
+    return out
