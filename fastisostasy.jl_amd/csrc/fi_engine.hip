@@ -1843,6 +1843,11 @@ struct fi_batch {
     DBuf<Bar> bars;
     DBuf<BatchArgs> args;
     std::vector<int> order_h;
+    // the members' parameter blocks and step controls of a launch, contiguous so that one copy
+    // each way moves them (a member's own kp_d / ctl_d serve its single-context launches)
+    DBuf<KP> kp_all;
+    DBuf<Ctl> ctl_all;
+    std::vector<Ctl> ctl_h;
 };
 
 namespace {
@@ -1871,6 +1876,9 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
                  const std::vector<double>& t_target, float* ms_out, double* work_out, double* cap_share_out) {
     const int nm = (int)b->mem.size();
     const int nteams = std::max(1, gmax / ts);
+    if (b->kp_all.n < (size_t)nm) BHIPCHECK(b, b->kp_all.alloc(nm));
+    if (b->ctl_all.n < (size_t)nm) BHIPCHECK(b, b->ctl_all.alloc(nm));
+    b->ctl_h.resize(nm);
     std::vector<KP> kph(nm);
     std::vector<const KP*> ptrs(nm);
     std::vector<long long> att0(nm);
@@ -1885,10 +1893,12 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
         kph[m].mode = MODE_INTEGRATE;
         kph[m].max_steps = max_steps;
         kph[m].write_sigma_always = 0;
-        ptrs[m] = c->kp_d.p;
-        BHIPCHECK(b, hipMemcpyAsync(c->kp_d.p, &kph[m], sizeof(KP), hipMemcpyHostToDevice, b->stream));
-        BHIPCHECK(b, hipMemcpyAsync(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
+        kph[m].ctl = b->ctl_all.p + m;
+        ptrs[m] = b->kp_all.p + m;
+        b->ctl_h[m] = c->ctl;
     }
+    BHIPCHECK(b, hipMemcpyAsync(b->kp_all.p, kph.data(), nm * sizeof(KP), hipMemcpyHostToDevice, b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(b->ctl_all.p, b->ctl_h.data(), nm * sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
     const size_t nbar = (size_t)nteams * (1 + 2 * (size_t)ts);
     if (b->bars.n < nbar) BHIPCHECK(b, b->bars.alloc(nbar));
     if (b->team_member.n < (size_t)nteams) BHIPCHECK(b, b->team_member.alloc(nteams));
@@ -1913,8 +1923,7 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     BHIPCHECK(b, hipEventRecord(b->ev0, b->stream));
     BHIPCHECK(b, hipLaunchCooperativeKernel(f, dim3(nteams * ts), dim3(NT), kargs, (unsigned)lds, b->stream));
     BHIPCHECK(b, hipEventRecord(b->ev1, b->stream));
-    for (int m = 0; m < nm; ++m)
-        BHIPCHECK(b, hipMemcpyAsync(&b->mem[m]->ctl, b->mem[m]->ctl_d.p, sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(b->ctl_h.data(), b->ctl_all.p, nm * sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
     std::vector<Bar> hb(nbar);
     BHIPCHECK(b, hipMemcpyAsync(hb.data(), b->bars.p, nbar * sizeof(Bar), hipMemcpyDeviceToHost, b->stream));
     BHIPCHECK(b, hipStreamSynchronize(b->stream));
@@ -1923,6 +1932,7 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
     double work = 0;
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
+        c->ctl = b->ctl_h[m];
         work += (double)(c->ctl.nacc + c->ctl.nrej - att0[m]) * (double)c->N;
         c->ctl.last_u = c->ctl.cur;
     }
