@@ -233,3 +233,27 @@ def test_gpu_ensemble_refuses_mixed_rheologies():
     mixed = [elra_ensemble(sizes=(32,))[0], small_ensemble(nmem=1)[0]]
     with pytest.raises(fastiso.EngineError):
         fastiso.Ensemble(mixed)
+
+
+@pytest.mark.gpu
+def test_gpu_elra_ensemble_tuner_settles_on_short_windows():
+    """The ELRA members take far fewer than the tuner's 64 attempts per 100-yr window, so its
+    capped trials run into the window end: it must switch to timing one whole window per
+    candidate and fix a team size within a handful of windows. The members stay bitwise equal to
+    a fixed-team-size run."""
+    inputs = elra_ensemble()
+    tuned = hip_ensemble(inputs, 0)
+    fixed = hip_ensemble(inputs, 2)
+    tuned.integrator_init()
+    fixed.integrator_init()
+    t = 0.0
+    for _ in range(12):
+        t += 100.0
+        tuned.integrate(t)
+        fixed.integrate(t)
+    for m in range(len(inputs)):
+        np.testing.assert_array_equal(tuned[m].get_field("u"), fixed[m].get_field("u"), err_msg=f"member {m}")
+        assert tuned[m].stats()["n_accept"] / 12 < 64, "windows are not shorter than a trial"
+    if not DRYRUN:
+        info = tuned.info()
+        assert info["tuned"] == 1 and info["team_size"] in (1, 2, 4, 8, 16, 32), info
