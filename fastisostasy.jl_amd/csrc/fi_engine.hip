@@ -1868,6 +1868,8 @@ int bfail(fi_batch* b, int code, const std::string& m) {
 constexpr int TUNE_ATTEMPTS = 64;
 // share of a trial's capped work (cells x attempts) that must run for the trial to be timed
 constexpr double TUNE_CAPPED_SHARE = 0.9;
+// a candidate timed below this share of the best rate so far ends the search
+constexpr double TUNE_STOP = 0.8;
 // attempts per member per launch otherwise (bounds one launch's duration)
 constexpr int LAUNCH_ATTEMPTS = 1 << 20;
 
@@ -2086,7 +2088,7 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
         BHIPCHECK(b, hipMemcpy(b->order.p, b->order_h.data(), nm * sizeof(int), hipMemcpyHostToDevice));
     }
     if (b->tsize_fixed == 0 && b->cands.empty()) {
-        for (int ts : {1, 2, 4, 8, 16, 32})
+        for (int ts : {32, 16, 8, 4, 2, 1})
             if (ts <= gmax) b->cands.push_back(ts);
         b->cand_rate.assign(b->cands.size(), 0.0);
     }
@@ -2123,14 +2125,16 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
             b->cand_next = 0;
         }
         if (timed) {
-            b->cand_rate[b->cand_next] = ms > 0 ? work / ms : 0.0;
+            const size_t k = b->cand_next;
+            b->cand_rate[k] = ms > 0 ? work / ms : 0.0;
             b->cand_next += 1;
-            if (b->cand_next == b->cands.size()) {
-                size_t best = 0;
-                for (size_t k = 1; k < b->cands.size(); ++k)
-                    if (b->cand_rate[k] > b->cand_rate[best]) best = k;
+            size_t best = 0;
+            for (size_t q = 1; q <= k; ++q)
+                if (b->cand_rate[q] > b->cand_rate[best]) best = q;
+            // candidates run from the largest team down; the rate rises to a peak and falls
+            // again, so a smaller team clearly slower than the best so far ends the search
+            if (b->cand_next == b->cands.size() || b->cand_rate[k] < TUNE_STOP * b->cand_rate[best])
                 b->tsize_best = b->cands[best];
-            }
         }
     }
     return FI_OK;

@@ -329,10 +329,11 @@ fi_ctx* fi_batch_member(fi_batch* b, int32_t m);
 void fi_batch_destroy(fi_batch* b);
 const char* fi_batch_last_error(const fi_batch* b);
 
-/* Workgroups per member team. 0 (the default) times the candidate sizes (1, 2, 4, ..., 32) on the
- * first launches of fi_batch_integrate and keeps the fastest: a trial of 64 attempts per member
- * counts when at least 90% of its capped work ran; when windows are shorter than that, each
- * candidate is timed over a whole window. Results do not depend on the team size. */
+/* Workgroups per member team. 0 (the default) times the candidate sizes (32, 16, ..., 1) on the
+ * first launches of fi_batch_integrate and keeps the fastest, stopping once a smaller team is
+ * timed below 0.8 of the best rate: a trial of 64 attempts per member counts when at least 90% of
+ * its capped work ran; when windows are shorter than that, each candidate is timed over a whole
+ * window. Results do not depend on the team size. */
 int fi_batch_set_team_size(fi_batch* b, int32_t team_size);
 
 /* Advances every member (after fi_integrator_init) to t_target with its own adaptive BS3 steps.
