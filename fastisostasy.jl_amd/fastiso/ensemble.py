@@ -61,6 +61,10 @@ class Ensemble:
         self._check(self.lib.fi_batch_integrate_each(self.h, tt.ctypes.data_as(C.POINTER(C.c_double)),
                                                      int(bool(fsal))))
 
+    def set_team_size(self, team_size):
+        """Workgroups per member team; 0 picks it by timing the next launches."""
+        self._check(self.lib.fi_batch_set_team_size(self.h, int(team_size)))
+
     def info(self):
         s = _abi.FiBatchInfo()
         self._check(self.lib.fi_batch_get_info(self.h, C.byref(s)))

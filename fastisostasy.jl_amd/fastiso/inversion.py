@@ -204,6 +204,9 @@ class InversionProblem:
             raise ValueError(f"the unscented process runs 2 N_par + 1 = {self.process.N_ens} members")
         self.out = [prior.to_physical(self.process.u_mean[-1])]
         self.G_ens = None
+        # every iteration runs the same number of members: once a batch has chosen its team size
+        # by timing, the later batches take it instead of timing the candidates again
+        self.team_size = 0
 
     def forward(self, params_phys):
         """One forward run per column of params_phys, all members in one ensemble; returns G
@@ -211,6 +214,8 @@ class InversionProblem:
         inputs = [self.reduction.reconstruct(params_phys[:, j]) for j in range(params_phys.shape[1])]
         batch = self.make_batch(inputs)
         try:
+            if self.team_size and hasattr(batch, "set_team_size"):
+                batch.set_team_size(self.team_size)
             batch.integrator_init()
             G = [[] for _ in inputs]
             for k, t in enumerate(self.data.t):
@@ -218,6 +223,10 @@ class InversionProblem:
                     batch.integrate(t)
                 for j in range(len(inputs)):
                     G[j].append(self.reduction.extract_output(batch[j], self.data, k))
+            if not self.team_size and hasattr(batch, "info"):
+                info = batch.info()
+                if info.get("tuned") and info.get("team_size"):
+                    self.team_size = int(info["team_size"])
             return np.stack([np.concatenate(g) for g in G], axis=1)
         finally:
             close = getattr(batch, "close", None)
