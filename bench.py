@@ -26,7 +26,9 @@ dx = 5.86 km, field seeds 5001/5002); ensemble is the uneven ensemble of 256 mem
 drawn from {64, 96, 128, 192, 256}, advanced together by fi_batch_integrate (value = member-kyr/s,
 the sum over members of simulated kyr per wall-clock second); elra_ensemble is 256 ELRA members
 with the same sides, each with its own relaxation-time anomaly (workloads.elra_ensemble), in 1-kyr
-windows; surrogate is the N=350 workload run
+windows; inversion is the unscented Kalman inversion of two regional upper-mantle viscosities on
+the N=256 disc benchmark (value = UKI iterations/s, each 5 forward runs to 3 kyr on the ensemble
+engine plus the host update); surrogate is the N=350 workload run
 in 500-yr macro steps by the trained surrogate with a plain-solver check every 10 macro steps and
 fallback above 1 m (--validate adds the plain run of the same window for the error).
 """
@@ -244,6 +246,114 @@ def bench_surrogate(a):
     return out
 
 
+def inversion_setup(n):
+    """Two-region upper-mantle viscosity inversion on the disc benchmark (n x n, W = 3000 km):
+    log10 viscosity inside and outside r = 1000 km (truth 21.0 / 20.6), displacement data at 1, 2
+    and 3 kyr within r < 1500 km, prior mean 20.7 / 20.9 with sd 0.5; 5 sigma points per UKI
+    iteration."""
+    import numpy as np
+    import fastiso
+    from fastiso import inversion, workloads
+    w = workloads.disc_benchmark(n=n, t_end=3000.0)
+    d = w["domain"]
+
+    def builder(eta):
+        se = fastiso.SolidEarth(d, layer_boundaries=[88e3], layer_viscosities=[eta], rho_uppermantle=3300.0,
+                                rho_litho=0.0)
+        return fastiso.Simulation(d, w["bcs"], w["sealevel"], se, (0.0, 3000.0), nout=fastiso.NativeOutput(),
+                                  opts=w["opts"], c=w["c"], engine=None).inputs
+
+    red = inversion.RegionalViscosityReduction(np.full((d.nx, d.ny), 21.0), [d.R < 1.0e6, d.R >= 1.0e6], builder)
+    return red, d, (1000.0, 2000.0, 3000.0)
+
+
+def bench_inversion(a):
+    import numpy as np
+    import fastiso
+    from fastiso import inversion
+    n = a.inv_n
+    red, d, tobs = inversion_setup(n)
+    log = []
+
+    class RecordingEnsemble(fastiso.Ensemble):
+        def __init__(self, inputs_list, **kw):
+            super().__init__(inputs_list, **kw)
+            self._cells = [inp["cfg"].nx * inp["cfg"].ny for inp in inputs_list]
+
+        def close(self):
+            if getattr(self, "h", None):
+                i = self.info()
+                cs = sum(c * self[m].stats()["n_accept"] for m, c in enumerate(self._cells))
+                log.append((i["device_ms"], cs, i["team_size"]))
+            super().close()
+
+    truth = fastiso.Ensemble([red.reconstruct(np.array([21.0, 20.6]))])
+    truth.integrator_init()
+    Y = []
+    for tk in tobs:
+        truth.integrate(tk)
+        Y.append(truth[0].get_field("u"))
+    truth.close()
+    data = inversion.InversionData(list(tobs), Y, d.R < 1.5e6)
+    prior = inversion.GaussianPrior([20.7, 20.9], [0.5, 0.5])
+    cfg = inversion.InversionConfig("unscented", N_iter=a.warmup + a.steps, alpha_reg=1.0, update_freq=1,
+                                    scale_obscov=1e-6)
+    prob = inversion.inversion_problem(cfg, data, red, prior, make_batch=RecordingEnsemble, seed=5)
+
+    def iteration():
+        U = prob.process.sigma_points()
+        prob.G_ens = prob.forward(prob.prior.to_physical(U))
+        prob.process.update(prob.G_ens, prob.y_obs, prob.Gamma)
+        prob.out.append(prob.prior.to_physical(prob.process.u_mean[-1]))
+
+    for k in range(a.warmup):
+        iteration()
+        print(f"warm-up iteration {k + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+    n0 = len(log)
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        iteration()
+        print(f"iteration {k + 1}/{a.steps} done: mean {prob.out[-1]}", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    dev_s = sum(x[0] for x in log[n0:]) / 1e3
+    cell_steps = float(sum(x[1] for x in log[n0:]))
+    achieved = BYTES_PER_CELL_STEP * cell_steps / dev_s / 1e9 if dev_s > 0 else None
+    out = {
+        "metric": f"UKI iterations/s, two-region viscosity inversion on the N={n} disc (5 sigma points, 3 kyr)",
+        "value": a.steps / el, "unit": "iterations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 fft/stencil-moments, f64 state/stencil/error-norm", "data": "synthetic (disc benchmark)",
+        "config": {"workload": f"inversion_disc_n{n}", "n": n, "members_per_iteration": prob.process.N_ens,
+                   "obs_times_yr": list(tobs), "device_s": dev_s, "cell_steps": cell_steps,
+                   "team_size": log[-1][2] if log else None, "final_mean_log10_eta": [float(x) for x in prob.out[-1]],
+                   "truth_log10_eta": [21.0, 20.6]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_inversion(n, prob, red)
+    return out
+
+
+def cpu_baseline_inversion(n, prob, red):
+    """One UKI forward map (the 5 sigma-point runs, 0 -> 3 kyr) on the FP64 oracle ensemble."""
+    threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = threads
+    from oracle import pyref
+    U = prob.process.sigma_points()
+    phys = prob.prior.to_physical(U)
+    inputs = [red.reconstruct(phys[:, j]) for j in range(phys.shape[1])]
+    t0 = time.perf_counter()
+    ens = pyref.RefEnsemble(inputs)
+    ens.integrator_init()
+    for tk in prob.data.t:
+        ens.integrate(tk)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / el, "unit": "iterations/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle FP64 ensemble, one iteration's {len(inputs)} forward runs at n={n}, 0 -> 3 kyr "
+                      f"(the UKI update, host numpy in both legs, takes milliseconds), {el:.1f} s wall"}
+
+
 def bench_fno_train(a):
     """Device training throughput of the surrogate network at N=350 (width 32, 16x16 modes, 4
     layers, 5 input channels) on a synthetic set of 64 samples: Adam steps of batch 8."""
@@ -282,29 +392,34 @@ def main():
     ap.add_argument("--window", type=float, default=None, help="window in yr (default 10; 5 for lv_elva_hr)")
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "elra_ensemble", "surrogate",
-                                               "fno_train"],
+    ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "elra_ensemble", "inversion",
+                                               "surrogate", "fno_train"],
                     default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=20.0)
     ap.add_argument("--validate", action="store_true")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--team-size", type=int, default=0)
+    ap.add_argument("--inv-n", type=int, default=256, help="grid side of the inversion workload")
     a = ap.parse_args()
     hr_defaults = a.workload == "lv_elva_hr"
     elra = a.workload == "elra"
     implicit = a.workload == "implicit"
     elra_ens = a.workload == "elra_ensemble"
+    inv = a.workload == "inversion"
     if a.steps is None:
-        a.steps = 3 if hr_defaults else 10 if elra_ens else 50 if (elra or implicit) else 200
+        a.steps = 3 if hr_defaults else 4 if inv else 10 if elra_ens else 50 if (elra or implicit) else 200
     if a.warmup is None:
-        a.warmup = 1 if hr_defaults else 8 if elra_ens else 2 if (elra or implicit) else 5
+        a.warmup = 1 if (hr_defaults or inv) else 8 if elra_ens else 2 if (elra or implicit) else 5
     if a.window is None:
         a.window = 5.0 if hr_defaults else 100.0 if elra else 1000.0 if (implicit or elra_ens) else 10.0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "inversion":
+        print(json.dumps(bench_inversion(a)), flush=True)
+        return
     if a.workload in ("ensemble", "elra_ensemble"):
         print(json.dumps(bench_ensemble(a)), flush=True)
         return

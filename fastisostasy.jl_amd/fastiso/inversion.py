@@ -129,22 +129,41 @@ class Unscented:
 
     def update(self, G, y, Gamma):
         """update_ensemble!(uki, G) with the forward evaluations G (N_obs x N_ens) of the sigma
-        points of sigma_points()."""
+        points of sigma_points(). Gamma is the observation covariance: an N_obs x N_obs matrix,
+        or the vector of its diagonal. With a diagonal the gain is formed through the Woodbury
+        identity: C_gg = A A^T + R with A = sqrt(w) dG (rank <= 2N) and R = 2 Gamma, so
+        K = C_ug C_gg^-1 = sqrt(w) dU M^-1 A^T R^-1 and K C_ug^T = w dU (I - M^-1) dU^T with
+        M = I + A^T R^-1 A (2N x 2N); no N_obs x N_obs matrix is formed. For these sigma points
+        w dU dU^T = L L^T is the predicted covariance C (2 c^2 w = 1), so the new covariance
+        C - K C_ug^T is formed as w dU M^-1 dU^T: a posterior far tighter than the prediction
+        (small Gamma) keeps its relative accuracy instead of being the difference of two nearly
+        equal matrices."""
         m, C, U = self._pred
         G = np.asarray(G, dtype=np.float64)
+        Gamma = np.asarray(Gamma, dtype=np.float64)
         g = G[:, 0]
         dU = U[:, 1:] - m[:, None]
         dG = G[:, 1:] - g[:, None]
-        C_ug = self.w_cov * dU @ dG.T
-        C_gg = self.w_cov * dG @ dG.T + 2.0 * Gamma
-        K = np.linalg.solve(C_gg, C_ug.T).T
-        m_new = m + K @ (y - g)
-        C_new = C - K @ C_ug.T
+        d = y - g
+        if Gamma.ndim == 1:
+            sw = np.sqrt(self.w_cov)
+            A = sw * dG
+            BR = A.T / (2.0 * Gamma)[None, :]          # A^T R^-1
+            M = np.eye(A.shape[1]) + BR @ A
+            m_new = m + sw * dU @ np.linalg.solve(M, BR @ d)
+            C_new = self.w_cov * dU @ np.linalg.solve(M, dU.T)
+            err = float(d @ (d / Gamma))
+        else:
+            C_ug = self.w_cov * dU @ dG.T
+            C_gg = self.w_cov * dG @ dG.T + 2.0 * Gamma
+            K = np.linalg.solve(C_gg, C_ug.T).T
+            m_new = m + K @ d
+            C_new = C - K @ C_ug.T
+            err = float(d @ np.linalg.solve(Gamma, d))
         self.u_mean.append(m_new)
         self.uu_cov.append(0.5 * (C_new + C_new.T))
         self.g_mean.append(g)
-        d = y - g
-        self.error.append(float(d @ np.linalg.solve(Gamma, d)))
+        self.error.append(err)
         self.iter += 1
         if self.update_freq > 0 and self.iter % self.update_freq == 0:
             self.S_w = (2.0 - self.alpha_reg ** 2) * self.uu_cov[-1]
@@ -195,7 +214,8 @@ class InversionProblem:
         self.make_batch = make_batch
         self.t0 = float(t0)
         nobs = data.countmask * data.nY
-        self.Gamma = np.diag(np.full(nobs, config.scale_obscov / 2.0))
+        # observation noise covariance (scale_obscov / 2) I, kept as its diagonal
+        self.Gamma = np.full(nobs, config.scale_obscov / 2.0)
         rng = np.random.default_rng(seed)
         yn = data.y[:, None] + rng.normal(0.0, np.sqrt(config.scale_obscov / 2.0), (nobs, config.n_samples))
         self.y_obs = yn.mean(axis=1)

@@ -37,12 +37,14 @@ def _linear(seed=0, npar=3, nobs=7):
     return A, m0, C0, Gamma, y
 
 
-def test_uki_first_step_is_the_gaussian_posterior():
+@pytest.mark.parametrize("diagonal", [False, True])
+def test_uki_first_step_is_the_gaussian_posterior(diagonal):
+    """Dense Gamma, and Gamma given as its diagonal (the Woodbury form of the gain)."""
     A, m0, C0, Gamma, y = _linear()
     p = inversion.Unscented(m0, C0, alpha_reg=1.0, update_freq=0)
     U = p.sigma_points()
     assert U.shape == (3, 7)
-    p.update(A @ U, y, Gamma)
+    p.update(A @ U, y, np.diag(Gamma) if diagonal else Gamma)
     K = C0 @ A.T @ np.linalg.inv(A @ C0 @ A.T + Gamma)
     np.testing.assert_allclose(p.u_mean[-1], m0 + K @ (y - A @ m0), rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(p.uu_cov[-1], 2 * (C0 - K @ A @ C0), rtol=1e-9, atol=1e-12)
@@ -60,6 +62,47 @@ def test_uki_converges_to_least_squares():
     m_ls = np.linalg.solve(P, A.T @ Gi @ y)
     np.testing.assert_allclose(p.u_mean[-1], m_ls, rtol=1e-9, atol=1e-10)
     np.testing.assert_allclose(p.uu_cov[-1], np.linalg.inv(P), rtol=1e-7, atol=1e-12)
+
+
+def test_uki_diagonal_and_dense_noise_agree_on_a_nonlinear_map():
+    """The Woodbury gain equals the dense solve when the forward map is not linear (the unscented
+    transform is then not exact), over several iterations, with more observations than members."""
+    rng = np.random.default_rng(3)
+    npar, nobs = 3, 40
+    A = rng.standard_normal((nobs, npar))
+    gam = 0.05 + rng.random(nobs)
+    y = np.tanh(A @ np.array([0.3, -0.2, 0.5])) + 0.01 * rng.standard_normal(nobs)
+    m0, C0 = np.zeros(npar), np.eye(npar)
+    pd = inversion.Unscented(m0, C0, alpha_reg=0.9, update_freq=1)
+    pv = inversion.Unscented(m0, C0, alpha_reg=0.9, update_freq=1)
+    for _ in range(5):
+        pd.update(np.tanh(A @ pd.sigma_points()), y, np.diag(gam))
+        pv.update(np.tanh(A @ pv.sigma_points()), y, gam)
+    np.testing.assert_allclose(pv.u_mean[-1], pd.u_mean[-1], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(pv.uu_cov[-1], pd.uu_cov[-1], rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(pv.error, pd.error, rtol=1e-12)
+
+
+def test_uki_diagonal_noise_keeps_a_tight_posterior_accurate():
+    """Observation noise 1e8 times below the prior: the update's covariance must still equal the
+    information form (C^-1 + A^T (2 Gamma)^-1 A)^-1 of the linear-Gaussian posterior (C the
+    predicted covariance), which involves no cancellation; a covariance formed as C - K C_ug^T
+    loses most of its digits here."""
+    rng = np.random.default_rng(4)
+    npar, nobs = 2, 3000
+    A = rng.standard_normal((nobs, npar)) * 5.0
+    gam = np.full(nobs, 5e-7)
+    y = A @ np.array([0.4, -0.7]) + 1e-4 * rng.standard_normal(nobs)
+    C0 = np.array([[0.25, 0.05], [0.05, 0.25]])
+    p = inversion.Unscented(np.zeros(npar), C0, alpha_reg=1.0, update_freq=1)
+    for _ in range(3):
+        U = p.sigma_points()
+        C = p._pred[1]
+        p.update(A @ U, y, gam)
+        C_info = np.linalg.inv(np.linalg.inv(C) + A.T @ (A / (2.0 * gam)[:, None]))
+        np.testing.assert_allclose(p.uu_cov[-1], C_info, rtol=1e-6)
+        m_info = p._pred[0] + C_info @ (A.T @ ((y - A @ p._pred[0]) / (2.0 * gam)))
+        np.testing.assert_allclose(p.u_mean[-1], m_info, rtol=1e-8, atol=1e-12)
 
 
 def test_transforms_and_observation_noise():
