@@ -41,6 +41,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "fastisostasy.jl_amd")]
 
+from fastiso import _abi  # noqa: E402
+
 BYTES_PER_CELL_STEP = 171.0
 # ELRA, per cell per RHS call, counted once per unit: rows phase reads the base state (8) and on
 # average 5/3 FP32 derivative slots (6.7), mask, bedrock offset, reference H_af, litho anomaly,
@@ -125,6 +127,7 @@ def bench_ensemble(a):
     achieved = alg_bytes / dev_s / 1e9 if dev_s > 0 else None
     sizes = sorted(set(inp["cfg"].nx for inp in inputs))
     kind = "ELRA" if elra else "LV-ELVA"
+    ens.close()
     return {
         "metric": f"simulated member-kyr/s, {len(inputs)}-member uneven {kind} ensemble (plain solver)",
         "value": member_kyr / el,
@@ -148,24 +151,52 @@ def bench_ensemble(a):
     }
 
 
-def cpu_baseline(n, window, sample_s, hr=False, elra=False, implicit=False):
-    threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
-    os.environ["OMP_NUM_THREADS"] = threads
+def host_cores():
+    """Host cores this process may use: OMP_NUM_THREADS where the host sets it (a one-GPU job's
+    share of the box), else the CPU affinity set."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env else len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(n, window, sample_s, hr=False, elra=False, implicit=False, start=None):
+    """The FP64 oracle on this host, on all cores and on one core, each for about sample_s of
+    wall time. With start = (t, u) both legs begin from the GPU leg's state at the start of its
+    timed windows (fi_set_field + fi_set_time, then a 1/100-window untimed lead-in so the step
+    size settles), so they sample the same part of the run; otherwise after one window from rest."""
     from oracle import pyref
     w, sim = make_sim(n, engine=None, hr=hr, elra=elra, implicit=implicit)
-    e = pyref.RefEngine.from_inputs(sim.inputs)
-    e.integrator_init()
-    e.integrate(window)                 # same warm-up as the GPU leg: the first window
-    t_sim0 = e.stats()["t"]
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < sample_s:
-        e.integrate(e.stats()["t"] + window / 10)
-    el = time.perf_counter() - t0
-    st = e.stats()
-    sim_yr = st["t"] - t_sim0
-    return {"value": sim_yr / 1e3 / el, "unit": "kyr/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle FP64, n={n}, t={t_sim0:.0f}->{st['t']:.2f} yr after a {window:.0f}-yr warm-up, "
-                      f"{el:.1f} s wall, {st['n_accept']} accepted steps total"}
+    legs = []
+    for cores in (host_cores(), 1):
+        pyref.set_threads(cores)
+        e = pyref.RefEngine.from_inputs(sim.inputs)
+        e.integrator_init()
+        if start is not None:
+            e.set_field("u", start[1])
+            e.set_time(start[0])
+            e.integrate(start[0] + window / 100)
+        else:
+            e.integrate(window)
+        s0 = e.stats()
+        t0 = time.perf_counter()
+        last = t0
+        while time.perf_counter() - t0 < sample_s:
+            e.integrate(e.stats()["t"] + window / 10)
+            if time.perf_counter() - last > 10:
+                last = time.perf_counter()
+                print(f"cpu baseline ({cores} cores): t = {e.stats()['t']:.2f} yr", file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        st = e.stats()
+        sim_yr = st["t"] - s0["t"]
+        legs.append({"value": sim_yr / 1e3 / el, "unit": "kyr/s", "cores": cores, "kind": "port",
+                     "sample": f"oracle FP64, n={n}, t={s0['t']:.2f}->{st['t']:.2f} yr "
+                               + ("from the GPU state at the start of the timed windows" if start is not None
+                                  else f"after a {window:.0f}-yr warm-up")
+                               + f", {el:.1f} s wall, {st['n_accept'] - s0['n_accept']} accepted steps"})
+        print(f"cpu baseline ({cores} cores): {legs[-1]['value']:.3g} kyr/s", file=sys.stderr, flush=True)
+    pyref.set_threads(legs[0]["cores"])
+    out = dict(legs[0])
+    out["one_core"] = legs[1]
+    return out
 
 
 WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "fastiso", "data", "fno_lv_elva_n350.npz")
@@ -243,6 +274,9 @@ def bench_surrogate(a):
             print(f"plain reference at {tk:.0f} yr: max |u_sur - u_plain| = {errs[f'{tk:.0f}']:.3f} m",
                   file=sys.stderr, flush=True)
         out["config"]["max_abs_u_error_vs_plain_m"] = errs
+        sim2.close()
+    sur.close()
+    sim.close()
     return out
 
 
@@ -337,9 +371,9 @@ def bench_inversion(a):
 
 def cpu_baseline_inversion(n, prob, red):
     """One UKI forward map (the 5 sigma-point runs, 0 -> 3 kyr) on the FP64 oracle ensemble."""
-    threads = os.environ.get("OMP_NUM_THREADS") or str(min(16, os.cpu_count() or 1))
-    os.environ["OMP_NUM_THREADS"] = threads
+    threads = str(host_cores())
     from oracle import pyref
+    pyref.set_threads(int(threads))
     U = prob.process.sigma_points()
     phys = prob.prior.to_physical(U)
     inputs = [red.reconstruct(phys[:, j]) for j in range(phys.shape[1])]
@@ -372,6 +406,7 @@ def bench_fno_train(a):
     sur.train(a.steps, batch=8, lr=1e-4, seed=2)
     el = time.perf_counter() - t0
     dev = (sur.device_ms() - ms0) / 1e3
+    sur.close()
     samples = a.steps * 8
     return {
         "metric": f"surrogate training samples/s at N={n} (forward + backward + Adam)",
@@ -441,6 +476,7 @@ def main():
     for _ in range(a.warmup):
         t += a.window
         eng.integrate(t)
+    start = (eng.stats()["t"], eng.get_field("u"))
     s0 = eng.stats()
     t0 = time.perf_counter()
     for k in range(a.steps):
@@ -466,6 +502,7 @@ def main():
     if a.n == 350 and not (hr or elra or implicit) and os.path.exists(TRAFFIC_N350) and launches > 0:
         with open(TRAFFIC_N350) as f:
             traffic = json.load(f)["bytes_per_accepted_step"] * acc / launches
+    engine_name, engine_lib = _abi.engine_identity(eng.lib)
     out = {
         "metric": f"simulated kyr/s at N={a.n} " + ("ELRA (plain solver)" if elra else "implicit LC step (Euler, dt_min 100 yr)"
                                                      if implicit else "LV-ELVA (plain solver)"),
@@ -488,10 +525,17 @@ def main():
                    "kernel_launches": int(launches), "kernel_ms_per_launch": dev_s * 1e3 / max(launches, 1),
                    "us_per_accepted_step": dev_s * 1e6 / max(acc, 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic},
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "traffic_unit": "B per launch" if traffic is not None else None,
+                     "traffic_source": (os.path.relpath(TRAFFIC_N350, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, "
+                                        "WRITE_SIZE per accepted step, scaled to this run's steps per launch)")
+                     if traffic is not None else None},
     }
+    out["config"]["engine"] = engine_name
+    out["config"]["engine_library"] = os.path.relpath(engine_lib, ROOT) if engine_lib else None
+    sim.close()
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr, elra, implicit)
+        out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr, elra, implicit, start=start)
     print(json.dumps(out), flush=True)
 
 

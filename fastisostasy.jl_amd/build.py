@@ -19,11 +19,6 @@ def build(verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(OUT + ".tmp", OUT)
-    # calibration program of the HBM traffic counters (profiles/)
-    probe = os.path.join(HERE, "lib", "copy_probe")
-    subprocess.check_call([hipcc, "--offload-arch=gfx950", "-O3", "-o", probe + ".tmp",
-                           os.path.join(HERE, "csrc", "copy_probe.hip")])
-    os.replace(probe + ".tmp", probe)
     return OUT
 
 

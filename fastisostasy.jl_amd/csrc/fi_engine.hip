@@ -125,11 +125,19 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
         if (threadIdx.x == 0) {
             const double S2 = Sm[0], S1 = Sm[1], S0 = Sm[2];
             const double e2 = (S2 - 2.0 * C * S1 + C * C * S0) / p.N;
-            const double EEst = sqrt(fmax(e2, 0.0));
-            s.EEst = EEst;
+            // a NaN moment means a NaN in u_n, u_{n+1} or a derivative: OrdinaryDiffEq's PI
+            // controller then proposes a NaN step and stops (retcode DtNaN); an infinite error
+            // norm (overflow) is rejected with the smallest step factor
+            const bool nan_state = isnan(S2) || isnan(S1) || isnan(S0);
+            const double EEst = isnan(e2) ? INFINITY : sqrt(fmax(e2, 0.0));
+            s.EEst = nan_state ? NAN : EEst;
             double qv;
             bool finite = isfinite(EEst);
-            if (!finite) {
+            if (nan_state) {
+                s.err = ERR_NONFINITE;
+                s.nrej += 1;
+                s.steps = 0;
+            } else if (!finite) {
                 s.q11 = PI_GAMMA / PI_QMIN;
                 qv = 1.0 / PI_QMIN;
             } else if (EEst == 0.0) {
@@ -140,7 +148,8 @@ __device__ __forceinline__ void integrate_loop(const KP& p, const Team& tm, Ctl&
                 qv = q11 / pow(s.qold, PI_BETA2);
                 qv = fmax(1.0 / PI_QMAX, fmin(1.0 / PI_QMIN, qv / PI_GAMMA));
             }
-            if (finite && EEst <= 1.0) {
+            if (nan_state) {
+            } else if (finite && EEst <= 1.0) {
                 s.qold = fmax(EEst, PI_QOLDINIT);
                 s.t = clipped ? s.stop : tn + dt;
                 s.cur = unew;
@@ -323,7 +332,8 @@ struct BatchArgs {
     int nmem, nteams, tsize, fsal;
     Bar* bars;              // nteams regions of (1 + 2 tsize) Bar
     int* queue;             // queue head (zeroed by the host)
-    int* team_member;       // per team: the member it is working on (written by its rank 0)
+    int* team_member;       // per team two slots (alternating pops): the member it is working on,
+                            // written by its rank 0
     unsigned long long spin_limit;
 };
 
@@ -368,16 +378,18 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent(const BatchArgs* __res
     const Team tm{(int)blockIdx.x / ba->nteams, ba->tsize, ba->bars + (long)team * (1 + 2 * ba->tsize)};
     unsigned gen = 0;
     unsigned long long tl = wall_clock64();
-    for (;;) {
-        // rank 0 pops the next member; the team barrier publishes it
+    for (unsigned pops = 0;; ++pops) {
+        // rank 0 pops the next member; the team barrier publishes it. The slot alternates with
+        // the pop count: a member with nothing left to do runs no team barrier, so rank 0 may pop
+        // again before every workgroup has read the previous member, but it cannot pop twice
+        // more before the next team barrier, which every workgroup reaches only after its read.
+        int* slot = &ba->team_member[2 * team + (int)(pops & 1u)];
         if (tm.rank == 0 && threadIdx.x == 0) {
             const int q = atomicAdd(ba->queue, 1);
-            __hip_atomic_store(&ba->team_member[team], q < ba->nmem ? ba->order[q] : -1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(slot, q < ba->nmem ? ba->order[q] : -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (!team_sync(tm, gen, &flag, ba->spin_limit)) return;
-        if (threadIdx.x == 0)
-            s_m = __hip_atomic_load(&ba->team_member[team], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) s_m = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const int m = __builtin_amdgcn_readfirstlane(s_m);
         if (m < 0) return;
@@ -402,15 +414,15 @@ __global__ void __launch_bounds__(NT) fi_batch_persistent_elra(const BatchArgs* 
     const Team tm{(int)blockIdx.x / ba->nteams, ba->tsize, ba->bars + (long)team * (1 + 2 * ba->tsize)};
     unsigned gen = 0;
     unsigned long long tl = wall_clock64();
-    for (;;) {
+    for (unsigned pops = 0;; ++pops) {
+        // alternating slots, as in fi_batch_persistent
+        int* slot = &ba->team_member[2 * team + (int)(pops & 1u)];
         if (tm.rank == 0 && threadIdx.x == 0) {
             const int q = atomicAdd(ba->queue, 1);
-            __hip_atomic_store(&ba->team_member[team], q < ba->nmem ? ba->order[q] : -1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(slot, q < ba->nmem ? ba->order[q] : -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (!team_sync(tm, gen, &flag, ba->spin_limit)) return;
-        if (threadIdx.x == 0)
-            s_m = __hip_atomic_load(&ba->team_member[team], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) s_m = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const int m = __builtin_amdgcn_readfirstlane(s_m);
         if (m < 0) return;
@@ -697,6 +709,8 @@ int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_si
     if (c->ctl.err == ERR_TIME_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
     if (c->ctl.err == ERR_BSL_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation0D out of range.");
     if (c->ctl.err == ERR_DT_MIN) return fail(c, FI_ERR_DT_MIN, "dt below minimum");
+    if (c->ctl.err == ERR_NONFINITE)
+        return fail(c, FI_ERR_NONFINITE, "NaN dt detected: NaN in the state or its derivative");
     if (c->ctl.err) return fail(c, FI_ERR_DEVICE, "device error");
     return FI_OK;
 }
@@ -1873,13 +1887,45 @@ constexpr double TUNE_STOP = 0.8;
 // attempts per member per launch otherwise (bounds one launch's duration)
 constexpr int LAUNCH_ATTEMPTS = 1 << 20;
 
-// one persistent ensemble launch with teams of ts workgroups; returns the device time
-int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int max_steps, int fsal,
-                 const std::vector<double>& t_target, float* ms_out, double* work_out, double* cap_share_out) {
-    const int nm = (int)b->mem.size();
-    const int nteams = std::max(1, gmax / ts);
+// Device memory of one ensemble launch beyond the members' own buffers: the members' parameter
+// blocks and step controls, the pointer table, the queue order and head, two member slots per team,
+// the teams' barrier regions and the launch arguments.
+struct BatchScratch {
+    KP* kp_all;
+    Ctl* ctl_all;
+    const KP** kps;
+    int* order;
+    int* queue;
+    int* team_member;
+    Bar* bars;
+    BatchArgs* args;
+};
+
+// the scratch of fi_batch_integrate_each, in buffers the batch allocates and grows as needed
+int batch_scratch_alloc(fi_batch* b, int nm, int nteams, int ts, BatchScratch& sc) {
     if (b->kp_all.n < (size_t)nm) BHIPCHECK(b, b->kp_all.alloc(nm));
     if (b->ctl_all.n < (size_t)nm) BHIPCHECK(b, b->ctl_all.alloc(nm));
+    const size_t nbar = (size_t)nteams * (1 + 2 * (size_t)ts);
+    if (b->bars.n < nbar) BHIPCHECK(b, b->bars.alloc(nbar));
+    if (b->team_member.n < 2 * (size_t)nteams) BHIPCHECK(b, b->team_member.alloc(2 * (size_t)nteams));
+    sc.kp_all = b->kp_all.p;
+    sc.ctl_all = b->ctl_all.p;
+    sc.kps = b->kps.p;
+    sc.order = b->order.p;
+    sc.queue = b->queue.p;
+    sc.team_member = b->team_member.p;
+    sc.bars = b->bars.p;
+    sc.args = b->args.p;
+    return FI_OK;
+}
+
+// one persistent ensemble launch with teams of ts workgroups; returns the device time. Only the
+// members still short of their targets are queued (largest first).
+int batch_launch(fi_batch* b, const BatchScratch& sc, bool mt_double, size_t lds, int gmax, int ts, int max_steps,
+                 int fsal, const std::vector<double>& t_target, float* ms_out, double* work_out,
+                 double* cap_share_out) {
+    const int nm = (int)b->mem.size();
+    const int nteams = std::max(1, gmax / ts);
     b->ctl_h.resize(nm);
     std::vector<KP> kph(nm);
     std::vector<const KP*> ptrs(nm);
@@ -1895,39 +1941,44 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
         kph[m].mode = MODE_INTEGRATE;
         kph[m].max_steps = max_steps;
         kph[m].write_sigma_always = 0;
-        kph[m].ctl = b->ctl_all.p + m;
-        ptrs[m] = b->kp_all.p + m;
+        kph[m].ctl = sc.ctl_all + m;
+        ptrs[m] = sc.kp_all + m;
         b->ctl_h[m] = c->ctl;
     }
-    BHIPCHECK(b, hipMemcpyAsync(b->kp_all.p, kph.data(), nm * sizeof(KP), hipMemcpyHostToDevice, b->stream));
-    BHIPCHECK(b, hipMemcpyAsync(b->ctl_all.p, b->ctl_h.data(), nm * sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
+    // the queue: members short of their targets, in the batch's size order
+    std::vector<int> act;
+    act.reserve(nm);
+    for (int m : b->order_h)
+        if (b->mem[m]->ctl.t < t_target[m] && !b->mem[m]->ctl.err) act.push_back(m);
+    BHIPCHECK(b, hipMemcpyAsync(sc.kp_all, kph.data(), nm * sizeof(KP), hipMemcpyHostToDevice, b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(sc.ctl_all, b->ctl_h.data(), nm * sizeof(Ctl), hipMemcpyHostToDevice, b->stream));
     const size_t nbar = (size_t)nteams * (1 + 2 * (size_t)ts);
-    if (b->bars.n < nbar) BHIPCHECK(b, b->bars.alloc(nbar));
-    if (b->team_member.n < (size_t)nteams) BHIPCHECK(b, b->team_member.alloc(nteams));
-    BHIPCHECK(b, hipMemsetAsync(b->bars.p, 0, nbar * sizeof(Bar), b->stream));
-    BHIPCHECK(b, hipMemsetAsync(b->queue.p, 0, sizeof(int), b->stream));
-    BHIPCHECK(b, hipMemcpyAsync(b->kps.p, ptrs.data(), nm * sizeof(const KP*), hipMemcpyHostToDevice, b->stream));
+    BHIPCHECK(b, hipMemsetAsync(sc.bars, 0, nbar * sizeof(Bar), b->stream));
+    BHIPCHECK(b, hipMemsetAsync(sc.queue, 0, sizeof(int), b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(sc.kps, ptrs.data(), nm * sizeof(const KP*), hipMemcpyHostToDevice, b->stream));
+    if (!act.empty())
+        BHIPCHECK(b, hipMemcpyAsync(sc.order, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
     BatchArgs a;
-    a.kps = b->kps.p;
-    a.order = b->order.p;
-    a.nmem = nm;
+    a.kps = sc.kps;
+    a.order = sc.order;
+    a.nmem = (int)act.size();
     a.nteams = nteams;
     a.tsize = ts;
     a.fsal = fsal;
-    a.bars = b->bars.p;
-    a.queue = b->queue.p;
-    a.team_member = b->team_member.p;
+    a.bars = sc.bars;
+    a.queue = sc.queue;
+    a.team_member = sc.team_member;
     a.spin_limit = 2000000000ull;
-    BHIPCHECK(b, hipMemcpyAsync(b->args.p, &a, sizeof(a), hipMemcpyHostToDevice, b->stream));
-    const BatchArgs* ap = b->args.p;
+    BHIPCHECK(b, hipMemcpyAsync(sc.args, &a, sizeof(a), hipMemcpyHostToDevice, b->stream));
+    const BatchArgs* ap = sc.args;
     void* kargs[] = {&ap};
     const void* f = batch_kernel(mt_double, b->dist, b->elra);
     BHIPCHECK(b, hipEventRecord(b->ev0, b->stream));
     BHIPCHECK(b, hipLaunchCooperativeKernel(f, dim3(nteams * ts), dim3(NT), kargs, (unsigned)lds, b->stream));
     BHIPCHECK(b, hipEventRecord(b->ev1, b->stream));
-    BHIPCHECK(b, hipMemcpyAsync(b->ctl_h.data(), b->ctl_all.p, nm * sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(b->ctl_h.data(), sc.ctl_all, nm * sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
     std::vector<Bar> hb(nbar);
-    BHIPCHECK(b, hipMemcpyAsync(hb.data(), b->bars.p, nbar * sizeof(Bar), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHECK(b, hipMemcpyAsync(hb.data(), sc.bars, nbar * sizeof(Bar), hipMemcpyDeviceToHost, b->stream));
     BHIPCHECK(b, hipStreamSynchronize(b->stream));
     float ms = 0;
     BHIPCHECK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -1953,6 +2004,8 @@ int batch_launch(fi_batch* b, bool mt_double, size_t lds, int gmax, int ts, int 
         if (e == ERR_TIME_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation2D out of range.");
         if (e == ERR_BSL_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation0D out of range.");
         if (e == ERR_DT_MIN) return bfail(b, FI_ERR_DT_MIN, "member " + std::to_string(m) + ": dt below minimum");
+        if (e == ERR_NONFINITE)
+            return bfail(b, FI_ERR_NONFINITE, "member " + std::to_string(m) + ": NaN in the state or its derivative");
         if (e) return bfail(b, FI_ERR_DEVICE, "member " + std::to_string(m) + ": device error");
     }
     return FI_OK;
@@ -2035,7 +2088,11 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
     return fi_batch_integrate_each(b, tt.data(), fsal);
 }
 
-int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) {
+}  // extern "C"
+
+namespace {
+
+int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal) {
     if (!b || !t_targets) return FI_ERR_INVALID;
     const int nm = (int)b->mem.size();
     const std::vector<double> t_target(t_targets, t_targets + nm);
@@ -2085,7 +2142,6 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
         for (int m = 0; m < nm; ++m) b->order_h[m] = m;
         std::stable_sort(b->order_h.begin(), b->order_h.end(),
                          [&](int x, int y) { return b->mem[x]->N > b->mem[y]->N; });
-        BHIPCHECK(b, hipMemcpy(b->order.p, b->order_h.data(), nm * sizeof(int), hipMemcpyHostToDevice));
     }
     if (b->tsize_fixed == 0 && b->cands.empty()) {
         for (int ts : {32, 16, 8, 4, 2, 1})
@@ -2110,7 +2166,10 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
         float ms = 0;
         double work = 0, cap_share = 0;
         const int cap = trial && !b->tune_window ? TUNE_ATTEMPTS : LAUNCH_ATTEMPTS;
-        int st = batch_launch(b, mt_double, lds, gmax, ts, cap, fsal, t_target, &ms, &work, &cap_share);
+        BatchScratch sc;
+        int st = batch_scratch_alloc(b, nm, std::max(1, gmax / ts), ts, sc);
+        if (st) return st;
+        st = batch_launch(b, sc, mt_double, lds, gmax, ts, cap, fsal, t_target, &ms, &work, &cap_share);
         if (st) return st;
         const bool window_start = first_launch;
         const bool whole_window = first_launch && done();
@@ -2140,16 +2199,33 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
     return FI_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) {
+    return batch_integrate(b, t_targets, fsal);
+}
+
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {
     if (!b || !o) return FI_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));
     o->n_members = (int32_t)b->mem.size();
-    o->team_size = b->last_tsize;
-    o->n_teams = b->last_nteams;
     o->tuned = (b->tsize_fixed > 0 || b->tsize_best > 0) ? 1 : 0;
+    // the size in use once tuned (a given size is capped by the co-resident workgroups at launch)
+    o->team_size = b->tsize_fixed > 0 ? (b->last_tsize > 0 ? b->last_tsize : b->tsize_fixed)
+                   : b->tsize_best > 0 ? b->tsize_best : b->last_tsize;
+    o->last_team_size = b->last_tsize;
+    o->n_teams = b->last_nteams;
     o->device_ms = b->device_ms;
     o->n_launch = b->n_launch;
     o->cell_attempts = b->cell_attempts;
+    const int nc = (int)std::min<size_t>(b->cands.size(), FI_BATCH_MAX_CANDIDATES);
+    o->n_candidates = b->tsize_fixed > 0 ? 0 : nc;
+    for (int k = 0; k < o->n_candidates; ++k) {
+        o->cand_size[k] = b->cands[k];
+        o->cand_rate[k] = b->cand_rate[k];
+    }
     return FI_OK;
 }
 

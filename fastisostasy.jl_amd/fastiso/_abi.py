@@ -10,7 +10,7 @@ import os
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ENGINE_PATH = os.path.join(PKG_ROOT, "lib", "libfastiso_hip.so")
 
-FI_ABI_VERSION = 4
+FI_ABI_VERSION = 5
 
 # status codes
 FI_OK = 0
@@ -78,10 +78,16 @@ class FiBatchInfo(C.Structure):
     _fields_ = [
         ("n_members", C.c_int32), ("team_size", C.c_int32), ("n_teams", C.c_int32), ("tuned", C.c_int32),
         ("device_ms", C.c_double), ("n_launch", C.c_int64), ("cell_attempts", C.c_double),
+        ("last_team_size", C.c_int32), ("n_candidates", C.c_int32),
+        ("cand_size", C.c_int32 * 8), ("cand_rate", C.c_double * 8),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        n = self.n_candidates
+        d["cand_size"] = list(self.cand_size)[:n]
+        d["cand_rate"] = list(self.cand_rate)[:n]
+        return d
 
 
 OUTPUT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
@@ -169,8 +175,14 @@ def declare_surrogate(lib):
 _ENGINE = None
 
 
-def load_engine(path=None):
-    """Load the HIP engine (libfastiso_hip.so). Raises if it is missing: no CPU fallback exists."""
+ENGINE_NAME = b"hip-gfx950"
+
+
+def load_engine(path=None, allow_other_engine=False):
+    """Load the HIP engine (libfastiso_hip.so). Raises if it is missing, if its ABI version differs,
+    or if the library is another implementation of the ABI (the CPU oracle exports the same
+    symbols): there is no CPU fallback. allow_other_engine lifts the last check for callers that
+    load another implementation on purpose."""
     global _ENGINE
     if path is None and _ENGINE is not None:
         return _ENGINE
@@ -180,9 +192,26 @@ def load_engine(path=None):
         raise RuntimeError(
             f"HIP engine not built: {p} is missing. Build it with "
             f"`python fastisostasy.jl_amd/build.py` (hipcc --offload-arch=gfx950).")
-    lib = declare_surrogate(declare(C.CDLL(p)))
+    lib = C.CDLL(p)
+    for sym in ("fi_abi_version", "fi_engine_name"):
+        if not hasattr(lib, sym):
+            raise RuntimeError(f"{p} does not export {sym}: not a build of include/fastiso.h")
+    lib.fi_abi_version.restype = C.c_int32
+    lib.fi_engine_name.restype = C.c_char_p
     if lib.fi_abi_version() != FI_ABI_VERSION:
-        raise RuntimeError("libfastiso_hip.so ABI version mismatch")
+        raise RuntimeError(f"{p}: ABI version {lib.fi_abi_version()}, expected {FI_ABI_VERSION}")
+    name = lib.fi_engine_name()
+    if name != ENGINE_NAME and not allow_other_engine:
+        raise RuntimeError(f"{p} is the {name.decode()} engine, not {ENGINE_NAME.decode()}")
+    lib = declare(lib)
+    if name == ENGINE_NAME:
+        declare_surrogate(lib)
+    lib._fastiso_path = os.path.abspath(p)
     if path is None:
         _ENGINE = lib
     return lib
+
+
+def engine_identity(lib):
+    """(engine name, absolute path of the loaded library) of a library returned by load_engine."""
+    return lib.fi_engine_name().decode(), getattr(lib, "_fastiso_path", None)

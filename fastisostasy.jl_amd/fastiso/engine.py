@@ -60,6 +60,13 @@ class Engine:
         e._owned = False
         return e
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def close(self):
         if getattr(self, "h", None):
             if getattr(self, "_owned", True):

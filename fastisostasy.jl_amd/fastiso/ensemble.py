@@ -70,6 +70,13 @@ class Ensemble:
         self._check(self.lib.fi_batch_get_info(self.h, C.byref(s)))
         return s.as_dict()
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def close(self):
         if getattr(self, "h", None):
             for e in self.members:

@@ -157,6 +157,18 @@ class Simulation:
     def stats(self):
         return self.engine.stats()
 
+    def close(self):
+        """Free the device context now (instead of at garbage collection)."""
+        if self.engine is not None:
+            self.engine.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
 
 def push_inputs(eng, inputs):
     """Load assembled inputs into an engine context and run fi_init."""

@@ -84,6 +84,13 @@ class Surrogate:
             msg = self.lib.fi_surrogate_last_error(self.h)
             raise EngineError(st, msg.decode() if msg else "")
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.fi_surrogate_destroy(self.h)

@@ -38,13 +38,28 @@ struct FiStats
     n_rhs::Int64; n_accept::Int64; n_reject::Int64; count_sparse_updates::Int64; n_output::Int64
     V_af::Float64; V_den::Float64; V_pov::Float64; delta_V::Float64; z_bsl::Float64; last_EEst::Float64
     device_ms::Float64; n_launch::Int64; n_accept_timed::Int64
+    A_ocean::Float64
 end
 
+# field ids of include/fastiso.h (FI_*), keyed by the reference's field names
 const FI = (litho_rigidity = 1, effective_viscosity = 2, scaled_pseudodiff_inv = 3, maskactive = 4,
             green_elastic = 5, green_geoid = 6, ref_u = 7, ref_ue = 8, ref_dz_ss = 9, ref_z_b = 10,
-            ref_H_ice = 11, distortion = 12, green_viscous = 13, tau = 14, u = 32, dudt = 33, ue = 34,
-            dz_ss = 35, z_ss = 36, z_b = 37, H_ice = 38, H_af = 39, H_water = 40, maskgrounded = 41,
-            maskocean = 42, u_eq = 49)
+            ref_H_ice = 11, distortion = 12, green_viscous = 13, tau = 14, litho_thickness = 15,
+            u = 32, dudt = 33, ue = 34, dz_ss = 35, z_ss = 36, z_b = 37, H_ice = 38, H_af = 39,
+            H_water = 40, maskgrounded = 41, maskocean = 42, canom_ice = 43, canom_seawater = 44,
+            canom_litho = 45, canom_mantle = 46, canom_load = 47, canom_full = 48, u_eq = 49,
+            u_x = 50, u_y = 51)
+
+# FI_ABI_VERSION of include/fastiso.h: the layouts of FiConfig and FiStats above follow it
+const FI_ABI_VERSION = 5
+
+function __init__()
+    v = ccall((:fi_abi_version, libfi), Int32, ())
+    v == FI_ABI_VERSION || error("$(libfi): ABI version $v, this binding expects $(FI_ABI_VERSION)")
+    name = unsafe_string(ccall((:fi_engine_name, libfi), Cstring, ()))
+    name == "hip-gfx950" || error("$(libfi) is the $(name) engine, not hip-gfx950")
+    return nothing
+end
 
 check(ctx, st) = st == 0 || error("fastiso HIP engine error $st: " *
     unsafe_string(ccall((:fi_last_error, libfi), Cstring, (Ptr{Cvoid},), ctx)))
@@ -100,6 +115,7 @@ function HipSimulation(sim)
         setfield!(ctx, FI.tau, se.tau .* ones(d.nx, d.ny))
     end
     d.correct_distortion && setfield!(ctx, FI.distortion, d.K)
+    setfield!(ctx, FI.litho_thickness, se.litho_thickness)   # read by the u_x / u_y outputs
     setfield!(ctx, FI.ref_u, sim.ref.u)
     setfield!(ctx, FI.ref_ue, sim.ref.ue)
     setfield!(ctx, FI.ref_z_b, sim.ref.z_b)

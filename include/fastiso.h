@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FI_ABI_VERSION 4
+#define FI_ABI_VERSION 5
 
 /* ---- status codes ------------------------------------------------------------------------ */
 enum {
@@ -311,14 +311,20 @@ int fi_checkpoint_load(fi_ctx* ctx, const void* buf, size_t bytes);
  * integrates members taken from a queue, largest first. */
 typedef struct fi_batch fi_batch;
 
+#define FI_BATCH_MAX_CANDIDATES 8
 typedef struct fi_batch_info {
     int32_t n_members;
-    int32_t team_size;     /* workgroups per team of the last launch */
-    int32_t n_teams;
+    int32_t team_size;     /* workgroups per team: the size in use once tuned (given, or chosen
+                              by timing), else the size of the last launch */
+    int32_t n_teams;       /* teams of the last launch */
     int32_t tuned;         /* 1 once the team size is fixed (given, or chosen by timing) */
     double device_ms;      /* kernel time of all ensemble launches (HIP events) */
     int64_t n_launch;
     double cell_attempts;  /* sum over members of BS3 attempts x grid cells, all launches */
+    int32_t last_team_size;              /* workgroups per team of the last launch */
+    int32_t n_candidates;                /* team sizes the tuner considers (0 with a given size) */
+    int32_t cand_size[FI_BATCH_MAX_CANDIDATES];
+    double cand_rate[FI_BATCH_MAX_CANDIDATES];  /* timed cell-attempts per ms, 0 if not timed */
 } fi_batch_info;
 
 /* Creates nmem member contexts, member m as fi_create(&cfgs[m]). Members are set up, initialised
@@ -345,6 +351,7 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal);
 
 /* As fi_batch_integrate with one target time per member (t_targets[m] for member m). */
 int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal);
+
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* out);
 
 #ifdef __cplusplus

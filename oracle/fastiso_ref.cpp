@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <omp.h>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -1122,6 +1123,12 @@ int fi_integrate(fi_ctx* c, double t_target, fi_output_cb cb, void* user) {
         }
         double EEst = std::sqrt(s / (double)N);
         c->EEst = EEst;
+        // OrdinaryDiffEq: a NaN error norm makes the PI controller propose a NaN step, and the
+        // integration stops with retcode DtNaN; an infinite one is rejected with factor 1/qmin
+        if (std::isnan(EEst)) {
+            c->nrej += 1;
+            return fail(c, FI_ERR_NONFINITE, "NaN dt detected: NaN in the state or its derivative");
+        }
         double q;
         if (!std::isfinite(EEst)) {
             c->q11 = 1.0 / QMIN * GAMMA * 10;
@@ -1332,6 +1339,10 @@ void ref_elastic_green(int nx, int ny, double Wx, double Wy, const double* rm, c
     vec G = elastic_green(nx, ny, Wx, Wy, r, g, nq);
     std::memcpy(out, G.data(), G.size() * sizeof(double));
 }
+
+// OpenMP threads of the oracle's parallel loops (the CPU baseline times all cores and one core)
+void ref_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int ref_max_threads(void) { return omp_get_max_threads(); }
 
 void ref_geoid_green(int nx, int ny, double Wx, double Wy, double r_pole, double mE, double* out) {
     vec G = geoid_green(nx, ny, Wx, Wy, r_pole, mE);

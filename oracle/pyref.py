@@ -87,6 +87,8 @@ def lib():
             "ref_pseudodiff": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp]),
             "ref_elastic_green": (None, [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_int, C.c_int, _dp]),
             "ref_geoid_green": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+            "ref_set_threads": (None, [C.c_int]),
+            "ref_max_threads": (C.c_int, []),
             "ref_solidearth": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
                                       C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
             "ref_analytic_solution": (C.c_double, [C.c_double] * 9 + [C.c_int]),
@@ -323,6 +325,15 @@ def pseudodiff(nx, ny, Wx, Wy):
     out = np.empty(nx * ny)
     lib().ref_pseudodiff(nx, ny, Wx, Wy, _p(out))
     return _shape(out, nx, ny)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops from now on."""
+    lib().ref_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().ref_max_threads())
 
 
 def elastic_green(nx, ny, Wx, Wy, rm, ge, nq=4):

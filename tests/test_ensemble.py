@@ -255,5 +255,41 @@ def test_gpu_elra_ensemble_tuner_settles_on_short_windows():
         np.testing.assert_array_equal(tuned[m].get_field("u"), fixed[m].get_field("u"), err_msg=f"member {m}")
         assert tuned[m].stats()["n_accept"] / 12 < 64, "windows are not shorter than a trial"
     if not DRYRUN:
+        t += 100.0
+        tuned.integrate(t)
         info = tuned.info()
         assert info["tuned"] == 1 and info["team_size"] in (1, 2, 4, 8, 16, 32), info
+        # the size in use is the fastest timed candidate, and later launches use it
+        timed = [(r, sz) for sz, r in zip(info["cand_size"], info["cand_rate"]) if r > 0]
+        assert timed and info["team_size"] == max(timed)[1], info
+        assert info["last_team_size"] == info["team_size"], info
+
+
+@pytest.mark.gpu
+def test_ensemble_members_held_at_their_time_across_tuner_launches():
+    """Members whose target equals their current time sit out a call while the others run
+    capped tuner trials (several launches per call): every member stays bitwise equal to a single
+    context stopped at the same times."""
+    inputs = small_ensemble(t_end=1200.0)
+    nm = len(inputs)
+    plan = []   # per call: target of each member; 200-yr windows exceed the 64-attempt trials
+    cur = [0.0] * nm
+    for k in range(1, 7):
+        tgt = [cur[m] if (m + k) % 3 == 0 else 200.0 * k for m in range(nm)]
+        plan.append(tgt)
+        cur = tgt
+    ens = hip_ensemble(inputs, 0)
+    ens.integrator_init()
+    for tgt in plan:
+        ens.integrate_each(tgt, fsal=True)
+    for m, inp in enumerate(inputs):
+        stops = sorted({tgt[m] for tgt in plan if tgt[m] > 0})
+        e = hip_single(with_tout(inp, stops))
+        e.integrator_init()
+        e.integrate(stops[-1], lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m}")
+        s0, s1 = ens[m].stats(), e.stats()
+        assert (s0["t"], s0["n_accept"], s0["n_reject"]) == (s1["t"], s1["n_accept"], s1["n_reject"]), m
+    if not DRYRUN:
+        info = ens.info()
+        assert info["n_launch"] > len(plan), info

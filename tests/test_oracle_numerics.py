@@ -161,6 +161,96 @@ def test_elastic_green_two_implementations_and_convergence(ref):
     assert Gr[12, 12] < 0
 
 
+def _cell_integral_r2(x1, x2, y1, y2):
+    """Closed form of the integral of x^2 + y^2 over [x1, x2] x [y1, y2]."""
+    return (x2 ** 3 - x1 ** 3) / 3 * (y2 - y1) + (x2 - x1) * (y2 ** 3 - y1 ** 3) / 3
+
+
+@pytest.mark.parametrize("n", [24, 25])
+def test_elastic_green_cells_closed_form(ref, n):
+    """get_elastic_green / get_loadgreen / quadrature2D (src/material.jl:299-364,
+    src/utils.jl:146-186) on a synthetic table GE = c r^3 (dense, so its linear interpolant is
+    r^3 to ~1e-7): the integrand GE(r) / (r 1e12) = c (x^2 + y^2) / 1e12 is a polynomial that
+    4-point Gauss-Legendre integrates exactly, so every cell inside the table's range equals
+    c / 1e12 times the closed-form integral over [p dx, (p+1) dx] x [q dy, (q+1) dy] with
+    p = i - mx - 1, q = j - my - 1 (1-based i, j; mx = nx div 2). A one-cell shift, a lost
+    1e12 or a wrong cell size fails; cells beyond rm[end] are zero."""
+    W = 3e6
+    d = fastiso.RegionalDomain(W, W, n, n)
+    r_end = 2.2e6
+    rm = np.linspace(0.0, r_end, 400001)
+    c = 7.0e-6
+    ge = c * rm ** 3
+    for G in (fastiso.elastic_green(d, rm, ge, quad_precision=4), ref.elastic_green(n, n, W, W, rm, ge, 4)):
+        p = np.arange(1, n + 1) - n // 2 - 1
+        x1, x2 = p * d.dx, (p + 1) * d.dx
+        want = c / 1e12 * _cell_integral_r2(x1[:, None], x2[:, None], x1[None, :], x2[None, :])
+        # distance of the cell's farthest / nearest corner from the origin
+        far = np.hypot(np.maximum(abs(x1), abs(x2))[:, None], np.maximum(abs(x1), abs(x2))[None, :])
+        near = np.hypot(np.where(p >= 0, x1, -x2)[:, None], np.where(p >= 0, x1, -x2)[None, :])
+        inside, outside = far < r_end, near > r_end
+        assert inside.sum() > n and outside.sum() > n
+        np.testing.assert_allclose(G[inside], want[inside], rtol=2e-6)
+        assert np.all(G[outside] == 0.0)
+        straddle = ~inside & ~outside
+        assert np.all((G[straddle] >= 0) & (G[straddle] <= want[straddle] * (1 + 1e-6)))
+        # the planted mistakes this test exists for
+        assert not np.allclose(np.roll(G, 1, axis=0)[inside], want[inside], rtol=1e-3)
+        assert not np.allclose(1e12 * G[inside], want[inside], rtol=1e-3)
+
+
+def test_elastic_green_small_distance_branch(ref):
+    """get_loadgreen below r = 0.01 m returns GE[1] / (rm[2] 1e12) (src/material.jl:323-324): on
+    a 4-cm domain (5-mm cells) every node of the four centre cells lies below 0.01 m, so those
+    cells equal that constant times the cell area; doubling GE[1] doubles them."""
+    n, W = 8, 0.02
+    d = fastiso.RegionalDomain(W, W, n, n)
+    rm = np.array([0.0, 0.5, 1.0, 2.0])
+    ge = np.array([-3.0, -2.0, -1.0, 0.5])
+    p = np.arange(1, n + 1) - n // 2 - 1
+    x1, x2 = p * d.dx, (p + 1) * d.dx
+    far = np.hypot(np.maximum(abs(x1), abs(x2))[:, None], np.maximum(abs(x1), abs(x2))[None, :])
+    small = far < 0.01
+    assert small.sum() >= 4
+    ge2 = ge.copy()
+    ge2[0] = -6.0
+    for table in (lambda g: fastiso.elastic_green(d, rm, g, quad_precision=4),
+                  lambda g: ref.elastic_green(n, n, W, W, rm, g, 4)):
+        G, G2 = table(ge), table(ge2)
+        np.testing.assert_allclose(G[small], ge[0] / (rm[1] * 1e12) * d.dx * d.dy, rtol=1e-12)
+        np.testing.assert_allclose(G2[small], 2 * G[small], rtol=1e-12)
+        assert np.all(G[~small] != G[small][0])
+
+
+def test_farrell_table_columns():
+    """The shipped table: rm is the arc length of the angular distance on a 6371-km sphere
+    (Farrell 1972, table A3; the reference reads rm and GE as columns 2 and 3,
+    src/dataloaders.jl:257-273), GE is negative near the load and its published first rows."""
+    d = np.loadtxt(fastiso.greens.os.path.join(fastiso.greens._DATA, "elasticgreencoeffs_farrell1972.csv"),
+                   delimiter=",", skiprows=1)
+    theta, rm, ge = d[:, 0], d[:, 1], d[:, 2]
+    arc = np.deg2rad(theta) * 6.371e6
+    np.testing.assert_allclose(rm[3:], arc[3:], rtol=3e-4)
+    np.testing.assert_allclose(rm[1:3], arc[1:3], rtol=2e-2)
+    assert rm[0] == 0 and theta[0] == 0 and np.all(np.diff(rm) > 0)
+    assert ge[0] == -33.6488 and ge[1] == -33.64 and ge[2] == -33.56
+
+
+def test_farrell_csv_equals_reference_jld2_bytes():
+    """The CSV holds the float64 arrays of the reference's src/input/elasticgreencoeffs_farrell1972.jld2
+    (read as data). Runs where the reference is mounted."""
+    import os
+    src = "/root/reference/src/input/elasticgreencoeffs_farrell1972.jld2"
+    if not os.path.exists(src):
+        pytest.skip("reference not mounted")
+    raw = open(src, "rb").read()
+    d = np.loadtxt(fastiso.greens.os.path.join(fastiso.greens._DATA, "elasticgreencoeffs_farrell1972.csv"),
+                   delimiter=",", skiprows=1)
+    for k, off in enumerate((622, 1024, 1426)):
+        col = np.frombuffer(raw[off:off + 42 * 8], dtype="<f8")
+        assert np.array_equal(col, d[:, k])
+
+
 def test_solidearth_two_implementations(ref):
     d = fastiso.RegionalDomain(3e6, 3e6, 16, 16)
     rng = np.random.default_rng(3)
