@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2092,9 +2093,81 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
 
 namespace {
 
-int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal) {
+// Every device buffer an ensemble launch needs beyond the members' own, for the entry point that
+// allocates nothing (fi_batch_integrate_each_noalloc): one file-scope array per device, carved
+// per launch. Its host-side name is not a device address; host code uses the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch()).
+constexpr int NOALLOC_MAX_MEMBERS = FI_NOALLOC_SCRATCH_MAX_MEMBERS;
+constexpr int NOALLOC_MAX_WORKGROUPS = FI_NOALLOC_SCRATCH_MAX_WORKGROUPS;
+constexpr size_t scratch_al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    scratch_al(NOALLOC_MAX_MEMBERS * sizeof(KP)) + scratch_al(NOALLOC_MAX_MEMBERS * sizeof(Ctl)) +
+    scratch_al(NOALLOC_MAX_MEMBERS * sizeof(const KP*)) + scratch_al(NOALLOC_MAX_MEMBERS * sizeof(int)) +
+    scratch_al(sizeof(int)) + scratch_al(2 * NOALLOC_MAX_WORKGROUPS * sizeof(int)) +
+    scratch_al(3 * (size_t)NOALLOC_MAX_WORKGROUPS * sizeof(Bar)) + scratch_al(sizeof(BatchArgs));
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+unsigned char* noalloc_ptr[NOALLOC_MAX_DEVICES];   // device address of scratch, per device
+
+// the device address of this device's scratch array (looked up once per device, with the device
+// current)
+int noalloc_scratch(fi_batch* b, unsigned char** out) {
+    if (b->dev < 0 || b->dev >= NOALLOC_MAX_DEVICES) return bfail(b, FI_ERR_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[b->dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return bfail(b, FI_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        noalloc_ptr[b->dev] = (unsigned char*)p;
+    }
+    *out = noalloc_ptr[b->dev];
+    return FI_OK;
+}
+
+// the scratch of one launch carved from the device's static array; sizes beyond its maxima are
+// refused
+int batch_scratch_static(fi_batch* b, unsigned char* base, int nm, int gmax, BatchScratch& sc) {
+    if (nm > NOALLOC_MAX_MEMBERS)
+        return bfail(b, FI_ERR_INVALID, "fi_batch_integrate_each_noalloc: " + std::to_string(nm) +
+                                            " members exceed the static scratch maximum of " +
+                                            std::to_string(NOALLOC_MAX_MEMBERS));
+    if (gmax > NOALLOC_MAX_WORKGROUPS)
+        return bfail(b, FI_ERR_INVALID, "fi_batch_integrate_each_noalloc: " + std::to_string(gmax) +
+                                            " co-resident workgroups exceed the static scratch maximum of " +
+                                            std::to_string(NOALLOC_MAX_WORKGROUPS));
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += scratch_al(bytes);
+        return p;
+    };
+    sc.kp_all = (KP*)take(NOALLOC_MAX_MEMBERS * sizeof(KP));
+    sc.ctl_all = (Ctl*)take(NOALLOC_MAX_MEMBERS * sizeof(Ctl));
+    sc.kps = (const KP**)take(NOALLOC_MAX_MEMBERS * sizeof(const KP*));
+    sc.order = (int*)take(NOALLOC_MAX_MEMBERS * sizeof(int));
+    sc.queue = (int*)take(sizeof(int));
+    sc.team_member = (int*)take(2 * NOALLOC_MAX_WORKGROUPS * sizeof(int));
+    // nteams (1 + 2 ts) <= 3 gmax barrier records for every team size the launch can use
+    sc.bars = (Bar*)take(3 * (size_t)NOALLOC_MAX_WORKGROUPS * sizeof(Bar));
+    sc.args = (BatchArgs*)take(sizeof(BatchArgs));
+    return off <= NOALLOC_SCRATCH_BYTES ? FI_OK : bfail(b, FI_ERR_INVALID, "static scratch layout overflow");
+}
+
+int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noalloc) {
     if (!b || !t_targets) return FI_ERR_INVALID;
     const int nm = (int)b->mem.size();
+    if (noalloc && nm > NOALLOC_MAX_MEMBERS)
+        return bfail(b, FI_ERR_INVALID, "fi_batch_integrate_each_noalloc: " + std::to_string(nm) +
+                                            " members exceed the static scratch maximum of " +
+                                            std::to_string(NOALLOC_MAX_MEMBERS));
     const std::vector<double> t_target(t_targets, t_targets + nm);
     bool mt_double = true, dist = false;
     size_t lds_d = 0, lds_dd = 0, lds_f = 0;
@@ -2143,6 +2216,11 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal) {
         std::stable_sort(b->order_h.begin(), b->order_h.end(),
                          [&](int x, int y) { return b->mem[x]->N > b->mem[y]->N; });
     }
+    unsigned char* sbase = nullptr;
+    if (noalloc) {
+        const int st = noalloc_scratch(b, &sbase);
+        if (st) return st;
+    }
     if (b->tsize_fixed == 0 && b->cands.empty()) {
         for (int ts : {32, 16, 8, 4, 2, 1})
             if (ts <= gmax) b->cands.push_back(ts);
@@ -2167,7 +2245,8 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal) {
         double work = 0, cap_share = 0;
         const int cap = trial && !b->tune_window ? TUNE_ATTEMPTS : LAUNCH_ATTEMPTS;
         BatchScratch sc;
-        int st = batch_scratch_alloc(b, nm, std::max(1, gmax / ts), ts, sc);
+        int st = noalloc ? batch_scratch_static(b, sbase, nm, gmax, sc)
+                         : batch_scratch_alloc(b, nm, std::max(1, gmax / ts), ts, sc);
         if (st) return st;
         st = batch_launch(b, sc, mt_double, lds, gmax, ts, cap, fsal, t_target, &ms, &work, &cap_share);
         if (st) return st;
@@ -2204,7 +2283,11 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal) {
 extern "C" {
 
 int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) {
-    return batch_integrate(b, t_targets, fsal);
+    return batch_integrate(b, t_targets, fsal, false);
+}
+
+int fi_batch_integrate_each_noalloc(fi_batch* b, const double* t_targets, int32_t fsal) {
+    return batch_integrate(b, t_targets, fsal, true);
 }
 
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {

@@ -129,6 +129,7 @@ def declare(lib):
         "fi_batch_set_team_size": (C.c_int, [P, C.c_int32]),
         "fi_batch_integrate": (C.c_int, [P, C.c_double, C.c_int32]),
         "fi_batch_integrate_each": (C.c_int, [P, _dp, C.c_int32]),
+        "fi_batch_integrate_each_noalloc": (C.c_int, [P, _dp, C.c_int32]),
         "fi_batch_get_info": (C.c_int, [P, C.POINTER(FiBatchInfo)]),
     }
     for name, (res, args) in sig.items():

@@ -352,6 +352,18 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal);
 /* As fi_batch_integrate with one target time per member (t_targets[m] for member m). */
 int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal);
 
+/* fi_batch_integrate_each without device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory while they run (tight serving loops, harnesses that
+ * forbid hipMalloc). fi_batch_integrate_each may allocate and grow per-launch buffers (member
+ * parameter blocks, barrier regions, queue); this variant takes them from one static device
+ * array per GPU of FI_NOALLOC_SCRATCH_MAX_MEMBERS members and FI_NOALLOC_SCRATCH_MAX_WORKGROUPS
+ * co-resident workgroups (about 2 MB, held until the process ends) and returns FI_ERR_INVALID
+ * beyond those sizes. Set-up (fi_create_batch, fi_init, fi_integrator_init) still allocates.
+ * Results equal fi_batch_integrate_each's bit for bit. Every call on one GPU shares that array:
+ * calls that overlap on one GPU (from different host threads) race. */
+#define FI_NOALLOC_SCRATCH_MAX_MEMBERS 1024
+#define FI_NOALLOC_SCRATCH_MAX_WORKGROUPS 1024
+int fi_batch_integrate_each_noalloc(fi_batch* b, const double* t_targets, int32_t fsal);
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* out);
 
 #ifdef __cplusplus

@@ -1536,6 +1536,11 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
     return FI_OK;
 }
 
+// the CPU oracle allocates no device memory in either entry point
+int fi_batch_integrate_each_noalloc(fi_batch* b, const double* t_targets, int32_t fsal) {
+    return fi_batch_integrate_each(b, t_targets, fsal);
+}
+
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {
     if (!b || !o) return FI_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));
