@@ -500,6 +500,42 @@ __global__ void k_dudt_out(const float* __restrict__ k, double off, long n, doub
 __global__ void k_sub_offset(double* u, double off, long n) {
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) u[c] -= off;
 }
+// One RKC2 stage on the FP64 state: y = a0 y0 + a1 ym1 + a2 ym2 + b1 f1 + b0 f0, the derivatives
+// f read from their FP32 K slots minus their corner-BC offsets (zero for ELRA, whose derivatives
+// are stored whole). y may be ym2 (the stage recursion overwrites Y_{j-2}).
+__device__ __forceinline__ double kslot_off_dev(const float* kc, double vbc) {
+    const double a = 0.25 * (double)kc[0], b = 0.25 * (double)kc[1], c = 0.25 * (double)kc[2], d = 0.25 * (double)kc[3];
+    return ((a + b) + (c + d)) - vbc;
+}
+__global__ void __launch_bounds__(256) k_rkc_stage(double* y, const double* __restrict__ y0, const double* __restrict__ ym1,
+                                                   const double* ym2, const float* __restrict__ f1,
+                                                   const float* __restrict__ f0, const float* __restrict__ kc1,
+                                                   const float* __restrict__ kc0, int offsets, double vbc, double a0,
+                                                   double a1, double a2, double b1, double b0, long n) {
+    const double off1 = (offsets && b1 != 0.0) ? kslot_off_dev(kc1, vbc) : 0.0;
+    const double off0 = (offsets && b0 != 0.0) ? kslot_off_dev(kc0, vbc) : 0.0;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        double v = a0 * y0[c];
+        if (a1 != 0.0) v += a1 * ym1[c];
+        if (a2 != 0.0) v += a2 * ym2[c];
+        if (b1 != 0.0) v += b1 * ((double)f1[c] - off1);
+        if (b0 != 0.0) v += b0 * ((double)f0[c] - off0);
+        y[c] = v;
+    }
+}
+// the corner-BC offset of u (src/boundary_conditions.jl:144-148) into *off, then u -= *off
+__global__ void k_corner_offset(const double* __restrict__ u, int nx, int ny, double xb, double* off) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long n2 = (long)(ny - 1) * nx;
+        const double a = 0.25 * u[0], b = 0.25 * u[nx - 1], c = 0.25 * u[n2], d = 0.25 * u[n2 + nx - 1];
+        *off = ((a + b) + (c + d)) - xb;
+    }
+}
+__global__ void k_sub_offset_dev(double* u, const double* __restrict__ off, long n) {
+    const double o = *off;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) u[c] -= o;
+}
+
 // thinplate_horizontal_displacement!(u_x, u_y, u + ue, h, domain) (src/deformation.jl:141-164):
 // -h/2 times the first derivative along x (axis 0) or y (axis 1) with the CPU edge rules
 // (src/derivatives.jl:171-191) and the per-cell step K dx or K dy. A constant offset of u (its
@@ -605,6 +641,8 @@ struct fi_ctx {
     DBuf<double2> ge_hat, gg_hat, gv_hat, tw64, gtmp;
     DBuf<float> inv_tau_d, ueq, litho_h_d, imp_am1, imp_b;
     DBuf<double> uxy_d;   // horizontal displacement output (FI_U_X / FI_U_Y)
+    DBuf<double> rkc_off;   // corner offset scratch of fi_integrate_rkc
+    double rkc_rho = 0;     // spectral radius estimate of fi_integrate_rkc (1/yr)
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
@@ -1706,6 +1744,173 @@ int fi_set_time(fi_ctx* c, double t) {
     c->ctl.t = t;
     c->ctl.last_u = c->ctl.cur;
     c->k1_stale = true;
+    return FI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// RKC2 (Sommeijer, Shampine & Verwer 1997): with w0 = 1 + eps/s^2 (eps = 2/13), Chebyshev values
+// T_j(w0) and their first two derivatives, w1 = T_s'/T_s'', b_j = T_j''/T_j'^2 (b_0 = b_1 = b_2):
+//   Y_1 = Y_0 + mut_1 h F_0,  Y_j = (1 - mu_j - nu_j) Y_0 + mu_j Y_{j-1} + nu_j Y_{j-2}
+//                                   + mut_j h F_{j-1} + gt_j h F_0,
+//   mu_j = 2 b_j w0/b_{j-1}, nu_j = -b_j/b_{j-2}, mut_j = 2 b_j w1/b_{j-1},
+//   gt_j = -(1 - b_{j-1} T_{j-1}) mut_j, stage times c_j = mu_j c_{j-1} + nu_j c_{j-2} + mut_j + gt_j;
+// real stability interval beta(s) = (w0 + 1) T_s''/T_s'.
+struct Rkc {
+    int s = 0;
+    std::vector<double> mu, nu, mut, gt, c;
+    double beta = 0;
+};
+Rkc rkc_make(int s) {
+    Rkc k;
+    k.s = s;
+    const double w0 = 1.0 + (2.0 / 13.0) / ((double)s * (double)s);
+    std::vector<double> T(s + 1), d1(s + 1), d2(s + 1), b(s + 1);
+    T[0] = 1.0, T[1] = w0, d1[0] = 0.0, d1[1] = 1.0, d2[0] = 0.0, d2[1] = 0.0;
+    for (int j = 2; j <= s; ++j) {
+        T[j] = 2.0 * w0 * T[j - 1] - T[j - 2];
+        d1[j] = 2.0 * T[j - 1] + 2.0 * w0 * d1[j - 1] - d1[j - 2];
+        d2[j] = 4.0 * d1[j - 1] + 2.0 * w0 * d2[j - 1] - d2[j - 2];
+    }
+    const double w1 = d1[s] / d2[s];
+    for (int j = 2; j <= s; ++j) b[j] = d2[j] / (d1[j] * d1[j]);
+    b[0] = b[1] = b[2];
+    k.mu.assign(s + 1, 0.0), k.nu.assign(s + 1, 0.0), k.mut.assign(s + 1, 0.0), k.gt.assign(s + 1, 0.0);
+    k.c.assign(s + 1, 0.0);
+    k.mut[1] = b[1] * w1;
+    k.c[1] = k.mut[1];
+    for (int j = 2; j <= s; ++j) {
+        k.mu[j] = 2.0 * b[j] * w0 / b[j - 1];
+        k.nu[j] = -b[j] / b[j - 2];
+        k.mut[j] = 2.0 * b[j] * w1 / b[j - 1];
+        k.gt[j] = -(1.0 - b[j - 1] * T[j - 1]) * k.mut[j];
+        k.c[j] = k.mu[j] * k.c[j - 1] + k.nu[j] * k.c[j - 2] + k.mut[j] + k.gt[j];
+    }
+    k.beta = (w0 + 1.0) * d2[s] / d1[s];
+    return k;
+}
+// the fewest stages (>= 2) with beta(s) >= x
+int rkc_stage_count(double x) {
+    int s = 2;
+    while (rkc_make(s).beta < x && s < 100000) s = std::max(s + 1, (int)(s * 1.05));
+    while (s > 2 && rkc_make(s - 1).beta >= x) --s;
+    return s;
+}
+// start vector of the power iteration: uniform in [-1, 1) per cell (splitmix64 of the cell index)
+double rkc_start(uint64_t p) {
+    uint64_t z = 0x9E3779B97F4A7C15ull * (p + 1) + 0x5EEDull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// spectral radius of the right-hand side's Jacobian at (U[cur], t) by 40 power iterations on
+// J v = f(u + v) - f(u) (the right-hand side is affine in u); U[scratch] and K slots 0/1 are
+// overwritten
+int rkc_spectral_radius(fi_ctx* c, double t, int cur, double* rho) {
+    const size_t N = c->N;
+    std::vector<double> u, f0, f1, v(N), w(N);
+    int st;
+    if ((st = read_u(c, cur, u))) return st;
+    if ((st = rhs_on_device(c, t, cur, 0)) || (st = read_k(c, 0, f0))) return st;
+    auto rms = [&](const std::vector<double>& a) {
+        double q = 0;
+        for (double x : a) q += x * x;
+        return std::sqrt(q / (double)N);
+    };
+    for (size_t p = 0; p < N; ++p) v[p] = rkc_start(p);
+    const double n0 = rms(v);
+    for (double& x : v) x /= n0;
+    const int sb = (cur + 1) % 3;
+    double lam = 0;
+    for (int it = 0; it < 40; ++it) {
+        for (size_t p = 0; p < N; ++p) w[p] = u[p] + v[p];
+        HIPCHECK(c, hipMemcpy(c->U.p + (size_t)sb * N, w.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        if ((st = rhs_on_device(c, t, sb, 1)) || (st = read_k(c, 1, f1))) return st;
+        for (size_t p = 0; p < N; ++p) v[p] = f1[p] - f0[p];
+        lam = rms(v);
+        if (!(lam > 0) || !std::isfinite(lam)) return fail(c, FI_ERR_NONFINITE, "fi_integrate_rkc: power iteration");
+        for (double& x : v) x /= lam;
+    }
+    *rho = lam;
+    return FI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_rkc_info* info) {
+    if (!c || !(dt_max > 0)) return FI_ERR_INVALID;
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_integrate_rkc before fi_integrator_init");
+    if (c->implicit) return fail(c, FI_ERR_UNSUPPORTED, "fi_integrate_rkc: not with the implicit step");
+    if (!(t_target >= c->ctl.t)) return fail(c, FI_ERR_INVALID, "fi_integrate_rkc: target before current time");
+    HIPCHECK(c, hipSetDevice(c->dev));
+    const long long r0 = c->ctl.nrhs;
+    const size_t N = c->N;
+    int st;
+    if (!(rho > 0)) {
+        if (!(c->rkc_rho > 0) && (st = rkc_spectral_radius(c, c->ctl.t, c->ctl.cur, &c->rkc_rho))) return st;
+        rho = c->rkc_rho;
+    }
+    const double t_start = c->ctl.t, span = t_target - t_start;
+    const int nsub = span > 0 ? (int)std::ceil(span / dt_max - 1e-12) : 0;
+    const double h = nsub > 0 ? span / nsub : 0.0;
+    const int s = nsub > 0 ? rkc_stage_count(1.2 * rho * h) : 0;
+    if (nsub > 0) {
+        if (!c->rkc_off.p) HIPCHECK(c, c->rkc_off.alloc(1));
+        const Rkc k = rkc_make(s);
+        const int offs = c->elra ? 0 : 1;
+        const double vbc = c->cfg.viscous_x_border;
+        const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
+        double* U = c->U.p;
+        const float* K0 = c->K.p;
+        const float* K1 = c->K.p + N;
+        const float* kc0 = c->kcorner.p;
+        const float* kc1 = c->kcorner.p + 4;
+        int y0 = c->ctl.cur;
+        for (int sub = 0; sub < nsub; ++sub) {
+            const double t0 = t_start + sub * h;
+            const int a = (y0 + 1) % 3, b = (y0 + 2) % 3;
+            if ((st = rhs_on_device(c, t0, y0, 0))) return st;   // F_0
+            hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + a * N, U + y0 * N, U + y0 * N,
+                               (const double*)nullptr, K1, K0, kc1, kc0, offs, vbc, 1.0, 0.0, 0.0, 0.0,
+                               k.mut[1] * h, (long)N);
+            HIPCHECK(c, hipGetLastError());
+            int pm1 = a, pm2 = y0;
+            for (int j = 2; j <= s; ++j) {
+                if ((st = rhs_on_device(c, t0 + k.c[j - 1] * h, pm1, 1))) return st;   // F_{j-1}
+                const int out = (j == 2) ? b : pm2;                                   // never Y_0
+                hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + out * N, U + y0 * N,
+                                   U + pm1 * N, U + pm2 * N, K1, K0, kc1, kc0, offs, vbc,
+                                   1.0 - k.mu[j] - k.nu[j], k.mu[j], k.nu[j], k.mut[j] * h, k.gt[j] * h, (long)N);
+                HIPCHECK(c, hipGetLastError());
+                pm2 = pm1;
+                pm1 = out;
+            }
+            hipLaunchKernelGGL(k_corner_offset, dim3(1), dim3(64), 0, c->stream, U + pm1 * N, c->nx, c->ny, vbc,
+                               c->rkc_off.p);
+            hipLaunchKernelGGL(k_sub_offset_dev, dim3(nb), dim3(256), 0, c->stream, U + pm1 * N, c->rkc_off.p,
+                               (long)N);
+            HIPCHECK(c, hipGetLastError());
+            y0 = pm1;
+        }
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->ctl.cur = y0;
+        c->ctl.last_u = y0;
+    }
+    c->ctl.t = t_target;
+    c->k1_stale = true;
+    if (info) {
+        info->rho = rho;
+        info->h = h;
+        info->substeps = nsub;
+        info->stages = s;
+        info->n_rhs = c->ctl.nrhs - r0;
+    }
     return FI_OK;
 }
 

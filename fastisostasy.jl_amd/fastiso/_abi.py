@@ -90,6 +90,14 @@ class FiBatchInfo(C.Structure):
         return d
 
 
+class FiRkcInfo(C.Structure):
+    _fields_ = [("rho", C.c_double), ("h", C.c_double), ("substeps", C.c_int32), ("stages", C.c_int32),
+                ("n_rhs", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 OUTPUT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
 
 _dp = C.POINTER(C.c_double)
@@ -119,6 +127,7 @@ def declare(lib):
         "fi_integrate": (C.c_int, [P, C.c_double, OUTPUT_CB, C.c_void_p]),
         "fi_get_stats": (C.c_int, [P, C.POINTER(FiStats)]),
         "fi_set_time": (C.c_int, [P, C.c_double]),
+        "fi_integrate_rkc": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.POINTER(FiRkcInfo)]),
         "fi_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
         "fi_checkpoint_save": (C.c_int, [P, C.c_void_p, C.c_size_t]),
         "fi_checkpoint_load": (C.c_int, [P, C.c_void_p, C.c_size_t]),

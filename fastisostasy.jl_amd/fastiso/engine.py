@@ -152,6 +152,14 @@ class Engine:
         buf = C.create_string_buffer(blob, len(blob))
         self._check(self.lib.fi_checkpoint_load(self.h, buf, len(blob)))
 
+    def integrate_rkc(self, t_target, dt_max, rho=0.0):
+        """fi_integrate_rkc: RKC2 sub-steps of at most dt_max years to t_target (rho > 0: the
+        spectral radius to size the stages with, else the context's estimate). Returns the
+        fi_rkc_info fields."""
+        info = _abi.FiRkcInfo()
+        self._check(self.lib.fi_integrate_rkc(self.h, float(t_target), float(dt_max), float(rho), C.byref(info)))
+        return info.as_dict()
+
     def set_time(self, t):
         """Move the integrator time to t without integrating (after set_field("u", ...))."""
         self._check(self.lib.fi_set_time(self.h, float(t)))

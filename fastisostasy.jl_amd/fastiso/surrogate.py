@@ -285,24 +285,83 @@ def predict(sur, macro, u, ue, du_prev, t):
 class SurrogateRunner:
     """Surrogate-with-fallback integration of a HIP Simulation in macro steps of macro_dt years.
 
-    The first `warm` macro steps are plain (they give the first increments to extrapolate, past the
-    initial transient). Without the network a step extrapolates the increments in time: constant
-    (order 1) or changing linearly (order 2, du_k = 2 du_{k-1} - du_{k-2}); the network corrects the
-    order-1 extrapolation. Then the run proceeds in periods of check_every macro steps: the
-    surrogate takes the first check_every - 1 (fi_set_field(u) + fi_set_time), the plain solver
-    takes the last one and is compared with the surrogate's prediction of that step. If
-    max |du_plain - du_sur| <= tol the period is kept; otherwise the run rolls back to the start
-    of the period and the plain solver integrates the whole period (the fallback). At each check the
-    predictor of the next period is the one -- extrapolation alone or extrapolation + network --
-    that was closer to the plain solver."""
+    The run proceeds in periods of check_every macro steps. The predictor takes the first
+    check_every - 1 steps; for the last one it predicts the step, the state is restored to the
+    step's start and the plain solver integrates it. If max |u_plain - u_predicted| <= tol the
+    period is kept (continuing from the plain state); otherwise the run restores the state of the
+    period's start and the plain solver integrates the whole period (the fallback). With an engine
+    that has checkpoints (fi_checkpoint_save / fi_checkpoint_load) every restore is exact, so a
+    fallen-back period equals the plain run bit for bit.
 
-    def __init__(self, sim, sur, macro, check_every=10, tol=1.0, select=True, warm=2, order=2):
+    predictor:
+      "rkc"            the step is integrated by RKC2 sub-steps of at most rkc_dt years
+                       (fi_integrate_rkc): the same right-hand side, stable at any step, so its
+                       cost grows like the square root of the stiffness instead of linearly; with a
+                       network (sur) the increment is corrected by it (inputs: u, load, log10
+                       viscosity, lithosphere thickness, RKC increment);
+      "net"            extrapolation of the previous increments in time plus the network's
+                       correction (needs sur);
+      "extrapolation"  constant (order 1) or linearly changing (order 2) increments.
+    The first `warm` steps (default 0 for "rkc", 2 otherwise) are plain: they give the
+    extrapolating predictors their first increments. With select, each check also evaluates the
+    predictor's alternative (RKC without the network, or extrapolation without it) and the next
+    period uses whichever was closer to the plain solver. perturb(period, du) may modify the
+    check step's prediction (tests force a fallback with it)."""
+
+    def __init__(self, sim, sur, macro, check_every=10, tol=1.0, predictor="rkc", rkc_dt=250.0, select=True,
+                 warm=None, order=2, perturb=None):
+        if predictor not in ("rkc", "net", "extrapolation"):
+            raise ValueError(f"unknown predictor {predictor}")
+        if predictor == "net" and sur is None:
+            raise ValueError("the net predictor needs a network")
         self.sim, self.sur, self.macro = sim, sur, macro
-        self.check_every, self.tol, self.warm = int(check_every), float(tol), int(warm)
+        self.check_every, self.tol = int(check_every), float(tol)
+        self.predictor, self.rkc_dt = predictor, float(rkc_dt)
+        self.warm = int(warm) if warm is not None else (0 if predictor == "rkc" else 2)
         self.order = int(order)
         self.select = select and sur is not None
         self.use_net = sur is not None
+        self.perturb = perturb
         self.log = []
+        self.rkc_rhs = 0
+
+    # ---- engine state -------------------------------------------------------------------------
+    def _save(self, eng):
+        if hasattr(eng, "checkpoint"):
+            return ("ck", eng.checkpoint())
+        return ("u", eng.get_field("u"), eng.stats()["t"])
+
+    @staticmethod
+    def _restore(eng, st):
+        if st[0] == "ck":
+            eng.restore(st[1])
+        else:
+            eng.set_field("u", st[1])
+            eng.set_time(st[2])
+
+    # ---- one predicted macro step from the engine's state (advances the engine) ------------------
+    def _step(self, eng, kind, net, inc, t, t1):
+        u0 = eng.get_field("u")
+        if kind == "rkc":
+            info = eng.integrate_rkc(t1, self.rkc_dt)
+            self.rkc_rhs += int(info["n_rhs"]) if isinstance(info, dict) else 0
+            u1 = eng.get_field("u")
+            du = u1 - u0
+            if net:
+                ue = eng.get_field("ue")
+                x = self.macro.features(u0, ue, du, t, self.macro.h)
+                r = self.sur.forward(x).astype(np.float64).reshape(u0.shape, order="F")
+                du = du + D_SCALE * r
+                eng.set_field("u", u0 + du)
+                eng.set_time(t1)
+            return du
+        if net:
+            du = predict(self.sur, self.macro, u0, eng.get_field("ue"), inc[-1], t)
+        else:
+            du = 2.0 * inc[-1] - inc[-2] if self.order == 2 else inc[-1].copy()
+        eng.set_field("u", u0 + du)
+        eng.set_time(t1)
+        return du
 
     def run(self, t_end, t0=None, progress=None):
         eng = self.sim.engine
@@ -310,17 +369,20 @@ class SurrogateRunner:
         t = eng.stats()["t"] if t0 is None else t0
         n_plain = n_sur = n_fallback = n_wasted = 0
         hist = []
+        kind = self.predictor
+        name = {True: "net", False: "extrapolation"} if kind != "rkc" else {True: "rkc-net", False: "rkc"}
 
-        def note(t1, kind, dev):
-            self.log.append((t1, kind, dev))
+        def note(t1, k, dev):
+            self.log.append((t1, k, dev))
             if progress:
-                progress(t1, kind, dev)
+                progress(t1, k, dev)
 
         def plain_to(t1):
             u0 = eng.get_field("u")
             eng.integrate(t1)
             return eng.get_field("u") - u0
 
+        n_period = 0
         while t < t_end - 1e-9:
             t1 = min(t + dt, t_end)
             remaining = int(round((t_end - t) / dt))
@@ -331,61 +393,62 @@ class SurrogateRunner:
                 note(t1, "plain", None)
                 t = t1
                 continue
-            # a surrogate period
-            u_ck, t_ck = eng.get_field("u"), t
-            inc, tp = [hist[-2], hist[-1]], t
-            du_start = hist[-1]
-
-            def extrapolated():
-                # constant increment (order 1) or linearly changing increment (order 2)
-                return 2.0 * inc[-1] - inc[-2] if self.order == 2 else inc[-1].copy()
-
+            n_period += 1
+            start = self._save(eng)
+            inc, tp = list(hist[-2:]), t
+            du_start = hist[-1] if hist else None
             for _ in range(period - 1):
-                u, ue = eng.get_field("u"), eng.get_field("ue")
-                du = predict(self.sur, self.macro, u, ue, inc[-1], tp) if self.use_net else extrapolated()
-                eng.set_field("u", u + du)
-                eng.set_time(tp + dt)
-                inc = [inc[-1], du]
+                du = self._step(eng, kind, self.use_net, inc, tp, tp + dt)
+                inc = [inc[-1], du] if inc else [du]
                 tp += dt
                 note(tp, "surrogate", None)
-            u, ue = eng.get_field("u"), eng.get_field("ue")
-            du_x = extrapolated()
-            du_n = predict(self.sur, self.macro, u, ue, inc[-1], tp) if self.sur is not None else du_x
+            # the check step: the predictor (and its alternative), then the plain solver
+            at_check = self._save(eng)
+            cands = [self.use_net] + ([not self.use_net] if self.select else [])
+            dus = {}
+            for net in cands:
+                self._restore(eng, at_check)
+                dus[net] = self._step(eng, kind, net, inc, tp, tp + dt)
+            if self.perturb is not None:
+                dus[self.use_net] = self.perturb(n_period, dus[self.use_net])
+            self._restore(eng, at_check)
             du_p = plain_to(tp + dt)
-            dev_x, dev_n = float(np.abs(du_p - du_x).max()), float(np.abs(du_p - du_n).max())
-            dev = dev_n if self.use_net else dev_x
-            used = "net" if self.use_net else "extrapolation"
+            devs = {net: float(np.abs(du_p - d).max()) for net, d in dus.items()}
+            dev = devs[self.use_net]
+            used = name[self.use_net]
             if self.select:
-                self.use_net = dev_n <= dev_x
+                self.use_net = devs[True] <= devs[False]
             if dev <= self.tol:
                 n_sur += period - 1
                 n_plain += 1
-                # seed the next period with the plain check increment and the slope of the plain
-                # increments across the period (from its first plain increment to the check, period
-                # steps apart). The slope between the last surrogate step and the check would carry
-                # the check's one-off relaxation of the surrogate's error, amplified period-fold.
-                slope = (du_p - du_start) / period
-                hist.append(du_p - slope)
+                if kind != "rkc" and du_start is not None:
+                    # seed the next period's extrapolation with the plain check increment and the
+                    # slope of the plain increments across the period (first plain increment to the
+                    # check, period steps apart); the slope between the last surrogate step and
+                    # the check would carry the check's one-off relaxation of the surrogate's
+                    # error, amplified period-fold
+                    slope = (du_p - du_start) / period
+                    hist.append(du_p - slope)
                 hist.append(du_p)
                 note(tp + dt, f"check-{used}", dev)
             else:
-                # fallback: roll back to the start of the period and integrate it with the plain solver
+                # fallback: back to the start of the period, integrated by the plain solver
                 n_fallback += 1
                 n_wasted += period - 1
                 note(tp + dt, f"check-{used}-fallback", dev)
-                eng.set_field("u", u_ck)
-                eng.set_time(t_ck)
-                tq = t_ck
+                self._restore(eng, start)
+                tq = t
                 for _ in range(period):
                     hist.append(plain_to(tq + dt))
                     tq += dt
                     n_plain += 1
                     note(tq, "plain", None)
             t = tp + dt
-        return {"plain_steps": n_plain, "surrogate_steps": n_sur, "fallbacks": n_fallback,
-                "discarded_surrogate_steps": n_wasted,
-                "network_periods": sum(1 for _, kind, _ in self.log if kind.startswith("check-net")),
-                "extrapolation_periods": sum(1 for _, kind, _ in self.log if kind.startswith("check-extrapolation"))}
+        return {"predictor": self.predictor, "plain_steps": n_plain, "surrogate_steps": n_sur,
+                "fallbacks": n_fallback, "discarded_surrogate_steps": n_wasted, "rkc_rhs_calls": self.rkc_rhs,
+                "network_periods": sum(1 for _, k, _ in self.log if k.startswith("check-net")
+                                       or k.startswith("check-rkc-net")),
+                "checks": sum(1 for _, k, _ in self.log if k.startswith("check-"))}
 
 
 def generate_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=3.15e6, t_range=(5e3, 115e3),

@@ -292,6 +292,27 @@ int fi_get_stats(fi_ctx* ctx, fi_stats* out);
  * implicit step, whose every call advances u. */
 int fi_set_time(fi_ctx* ctx, double t);
 
+/* Second-order Runge-Kutta-Chebyshev integration (RKC2: Sommeijer, Shampine & Verwer 1997,
+ * Appl. Numer. Math. 22; damping eps = 2/13) of the same right-hand side (fi_rhs semantics,
+ * sparse trigger included) from the current time to t_target, in equal sub-steps of at most
+ * dt_max years. Each sub-step takes the fewest stages s whose real stability interval
+ * beta(s) = (w0 + 1) T_s''(w0) / T_s'(w0) covers 1.2 rho h, rho being the spectral radius of the
+ * right-hand side's Jacobian (1/yr): the caller's value if rho > 0, else an estimate by 40
+ * power iterations from a fixed pseudo-random field, made once per context. Stage vectors are
+ * not corner-corrected (the right-hand side corrects its input); the state after each sub-step
+ * is. No counterpart in the reference, which integrates with OrdinaryDiffEq's BS3: it is the
+ * macro-step predictor of the surrogate-with-fallback mode, whose result the plain solver checks.
+ * The next fi_integrate re-evaluates the first-same-as-last derivative. LV-ELVA and ELRA
+ * contexts; FI_ERR_UNSUPPORTED under the implicit step. */
+typedef struct fi_rkc_info {
+    double rho;          /* spectral radius used (1/yr) */
+    double h;            /* sub-step (yr) */
+    int32_t substeps;
+    int32_t stages;      /* stages per sub-step */
+    int64_t n_rhs;       /* right-hand-side evaluations of the call (power iterations included) */
+} fi_rkc_info;
+int fi_integrate_rkc(fi_ctx* ctx, double t_target, double dt_max, double rho, fi_rkc_info* info);
+
 /* Checkpoints of a context's dynamic state: the displacement, the first-same-as-last derivative,
  * the step-size controller, the sparse-update state (elastic displacement, sea surface, masks,
  * stored loads, BSL volumes, counters) and the output counter. fi_checkpoint_size gives the byte

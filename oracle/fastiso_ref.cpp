@@ -357,6 +357,7 @@ struct fi_ctx {
     double V_af = 0, V_pov = 0, V_den = 0, delta_V = 0, z_bsl = 0, A_bsl = 0;
     int64_t count = 0;
     bool k1_stale = false;
+    double rkc_rho = 0;   // spectral radius estimate of fi_integrate_rkc (1/yr)
     double t0 = 0, t_end = 0, t_timer = 0;
     bool initialized = false;
     // preallocated buffers (PreAllocated, src/tools.jl:4-16)
@@ -1183,6 +1184,156 @@ int fi_set_time(fi_ctx* c, double t) {
     c->t = t;
     c->u = c->uprev;
     c->k1_stale = true;
+    return FI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// RKC2 (Sommeijer, Shampine & Verwer 1997, Appl. Numer. Math. 22, 315-326): damped Chebyshev
+// stability polynomial R_s(z) = a_s + b_s T_s(w0 + w1 z), w0 = 1 + eps/s^2, w1 = T_s'(w0)/T_s''(w0),
+// b_j = T_j''(w0)/T_j'(w0)^2 (b_0 = b_1 = b_2), a_j = 1 - b_j T_j(w0), and the three-term stage
+// recursion
+//   Y_1 = Y_0 + mut_1 h F_0,   mut_1 = b_1 w1,
+//   Y_j = (1 - mu_j - nu_j) Y_0 + mu_j Y_{j-1} + nu_j Y_{j-2} + mut_j h F_{j-1} + gt_j h F_0,
+//   mu_j = 2 b_j w0 / b_{j-1}, nu_j = -b_j / b_{j-2}, mut_j = 2 b_j w1 / b_{j-1},
+//   gt_j = -(1 - b_{j-1} T_{j-1}(w0)) mut_j,
+// at stage times c_1 = mut_1, c_j = mu_j c_{j-1} + nu_j c_{j-2} + mut_j + gt_j.
+struct RkcCoef {
+    std::vector<double> mu, nu, mut, gt, c;
+    double beta = 0;
+};
+
+static RkcCoef rkc_coef(int s) {
+    const double eps = 2.0 / 13.0, w0 = 1.0 + eps / ((double)s * s);
+    std::vector<double> T(s + 1), dT(s + 1), ddT(s + 1);
+    T[0] = 1.0; T[1] = w0;
+    dT[0] = 0.0; dT[1] = 1.0;
+    ddT[0] = 0.0; ddT[1] = 0.0;
+    for (int j = 2; j <= s; ++j) {
+        T[j] = 2.0 * w0 * T[j - 1] - T[j - 2];
+        dT[j] = 2.0 * T[j - 1] + 2.0 * w0 * dT[j - 1] - dT[j - 2];
+        ddT[j] = 4.0 * dT[j - 1] + 2.0 * w0 * ddT[j - 1] - ddT[j - 2];
+    }
+    const double w1 = dT[s] / ddT[s];
+    std::vector<double> b(s + 1);
+    for (int j = 2; j <= s; ++j) b[j] = ddT[j] / (dT[j] * dT[j]);
+    b[0] = b[1] = b[2];
+    RkcCoef k;
+    k.mu.assign(s + 1, 0.0); k.nu.assign(s + 1, 0.0); k.mut.assign(s + 1, 0.0); k.gt.assign(s + 1, 0.0);
+    k.c.assign(s + 1, 0.0);
+    k.mut[1] = b[1] * w1;
+    k.c[1] = k.mut[1];
+    for (int j = 2; j <= s; ++j) {
+        k.mu[j] = 2.0 * b[j] * w0 / b[j - 1];
+        k.nu[j] = -b[j] / b[j - 2];
+        k.mut[j] = 2.0 * b[j] * w1 / b[j - 1];
+        k.gt[j] = -(1.0 - b[j - 1] * T[j - 1]) * k.mut[j];
+        k.c[j] = k.mu[j] * k.c[j - 1] + k.nu[j] * k.c[j - 2] + k.mut[j] + k.gt[j];
+    }
+    k.beta = (w0 + 1.0) * ddT[s] / dT[s];
+    return k;
+}
+
+// fewest stages (>= 2) whose stability interval covers x = 1.2 rho h
+static int rkc_stages(double x) {
+    int s = 2;
+    while (rkc_coef(s).beta < x && s < 100000) s = std::max(s + 1, (int)(s * 1.05));
+    while (s > 2 && rkc_coef(s - 1).beta >= x) --s;
+    return s;
+}
+
+// the pseudo-random start vector of the power iteration: uniform in [-1, 1) per cell (splitmix64)
+static double rkc_start_value(uint64_t p) {
+    uint64_t z = 0x9E3779B97F4A7C15ull * (p + 1) + 0x5EEDull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_rkc_info* info) {
+    if (!c || !(dt_max > 0)) return FI_ERR_INVALID;
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_integrate_rkc before fi_integrator_init");
+    if (implicit_step(c->cfg)) return fail(c, FI_ERR_UNSUPPORTED, "fi_integrate_rkc: not with the implicit step");
+    if (!(t_target >= c->t)) return fail(c, FI_ERR_INVALID, "fi_integrate_rkc: target before current time");
+    const size_t N = c->N;
+    const int64_t r0 = c->nrhs;
+    int st;
+    vec F0(N), F1(N), w(N);
+    auto rms = [&](const vec& a) {
+        double q = 0;
+        for (double x : a) q += x * x;
+        return std::sqrt(q / (double)N);
+    };
+    if (!(rho > 0)) {
+        if (!(c->rkc_rho > 0)) {
+            // 40 power iterations on J v = f(u + v) - f(u) (the right-hand side is affine in u)
+            vec v(N);
+            for (size_t p = 0; p < N; ++p) v[p] = rkc_start_value(p);
+            const double n0 = rms(v);
+            for (double& x : v) x /= n0;
+            w = c->uprev;
+            if ((st = update_diagnostics(c, F0.data(), w.data(), c->t))) return st;
+            double lam = 0;
+            for (int it = 0; it < 40; ++it) {
+                for (size_t p = 0; p < N; ++p) w[p] = c->uprev[p] + v[p];
+                if ((st = update_diagnostics(c, F1.data(), w.data(), c->t))) return st;
+                for (size_t p = 0; p < N; ++p) v[p] = F1[p] - F0[p];
+                lam = rms(v);
+                if (!(lam > 0) || !std::isfinite(lam)) return fail(c, FI_ERR_NONFINITE, "fi_integrate_rkc: power iteration");
+                for (double& x : v) x /= lam;
+            }
+            c->rkc_rho = lam;
+        }
+        rho = c->rkc_rho;
+    }
+    const double t_start = c->t, span = t_target - t_start;
+    int nsub = span > 0 ? (int)std::ceil(span / dt_max - 1e-12) : 0;
+    const double h = nsub > 0 ? span / nsub : 0.0;
+    const int s = nsub > 0 ? rkc_stages(1.2 * rho * h) : 0;
+    if (nsub > 0) {
+        const RkcCoef k = rkc_coef(s);
+        vec Y0 = c->uprev, Ym1(N), Ym2(N), Y(N);
+        for (int sub = 0; sub < nsub; ++sub) {
+            const double t0 = t_start + sub * h;
+            w = Y0;
+            if ((st = update_diagnostics(c, F0.data(), w.data(), t0))) return st;
+            for (size_t p = 0; p < N; ++p) Ym1[p] = Y0[p] + k.mut[1] * h * F0[p];
+            Ym2 = Y0;
+            for (int j = 2; j <= s; ++j) {
+                w = Ym1;
+                if ((st = update_diagnostics(c, F1.data(), w.data(), t0 + k.c[j - 1] * h))) return st;
+                const double a0 = 1.0 - k.mu[j] - k.nu[j];
+                for (size_t p = 0; p < N; ++p)
+                    Y[p] = a0 * Y0[p] + k.mu[j] * Ym1[p] + k.nu[j] * Ym2[p] + k.mut[j] * h * F1[p] +
+                           k.gt[j] * h * F0[p];
+                Ym2.swap(Ym1);
+                Ym1.swap(Y);
+            }
+            // corner BC on the new state (src/boundary_conditions.jl:144-148)
+            const int nx = c->nx, ny = c->ny;
+            const double off = 0.25 * (Ym1[0] + Ym1[nx - 1] + Ym1[(size_t)(ny - 1) * nx] + Ym1[(size_t)(ny - 1) * nx + nx - 1])
+                               - c->cfg.viscous_x_border;
+            for (size_t p = 0; p < N; ++p) Y0[p] = Ym1[p] - off;
+        }
+        c->uprev = Y0;
+    }
+    c->t = t_target;
+    c->u = c->uprev;
+    c->k1_stale = true;
+    if (info) {
+        info->rho = rho;
+        info->h = h;
+        info->substeps = nsub;
+        info->stages = s;
+        info->n_rhs = c->nrhs - r0;
+    }
     return FI_OK;
 }
 

@@ -34,6 +34,11 @@ class RefStats(C.Structure):
     ]
 
 
+class RefRkcInfo(C.Structure):
+    _fields_ = [("rho", C.c_double), ("h", C.c_double), ("substeps", C.c_int32), ("stages", C.c_int32),
+                ("n_rhs", C.c_int64)]
+
+
 CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
 _dp = C.POINTER(C.c_double)
 _lib = None
@@ -67,6 +72,7 @@ def lib():
             "fi_integrator_init": (C.c_int, [P]),
             "fi_integrate": (C.c_int, [P, C.c_double, CB, C.c_void_p]),
             "fi_get_stats": (C.c_int, [P, C.POINTER(RefStats)]),
+            "fi_integrate_rkc": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.POINTER(RefRkcInfo)]),
             "fi_set_time": (C.c_int, [P, C.c_double]),
             "fi_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
             "fi_checkpoint_save": (C.c_int, [P, C.c_void_p, C.c_size_t]),
@@ -205,6 +211,11 @@ class RefEngine:
 
     def set_time(self, t):
         self._ck(self.L.fi_set_time(self.h, float(t)))
+
+    def integrate_rkc(self, t_target, dt_max, rho=0.0):
+        info = RefRkcInfo()
+        self._ck(self.L.fi_integrate_rkc(self.h, float(t_target), float(dt_max), float(rho), C.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
 
     def checkpoint(self):
         n = C.c_size_t()

@@ -52,7 +52,8 @@ class Sim:
 
 def run(G, t_end, dt=100.0, tol=1.0, check_every=10, order=2):
     eng = ScriptedEngine(G)
-    r = SurrogateRunner(Sim(eng), None, Macro(dt), check_every=check_every, tol=tol, order=order)
+    r = SurrogateRunner(Sim(eng), None, Macro(dt), check_every=check_every, tol=tol, order=order,
+                        predictor="extrapolation")
     res = r.run(t_end, t0=0.0)
     return eng, r, res
 
@@ -98,3 +99,49 @@ def test_kink_above_tolerance_falls_back_to_the_plain_solution():
     kinds = [kind for _, kind, _ in r.log]
     assert "check-extrapolation-fallback" in kinds
     np.testing.assert_allclose(eng.u, G(2200.0), rtol=0, atol=1e-9)
+
+
+class CheckpointEngine(ScriptedEngine):
+    """The scripted engine with checkpoints and an RKC step that is exact up to a fixed bias."""
+
+    def __init__(self, G, bias=0.0, shape=(6, 5)):
+        super().__init__(G, shape)
+        self.bias = bias
+        self.n_restore = 0
+
+    def checkpoint(self):
+        return (self.u.copy(), self.t, self.n_integrate)
+
+    def restore(self, blob):
+        self.u, self.t = blob[0].copy(), blob[1]
+        self.n_restore += 1
+
+    def integrate_rkc(self, t1, dt_max):
+        self.u = self.u + (self.G(t1) - self.G(self.t)) + self.bias
+        self.t = float(t1)
+        return {"n_rhs": 10}
+
+
+def test_rkc_predictor_periods_and_exact_fallback():
+    """RKC predictor: 9 predicted steps and one checked step per period; a check perturbed beyond
+    the tolerance rolls the period back to its start and integrates it with the plain solver."""
+    def G(t):
+        return np.full((6, 5), 0.01 * t + 1e-6 * t * t)
+
+    eng = CheckpointEngine(G, bias=1e-3)
+    bumped = []
+
+    def perturb(k, du):
+        if k == 2:
+            bumped.append(k)
+            return du + 2.0
+        return du
+
+    r = SurrogateRunner(Sim(eng), None, Macro(100.0), check_every=10, tol=1.0, predictor="rkc", perturb=perturb)
+    res = r.run(3000.0, t0=0.0)
+    assert bumped == [2] and res["fallbacks"] == 1 and res["checks"] == 3
+    assert res["surrogate_steps"] == 18 and res["plain_steps"] == 2 + 10
+    # periods 1 and 3 carry 9 steps of RKC bias each; period 2 was integrated by the plain solver
+    np.testing.assert_allclose(eng.u, G(3000.0) + 18e-3, rtol=0, atol=1e-9)
+    kinds = [k for _, k, _ in r.log]
+    assert kinds.count("check-rkc") == 2 and kinds.count("check-rkc-fallback") == 1
