@@ -716,6 +716,10 @@ int host_ice(fi_ctx* c, double t, std::vector<double>& out) {
 double hab_h(double H, double zb, double zss, double rsi) { return std::max(H + std::min(zb - zss, 0.0) * rsi, 0.0); }
 
 int launch(fi_ctx* c, int mode, int ubuf, int kslot, int max_steps, int write_sigma_always) {
+    // the integration loop writes u_{n+1} into U[1 - cur]: cur must be 0 or 1
+    if (mode == MODE_INTEGRATE && c->ctl.cur != 0 && c->ctl.cur != 1)
+        return fail(c, FI_ERR_STATE, "state buffer index out of range");
+    if (ubuf < 0 || ubuf > 2 || kslot < 0 || kslot > 3) return fail(c, FI_ERR_STATE, "buffer index out of range");
     KP kp = c->kp;
     kp.mode = mode;
     kp.rhs_ubuf = ubuf;
@@ -1898,6 +1902,12 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
             HIPCHECK(c, hipGetLastError());
             y0 = pm1;
         }
+        // the BS3 loop ping-pongs u_n between U[0] and U[1] (U[2] is scratch): a state left in
+        // U[2] moves to U[0]
+        if (y0 == 2) {
+            HIPCHECK(c, hipMemcpyAsync(U, U + 2 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            y0 = 0;
+        }
         HIPCHECK(c, hipStreamSynchronize(c->stream));
         c->ctl.cur = y0;
         c->ctl.last_u = y0;
@@ -2008,7 +2018,11 @@ int fi_checkpoint_load(fi_ctx* c, const void* buf, size_t bytes) {
     (void)hipSetDevice(c->dev);
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     size_t off = 0;
-    return ck_walk(c, 2, (unsigned char*)const_cast<void*>(buf), off);
+    if ((st = ck_walk(c, 2, (unsigned char*)const_cast<void*>(buf), off))) return st;
+    const Ctl& g = c->ctl;
+    if ((g.cur != 0 && g.cur != 1) || g.k1 < 0 || g.k1 > 3 || g.last_k < 0 || g.last_k > 3 || g.last_u < 0 || g.last_u > 2)
+        return fail(c, FI_ERR_INVALID, "fi_checkpoint_load: buffer indices out of range");
+    return FI_OK;
 }
 
 int fi_get_stats(fi_ctx* c, fi_stats* o) {
@@ -2139,6 +2153,9 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, bool mt_double, size_t lds
     double cap_work = 0;  // cells x attempts if every member still short of its target used the cap
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
+        // the integration loop writes u_{n+1} into U[1 - cur]: cur must be 0 or 1
+        if (c->ctl.cur != 0 && c->ctl.cur != 1)
+            return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": state buffer index out of range");
         if (c->ctl.t < t_target[m]) cap_work += (double)max_steps * (double)c->N;
         c->ctl.stop = t_target[m];
         c->ctl.next_out = t_target[m];

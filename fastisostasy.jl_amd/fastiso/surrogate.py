@@ -159,6 +159,7 @@ class Surrogate:
 U_SCALE = 500.0          # m, displacement
 S_SCALE = 1.0e6          # kg m^-2, load anomaly (about 1100 m of ice)
 D_SCALE = 2.0            # m per macro step, correction of the extrapolated increment
+R_SCALE = 0.02           # m per macro step, correction of the RKC increment
 LOGETA_MID, LOGETA_HALF = 21.0, 2.0
 H_MID, H_HALF = 150e3, 100e3
 
@@ -324,6 +325,12 @@ class SurrogateRunner:
         self.perturb = perturb
         self.log = []
         self.rkc_rhs = 0
+        # output scale of a network correcting the RKC increment (its saved scales: U, S, scale,
+        # macro_dt, base 1 = RKC); a network for the extrapolated increment is not one
+        sc = tuple(getattr(sur, "scales", None) or ()) if sur is not None else ()
+        self.net_scale = float(sc[2]) if len(sc) > 2 else R_SCALE
+        if predictor == "rkc" and sur is not None and not (len(sc) > 4 and sc[4] == 1.0):
+            raise ValueError("the network was not trained on RKC increments (its scales lack base 1)")
 
     # ---- engine state -------------------------------------------------------------------------
     def _save(self, eng):
@@ -351,7 +358,7 @@ class SurrogateRunner:
                 ue = eng.get_field("ue")
                 x = self.macro.features(u0, ue, du, t, self.macro.h)
                 r = self.sur.forward(x).astype(np.float64).reshape(u0.shape, order="F")
-                du = du + D_SCALE * r
+                du = du + self.net_scale * r
                 eng.set_field("u", u0 + du)
                 eng.set_time(t1)
             return du
@@ -449,6 +456,59 @@ class SurrogateRunner:
                 "network_periods": sum(1 for _, k, _ in self.log if k.startswith("check-net")
                                        or k.startswith("check-rkc-net")),
                 "checks": sum(1 for _, k, _ in self.log if k.startswith("check-"))}
+
+
+def generate_rkc_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=3.15e6, t_range=(5e3, 115e3),
+                       rkc_dt=250.0, team_size=0, spin=4, progress=None):
+    """Training pairs for a network that corrects the RKC macro step: member m gets C1-style
+    fields and domes (seeds seed0 + m, seed0 + 100000 + m, seed0 + 200000 + m) and a random time
+    t_m of the glacial cycle; its state there is spun up by spin RKC macro steps from a random
+    fraction of the equilibrium. Then, macro step by macro step, every member takes an RKC step
+    (fi_integrate_rkc from a checkpoint, which is restored) and the ensemble engine integrates
+    all members with the plain solver; each gives a pair (input with the RKC increment, scaled
+    residual (du_plain - du_rkc) / R_SCALE). Returns (X, Y, meta): X (pairs, cells, 5),
+    Y (pairs, cells), meta (member, t, max |du_plain|, max |du_plain - du_rkc|)."""
+    from . import workloads
+    from .ensemble import Ensemble
+    from .simulation import Simulation
+    rng = np.random.default_rng(seed0)
+    macros, inputs, starts = [], [], []
+    for m in range(n_members):
+        w = workloads.lv_elva(n=n, W=W, t_end=120e3, seeds=(seed0 + m, seed0 + 100000 + m, seed0 + 200000 + m),
+                              t_out=[])
+        macros.append(lv_macro(w, macro_dt))
+        inputs.append(Simulation(w["domain"], w["bcs"], w["sealevel"], w["solidearth"], w["t_span"],
+                                 nout=w["nout"], opts=w["opts"], c=w["c"], engine=None).inputs)
+        starts.append((float(rng.uniform(*t_range)), float(rng.uniform(0.5, 1.0))))
+    ens = Ensemble(inputs, team_size=team_size)
+    ens.integrator_init()
+    for m, (ts, f) in enumerate(starts):
+        t = ts - spin * macro_dt
+        ens[m].set_field("u", f * macros[m].equilibrium(t))
+        ens[m].set_time(t)
+        ens[m].integrate_rkc(ts, rkc_dt)
+    X, Y, meta = [], [], []
+    for s in range(steps_per_member):
+        base = []
+        for m in range(n_members):
+            e = ens[m]
+            t_m = starts[m][0] + s * macro_dt
+            ck = e.checkpoint()
+            u0, ue0 = e.get_field("u"), e.get_field("ue")
+            e.integrate_rkc(t_m + macro_dt, rkc_dt)
+            base.append((u0, ue0, e.get_field("u") - u0, t_m))
+            e.restore(ck)
+        ens.integrate_each([starts[m][0] + (s + 1) * macro_dt for m in range(n_members)], fsal=False)
+        for m in range(n_members):
+            u0, ue0, du_r, t_m = base[m]
+            du = ens[m].get_field("u") - u0
+            X.append(macros[m].features(u0, ue0, du_r, t_m, macros[m].h))
+            Y.append(cells((du - du_r) / R_SCALE).astype(np.float32))
+            meta.append((m, t_m, float(np.abs(du).max()), float(np.abs(du - du_r).max())))
+        if progress:
+            progress(s, ens.info())
+    ens.close()
+    return np.array(X), np.array(Y), meta
 
 
 def generate_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=3.15e6, t_range=(5e3, 115e3),

@@ -293,3 +293,32 @@ def test_ensemble_members_held_at_their_time_across_tuner_launches():
     if not DRYRUN:
         info = ens.info()
         assert info["n_launch"] > len(plan), info
+
+
+@pytest.mark.gpu
+def test_gpu_full_ensemble_one_window_sampled_members_bitwise():
+    """The 256-member uneven ensemble of the ensemble benchmark (sides 64..256), one 10-yr window
+    in one launch sequence with the tuner on; one sampled member of every side equals its single
+    context bit for bit."""
+    import bench
+    if DRYRUN:
+        pytest.skip("256 oracle members take minutes on the CPU")
+    inputs = bench.ensemble_inputs(256)
+    ens = hip_ensemble(inputs, 0)
+    ens.integrator_init()
+    ens.integrate(10.0)
+    sides = {}
+    for m, inp in enumerate(inputs):
+        sides.setdefault(inp["cfg"].nx, m)
+    assert sorted(sides) == [64, 96, 128, 192, 256], sorted(sides)
+    for n, m in sorted(sides.items()):
+        e = hip_single(with_tout(inputs[m], [10.0]))
+        e.integrator_init()
+        e.integrate(10.0, lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m} (side {n})")
+        s0, s1 = ens[m].stats(), e.stats()
+        assert (s0["n_accept"], s0["n_reject"], s0["n_rhs"]) == (s1["n_accept"], s1["n_reject"], s1["n_rhs"])
+        e.close()
+    info = ens.info()
+    assert info["n_members"] == 256 and info["n_launch"] >= 1
+    ens.close()

@@ -172,13 +172,18 @@ def test_disc_trajectory_vs_oracle(disc_runs):
 
 def test_disc_trajectory_vs_analytic(disc_runs):
     """64x64 disc benchmark (docs/src/examples/benchmark_analytic.jl) against
-    analytic_solution (src/analytic_solutions.jl:11-65) on the centre transect."""
+    analytic_solution (src/analytic_solutions.jl:11-65) on the centre transect. The FP64 oracle's
+    own transect error is the discretisation error of the 93.75-km grid (largest at the disc edge:
+    11.4 m of 287 m at 50 kyr, 7.4 m of 88 m at 1 kyr); the device run may exceed it by at most
+    1e-3 of the analytic maximum, and the centre is within 0.5 m from 10 kyr on."""
     w, sim, ref, og, orf = disc_runs
     d, c, se = w["domain"], w["c"], w["solidearth"]
     j = d.my - 1
-    for t, u in og:
+    for (t, u), (_, ur) in zip(og, orf):
         ua = analytic_solution(np.abs(d.x), t * c.seconds_per_year, c, se, 1e3, 1e6)
-        assert np.abs(u[:, j] - ua).max() < 0.05 * np.abs(ua).max() + 5.0, t
+        e_ref = np.abs(ur[:, j] - ua).max()
+        assert e_ref < 11.5 and e_ref < 0.085 * np.abs(ua).max(), (t, e_ref)
+        assert np.abs(u[:, j] - ua).max() <= e_ref + 1e-3 * np.abs(ua).max(), t
         if t >= 10e3:
             assert abs(u[d.mx - 1, j] - ua[d.mx - 1]) < 0.5, t
 
