@@ -643,6 +643,9 @@ struct fi_ctx {
     DBuf<double> uxy_d;   // horizontal displacement output (FI_U_X / FI_U_Y)
     DBuf<double> rkc_off;   // corner offset scratch of fi_integrate_rkc
     double rkc_rho = 0;     // spectral radius estimate of fi_integrate_rkc (1/yr)
+    DBuf<KP> rkc_kp;        // single-RHS parameter blocks of fi_integrate_rkc: [U buffer][K slot 0/1]
+    double* rkc_times = nullptr;   // pinned stage times of one fi_integrate_rkc call
+    size_t rkc_times_n = 0;
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
@@ -932,6 +935,7 @@ void fi_destroy(fi_ctx* c) {
     hipStream_t s = c->stream;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->rkc_times) (void)hipHostFree(c->rkc_times);
     delete c;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -1866,6 +1870,50 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
     const int s = nsub > 0 ? rkc_stage_count(1.2 * rho * h) : 0;
     if (nsub > 0) {
         if (!c->rkc_off.p) HIPCHECK(c, c->rkc_off.alloc(1));
+        // Every right-hand side of the call is queued on the stream without a host round trip:
+        // the control block stays on the device between the launches (its sparse-update count
+        // evolves there), each launch's stage time is copied into it from pinned memory, and its
+        // U buffer and K slot come from one of six prepared parameter blocks.
+        if (!c->rkc_kp.p) {
+            HIPCHECK(c, c->rkc_kp.alloc(6));
+            KP kps[6];
+            for (int q = 0; q < 6; ++q) {
+                kps[q] = c->kp;
+                kps[q].mode = MODE_RHS;
+                kps[q].rhs_ubuf = q / 2;
+                kps[q].rhs_kslot = q % 2;
+                kps[q].max_steps = 0;
+                kps[q].write_sigma_always = 1;
+            }
+            HIPCHECK(c, hipMemcpy(c->rkc_kp.p, kps, sizeof(kps), hipMemcpyHostToDevice));
+        }
+        // pinned: the stage times, then each launch's barrier abort word (copied back after the
+        // launch, before the next one clears the barrier)
+        const size_t ntimes = (size_t)nsub * (size_t)s;
+        if (c->rkc_times_n < ntimes) {
+            if (c->rkc_times) (void)hipHostFree(c->rkc_times);
+            c->rkc_times = nullptr;
+            c->rkc_times_n = 0;
+            HIPCHECK(c, hipHostMalloc((void**)&c->rkc_times, 2 * ntimes * sizeof(double)));
+            c->rkc_times_n = ntimes;
+        }
+        int* aborts = (int*)(c->rkc_times + c->rkc_times_n);
+        HIPCHECK(c, hipMemcpy(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice));
+        const void* fk = persistent_kernel(c->mt_double, c->has_dist, c->elra, c->implicit);
+        size_t nt = 0;
+        auto rhs_async = [&](double t, int ubuf, int slot) -> int {
+            c->rkc_times[nt] = t;
+            HIPCHECK(c, hipMemcpyAsync((char*)c->ctl_d.p + offsetof(Ctl, t_rhs), &c->rkc_times[nt], sizeof(double),
+                                       hipMemcpyHostToDevice, c->stream));
+            ++nt;
+            HIPCHECK(c, hipMemsetAsync(c->bar_d.p, 0, sizeof(Bar) * (1 + 2 * (size_t)c->G), c->stream));
+            const KP* kptr = c->rkc_kp.p + 2 * ubuf + slot;
+            void* args[] = {&kptr};
+            HIPCHECK(c, hipLaunchCooperativeKernel(fk, dim3(c->G), dim3(NT), args, (unsigned)c->lds, c->stream));
+            HIPCHECK(c, hipMemcpyAsync(&aborts[nt - 1], &c->bar_d.p->abort, sizeof(int), hipMemcpyDeviceToHost,
+                                       c->stream));
+            return FI_OK;
+        };
         const Rkc k = rkc_make(s);
         const int offs = c->elra ? 0 : 1;
         const double vbc = c->cfg.viscous_x_border;
@@ -1879,14 +1927,14 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
         for (int sub = 0; sub < nsub; ++sub) {
             const double t0 = t_start + sub * h;
             const int a = (y0 + 1) % 3, b = (y0 + 2) % 3;
-            if ((st = rhs_on_device(c, t0, y0, 0))) return st;   // F_0
+            if ((st = rhs_async(t0, y0, 0))) return st;   // F_0
             hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + a * N, U + y0 * N, U + y0 * N,
                                (const double*)nullptr, K1, K0, kc1, kc0, offs, vbc, 1.0, 0.0, 0.0, 0.0,
                                k.mut[1] * h, (long)N);
             HIPCHECK(c, hipGetLastError());
             int pm1 = a, pm2 = y0;
             for (int j = 2; j <= s; ++j) {
-                if ((st = rhs_on_device(c, t0 + k.c[j - 1] * h, pm1, 1))) return st;   // F_{j-1}
+                if ((st = rhs_async(t0 + k.c[j - 1] * h, pm1, 1))) return st;   // F_{j-1}
                 const int out = (j == 2) ? b : pm2;                                   // never Y_0
                 hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + out * N, U + y0 * N,
                                    U + pm1 * N, U + pm2 * N, K1, K0, kc1, kc0, offs, vbc,
@@ -1908,9 +1956,15 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
             HIPCHECK(c, hipMemcpyAsync(U, U + 2 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             y0 = 0;
         }
+        HIPCHECK(c, hipMemcpyAsync(&c->ctl, c->ctl_d.p, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
         c->ctl.cur = y0;
         c->ctl.last_u = y0;
+        for (size_t q = 0; q < nt; ++q)
+            if (aborts[q]) return fail(c, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
+        if (c->ctl.err == ERR_TIME_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+        if (c->ctl.err == ERR_BSL_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation0D out of range.");
+        if (c->ctl.err) return fail(c, FI_ERR_DEVICE, "device error");
     }
     c->ctl.t = t_target;
     c->k1_stale = true;
