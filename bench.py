@@ -429,6 +429,63 @@ def bench_fno_train(a):
     }
 
 
+def bench_hbm_copy(a):
+    """A plain streaming kernel of known size: out = 1.0 * x on 2**28 FP32 values (1 GiB read,
+    1 GiB written per launch), --steps launches after --warmup. It calibrates the HBM traffic
+    counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch against the known bytes) and
+    shows the copy bandwidth the box attains."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(a.warmup):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.steps):
+        torch.mul(x, 1.0, out=y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    moved = 2.0 * n * 4
+    return {"metric": "HBM copy bandwidth (counter calibration kernel)", "value": moved / ms / 1e6, "unit": "GB/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "hbm_copy", "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4}}
+
+
+def profile_suite(a):
+    """The workloads whose kernels profiles/ covers, in one process, so that one profiler pass
+    (rocprofv3 --kernel-trace, or one --pmc counter set) collects all of them: the HBM copy
+    calibration kernel, the N=350 plain solver (3 windows after 1), the 256-member ensemble (2
+    windows after 1), the N=1024 instance (1 window after 1) and the RKC predictor (one 500-yr
+    macro step at N=350). One JSON line per part."""
+    import copy
+    parts = []
+    b = copy.copy(a)
+    b.steps, b.warmup = 5, 2
+    parts.append(bench_hbm_copy(b))
+    for wl, steps, warm in (("lv_elva", 3, 1), ("lv_elva_hr", 1, 1)):
+        b = copy.copy(a)
+        b.workload, b.steps, b.warmup, b.no_cpu_baseline, b.no_surrogate, b.n = wl, steps, warm, True, True, 350
+        b.window = 5.0 if wl == "lv_elva_hr" else 10.0
+        parts.append(run_plain(b))
+    b = copy.copy(a)
+    b.workload, b.steps, b.warmup, b.window = "ensemble", 2, 1, 10.0
+    parts.append(bench_ensemble(b))
+    w, sim = make_sim(350, engine="hip")
+    e = sim.engine
+    e.integrator_init()
+    e.integrate(100.0)
+    t0 = time.perf_counter()
+    info = e.integrate_rkc(600.0, 250.0)
+    el = time.perf_counter() - t0
+    sim.close()
+    parts.append({"workload": "rkc_macro_step_n350", "wall_s": el, **info})
+    return parts
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,7 +496,7 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "elra_ensemble", "inversion",
-                                               "surrogate", "fno_train"],
+                                               "surrogate", "fno_train", "hbm_copy", "profile_suite"],
                     default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=10.0, help="surrogate window (kyr)")
@@ -480,6 +537,20 @@ def main():
     if a.workload == "fno_train":
         print(json.dumps(bench_fno_train(a)), flush=True)
         return
+    if a.workload == "hbm_copy":
+        print(json.dumps(bench_hbm_copy(a)), flush=True)
+        return
+    if a.workload == "profile_suite":
+        for part in profile_suite(a):
+            print(json.dumps(part), flush=True)
+        return
+    print(json.dumps(run_plain(a)), flush=True)
+
+
+def run_plain(a):
+    """The single-context workloads (lv_elva, lv_elva_hr, elra, implicit): returns the bench line."""
+    elra = a.workload == "elra"
+    implicit = a.workload == "implicit"
     hr = a.workload == "lv_elva_hr"
     if hr and a.n == 350:
         a.n = 1024
@@ -559,7 +630,7 @@ def main():
                                          weights=a.weights if (a.predictor != "rkc" or a.with_net) else None)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.n, a.window, a.cpu_sample_s, hr, elra, implicit, start=start)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
