@@ -66,6 +66,21 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM-side traffic of the persistent kernel at N=350 (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, calibrated on the copy probe); scaled to this run's accepted steps per launch
 TRAFFIC_N350 = os.path.join(ROOT, "profiles", "fi_persistent_lv_elva_n350_traffic.json")
+TRAFFIC_N1024 = os.path.join(ROOT, "profiles", "fi_persistent_lv_elva_hr_n1024_traffic.json")
+TRAFFIC_ENS = os.path.join(ROOT, "profiles", "fi_batch_persistent_ensemble256_traffic.json")
+
+
+def traffic_fields(path, per_unit_key, units, launches):
+    """roofline.traffic (HBM bytes per launch) from a committed PMC summary: its bytes per unit
+    (accepted step or cell-step, rocprofv3 FETCH_SIZE / WRITE_SIZE calibrated on a copy kernel)
+    times this run's units per launch."""
+    if not os.path.exists(path) or launches <= 0:
+        return {"traffic": None}
+    with open(path) as f:
+        per = json.load(f)[per_unit_key]
+    return {"traffic": per * units / launches, "traffic_unit": "B per launch",
+            "traffic_source": os.path.relpath(path, ROOT) + f" ({per_unit_key} from rocprofv3 --pmc FETCH_SIZE, "
+                              "WRITE_SIZE, scaled to this run's units per launch)"}
 
 
 def make_sim(n, engine, hr=False, elra=False, implicit=False):
@@ -153,7 +168,9 @@ def bench_ensemble(a):
                    "team_size": i1["team_size"], "n_teams": i1["n_teams"], "autotuned": a.team_size == 0,
                    "kernel_launches": int(i1["n_launch"] - i0["n_launch"]), "setup_s": setup_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     **(traffic_fields(TRAFFIC_ENS, "bytes_per_cell_step", cell_steps,
+                                       int(i1["n_launch"] - i0["n_launch"])) if not elra else {"traffic": None})},
     }
 
 
@@ -586,10 +603,11 @@ def run_plain(a):
         achieved = IMPLICIT_BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
     else:
         achieved = BYTES_PER_CELL_STEP * N * acc / dev_s / 1e9 if dev_s > 0 else None
-    traffic = None
-    if a.n == 350 and not (hr or elra or implicit) and os.path.exists(TRAFFIC_N350) and launches > 0:
-        with open(TRAFFIC_N350) as f:
-            traffic = json.load(f)["bytes_per_accepted_step"] * acc / launches
+    tr = {"traffic": None}
+    if a.n == 350 and not (hr or elra or implicit):
+        tr = traffic_fields(TRAFFIC_N350, "bytes_per_accepted_step", acc, launches)
+    elif hr and a.n == 1024:
+        tr = traffic_fields(TRAFFIC_N1024, "bytes_per_accepted_step", acc, launches)
     engine_name, engine_lib = _abi.engine_identity(eng.lib)
     out = {
         "metric": (f"simulated kyr/s at N={a.n} " + ("ELRA (plain solver)" if elra else "implicit LC step (Euler, dt_min 100 yr)"
@@ -613,11 +631,7 @@ def run_plain(a):
                    "kernel_launches": int(launches), "kernel_ms_per_launch": dev_s * 1e3 / max(launches, 1),
                    "us_per_accepted_step": dev_s * 1e6 / max(acc, 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "traffic_unit": "B per launch" if traffic is not None else None,
-                     "traffic_source": (os.path.relpath(TRAFFIC_N350, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, "
-                                        "WRITE_SIZE per accepted step, scaled to this run's steps per launch)")
-                     if traffic is not None else None},
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, **tr},
     }
     out["config"]["engine"] = engine_name
     out["config"]["engine_library"] = os.path.relpath(engine_lib, ROOT) if engine_lib else None
