@@ -531,6 +531,10 @@ __global__ void k_corner_offset(const double* __restrict__ u, int nx, int ny, do
         *off = ((a + b) + (c + d)) - xb;
     }
 }
+__global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, long n) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+        out[c] = (float)in[c];
+}
 __global__ void k_sub_offset_dev(double* u, const double* __restrict__ off, long n) {
     const double o = *off;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) u[c] -= o;
@@ -990,6 +994,57 @@ int fi_set_field(fi_ctx* c, int32_t f, const double* h, size_t n) {
         return FI_OK;
     }
     return fail(c, FI_ERR_INVALID, "fi_set_field: field not writable");
+}
+
+int fi_set_field_dev(fi_ctx* c, int32_t f, const void* d, size_t n) {
+    if (!c || !d || n != c->N) return fail(c, FI_ERR_INVALID, "fi_set_field_dev: size mismatch");
+    HIPCHECK(c, hipSetDevice(c->dev));
+    if (c->initialized && f == FI_U) {
+        const int buf = c->integ ? c->ctl.cur : 0;
+        HIPCHECK(c, hipMemcpyAsync(c->U.p + (size_t)buf * c->N, d, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                   c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->ctl.last_u = buf;
+        if (c->integ) c->k1_stale = true;
+        return FI_OK;
+    }
+    if (c->initialized && f == FI_H_ICE && c->ice_mode == ICE_EXTERNAL) {
+        const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(nb), dim3(256), 0, c->stream, (const double*)d, c->H_ext.p, (long)n);
+        HIPCHECK(c, hipGetLastError());
+        // the host copy serves the host-side diagnostics (fi_get_field(FI_H_ICE), volumes)
+        c->H_ext_host.resize(n);
+        HIPCHECK(c, hipMemcpyAsync(c->H_ext_host.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        return FI_OK;
+    }
+    std::vector<double> h(n);
+    HIPCHECK(c, hipMemcpy(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return fi_set_field(c, f, h.data(), n);
+}
+
+int fi_get_field_dev(fi_ctx* c, int32_t f, void* d, size_t n) {
+    if (!c || !d || n != c->N) return fail(c, FI_ERR_INVALID, "fi_get_field_dev: size mismatch");
+    HIPCHECK(c, hipSetDevice(c->dev));
+    if (c->initialized && f == FI_U) {
+        HIPCHECK(c, hipMemcpyAsync(d, c->U.p + (size_t)c->ctl.last_u * c->N, n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        if (c->ctl.last_u == 2) {   // fi_rhs input buffer kept unmodified: its corner BC applied here
+            if (!c->rkc_off.p) HIPCHECK(c, c->rkc_off.alloc(1));
+            const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_corner_offset, dim3(1), dim3(64), 0, c->stream, (const double*)d, c->nx, c->ny,
+                               c->cfg.viscous_x_border, c->rkc_off.p);
+            hipLaunchKernelGGL(k_sub_offset_dev, dim3(nb), dim3(256), 0, c->stream, (double*)d, c->rkc_off.p, (long)n);
+            HIPCHECK(c, hipGetLastError());
+        }
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        return FI_OK;
+    }
+    std::vector<double> h(n);
+    const int st = fi_get_field(c, f, h.data(), n);
+    if (st) return st;
+    HIPCHECK(c, hipMemcpy(d, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return FI_OK;
 }
 
 int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H, int32_t flat) {

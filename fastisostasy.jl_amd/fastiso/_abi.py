@@ -115,6 +115,8 @@ def declare(lib):
         "fi_last_error": (C.c_char_p, [P]),
         "fi_set_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
         "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
+        "fi_set_field_dev": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_size_t]),
+        "fi_get_field_dev": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_size_t]),
         "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),

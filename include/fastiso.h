@@ -212,6 +212,16 @@ const char* fi_last_error(const fi_ctx* ctx);
 int fi_set_field(fi_ctx* ctx, int32_t field, const double* host, size_t count);
 int fi_get_field(fi_ctx* ctx, int32_t field, double* host, size_t count);
 
+/* fi_set_field / fi_get_field on device-resident FP64 arrays (HIP engine: device pointers on the
+ * context's device; CPU oracle: host pointers): the array-type seam of the reference
+ * (KernelMatrix and the CuMatrix methods, src/convenience_types.jl:2-29, src/domain.jl:66-67) for
+ * a caller that holds its fields on the GPU. FI_U moves device to device in both directions (no
+ * host round trip; the read applies the pending corner BC on the device); FI_H_ICE of an
+ * externally updated ice thickness is converted to FP32 on the device. Other fields are staged
+ * through host memory: static inputs are preprocessed on the host once, before fi_init. */
+int fi_set_field_dev(fi_ctx* ctx, int32_t field, const void* dev, size_t count);
+int fi_get_field_dev(fi_ctx* ctx, int32_t field, void* dev, size_t count);
+
 /* TimeInterpolatedIceThickness(t_vec, H_vec) (src/boundary_conditions.jl:25-40) with the
  * piecewise-linear TimeInterpolation2D (src/interpolations.jl:51-74): H holds nt snapshots of
  * nx*ny each, t ascending (yr). flat_bc != 0 clamps outside [t_1, t_nt], else FI_ERR_TIME_RANGE. */

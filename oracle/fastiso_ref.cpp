@@ -1687,6 +1687,12 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
     return FI_OK;
 }
 
+// the oracle's "device" arrays are host arrays (as for fi_rhs_device)
+int fi_set_field_dev(fi_ctx* c, int32_t f, const void* d, size_t n) {
+    return fi_set_field(c, f, (const double*)d, n);
+}
+int fi_get_field_dev(fi_ctx* c, int32_t f, void* d, size_t n) { return fi_get_field(c, f, (double*)d, n); }
+
 // the CPU oracle allocates no device memory in either entry point
 int fi_batch_integrate_each_noalloc(fi_batch* b, const double* t_targets, int32_t fsal) {
     return fi_batch_integrate_each(b, t_targets, fsal);
