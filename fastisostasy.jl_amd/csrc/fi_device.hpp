@@ -87,6 +87,10 @@ struct KP {
     const double* ice_t;
     const double* ice_a;
     const float* ice_H;
+    // ImposedSeaSurface: dz_ss snapshots [ss_nt][nx*ny] (FP64) at ascending times ss_t (yr)
+    int imposed_ss, ss_nt, ss_flat;
+    const double* ss_t;
+    const double* ss_X;
     // state and fields (float unless noted); U buffer b at Ub + b*Nl, K slot k at Kb + k*Nl
     long Nl;
     double* Ub;

@@ -625,6 +625,9 @@ struct fi_ctx {
     bool implicit = false;   // Maxwell mantle under a rigid or laterally constant lithosphere
     int ice_mode = ICE_EXTERNAL, ice_flat = 0;
     std::vector<double> ice_t, ice_a, ice_H;
+    // ImposedSeaSurface snapshots of dz_ss (FP64 on the device: read once per sparse update)
+    std::vector<double> ss_t, ss_X;
+    int ss_flat = 0;
     // barystatic sea-level rule (fi_set_bsl); A_OCEAN_PD (src/barystatic_sealevel.jl:1) by default
     int bsl_rule = FI_BSL_CONSTANT, bsl_update = FI_BSL_INTERNAL_UPDATE, bsl_flat = 0;
     double bsl_A_ref = 3.625e14;
@@ -636,7 +639,7 @@ struct fi_ctx {
     std::vector<double> H_ice_ref, H_af_ref, H_water_ref, z_ss_ref;
     std::vector<double> H_ext_host;
     // device memory
-    DBuf<double> U, ice_t_d, ice_a_d, part, vpart, bsl_x_d, bsl_y_d;
+    DBuf<double> U, ice_t_d, ice_a_d, part, vpart, bsl_x_d, bsl_y_d, ss_t_d, ss_X_d;
     DBuf<float> kinv_d, k2_d;   // 1/K and K^2 per cell (distortion only)
     DBuf<float> K, kcorner, mult, Df, inv2eta, H_af_ref_d, u_ref_d, z_b_ref_d, z_ss_ref_d, ue_ref_d, H_ext, ue, dz_ss,
         z_ss, zoff, zoff_trig, LW, LT, H_water, sig_load_prev, sig_geo_prev, ue_raw, dz_raw, ext_corners, ice_H_d;
@@ -870,6 +873,8 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     if (!cfg || !out) return FI_ERR_INVALID;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->Wx > 0) || !(cfg->Wy > 0)) return FI_ERR_INVALID;
+    if (cfg->sea_surface < FI_LATERALLY_CONSTANT_SEA_SURFACE || cfg->sea_surface > FI_IMPOSED_SEA_SURFACE)
+        return FI_ERR_INVALID;
     // RelaxedMantle with the laterally variable lithosphere errors in the reference
     // (src/deformation.jl:47-50)
     if (cfg->mantle == FI_RELAXED_MANTLE && cfg->lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE)
@@ -1059,6 +1064,16 @@ int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H
     return FI_OK;
 }
 
+int fi_set_sea_surface_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* X, int32_t flat) {
+    if (!c || !t || !X || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_sea_surface_snapshots: bad args");
+    for (int k = 1; k < nt; ++k)
+        if (!(t[k] > t[k - 1])) return fail(c, FI_ERR_INVALID, "sea-surface times must increase");
+    c->ss_t.assign(t, t + nt);
+    c->ss_X.assign(X, X + (size_t)nt * c->N);
+    c->ss_flat = flat;
+    return FI_OK;
+}
+
 int fi_set_ice_separable(fi_ctx* c, const double* t, const double* a, int32_t nt, const double* H, int32_t flat) {
     if (!c || !t || !a || !H || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_ice_separable: bad args");
     for (int k = 1; k < nt; ++k)
@@ -1115,6 +1130,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     if (g.lithosphere != FI_RIGID_LITHOSPHERE && !c->has_ge) return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
         return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
+    if (g.sea_surface == FI_IMPOSED_SEA_SURFACE && c->ss_t.empty())
+        return fail(c, FI_ERR_INVALID, "ImposedSeaSurface needs dz_ss snapshots (fi_set_sea_surface_snapshots)");
     if (g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD) {
         double s = 0;
         for (double m : c->mask) s += m;
@@ -1276,6 +1293,9 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
         HIPCHECK(c, c->bsl_x_d.alloc(1));
         HIPCHECK(c, c->bsl_y_d.alloc(1));
     }
+    if (g.sea_surface == FI_IMPOSED_SEA_SURFACE) {
+        if (upload(c, c->ss_t_d, c->ss_t) || upload(c, c->ss_X_d, c->ss_X)) return FI_ERR_DEVICE;
+    }
     if (c->ice_mode != ICE_EXTERNAL) {
         if (upload(c, c->ice_t_d, c->ice_t) || upload(c, c->ice_a_d, c->ice_a) || upload(c, c->ice_H_d, to_f(c->ice_H)))
             return FI_ERR_DEVICE;
@@ -1380,6 +1400,11 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.ice_t = c->ice_t_d.p;
     kp.ice_a = c->ice_a_d.p;
     kp.ice_H = c->ice_H_d.p;
+    kp.imposed_ss = g.sea_surface == FI_IMPOSED_SEA_SURFACE;
+    kp.ss_nt = (int)c->ss_t.size();
+    kp.ss_flat = c->ss_flat;
+    kp.ss_t = c->ss_t_d.p;
+    kp.ss_X = c->ss_X_d.p;
     kp.Nl = (long)N;
     kp.Ub = c->U.p;
     kp.Kb = c->K.p;

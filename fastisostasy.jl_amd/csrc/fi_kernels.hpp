@@ -1368,16 +1368,57 @@ __device__ __forceinline__ void volumes_and_bsl(const KP& p, Ctl& s, double t, d
                 }
                 if (e) s.err = ERR_BSL_RANGE;
             }
+            // update_dz_ss! of an ImposedSeaSurface runs after the BSL update and shares its
+            // out-of-range error (src/sealevel.jl:188-191)
+            if (p.imposed_ss && !p.ss_flat && !s.err && (t < p.ss_t[0] || t > p.ss_t[p.ss_nt - 1]))
+                s.err = ERR_TIME_RANGE;
             s.count += 1;
         }
     }
     __syncthreads();
 }
 
+// TimeInterpolation2D bracket of the ImposedSeaSurface snapshots at t (src/interpolations.jl:
+// 51-74): dz_ss = X[i0] + (X[i1] - X[i0]) / dtk * tt, or X[i0] when exact. Every thread runs the
+// same binary search (uniform loads); out of range clamps to the end snapshot (the range error
+// is raised in volumes_and_bsl).
+struct SsBracket {
+    int i0, i1, exact;
+    double tt, dtk;
+};
+__device__ __forceinline__ SsBracket ss_bracket(const KP& p, double t) {
+    SsBracket b{0, 0, 1, 0.0, 1.0};
+    const int nt = p.ss_nt;
+    if (t < p.ss_t[0]) return b;
+    if (t > p.ss_t[nt - 1]) {
+        b.i0 = b.i1 = nt - 1;
+        return b;
+    }
+    int lo = 0, hi = nt - 1;   // first k with ss_t[k] >= t (exists: t <= ss_t[nt-1])
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (p.ss_t[mid] >= t) hi = mid; else lo = mid + 1;
+    }
+    const double tk = p.ss_t[lo];
+    if (tk == t) {
+        b.i0 = b.i1 = lo;
+        return b;
+    }
+    const double tp = p.ss_t[lo - 1];
+    b.i0 = lo - 1;
+    b.i1 = lo;
+    b.exact = 0;
+    b.tt = t - tp;
+    b.dtk = tk - tp;
+    return b;
+}
+
 // pointwise sparse diagnostics with the updated barystatic sea level z_bsl
 __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, const StageDesc& st, double uoff,
                                                const IceT it, double z_bsl) {
     const float* ec = p.ext_corners;
+    SsBracket sb{0, 0, 1, 0.0, 1.0};
+    if (p.imposed_ss) sb = ss_bracket(p, st.t);
     const double oe = ((0.25 * ec[0] + 0.25 * ec[1]) + (0.25 * ec[2] + 0.25 * ec[3])) - p.ebc_x;
     const double og = ((0.25 * ec[4] + 0.25 * ec[5]) + (0.25 * ec[6] + 0.25 * ec[7])) - p.sbc_x;
     const int nx = p.nx;
@@ -1390,7 +1431,15 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
         const double ue_old = p.ue[c];
         const double zb_old = (double)p.z_b_ref[c] + ue_old + u;
         const double ue_new = p.rigid_litho ? 0.0 : (double)p.ue_raw[c] - oe;
-        const double dz_new = p.lv_surface ? (double)p.dz_raw[c] - og : (double)p.dz_ss[c];
+        double dz_new;
+        if (p.lv_surface) {
+            dz_new = (double)p.dz_raw[c] - og;
+        } else if (p.imposed_ss) {
+            const double x0 = p.ss_X[sb.i0 * p.Nl + c];
+            dz_new = sb.exact ? x0 : x0 + (p.ss_X[sb.i1 * p.Nl + c] - x0) / sb.dtk * sb.tt;
+        } else {
+            dz_new = (double)p.dz_ss[c];
+        }
         const double zss_new = (double)p.z_ss_ref[c] + dz_new + z_bsl;
         const double haf = hab(H, zb_old - zss_new, p.rho_sw_ice);
         const bool mgc = haf > 0;

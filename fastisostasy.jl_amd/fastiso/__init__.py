@@ -9,7 +9,7 @@ from .bcs import (BoundaryConditions, CornerBC, ExtendedBCSpace, ExternallyUpdat
                   RegularBCSpace, SeparableIceThickness, TimeInterpolatedIceThickness)
 from .sealevel import (A_OCEAN_PD, ConstantBSL, ConstantOceanSurfaceBSL, ExternalBSLUpdate,
                        GoelzerAdjustmentContribution, GoelzerDensityContribution, GoelzerVolumeContribution,
-                       ImposedBSL, InteractiveSealevelLoad, InternalBSLUpdate, LaterallyConstantSeaSurface,
+                       ImposedBSL, ImposedSeaSurface, InteractiveSealevelLoad, InternalBSLUpdate, LaterallyConstantSeaSurface,
                        LaterallyVariableSeaSurface, NoAdjustmentContribution, NoDensityContribution,
                        NoSealevelLoad, NoVolumeContribution, PiecewiseConstantBSL, ReferenceBSL,
                        RegionalSeaLevel, interpolate0d, update_bsl)

@@ -26,7 +26,7 @@ FI_ERR_DT_MIN = -7
 FI_RIGID_LITHOSPHERE, FI_LATERALLY_CONSTANT_LITHOSPHERE, FI_LATERALLY_VARIABLE_LITHOSPHERE = 0, 1, 2
 FI_RIGID_MANTLE, FI_RELAXED_MANTLE, FI_MAXWELL_MANTLE = 0, 1, 2
 FI_INCOMPRESSIBLE_LITHOSPHERE_COLUMN, FI_COMPRESSIBLE_LITHOSPHERE_COLUMN = 0, 1
-FI_LATERALLY_CONSTANT_SEA_SURFACE, FI_LATERALLY_VARIABLE_SEA_SURFACE = 0, 1
+FI_LATERALLY_CONSTANT_SEA_SURFACE, FI_LATERALLY_VARIABLE_SEA_SURFACE, FI_IMPOSED_SEA_SURFACE = 0, 1, 2
 FI_NO_SEALEVEL_LOAD, FI_INTERACTIVE_SEALEVEL_LOAD = 0, 1
 FI_NO_CONTRIBUTION, FI_GOELZER_CONTRIBUTION = 0, 1
 
@@ -118,6 +118,7 @@ def declare(lib):
         "fi_set_field_dev": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_size_t]),
         "fi_get_field_dev": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_size_t]),
         "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
+        "fi_set_sea_surface_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),
         "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),

@@ -98,6 +98,12 @@ class Engine:
         H = np.concatenate([fortran_flat(H) for H in H_list])
         self._check(self.lib.fi_set_ice_snapshots(self.h, _ptr(t), len(t), _ptr(H), int(flat_bc)))
 
+    def set_sea_surface_snapshots(self, t, dz_list, flat_bc=False):
+        """fi_set_sea_surface_snapshots: the dz_ss snapshots of an ImposedSeaSurface."""
+        t = _f64(t)
+        X = np.concatenate([fortran_flat(x) for x in dz_list])
+        self._check(self.lib.fi_set_sea_surface_snapshots(self.h, _ptr(t), len(t), _ptr(X), int(flat_bc)))
+
     def set_ice_separable(self, t, a, H_shape, flat_bc=False):
         t, a, H = _f64(t), _f64(a), fortran_flat(H_shape)
         self._check(self.lib.fi_set_ice_separable(self.h, _ptr(t), _ptr(a), len(t), _ptr(H), int(flat_bc)))

@@ -4,6 +4,20 @@ import numpy as np
 
 class LaterallyConstantSeaSurface:  pass   # src/sealevel.jl:122
 class LaterallyVariableSeaSurface:  pass   # src/sealevel.jl:130
+
+
+class ImposedSeaSurface:
+    """ImposedSeaSurface(dz_ss_itp) (src/sealevel.jl:132-134): every sparse update sets dz_ss to
+    the piecewise-linear interpolation of the snapshots dz_ss_vec at times t_vec (yr)
+    (update_dz_ss!, src/sealevel.jl:188-191; TimeInterpolation2D, src/interpolations.jl:51-74)."""
+    def __init__(self, t_vec, dz_ss_vec, flat_bc=False):
+        self.t_vec = np.asarray(t_vec, dtype=np.float64)
+        self.dz_ss_vec = [np.asarray(x, dtype=np.float64) for x in dz_ss_vec]
+        if len(self.t_vec) != len(self.dz_ss_vec) or len(self.t_vec) < 1:
+            raise ValueError("t_vec and dz_ss_vec must have the same, non-zero length")
+        if np.any(np.diff(self.t_vec) <= 0):
+            raise ValueError("t_vec must increase")
+        self.flat_bc = bool(flat_bc)
 class NoSealevelLoad:               pass   # src/loads.jl:19
 class InteractiveSealevelLoad:      pass   # src/loads.jl:26
 class InternalBSLUpdate:            pass   # src/barystatic_sealevel.jl:60

@@ -15,7 +15,7 @@ from .domain import PhysicalConstants
 from .engine import Engine
 from .greens import elastic_green, geoid_green, green_viscous
 from .sealevel import (GoelzerAdjustmentContribution, GoelzerDensityContribution, GoelzerVolumeContribution,
-                       InteractiveSealevelLoad, LaterallyVariableSeaSurface, bsl_inputs)
+                       ImposedSeaSurface, InteractiveSealevelLoad, LaterallyVariableSeaSurface, bsl_inputs)
 from .solidearth import (CompressibleLithosphereColumn, LaterallyConstantLithosphere, MaxwellMantle,
                          RelaxedMantle, RigidLithosphere, RigidMantle)
 
@@ -70,6 +70,7 @@ def build_config(domain, c, solidearth, sealevel, bcs, opts, device=0):
                               else _abi.FI_INCOMPRESSIBLE_LITHOSPHERE_COLUMN)
     cfg.sea_surface = (_abi.FI_LATERALLY_VARIABLE_SEA_SURFACE
                        if isinstance(sealevel.surface, LaterallyVariableSeaSurface)
+                       else _abi.FI_IMPOSED_SEA_SURFACE if isinstance(sealevel.surface, ImposedSeaSurface)
                        else _abi.FI_LATERALLY_CONSTANT_SEA_SURFACE)
     cfg.sealevel_load = (_abi.FI_INTERACTIVE_SEALEVEL_LOAD if isinstance(sealevel.load, InteractiveSealevelLoad)
                          else _abi.FI_NO_SEALEVEL_LOAD)
@@ -144,7 +145,9 @@ class Simulation:
             raise TypeError("unknown ice thickness forcing")
         self.inputs = {"cfg": cfg, "fields": fields, "ice": ice, "t_span": self.t_span,
                        "tout": sorted(set(self.nout.t) | set(self.ncout.t if self.ncout else ())),
-                       "bsl": bsl_inputs(sealevel.bsl, sealevel.update_bsl)}
+                       "bsl": bsl_inputs(sealevel.bsl, sealevel.update_bsl),
+                       "sea_surface": ((sealevel.surface.t_vec, sealevel.surface.dz_ss_vec, sealevel.surface.flat_bc)
+                                       if isinstance(sealevel.surface, ImposedSeaSurface) else None)}
         self.engine = None
         if engine == "hip":
             self.engine = Engine(cfg)
@@ -179,6 +182,8 @@ def push_inputs(eng, inputs):
         eng.set_ice_snapshots(ice[1], ice[2], ice[3])
     elif ice[0] == "separable":
         eng.set_ice_separable(ice[1], ice[2], ice[3], ice[4])
+    if inputs.get("sea_surface") is not None:
+        eng.set_sea_surface_snapshots(*inputs["sea_surface"])
     if "bsl" in inputs:
         eng.set_bsl(*inputs["bsl"])
     eng.set_output_times(inputs["tout"])

@@ -90,7 +90,7 @@ function HipSimulation(sim)
     cfg.lithosphere = se.lithosphere isa RigidLithosphere ? 0 :
         se.lithosphere isa LaterallyConstantLithosphere ? 1 : 2
     cfg.mantle = se.mantle isa RigidMantle ? 0 : se.mantle isa RelaxedMantle ? 1 : 2
-    cfg.sea_surface = sl.surface isa LaterallyVariableSeaSurface ? 1 : 0
+    cfg.sea_surface = sl.surface isa LaterallyVariableSeaSurface ? 1 : sl.surface isa ImposedSeaSurface ? 2 : 0
     cfg.sealevel_load = sl.load isa InteractiveSealevelLoad ? 1 : 0
     cfg.lithosphere_column = se.lithosphere_column isa CompressibleLithosphereColumn ? 1 : 0
     cfg.bsl_z = sl.bsl.z
@@ -127,6 +127,13 @@ function HipSimulation(sim)
         check(ctx, ccall((:fi_set_ice_snapshots, libfi), Cint,
             (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32), ctx, Float64.(it.t_vec),
             length(it.t_vec), H, Int32(it.H_itp.flat_bc)))
+    end
+    if sl.surface isa ImposedSeaSurface        # TimeInterpolation2D (src/interpolations.jl:12-26)
+        itp = sl.surface.dz_ss_itp
+        X = reduce(vcat, vec.(Array.(itp.X)))
+        check(ctx, ccall((:fi_set_sea_surface_snapshots, libfi), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32), ctx, Float64.(itp.t),
+            length(itp.t), X, Int32(itp.flat_bc)))
     end
     tout = Float64.(sort(vcat(sim.nout.t, sim.ncout.t)))
     check(ctx, ccall((:fi_set_output_times, libfi), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int32),

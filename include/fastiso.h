@@ -57,7 +57,8 @@ enum { /* src/solidearth.jl:95-109 */
 };
 enum { /* src/sealevel.jl:114-134 */
     FI_LATERALLY_CONSTANT_SEA_SURFACE = 0,
-    FI_LATERALLY_VARIABLE_SEA_SURFACE = 1
+    FI_LATERALLY_VARIABLE_SEA_SURFACE = 1,
+    FI_IMPOSED_SEA_SURFACE = 2          /* dz_ss from fi_set_sea_surface_snapshots */
 };
 enum { /* src/loads.jl:12-26 */
     FI_NO_SEALEVEL_LOAD = 0,
@@ -226,6 +227,12 @@ int fi_get_field_dev(fi_ctx* ctx, int32_t field, void* dev, size_t count);
  * piecewise-linear TimeInterpolation2D (src/interpolations.jl:51-74): H holds nt snapshots of
  * nx*ny each, t ascending (yr). flat_bc != 0 clamps outside [t_1, t_nt], else FI_ERR_TIME_RANGE. */
 int fi_set_ice_snapshots(fi_ctx* ctx, const double* t, int32_t nt, const double* H, int32_t flat_bc);
+
+/* ImposedSeaSurface(dz_ss_itp) (src/sealevel.jl:132-134): with cfg.sea_surface =
+ * FI_IMPOSED_SEA_SURFACE every sparse update sets dz_ss to the TimeInterpolation2D of these nt
+ * snapshots (nx*ny each, t ascending, yr) at the update's time (update_dz_ss!,
+ * src/sealevel.jl:188-191), instead of the sea-surface convolution. flat_bc as for the ice. */
+int fi_set_sea_surface_snapshots(fi_ctx* ctx, const double* t, int32_t nt, const double* dz_ss, int32_t flat_bc);
 
 /* The same forcing when every snapshot is a(t_k) * H_shape: H(t) = a(t) H_shape with a(t)
  * piecewise linear over the knots (t_k, a_k). Mathematically identical to fi_set_ice_snapshots

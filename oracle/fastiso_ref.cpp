@@ -342,6 +342,9 @@ struct fi_ctx {
     int ice_mode = 0;  // 0 external, 1 snapshots, 2 separable
     vec ice_t, ice_a, ice_H;
     int ice_flat = 0;
+    // ImposedSeaSurface snapshots of dz_ss (src/sealevel.jl:132-134)
+    vec ss_t, ss_X;
+    int ss_flat = 0;
     // barystatic sea-level rule (fi_set_bsl); A_OCEAN_PD (src/barystatic_sealevel.jl:1) by default
     int bsl_rule = FI_BSL_CONSTANT, bsl_update = FI_BSL_INTERNAL_UPDATE, bsl_flat = 0;
     double bsl_A_ref = 3.625e14;
@@ -406,6 +409,28 @@ int interpolate_ice(fi_ctx* c, double t, double* out) {
             }
         }
     }
+    return FI_OK;
+}
+
+// TimeInterpolation2D interpolate! (src/interpolations.jl:51-74) over nt snapshots of N cells
+int interpolate_snapshots(fi_ctx* c, const vec& tv, const vec& X, int flat, double t, double* out) {
+    const int nt = (int)tv.size();
+    const size_t N = c->N;
+    if (t < tv.front() || t > tv.back()) {
+        if (!flat) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
+        const size_t k = t < tv.front() ? 0 : (size_t)nt - 1;
+        std::copy(X.begin() + k * N, X.begin() + (k + 1) * N, out);
+        return FI_OK;
+    }
+    const int i = (int)(std::lower_bound(tv.begin(), tv.end(), t) - tv.begin());  // searchsortedfirst
+    if (tv[i] == t) {
+        std::copy(X.begin() + (size_t)i * N, X.begin() + (size_t)(i + 1) * N, out);
+        return FI_OK;
+    }
+    const double f = t - tv[i - 1], dtk = tv[i] - tv[i - 1];
+    const double* a = X.data() + (size_t)(i - 1) * N;
+    const double* b = X.data() + (size_t)i * N;
+    for (size_t p = 0; p < N; ++p) out[p] = a[p] + (b[p] - a[p]) / dtk * f;
     return FI_OK;
 }
 
@@ -528,6 +553,10 @@ int sparse_block(fi_ctx* c) {
         }
         samesize_conv_ext(c->dz_ss.data(), c->buf.data(), c->cp, c->gg_hat, c->i1, c->i2, c->j1, c->j2,
                           g.ssh_x_border);
+    } else if (g.sea_surface == FI_IMPOSED_SEA_SURFACE) {
+        // update_dz_ss!(sim, ::ImposedSeaSurface) (src/sealevel.jl:188-191) at sim.timer.t
+        const int st = interpolate_snapshots(c, c->ss_t, c->ss_X, c->ss_flat, c->t_timer, c->dz_ss.data());
+        if (st) return st;
     }
     // update_z_ss! (src/sealevel.jl:228-231)
     for (size_t p = 0; p < N; ++p) c->z_ss[p] = c->z_ss_ref[p] + c->dz_ss[p] + c->z_bsl;
@@ -733,6 +762,8 @@ int fi_create(const fi_config* cfg, fi_ctx** out) {
     if (!cfg || !out) return FI_ERR_INVALID;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->Wx > 0) || !(cfg->Wy > 0)) return FI_ERR_INVALID;
+    if (cfg->sea_surface < FI_LATERALLY_CONSTANT_SEA_SURFACE || cfg->sea_surface > FI_IMPOSED_SEA_SURFACE)
+        return FI_ERR_INVALID;
     // RelaxedMantle with the laterally variable lithosphere errors in the reference
     // (src/deformation.jl:47-50)
     if (cfg->mantle == FI_RELAXED_MANTLE && cfg->lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE)
@@ -888,6 +919,16 @@ int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H
     return FI_OK;
 }
 
+int fi_set_sea_surface_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* X, int32_t flat) {
+    if (!c || !t || !X || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_sea_surface_snapshots: bad args");
+    for (int k = 1; k < nt; ++k)
+        if (!(t[k] > t[k - 1])) return fail(c, FI_ERR_INVALID, "sea-surface times must increase");
+    c->ss_t.assign(t, t + nt);
+    c->ss_X.assign(X, X + (size_t)nt * c->N);
+    c->ss_flat = flat;
+    return FI_OK;
+}
+
 int fi_set_ice_separable(fi_ctx* c, const double* t, const double* a, int32_t nt, const double* H, int32_t flat) {
     if (!c || !t || !a || !H || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_ice_separable: bad args");
     for (int k = 1; k < nt; ++k)
@@ -946,6 +987,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
         return fail(c, FI_ERR_INVALID, "elastic Green kernel required");
     if (g.sea_surface == FI_LATERALLY_VARIABLE_SEA_SURFACE && !c->has_gg)
         return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
+    if (g.sea_surface == FI_IMPOSED_SEA_SURFACE && c->ss_t.empty())
+        return fail(c, FI_ERR_INVALID, "ImposedSeaSurface needs dz_ss snapshots (fi_set_sea_surface_snapshots)");
     if (g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD) {
         double s = 0;
         for (double m : c->maskactive) s += m;

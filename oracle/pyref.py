@@ -63,6 +63,7 @@ def lib():
             "fi_set_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
             "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
             "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
+            "fi_set_sea_surface_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
             "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
             "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),
             "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),
@@ -149,6 +150,10 @@ class RefEngine:
             a = np.asarray(ice[2], dtype=np.float64)
             H = _flat(ice[3])
             e._ck(e.L.fi_set_ice_separable(e.h, _p(t), _p(a), len(t), _p(H), int(ice[4])))
+        if inputs.get("sea_surface") is not None:
+            t = np.asarray(inputs["sea_surface"][0], dtype=np.float64)
+            X = np.concatenate([_flat(x) for x in inputs["sea_surface"][1]])
+            e._ck(e.L.fi_set_sea_surface_snapshots(e.h, _p(t), len(t), _p(X), int(inputs["sea_surface"][2])))
         if "bsl" in inputs:
             e.set_bsl(*inputs["bsl"])
         tout = np.asarray(inputs["tout"], dtype=np.float64)
@@ -172,6 +177,9 @@ class RefEngine:
             if getattr(self, "_owned", True):
                 self.L.fi_destroy(self.h)
             self.h = None
+
+    def close(self):
+        self.__del__()
 
     def _ck(self, st):
         if st != 0:
