@@ -142,6 +142,11 @@ struct KP {
     unsigned long long spin_limit;  // wall-clock ticks before a barrier wait gives up
     // optional phase timing (workgroup 0): [tag] work ticks, [8+tag] wait ticks, [16+tag] count
     unsigned long long* ptime;
+    // Timer history (null when off): RHS call k since fi_init (k < hist_cap) stores its time and
+    // the device wall clock at its end at index k, written by workgroup 0 alone
+    double* hist_t;
+    unsigned long long* hist_clk;
+    long long hist_cap;
     // relaxed mantle (ELRA): viscous Green half spectrum [nfhx][nfy] (FP64, pre-scaled by
     // 1/(nfx nfy)), 1/tau per cell (1/yr) and the equilibrium displacement u_eq
     int elra;

@@ -657,6 +657,10 @@ struct fi_ctx {
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
     DBuf<unsigned long long> ptime;
+    // Timer history: capacity (0: off) and the device buffers
+    long long hist_cap = 0;
+    DBuf<double> hist_t;
+    DBuf<unsigned long long> hist_clk;
     KP kp;
     Ctl ctl;
     int G = 0;
@@ -1061,6 +1065,47 @@ int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H
     c->ice_H.assign(H, H + (size_t)nt * c->N);
     c->ice_a.assign(nt, 1.0);
     c->ice_flat = flat;
+    return FI_OK;
+}
+
+int fi_set_timer_history(fi_ctx* c, int64_t capacity) {
+    if (!c || capacity < 0) return fail(c, FI_ERR_INVALID, "fi_set_timer_history: bad args");
+    if (c->initialized) return fail(c, FI_ERR_STATE, "fi_set_timer_history: call it before fi_init");
+    c->hist_cap = capacity;
+    return FI_OK;
+}
+
+int fi_get_timer_history(fi_ctx* c, double* t, double* seconds, int64_t capacity, int64_t* n_total) {
+    if (!c || !n_total || capacity < 0 || (capacity > 0 && (!t || !seconds)))
+        return fail(c, FI_ERR_INVALID, "fi_get_timer_history: bad args");
+    *n_total = 0;
+    if (!c->initialized || !c->hist_t.p) return FI_OK;
+    HIPCHECK(c, hipSetDevice(c->dev));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const long long n = c->ctl.nrhs;
+    const size_t m = (size_t)std::min(n, c->hist_cap);
+    std::vector<double> tv(m);
+    std::vector<unsigned long long> clk(m);
+    if (m) {
+        HIPCHECK(c, hipMemcpy(tv.data(), c->hist_t.p, m * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(c, hipMemcpy(clk.data(), c->hist_clk.p, m * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    }
+    int khz = 0;
+    HIPCHECK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->dev));
+    if (khz <= 0) return fail(c, FI_ERR_DEVICE, "fi_get_timer_history: no wall-clock rate");
+    // t_computation! keeps the calls past t_span[0]; run! subtracts the first one's time
+    int64_t j = 0;
+    unsigned long long clk0 = 0;
+    for (size_t k = 0; k < m; ++k) {
+        if (!(tv[k] > c->ctl.t0)) continue;
+        if (j == 0) clk0 = clk[k];
+        if (j < capacity) {
+            t[j] = tv[k];
+            seconds[j] = (double)(clk[k] - clk0) / (1e3 * (double)khz);
+        }
+        ++j;
+    }
+    *n_total = j;
     return FI_OK;
 }
 
@@ -1495,6 +1540,16 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     HIPCHECK(c, c->kp_d.alloc(1));
     kp.ctl = c->ctl_d.p;
     kp.bar = c->bar_d.p;
+    kp.hist_t = nullptr;
+    kp.hist_clk = nullptr;
+    kp.hist_cap = 0;
+    if (c->hist_cap > 0) {
+        HIPCHECK(c, c->hist_t.alloc((size_t)c->hist_cap));
+        HIPCHECK(c, c->hist_clk.alloc((size_t)c->hist_cap));
+        kp.hist_t = c->hist_t.p;
+        kp.hist_clk = c->hist_clk.p;
+        kp.hist_cap = c->hist_cap;
+    }
     kp.ptime = nullptr;
     if (getenv("FASTISO_PHASE_TIMING")) {
         HIPCHECK(c, c->ptime.alloc(32));
@@ -2295,6 +2350,7 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, bool mt_double, size_t lds
         c->ctl.next_out = t_target[m];
         att0[m] = c->ctl.nacc + c->ctl.nrej;
         kph[m] = c->kp;
+        kph[m].hist_t = nullptr;   // the Timer history is kept for single contexts only
         kph[m].mode = MODE_INTEGRATE;
         kph[m].max_steps = max_steps;
         kph[m].write_sigma_always = 0;

@@ -179,6 +179,16 @@ __device__ __forceinline__ double hab(double H, double zd, double rsi) {
 }
 
 __device__ __forceinline__ const float* kslot(const KP& p, int k) { return p.Kb + (long)k * p.Nl; }
+
+// t_computation! (src/simulation.jl:46-52), called by thread 0 at the end of RHS call k (0-based
+// since fi_init): workgroup 0 of the team stores (t, wall clock) at index k; the host keeps the
+// calls past t0. Stores only, no read back.
+__device__ __forceinline__ void timer_push(const KP& p, const Team& tm, long long k, double t) {
+    if (p.hist_t && tm.rank == 0 && k < p.hist_cap) {
+        p.hist_t[k] = t;
+        p.hist_clk[k] = wall_clock64();
+    }
+}
 __device__ __forceinline__ float2* specbuf(const KP& p, int b) { return p.specb + (long)b * p.specN; }
 __device__ __forceinline__ double* ubuf(const KP& p, int b) { return p.Ub + (long)b * p.Nl; }
 
@@ -1724,6 +1734,7 @@ __device__ __forceinline__ bool elra_call(const KP& p, const Team& tm, Ctl& s, S
     if (!grid_sync(p, tm, gen, flag, TAG_C2R, tl)) return false;
     if (threadIdx.x == 0) {
         s.nrhs += 1;
+        timer_push(p, tm, s.nrhs - 1, st.t);
         s.t_last = st.t;
         s.last_k = st.kout;
         s.last_trig = trig ? 1 : 0;
@@ -1917,6 +1928,7 @@ __device__ __forceinline__ bool implicit_call(const KP& p, const Team& tm, Ctl& 
     if (!grid_sync(p, tm, gen, flag, TAG_C2R, tl)) return false;
     if (threadIdx.x == 0) {
         s.nrhs += 1;
+        timer_push(p, tm, s.nrhs - 1, st.t);
         s.t_last = st.t;
         s.last_trig = trig ? 1 : 0;
     }
@@ -1981,6 +1993,7 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
     }
     if (threadIdx.x == 0) {
         s.nrhs += 1;
+        timer_push(p, tm, s.nrhs - 1, st.t);
         s.t_last = st.t;
         s.last_k = st.kout;
         s.last_trig = trig ? 1 : 0;

@@ -119,6 +119,8 @@ def declare(lib):
         "fi_get_field_dev": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_size_t]),
         "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_sea_surface_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
+        "fi_set_timer_history": (C.c_int, [P, C.c_int64]),
+        "fi_get_timer_history": (C.c_int, [P, _dp, _dp, C.c_int64, C.POINTER(C.c_int64)]),
         "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
         "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),
         "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),

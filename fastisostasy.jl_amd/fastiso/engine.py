@@ -104,6 +104,19 @@ class Engine:
         X = np.concatenate([fortran_flat(x) for x in dz_list])
         self._check(self.lib.fi_set_sea_surface_snapshots(self.h, _ptr(t), len(t), _ptr(X), int(flat_bc)))
 
+    def set_timer_history(self, capacity):
+        """fi_set_timer_history: record (t, device wall clock) of the first `capacity` RHS calls."""
+        self._check(self.lib.fi_set_timer_history(self.h, int(capacity)))
+
+    def timer_history(self, capacity):
+        """fi_get_timer_history: (t_vec, t_computation, n_total) as Timer holds them."""
+        t = np.empty(max(int(capacity), 1))
+        sec = np.empty_like(t)
+        n = C.c_int64(0)
+        self._check(self.lib.fi_get_timer_history(self.h, _ptr(t), _ptr(sec), int(capacity), C.byref(n)))
+        m = min(n.value, int(capacity))
+        return t[:m].copy(), sec[:m].copy(), n.value
+
     def set_ice_separable(self, t, a, H_shape, flat_bc=False):
         t, a, H = _f64(t), _f64(a), fortran_flat(H_shape)
         self._check(self.lib.fi_set_ice_separable(self.h, _ptr(t), _ptr(a), len(t), _ptr(H), int(flat_bc)))

@@ -30,7 +30,7 @@ class Ensemble:
         self.members = []
         for m, inp in enumerate(inputs_list):
             e = Engine.member(self.lib, inp["cfg"], self.lib.fi_batch_member(self.h, m))
-            push_inputs(e, inp)
+            push_inputs(e, inp, timer=False)
             self.members.append(e)
         self._check(self.lib.fi_batch_set_team_size(self.h, int(team_size)))
 

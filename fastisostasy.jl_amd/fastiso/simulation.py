@@ -48,6 +48,37 @@ class NativeOutput:
         self.k = 1
 
 
+class Timer:
+    """Timer (src/simulation.jl:28-52): t_vec and t_computation hold, for every right-hand-side
+    call past t_span[0], its time (yr) and the seconds since the first such call, recorded on the
+    device (fi_set_timer_history); they are read from the engine on access. n_total counts every
+    call; the first `capacity` are kept."""
+    def __init__(self, sim, capacity):
+        self._sim, self.capacity = sim, int(capacity)
+        self.t_span = sim.t_span
+
+    @property
+    def t(self):
+        return self._sim.engine.stats()["t"]
+
+    def _read(self):
+        if self._sim.engine is None or self.capacity == 0:
+            return np.zeros(0), np.zeros(0), 0
+        return self._sim.engine.timer_history(self.capacity)
+
+    @property
+    def t_vec(self):
+        return self._read()[0]
+
+    @property
+    def t_computation(self):
+        return self._read()[1]
+
+    @property
+    def n_total(self):
+        return self._read()[2]
+
+
 def build_config(domain, c, solidearth, sealevel, bcs, opts, device=0):
     """The POD fi_config for a set of reference-style model objects."""
     cfg = _abi.FiConfig()
@@ -102,7 +133,7 @@ class Simulation:
 
     def __init__(self, domain, bcs, sealevel, solidearth, t_span, opts=None, u_ref=None, ue_ref=None,
                  dz_ss_ref=None, z_b_ref=None, nout=None, ncout=None, c=None, engine="hip", device=0,
-                 quad_precision=4):
+                 quad_precision=4, timer_capacity=1 << 20):
         self.domain, self.bcs, self.sealevel, self.solidearth = domain, bcs, sealevel, solidearth
         self.t_span = (float(t_span[0]), float(t_span[1]))
         self.opts = opts or SolverOptions()
@@ -147,7 +178,9 @@ class Simulation:
                        "tout": sorted(set(self.nout.t) | set(self.ncout.t if self.ncout else ())),
                        "bsl": bsl_inputs(sealevel.bsl, sealevel.update_bsl),
                        "sea_surface": ((sealevel.surface.t_vec, sealevel.surface.dz_ss_vec, sealevel.surface.flat_bc)
-                                       if isinstance(sealevel.surface, ImposedSeaSurface) else None)}
+                                       if isinstance(sealevel.surface, ImposedSeaSurface) else None),
+                       "timer_capacity": int(timer_capacity)}
+        self.timer = Timer(self, timer_capacity)
         self.engine = None
         if engine == "hip":
             self.engine = Engine(cfg)
@@ -173,8 +206,9 @@ class Simulation:
         return False
 
 
-def push_inputs(eng, inputs):
-    """Load assembled inputs into an engine context and run fi_init."""
+def push_inputs(eng, inputs, timer=True):
+    """Load assembled inputs into an engine context and run fi_init (timer=False: no Timer
+    history, as for ensemble members)."""
     for name, X in inputs["fields"].items():
         eng.set_field(name, X)
     ice = inputs["ice"]
@@ -184,6 +218,8 @@ def push_inputs(eng, inputs):
         eng.set_ice_separable(ice[1], ice[2], ice[3], ice[4])
     if inputs.get("sea_surface") is not None:
         eng.set_sea_surface_snapshots(*inputs["sea_surface"])
+    if timer and inputs.get("timer_capacity"):
+        eng.set_timer_history(inputs["timer_capacity"])
     if "bsl" in inputs:
         eng.set_bsl(*inputs["bsl"])
     eng.set_output_times(inputs["tout"])

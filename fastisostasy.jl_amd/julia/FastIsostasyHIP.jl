@@ -64,6 +64,9 @@ end
 check(ctx, st) = st == 0 || error("fastiso HIP engine error $st: " *
     unsafe_string(ccall((:fi_last_error, libfi), Cstring, (Ptr{Cvoid},), ctx)))
 
+# Timer history kept per context: the first TIMER_CAPACITY right-hand-side calls
+const TIMER_CAPACITY = Int64(1) << 20
+
 mutable struct HipSimulation
     sim::Any                 # the reference Simulation (host copy of setup and outputs)
     ctx::Ptr{Cvoid}
@@ -135,6 +138,7 @@ function HipSimulation(sim)
             (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32), ctx, Float64.(itp.t),
             length(itp.t), X, Int32(itp.flat_bc)))
     end
+    check(ctx, ccall((:fi_set_timer_history, libfi), Cint, (Ptr{Cvoid}, Int64), ctx, TIMER_CAPACITY))
     tout = Float64.(sort(vcat(sim.nout.t, sim.ncout.t)))
     check(ctx, ccall((:fi_set_output_times, libfi), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int32),
         ctx, tout, length(tout)))
@@ -163,6 +167,21 @@ function run!(h::HipSimulation)
     GC.@preserve h check(h.ctx, ccall((:fi_integrate, libfi), Cint,
         (Ptr{Cvoid}, Float64, Ptr{Cvoid}, Ptr{Cvoid}), h.ctx, h.sim.timer.t_span[2], cb,
         pointer_from_objref(h)))
+    read_timer!(h)
+    return nothing
+end
+
+# sim.timer.t_vec / t_computation as t_computation! and run! leave them (src/simulation.jl:46-52,260)
+function read_timer!(h::HipSimulation)
+    t = Vector{Float64}(undef, TIMER_CAPACITY)
+    s = similar(t)
+    n = Ref{Int64}(0)
+    check(h.ctx, ccall((:fi_get_timer_history, libfi), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ref{Int64}), h.ctx, t, s, TIMER_CAPACITY, n))
+    m = min(n[], TIMER_CAPACITY)
+    tm = h.sim.timer
+    empty!(tm.t_vec); append!(tm.t_vec, t[1:m])
+    empty!(tm.t_computation); append!(tm.t_computation, s[1:m])
     return nothing
 end
 

@@ -300,6 +300,18 @@ int fi_integrate(fi_ctx* ctx, double t_target, fi_output_cb cb, void* user);
 
 int fi_get_stats(fi_ctx* ctx, fi_stats* out);
 
+/* Timer history (Timer.t_vec and Timer.t_computation, src/simulation.jl:28-52, pushed by
+ * t_computation! at the end of every update_diagnostics!, src/simulation.jl:391). With
+ * fi_set_timer_history(capacity > 0) before fi_init, right-hand-side call k of the context
+ * (0-based since fi_init, k < capacity) stores its time and the device wall clock at its end, on
+ * the device. fi_get_timer_history returns, as t_computation! keeps them, the recorded calls whose
+ * time is past t0: *n_total of them, the first `capacity` copied to t (yr) and seconds (since the
+ * first of them, as run! leaves t_computation, src/simulation.jl:260). A checkpoint load rewinds
+ * the call count, so a rolled-back period is overwritten by its rerun. Single contexts only
+ * (ensemble members record nothing). */
+int fi_set_timer_history(fi_ctx* ctx, int64_t capacity);
+int fi_get_timer_history(fi_ctx* ctx, double* t, double* seconds, int64_t capacity, int64_t* n_total);
+
 /* Sets the integrator's time to t >= t0 without integrating (after fi_integrator_init), for
  * drivers that set the displacement themselves (the surrogate macro step and its rollback:
  * fi_set_field(FI_U) then fi_set_time). The column loads are stored at (t, u) and the sparse

@@ -29,6 +29,7 @@
 //   the reference; constants chosen below are the published PI-controller defaults).
 // =====================================================================================
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <omp.h>
@@ -342,6 +343,9 @@ struct fi_ctx {
     int ice_mode = 0;  // 0 external, 1 snapshots, 2 separable
     vec ice_t, ice_a, ice_H;
     int ice_flat = 0;
+    // Timer history (src/simulation.jl:28-52): capacity, the recorded calls' times and wall seconds
+    long long hist_cap = 0;
+    vec hist_t, hist_wall;
     // ImposedSeaSurface snapshots of dz_ss (src/sealevel.jl:132-134)
     vec ss_t, ss_X;
     int ss_flat = 0;
@@ -696,6 +700,16 @@ int update_diagnostics(fi_ctx* c, double* dudt, double* u, double t) {
         update_dudt_lvelva(c, u, dudt);
     }
     if (dudt != c->dudt.data()) std::memcpy(c->dudt.data(), dudt, N * sizeof(double));
+    // t_computation! (src/simulation.jl:46-52): call nrhs - 1 since fi_init, kept below the capacity
+    if (c->nrhs - 1 < c->hist_cap) {
+        const size_t k = (size_t)(c->nrhs - 1);
+        if (c->hist_t.size() <= k) {
+            c->hist_t.resize(k + 1);
+            c->hist_wall.resize(k + 1);
+        }
+        c->hist_t[k] = t;
+        c->hist_wall[k] = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
     return FI_OK;
 }
 
@@ -919,6 +933,33 @@ int fi_set_ice_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* H
     return FI_OK;
 }
 
+int fi_set_timer_history(fi_ctx* c, int64_t capacity) {
+    if (!c || capacity < 0) return fail(c, FI_ERR_INVALID, "fi_set_timer_history: bad args");
+    if (c->initialized) return fail(c, FI_ERR_STATE, "fi_set_timer_history: call it before fi_init");
+    c->hist_cap = capacity;
+    return FI_OK;
+}
+
+int fi_get_timer_history(fi_ctx* c, double* t, double* seconds, int64_t capacity, int64_t* n_total) {
+    if (!c || !n_total || capacity < 0 || (capacity > 0 && (!t || !seconds)))
+        return fail(c, FI_ERR_INVALID, "fi_get_timer_history: bad args");
+    // t_computation! keeps the calls past t_span[0]; run! subtracts the first one's time
+    const size_t m = std::min(c->hist_t.size(), (size_t)std::max<long long>(c->nrhs, 0));
+    int64_t j = 0;
+    double w0 = 0.0;
+    for (size_t k = 0; k < m; ++k) {
+        if (!(c->hist_t[k] > c->t0)) continue;
+        if (j == 0) w0 = c->hist_wall[k];
+        if (j < capacity) {
+            t[j] = c->hist_t[k];
+            seconds[j] = c->hist_wall[k] - w0;
+        }
+        ++j;
+    }
+    *n_total = j;
+    return FI_OK;
+}
+
 int fi_set_sea_surface_snapshots(fi_ctx* c, const double* t, int32_t nt, const double* X, int32_t flat) {
     if (!c || !t || !X || nt < 1) return fail(c, FI_ERR_INVALID, "fi_set_sea_surface_snapshots: bad args");
     for (int k = 1; k < nt; ++k)
@@ -989,6 +1030,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
         return fail(c, FI_ERR_INVALID, "sea-surface Green kernel required");
     if (g.sea_surface == FI_IMPOSED_SEA_SURFACE && c->ss_t.empty())
         return fail(c, FI_ERR_INVALID, "ImposedSeaSurface needs dz_ss snapshots (fi_set_sea_surface_snapshots)");
+    c->hist_t.clear();
+    c->hist_wall.clear();
     if (g.sealevel_load == FI_INTERACTIVE_SEALEVEL_LOAD) {
         double s = 0;
         for (double m : c->maskactive) s += m;

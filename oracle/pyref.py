@@ -64,6 +64,8 @@ def lib():
             "fi_get_field": (C.c_int, [P, C.c_int32, _dp, C.c_size_t]),
             "fi_set_ice_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
             "fi_set_sea_surface_snapshots": (C.c_int, [P, _dp, C.c_int32, _dp, C.c_int32]),
+            "fi_set_timer_history": (C.c_int, [P, C.c_int64]),
+            "fi_get_timer_history": (C.c_int, [P, _dp, _dp, C.c_int64, C.POINTER(C.c_int64)]),
             "fi_set_ice_separable": (C.c_int, [P, _dp, _dp, C.c_int32, _dp, C.c_int32]),
             "fi_set_bsl": (C.c_int, [P, C.c_int32, C.c_int32, C.c_double, _dp, _dp, C.c_int32, C.c_int32]),
             "fi_set_output_times": (C.c_int, [P, _dp, C.c_int32]),
@@ -154,6 +156,8 @@ class RefEngine:
             t = np.asarray(inputs["sea_surface"][0], dtype=np.float64)
             X = np.concatenate([_flat(x) for x in inputs["sea_surface"][1]])
             e._ck(e.L.fi_set_sea_surface_snapshots(e.h, _p(t), len(t), _p(X), int(inputs["sea_surface"][2])))
+        if handle is None and inputs.get("timer_capacity"):
+            e._ck(e.L.fi_set_timer_history(e.h, int(inputs["timer_capacity"])))
         if "bsl" in inputs:
             e.set_bsl(*inputs["bsl"])
         tout = np.asarray(inputs["tout"], dtype=np.float64)
@@ -180,6 +184,14 @@ class RefEngine:
 
     def close(self):
         self.__del__()
+
+    def timer_history(self, capacity):
+        t = np.empty(max(int(capacity), 1))
+        sec = np.empty_like(t)
+        n = C.c_int64(0)
+        self._ck(self.L.fi_get_timer_history(self.h, _p(t), _p(sec), int(capacity), C.byref(n)))
+        m = min(n.value, int(capacity))
+        return t[:m].copy(), sec[:m].copy(), n.value
 
     def _ck(self, st):
         if st != 0:
