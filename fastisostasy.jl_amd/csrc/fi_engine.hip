@@ -2271,6 +2271,13 @@ struct fi_batch {
     DBuf<KP> kp_all;
     DBuf<Ctl> ctl_all;
     std::vector<Ctl> ctl_h;
+    // An ensemble that mixes member kinds is a composite: one sub-ensemble per ensemble kernel
+    // (LV-ELVA, ELRA), with the members' indices in the composite (gid), plus the members no
+    // ensemble kernel runs (the implicit step), advanced one after another by fi_integrate.
+    // mem then lists every member in the caller's order; the subs and solo contexts own them.
+    std::vector<fi_batch*> subs;
+    std::vector<int> solo;   // composite indices of the single-context members
+    std::vector<int> gid;    // a sub-ensemble's member index -> the composite's
 };
 
 namespace {
@@ -2278,6 +2285,11 @@ namespace {
 int bfail(fi_batch* b, int code, const std::string& m) {
     if (b) b->err = m;
     return code;
+}
+
+// a member as the caller numbers it (the composite's index for a sub-ensemble's member)
+std::string mname(const fi_batch* b, int m) {
+    return "member " + std::to_string(b->gid.empty() ? m : b->gid[m]);
 }
 
 #define BHIPCHECK(b, x)                                                                        \
@@ -2344,7 +2356,7 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, bool mt_double, size_t lds
         fi_ctx* c = b->mem[m];
         // the integration loop writes u_{n+1} into U[1 - cur]: cur must be 0 or 1
         if (c->ctl.cur != 0 && c->ctl.cur != 1)
-            return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": state buffer index out of range");
+            return bfail(b, FI_ERR_STATE, mname(b, m) + ": state buffer index out of range");
         if (c->ctl.t < t_target[m]) cap_work += (double)max_steps * (double)c->N;
         c->ctl.stop = t_target[m];
         c->ctl.next_out = t_target[m];
@@ -2414,12 +2426,12 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, bool mt_double, size_t lds
         if (hb[(size_t)t * (1 + 2 * (size_t)ts)].abort) return bfail(b, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
     for (int m = 0; m < nm; ++m) {
         const int e = b->mem[m]->ctl.err;
-        if (e == ERR_TIME_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation2D out of range.");
-        if (e == ERR_BSL_RANGE) return bfail(b, FI_ERR_TIME_RANGE, "member " + std::to_string(m) + ": TimeInterpolation0D out of range.");
-        if (e == ERR_DT_MIN) return bfail(b, FI_ERR_DT_MIN, "member " + std::to_string(m) + ": dt below minimum");
+        if (e == ERR_TIME_RANGE) return bfail(b, FI_ERR_TIME_RANGE, mname(b, m) + ": TimeInterpolation2D out of range.");
+        if (e == ERR_BSL_RANGE) return bfail(b, FI_ERR_TIME_RANGE, mname(b, m) + ": TimeInterpolation0D out of range.");
+        if (e == ERR_DT_MIN) return bfail(b, FI_ERR_DT_MIN, mname(b, m) + ": dt below minimum");
         if (e == ERR_NONFINITE)
-            return bfail(b, FI_ERR_NONFINITE, "member " + std::to_string(m) + ": NaN in the state or its derivative");
-        if (e) return bfail(b, FI_ERR_DEVICE, "member " + std::to_string(m) + ": device error");
+            return bfail(b, FI_ERR_NONFINITE, mname(b, m) + ": NaN in the state or its derivative");
+        if (e) return bfail(b, FI_ERR_DEVICE, mname(b, m) + ": device error");
     }
     return FI_OK;
 }
@@ -2431,21 +2443,53 @@ extern "C" {
 int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out) {
     if (!cfgs || !out || nmem < 1) return FI_ERR_INVALID;
     *out = nullptr;
+    // member kinds: 0 LV-ELVA (fi_batch_persistent), 1 ELRA (fi_batch_persistent_elra), 2 the
+    // implicit step (no ensemble kernel: advanced by fi_integrate)
+    std::vector<int> kind(nmem);
+    for (int m = 0; m < nmem; ++m) {
+        if (cfgs[m].device != cfgs[0].device) return FI_ERR_INVALID;  // one device per ensemble
+        const fi_config& g = cfgs[m];
+        kind[m] = g.mantle == FI_RELAXED_MANTLE ? 1 : g.lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE ? 0 : 2;
+    }
+    const bool mixed = std::any_of(kind.begin(), kind.end(), [&](int k) { return k != kind[0]; }) || kind[0] == 2;
     fi_batch* b = new fi_batch();
     b->dev = cfgs[0].device;
+    if (mixed) {
+        b->mem.assign(nmem, nullptr);
+        for (int k = 0; k < 2; ++k) {
+            std::vector<fi_config> sub_cfg;
+            std::vector<int> idx;
+            for (int m = 0; m < nmem; ++m)
+                if (kind[m] == k) {
+                    sub_cfg.push_back(cfgs[m]);
+                    idx.push_back(m);
+                }
+            if (idx.empty()) continue;
+            fi_batch* sb = nullptr;
+            const int st = fi_create_batch(sub_cfg.data(), (int32_t)idx.size(), &sb);
+            if (st) {
+                fi_batch_destroy(b);
+                return st;
+            }
+            sb->gid = idx;
+            for (size_t q = 0; q < idx.size(); ++q) b->mem[idx[q]] = sb->mem[q];
+            b->subs.push_back(sb);
+        }
+        for (int m = 0; m < nmem; ++m)
+            if (kind[m] == 2) {
+                fi_ctx* c = nullptr;
+                const int st = fi_create(&cfgs[m], &c);
+                if (st) {
+                    fi_batch_destroy(b);
+                    return st;
+                }
+                b->mem[m] = c;
+                b->solo.push_back(m);
+            }
+        *out = b;
+        return FI_OK;
+    }
     for (int m = 0; m < nmem; ++m) {
-        if (cfgs[m].device != b->dev) {
-            fi_batch_destroy(b);
-            return FI_ERR_INVALID;  // one device per ensemble
-        }
-        // one kernel per ensemble: every member LV-ELVA, or every member ELRA
-        const bool lv = cfgs[m].mantle == FI_MAXWELL_MANTLE && cfgs[m].lithosphere == FI_LATERALLY_VARIABLE_LITHOSPHERE;
-        const bool elra = cfgs[m].mantle == FI_RELAXED_MANTLE;
-        const bool elra0 = cfgs[0].mantle == FI_RELAXED_MANTLE;
-        if (!(lv || elra) || elra != elra0) {
-            fi_batch_destroy(b);
-            return FI_ERR_UNSUPPORTED;
-        }
         fi_ctx* c = nullptr;
         const int st = fi_create(&cfgs[m], &c);
         if (st) {
@@ -2474,6 +2518,11 @@ void fi_batch_destroy(fi_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->dev);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (!b->subs.empty() || !b->solo.empty()) {   // a composite owns its subs and solo members only
+        for (fi_batch* sb : b->subs) fi_batch_destroy(sb);
+        for (int m : b->solo) fi_destroy(b->mem[m]);
+        b->mem.clear();
+    }
     for (fi_ctx* c : b->mem) fi_destroy(c);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -2486,6 +2535,10 @@ const char* fi_batch_last_error(const fi_batch* b) { return b ? b->err.c_str() :
 
 int fi_batch_set_team_size(fi_batch* b, int32_t ts) {
     if (!b || ts < 0) return FI_ERR_INVALID;
+    for (fi_batch* sb : b->subs) {
+        const int st = fi_batch_set_team_size(sb, ts);
+        if (st) return st;
+    }
     b->tsize_fixed = ts;
     b->tsize_best = 0;
     b->cand_next = 0;
@@ -2573,8 +2626,48 @@ int batch_scratch_static(fi_batch* b, unsigned char* base, int nm, int gmax, Bat
     return off <= NOALLOC_SCRATCH_BYTES ? FI_OK : bfail(b, FI_ERR_INVALID, "static scratch layout overflow");
 }
 
+int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noalloc);
+
+// a composite ensemble: each sub-ensemble in one launch sequence of its own kernel, then the
+// single-context members one after another, all on this device and in this order; every member's
+// state is checked first so that no part runs when another would be refused
+int composite_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noalloc) {
+    const int nm = (int)b->mem.size();
+    for (int m = 0; m < nm; ++m) {
+        const fi_ctx* c = b->mem[m];
+        if (!c->integ) return bfail(b, FI_ERR_STATE, mname(b, m) + ": fi_batch_integrate before fi_integrator_init");
+        if (t_targets[m] < c->ctl.t) return bfail(b, FI_ERR_INVALID, mname(b, m) + ": target before current time");
+    }
+    for (fi_batch* sb : b->subs) {
+        std::vector<double> tt;
+        for (int g : sb->gid) tt.push_back(t_targets[g]);
+        const int st = batch_integrate(sb, tt.data(), fsal, noalloc);
+        if (st) return bfail(b, st, sb->err);
+    }
+    for (int g : b->solo) {
+        fi_ctx* c = b->mem[g];
+        if (!(c->ctl.t < t_targets[g])) continue;
+        // as the ensemble kernels treat their members: the member's own output times are not
+        // used, and with fsal the FSAL derivative is re-evaluated at the target, as after an
+        // output callback (under the implicit step that call advances u, as it does there)
+        std::vector<double> tout;
+        tout.swap(c->tout);
+        const long long kout = c->kout;
+        int st = fi_integrate(c, t_targets[g], nullptr, nullptr);
+        c->tout.swap(tout);
+        c->kout = kout;
+        if (!st && fsal) {
+            st = rhs_on_device(c, c->ctl.t, c->ctl.cur, c->ctl.k1);
+            c->ctl.last_u = c->ctl.cur;
+        }
+        if (st) return bfail(b, st, mname(b, g) + ": " + c->err);
+    }
+    return FI_OK;
+}
+
 int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noalloc) {
     if (!b || !t_targets) return FI_ERR_INVALID;
+    if (!b->subs.empty() || !b->solo.empty()) return composite_integrate(b, t_targets, fsal, noalloc);
     const int nm = (int)b->mem.size();
     if (noalloc && nm > NOALLOC_MAX_MEMBERS)
         return bfail(b, FI_ERR_INVALID, "fi_batch_integrate_each_noalloc: " + std::to_string(nm) +
@@ -2585,12 +2678,12 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noa
     size_t lds_d = 0, lds_dd = 0, lds_f = 0;
     for (int m = 0; m < nm; ++m) {
         fi_ctx* c = b->mem[m];
-        if (!c->integ) return bfail(b, FI_ERR_STATE, "member " + std::to_string(m) + ": fi_batch_integrate before fi_integrator_init");
-        if (t_target[m] < c->ctl.t) return bfail(b, FI_ERR_INVALID, "member " + std::to_string(m) + ": target before current time");
+        if (!c->integ) return bfail(b, FI_ERR_STATE, mname(b, m) + ": fi_batch_integrate before fi_integrator_init");
+        if (t_target[m] < c->ctl.t) return bfail(b, FI_ERR_INVALID, mname(b, m) + ": target before current time");
         if (c->k1_stale) {
             // state written since the last step: re-evaluate the FSAL derivative as fi_integrate does
             const int st = rhs_on_device(c, c->ctl.t, c->ctl.cur, c->ctl.k1);
-            if (st) return bfail(b, st, "member " + std::to_string(m) + ": " + c->err);
+            if (st) return bfail(b, st, mname(b, m) + ": " + c->err);
             c->ctl.last_u = c->ctl.cur;
             c->k1_stale = false;
         }
@@ -2705,6 +2798,23 @@ int fi_batch_integrate_each_noalloc(fi_batch* b, const double* t_targets, int32_
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {
     if (!b || !o) return FI_ERR_INVALID;
     std::memset(o, 0, sizeof(*o));
+    if (!b->subs.empty() || !b->solo.empty()) {
+        // a composite: the team sizes of its largest sub-ensemble, launches and device time
+        // summed over the sub-ensembles (the single-context members are not counted)
+        const fi_batch* big = nullptr;
+        for (const fi_batch* sb : b->subs)
+            if (!big || sb->mem.size() > big->mem.size()) big = sb;
+        if (big) fi_batch_get_info(big, o);
+        o->device_ms = o->n_launch = 0;
+        o->cell_attempts = 0;
+        for (const fi_batch* sb : b->subs) {
+            o->device_ms += sb->device_ms;
+            o->n_launch += sb->n_launch;
+            o->cell_attempts += sb->cell_attempts;
+        }
+        o->n_members = (int32_t)b->mem.size();
+        return FI_OK;
+    }
     o->n_members = (int32_t)b->mem.size();
     o->tuned = (b->tsize_fixed > 0 || b->tsize_best > 0) ? 1 : 0;
     // the size in use once tuned (a given size is capped by the co-resident workgroups at launch)

@@ -379,7 +379,12 @@ typedef struct fi_batch_info {
 
 /* Creates nmem member contexts, member m as fi_create(&cfgs[m]). Members are set up, initialised
  * and read through fi_batch_member(m) with the single-context calls (fi_set_field, fi_set_ice_*,
- * fi_init, fi_integrator_init, fi_get_field, fi_get_stats); they are owned by the batch. */
+ * fi_init, fi_integrator_init, fi_get_field, fi_get_stats); they are owned by the batch. Any
+ * mix of member kinds is accepted, as the reference's ensemble loop runs any Simulation
+ * (ext/FastIsostasyEnsembleKalmanProcessesExt.jl:114-122): LV-ELVA members share one ensemble
+ * kernel, ELRA members another, and implicit-step members advance one after another through
+ * their own integration; fi_batch_get_info then reports the largest group's team sizes and the
+ * launches and device time of both ensemble kernels. */
 int fi_create_batch(const fi_config* cfgs, int32_t nmem, fi_batch** out);
 fi_ctx* fi_batch_member(fi_batch* b, int32_t m);
 void fi_batch_destroy(fi_batch* b);

@@ -1744,6 +1744,11 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal) {
 
 int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) {
     if (!b || !t_targets) return FI_ERR_INVALID;
+    for (size_t m = 0; m < b->mem.size(); ++m)
+        if (b->mem[m]->integ && t_targets[m] < b->mem[m]->t) {
+            b->err = "member " + std::to_string(m) + ": target before current time";
+            return FI_ERR_INVALID;
+        }
     for (size_t m = 0; m < b->mem.size(); ++m) {
         fi_ctx* c = b->mem[m];
         const double t_target = t_targets[m];

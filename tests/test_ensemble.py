@@ -225,14 +225,75 @@ def test_gpu_elra_ensemble_members_equal_single_contexts_bitwise(team_size):
         assert np.abs(a - b).max() <= 2e-3 * np.abs(b).max(), m
 
 
-@pytest.mark.gpu
-def test_gpu_ensemble_refuses_mixed_rheologies():
-    if DRYRUN:
-        pytest.skip("the oracle's ensemble runs any member")
+# ---- ensembles mixing member kinds ------------------------------------------------------------
+def implicit_member(n=32):
+    """The disc benchmark under a laterally constant lithosphere: the implicit step (Euler,
+    dt_min 20 yr), which no ensemble kernel runs."""
     import fastiso
-    mixed = [elra_ensemble(sizes=(32,))[0], small_ensemble(nmem=1)[0]]
-    with pytest.raises(fastiso.EngineError):
-        fastiso.Ensemble(mixed)
+    w = workloads.disc_benchmark(n=n, t_end=1000.0, t_out=[])
+    d = w["domain"]
+    w["solidearth"] = fastiso.SolidEarth(d, lithosphere=fastiso.LaterallyConstantLithosphere(),
+                                         layer_boundaries=[88e3], layer_viscosities=[1e21], rho_uppermantle=3300.0,
+                                         rho_litho=0.0)
+    w["opts"] = fastiso.SolverOptions(diffeq=fastiso.DiffEqOptions(alg="Euler", dt_min=20.0))
+    return sim_inputs(w).inputs
+
+
+def short_elra_member(n=36, t_end=150.0):
+    """An ELRA member whose ice forcing ends at t_end (no flat extension)."""
+    w = workloads.elra(n=n, W=3e6, t_end=t_end, tau=2500.0, lithosphere="constant", t_out=[])
+    return sim_inputs(w).inputs
+
+
+def mixed_ensemble():
+    """LV-ELVA, ELRA and implicit-step members interleaved, so that the composite's member indices
+    differ from its sub-ensembles' own."""
+    lv = small_ensemble(nmem=2)
+    el = elra_ensemble(sizes=(32, 40))
+    return [lv[0], el[0], implicit_member(), lv[1], el[1], short_elra_member()]
+
+
+def test_oracle_mixed_ensemble_equals_single_contexts():
+    from oracle import pyref
+    inputs = mixed_ensemble()
+    ens = pyref.RefEnsemble(inputs)
+    ens.integrator_init()
+    for t in T_OUT:
+        ens.integrate(t)
+    for m, inp in enumerate(inputs):
+        e = pyref.RefEngine.from_inputs(with_tout(inp, T_OUT))
+        e.integrator_init()
+        e.integrate(T_OUT[-1], lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m}")
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_ensemble_members_equal_single_contexts_bitwise():
+    """An ensemble of LV-ELVA, ELRA and implicit-step members: one sub-ensemble per ensemble
+    kernel plus the implicit member through its own integration. Every member equals its
+    single-context run bit for bit; errors name the caller's member index, from the composite's
+    own checks and from a sub-ensemble's device error (the ELRA member whose ice ends at 150 yr)."""
+    import fastiso
+    inputs = mixed_ensemble()
+    ens = hip_ensemble(inputs, 0)
+    ens.integrator_init()
+    for t in T_OUT:
+        ens.integrate(t)
+    for m, inp in enumerate(inputs):
+        e = hip_single(with_tout(inp, T_OUT))
+        e.integrator_init()
+        e.integrate(T_OUT[-1], lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m}")
+        assert ens[m].stats()["t"] == T_OUT[-1]
+    assert np.abs(ens[2].get_field("u")).max() > 0
+    if not DRYRUN:
+        assert ens.info()["n_members"] == len(inputs)
+    err = RuntimeError if DRYRUN else fastiso.EngineError
+    t = T_OUT[-1]
+    with pytest.raises(err, match="member 2"):
+        ens.integrate_each([t, t, t - 1.0, t, t, t])
+    with pytest.raises(err, match="member 5.*TimeInterpolation2D"):
+        ens.integrate_each([t, t, t, t, t, 400.0])
 
 
 @pytest.mark.gpu
