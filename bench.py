@@ -120,6 +120,12 @@ def bench_ensemble(a):
     ens.integrator_init()
     setup_s = time.perf_counter() - t0
     print(f"ensemble of {len(inputs)} members set up in {setup_s:.1f} s", file=sys.stderr, flush=True)
+    tune_s = None
+    if a.autotune == "all":
+        t1 = time.perf_counter()
+        ens.autotune_fft(attempts=20)
+        tune_s = time.perf_counter() - t1
+        print(f"FFT radix orders tuned in {tune_s:.1f} s", file=sys.stderr, flush=True)
     cells = [inp["cfg"].nx * inp["cfg"].ny for inp in inputs]
     t = 0.0
     for k in range(a.warmup):
@@ -166,7 +172,8 @@ def bench_ensemble(a):
                    "window_yr": a.window, "integrator": "BS3 reltol 1e-5 abstol 1e-6",
                    "accepted_steps_total": int(sum(acc)), "rhs_calls_total": int(sum(rhs)), "cell_steps": cell_steps,
                    "team_size": i1["team_size"], "n_teams": i1["n_teams"], "autotuned": a.team_size == 0,
-                   "kernel_launches": int(i1["n_launch"] - i0["n_launch"]), "setup_s": setup_s},
+                   "kernel_launches": int(i1["n_launch"] - i0["n_launch"]), "setup_s": setup_s,
+                   "fft_plans_tuned_s": tune_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      **(traffic_fields(TRAFFIC_ENS, "bytes_per_cell_step", cell_steps,
@@ -523,6 +530,8 @@ def main():
     ap.add_argument("--rkc-dt", type=float, default=250.0, help="largest RKC sub-step of the predictor (yr)")
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--no-surrogate", action="store_true", help="default workload: skip the surrogate leg")
+    ap.add_argument("--autotune", choices=["none", "workgroups", "all"], default="all",
+                    help="tune the grid size and FFT radix orders (ensembles: the radix orders) before timing")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--inv-n", type=int, default=256, help="grid side of the inversion workload")
@@ -581,6 +590,17 @@ def run_plain(a):
     for _ in range(a.warmup):
         t += a.window
         eng.integrate(t)
+    tune = None
+    if a.autotune != "none":
+        # timed on this run's own state after the warm-up (restored afterwards)
+        what = _abi.FI_TUNE_WORKGROUPS | (_abi.FI_TUNE_FFT_PLANS if a.autotune == "all" else 0)
+        info = eng.autotune(what, attempts=60)
+        tune = {"what": a.autotune, "workgroups": info["workgroups"],
+                "wg_candidates": info["wg_candidates"], "wg_us_per_attempt": info["wg_us"],
+                "plans": [_abi.plan_radices(c) for c in info["plans"]],
+                "plan_default_us_per_attempt": info["plan_default_us"],
+                "plan_best_us_per_attempt": info["plan_best_us"]}
+        print(f"autotune: {tune}", file=sys.stderr, flush=True)
     start = (eng.stats()["t"], eng.get_field("u"))
     s0 = eng.stats()
     t0 = time.perf_counter()
@@ -633,6 +653,8 @@ def run_plain(a):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, **tr},
     }
+    if tune is not None:
+        out["config"]["autotune"] = tune
     out["config"]["engine"] = engine_name
     out["config"]["engine_library"] = os.path.relpath(engine_lib, ROOT) if engine_lib else None
     sim.close()

@@ -572,6 +572,11 @@ struct DBuf {
     T* p = nullptr;
     size_t n = 0;
     ~DBuf() { if (p) (void)hipFree(p); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
     hipError_t alloc(size_t cnt) {
         if (p) { (void)hipFree(p); p = nullptr; }
         n = cnt;
@@ -663,7 +668,7 @@ struct fi_ctx {
     DBuf<unsigned long long> hist_clk;
     KP kp;
     Ctl ctl;
-    int G = 0;
+    int G = 0, maxg = 0;   // persistent grid size in use; co-resident maximum (bar_d holds maxg)
     size_t lds = 0, lds_d = 0, lds_f = 0;   // LDS bytes of this context's launch; with FP64 / FP32 moments
     size_t lds_dist = 0;                     // FP64 moments with per-cell grid steps
     bool mt_double = true;
@@ -1529,6 +1534,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     int maxg = per_cu * ncu;
     int want = std::max(kp.nrowitems, c->elra ? (c->nfhx + 1) / 2 : c->implicit ? c->nhx : kp.ncolitems);
     c->G = std::max(1, std::min(maxg, want));
+    c->maxg = maxg;
     if (maxg < 1) return fail(c, FI_ERR_UNSUPPORTED, "persistent kernel does not fit on the device");
     // one error-norm and one volume partial per row pair, whatever the number of workgroups
     HIPCHECK(c, c->part.alloc(4 * (size_t)kp.nrowitems));
@@ -1536,7 +1542,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.part = c->part.p;
     kp.vpart = c->vpart.p;
     HIPCHECK(c, c->ctl_d.alloc(1));
-    HIPCHECK(c, c->bar_d.alloc(1 + 2 * (size_t)c->G));  // words + 128-byte arrival and release slots per workgroup
+    // words + 128-byte arrival and release slots per workgroup, for any grid size up to maxg
+    HIPCHECK(c, c->bar_d.alloc(1 + 2 * (size_t)maxg));
     HIPCHECK(c, c->kp_d.alloc(1));
     kp.ctl = c->ctl_d.p;
     kp.bar = c->bar_d.p;
@@ -2239,6 +2246,205 @@ int fi_get_stats(fi_ctx* c, fi_stats* o) {
 
 }  // extern "C"
 
+namespace {
+
+// ---- autotuner: persistent grid size and FFT radix orders, timed on the context's own run ----
+
+// every distinct radix order of length L (powers of two grouped into radices 8, 4, 2 in every
+// way, fewest passes first, and each multiset in every order), the default plan first, at most cap
+std::vector<FftPlan> plan_candidates(int L, int twoff, size_t cap) {
+    std::vector<FftPlan> out;
+    FftPlan def;
+    if (!plan_fft(L, twoff, def)) return out;
+    out.push_back(def);
+    int m = L, a2 = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++a2;
+    }
+    std::vector<int> odd;
+    for (int r : {3, 5, 7})
+        while (m % r == 0) {
+            odd.push_back(r);
+            m /= r;
+        }
+    for (int n8 = a2 / 3; n8 >= 0; --n8)
+        for (int n4 = (a2 - 3 * n8) / 2; n4 >= 0; --n4) {
+            std::vector<int> f(n8, 8);
+            f.insert(f.end(), n4, 4);
+            f.insert(f.end(), a2 - 3 * n8 - 2 * n4, 2);
+            f.insert(f.end(), odd.begin(), odd.end());
+            if (f.size() > 16) continue;
+            std::sort(f.begin(), f.end());
+            do {
+                FftPlan pl;
+                std::memset(&pl, 0, sizeof(pl));
+                pl.L = L;
+                pl.tw = twoff;
+                pl.nr = (int)f.size();
+                for (size_t q = 0; q < f.size(); ++q) pl.code |= (unsigned long long)f[q] << (4 * q);
+                if (pl.code != def.code) out.push_back(pl);
+                if (out.size() >= cap) return out;
+            } while (std::next_permutation(f.begin(), f.end()));
+        }
+    return out;
+}
+
+bool plan_valid(int L, unsigned long long code, int& nr) {
+    long long prod = 1;
+    nr = 0;
+    for (int q = 0; q < 16; ++q) {
+        const int r = (int)((code >> (4 * q)) & 15ull);
+        if (r == 0) break;
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8) return false;
+        prod *= r;
+        nr = q + 1;
+    }
+    return nr > 0 && (code >> (4 * nr)) == 0 && prod == L;
+}
+
+// µs per step attempt of up to `attempts` integration attempts from the checkpointed state, the
+// fastest of `reps` runs; the state is restored after each run
+int tune_time(fi_ctx* c, const std::vector<unsigned char>& ck, int attempts, int reps, double* us) {
+    *us = INFINITY;
+    for (int r = 0; r < reps; ++r) {
+        int st = fi_checkpoint_load(c, ck.data(), ck.size());
+        if (st) return st;
+        if (c->k1_stale) {
+            if ((st = rhs_on_device(c, c->ctl.t, c->ctl.cur, c->ctl.k1))) return st;
+            c->ctl.last_u = c->ctl.cur;
+            c->k1_stale = false;
+        }
+        c->ctl.stop = c->t_end;
+        c->ctl.next_out = INFINITY;
+        const long long a0 = c->ctl.nacc + c->ctl.nrej;
+        const double ms0 = c->device_ms;
+        if ((st = launch(c, MODE_INTEGRATE, 0, 0, attempts, 0))) return st;
+        const long long done = c->ctl.nacc + c->ctl.nrej - a0;
+        if (done > 0) *us = std::min(*us, (c->device_ms - ms0) * 1e3 / (double)done);
+    }
+    return fi_checkpoint_load(c, ck.data(), ck.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_set_fft_plans(fi_ctx* c, const uint64_t plans[4]) {
+    if (!c || !plans) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_fft_plans before fi_init");
+    FftPlan* pl[4] = {&c->kp.px, &c->kp.py, &c->kp.pfx, &c->kp.pfy};
+    for (int q = 0; q < 4; ++q) {
+        int nr = 0;
+        if (plans[q] && !plan_valid(pl[q]->L, plans[q], nr))
+            return fail(c, FI_ERR_INVALID, "fi_set_fft_plans: radices (2, 3, 4, 5, 7, 8) must multiply to the length");
+    }
+    for (int q = 0; q < 4; ++q) {
+        int nr = 0;
+        if (plans[q] && plan_valid(pl[q]->L, plans[q], nr)) {
+            pl[q]->code = plans[q];
+            pl[q]->nr = nr;
+        }
+    }
+    c->rkc_kp.reset();   // rebuilt with the new plans by the next fi_integrate_rkc
+    return FI_OK;
+}
+
+int fi_get_fft_plans(fi_ctx* c, uint64_t plans[4]) {
+    if (!c || !plans) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_get_fft_plans before fi_init");
+    plans[0] = c->kp.px.code;
+    plans[1] = c->kp.py.code;
+    plans[2] = c->kp.pfx.code;
+    plans[3] = c->kp.pfy.code;
+    return FI_OK;
+}
+
+int fi_set_workgroups(fi_ctx* c, int32_t g) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_workgroups before fi_init");
+    if (g < 1 || g > c->maxg) return fail(c, FI_ERR_INVALID, "fi_set_workgroups: 1 <= g <= the co-resident maximum");
+    c->G = g;
+    return FI_OK;
+}
+
+int fi_autotune(fi_ctx* c, int32_t what, int32_t attempts, fi_tune_info* info) {
+    if (!c || !info || attempts < 1 || (what & ~(FI_TUNE_WORKGROUPS | FI_TUNE_FFT_PLANS)))
+        return FI_ERR_INVALID;
+    std::memset(info, 0, sizeof(*info));
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_autotune before fi_integrator_init");
+    if (!(c->ctl.t < c->t_end)) return fail(c, FI_ERR_STATE, "fi_autotune: nothing left to integrate before t_end");
+    (void)hipSetDevice(c->dev);
+    size_t bytes = 0;
+    int st = fi_checkpoint_size(c, &bytes);
+    if (st) return st;
+    std::vector<unsigned char> ck(bytes);
+    if ((st = fi_checkpoint_save(c, ck.data(), bytes))) return st;
+    const double dms = c->device_ms;
+    const long long nl = c->n_launch, nat = c->n_accept_timed;
+    const int reps = 2;
+    double us = 0;
+    if (what & FI_TUNE_WORKGROUPS) {
+        const int nrow = c->kp.nrowitems, ncol = c->kp.ncolitems;
+        std::vector<int> cand = {c->G, nrow, ncol, (c->G + 1) / 2, (3 * c->G + 3) / 4, c->maxg};
+        std::vector<int> uniq;
+        for (int g : cand)
+            if (g >= 1 && g <= c->maxg && std::find(uniq.begin(), uniq.end(), g) == uniq.end()) uniq.push_back(g);
+        const int g0 = c->G;
+        int best = g0;
+        double best_us = INFINITY;
+        for (size_t q = 0; q < uniq.size() && q < 8; ++q) {
+            c->G = uniq[q];
+            if ((st = tune_time(c, ck, attempts, reps, &us))) {
+                c->G = g0;
+                break;
+            }
+            info->wg_candidates[q] = uniq[q];
+            info->wg_us[q] = us;
+            info->n_wg_candidates = (int32_t)q + 1;
+            if (us < best_us) {
+                best_us = us;
+                best = uniq[q];
+            }
+        }
+        c->G = st ? g0 : best;
+    }
+    // the transforms of the hot path: nx, ny for the Maxwell mantle, the padded lengths under ELRA
+    const int first = c->elra ? 2 : 0;
+    FftPlan* pl[4] = {&c->kp.px, &c->kp.py, &c->kp.pfx, &c->kp.pfy};
+    if (!st && (what & FI_TUNE_FFT_PLANS)) {
+        for (int q = first; q < first + 2 && !st; ++q) {
+            const FftPlan def = *pl[q];
+            const std::vector<FftPlan> cand = plan_candidates(def.L, def.tw, 24);
+            FftPlan best = def;
+            double best_us = INFINITY;
+            for (size_t k = 0; k < cand.size(); ++k) {
+                *pl[q] = cand[k];
+                if ((st = tune_time(c, ck, attempts, reps, &us))) break;
+                if (k == 0) info->plan_default_us[q] = us;
+                if (us < best_us) {
+                    best_us = us;
+                    best = cand[k];
+                }
+            }
+            *pl[q] = st ? def : best;
+            info->n_plan_candidates[q] = (int32_t)cand.size();
+            info->plan_best_us[q] = best_us;
+        }
+        c->rkc_kp.reset();
+    }
+    info->workgroups = c->G;
+    for (int q = 0; q < 4; ++q) info->plans[q] = pl[q]->code;
+    // the run continues from the state it had; the tuning launches are not counted
+    const int st2 = fi_checkpoint_load(c, ck.data(), ck.size());
+    c->device_ms = dms;
+    c->n_launch = nl;
+    c->n_accept_timed = nat;
+    return st ? st : st2;
+}
+
+}  // extern "C"
+
 // ==========================================================================================
 // ensembles
 // ==========================================================================================
@@ -2793,6 +2999,26 @@ int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal) 
 
 int fi_batch_integrate_each_noalloc(fi_batch* b, const double* t_targets, int32_t fsal) {
     return batch_integrate(b, t_targets, fsal, true);
+}
+
+int fi_batch_autotune_fft(fi_batch* b, int32_t attempts) {
+    if (!b || attempts < 1) return FI_ERR_INVALID;
+    // one member of each kind and grid is tuned on its own context; the others take its plans
+    std::vector<char> done(b->mem.size(), 0);
+    for (size_t m = 0; m < b->mem.size(); ++m) {
+        if (done[m]) continue;
+        fi_ctx* c = b->mem[m];
+        fi_tune_info info;
+        int st = fi_autotune(c, FI_TUNE_FFT_PLANS, attempts, &info);
+        if (st) return bfail(b, st, mname(b, (int)m) + ": " + c->err);
+        for (size_t q = m; q < b->mem.size(); ++q) {
+            fi_ctx* d = b->mem[q];
+            if (d->nx != c->nx || d->ny != c->ny || d->elra != c->elra || d->implicit != c->implicit) continue;
+            if ((st = fi_set_fft_plans(d, info.plans))) return bfail(b, st, mname(b, (int)q) + ": " + d->err);
+            done[q] = 1;
+        }
+    }
+    return FI_OK;
 }
 
 int fi_batch_get_info(const fi_batch* b, fi_batch_info* o) {

@@ -98,6 +98,39 @@ class FiRkcInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+FI_TUNE_WORKGROUPS, FI_TUNE_FFT_PLANS = 1, 2
+
+
+class FiTuneInfo(C.Structure):
+    _fields_ = [("workgroups", C.c_int32), ("n_wg_candidates", C.c_int32), ("wg_candidates", C.c_int32 * 8),
+                ("wg_us", C.c_double * 8), ("plans", C.c_uint64 * 4), ("n_plan_candidates", C.c_int32 * 4),
+                ("plan_default_us", C.c_double * 4), ("plan_best_us", C.c_double * 4)]
+
+    def as_dict(self):
+        n = self.n_wg_candidates
+        return {"workgroups": self.workgroups, "wg_candidates": list(self.wg_candidates)[:n],
+                "wg_us": list(self.wg_us)[:n], "plans": list(self.plans),
+                "plan_radices": [plan_radices(c) for c in self.plans],
+                "n_plan_candidates": list(self.n_plan_candidates), "plan_default_us": list(self.plan_default_us),
+                "plan_best_us": list(self.plan_best_us)}
+
+
+def plan_radices(code):
+    """The radix of each pass of an FFT plan code (4 bits per pass, first pass lowest)."""
+    out = []
+    while code:
+        out.append(int(code & 15))
+        code >>= 4
+    return out
+
+
+def plan_code(radices):
+    code = 0
+    for s, r in enumerate(radices):
+        code |= int(r) << (4 * s)
+    return code
+
+
 OUTPUT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
 
 _dp = C.POINTER(C.c_double)
@@ -133,6 +166,11 @@ def declare(lib):
         "fi_get_stats": (C.c_int, [P, C.POINTER(FiStats)]),
         "fi_set_time": (C.c_int, [P, C.c_double]),
         "fi_integrate_rkc": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.POINTER(FiRkcInfo)]),
+        "fi_autotune": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(FiTuneInfo)]),
+        "fi_set_fft_plans": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+        "fi_get_fft_plans": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+        "fi_set_workgroups": (C.c_int, [P, C.c_int32]),
+        "fi_batch_autotune_fft": (C.c_int, [P, C.c_int32]),
         "fi_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
         "fi_checkpoint_save": (C.c_int, [P, C.c_void_p, C.c_size_t]),
         "fi_checkpoint_load": (C.c_int, [P, C.c_void_p, C.c_size_t]),

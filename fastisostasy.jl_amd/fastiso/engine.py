@@ -179,6 +179,25 @@ class Engine:
         self._check(self.lib.fi_integrate_rkc(self.h, float(t_target), float(dt_max), float(rho), C.byref(info)))
         return info.as_dict()
 
+    def autotune(self, what=_abi.FI_TUNE_WORKGROUPS | _abi.FI_TUNE_FFT_PLANS, attempts=40):
+        """fi_autotune: time the persistent grid size and the FFT radix orders on this run (the
+        state is restored) and keep the fastest; returns the fi_tune_info fields."""
+        info = _abi.FiTuneInfo()
+        self._check(self.lib.fi_autotune(self.h, int(what), int(attempts), C.byref(info)))
+        return info.as_dict()
+
+    def fft_plans(self):
+        a = (C.c_uint64 * 4)()
+        self._check(self.lib.fi_get_fft_plans(self.h, a))
+        return list(a)
+
+    def set_fft_plans(self, plans):
+        a = (C.c_uint64 * 4)(*[int(x) for x in plans])
+        self._check(self.lib.fi_set_fft_plans(self.h, a))
+
+    def set_workgroups(self, g):
+        self._check(self.lib.fi_set_workgroups(self.h, int(g)))
+
     def set_time(self, t):
         """Move the integrator time to t without integrating (after set_field("u", ...))."""
         self._check(self.lib.fi_set_time(self.h, float(t)))

@@ -49,6 +49,10 @@ class Ensemble:
         for e in self.members:
             e.integrator_init()
 
+    def autotune_fft(self, attempts=40):
+        """fi_batch_autotune_fft: FFT radix orders tuned on one member per kind and grid."""
+        self._check(self.lib.fi_batch_autotune_fft(self.h, int(attempts)))
+
     def integrate(self, t_target, fsal=True):
         """Advance every member to t_target; fsal re-evaluates each member's FSAL derivative
         there, as after an output callback."""

@@ -300,6 +300,38 @@ int fi_integrate(fi_ctx* ctx, double t_target, fi_output_cb cb, void* user);
 
 int fi_get_stats(fi_ctx* ctx, fi_stats* out);
 
+/* Autotuner. No counterpart in the reference, whose transforms are planned by FFTW/CUFFT
+ * (plan_fft / plan_rfft in GIATools, src/tools.jl:100-109). After fi_integrator_init, with the
+ * run not at t_end: for each candidate, `attempts` integration attempts (BS3 step attempts, or
+ * implicit steps) from the current state are timed on the device (fastest of 2) and the state
+ * restored, so the run then continues as if the call had not been made (the tuning launches
+ * are not counted in fi_stats).
+ *   FI_TUNE_WORKGROUPS: the persistent grid size among the row-pair and column-pair counts,
+ *     3/4 and 1/2 of the default and the co-resident maximum. Results do not depend on it:
+ *     every cross-workgroup sum runs over fixed per-row-pair partials.
+ *   FI_TUNE_FFT_PLANS: the radix order of the hot path's transforms (nx and ny; the padded
+ *     lengths under ELRA): radices 8, 4, 2 for the powers of two grouped in every way, and 3,
+ *     5, 7, in every distinct order (at most 24 per length, the default first). Another order
+ *     rounds differently: fi_set_fft_plans with the chosen codes reproduces a run bit for bit.
+ * Plan codes hold the radix of pass s in bits [4s, 4s+4), first pass lowest, zero after the last. */
+enum { FI_TUNE_WORKGROUPS = 1, FI_TUNE_FFT_PLANS = 2 };
+typedef struct fi_tune_info {
+    int32_t workgroups;          /* grid size in use after the call */
+    int32_t n_wg_candidates;
+    int32_t wg_candidates[8];
+    double wg_us[8];             /* us per attempt of each grid size */
+    uint64_t plans[4];           /* plan codes in use after the call: nx, ny, nfx, nfy */
+    int32_t n_plan_candidates[4];
+    double plan_default_us[4];   /* us per attempt with the default plan of that length */
+    double plan_best_us[4];      /* ... with the fastest */
+} fi_tune_info;
+int fi_autotune(fi_ctx* ctx, int32_t what, int32_t attempts, fi_tune_info* info);
+/* Pin plans (after fi_init; 0 keeps a length's plan) and read them back. */
+int fi_set_fft_plans(fi_ctx* ctx, const uint64_t plans[4]);
+int fi_get_fft_plans(fi_ctx* ctx, uint64_t plans[4]);
+/* Pin the persistent grid size (after fi_init; 1 <= g <= the co-resident maximum). */
+int fi_set_workgroups(fi_ctx* ctx, int32_t g);
+
 /* Timer history (Timer.t_vec and Timer.t_computation, src/simulation.jl:28-52, pushed by
  * t_computation! at the end of every update_diagnostics!, src/simulation.jl:391). With
  * fi_set_timer_history(capacity > 0) before fi_init, right-hand-side call k of the context
@@ -406,6 +438,11 @@ int fi_batch_integrate(fi_batch* b, double t_target, int32_t fsal);
 
 /* As fi_batch_integrate with one target time per member (t_targets[m] for member m). */
 int fi_batch_integrate_each(fi_batch* b, const double* t_targets, int32_t fsal);
+
+/* FI_TUNE_FFT_PLANS of fi_autotune for an ensemble: one member of each kind and grid is tuned
+ * on its own context (after fi_integrator_init), the members of the same kind and grid take its
+ * plans. */
+int fi_batch_autotune_fft(fi_batch* b, int32_t attempts);
 
 /* fi_batch_integrate_each without device allocation. No counterpart in the reference: it is for
  * callers that cannot allocate device memory while they run (tight serving loops, harnesses that

@@ -365,6 +365,7 @@ struct fi_ctx {
     int64_t count = 0;
     bool k1_stale = false;
     double rkc_rho = 0;   // spectral radius estimate of fi_integrate_rkc (1/yr)
+    uint64_t plans[4] = {0, 0, 0, 0};   // FFT plan codes pinned by fi_set_fft_plans (unused here)
     double t0 = 0, t_end = 0, t_timer = 0;
     bool initialized = false;
     // preallocated buffers (PreAllocated, src/tools.jl:4-16)
@@ -1487,6 +1488,77 @@ int fi_checkpoint_load(fi_ctx* c, const void* buf, size_t bytes) {
     size_t off = 0;
     return ck_walk(c, 2, (unsigned char*)const_cast<void*>(buf), off);
 }
+
+// The device engine's autotuner (persistent grid size, FFT radix orders) has nothing to tune in
+// this FP64 CPU restatement: fi_autotune leaves the run unchanged and reports the default plans;
+// pinned plan codes are validated and stored, and change nothing.
+namespace {
+uint64_t default_plan_code(int L) {
+    uint64_t code = 0;
+    int nr = 0, m = L;
+    for (int r : {8, 4, 2, 3, 5, 7})
+        while (m % r == 0 && nr < 16) {
+            code |= (uint64_t)r << (4 * nr++);
+            m /= r;
+        }
+    return m == 1 ? code : 0;
+}
+bool plan_code_ok(int L, uint64_t code) {
+    long long prod = 1;
+    int nr = 0;
+    for (; nr < 16; ++nr) {
+        const int r = (int)((code >> (4 * nr)) & 15u);
+        if (r == 0) break;
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8) return false;
+        prod *= r;
+    }
+    return nr > 0 && (nr == 16 || (code >> (4 * nr)) == 0) && prod == L;
+}
+void plan_lengths(const fi_ctx* c, int L[4]) {
+    L[0] = c->nx;
+    L[1] = c->ny;
+    L[2] = nextfastfft(2 * c->nx - 1);
+    L[3] = nextfastfft(2 * c->ny - 1);
+}
+}  // namespace
+
+int fi_set_fft_plans(fi_ctx* c, const uint64_t plans[4]) {
+    if (!c || !plans) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_fft_plans before fi_init");
+    int L[4];
+    plan_lengths(c, L);
+    for (int q = 0; q < 4; ++q)
+        if (plans[q] && !plan_code_ok(L[q], plans[q]))
+            return fail(c, FI_ERR_INVALID, "fi_set_fft_plans: radices (2, 3, 4, 5, 7, 8) must multiply to the length");
+    for (int q = 0; q < 4; ++q)
+        if (plans[q]) c->plans[q] = plans[q];
+    return FI_OK;
+}
+
+int fi_get_fft_plans(fi_ctx* c, uint64_t plans[4]) {
+    if (!c || !plans) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_get_fft_plans before fi_init");
+    int L[4];
+    plan_lengths(c, L);
+    for (int q = 0; q < 4; ++q) plans[q] = c->plans[q] ? c->plans[q] : default_plan_code(L[q]);
+    return FI_OK;
+}
+
+int fi_set_workgroups(fi_ctx* c, int32_t g) {
+    if (!c) return FI_ERR_INVALID;
+    if (!c->initialized) return fail(c, FI_ERR_STATE, "fi_set_workgroups before fi_init");
+    return g >= 1 ? FI_OK : fail(c, FI_ERR_INVALID, "fi_set_workgroups: g >= 1");
+}
+
+int fi_autotune(fi_ctx* c, int32_t what, int32_t attempts, fi_tune_info* info) {
+    if (!c || !info || attempts < 1 || (what & ~(FI_TUNE_WORKGROUPS | FI_TUNE_FFT_PLANS))) return FI_ERR_INVALID;
+    std::memset(info, 0, sizeof(*info));
+    if (!c->integ) return fail(c, FI_ERR_STATE, "fi_autotune before fi_integrator_init");
+    info->workgroups = 1;
+    return fi_get_fft_plans(c, info->plans);
+}
+
+int fi_batch_autotune_fft(fi_batch* b, int32_t attempts) { return (b && attempts >= 1) ? FI_OK : FI_ERR_INVALID; }
 
 int fi_get_stats(fi_ctx* c, fi_stats* o) {
     if (!c || !o) return FI_ERR_INVALID;

@@ -39,6 +39,12 @@ class RefRkcInfo(C.Structure):
                 ("n_rhs", C.c_int64)]
 
 
+class RefTuneInfo(C.Structure):
+    _fields_ = [("workgroups", C.c_int32), ("n_wg_candidates", C.c_int32), ("wg_candidates", C.c_int32 * 8),
+                ("wg_us", C.c_double * 8), ("plans", C.c_uint64 * 4), ("n_plan_candidates", C.c_int32 * 4),
+                ("plan_default_us", C.c_double * 4), ("plan_best_us", C.c_double * 4)]
+
+
 CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_int32)
 _dp = C.POINTER(C.c_double)
 _lib = None
@@ -76,6 +82,11 @@ def lib():
             "fi_integrate": (C.c_int, [P, C.c_double, CB, C.c_void_p]),
             "fi_get_stats": (C.c_int, [P, C.POINTER(RefStats)]),
             "fi_integrate_rkc": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.POINTER(RefRkcInfo)]),
+            "fi_autotune": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(RefTuneInfo)]),
+            "fi_set_fft_plans": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+            "fi_get_fft_plans": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+            "fi_set_workgroups": (C.c_int, [P, C.c_int32]),
+            "fi_batch_autotune_fft": (C.c_int, [P, C.c_int32]),
             "fi_set_time": (C.c_int, [P, C.c_double]),
             "fi_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_size_t)]),
             "fi_checkpoint_save": (C.c_int, [P, C.c_void_p, C.c_size_t]),
@@ -236,6 +247,22 @@ class RefEngine:
         info = RefRkcInfo()
         self._ck(self.L.fi_integrate_rkc(self.h, float(t_target), float(dt_max), float(rho), C.byref(info)))
         return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def autotune(self, what=3, attempts=40):
+        info = RefTuneInfo()
+        self._ck(self.L.fi_autotune(self.h, int(what), int(attempts), C.byref(info)))
+        return {"workgroups": info.workgroups, "plans": list(info.plans), "wg_candidates": [], "wg_us": []}
+
+    def fft_plans(self):
+        a = (C.c_uint64 * 4)()
+        self._ck(self.L.fi_get_fft_plans(self.h, a))
+        return list(a)
+
+    def set_fft_plans(self, plans):
+        self._ck(self.L.fi_set_fft_plans(self.h, (C.c_uint64 * 4)(*[int(x) for x in plans])))
+
+    def set_workgroups(self, g):
+        self._ck(self.L.fi_set_workgroups(self.h, int(g)))
 
     def checkpoint(self):
         n = C.c_size_t()

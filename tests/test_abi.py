@@ -77,7 +77,8 @@ def test_engine_loader_fails_loudly(tmp_path):
 # ---- the bindings against the header: field names, order and types, field ids, ABI version ----
 JULIA = os.path.join(ROOT, "fastisostasy.jl_amd", "julia", "FastIsostasyHIP.jl")
 C_TO_JULIA = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64"}
-C_TO_CTYPES = {"int32_t": ctypes.c_int32, "int64_t": ctypes.c_int64, "double": ctypes.c_double}
+C_TO_CTYPES = {"int32_t": ctypes.c_int32, "int64_t": ctypes.c_int64, "uint64_t": ctypes.c_uint64,
+               "double": ctypes.c_double}
 
 
 def header_text():
@@ -127,7 +128,8 @@ def test_julia_structs_match_header(cname, jname):
 
 
 @pytest.mark.parametrize("cname,cls", [("fi_config", "FiConfig"), ("fi_stats", "FiStats"),
-                                       ("fi_batch_info", "FiBatchInfo")])
+                                       ("fi_batch_info", "FiBatchInfo"), ("fi_rkc_info", "FiRkcInfo"),
+                                       ("fi_tune_info", "FiTuneInfo")])
 def test_ctypes_structs_match_header(cname, cls):
     fields = getattr(_abi, cls)._fields_
     h = header_struct(cname)
