@@ -507,11 +507,39 @@ __device__ __forceinline__ double kslot_off_dev(const float* kc, double vbc) {
     const double a = 0.25 * (double)kc[0], b = 0.25 * (double)kc[1], c = 0.25 * (double)kc[2], d = 0.25 * (double)kc[3];
     return ((a + b) + (c + d)) - vbc;
 }
+// The single-RHS launches of an RKC chain follow each other without host work: the kernel that
+// runs between two of them (a stage, or the sub-step's corner offset) prepares the next one in
+// its workgroup 0: it keeps the previous launch's barrier abort word in *abort_acc, clears the
+// barrier region (fresh generations for the next launch) and writes the next right-hand side's
+// time into the control block from the chain's time table (idx < 0: no next right-hand side).
+struct RkcPrep {
+    Bar* bar;          // null: nothing to prepare
+    int words;         // 32-bit words of the barrier region
+    int* abort_acc;
+    Ctl* ctl;
+    const double* times;
+    int idx;
+    int first;         // the chain's first kernel: clears *abort_acc
+};
+__device__ __forceinline__ void rkc_prep(const RkcPrep& q) {
+    if (!q.bar || blockIdx.x != 0) return;
+    if (threadIdx.x == 0) {
+        // the chain's first kernel follows launches of other calls: their abort words are theirs
+        *q.abort_acc = q.first ? 0 : (*q.abort_acc | q.bar->abort);
+        if (q.idx >= 0) q.ctl->t_rhs = q.times[q.idx];
+    }
+    __syncthreads();
+    unsigned* w = (unsigned*)q.bar;
+    for (int k = threadIdx.x; k < q.words; k += blockDim.x) w[k] = 0u;
+}
+__global__ void k_rkc_prep(RkcPrep q) { rkc_prep(q); }
+
 __global__ void __launch_bounds__(256) k_rkc_stage(double* y, const double* __restrict__ y0, const double* __restrict__ ym1,
                                                    const double* ym2, const float* __restrict__ f1,
                                                    const float* __restrict__ f0, const float* __restrict__ kc1,
                                                    const float* __restrict__ kc0, int offsets, double vbc, double a0,
-                                                   double a1, double a2, double b1, double b0, long n) {
+                                                   double a1, double a2, double b1, double b0, long n, RkcPrep q) {
+    rkc_prep(q);
     const double off1 = (offsets && b1 != 0.0) ? kslot_off_dev(kc1, vbc) : 0.0;
     const double off0 = (offsets && b0 != 0.0) ? kslot_off_dev(kc0, vbc) : 0.0;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
@@ -535,7 +563,8 @@ __global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ 
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
         out[c] = (float)in[c];
 }
-__global__ void k_sub_offset_dev(double* u, const double* __restrict__ off, long n) {
+__global__ void k_sub_offset_dev(double* u, const double* __restrict__ off, long n, RkcPrep q) {
+    rkc_prep(q);
     const double o = *off;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) u[c] -= o;
 }
@@ -656,8 +685,19 @@ struct fi_ctx {
     DBuf<double> rkc_off;   // corner offset scratch of fi_integrate_rkc
     double rkc_rho = 0;     // spectral radius estimate of fi_integrate_rkc (1/yr)
     DBuf<KP> rkc_kp;        // single-RHS parameter blocks of fi_integrate_rkc: [U buffer][K slot 0/1]
-    double* rkc_times = nullptr;   // pinned stage times of one fi_integrate_rkc call
+    double* rkc_times = nullptr;   // pinned stage times of one fi_integrate_rkc call, then its abort word
     size_t rkc_times_n = 0;
+    DBuf<double> rkc_tdev;         // the stage times on the device (read by the chain's prep)
+    DBuf<int> rkc_abort;           // barrier abort words of the chain, or-ed on the device
+    // captured chains, one per (sub-steps, stages, sub-step, starting buffer, grid, buffers) key
+    // (at most four, the oldest replaced), with the buffer each chain's state ends in
+    struct RkcGraph {
+        std::vector<unsigned char> key;
+        hipGraphExec_t exec = nullptr;
+        int y_end = 0;
+    };
+    std::vector<RkcGraph> rkc_graphs;
+    bool rkc_graph_off = false;    // capture failed once, or FASTISO_RKC_GRAPH=0
     DBuf<Ctl> ctl_d;
     DBuf<Bar> bar_d;
     DBuf<KP> kp_d;
@@ -954,6 +994,7 @@ void fi_destroy(fi_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->rkc_times) (void)hipHostFree(c->rkc_times);
+    for (auto& g : c->rkc_graphs) (void)hipGraphExecDestroy(g.exec);
     delete c;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -1048,7 +1089,8 @@ int fi_get_field_dev(fi_ctx* c, int32_t f, void* d, size_t n) {
             const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
             hipLaunchKernelGGL(k_corner_offset, dim3(1), dim3(64), 0, c->stream, (const double*)d, c->nx, c->ny,
                                c->cfg.viscous_x_border, c->rkc_off.p);
-            hipLaunchKernelGGL(k_sub_offset_dev, dim3(nb), dim3(256), 0, c->stream, (double*)d, c->rkc_off.p, (long)n);
+            hipLaunchKernelGGL(k_sub_offset_dev, dim3(nb), dim3(256), 0, c->stream, (double*)d, c->rkc_off.p, (long)n,
+                               RkcPrep{});
             HIPCHECK(c, hipGetLastError());
         }
         HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -2010,12 +2052,14 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
     const int nsub = span > 0 ? (int)std::ceil(span / dt_max - 1e-12) : 0;
     const double h = nsub > 0 ? span / nsub : 0.0;
     const int s = nsub > 0 ? rkc_stage_count(1.2 * rho * h) : 0;
+    int graph_state = 0;
     if (nsub > 0) {
         if (!c->rkc_off.p) HIPCHECK(c, c->rkc_off.alloc(1));
         // Every right-hand side of the call is queued on the stream without a host round trip:
         // the control block stays on the device between the launches (its sparse-update count
-        // evolves there), each launch's stage time is copied into it from pinned memory, and its
-        // U buffer and K slot come from one of six prepared parameter blocks.
+        // evolves there), each launch's stage time is written into it on the device from a table
+        // copied once per call, and its U buffer and K slot come from one of six prepared
+        // parameter blocks.
         if (!c->rkc_kp.p) {
             HIPCHECK(c, c->rkc_kp.alloc(6));
             KP kps[6];
@@ -2029,34 +2073,35 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
             }
             HIPCHECK(c, hipMemcpy(c->rkc_kp.p, kps, sizeof(kps), hipMemcpyHostToDevice));
         }
-        // pinned: the stage times, then each launch's barrier abort word (copied back after the
-        // launch, before the next one clears the barrier)
+        // pinned: the stage times in launch order, then the chain's abort word
         const size_t ntimes = (size_t)nsub * (size_t)s;
         if (c->rkc_times_n < ntimes) {
             if (c->rkc_times) (void)hipHostFree(c->rkc_times);
             c->rkc_times = nullptr;
             c->rkc_times_n = 0;
-            HIPCHECK(c, hipHostMalloc((void**)&c->rkc_times, 2 * ntimes * sizeof(double)));
+            HIPCHECK(c, hipHostMalloc((void**)&c->rkc_times, (ntimes + 1) * sizeof(double)));
             c->rkc_times_n = ntimes;
         }
-        int* aborts = (int*)(c->rkc_times + c->rkc_times_n);
-        HIPCHECK(c, hipMemcpy(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice));
-        const void* fk = persistent_kernel(c->mt_double, c->has_dist, c->elra, c->implicit);
-        size_t nt = 0;
-        auto rhs_async = [&](double t, int ubuf, int slot) -> int {
-            c->rkc_times[nt] = t;
-            HIPCHECK(c, hipMemcpyAsync((char*)c->ctl_d.p + offsetof(Ctl, t_rhs), &c->rkc_times[nt], sizeof(double),
-                                       hipMemcpyHostToDevice, c->stream));
-            ++nt;
-            HIPCHECK(c, hipMemsetAsync(c->bar_d.p, 0, sizeof(Bar) * (1 + 2 * (size_t)c->G), c->stream));
-            const KP* kptr = c->rkc_kp.p + 2 * ubuf + slot;
-            void* args[] = {&kptr};
-            HIPCHECK(c, hipLaunchCooperativeKernel(fk, dim3(c->G), dim3(NT), args, (unsigned)c->lds, c->stream));
-            HIPCHECK(c, hipMemcpyAsync(&aborts[nt - 1], &c->bar_d.p->abort, sizeof(int), hipMemcpyDeviceToHost,
-                                       c->stream));
-            return FI_OK;
-        };
+        if (c->rkc_tdev.n < ntimes) HIPCHECK(c, c->rkc_tdev.alloc(ntimes));
+        if (!c->rkc_abort.p) HIPCHECK(c, c->rkc_abort.alloc(1));
+        int* abort_h = (int*)(c->rkc_times + c->rkc_times_n);
         const Rkc k = rkc_make(s);
+        {
+            size_t q = 0;
+            for (int sub = 0; sub < nsub; ++sub) {
+                const double t0 = t_start + sub * h;
+                c->rkc_times[q++] = t0;   // F_0
+                for (int j = 2; j <= s; ++j) c->rkc_times[q++] = t0 + k.c[j - 1] * h;   // F_{j-1}
+            }
+        }
+        HIPCHECK(c, hipMemcpy(c->ctl_d.p, &c->ctl, sizeof(Ctl), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpyAsync(c->rkc_tdev.p, c->rkc_times, ntimes * sizeof(double), hipMemcpyHostToDevice,
+                                   c->stream));
+        const void* fk = persistent_kernel(c->mt_double, c->has_dist, c->elra, c->implicit);
+        const int words = (int)((1 + 2 * (size_t)c->G) * sizeof(Bar) / sizeof(unsigned));
+        auto prep = [&](int idx, int first) {
+            return RkcPrep{c->bar_d.p, words, c->rkc_abort.p, c->ctl_d.p, c->rkc_tdev.p, idx, first};
+        };
         const int offs = c->elra ? 0 : 1;
         const double vbc = c->cfg.viscous_x_border;
         const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
@@ -2065,45 +2110,136 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
         const float* K1 = c->K.p + N;
         const float* kc0 = c->kcorner.p;
         const float* kc1 = c->kcorner.p + 4;
-        int y0 = c->ctl.cur;
-        for (int sub = 0; sub < nsub; ++sub) {
-            const double t0 = t_start + sub * h;
-            const int a = (y0 + 1) % 3, b = (y0 + 2) % 3;
-            if ((st = rhs_async(t0, y0, 0))) return st;   // F_0
-            hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + a * N, U + y0 * N, U + y0 * N,
-                               (const double*)nullptr, K1, K0, kc1, kc0, offs, vbc, 1.0, 0.0, 0.0, 0.0,
-                               k.mut[1] * h, (long)N);
-            HIPCHECK(c, hipGetLastError());
-            int pm1 = a, pm2 = y0;
-            for (int j = 2; j <= s; ++j) {
-                if ((st = rhs_async(t0 + k.c[j - 1] * h, pm1, 1))) return st;   // F_{j-1}
-                const int out = (j == 2) ? b : pm2;                                   // never Y_0
-                hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + out * N, U + y0 * N,
-                                   U + pm1 * N, U + pm2 * N, K1, K0, kc1, kc0, offs, vbc,
-                                   1.0 - k.mu[j] - k.nu[j], k.mu[j], k.nu[j], k.mut[j] * h, k.gt[j] * h, (long)N);
-                HIPCHECK(c, hipGetLastError());
-                pm2 = pm1;
-                pm1 = out;
+        const int y_start = c->ctl.cur;
+        int y_end = y_start;
+        // the chain: the first prep; per sub-step F_0, stage 1, then F_{j-1} and stage j for
+        // j = 2..s, and the sub-step's corner offset. Each kernel between two right-hand sides
+        // prepares the next one; the last one of the chain keeps the last launch's abort word.
+        auto enqueue = [&]() -> hipError_t {
+            hipError_t e;
+            int idx = 0;   // the next right-hand side's entry in the time table
+            hipLaunchKernelGGL(k_rkc_prep, dim3(1), dim3(256), 0, c->stream, prep(idx, 1));
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            int y0 = y_start;
+            for (int sub = 0; sub < nsub; ++sub) {
+                const int a = (y0 + 1) % 3, b = (y0 + 2) % 3;
+                {
+                    const KP* kptr = c->rkc_kp.p + 2 * y0 + 0;   // F_0 into K slot 0
+                    void* args[] = {&kptr};
+                    if ((e = hipLaunchCooperativeKernel(fk, dim3(c->G), dim3(NT), args, (unsigned)c->lds, c->stream)))
+                        return e;
+                    ++idx;
+                }
+                hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + a * N, U + y0 * N, U + y0 * N,
+                                   (const double*)nullptr, K1, K0, kc1, kc0, offs, vbc, 1.0, 0.0, 0.0, 0.0,
+                                   k.mut[1] * h, (long)N, prep(s >= 2 ? idx : -1, 0));
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+                int pm1 = a, pm2 = y0;
+                for (int j = 2; j <= s; ++j) {
+                    const KP* kptr = c->rkc_kp.p + 2 * pm1 + 1;   // F_{j-1} into K slot 1
+                    void* args[] = {&kptr};
+                    if ((e = hipLaunchCooperativeKernel(fk, dim3(c->G), dim3(NT), args, (unsigned)c->lds, c->stream)))
+                        return e;
+                    ++idx;
+                    const int out = (j == 2) ? b : pm2;   // never Y_0
+                    hipLaunchKernelGGL(k_rkc_stage, dim3(nb), dim3(256), 0, c->stream, U + out * N, U + y0 * N,
+                                       U + pm1 * N, U + pm2 * N, K1, K0, kc1, kc0, offs, vbc,
+                                       1.0 - k.mu[j] - k.nu[j], k.mu[j], k.nu[j], k.mut[j] * h, k.gt[j] * h, (long)N,
+                                       prep(j < s ? idx : -1, 0));
+                    if ((e = hipGetLastError()) != hipSuccess) return e;
+                    pm2 = pm1;
+                    pm1 = out;
+                }
+                hipLaunchKernelGGL(k_corner_offset, dim3(1), dim3(64), 0, c->stream, U + pm1 * N, c->nx, c->ny, vbc,
+                                   c->rkc_off.p);
+                hipLaunchKernelGGL(k_sub_offset_dev, dim3(nb), dim3(256), 0, c->stream, U + pm1 * N, c->rkc_off.p,
+                                   (long)N, prep(sub + 1 < nsub ? idx : -1, 0));
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+                y0 = pm1;
             }
-            hipLaunchKernelGGL(k_corner_offset, dim3(1), dim3(64), 0, c->stream, U + pm1 * N, c->nx, c->ny, vbc,
-                               c->rkc_off.p);
-            hipLaunchKernelGGL(k_sub_offset_dev, dim3(nb), dim3(256), 0, c->stream, U + pm1 * N, c->rkc_off.p,
-                               (long)N);
-            HIPCHECK(c, hipGetLastError());
-            y0 = pm1;
+            // the BS3 loop ping-pongs u_n between U[0] and U[1] (U[2] is scratch): a state left in
+            // U[2] moves to U[0]
+            if (y0 == 2) {
+                if ((e = hipMemcpyAsync(U, U + 2 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream)))
+                    return e;
+                y0 = 0;
+            }
+            y_end = y0;
+            return hipSuccess;
+        };
+        // every value the captured chain holds: counts, sub-step, buffers and launch shape
+        struct Key {
+            int nsub, s, y0, G;
+            size_t lds;
+            double h, vbc;
+            const void *kp, *tdev, *abrt, *U, *K, *fk;
+        };
+        Key key;
+        std::memset(&key, 0, sizeof(key));
+        key.nsub = nsub;
+        key.s = s;
+        key.y0 = y_start;
+        key.G = c->G;
+        key.lds = c->lds;
+        key.h = h;
+        key.vbc = vbc;
+        key.kp = c->rkc_kp.p;
+        key.tdev = c->rkc_tdev.p;
+        key.abrt = c->rkc_abort.p;
+        key.U = c->U.p;
+        key.K = c->K.p;
+        key.fk = fk;
+        std::vector<unsigned char> kb((const unsigned char*)&key, (const unsigned char*)&key + sizeof(key));
+        const char* genv = getenv("FASTISO_RKC_GRAPH");
+        if (genv && genv[0] == '0') c->rkc_graph_off = true;
+        hipGraphExec_t exec = nullptr;
+        int y0 = y_start;
+        if (!c->rkc_graph_off) {
+            for (auto& g : c->rkc_graphs)
+                if (g.key == kb) {
+                    exec = g.exec;
+                    y0 = g.y_end;
+                    graph_state = 1;
+                }
+            if (!exec) {
+                hipGraph_t gr = nullptr;
+                hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    const hipError_t e1 = enqueue();
+                    const hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
+                    e = e1 != hipSuccess ? e1 : e2;
+                }
+                if (e == hipSuccess && gr) e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+                if (gr) (void)hipGraphDestroy(gr);
+                if (e != hipSuccess || !exec) {
+                    // the chain cannot be captured here: it is launched directly from now on
+                    (void)hipGetLastError();
+                    if (exec) (void)hipGraphExecDestroy(exec);
+                    exec = nullptr;
+                    c->rkc_graph_off = true;
+                } else {
+                    if (c->rkc_graphs.size() >= 4) {
+                        (void)hipGraphExecDestroy(c->rkc_graphs.front().exec);
+                        c->rkc_graphs.erase(c->rkc_graphs.begin());
+                    }
+                    c->rkc_graphs.push_back({kb, exec, y_end});
+                    y0 = y_end;
+                    graph_state = 2;
+                }
+            }
         }
-        // the BS3 loop ping-pongs u_n between U[0] and U[1] (U[2] is scratch): a state left in
-        // U[2] moves to U[0]
-        if (y0 == 2) {
-            HIPCHECK(c, hipMemcpyAsync(U, U + 2 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-            y0 = 0;
+        if (exec) {
+            HIPCHECK(c, hipGraphLaunch(exec, c->stream));
+        } else {
+            HIPCHECK(c, enqueue());
+            y0 = y_end;
         }
+        HIPCHECK(c, hipMemcpyAsync(abort_h, c->rkc_abort.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipMemcpyAsync(&c->ctl, c->ctl_d.p, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
         c->ctl.cur = y0;
         c->ctl.last_u = y0;
-        for (size_t q = 0; q < nt; ++q)
-            if (aborts[q]) return fail(c, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
+        if (*abort_h) return fail(c, FI_ERR_DEVICE, "device barrier timed out; kernel aborted");
         if (c->ctl.err == ERR_TIME_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation2D out of range.");
         if (c->ctl.err == ERR_BSL_RANGE) return fail(c, FI_ERR_TIME_RANGE, "TimeInterpolation0D out of range.");
         if (c->ctl.err) return fail(c, FI_ERR_DEVICE, "device error");
@@ -2116,6 +2252,8 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
         info->substeps = nsub;
         info->stages = s;
         info->n_rhs = c->ctl.nrhs - r0;
+        info->graph = graph_state;
+        info->reserved = 0;
     }
     return FI_OK;
 }

@@ -92,7 +92,7 @@ class FiBatchInfo(C.Structure):
 
 class FiRkcInfo(C.Structure):
     _fields_ = [("rho", C.c_double), ("h", C.c_double), ("substeps", C.c_int32), ("stages", C.c_int32),
-                ("n_rhs", C.c_int64)]
+                ("n_rhs", C.c_int64), ("graph", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -364,13 +364,19 @@ int fi_set_time(fi_ctx* ctx, double t);
  * is. No counterpart in the reference, which integrates with OrdinaryDiffEq's BS3: it is the
  * macro-step predictor of the surrogate-with-fallback mode, whose result the plain solver checks.
  * The next fi_integrate re-evaluates the first-same-as-last derivative. LV-ELVA and ELRA
- * contexts; FI_ERR_UNSUPPORTED under the implicit step. */
+ * contexts; FI_ERR_UNSUPPORTED under the implicit step. The call's kernels (one persistent
+ * launch per right-hand side, one stage kernel between two of them, which also prepares the next
+ * launch on the device) are captured once into a HIP graph per sub-step count, stage count,
+ * sub-step and starting buffer, and the graph is replayed by later calls with the same ones;
+ * FASTISO_RKC_GRAPH=0 in the environment launches them one by one instead. */
 typedef struct fi_rkc_info {
     double rho;          /* spectral radius used (1/yr) */
     double h;            /* sub-step (yr) */
     int32_t substeps;
     int32_t stages;      /* stages per sub-step */
     int64_t n_rhs;       /* right-hand-side evaluations of the call (power iterations included) */
+    int32_t graph;       /* 1: replayed from a HIP graph, 2: captured by this call, 0: launched directly */
+    int32_t reserved;
 } fi_rkc_info;
 int fi_integrate_rkc(fi_ctx* ctx, double t_target, double dt_max, double rho, fi_rkc_info* info);
 

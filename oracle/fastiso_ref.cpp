@@ -1420,6 +1420,8 @@ int fi_integrate_rkc(fi_ctx* c, double t_target, double dt_max, double rho, fi_r
         info->substeps = nsub;
         info->stages = s;
         info->n_rhs = c->nrhs - r0;
+        info->graph = 0;
+        info->reserved = 0;
     }
     return FI_OK;
 }

@@ -36,7 +36,7 @@ class RefStats(C.Structure):
 
 class RefRkcInfo(C.Structure):
     _fields_ = [("rho", C.c_double), ("h", C.c_double), ("substeps", C.c_int32), ("stages", C.c_int32),
-                ("n_rhs", C.c_int64)]
+                ("n_rhs", C.c_int64), ("graph", C.c_int32), ("reserved", C.c_int32)]
 
 
 class RefTuneInfo(C.Structure):

@@ -223,3 +223,54 @@ def test_gpu_rkc_matches_oracle_rkc_and_bs3_on_lv_case():
     if not DRYRUN:
         e.close()
         e2.close()
+
+
+@pytest.mark.gpu
+def test_gpu_rkc_graph_replay_equals_direct_launches():
+    """The chain of an fi_integrate_rkc call is captured into a HIP graph on its first call with a
+    given sub-step count, stage count, sub-step and starting buffer, and replayed by later calls
+    (the stage times are a device table written once per call). Two contexts from the same state,
+    one launching the chain directly (FASTISO_RKC_GRAPH=0), advance through three macro steps
+    with a BS3 step in between (the starting buffer changes): equal bit for bit, with the graph
+    captured, replayed and captured again for the other starting buffer."""
+    import os
+    from oracle import pyref
+    w = lv_rkc_case(n=48)
+    inp = sim_inputs(w).inputs
+    plain = pyref.RefEngine.from_inputs(inp)
+    plain.integrator_init()
+    plain.integrate(100.0)
+    u0 = plain.get_field("u")
+
+    def fresh():
+        e = hip_sim(w).engine
+        e.integrator_init()
+        e.set_field("u", u0)
+        e.set_time(100.0)
+        return e
+
+    a, b = fresh(), fresh()
+    graphs = []
+    os.environ["FASTISO_RKC_GRAPH"] = "0"
+    try:
+        ib = b.integrate_rkc(200.0, 50.0, rho=120.0)
+    finally:
+        del os.environ["FASTISO_RKC_GRAPH"]
+    assert ib["graph"] == 0
+    ia = a.integrate_rkc(200.0, 50.0, rho=120.0)
+    graphs.append(ia["graph"])
+    np.testing.assert_array_equal(a.get_field("u"), b.get_field("u"))
+    for t in (300.0, 300.5, 400.5):
+        for e in (a, b):
+            if t == 300.5:
+                e.integrate(t)          # one BS3 stretch: the state moves to the other buffer
+            else:
+                info = e.integrate_rkc(t, 50.0, rho=120.0)
+                if e is a:
+                    graphs.append(info["graph"])
+                else:
+                    assert info["graph"] == 0
+        np.testing.assert_array_equal(a.get_field("u"), b.get_field("u"), err_msg=f"t={t}")
+    if not DRYRUN:
+        # three calls of the same counts and sub-step from at most two starting buffers
+        assert graphs[0] == 2 and 1 in graphs[1:] and set(graphs) <= {1, 2}, graphs
