@@ -301,6 +301,40 @@ def surrogate_leg(n, window_kyr, predictor="rkc", rkc_dt=250.0, check_every=10, 
     return out
 
 
+def bench_rkc(a):
+    """RKC2 macro steps of 500 yr (sub-steps of at most --rkc-dt yr) at N=350 from the BS3 state
+    at 100 yr: the first call estimates the spectral radius and captures the chain's HIP graph,
+    the next --steps calls are timed one by one (FASTISO_RKC_GRAPH=0 launches the chain
+    directly)."""
+    w, sim = make_sim(a.n, engine="hip")
+    e = sim.engine
+    e.integrator_init()
+    e.integrate(100.0)
+    t = 100.0
+    first = None
+    times, graphs = [], []
+    steps = a.steps or 10
+    for k in range(steps + 1):
+        t += 500.0
+        t0 = time.perf_counter()
+        info = e.integrate_rkc(t, a.rkc_dt)
+        el = time.perf_counter() - t0
+        if k == 0:
+            first = {"wall_ms": el * 1e3, **info}
+        else:
+            times.append(el * 1e3)
+            graphs.append(info["graph"])
+    sim.close()
+    ms = sorted(times)[len(times) // 2]
+    return {"metric": f"RKC2 500-yr macro steps at N={a.n} LV-ELVA (ms per macro step, median)", "value": ms,
+            "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": 1, "higher_is_better": False,
+            "dtype": "f32 fft/stencil-moments, f64 state", "data": "synthetic (seeded fields)",
+            "config": {"workload": f"rkc_n{a.n}", "rkc_max_substep_yr": a.rkc_dt, "substeps": info["substeps"],
+                       "stages": info["stages"], "rhs_per_macro_step": info["n_rhs"], "graph_states": graphs,
+                       "ms_per_macro_step": times, "first_call": first,
+                       "graph_disabled": os.environ.get("FASTISO_RKC_GRAPH") == "0"}}
+
+
 def bench_surrogate(a):
     """--workload surrogate: the surrogate leg alone, over --window-kyr."""
     leg = surrogate_leg(a.n, a.window_kyr, predictor=a.predictor, rkc_dt=a.rkc_dt, check_every=a.check_every,
@@ -484,9 +518,13 @@ def profile_suite(a):
     (rocprofv3 --kernel-trace, or one --pmc counter set) collects all of them: the HBM copy
     calibration kernel, the N=350 plain solver (3 windows after 1), the 256-member ensemble (2
     windows after 1), the N=1024 instance (1 window after 1) and the RKC predictor (one 500-yr
-    macro step at N=350). One JSON line per part."""
+    macro step at N=350). One JSON line per part. The parts run the engine's default FFT plans and
+    grid size (no autotuning): the profiles describe those kernels, and their dispatches stay
+    free of tuning launches."""
     import copy
     parts = []
+    a = copy.copy(a)
+    a.autotune = "none"
     b = copy.copy(a)
     b.steps, b.warmup = 5, 2
     parts.append(bench_hbm_copy(b))
@@ -520,7 +558,7 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["lv_elva", "lv_elva_hr", "elra", "implicit", "ensemble", "elra_ensemble", "inversion",
-                                               "surrogate", "fno_train", "hbm_copy", "profile_suite"],
+                                               "surrogate", "rkc", "fno_train", "hbm_copy", "profile_suite"],
                     default="lv_elva")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--window-kyr", type=float, default=10.0, help="surrogate window (kyr)")
@@ -542,7 +580,7 @@ def main():
     elra_ens = a.workload == "elra_ensemble"
     inv = a.workload == "inversion"
     if a.steps is None:
-        a.steps = 3 if hr_defaults else 4 if inv else 10 if elra_ens else 50 if (elra or implicit) else 200
+        a.steps = 3 if hr_defaults else 4 if inv else 10 if (elra_ens or a.workload == "rkc") else 50 if (elra or implicit) else 200
     if a.warmup is None:
         a.warmup = 1 if (hr_defaults or inv) else 8 if elra_ens else 2 if (elra or implicit) else 5
     if a.window is None:
@@ -559,6 +597,9 @@ def main():
         return
     if a.workload == "surrogate":
         print(json.dumps(bench_surrogate(a)), flush=True)
+        return
+    if a.workload == "rkc":
+        print(json.dumps(bench_rkc(a)), flush=True)
         return
     if a.workload == "fno_train":
         print(json.dumps(bench_fno_train(a)), flush=True)
