@@ -172,6 +172,7 @@ def bench_ensemble(a):
                    "window_yr": a.window, "integrator": "BS3 reltol 1e-5 abstol 1e-6",
                    "accepted_steps_total": int(sum(acc)), "rhs_calls_total": int(sum(rhs)), "cell_steps": cell_steps,
                    "team_size": i1["team_size"], "n_teams": i1["n_teams"], "autotuned": a.team_size == 0,
+                   "wg_threads": i1["wg_threads"],
                    "kernel_launches": int(i1["n_launch"] - i0["n_launch"]), "setup_s": setup_s,
                    "fft_plans_tuned_s": tune_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -3,11 +3,26 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Threads per workgroup and the namespace of this build of the device code: 512 threads (eight
+// wave64, two per SIMD) in namespace fi; the ensemble kernels are also built with 256 threads in
+// namespace fi256 (fi_batch256.hip).
+#ifndef FI_NT
+#define FI_NT 512
+#endif
+#ifndef FI_NS
+#define FI_NS fi
+#endif
+
 #include "fi_fft.hpp"
 
-namespace fi {
+namespace FI_NS {
 
-constexpr int NT = 512;  // threads per workgroup (eight wave64: two per SIMD)
+constexpr int NT = FI_NT;
+// the canonical workgroup width of every block reduction: a build with fewer threads gives each
+// thread VNT / NT virtual lanes, so sums round identically in both builds
+constexpr int VNT = 512;
+constexpr int NV = VNT / NT;
+static_assert(VNT % NT == 0, "NT must divide 512");
 
 enum { MODE_RHS = 0, MODE_INTEGRATE = 1, MODE_SPARSE_ONLY = 2 };
 enum { ICE_EXTERNAL = 0, ICE_SNAPSHOTS = 1, ICE_SEPARABLE = 2 };
@@ -163,4 +178,4 @@ struct KP {
 };
 enum { TAG_ROWS = 0, TAG_COLS = 1, TAG_C2R = 2, TAG_SPARSE = 3, TAG_KIN = 4 };
 
-}  // namespace fi
+}  // namespace FI_NS

@@ -9,7 +9,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-namespace fi {
+#ifndef FI_NS
+#define FI_NS fi
+#endif
+
+namespace FI_NS {
 
 struct FftPlan {
     int L;                   // transform length
@@ -225,4 +229,4 @@ __device__ __forceinline__ C* fft_lds(C* a, C* b, int nb, int seq, const FftPlan
     return in;
 }
 
-}  // namespace fi
+}  // namespace FI_NS

@@ -22,7 +22,7 @@
 
 #include "fi_device.hpp"
 
-namespace fi {
+namespace FI_NS {
 
 // ------------------------------------------------------------------------------------------
 // grid barrier (all workgroups co-resident: cooperative launch). Every workgroup publishes its
@@ -100,28 +100,37 @@ __device__ __forceinline__ bool grid_sync(const KP& p, const Team& tm, unsigned&
     return ok;
 }
 
-// Block-wide sums of M doubles per thread: wave64 butterfly, then the waves in fixed order
-// (identical rounding in every workgroup). red holds at least M * NT/64 doubles.
+// Block-wide sums in the canonical order of a 512-thread workgroup (VNT lanes), whatever the
+// build's width: thread t holds the partial sums of virtual lanes t + v NT (v < NV), each lane's
+// elements in increasing order; each virtual wave reduces by the wave64 butterfly, then the
+// virtual waves are added in fixed order. Identical rounding in every workgroup and in both
+// builds. The result is in v[0]; red holds at least M * VNT/64 doubles.
 template <int M>
-__device__ __forceinline__ void block_sum(double* red, double v[M]) {
+__device__ __forceinline__ void block_sum(double* red, double (&v)[NV][M]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < M; ++q) {
-        double x = v[q];
+    for (int vv = 0; vv < NV; ++vv) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        v[q] = x;
+        for (int q = 0; q < M; ++q) {
+            double x = v[vv][q];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            v[vv][q] = x;
+        }
     }
     if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < M; ++q) red[w * M + q] = v[q];
+        for (int vv = 0; vv < NV; ++vv) {
+#pragma unroll
+            for (int q = 0; q < M; ++q) red[(w + vv * (NT / 64)) * M + q] = v[vv][q];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < M; ++q) {
         double s = 0.0;
-        for (int ww = 0; ww < NT / 64; ++ww) s += red[ww * M + q];
-        v[q] = s;
+        for (int ww = 0; ww < VNT / 64; ++ww) s += red[ww * M + q];
+        v[0][q] = s;
     }
     __syncthreads();
 }
@@ -304,32 +313,39 @@ __device__ __forceinline__ void stage_preamble(const KP& p, const StageDesc& st,
         }
     }
     // ---- spectral corners: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx}) ----
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double vs[NV][4] = {};
     if (spec_on) {
         const bool even = (nx % 2) == 0;
-        for (int kx = tid; kx < p.nhx; kx += NT) {
-            float2 a = sa, b = sb, tw_k = tw;
-            if (kx >= NT) {
-                a = spec[(long)kx * ny];
-                b = spec[(long)kx * ny + ny - 1];
-                tw_k = twx[kx];
-            }
-            const bool nyq = even && kx == nx / 2;
-            const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
-            v[0] += w * (double)a.x;
-            v[2] += w * (double)b.x;
-            if (nyq) {
-                v[1] -= (double)a.x;
-                v[3] -= (double)b.x;
-            } else {
-                v[1] += w * ((double)a.x * tw_k.x - (double)a.y * tw_k.y);
-                v[3] += w * ((double)b.x * tw_k.x - (double)b.y * tw_k.y);
+        for (int k0 = tid; k0 < p.nhx; k0 += VNT) {
+#pragma unroll
+            for (int vv = 0; vv < NV; ++vv) {
+                const int kx = k0 + vv * NT;
+                if (kx >= p.nhx) break;
+                float2 a = sa, b = sb, tw_k = tw;
+                if (kx >= NT) {
+                    a = spec[(long)kx * ny];
+                    b = spec[(long)kx * ny + ny - 1];
+                    tw_k = twx[kx];
+                }
+                const bool nyq = even && kx == nx / 2;
+                const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
+                double* v = vs[vv];
+                v[0] += w * (double)a.x;
+                v[2] += w * (double)b.x;
+                if (nyq) {
+                    v[1] -= (double)a.x;
+                    v[3] -= (double)b.x;
+                } else {
+                    v[1] += w * ((double)a.x * tw_k.x - (double)a.y * tw_k.y);
+                    v[3] += w * ((double)b.x * tw_k.x - (double)b.y * tw_k.y);
+                }
             }
         }
-        block_sum<4>(red, v);
+        block_sum<4>(red, vs);
     } else {
         __syncthreads();
     }
+    const double* v = vs[0];
     KC4 kc;
     kc.v0 = (float)v[0];
     kc.v1 = (float)v[1];
@@ -1110,8 +1126,13 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
         for (int k = threadIdx.x; k < nx; k += NT) sf[k] = c2r_pair_value(p, spec, k, j0, j0 + 1);
         __syncthreads();
         float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, +1);
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int idx = threadIdx.x, q = 0; idx < 2 * nx; idx += NT, ++q) {
+        double accv[NV][3] = {};
+        for (int i0 = threadIdx.x, q0 = 0; i0 < 2 * nx; i0 += VNT, q0 += NV)
+#pragma unroll
+        for (int vv = 0; vv < NV; ++vv) {
+            const int idx = i0 + vv * NT, q = q0 + vv;
+            if (idx >= 2 * nx) break;
+            double* acc = accv[vv];
             const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
@@ -1143,11 +1164,11 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
             }
         }
         if (st.final_err) {
-            block_sum<3>(red, acc);
+            block_sum<3>(red, accv);
             if (threadIdx.x == 0) {
-                st_shared(&p.part[4 * item + 0], acc[0]);
-                st_shared(&p.part[4 * item + 1], acc[1]);
-                st_shared(&p.part[4 * item + 2], acc[2]);
+                st_shared(&p.part[4 * item + 0], accv[0][0]);
+                st_shared(&p.part[4 * item + 1], accv[0][1]);
+                st_shared(&p.part[4 * item + 2], accv[0][2]);
             }
         }
         __syncthreads();
@@ -1299,9 +1320,14 @@ __device__ __forceinline__ void phase_sp_volumes(const KP& p, const Team& tm, co
                                                  const IceT it, double* red) {
     const int nx = p.nx;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
-        double acc[3] = {0.0, 0.0, 0.0};
+        double accv[NV][3] = {};
         const int ncell = (min(2 * item + 2, p.ny) - 2 * item) * nx;
-        for (int q = threadIdx.x; q < ncell; q += NT) {
+        for (int q0 = threadIdx.x; q0 < ncell; q0 += VNT)
+#pragma unroll
+        for (int vv = 0; vv < NV; ++vv) {
+            const int q = q0 + vv * NT;
+            if (q >= ncell) break;
+            double* acc = accv[vv];
             const long c = (long)2 * item * nx + q;
             const double u = u_raw(p, st, c) - uoff;
             const double H = ice_at(p, it, c);
@@ -1312,11 +1338,11 @@ __device__ __forceinline__ void phase_sp_volumes(const KP& p, const Team& tm, co
             acc[1] += H * a;
             acc[2] += fmax(zss_old - zb_old, 0.0) * a;
         }
-        block_sum<3>(red, acc);
+        block_sum<3>(red, accv);
         if (threadIdx.x == 0) {
-            st_shared(&p.vpart[4 * item + 0], acc[0]);
-            st_shared(&p.vpart[4 * item + 1], acc[1]);
-            st_shared(&p.vpart[4 * item + 2], acc[2]);
+            st_shared(&p.vpart[4 * item + 0], accv[0][0]);
+            st_shared(&p.vpart[4 * item + 1], accv[0][1]);
+            st_shared(&p.vpart[4 * item + 2], accv[0][2]);
         }
     }
 }
@@ -1348,13 +1374,18 @@ __device__ __forceinline__ double interp0d(const double* x, const double* y, int
 // barystatic sea level and ocean surface (update_bsl!(bsl, -delta_V, t),
 // src/barystatic_sealevel.jl:215-234); count the sparse update.
 __device__ __forceinline__ void volumes_and_bsl(const KP& p, Ctl& s, double t, double* red) {
-    double vs[3] = {0.0, 0.0, 0.0};
-    for (int w = threadIdx.x; w < p.nrowitems; w += NT) {
-        vs[0] += p.vpart[4 * w];
-        vs[1] += p.vpart[4 * w + 1];
-        vs[2] += p.vpart[4 * w + 2];
+    double vsv[NV][3] = {};
+    for (int w0 = threadIdx.x; w0 < p.nrowitems; w0 += VNT)
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv) {
+        const int w = w0 + vv * NT;
+        if (w >= p.nrowitems) break;
+        vsv[vv][0] += p.vpart[4 * w];
+        vsv[vv][1] += p.vpart[4 * w + 1];
+        vsv[vv][2] += p.vpart[4 * w + 2];
     }
-    block_sum<3>(red, vs);
+    block_sum<3>(red, vsv);
+    const double* vs = vsv[0];
     if (threadIdx.x == 0) {
         const double Vold = s.V_af + s.V_den + s.V_pov;
         s.V_af = p.vol_goelzer ? vs[0] * p.rho_ice_sw : 0.0;
@@ -1630,8 +1661,13 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
     // at m = i1-1 and i1-1+nx-1 of the kept rows j = 0 and ny-1 (w = 1 for the DC and Nyquist
     // terms, whose imaginary parts the real transform ignores; 2 otherwise)
     const int m0 = p.i1 - 1, m1 = p.i1 - 1 + nx - 1;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < nfhx; k += NT) {
+    double vs[NV][4] = {};
+    for (int k0 = threadIdx.x; k0 < nfhx; k0 += VNT)
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv) {
+        const int k = k0 + vv * NT;
+        if (k >= nfhx) break;
+        double* v = vs[vv];
         float2 a = p.oA[(long)k * ny], b = p.oA[(long)k * ny + ny - 1];
         const bool dcn = k == 0 || (even && k == nfx / 2);
         if (dcn) {
@@ -1647,7 +1683,8 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
         v[2] += w * ((double)b.x * c0 - (double)b.y * s0);
         v[3] += w * ((double)b.x * c1 - (double)b.y * s1);
     }
-    block_sum<4>(red, v);
+    block_sum<4>(red, vs);
+    const double* v = vs[0];
     const double off = ((0.25 * v[0] + 0.25 * v[1]) + (0.25 * v[2] + 0.25 * v[3])) - p.vbc_x;
     float* K = p.Kb + (long)st.kout * p.Nl;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
@@ -1670,8 +1707,13 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
         }
         __syncthreads();
         float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, +1);
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int idx = threadIdx.x; idx < 2 * nx; idx += NT) {
+        double accv[NV][3] = {};
+        for (int i0 = threadIdx.x; i0 < 2 * nx; i0 += VNT)
+#pragma unroll
+        for (int vv = 0; vv < NV; ++vv) {
+            const int idx = i0 + vv * NT;
+            if (idx >= 2 * nx) break;
+            double* acc = accv[vv];
             const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
@@ -1693,11 +1735,11 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
             }
         }
         if (st.final_err) {
-            block_sum<3>(red, acc);
+            block_sum<3>(red, accv);
             if (threadIdx.x == 0) {
-                st_shared(&p.part[4 * item + 0], acc[0]);
-                st_shared(&p.part[4 * item + 1], acc[1]);
-                st_shared(&p.part[4 * item + 2], acc[2]);
+                st_shared(&p.part[4 * item + 0], accv[0][0]);
+                st_shared(&p.part[4 * item + 1], accv[0][1]);
+                st_shared(&p.part[4 * item + 2], accv[0][2]);
             }
         }
         __syncthreads();
@@ -1856,8 +1898,13 @@ __device__ __forceinline__ void phase_imp_c2r(const KP& p, const Team& tm, const
     float2* sf = (float2*)smem;
     float2* sg = sf + nx;
     // increment at the four corners: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx})
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int kx = threadIdx.x; kx < p.nhx; kx += NT) {
+    double vs[NV][4] = {};
+    for (int k0 = threadIdx.x; k0 < p.nhx; k0 += VNT)
+#pragma unroll
+    for (int vv = 0; vv < NV; ++vv) {
+        const int kx = k0 + vv * NT;
+        if (kx >= p.nhx) break;
+        double* v = vs[vv];
         const float2 a = spec[(long)kx * ny], b = spec[(long)kx * ny + ny - 1], tw = twx[kx];
         const bool nyq = even && kx == nx / 2;
         const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
@@ -1871,7 +1918,8 @@ __device__ __forceinline__ void phase_imp_c2r(const KP& p, const Team& tm, const
             v[3] += w * ((double)b.x * tw.x - (double)b.y * tw.y);
         }
     }
-    block_sum<4>(red, v);
+    block_sum<4>(red, vs);
+    const double* v = vs[0];
     // corners of the new u: the stage's corners (read before this phase overwrites u) + increment
     const double off = ((0.25 * (ucorner[0] + v[0]) + 0.25 * (ucorner[1] + v[1])) +
                         (0.25 * (ucorner[2] + v[2]) + 0.25 * (ucorner[3] + v[3]))) - p.vbc_x;
@@ -2002,4 +2050,4 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
     return true;
 }
 
-}  // namespace fi
+}  // namespace FI_NS

@@ -10,7 +10,7 @@ import os
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ENGINE_PATH = os.path.join(PKG_ROOT, "lib", "libfastiso_hip.so")
 
-FI_ABI_VERSION = 5
+FI_ABI_VERSION = 6
 
 # status codes
 FI_OK = 0
@@ -80,6 +80,7 @@ class FiBatchInfo(C.Structure):
         ("device_ms", C.c_double), ("n_launch", C.c_int64), ("cell_attempts", C.c_double),
         ("last_team_size", C.c_int32), ("n_candidates", C.c_int32),
         ("cand_size", C.c_int32 * 8), ("cand_rate", C.c_double * 8),
+        ("wg_threads", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self):

@@ -51,7 +51,7 @@ const FI = (litho_rigidity = 1, effective_viscosity = 2, scaled_pseudodiff_inv =
             u_x = 50, u_y = 51)
 
 # FI_ABI_VERSION of include/fastiso.h: the layouts of FiConfig and FiStats above follow it
-const FI_ABI_VERSION = 5
+const FI_ABI_VERSION = 6
 
 function __init__()
     v = ccall((:fi_abi_version, libfi), Int32, ())

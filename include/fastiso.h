@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FI_ABI_VERSION 5
+#define FI_ABI_VERSION 6
 
 /* ---- status codes ------------------------------------------------------------------------ */
 enum {
@@ -413,6 +413,9 @@ typedef struct fi_batch_info {
     int32_t n_candidates;                /* team sizes the tuner considers (0 with a given size) */
     int32_t cand_size[FI_BATCH_MAX_CANDIDATES];
     double cand_rate[FI_BATCH_MAX_CANDIDATES];  /* timed cell-attempts per ms, 0 if not timed */
+    int32_t wg_threads;    /* threads per workgroup of the last launch: 256 (two workgroups per CU,
+                              members up to 256 cells wide) or 512; 0 before any launch */
+    int32_t reserved;
 } fi_batch_info;
 
 /* Creates nmem member contexts, member m as fi_create(&cfgs[m]). Members are set up, initialised
@@ -428,11 +431,14 @@ fi_ctx* fi_batch_member(fi_batch* b, int32_t m);
 void fi_batch_destroy(fi_batch* b);
 const char* fi_batch_last_error(const fi_batch* b);
 
-/* Workgroups per member team. 0 (the default) times the candidate sizes (32, 16, ..., 1) on the
- * first launches of fi_batch_integrate and keeps the fastest, stopping once a smaller team is
- * timed below 0.8 of the best rate: a trial of 64 attempts per member counts when at least 90% of
- * its capped work ran; when windows are shorter than that, each candidate is timed over a whole
- * window. Results do not depend on the team size. */
+/* Workgroups per member team. 0 (the default) times the candidate sizes (64 with 256-thread
+ * workgroups, then 32, 16, ..., 1) on the first launches of fi_batch_integrate and keeps the
+ * fastest, stopping once a smaller team is timed below 0.8 of the best rate: a trial of 64
+ * attempts per member counts when at least 90% of its capped work ran; when windows are shorter
+ * than that, each candidate is timed over a whole window. Ensembles whose members are at most 256
+ * cells wide run workgroups of 256 threads, two per CU (FASTISO_BATCH_WG_THREADS=512 forces the
+ * 512-thread build); fi_batch_info.wg_threads reports which ran. Results do not depend on the
+ * team size or the workgroup width. */
 int fi_batch_set_team_size(fi_batch* b, int32_t team_size);
 
 /* Advances every member (after fi_integrator_init) to t_target with its own adaptive BS3 steps.

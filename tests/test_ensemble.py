@@ -120,6 +120,32 @@ def test_ensemble_independent_of_team_size(team_size):
         # with team_size 0 this short run may end before every candidate size has been timed
 
 
+@pytest.mark.gpu
+def test_ensemble_builds_of_256_and_512_threads_agree_bitwise(monkeypatch):
+    """Members up to 256 cells wide run the 256-thread build of the ensemble kernel (two
+    workgroups per CU); FASTISO_BATCH_WG_THREADS=512 forces the 512-thread build, which wider
+    members always use. Both sum in the canonical 512-lane order, so members agree bit for bit;
+    an ensemble with a member wider than 256 cells runs the 512-thread build."""
+    inputs = small_ensemble()
+    res256, info256 = run_ensemble(inputs, team_size=4)
+    monkeypatch.setenv("FASTISO_BATCH_WG_THREADS", "512")
+    res512, info512 = run_ensemble(inputs, team_size=4)
+    monkeypatch.delenv("FASTISO_BATCH_WG_THREADS")
+    for m in range(len(inputs)):
+        np.testing.assert_array_equal(res256[m][0], res512[m][0], err_msg=f"member {m}")
+        np.testing.assert_array_equal(res256[m][1], res512[m][1], err_msg=f"member {m}")
+        s0, s1 = res256[m][2], res512[m][2]
+        assert (s0["n_accept"], s0["n_reject"], s0["n_rhs"]) == (s1["n_accept"], s1["n_reject"], s1["n_rhs"])
+    if not DRYRUN:
+        assert info256["wg_threads"] == 256 and info512["wg_threads"] == 512
+        wide = [sim_inputs(w).inputs for w in workloads.ensemble(nmem=1, sizes=(300,), t_end=10.0)]
+        ens = hip_ensemble(inputs[:1] + wide, 4)
+        ens.integrator_init()
+        ens.integrate(1.0)
+        assert ens.info()["wg_threads"] == 512, ens.info()
+        ens.close()
+
+
 def scaled_reltol(inp, factor):
     import ctypes
     cfg = type(inp["cfg"])()
@@ -319,7 +345,9 @@ def test_gpu_elra_ensemble_tuner_settles_on_short_windows():
         t += 100.0
         tuned.integrate(t)
         info = tuned.info()
-        assert info["tuned"] == 1 and info["team_size"] in (1, 2, 4, 8, 16, 32), info
+        assert info["tuned"] == 1 and info["team_size"] in info["cand_size"], info
+        # members up to 256 cells wide run the 256-thread build, whose candidates start at 64
+        assert info["wg_threads"] == 256 and info["cand_size"][0] == 64, info
         # the size in use is the fastest timed candidate, and later launches use it
         timed = [(r, sz) for sz, r in zip(info["cand_size"], info["cand_rate"]) if r > 0]
         assert timed and info["team_size"] == max(timed)[1], info
