@@ -518,8 +518,8 @@ def profile_suite(a):
     """The workloads whose kernels profiles/ covers, in one process, so that one profiler pass
     (rocprofv3 --kernel-trace, or one --pmc counter set) collects all of them: the HBM copy
     calibration kernel, the N=350 plain solver (3 windows after 1), the 256-member ensemble (2
-    windows after 1), the N=1024 instance (1 window after 1) and the RKC predictor (one 500-yr
-    macro step at N=350). One JSON line per part. The parts run the engine's default FFT plans and
+    windows after 1, at the tuner's team size), the N=1024 instance (1 window after 1) and the RKC
+    predictor (one 500-yr macro step at N=350). One JSON line per part. The parts run the engine's default FFT plans and
     grid size (no autotuning): the profiles describe those kernels, and their dispatches stay
     free of tuning launches."""
     import copy
@@ -535,7 +535,9 @@ def profile_suite(a):
         b.window = 5.0 if wl == "lv_elva_hr" else 10.0
         parts.append(run_plain(b))
     b = copy.copy(a)
-    b.workload, b.steps, b.warmup, b.window = "ensemble", 2, 1, 10.0
+    # the team size the tuner settles on for this ensemble (32 workgroups of 256 threads), fixed
+    # so that the profiled launches are whole windows rather than the tuner's trials
+    b.workload, b.steps, b.warmup, b.window, b.team_size = "ensemble", 2, 1, 10.0, 32
     parts.append(bench_ensemble(b))
     w, sim = make_sim(350, engine="hip")
     e = sim.engine
