@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 // Threads per workgroup and the namespace of this build of the device code: 512 threads (eight
 // wave64, two per SIMD) in namespace fi; the ensemble kernels are also built with 256 threads in
 // namespace fi256 (fi_batch256.hip).
@@ -16,6 +18,39 @@
 #include "fi_fft.hpp"
 
 namespace FI_NS {
+
+// Coherent load of device memory that another workgroup may have written earlier in the same
+// launch: an agent-scope relaxed atomic load (global_load ... sc1), which reads the device's
+// coherence point instead of a line the XCD's L2 may hold from before that write. With every such
+// read coherent, the grid barrier needs no agent-scope acquire (buffer_inv sc1, an invalidate of
+// the whole L2 of the XCD).
+template <typename T>
+__device__ __forceinline__ typename std::remove_const<T>::type ldc(T* a) {
+    typedef typename std::remove_const<T>::type U;
+    if constexpr (sizeof(U) == 8) {
+        return __builtin_bit_cast(U, __hip_atomic_load((unsigned long long*)a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    } else if constexpr (sizeof(U) == 4) {
+        return __builtin_bit_cast(U, __hip_atomic_load((unsigned int*)a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    } else {
+        static_assert(sizeof(U) == 1, "coherent loads of 1, 4 or 8 bytes");
+        return __builtin_bit_cast(U, __hip_atomic_load((unsigned char*)a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+}
+
+// A buffer that the persistent kernels write while they run (state, derivative slots, spectra,
+// partial sums, sparse-update fields): indexing loads through ldc, so no read of such a buffer can
+// bypass it. Stores go through st_shared (write-through) on .p. Same layout as T*.
+template <typename T>
+struct G {
+    T* p;
+    G() = default;
+    __host__ __device__ constexpr G(T* q) : p(q) {}
+    template <typename U, typename = typename std::enable_if<std::is_same<const U, T>::value>::type>
+    __host__ __device__ constexpr G(G<U> o) : p(o.p) {}
+    __device__ __forceinline__ typename std::remove_const<T>::type operator[](long i) const { return ldc(p + i); }
+    __host__ __device__ G operator+(long i) const { return G(p + i); }
+    __host__ __device__ explicit operator bool() const { return p != nullptr; }
+};
 
 constexpr int NT = FI_NT;
 // the canonical workgroup width of every block reduction: a build with fewer threads gives each
@@ -108,10 +143,10 @@ struct KP {
     const double* ss_X;
     // state and fields (float unless noted); U buffer b at Ub + b*Nl, K slot k at Kb + k*Nl
     long Nl;
-    double* Ub;
-    float* Kb;
-    float* kcorner;  // [4 slots][4 corners], raw
-    float2* specb;   // two spectrum buffers [2][nhx][ny]; an RHS call writes buffer (call count & 1)
+    G<double> Ub;
+    G<float> Kb;
+    G<float> kcorner;  // [4 slots][4 corners], raw
+    G<float2> specb;   // two spectrum buffers [2][nhx][ny]; an RHS call writes buffer (call count & 1)
     long specN;      // nhx * ny
     const float* mult;
     const float* D;
@@ -125,29 +160,29 @@ struct KP {
     const float* z_ss_ref;
     const float* ue_ref;
     float* H_ext;
-    float* ue;
-    float* dz_ss;
-    float* z_ss;
-    float* zoff;
-    float* zoff_trig;
-    float* LW;
-    float* LT;
-    float* H_water;
-    unsigned char* mg;
-    unsigned char* mo;
-    float* sig_load_prev;
-    float* sig_geo_prev;
-    float2* pA;
-    float2* pB;
-    float2* oA;
-    float2* oB;
-    float* ue_raw;
-    float* dz_raw;
-    float* ext_corners;  // [2][4]
+    G<float> ue;
+    G<float> dz_ss;
+    G<float> z_ss;
+    G<float> zoff;
+    G<float> zoff_trig;
+    G<float> LW;
+    G<float> LT;
+    G<float> H_water;
+    G<unsigned char> mg;
+    G<unsigned char> mo;
+    G<float> sig_load_prev;
+    G<float> sig_geo_prev;
+    G<float2> pA;
+    G<float2> pB;
+    G<float2> oA;
+    G<float2> oB;
+    G<float> ue_raw;
+    G<float> dz_raw;
+    G<float> ext_corners;  // [2][4]
     const double2* ge_hat;
     const double2* gg_hat;
-    double* part;    // [G][4] error moments
-    double* vpart;   // [G][4] volume partial sums
+    G<double> part;    // [G][4] error moments
+    G<double> vpart;   // [G][4] volume partial sums
     FftPlan px, py, pfx, pfy;
     const float2* tw32;
     Ctl* ctl;
@@ -167,7 +202,7 @@ struct KP {
     int elra;
     const double2* gv_hat;
     const float* inv_tau;
-    float* ueq;
+    G<float> ueq;
     // implicit step of a Maxwell mantle under a rigid or laterally constant lithosphere: the
     // even parts of (a - 1) and b of u <- a u + b F on the n x n spectrum, [nhx][ny], scaled by
     // 1/(nx ny); dt_min (yr) is the integrator step

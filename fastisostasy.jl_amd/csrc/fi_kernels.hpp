@@ -41,7 +41,9 @@ __device__ __forceinline__ bool team_sync(const Team& tm, unsigned& gen, int* s_
     // Every store that another workgroup reads after this barrier is a write-through agent-scope
     // store (st_shared). Waiting until each wave's stores have completed therefore publishes
     // them, and the arrival and release flags need no L2 writeback (release fence) of their own.
-    // The acquire side keeps its L2 invalidate: the data are then read with plain loads.
+    // Every read of such data is a coherent load (G<T>, ldc), so the acquire side needs no L2
+    // invalidate either: a workgroup-scope acquire keeps the compiler from moving those loads
+    // above the flag wait.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -57,7 +59,7 @@ __device__ __forceinline__ bool team_sync(const Team& tm, unsigned& gen, int* s_
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         __syncthreads();
         // publish the generation into every workgroup's own release slot
         for (int w = 1 + (int)threadIdx.x; w < tm.size; w += NT)
@@ -75,7 +77,7 @@ __device__ __forceinline__ bool team_sync(const Team& tm, unsigned& gen, int* s_
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         *s_flag = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
@@ -187,7 +189,7 @@ __device__ __forceinline__ double hab(double H, double zd, double rsi) {
     return fmax(H + fmin(zd, 0.0) * rsi, 0.0);
 }
 
-__device__ __forceinline__ const float* kslot(const KP& p, int k) { return p.Kb + (long)k * p.Nl; }
+__device__ __forceinline__ G<float> kslot(const KP& p, int k) { return p.Kb + (long)k * p.Nl; }
 
 // t_computation! (src/simulation.jl:46-52), called by thread 0 at the end of RHS call k (0-based
 // since fi_init): workgroup 0 of the team stores (t, wall clock) at index k; the host keeps the
@@ -198,8 +200,8 @@ __device__ __forceinline__ void timer_push(const KP& p, const Team& tm, long lon
         p.hist_clk[k] = wall_clock64();
     }
 }
-__device__ __forceinline__ float2* specbuf(const KP& p, int b) { return p.specb + (long)b * p.specN; }
-__device__ __forceinline__ double* ubuf(const KP& p, int b) { return p.Ub + (long)b * p.Nl; }
+__device__ __forceinline__ G<float2> specbuf(const KP& p, int b) { return p.specb + (long)b * p.specN; }
+__device__ __forceinline__ G<double> ubuf(const KP& p, int b) { return p.Ub + (long)b * p.Nl; }
 
 // sum of the global derivative terms of a stage at cell c
 __device__ __forceinline__ double kterms(const KP& p, const StageDesc& st, long c) {
@@ -218,7 +220,7 @@ __device__ __forceinline__ double u_raw(const KP& p, const StageDesc& st, long c
 
 // corner-BC offset of a raw derivative slot
 __device__ __forceinline__ double k_offset(const KP& p, int slot) {
-    const float* q = p.kcorner + 4 * slot;
+    const G<float> q = p.kcorner + 4 * slot;
     const double a = 0.25 * (double)q[0], b = 0.25 * (double)q[1], c = 0.25 * (double)q[2], d = 0.25 * (double)q[3];
     return ((a + b) + (c + d)) - p.vbc_x;
 }
@@ -233,7 +235,7 @@ __device__ __forceinline__ double k_offset(const KP& p, int slot) {
 //    (w = 1 for the DC and Nyquist terms, 2 otherwise);
 //  * the apply_bc!(u, CornerBC) offset of the stage displacement (src/boundary_conditions.jl:
 //    144-148): the mean of its four corners minus the border value.
-__device__ __forceinline__ void stage_preamble(const KP& p, const StageDesc& st, const float2* spec, const float2* twx,
+__device__ __forceinline__ void stage_preamble(const KP& p, const StageDesc& st, const G<float2> spec, const float2* twx,
                                                double* red, IceT& it_out, KC4& kc_out, double& uoff_out) {
     __shared__ IceT sh_it;
     __shared__ double sh_uc[4], sh_u0[4];
@@ -414,7 +416,7 @@ __device__ __forceinline__ double dxy(const V* F, int r0, int j, int i, int nx, 
 
 // Hermitian-extended pair value Z[k] = Aext[k] + i Bext[k] of spectral rows ja, jb (jb may be a
 // phantom row) for the inverse real transform
-__device__ __forceinline__ float2 c2r_pair_value(const KP& p, const float2* spec, int k, int ja, int jb) {
+__device__ __forceinline__ float2 c2r_pair_value(const KP& p, const G<float2> spec, int k, int ja, int jb) {
     const int nx = p.nx, ny = p.ny;
     float2 A, B;
     const int kk = (k < p.nhx) ? k : nx - k;
@@ -454,6 +456,11 @@ __device__ __forceinline__ float2 c2r_pair_value(const KP& p, const float2* spec
 // slots, the stage displacement, partial sums). Relaxed agent-scope atomic stores write through
 // the L2 (sc1), so the barrier's release has no dirty lines of theirs to write back; the
 // barrier's release/acquire pair orders them as before. Vector stores only.
+// address of element i of a buffer, for the write-through stores below
+template <typename T>
+__device__ __forceinline__ T* addr(G<T> b, long i) { return b.p + i; }
+template <typename T>
+__device__ __forceinline__ T* addr(T* b, long i) { return b + i; }
 __device__ __forceinline__ void st_shared(float* a, float v) {
     __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -491,12 +498,12 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
     float* skin = (float*)(sg + nx);            // 6 rows of the spectral-stage derivative
     // the inverse transforms of the halo pairs use the moment region as scratch
     float2* ca = (float2*)sMxx;
-    const float2* spin = specbuf(p, st.sb_in);
-    float2* spout = specbuf(p, st.sb_out);
+    const G<float2> spin = specbuf(p, st.sb_in);
+    G<float2> spout = specbuf(p, st.sb_out);
     const int maxpairs = (int)((12 * nx * sizeof(MT)) / (16 * nx));
     const double nu = p.nu;
     const bool wsig = st.write_sigma != 0;
-    const double* U0 = ubuf(p, st.ubase);
+    const G<double> U0 = ubuf(p, st.ubase);
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item, je = min(j0 + 2, ny);
         const int jm0 = max(0, min(j0 - 1, ny - 3)), jm1 = min(ny, max(je + 1, 3));
@@ -524,7 +531,7 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
         FI_SUBMARK(0);
         // previous stage's derivative on rows [ju0, ju1): inverse real transforms of row pairs
         if (st.kin >= 0) {
-            float* Kin = p.Kb + (long)st.kin * p.Nl;
+            G<float> Kin = p.Kb + (long)st.kin * p.Nl;
             const int p0 = ju0 & ~1;
             const int np = (ju1 - p0 + 1) / 2;
             for (int q0 = 0; q0 < np; q0 += maxpairs) {
@@ -558,8 +565,8 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                     if (ci >= 0) v = ci == 0 ? kc.v0 : ci == 1 ? kc.v1 : ci == 2 ? kc.v2 : kc.v3;
                     skin[(j - ju0) * nx + i] = v;
                     if (j >= j0 && j < je) {
-                        st_shared(&Kin[i + (long)j * nx], v);
-                        if (ci >= 0) st_shared(&p.kcorner[4 * st.kin + ci], v);
+                        st_shared(addr(Kin, i + (long)j * nx), v);
+                        if (ci >= 0) st_shared(addr(p.kcorner, 4 * st.kin + ci), v);
                     }
                 }
             }
@@ -597,7 +604,7 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             su[idx] = u;
             const int r = fdivi(idx, inv_nx);
             const int j = ju0 + r;
-            if (st.write_u >= 0 && j >= j0 && j < je) st_shared(&ubuf(p, st.write_u)[idx - r * nx + (long)j * nx], u);
+            if (st.write_u >= 0 && j >= j0 && j < je) st_shared(addr(ubuf(p, st.write_u), idx - r * nx + (long)j * nx), u);
         }
         __syncthreads();
         FI_SUBMARK(4);
@@ -640,8 +647,8 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
                     f = (float)(rhs * (double)fr[q].i2e);
                     if (wsig) {
-                        st_shared(&p.sig_load_prev[c], (float)s_load);
-                        st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
+                        st_shared(addr(p.sig_load_prev, c), (float)s_load);
+                        st_shared(addr(p.sig_geo_prev, c), (float)(p.compressible_column
                                                         ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
                                                         : s_full));
                     }
@@ -660,8 +667,8 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            st_shared(&spout[(long)kx * ny + j0], X0);
-            if (j0 + 1 < ny) st_shared(&spout[(long)kx * ny + j0 + 1], X1);
+            st_shared(addr(spout, (long)kx * ny + j0), X0);
+            if (j0 + 1 < ny) st_shared(addr(spout, (long)kx * ny + j0 + 1), X1);
         }
         __syncthreads();
         FI_SUBMARK(7);
@@ -697,12 +704,12 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
     float* sM = sI2e + 2 * nx;
     float* sKi = sM + 2 * nx;                   // 6 rows of 1/K (map distortion only)
     float2* ca = (float2*)sMxx;                 // inverse-transform scratch (moment region)
-    const float2* spin = specbuf(p, st.sb_in);
-    float2* spout = specbuf(p, st.sb_out);
+    const G<float2> spin = specbuf(p, st.sb_in);
+    G<float2> spout = specbuf(p, st.sb_out);
     const double nu = p.nu;
     const bool wsig = st.write_sigma != 0;
-    const double* U0 = ubuf(p, st.ubase);
-    const float* zoff = st.trig ? p.zoff_trig : p.zoff;
+    const G<double> U0 = ubuf(p, st.ubase);
+    const G<float> zoff = st.trig ? p.zoff_trig : p.zoff;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item, je = min(j0 + 2, ny);
         const int jm0 = max(0, min(j0 - 1, ny - 3)), jm1 = min(ny, max(je + 1, 3));
@@ -800,7 +807,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
         FI_SUBMARK(0);
         // previous stage's derivative on rows [ju0, ju1): inverse real transforms of the halo pairs
         if (np > 0) {
-            float* Kin = p.Kb + (long)st.kin * p.Nl;
+            G<float> Kin = p.Kb + (long)st.kin * p.Nl;
             float2* res = fft_lds<float2>(ca, ca + totC, np, nx, p.px, twx, +1);
             FI_SUBMARK(2);
             for (int idx = threadIdx.x; idx < 2 * totC; idx += NT) {
@@ -813,8 +820,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                 if (ci >= 0) v = ci == 0 ? kc.v0 : ci == 1 ? kc.v1 : ci == 2 ? kc.v2 : kc.v3;
                 skin[(j - ju0) * nx + i] = v;
                 if (j >= j0 && j < je) {
-                    st_shared(&Kin[i + (long)j * nx], v);
-                    if (ci >= 0) st_shared(&p.kcorner[4 * st.kin + ci], v);
+                    st_shared(addr(Kin, i + (long)j * nx), v);
+                    if (ci >= 0) st_shared(addr(p.kcorner, 4 * st.kin + ci), v);
                 }
             }
             __syncthreads();
@@ -864,9 +871,9 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
 #pragma unroll
                 for (int o = 0; o < NO; ++o) uo[o] = cv[o - JU];
                 if (st.write_u >= 0) {
-                    double* un = ubuf(p, st.write_u);
+                    G<double> un = ubuf(p, st.write_u);
 #pragma unroll
-                    for (int o = 0; o < NO; ++o) st_shared(&un[i + (long)(j0 + o) * nx], uo[o]);
+                    for (int o = 0; o < NO; ++o) st_shared(addr(un, i + (long)(j0 + o) * nx), uo[o]);
                 }
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
@@ -931,8 +938,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
                     f[o] = (float)(rhs * (double)sI2e[idx]);
                     if (wsig) {
-                        st_shared(&p.sig_load_prev[cc], (float)s_load);
-                        st_shared(&p.sig_geo_prev[cc], (float)(p.compressible_column
+                        st_shared(addr(p.sig_load_prev, cc), (float)s_load);
+                        st_shared(addr(p.sig_geo_prev, cc), (float)(p.compressible_column
                                                          ? s_load + msk * (s_mant - p.rho_um * (double)p.u_ref[cc])
                                                          : s_full));
                     }
@@ -974,7 +981,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                 su[idx] = u;
                 const int r = fdivi(idx, inv_nx);
                 const int j = ju0 + r;
-                if (st.write_u >= 0 && j >= j0 && j < je) st_shared(&ubuf(p, st.write_u)[idx - r * nx + (long)j * nx], u);
+                if (st.write_u >= 0 && j >= j0 && j < je) st_shared(addr(ubuf(p, st.write_u), idx - r * nx + (long)j * nx), u);
             }
             __syncthreads();
             FI_SUBMARK(4);
@@ -1010,8 +1017,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
                     const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
                     f = (float)(rhs * (double)sI2e[idx]);
                     if (wsig) {
-                        st_shared(&p.sig_load_prev[c], (float)s_load);
-                        st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
+                        st_shared(addr(p.sig_load_prev, c), (float)s_load);
+                        st_shared(addr(p.sig_geo_prev, c), (float)(p.compressible_column
                                                         ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
                                                         : s_full));
                     }
@@ -1031,8 +1038,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            st_shared(&spout[(long)kx * ny + j0], X0);
-            if (j0 + 1 < ny) st_shared(&spout[(long)kx * ny + j0 + 1], X1);
+            st_shared(addr(spout, (long)kx * ny + j0), X0);
+            if (j0 + 1 < ny) st_shared(addr(spout, (long)kx * ny + j0 + 1), X1);
         }
         __syncthreads();
         FI_SUBMARK(7);
@@ -1042,7 +1049,7 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
 // ------------------------------------------------------------------------------------------
 // columns phase: FFT_y, * spy/(nx ny)/(R kappa), inverse FFT_y
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* spec, const float2* twy, char* smem) {
+__device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, G<float2> spec, const float2* twy, char* smem) {
     const int ny = p.ny, nb = p.colnb;
     float2* sa = (float2*)smem;
     float2* sb = sa + nb * ny;
@@ -1073,7 +1080,7 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* 
             __syncthreads();
             float2* o = (r == sa) ? sb : sa;
             float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
-            for (int idx = threadIdx.x; idx < tot; idx += NT) st_shared(&spec[base + idx], r2[idx]);
+            for (int idx = threadIdx.x; idx < tot; idx += NT) st_shared(addr(spec, base + idx), r2[idx]);
             __syncthreads();
             continue;
         }
@@ -1088,7 +1095,7 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* 
         __syncthreads();
         float2* o = (r == sa) ? sb : sa;
         float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) st_shared(&spec[base + idx], r2[idx]);
+        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) st_shared(addr(spec, base + idx), r2[idx]);
         __syncthreads();
     }
 }
@@ -1098,12 +1105,12 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, float2* 
 // moments, one partial per row pair in p.part (so the step-size decision does not depend on how
 // many workgroups share the work)
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const StageDesc& st, const float2* spec, int slot,
+__device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const StageDesc& st, const G<float2> spec, int slot,
                                           const float2* twx, char* smem, double* red) {
     const int nx = p.nx, ny = p.ny;
     float2* sf = (float2*)smem;
     float2* sg = sf + nx;
-    float* K = p.Kb + (long)slot * p.Nl;
+    G<float> K = p.Kb + (long)slot * p.Nl;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item;
         const bool pre = st.final_err && 2 * nx <= 2 * NT;
@@ -1137,9 +1144,9 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
             const float v = r ? res[i].y : res[i].x;
-            st_shared(&K[c], v);
+            st_shared(addr(K, c), v);
             const int ci = corner_index(p, i, j);
-            if (ci >= 0) st_shared(&p.kcorner[4 * slot + ci], v);
+            if (ci >= 0) st_shared(addr(p.kcorner, 4 * slot + ci), v);
             if (st.final_err) {
                 double k1, k2, k3, un, up;
                 if (pre) {
@@ -1166,9 +1173,9 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
         if (st.final_err) {
             block_sum<3>(red, accv);
             if (threadIdx.x == 0) {
-                st_shared(&p.part[4 * item + 0], accv[0][0]);
-                st_shared(&p.part[4 * item + 1], accv[0][1]);
-                st_shared(&p.part[4 * item + 2], accv[0][2]);
+                st_shared(addr(p.part, 4 * item + 0), accv[0][0]);
+                st_shared(addr(p.part, 4 * item + 1), accv[0][1]);
+                st_shared(addr(p.part, 4 * item + 2), accv[0][2]);
             }
         }
         __syncthreads();
@@ -1214,8 +1221,8 @@ __device__ __forceinline__ void phase_sp_rows(const KP& p, const Team& tm, char*
             A.y = 0.5f * (Z.y - Zm.y);
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
-            st_shared(&p.pA[(long)kx * ny + j], A);
-            st_shared(&p.pB[(long)kx * ny + j], B);
+            st_shared(addr(p.pA, (long)kx * ny + j), A);
+            st_shared(addr(p.pB, (long)kx * ny + j), B);
         }
         __syncthreads();
     }
@@ -1258,7 +1265,7 @@ __device__ __forceinline__ void phase_sp_cols(const KP& p, const Team& tm, char*
         float2* r2 = fft_lds<float2>(r, o, 2, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < 2 * no; idx += NT) {
             const int s = idx / no, jj = idx - s * no;
-            st_shared(&(s ? p.oB : p.oA)[(long)kx * no + jj], r2[s * nfy + sp_row_of(p, jj)]);
+            st_shared(addr((s ? p.oB : p.oA), (long)kx * no + jj), r2[s * nfy + sp_row_of(p, jj)]);
         }
         __syncthreads();
     }
@@ -1296,17 +1303,17 @@ __device__ __forceinline__ void phase_sp_c2r(const KP& p, const Team& tm, char* 
         if (jj == 0 || jj == no - 1) {
             if (threadIdx.x == 0) {
                 const int q = (jj == 0) ? 0 : 2;
-                st_shared(&p.ext_corners[q], res[0].x);
-                st_shared(&p.ext_corners[q + 1], res[2 * nx - 2].x);
-                st_shared(&p.ext_corners[4 + q], res[0].y);
-                st_shared(&p.ext_corners[4 + q + 1], res[2 * nx - 2].y);
+                st_shared(addr(p.ext_corners, q), res[0].x);
+                st_shared(addr(p.ext_corners, q + 1), res[2 * nx - 2].x);
+                st_shared(addr(p.ext_corners, 4 + q), res[0].y);
+                st_shared(addr(p.ext_corners, 4 + q + 1), res[2 * nx - 2].y);
             }
         } else {
             const int j = jj - 1;
             for (int ii = threadIdx.x; ii < nx; ii += NT) {
                 const float2 v = res[p.i1 - 1 + ii];
-                st_shared(&p.ue_raw[ii + (long)j * nx], v.x);
-                st_shared(&p.dz_raw[ii + (long)j * nx], v.y);
+                st_shared(addr(p.ue_raw, ii + (long)j * nx), v.x);
+                st_shared(addr(p.dz_raw, ii + (long)j * nx), v.y);
             }
         }
         __syncthreads();
@@ -1340,9 +1347,9 @@ __device__ __forceinline__ void phase_sp_volumes(const KP& p, const Team& tm, co
         }
         block_sum<3>(red, accv);
         if (threadIdx.x == 0) {
-            st_shared(&p.vpart[4 * item + 0], accv[0][0]);
-            st_shared(&p.vpart[4 * item + 1], accv[0][1]);
-            st_shared(&p.vpart[4 * item + 2], accv[0][2]);
+            st_shared(addr(p.vpart, 4 * item + 0), accv[0][0]);
+            st_shared(addr(p.vpart, 4 * item + 1), accv[0][1]);
+            st_shared(addr(p.vpart, 4 * item + 2), accv[0][2]);
         }
     }
 }
@@ -1457,7 +1464,7 @@ __device__ __forceinline__ SsBracket ss_bracket(const KP& p, double t) {
 // pointwise sparse diagnostics with the updated barystatic sea level z_bsl
 __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, const StageDesc& st, double uoff,
                                                const IceT it, double z_bsl) {
-    const float* ec = p.ext_corners;
+    const G<float> ec = p.ext_corners;
     SsBracket sb{0, 0, 1, 0.0, 1.0};
     if (p.imposed_ss) sb = ss_bracket(p, st.t);
     const double oe = ((0.25 * ec[0] + 0.25 * ec[1]) + (0.25 * ec[2] + 0.25 * ec[3])) - p.ebc_x;
@@ -1490,16 +1497,16 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
         const double m = p.mask[c] ? 1.0 : 0.0;
         const double sw = p.interactive ? p.rho_seawater * (zss_new - (double)p.z_ss_ref[c]) * (moc ? 1.0 : 0.0) * m : 0.0;
         const double ca_litho = p.rho_litho * (ue_new - (double)p.ue_ref[c]);
-        st_shared(&p.ue[c], (float)ue_new);
-        st_shared(&p.dz_ss[c], (float)dz_new);
-        st_shared(&p.z_ss[c], (float)zss_new);
-        st_shared(&p.mg[c], (unsigned char)mgc);
-        st_shared(&p.mo[c], (unsigned char)moc);
-        st_shared(&p.H_water[c], (float)hw);
-        st_shared(&p.LW[c], (float)sw);
-        st_shared(&p.LT[c], (float)(ca_litho - p.rho_um * (double)p.u_ref[c]));
-        st_shared(&p.zoff[c], (float)((double)p.z_b_ref[c] + ue_new - zss_new));
-        st_shared(&p.zoff_trig[c], (float)((double)p.z_b_ref[c] + ue_old - zss_new));
+        st_shared(addr(p.ue, c), (float)ue_new);
+        st_shared(addr(p.dz_ss, c), (float)dz_new);
+        st_shared(addr(p.z_ss, c), (float)zss_new);
+        st_shared(addr(p.mg, c), (unsigned char)mgc);
+        st_shared(addr(p.mo, c), (unsigned char)moc);
+        st_shared(addr(p.H_water, c), (float)hw);
+        st_shared(addr(p.LW, c), (float)sw);
+        st_shared(addr(p.LT, c), (float)(ca_litho - p.rho_um * (double)p.u_ref[c]));
+        st_shared(addr(p.zoff, c), (float)((double)p.z_b_ref[c] + ue_new - zss_new));
+        st_shared(addr(p.zoff_trig, c), (float)((double)p.z_b_ref[c] + ue_old - zss_new));
       }
     }
 }
@@ -1574,7 +1581,7 @@ __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, con
                     if (j >= ny) continue;
                     const long c = i + (long)j * nx;
                     const double u = u_raw(p, st, c) - uoff;
-                    if (st.write_u >= 0) st_shared(&ubuf(p, st.write_u)[c], u);
+                    if (st.write_u >= 0) st_shared(addr(ubuf(p, st.write_u), c), u);
                     const double m = p.mask[c] ? 1.0 : 0.0;
                     const double zo = st.trig ? p.zoff_trig[c] : p.zoff[c];
                     const double Haf = hab(ice_at(p, it, c), zo + u, p.rho_sw_ice);
@@ -1589,8 +1596,8 @@ __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, con
                     if (wsig) {
                         const double s_mant = p.rho_um * u;
                         const double s_full = s_load + m * (lt + s_mant);
-                        st_shared(&p.sig_load_prev[c], (float)s_load);
-                        st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
+                        st_shared(addr(p.sig_load_prev, c), (float)s_load);
+                        st_shared(addr(p.sig_geo_prev, c), (float)(p.compressible_column
                                                         ? s_load + m * (s_mant - p.rho_um * uref)
                                                         : s_full));
                     }
@@ -1607,8 +1614,8 @@ __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, con
             A.y = 0.5f * (Z.y - Zm.y);
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
-            st_shared(&p.pA[(long)kx * ny + j0], A);
-            if (j0 + 1 < ny) st_shared(&p.pA[(long)kx * ny + j0 + 1], B);
+            st_shared(addr(p.pA, (long)kx * ny + j0), A);
+            if (j0 + 1 < ny) st_shared(addr(p.pA, (long)kx * ny + j0 + 1), B);
         }
         __syncthreads();
     }
@@ -1643,7 +1650,7 @@ __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, cha
         float2* r2 = fft_lds<float2>(r, o, nb, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < nb * ny; idx += NT) {
             const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
-            st_shared(&p.oA[(long)(kx0 + s) * ny + j], r2[s * nfy + (p.j1 - 1) + j]);
+            st_shared(addr(p.oA, (long)(kx0 + s) * ny + j), r2[s * nfy + (p.j1 - 1) + j]);
         }
         __syncthreads();
     }
@@ -1686,7 +1693,7 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
     block_sum<4>(red, vs);
     const double* v = vs[0];
     const double off = ((0.25 * v[0] + 0.25 * v[1]) + (0.25 * v[2] + 0.25 * v[3])) - p.vbc_x;
-    float* K = p.Kb + (long)st.kout * p.Nl;
+    G<float> K = p.Kb + (long)st.kout * p.Nl;
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item;
         const bool two = j0 + 1 < ny;
@@ -1722,8 +1729,8 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
             // the stage displacement: stored by the rows phase when this call keeps it
             const double u = st.write_u >= 0 ? ubuf(p, st.write_u)[c] : u_raw(p, st, c) - uoff;
             const float dv = (float)((ueq - u) * (double)p.inv_tau[c]);
-            st_shared(&K[c], dv);
-            st_shared(&p.ueq[c], (float)ueq);
+            st_shared(addr(K, c), dv);
+            st_shared(addr(p.ueq, c), (float)ueq);
             if (st.final_err) {
                 const double k1 = kslot(p, st.e1)[c], k2 = kslot(p, st.e2)[c], k3 = kslot(p, st.e3)[c];
                 const double un = ubuf(p, st.u_prev)[c];
@@ -1737,9 +1744,9 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
         if (st.final_err) {
             block_sum<3>(red, accv);
             if (threadIdx.x == 0) {
-                st_shared(&p.part[4 * item + 0], accv[0][0]);
-                st_shared(&p.part[4 * item + 1], accv[0][1]);
-                st_shared(&p.part[4 * item + 2], accv[0][2]);
+                st_shared(addr(p.part, 4 * item + 0), accv[0][0]);
+                st_shared(addr(p.part, 4 * item + 1), accv[0][1]);
+                st_shared(addr(p.part, 4 * item + 2), accv[0][2]);
             }
         }
         __syncthreads();
@@ -1803,9 +1810,9 @@ __device__ __forceinline__ void phase_imp_rows(const KP& p, const Team& tm, cons
     float2* sa = (float2*)smem;   // [2][nx]: u rows, F rows
     float2* sb = sa + 2 * nx;
     const bool wsig = st.write_sigma != 0;
-    const double* U0 = ubuf(p, st.ubase);
-    float2* specU = specbuf(p, 0);
-    float2* specF = specbuf(p, 1);
+    const G<double> U0 = ubuf(p, st.ubase);
+    G<float2> specU = specbuf(p, 0);
+    G<float2> specF = specbuf(p, 1);
     for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
         const int j0 = 2 * item;
         for (int i = threadIdx.x; i < nx; i += NT) {
@@ -1835,8 +1842,8 @@ __device__ __forceinline__ void phase_imp_rows(const KP& p, const Team& tm, cons
                 if (wsig) {
                     const double s_mant = p.rho_um * u;
                     const double s_full = s_load + m * (lt + s_mant);
-                    st_shared(&p.sig_load_prev[c], (float)s_load);
-                    st_shared(&p.sig_geo_prev[c], (float)(p.compressible_column
+                    st_shared(addr(p.sig_load_prev, c), (float)s_load);
+                    st_shared(addr(p.sig_geo_prev, c), (float)(p.compressible_column
                                                     ? s_load + m * (s_mant - p.rho_um * uref)
                                                     : s_full));
                 }
@@ -1854,9 +1861,9 @@ __device__ __forceinline__ void phase_imp_rows(const KP& p, const Team& tm, cons
             A.y = 0.5f * (Z.y - Zm.y);
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
-            float2* spec = s ? specF : specU;
-            st_shared(&spec[(long)kx * ny + j0], A);
-            if (j0 + 1 < ny) st_shared(&spec[(long)kx * ny + j0 + 1], B);
+            G<float2> spec = s ? specF : specU;
+            st_shared(addr(spec, (long)kx * ny + j0), A);
+            if (j0 + 1 < ny) st_shared(addr(spec, (long)kx * ny + j0 + 1), B);
         }
         __syncthreads();
     }
@@ -1866,8 +1873,8 @@ __device__ __forceinline__ void phase_imp_cols(const KP& p, const Team& tm, cons
     const int ny = p.ny;
     float2* sa = (float2*)smem;   // [2][ny]: U column, F column
     float2* sb = sa + 2 * ny;
-    float2* specU = specbuf(p, 0);
-    const float2* specF = specbuf(p, 1);
+    G<float2> specU = specbuf(p, 0);
+    const G<float2> specF = specbuf(p, 1);
     for (int kx = tm.rank; kx < p.nhx; kx += tm.size) {
         for (int idx = threadIdx.x; idx < 2 * ny; idx += NT) {
             const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
@@ -1883,7 +1890,7 @@ __device__ __forceinline__ void phase_imp_cols(const KP& p, const Team& tm, cons
         }
         __syncthreads();
         float2* r2 = fft_lds<float2>(o, r, 1, ny, p.py, twy, +1);
-        for (int j = threadIdx.x; j < ny; j += NT) st_shared(&specU[(long)kx * ny + j], r2[j]);
+        for (int j = threadIdx.x; j < ny; j += NT) st_shared(addr(specU, (long)kx * ny + j), r2[j]);
         __syncthreads();
     }
 }
@@ -1892,9 +1899,9 @@ __device__ __forceinline__ void phase_imp_c2r(const KP& p, const Team& tm, const
                                               const double* ucorner, const float2* twx, char* smem, double* red) {
     const int nx = p.nx, ny = p.ny;
     const bool even = (nx % 2) == 0;
-    const float2* spec = specbuf(p, 0);
-    const double* U0 = ubuf(p, st.ubase);
-    double* Uw = ubuf(p, st.write_u);
+    const G<float2> spec = specbuf(p, 0);
+    const G<double> U0 = ubuf(p, st.ubase);
+    G<double> Uw = ubuf(p, st.write_u);
     float2* sf = (float2*)smem;
     float2* sg = sf + nx;
     // increment at the four corners: x(0, j) = sum_k w_k Re X_k, x(nx-1, j) = sum_k w_k Re(X_k e^{-2 pi i k/nx})
@@ -1934,7 +1941,7 @@ __device__ __forceinline__ void phase_imp_c2r(const KP& p, const Team& tm, const
             const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
             if (j >= ny) continue;
             const long c = i + (long)j * nx;
-            st_shared(&Uw[c], (U0[c] - uoff) + (double)(r ? res[i].y : res[i].x) - off);
+            st_shared(addr(Uw, c), (U0[c] - uoff) + (double)(r ? res[i].y : res[i].x) - off);
         }
         __syncthreads();
     }
@@ -1963,7 +1970,7 @@ __device__ __forceinline__ bool implicit_call(const KP& p, const Team& tm, Ctl& 
         if (sparse_only) return true;
     }
     // the stage's corners (with their BC), before the last phase overwrites u in place
-    const double* U0 = ubuf(p, st.ubase);
+    const G<double> U0 = ubuf(p, st.ubase);
     const long n1 = p.nx - 1, n2 = (long)(p.ny - 1) * p.nx;
     const double ucorner[4] = {U0[0] - uoff, U0[n1] - uoff, U0[n2] - uoff, U0[n2 + n1] - uoff};
     st.write_sigma = (st.force_sigma || p.write_sigma_always ||
