@@ -1058,7 +1058,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     HIPCHECK(c, hipMemset(c->K.p, 0, 4 * N * sizeof(float)));
     HIPCHECK(c, c->kcorner.alloc(16));
     HIPCHECK(c, hipMemset(c->kcorner.p, 0, 16 * sizeof(float)));
-    HIPCHECK(c, c->spec.alloc(2 * (size_t)c->nhx * ny));
+    // two spectrum buffers, each with its columns padded to a multiple of 4 (blocked [kx/4][j][4])
+    HIPCHECK(c, c->spec.alloc(2 * (size_t)((c->nhx + 3) & ~3) * ny));
     if (upload(c, c->H_af_ref_d, to_f(c->H_af_ref)) || upload(c, c->u_ref_d, to_f(c->u_ref)) ||
         upload(c, c->z_b_ref_d, to_f(c->z_b_ref)) || upload(c, c->z_ss_ref_d, to_f(c->z_ss_ref)) ||
         upload(c, c->ue_ref_d, to_f(c->ue_ref)) || upload(c, c->H_ext, to_f(c->H_ext_host.empty() ? c->H_ice_ref : c->H_ext_host)) ||
@@ -1209,7 +1210,7 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.Kb = c->K.p;
     kp.kcorner = c->kcorner.p;
     kp.specb = c->spec.p;
-    kp.specN = (long)c->nhx * ny;
+    kp.specN = (long)((c->nhx + 3) & ~3) * ny;
     kp.mult = c->mult.p;
     kp.D = c->Df.p;
     kp.inv2eta = c->inv2eta.p;

@@ -201,6 +201,15 @@ __device__ __forceinline__ void timer_push(const KP& p, const Team& tm, long lon
     }
 }
 __device__ __forceinline__ G<float2> specbuf(const KP& p, int b) { return p.specb + (long)b * p.specN; }
+// Element (kx, j) of a spectrum buffer, blocked [kx/4][j][4]: the four columns of a block are
+// interleaved per row. A row pair's half spectrum is then 64 contiguous bytes per block (the rows
+// phase's stores and its halo reads touch 16x fewer cache lines than in a [kx][j] layout) and a
+// column is read and written with a 32-byte stride (columns phase). Every line of the buffer
+// crosses the device's coherence point on each read (coherent loads), so lines, not bytes, are
+// the traffic. The implicit step's two spectra use the same layout.
+__device__ __forceinline__ long sidx(const KP& p, int kx, int j) {
+    return (((long)(kx >> 2) * p.ny + j) << 2) + (kx & 3);
+}
 __device__ __forceinline__ G<double> ubuf(const KP& p, int b) { return p.Ub + (long)b * p.Nl; }
 
 // sum of the global derivative terms of a stage at cell c
@@ -262,8 +271,8 @@ __device__ __forceinline__ void stage_preamble(const KP& p, const StageDesc& st,
     const bool spec_on = st.kin >= 0;
     float2 sa = make_float2(0.f, 0.f), sb = make_float2(0.f, 0.f), tw = make_float2(1.f, 0.f);
     if (spec_on && tid < p.nhx) {
-        sa = spec[(long)tid * ny];
-        sb = spec[(long)tid * ny + ny - 1];
+        sa = spec[sidx(p, tid, 0)];
+        sb = spec[sidx(p, tid, ny - 1)];
         tw = twx[tid];
     }
     if (tid < 4) {
@@ -325,8 +334,8 @@ __device__ __forceinline__ void stage_preamble(const KP& p, const StageDesc& st,
                 if (kx >= p.nhx) break;
                 float2 a = sa, b = sb, tw_k = tw;
                 if (kx >= NT) {
-                    a = spec[(long)kx * ny];
-                    b = spec[(long)kx * ny + ny - 1];
+                    a = spec[sidx(p, kx, 0)];
+                    b = spec[sidx(p, kx, ny - 1)];
                     tw_k = twx[kx];
                 }
                 const bool nyq = even && kx == nx / 2;
@@ -420,8 +429,8 @@ __device__ __forceinline__ float2 c2r_pair_value(const KP& p, const G<float2> sp
     const int nx = p.nx, ny = p.ny;
     float2 A, B;
     const int kk = (k < p.nhx) ? k : nx - k;
-    A = spec[(long)kk * ny + ja];
-    if (jb < ny) B = spec[(long)kk * ny + jb];
+    A = spec[sidx(p, kk, ja)];
+    if (jb < ny) B = spec[sidx(p, kk, jb)];
     else { B.x = 0; B.y = 0; }
     if (k >= p.nhx) { A.y = -A.y; B.y = -B.y; }
     else if (k == 0 || ((nx % 2) == 0 && k == nx / 2)) { A.y = 0; B.y = 0; }
@@ -667,8 +676,8 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            st_shared(addr(spout, (long)kx * ny + j0), X0);
-            if (j0 + 1 < ny) st_shared(addr(spout, (long)kx * ny + j0 + 1), X1);
+            st_shared(addr(spout, sidx(p, kx, j0)), X0);
+            if (j0 + 1 < ny) st_shared(addr(spout, sidx(p, kx, j0 + 1)), X1);
         }
         __syncthreads();
         FI_SUBMARK(7);
@@ -1038,8 +1047,8 @@ __device__ __forceinline__ void phase_rows_lds(const KP& p, const Team& tm, cons
             X0.y = 0.5f * (Z.y - Zm.y);
             X1.x = 0.5f * (Z.y + Zm.y);
             X1.y = -0.5f * (Z.x - Zm.x);
-            st_shared(addr(spout, (long)kx * ny + j0), X0);
-            if (j0 + 1 < ny) st_shared(addr(spout, (long)kx * ny + j0 + 1), X1);
+            st_shared(addr(spout, sidx(p, kx, j0)), X0);
+            if (j0 + 1 < ny) st_shared(addr(spout, sidx(p, kx, j0 + 1)), X1);
         }
         __syncthreads();
         FI_SUBMARK(7);
@@ -1056,15 +1065,21 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, G<float2
     for (int item = tm.rank; item < p.ncolitems; item += tm.size) {
         const int kx0 = item * nb;
         const int nbb = min(nb, p.nhx - kx0);
-        const long base = (long)kx0 * ny;
+        const long base = (long)kx0 * ny;   // the multiplier table is [kx][ky]
         const int tot = nbb * ny;
+        const float inv_ny = 1.0f / (float)ny;
+        // spectrum element of column-phase index idx (column kx0 + idx / ny, row idx % ny)
+        auto se = [&](int idx) {
+            const int b = fdivi(idx, inv_ny);
+            return sidx(p, kx0 + b, idx - b * ny);
+        };
         if (tot <= 2 * NT) {
             // the multiplier is loaded with the column, in the same round
             float mv[2];
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int idx = min((int)threadIdx.x + q * NT, tot - 1);
-                sa[idx] = spec[base + idx];
+                sa[idx] = spec[se(idx)];
                 mv[q] = p.mult[base + idx];
             }
             __syncthreads();
@@ -1080,11 +1095,11 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, G<float2
             __syncthreads();
             float2* o = (r == sa) ? sb : sa;
             float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
-            for (int idx = threadIdx.x; idx < tot; idx += NT) st_shared(addr(spec, base + idx), r2[idx]);
+            for (int idx = threadIdx.x; idx < tot; idx += NT) st_shared(addr(spec, se(idx)), r2[idx]);
             __syncthreads();
             continue;
         }
-        for (int idx = threadIdx.x; idx < tot; idx += NT) sa[idx] = spec[base + idx];
+        for (int idx = threadIdx.x; idx < tot; idx += NT) sa[idx] = spec[se(idx)];
         __syncthreads();
         float2* r = fft_lds<float2>(sa, sb, nbb, ny, p.py, twy, -1);
         for (int idx = threadIdx.x; idx < tot; idx += NT) {
@@ -1095,7 +1110,7 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, G<float2
         __syncthreads();
         float2* o = (r == sa) ? sb : sa;
         float2* r2 = fft_lds<float2>(r, o, nbb, ny, p.py, twy, +1);
-        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) st_shared(addr(spec, base + idx), r2[idx]);
+        for (int idx = threadIdx.x; idx < nbb * ny; idx += NT) st_shared(addr(spec, se(idx)), r2[idx]);
         __syncthreads();
     }
 }
@@ -1862,8 +1877,8 @@ __device__ __forceinline__ void phase_imp_rows(const KP& p, const Team& tm, cons
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
             G<float2> spec = s ? specF : specU;
-            st_shared(addr(spec, (long)kx * ny + j0), A);
-            if (j0 + 1 < ny) st_shared(addr(spec, (long)kx * ny + j0 + 1), B);
+            st_shared(addr(spec, sidx(p, kx, j0)), A);
+            if (j0 + 1 < ny) st_shared(addr(spec, sidx(p, kx, j0 + 1)), B);
         }
         __syncthreads();
     }
@@ -1878,7 +1893,7 @@ __device__ __forceinline__ void phase_imp_cols(const KP& p, const Team& tm, cons
     for (int kx = tm.rank; kx < p.nhx; kx += tm.size) {
         for (int idx = threadIdx.x; idx < 2 * ny; idx += NT) {
             const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
-            sa[idx] = (s ? specF : specU)[(long)kx * ny + j];
+            sa[idx] = (s ? specF : specU)[sidx(p, kx, j)];
         }
         __syncthreads();
         float2* r = fft_lds<float2>(sa, sb, 2, ny, p.py, twy, -1);
@@ -1890,7 +1905,7 @@ __device__ __forceinline__ void phase_imp_cols(const KP& p, const Team& tm, cons
         }
         __syncthreads();
         float2* r2 = fft_lds<float2>(o, r, 1, ny, p.py, twy, +1);
-        for (int j = threadIdx.x; j < ny; j += NT) st_shared(addr(specU, (long)kx * ny + j), r2[j]);
+        for (int j = threadIdx.x; j < ny; j += NT) st_shared(addr(specU, sidx(p, kx, j)), r2[j]);
         __syncthreads();
     }
 }
@@ -1912,7 +1927,7 @@ __device__ __forceinline__ void phase_imp_c2r(const KP& p, const Team& tm, const
         const int kx = k0 + vv * NT;
         if (kx >= p.nhx) break;
         double* v = vs[vv];
-        const float2 a = spec[(long)kx * ny], b = spec[(long)kx * ny + ny - 1], tw = twx[kx];
+        const float2 a = spec[sidx(p, kx, 0)], b = spec[sidx(p, kx, ny - 1)], tw = twx[kx];
         const bool nyq = even && kx == nx / 2;
         const double w = (kx == 0 || nyq) ? 1.0 : 2.0;
         v[0] += w * (double)a.x;

@@ -100,16 +100,24 @@ def test_rhs_large_grid_vs_oracle(n):
 
 
 def test_rhs_snapshots_equal_separable():
-    """TimeInterpolatedIceThickness snapshots and the separable form give the same RHS."""
+    """TimeInterpolatedIceThickness snapshots (src/interpolations.jl:51-74) and the separable
+    form a(t) H give the same RHS, each against the FP64 oracle fed the same snapshot inputs, at a
+    knot (t = 500) and between knots (t = 1500.5)."""
+    from oracle import pyref
     a = lv_case(n=32, separable=True)
     b = lv_case(n=32, separable=False)
     sa, sb = hip_sim(a), hip_sim(b)
+    rb = pyref.RefEngine.from_inputs(sb.inputs)
     u = np.zeros((32, 32))
     for t in (500.0, 1500.5):
         da, _ = sa.engine.rhs(t, u)
         db, _ = sb.engine.rhs(t, u)
+        dr, _ = rb.rhs(t, u)
         m, l2 = rel(da, db)
         assert m < 1e-6 and l2 < 1e-6
+        for d in (da, db):
+            m, l2 = rel(d, dr)
+            assert m < RHS_MAX and l2 < RHS_L2, (t, m, l2)
 
 
 @pytest.mark.skipif(DRYRUN, reason="device pointers need the GPU")
