@@ -234,16 +234,18 @@ WEIGHTS = os.path.join(ROOT, "fastisostasy.jl_amd", "fastiso", "data", "fno_lv_e
 
 
 def surrogate_leg(n, window_kyr, predictor="rkc", rkc_dt=250.0, check_every=10, macro_dt=500.0, tol=1.0,
-                  weights=None, validate=True):
+                  weights=None, validate=True, hr=False):
     """Surrogate-with-fallback over [0, window_kyr] of the N=n lv_elva workload: macro steps of
     macro_dt years in periods of check_every (check_every - 1 predicted steps, then a plain-solver
     check of the next one), the period rolled back and integrated by the plain solver when the
     check deviates by more than tol. With validate the plain solver integrates the same window in
     a second simulation (stopping at the same macro-step times); its wall time gives the plain
-    kyr/s of the same window, and max |u_surrogate - u_plain| is taken at every 5-kyr mark."""
+    kyr/s of the same window, and max |u_surrogate - u_plain| is taken at every 5-kyr mark. Without
+    it the plain solver's rate is that of the run's own plain checks (plain_kyr_per_s_in_run). hr:
+    the N=1024 workload's fields and domain."""
     import numpy as np
     from fastiso import surrogate as S
-    w, sim = make_sim(n, engine="hip")
+    w, sim = make_sim(n, engine="hip", hr=hr)
     sur = None
     if weights is not None:
         d = np.load(weights, allow_pickle=False)
@@ -263,8 +265,7 @@ def surrogate_leg(n, window_kyr, predictor="rkc", rkc_dt=250.0, check_every=10, 
     def prog(t1, kind, dev):
         if abs(t1 / 5e3 - round(t1 / 5e3)) < 1e-9:
             marks[t1] = eng.get_field("u")
-        if kind.startswith("check") or abs(t1 / 5e3 - round(t1 / 5e3)) < 1e-9:
-            print(f"surrogate run at {t1:.0f} yr: {kind} dev={dev}", file=sys.stderr, flush=True)
+        print(f"surrogate run at {t1:.0f} yr: {kind} dev={dev}", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     res = runner.run(t_end, t0=0.0, progress=prog)
@@ -274,12 +275,15 @@ def surrogate_leg(n, window_kyr, predictor="rkc", rkc_dt=250.0, check_every=10, 
            "rkc_max_substep_yr": rkc_dt if predictor == "rkc" else None, "macro_step_yr": macro_dt,
            "check_every": check_every, "tolerance_m": tol, "wall_s": el,
            "max_check_deviation_m": max(devs) if devs else None, **res}
+    if res["plain_s_in_run"] > 0:
+        out["plain_kyr_per_s_in_run"] = res["plain_yr_in_run"] / 1e3 / res["plain_s_in_run"]
+        out["speedup_vs_plain_in_run"] = out["value"] / out["plain_kyr_per_s_in_run"]
     if sur is not None:
         out["network_device_ms"] = sur.device_ms()
         sur.close()
     sim.close()
     if validate:
-        w2, sim2 = make_sim(n, engine="hip")
+        w2, sim2 = make_sim(n, engine="hip", hr=hr)
         e2 = sim2.engine
         e2.integrator_init()
         errs = {}
@@ -339,7 +343,8 @@ def bench_rkc(a):
 def bench_surrogate(a):
     """--workload surrogate: the surrogate leg alone, over --window-kyr."""
     leg = surrogate_leg(a.n, a.window_kyr, predictor=a.predictor, rkc_dt=a.rkc_dt, check_every=a.check_every,
-                        weights=a.weights if a.predictor != "rkc" or a.with_net else None, validate=a.validate)
+                        weights=a.weights if a.predictor != "rkc" or a.with_net else None, validate=a.validate,
+                        hr=a.n == 1024)
     return {
         "metric": f"simulated kyr/s at N={a.n} LV-ELVA (surrogate with plain-solver fallback)",
         "value": leg["value"], "unit": "kyr/s", "n_gpus": 1, "steps": leg["checks"], "warmup": 0,

@@ -5,6 +5,7 @@ The network maps [u, load anomaly, log10 viscosity, lithosphere thickness] (scal
 displacement increment over one macro step (scaled). It has no counterpart in the reference.
 """
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -325,6 +326,10 @@ class SurrogateRunner:
         self.perturb = perturb
         self.log = []
         self.rkc_rhs = 0
+        # wall time and simulated years of the plain-solver integrations of run() (checks and
+        # fallbacks): the plain solver's rate on the run's own states
+        self.plain_s = 0.0
+        self.plain_yr = 0.0
         # output scale of a network correcting the RKC increment (its saved scales: U, S, scale,
         # macro_dt, base 1 = RKC); a network for the extrapolated increment is not one
         sc = tuple(getattr(sur, "scales", None) or ()) if sur is not None else ()
@@ -386,8 +391,12 @@ class SurrogateRunner:
 
         def plain_to(t1):
             u0 = eng.get_field("u")
+            ta, w0 = eng.stats()["t"], time.perf_counter()
             eng.integrate(t1)
-            return eng.get_field("u") - u0
+            u1 = eng.get_field("u")
+            self.plain_s += time.perf_counter() - w0
+            self.plain_yr += t1 - ta
+            return u1 - u0
 
         n_period = 0
         while t < t_end - 1e-9:
@@ -455,7 +464,8 @@ class SurrogateRunner:
                 "fallbacks": n_fallback, "discarded_surrogate_steps": n_wasted, "rkc_rhs_calls": self.rkc_rhs,
                 "network_periods": sum(1 for _, k, _ in self.log if k.startswith("check-net")
                                        or k.startswith("check-rkc-net")),
-                "checks": sum(1 for _, k, _ in self.log if k.startswith("check-"))}
+                "checks": sum(1 for _, k, _ in self.log if k.startswith("check-")),
+                "plain_yr_in_run": self.plain_yr, "plain_s_in_run": self.plain_s}
 
 
 def generate_rkc_pairs(n_members, seed0, steps_per_member, macro_dt=500.0, n=350, W=3.15e6, t_range=(5e3, 115e3),
