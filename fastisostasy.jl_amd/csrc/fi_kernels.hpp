@@ -581,20 +581,20 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             }
         }
         FI_SUBMARK(3);
-        // ---- plate rigidity and per-cell fields: loads in flight during the LDS-only work below ----
+        // ---- plate rigidity and per-cell fields: loads in flight during the LDS-only work below.
+        // Column-major per thread (nx <= 2 NT): Dr[2m + h] is the rigidity of moment row jm0 + m
+        // and fr[2o + h] the fields of output row j0 + o, both at column threadIdx.x + h NT ----
+        static_assert(QD == 8 && QF == 4, "two columns per thread: 4 moment rows, 2 output rows");
         float Dr[QD];
         CellF fr[QF];
 #pragma unroll
         for (int q = 0; q < QD; ++q) {
-            const int idx = min((int)threadIdx.x + q * NT, totM - 1);
-            const int r = fdivi(idx, inv_nx), i = idx - r * nx;
-            Dr[q] = p.D[i + (long)(jm0 + r) * nx];
+            const int i = min((int)threadIdx.x + (q & 1) * NT, nx - 1), j = min(jm0 + (q >> 1), jm1 - 1);
+            Dr[q] = p.D[i + (long)j * nx];
         }
 #pragma unroll
         for (int q = 0; q < QF; ++q) {
-            const int idx = (int)threadIdx.x + q * NT;
-            const int r = fdivi(min(idx, totO - 1), inv_nx), i = min(idx, totO - 1) - r * nx;
-            const int j = min(j0 + r, ny - 1);
+            const int i = min((int)threadIdx.x + (q & 1) * NT, nx - 1), j = min(j0 + (q >> 1), ny - 1);
             const long c = i + (long)j * nx;
             fr[q].H = (float)ice_at(p, it, c);
             fr[q].zo = st.trig ? p.zoff_trig[c] : p.zoff[c];
@@ -605,25 +605,155 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             fr[q].m = p.mask[c] ? 1.0f : 0.0f;
         }
         __syncthreads();
-        // finish the stage displacement: spectral term, corner BC; keep u_{n+1} when asked
-        for (int idx = threadIdx.x; idx < totU; idx += NT) {
-            double u = su[idx];
-            if (st.kin >= 0) u += st.dt * (st.ain * (double)skin[idx]);
-            u -= uoff;
-            su[idx] = u;
-            const int r = fdivi(idx, inv_nx);
-            const int j = ju0 + r;
-            if (st.write_u >= 0 && j >= j0 && j < je) st_shared(addr(ubuf(p, st.write_u), idx - r * nx + (long)j * nx), u);
-        }
-        __syncthreads();
-        FI_SUBMARK(4);
-        // plate moments M = -D (...) on rows [jm0, jm1)
+        const bool spec_in = st.kin >= 0;
+        // right-hand side of output cell (i, j0 + o) from the moment derivatives, its displacement
+        // u and its fields; stores the loads for a later sparse update when asked
+        auto rhs_cell = [&](int i, int j, double bxx, double bxy, double byy, double u, const CellF& F) {
+            const long c = i + (long)j * nx;
+            const double m = F.m;
+            const double Haf = hab((double)F.H, (double)F.zo + u, p.rho_sw_ice);
+            const double s_ice = p.rho_ice * (Haf - (double)F.hr);
+            const double s_load = m * (s_ice + (double)F.lw);
+            const double s_mant = p.rho_um * u;
+            const double s_full = s_load + m * ((double)F.lt + s_mant);
+            const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
+            if (wsig) {
+                st_shared(addr(p.sig_load_prev, c), (float)s_load);
+                st_shared(addr(p.sig_geo_prev, c), (float)(p.compressible_column
+                                                ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
+                                                : s_full));
+            }
+            return (float)(rhs * (double)F.i2e);
+        };
+        // Register-window plate stencil for two columns per thread (as in phase_rows_lds, whose
+        // comments give the row rules): column i = threadIdx.x + h NT finishes the stage
+        // displacement on an NU-row x 3-column window in registers and writes its moment rows;
+        // after one workgroup barrier each column evaluates its output rows.
+        auto window2 = [&](auto JU_, auto NU_, auto JM_, auto NM_, auto NO_, auto TOP_, auto BOT_, auto LO_) {
+            constexpr int JU = decltype(JU_)::value, NU = decltype(NU_)::value, JM = decltype(JM_)::value,
+                          NM = decltype(NM_)::value, NO = decltype(NO_)::value, LO = decltype(LO_)::value;
+            constexpr bool TOP = decltype(TOP_)::value, BOT = decltype(BOT_)::value;
+            double uo[2][NO];
 #pragma unroll
-        for (int q = 0; q < QD; ++q) {
-            const int idx = (int)threadIdx.x + q * NT;
-            if (idx < totM) {
+            for (int h = 0; h < 2; ++h) {
+                int i = (int)threadIdx.x + h * NT;
+                asm volatile("" : "+v"(i));
+                if (i < nx) {
+                    const int a = min(max(i - 1, 0), nx - 3);
+                    const int ci = i - a;
+                    double cv[NU], dd[NU], xx[NU];
+#pragma unroll
+                    for (int r = 0; r < NU; ++r) {
+                        const int b0 = r * nx + a;
+                        double x0 = su[b0], x1 = su[b0 + 1], x2 = su[b0 + 2];
+                        if (spec_in) {
+                            x0 += st.dt * (st.ain * (double)skin[b0]);
+                            x1 += st.dt * (st.ain * (double)skin[b0 + 1]);
+                            x2 += st.dt * (st.ain * (double)skin[b0 + 2]);
+                        }
+                        x0 -= uoff;
+                        x1 -= uoff;
+                        x2 -= uoff;
+                        cv[r] = ci == 0 ? x0 : (ci == 1 ? x1 : x2);
+                        dd[r] = ci == 1 ? (x2 - x0) * (0.5 * p.inv_dx)
+                                        : (ci == 0 ? (x1 - x0) * p.inv_dx : (x2 - x1) * p.inv_dx);
+                        xx[r] = (x2 - 2.0 * x1 + x0) * p.inv_dx2;
+                    }
+#pragma unroll
+                    for (int o = 0; o < NO; ++o) uo[h][o] = cv[o - JU];
+                    if (st.write_u >= 0) {
+                        const G<double> un = ubuf(p, st.write_u);
+#pragma unroll
+                        for (int o = 0; o < NO; ++o) st_shared(addr(un, i + (long)(j0 + o) * nx), uo[h][o]);
+                    }
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) {
+                        const int ur = JM + m - JU;   // window row of this moment row
+                        const bool first = TOP && m == 0, last = BOT && m == NM - 1;
+                        const int c = first ? ur + 1 : (last ? ur - 1 : ur);
+                        const int lo = first ? ur : ur - 1, hi = last ? ur : ur + 1;
+                        const double D = (double)(h ? Dr[2 * m + 1] : Dr[2 * m]);
+                        const double uxx = xx[ur];
+                        const double uyy = (cv[c + 1] - 2.0 * cv[c] + cv[c - 1]) * p.inv_dy2;
+                        const double uxy = (first || last) ? (dd[hi] - dd[lo]) * p.inv_dy
+                                                           : (dd[hi] - dd[lo]) * (0.5 * p.inv_dy);
+                        sMxx[m * nx + i] = (MT)(-D * (nu * uyy + uxx));
+                        sMyy[m * nx + i] = (MT)(-D * (nu * uxx + uyy));
+                        sMxy[m * nx + i] = (MT)(-D * (1.0 - nu) * uxy);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = (int)threadIdx.x + h * NT;
+                if (i < nx) {
+                    const int a = min(max(i - 1, 0), nx - 3);
+                    const int ci = i - a;
+                    auto mx = [&](const MT* M, int mm, int cc) { return (double)M[mm * nx + a + cc]; };
+                    auto d1m = [&](int mm) {
+                        return ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
+                                       : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
+                                                  : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                    };
+                    float f[2] = {0.0f, 0.0f};
+#pragma unroll
+                    for (int o = 0; o < NO; ++o) {
+                        const int mr = o - JM;   // moment row of output row j0 + o
+                        const bool first = TOP && o == 0, last = o == LO;
+                        const int c = first ? mr + 1 : (last ? mr - 1 : mr);
+                        const int lo = first ? mr : mr - 1, hi = last ? mr : mr + 1;
+                        const double bxx = (mx(sMxx, mr, 2) - 2.0 * mx(sMxx, mr, 1) + mx(sMxx, mr, 0)) * p.inv_dx2;
+                        const double byy = ((double)sMyy[(c + 1) * nx + i] - 2.0 * (double)sMyy[c * nx + i] +
+                                            (double)sMyy[(c - 1) * nx + i]) * p.inv_dy2;
+                        const double bxy = (first || last) ? (d1m(hi) - d1m(lo)) * p.inv_dy
+                                                           : (d1m(hi) - d1m(lo)) * (0.5 * p.inv_dy);
+                        const CellF F = h ? fr[2 * o + 1] : fr[2 * o];
+                        f[o] = rhs_cell(i, j0 + o, bxx, bxy, byy, uo[h][o], F);
+                    }
+                    sf[i] = make_float2(f[0], f[1]);
+                }
+            }
+        };
+        using std::integral_constant;
+#define FI_WIN2(JU, NU, JM, NM, NO, TOP, BOT, LO)                                                          \
+    window2(integral_constant<int, JU>{}, integral_constant<int, NU>{}, integral_constant<int, JM>{},     \
+            integral_constant<int, NM>{}, integral_constant<int, NO>{}, integral_constant<bool, TOP>{},    \
+            integral_constant<bool, BOT>{}, integral_constant<int, LO>{})
+        // the kind of this row pair (as in phase_rows_lds); anything else takes the general passes
+        const int rj = ju0 - j0, rm = jm0 - j0, nu_ = ju1 - ju0, nm_ = jm1 - jm0, no_ = je - j0;
+        const int kind = (ny < 6 || nx < 3) ? -1
+                         : (rj == -2 && nu_ == 6 && rm == -1 && nm_ == 4 && no_ == 2 && j0 >= 2 && j0 + 3 <= ny - 1) ? 0
+                         : (j0 == 0 && rj == 0 && nu_ == 4 && rm == 0 && nm_ == 3 && no_ == 2)                     ? 1
+                         : (j0 == ny - 2 && rj == -2 && nu_ == 4 && rm == -1 && nm_ == 3 && no_ == 2)              ? 2
+                         : (j0 == ny - 3 && rj == -2 && nu_ == 5 && rm == -1 && nm_ == 4 && no_ == 2)              ? 3
+                         : (j0 == ny - 1 && rj == -3 && nu_ == 4 && rm == -2 && nm_ == 3 && no_ == 1)              ? 4
+                                                                                                                  : -1;
+        if (kind == 0) {
+            FI_WIN2(-2, 6, -1, 4, 2, false, false, -1);
+        } else if (kind == 1) {
+            FI_WIN2(0, 4, 0, 3, 2, true, false, -1);
+        } else if (kind == 2) {
+            FI_WIN2(-2, 4, -1, 3, 2, false, true, 1);
+        } else if (kind == 3) {
+            FI_WIN2(-2, 5, -1, 4, 2, false, true, -1);
+        } else if (kind == 4) {
+            FI_WIN2(-3, 4, -2, 3, 1, false, true, 0);
+        } else {
+            // general passes (grids with fewer than 6 rows): rigidity and fields read where used
+            for (int idx = threadIdx.x; idx < totU; idx += NT) {
+                double u = su[idx];
+                if (spec_in) u += st.dt * (st.ain * (double)skin[idx]);
+                u -= uoff;
+                su[idx] = u;
+                const int r = fdivi(idx, inv_nx);
+                const int j = ju0 + r;
+                if (st.write_u >= 0 && j >= j0 && j < je) st_shared(addr(ubuf(p, st.write_u), idx - r * nx + (long)j * nx), u);
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < totM; idx += NT) {
                 const int r = fdivi(idx, inv_nx), i = idx - r * nx, j = jm0 + r;
-                const double D = (double)Dr[q];
+                const double D = (double)p.D[i + (long)j * nx];
                 const double uxx = d2x(su + (j - ju0) * nx, i, nx, p.inv_dx2);
                 const double uyy = d2y(su, ju0, j, i, nx, ny, p.inv_dy2);
                 const double uxy = dxy(su, ju0, j, i, nx, ny, p.inv_dx, p.inv_dy);
@@ -631,41 +761,30 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                 sMyy[idx] = (MT)(-D * (nu * uxx + uyy));
                 sMxy[idx] = (MT)(-D * (1.0 - nu) * uxy);
             }
-        }
-        __syncthreads();
-        FI_SUBMARK(5);
-        // rhs = -g sigma_full + dxx Mxx + 2 dxy Mxy + dyy Myy, times K/(2 eta)
-#pragma unroll
-        for (int q = 0; q < QF; ++q) {
-            const int idx = (int)threadIdx.x + q * NT;
-            if (idx < totO) {
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < totO; idx += NT) {
                 const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
                 float f = 0.0f;
                 if (j < ny) {
                     const long c = i + (long)j * nx;
+                    CellF F;
+                    F.H = (float)ice_at(p, it, c);
+                    F.zo = st.trig ? p.zoff_trig[c] : p.zoff[c];
+                    F.hr = p.H_af_ref[c];
+                    F.lt = p.LT[c];
+                    F.lw = p.interactive ? p.LW[c] : 0.0f;
+                    F.i2e = p.inv2eta[c];
+                    F.m = p.mask[c] ? 1.0f : 0.0f;
                     const double bxx = d2x(sMxx + (j - jm0) * nx, i, nx, p.inv_dx2);
                     const double byy = d2y(sMyy, jm0, j, i, nx, ny, p.inv_dy2);
                     const double bxy = dxy(sMxy, jm0, j, i, nx, ny, p.inv_dx, p.inv_dy);
-                    const double u = su[(j - ju0) * nx + i];
-                    const double m = fr[q].m;
-                    const double Haf = hab((double)fr[q].H, (double)fr[q].zo + u, p.rho_sw_ice);
-                    const double s_ice = p.rho_ice * (Haf - (double)fr[q].hr);
-                    const double s_load = m * (s_ice + (double)fr[q].lw);
-                    const double s_mant = p.rho_um * u;
-                    const double s_full = s_load + m * ((double)fr[q].lt + s_mant);
-                    const double rhs = -p.g * s_full + (bxx + (2.0 * bxy + byy));
-                    f = (float)(rhs * (double)fr[q].i2e);
-                    if (wsig) {
-                        st_shared(addr(p.sig_load_prev, c), (float)s_load);
-                        st_shared(addr(p.sig_geo_prev, c), (float)(p.compressible_column
-                                                        ? s_load + m * (s_mant - p.rho_um * (double)p.u_ref[c])
-                                                        : s_full));
-                    }
+                    f = rhs_cell(i, j, bxx, bxy, byy, su[(j - ju0) * nx + i], F);
                 }
                 if (r == 0) sf[i].x = f;
                 else sf[i].y = f;
             }
         }
+#undef FI_WIN2
         __syncthreads();
         FI_SUBMARK(6);
         float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, -1);
