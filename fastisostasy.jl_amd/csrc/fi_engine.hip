@@ -1148,7 +1148,14 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     kp.nfy = c->nfy;
     kp.nfhx = c->nfhx;
     kp.nrowitems = (ny + 1) / 2;
-    kp.colnb = 1;
+    // spectral columns per columns-phase item: one, unless there are more columns than CUs (one
+    // workgroup per CU), where a workgroup transforms its columns as one batch instead of one
+    // after another (N=1024: 513 columns, 3 per item); bitwise neutral
+    {
+        int ncu = 0;
+        HIPCHECK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+        kp.colnb = (ncu > 0 && c->nhx > ncu) ? (c->nhx + ncu - 1) / ncu : 1;
+    }
     kp.ncolitems = (c->nhx + kp.colnb - 1) / kp.colnb;
     kp.i1 = c->i1;
     kp.i2 = c->i2;
