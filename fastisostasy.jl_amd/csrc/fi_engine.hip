@@ -2426,6 +2426,17 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, const void* f, size_t lds,
         kph[m].mode = MODE_INTEGRATE;
         kph[m].max_steps = max_steps;
         kph[m].write_sigma_always = 0;
+        // columns per columns-phase item so that a team of ts workgroups transforms the member's
+        // columns in one round (a batch per workgroup), as far as the launch's LDS holds the batch
+        // (two buffers of nb columns after the twiddles); bitwise neutral
+        if (!c->elra && !c->implicit) {
+            const size_t twb = (size_t)(c->nx + c->ny) * 8;
+            int nb = std::max(1, (c->nhx + ts - 1) / ts);
+            while (nb > 1 && twb + (size_t)16 * nb * c->ny > lds) --nb;
+            nb = std::max(nb, c->kp.colnb);
+            kph[m].colnb = nb;
+            kph[m].ncolitems = (c->nhx + nb - 1) / nb;
+        }
         kph[m].ctl = sc.ctl_all + m;
         ptrs[m] = sc.kp_all + m;
         b->ctl_h[m] = c->ctl;
