@@ -1155,8 +1155,11 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
         int ncu = 0;
         HIPCHECK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
         kp.colnb = (ncu > 0 && c->nhx > ncu) ? (c->nhx + ncu - 1) / ncu : 1;
+        // likewise row pairs per item of the standalone inverse transform (N=1024: 2)
+        kp.c2rnb = (ncu > 0 && kp.nrowitems > ncu) ? (kp.nrowitems + ncu - 1) / ncu : 1;
     }
     kp.ncolitems = (c->nhx + kp.colnb - 1) / kp.colnb;
+    kp.nc2ritems = (kp.nrowitems + kp.c2rnb - 1) / kp.c2rnb;
     kp.i1 = c->i1;
     kp.i2 = c->i2;
     kp.j1 = c->j1;
@@ -1272,7 +1275,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     const size_t twb = (size_t)(nx + ny) * 8;
     // FP64-moment rows phase also stages the rigidity (4 rows) and 7 per-cell fields (2 rows)
     size_t rows_d = (size_t)nx * (6 * 8 + 12 * 8 + 16 + 24 + 16 + 56), rows_f = (size_t)nx * (6 * 8 + 12 * 4 + 16 + 24);
-    size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)3 * c->nfx * 8, (size_t)5 * c->nfy * 8});
+    size_t other = std::max({(size_t)2 * kp.colnb * ny * 8, (size_t)2 * kp.c2rnb * nx * 8, (size_t)3 * c->nfx * 8,
+                             (size_t)5 * c->nfy * 8});
     c->mt_double = nx <= 512 && twb + rows_d + (c->has_dist ? (size_t)nx * 24 : 0) <= 160 * 1024;   // FP64 moments, register prefetch sized for nx <= 512
     c->lds_dist = twb + std::max(rows_d + (size_t)nx * 24, other);   // also 6 rows of 1/K
     c->lds_d = c->has_dist ? c->lds_dist : twb + std::max(rows_d, other);
@@ -2436,6 +2440,12 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, const void* f, size_t lds,
             nb = std::max(nb, c->kp.colnb);
             kph[m].colnb = nb;
             kph[m].ncolitems = (c->nhx + nb - 1) / nb;
+            // and row pairs per item of the standalone inverse transform
+            int np2 = std::max(1, (c->kp.nrowitems + ts - 1) / ts);
+            while (np2 > 1 && twb + (size_t)16 * np2 * c->nx > lds) --np2;
+            np2 = std::max(np2, c->kp.c2rnb);
+            kph[m].c2rnb = np2;
+            kph[m].nc2ritems = (c->kp.nrowitems + np2 - 1) / np2;
         }
         kph[m].ctl = sc.ctl_all + m;
         ptrs[m] = sc.kp_all + m;

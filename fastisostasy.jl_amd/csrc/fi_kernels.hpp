@@ -1241,17 +1241,20 @@ __device__ __forceinline__ void phase_cols(const KP& p, const Team& tm, G<float2
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const StageDesc& st, const G<float2> spec, int slot,
                                           const float2* twx, char* smem, double* red) {
-    const int nx = p.nx, ny = p.ny;
+    const int nx = p.nx, ny = p.ny, nbp = p.c2rnb;
     float2* sf = (float2*)smem;
-    float2* sg = sf + nx;
+    float2* sg = sf + nbp * nx;
+    const float inv_nx = 1.0f / (float)nx;
     G<float> K = p.Kb + (long)slot * p.Nl;
-    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
-        const int j0 = 2 * item;
-        const bool pre = st.final_err && 2 * nx <= 2 * NT;
+    // an item is c2rnb consecutive row pairs, inverse-transformed as one batch
+    for (int item = tm.rank; item < p.nc2ritems; item += tm.size) {
+        const int pr0 = item * nbp, npr = min(nbp, p.nrowitems - pr0);
+        const bool pre = st.final_err && npr == 1 && 2 * nx <= 2 * NT;
         // error-norm inputs of the pair's cells, loaded with the spectrum (one round)
         float e1v[2] = {0.f, 0.f}, e2v[2] = {0.f, 0.f}, e3v[2] = {0.f, 0.f};
         double unv[2] = {0.0, 0.0}, upv[2] = {0.0, 0.0};
         if (pre) {
+            const int j0 = 2 * pr0;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int idx = min((int)threadIdx.x + q * NT, 2 * nx - 1);
@@ -1264,52 +1267,59 @@ __device__ __forceinline__ void phase_c2r(const KP& p, const Team& tm, const Sta
                 upv[q] = ubuf(p, st.write_u)[c];
             }
         }
-        for (int k = threadIdx.x; k < nx; k += NT) sf[k] = c2r_pair_value(p, spec, k, j0, j0 + 1);
-        __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nx, p.px, twx, +1);
-        double accv[NV][3] = {};
-        for (int i0 = threadIdx.x, q0 = 0; i0 < 2 * nx; i0 += VNT, q0 += NV)
-#pragma unroll
-        for (int vv = 0; vv < NV; ++vv) {
-            const int idx = i0 + vv * NT, q = q0 + vv;
-            if (idx >= 2 * nx) break;
-            double* acc = accv[vv];
-            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
-            if (j >= ny) continue;
-            const long c = i + (long)j * nx;
-            const float v = r ? res[i].y : res[i].x;
-            st_shared(addr(K, c), v);
-            const int ci = corner_index(p, i, j);
-            if (ci >= 0) st_shared(addr(p.kcorner, 4 * slot + ci), v);
-            if (st.final_err) {
-                double k1, k2, k3, un, up;
-                if (pre) {
-                    const int qq = q & 1;   // q <= 1 when pre
-                    k1 = qq ? e1v[1] : e1v[0];
-                    k2 = qq ? e2v[1] : e2v[0];
-                    k3 = qq ? e3v[1] : e3v[0];
-                    un = qq ? unv[1] : unv[0];
-                    up = qq ? upv[1] : upv[0];
-                } else {
-                    k1 = kslot(p, st.e1)[c];
-                    k2 = kslot(p, st.e2)[c];
-                    k3 = kslot(p, st.e3)[c];
-                    un = ubuf(p, st.u_prev)[c];
-                    up = ubuf(p, st.write_u)[c];
-                }
-                const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)v);
-                const double w = 1.0 / (p.abstol + fmax(fabs(un), fabs(up)) * p.reltol);
-                acc[0] += (a * w) * (a * w);
-                acc[1] += a * w * w;
-                acc[2] += w * w;
-            }
+        for (int idx = threadIdx.x; idx < npr * nx; idx += NT) {
+            const int b = fdivi(idx, inv_nx), k = idx - b * nx, j0 = 2 * (pr0 + b);
+            sf[idx] = c2r_pair_value(p, spec, k, j0, j0 + 1);
         }
-        if (st.final_err) {
-            block_sum<3>(red, accv);
-            if (threadIdx.x == 0) {
-                st_shared(addr(p.part, 4 * item + 0), accv[0][0]);
-                st_shared(addr(p.part, 4 * item + 1), accv[0][1]);
-                st_shared(addr(p.part, 4 * item + 2), accv[0][2]);
+        __syncthreads();
+        float2* resb = fft_lds<float2>(sf, sg, npr, nx, p.px, twx, +1);
+        for (int pb = 0; pb < npr; ++pb) {
+            const int j0 = 2 * (pr0 + pb), pair = pr0 + pb;
+            const float2* res = resb + pb * nx;
+            double accv[NV][3] = {};
+            for (int i0 = threadIdx.x, q0 = 0; i0 < 2 * nx; i0 += VNT, q0 += NV)
+    #pragma unroll
+            for (int vv = 0; vv < NV; ++vv) {
+                const int idx = i0 + vv * NT, q = q0 + vv;
+                if (idx >= 2 * nx) break;
+                double* acc = accv[vv];
+                const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+                if (j >= ny) continue;
+                const long c = i + (long)j * nx;
+                const float v = r ? res[i].y : res[i].x;
+                st_shared(addr(K, c), v);
+                const int ci = corner_index(p, i, j);
+                if (ci >= 0) st_shared(addr(p.kcorner, 4 * slot + ci), v);
+                if (st.final_err) {
+                    double k1, k2, k3, un, up;
+                    if (pre) {
+                        const int qq = q & 1;   // q <= 1 when pre
+                        k1 = qq ? e1v[1] : e1v[0];
+                        k2 = qq ? e2v[1] : e2v[0];
+                        k3 = qq ? e3v[1] : e3v[0];
+                        un = qq ? unv[1] : unv[0];
+                        up = qq ? upv[1] : upv[0];
+                    } else {
+                        k1 = kslot(p, st.e1)[c];
+                        k2 = kslot(p, st.e2)[c];
+                        k3 = kslot(p, st.e3)[c];
+                        un = ubuf(p, st.u_prev)[c];
+                        up = ubuf(p, st.write_u)[c];
+                    }
+                    const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)v);
+                    const double w = 1.0 / (p.abstol + fmax(fabs(un), fabs(up)) * p.reltol);
+                    acc[0] += (a * w) * (a * w);
+                    acc[1] += a * w * w;
+                    acc[2] += w * w;
+                }
+            }
+            if (st.final_err) {
+                block_sum<3>(red, accv);
+                if (threadIdx.x == 0) {
+                    st_shared(addr(p.part, 4 * pair + 0), accv[0][0]);
+                    st_shared(addr(p.part, 4 * pair + 1), accv[0][1]);
+                    st_shared(addr(p.part, 4 * pair + 2), accv[0][2]);
+                }
             }
         }
         __syncthreads();
