@@ -1083,10 +1083,13 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     HIPCHECK(c, c->dz_raw.alloc(N));
     HIPCHECK(c, c->ext_corners.alloc(8));
     HIPCHECK(c, hipMemset(c->ext_corners.p, 0, 8 * sizeof(float)));
-    HIPCHECK(c, c->pA.alloc((size_t)c->nfhx * ny));
-    HIPCHECK(c, c->pB.alloc((size_t)c->nfhx * ny));
-    HIPCHECK(c, c->oA.alloc((size_t)c->nfhx * (ny + 2)));
-    HIPCHECK(c, c->oB.alloc((size_t)c->nfhx * (ny + 2)));
+    // padded half spectra, blocked [kx/4][row][4] like the n x n spectrum: columns padded to a
+    // multiple of 4
+    const size_t nfhx4 = (size_t)((c->nfhx + 3) & ~3);
+    HIPCHECK(c, c->pA.alloc(nfhx4 * ny));
+    HIPCHECK(c, c->pB.alloc(nfhx4 * ny));
+    HIPCHECK(c, c->oA.alloc(nfhx4 * (ny + 2)));
+    HIPCHECK(c, c->oB.alloc(nfhx4 * (ny + 2)));
     if (!c->bsl_x.empty()) {
         if (upload(c, c->bsl_x_d, c->bsl_x) || upload(c, c->bsl_y_d, c->bsl_y)) return FI_ERR_DEVICE;
     } else {

@@ -210,6 +210,11 @@ __device__ __forceinline__ G<float2> specbuf(const KP& p, int b) { return p.spec
 __device__ __forceinline__ long sidx(const KP& p, int kx, int j) {
     return (((long)(kx >> 2) * p.ny + j) << 2) + (kx & 3);
 }
+// the same blocked layout for a padded-grid half spectrum of `rows` rows (pA/pB: ny rows, the
+// sparse update's oA/oB: ny + 2, the relaxed mantle's oA: ny)
+__device__ __forceinline__ long bidx(int kx, int j, int rows) {
+    return (((long)(kx >> 2) * rows + j) << 2) + (kx & 3);
+}
 __device__ __forceinline__ G<double> ubuf(const KP& p, int b) { return p.Ub + (long)b * p.Nl; }
 
 // sum of the global derivative terms of a stage at cell c
@@ -1365,8 +1370,8 @@ __device__ __forceinline__ void phase_sp_rows(const KP& p, const Team& tm, char*
             A.y = 0.5f * (Z.y - Zm.y);
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
-            st_shared(addr(p.pA, (long)kx * ny + j), A);
-            st_shared(addr(p.pB, (long)kx * ny + j), B);
+            st_shared(addr(p.pA, bidx(kx, j, ny)), A);
+            st_shared(addr(p.pB, bidx(kx, j, ny)), B);
         }
         __syncthreads();
     }
@@ -1392,7 +1397,7 @@ __device__ __forceinline__ void phase_sp_cols(const KP& p, const Team& tm, char*
             float2 v;
             v.x = 0;
             v.y = 0;
-            if (j < ny) v = (s ? p.pB : p.pA)[(long)kx * ny + j];
+            if (j < ny) v = (s ? p.pB : p.pA)[bidx(kx, j, ny)];
             sa[idx] = v;
         }
         __syncthreads();
@@ -1409,7 +1414,7 @@ __device__ __forceinline__ void phase_sp_cols(const KP& p, const Team& tm, char*
         float2* r2 = fft_lds<float2>(r, o, 2, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < 2 * no; idx += NT) {
             const int s = idx / no, jj = idx - s * no;
-            st_shared(addr((s ? p.oB : p.oA), (long)kx * no + jj), r2[s * nfy + sp_row_of(p, jj)]);
+            st_shared(addr((s ? p.oB : p.oA), bidx(kx, jj, no)), r2[s * nfy + sp_row_of(p, jj)]);
         }
         __syncthreads();
     }
@@ -1427,13 +1432,13 @@ __device__ __forceinline__ void phase_sp_c2r(const KP& p, const Team& tm, char* 
         for (int k = threadIdx.x; k < nfx; k += NT) {
             float2 A, B;
             if (k < nfhx) {
-                A = p.oA[(long)k * no + jj];
-                B = p.oB[(long)k * no + jj];
+                A = p.oA[bidx(k, jj, no)];
+                B = p.oB[bidx(k, jj, no)];
                 if (k == 0 || (even && k == nfx / 2)) { A.y = 0; B.y = 0; }
             } else {
                 const int kk = nfx - k;
-                A = p.oA[(long)kk * no + jj];
-                B = p.oB[(long)kk * no + jj];
+                A = p.oA[bidx(kk, jj, no)];
+                B = p.oB[bidx(kk, jj, no)];
                 A.y = -A.y;
                 B.y = -B.y;
             }
@@ -1758,8 +1763,8 @@ __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, con
             A.y = 0.5f * (Z.y - Zm.y);
             B.x = 0.5f * (Z.y + Zm.y);
             B.y = -0.5f * (Z.x - Zm.x);
-            st_shared(addr(p.pA, (long)kx * ny + j0), A);
-            if (j0 + 1 < ny) st_shared(addr(p.pA, (long)kx * ny + j0 + 1), B);
+            st_shared(addr(p.pA, bidx(kx, j0, ny)), A);
+            if (j0 + 1 < ny) st_shared(addr(p.pA, bidx(kx, j0 + 1, ny)), B);
         }
         __syncthreads();
     }
@@ -1777,7 +1782,7 @@ __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, cha
         for (int idx = threadIdx.x; idx < nb * nfy; idx += NT) {
             const int s = idx >= nfy ? 1 : 0, j = idx - s * nfy;
             float2 v = make_float2(0.f, 0.f);
-            if (j < ny) v = p.pA[(long)(kx0 + s) * ny + j];
+            if (j < ny) v = p.pA[bidx(kx0 + s, j, ny)];
             sa[idx] = v;
         }
         __syncthreads();
@@ -1794,7 +1799,7 @@ __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, cha
         float2* r2 = fft_lds<float2>(r, o, nb, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < nb * ny; idx += NT) {
             const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
-            st_shared(addr(p.oA, (long)(kx0 + s) * ny + j), r2[s * nfy + (p.j1 - 1) + j]);
+            st_shared(addr(p.oA, bidx(kx0 + s, j, ny)), r2[s * nfy + (p.j1 - 1) + j]);
         }
         __syncthreads();
     }
@@ -1819,7 +1824,7 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
         const int k = k0 + vv * NT;
         if (k >= nfhx) break;
         double* v = vs[vv];
-        float2 a = p.oA[(long)k * ny], b = p.oA[(long)k * ny + ny - 1];
+        float2 a = p.oA[bidx(k, 0, ny)], b = p.oA[bidx(k, ny - 1, ny)];
         const bool dcn = k == 0 || (even && k == nfx / 2);
         if (dcn) {
             a.y = 0.f;
@@ -1844,8 +1849,8 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
         for (int k = threadIdx.x; k < nfx; k += NT) {
             const bool cj = k >= nfhx;
             const int kk = cj ? nfx - k : k;
-            float2 A = p.oA[(long)kk * ny + j0];
-            float2 B = two ? p.oA[(long)kk * ny + j0 + 1] : make_float2(0.f, 0.f);
+            float2 A = p.oA[bidx(kk, j0, ny)];
+            float2 B = two ? p.oA[bidx(kk, j0 + 1, ny)] : make_float2(0.f, 0.f);
             if (k == 0 || (even && k == nfx / 2)) {
                 A.y = 0.f;
                 B.y = 0.f;
