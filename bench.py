@@ -193,7 +193,11 @@ def cpu_baseline(n, window, sample_s, hr=False, elra=False, implicit=False, star
     """The FP64 oracle on this host, on all cores and on one core, each for about sample_s of
     wall time. With start = (t, u) both legs begin from the GPU leg's state at the start of its
     timed windows (fi_set_field + fi_set_time, then a 1/100-window untimed lead-in so the step
-    size settles), so they sample the same part of the run; otherwise after one window from rest."""
+    size settles), so they sample the same part of the run; otherwise after one window from rest.
+    The implicit step takes fixed steps of dt_min and refuses fi_set_time (its calls advance u), so
+    its legs start from rest: a step costs the same whatever the state."""
+    if implicit:
+        start = None
     from oracle import pyref
     w, sim = make_sim(n, engine=None, hr=hr, elra=elra, implicit=implicit)
     legs = []
