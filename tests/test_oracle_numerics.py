@@ -274,6 +274,44 @@ def test_solidearth_two_implementations(ref):
     np.testing.assert_allclose(se2.pseudodiff_scaling, 1.0, rtol=1e-12)
 
 
+def test_channel_scaling_limits(ref):
+    """R(kappa) of one low-viscosity channel over the half-space (channel_scaling,
+    src/material.jl:232-262) against what the formula's mathematics fixes, for the host set-up and
+    the oracle's restatement: with x = kappa T and nu = visc_ratio, R = 1 + (2/3)(nu - 1/nu) x^3
+    + O(x^4) for a thin channel (its Taylor series: numerator and denominator agree to second
+    order), R -> nu for a thick one (cosh ~ sinh ~ e^x / 2), and R = 1 at nu = 1. A dropped or
+    sign-flipped term changes the x, x^2 or x^3 coefficient or the thick limit."""
+    from fastiso.solidearth import channel_scaling
+    for nu in (0.1, 0.01, 3.0):
+        x = np.array([1e-3, 3e-3, 1e-2])
+        R = channel_scaling(x, 1.0, nu)
+        np.testing.assert_allclose((R - 1) / x ** 3, 2.0 / 3.0 * (nu - 1 / nu), rtol=0.05)
+        xl = np.array([15.0, 20.0, 30.0])
+        np.testing.assert_allclose(channel_scaling(xl, 1.0, nu), nu, rtol=1e-6)
+        np.testing.assert_allclose(channel_scaling(np.array([0.5, 2.0]), 1.0, 1.0), 1.0, rtol=1e-14)
+    # the oracle's two-layer set-up: a channel of viscosity eta_c from h to h + T over the
+    # half-space eta_h (visc_ratio = eta_c / eta_h; layer boundaries uniform)
+    n, W = 16, 3e6
+    d = fastiso.RegionalDomain(W, W, n, n)
+    kappa = d.pseudodiff
+    eta_c, eta_h = 1e19, 1e21
+    nu = eta_c / eta_h
+    for T, check in ((200.0, "thin"), (6e6, "thick")):
+        lb = [np.full((n, n), 88e3), np.full((n, n), 88e3 + T)]
+        lv = [np.full((n, n), eta_c), np.full((n, n), eta_h)]
+        _, _, R, _ = ref.solidearth(n, n, W, W, lb, lv)
+        x = kappa * T
+        if check == "thin":
+            assert x.max() < 1e-2
+            ok = x > 1e-5   # (R - 1) well above rounding; the DC mode's kappa is 1e-3 of its neighbours'
+            assert ok.sum() == n * n - 1
+            np.testing.assert_allclose((R[ok] - 1) / x[ok] ** 3, 2.0 / 3.0 * (nu - 1 / nu), rtol=0.05)
+        else:
+            big = x > 15
+            assert big.sum() > n * n // 2
+            np.testing.assert_allclose(R[big], nu, rtol=1e-6)
+
+
 def test_bs3_order_and_adaptivity(ref):
     lam, T = -0.5, 2.0
     errs = []
