@@ -1,9 +1,13 @@
 // Fourier-neural-operator surrogate of the LV-ELVA macro step (include/fastiso_surrogate.h).
 //
 // Every product of the network -- the per-cell channel mixes and the truncated spectral
-// transforms, written as partial DFTs -- is one batched strided GEMM on BF16 MFMA with FP32
-// accumulation (k_gemm). Forward and backward passes are sequences of those GEMMs plus a few
-// elementwise kernels; training (Adam) runs on the device, the host only sequences launches.
+// transforms, written as partial DFTs -- runs on BF16 MFMA with FP32 accumulation: a batched
+// strided GEMM (k_gemm), and for each layer's output one fused product (k_gemm2) that adds the
+// channel mix and the inverse x-transform in the same accumulators and applies bias and GELU (or,
+// backward, GELU's derivative) before it stores. Samples of a batch go through side by side, so a
+// launch covers every row of every sample. Forward and backward passes are sequences of these plus
+// a few small kernels (spectral mode mixing, loss, Adam); training runs on the device, the host
+// only sequences launches.
 //
 // Truncated real 2D transform of a channel v(i, j) (i fastest), modes kx < mx and the 2 my
 // modes ky in {0..my-1, ny-my..ny-1}:
@@ -32,46 +36,116 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int GEMM_THREADS = 256;   // 4 waves; block tile 64 (M) x 32 (N)
 
-// C[b] = alpha A[b] B[b] + beta C[b] with element (m, k) of A[b] at A + b sAb + m sAm + k sAk
-// (likewise B, C). With kchunk > 0, batch b is the K-slice [b kchunk, (b+1) kchunk) of one product
-// and writes its partial into C + b sCb (split-K; reduced by k_splitk_reduce). With nslice > 0 as
-// well, grid batch z is slice (z mod nslice) of product (z / nslice) (A and B batch strides apply
-// to the product), its partial at C + z sCb (batched split-K; reduced by k_splitk_reduce_batched).
+// C[b] = alpha A[b] B[b] + beta C[b] with element (m, k) of A[b] at A + off_A(b) + m sAm + k sAk
+// (likewise B, C). A and B hold FP32 or BF16 (abf, bbf); C is FP32. Batch b splits as
+// b0 = b mod nb0, b1 = b / nb0 and off_X(b) = b0 sXb + b1 sXb1, so that a batch over samples
+// and rows or spectral columns addresses buffers whose two strides do not compose. With
+// kchunk > 0, batch b is the K-slice [b kchunk, (b+1) kchunk) of one product and writes its partial
+// into C + b sCb (split-K; reduced by k_splitk_reduce). With nslice > 0 as well, grid batch z is
+// slice (z mod nslice) of product (z / nslice) (the A and B batch offsets apply to the product),
+// its partial at C + z sCb (batched split-K; reduced by k_splitk_reduce_batched).
 struct Gemm {
-    const float* A;
-    const float* B;
+    const void* A;
+    const void* B;
     float* C;
     long sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn;
     int M, N, K, kchunk;
     float alpha, beta;
     int nslice;
+    int nb0;
+    long sAb1, sBb1, sCb1;
 };
 
-// The 8 K values k .. k+7 of one operand row or column: two 16-byte loads when the operand is
-// contiguous in K, the row is in range, all 8 values are inside the K range and the address is
-// 16-byte aligned; element loads (zero outside the ranges) otherwise.
-__device__ __forceinline__ void load8(const float* p, long sk, int k, int kend, bool ok, float (&v)[8]) {
+__device__ __forceinline__ long boff(int b, int nb0, long s0, long s1) {
+    const int b1 = b / nb0;
+    return (long)(b - b1 * nb0) * s0 + (long)b1 * s1;
+}
+
+// The 8 K values k .. k+7 of one operand row or column, as loaded (FP32 or BF16): one or two
+// 16-byte loads when the operand is contiguous in K, the row is in range, all 8 values are inside
+// the K range and the address is 16-byte aligned; element loads (zero outside the ranges)
+// otherwise, none when k is past the K range.
+template <typename T>
+struct Raw;
+template <>
+struct Raw<float> {
+    float v[8];
+};
+template <>
+struct Raw<__bf16> {
+    bf16x8 v;
+};
+
+__device__ __forceinline__ void load8(const float* p, long sk, int k, int kend, bool ok, Raw<float>& r) {
     if (ok && sk == 1 && k + 8 <= kend && ((reinterpret_cast<unsigned long long>(p + k) & 15ull) == 0)) {
         const float4 lo = *reinterpret_cast<const float4*>(p + k);
         const float4 hi = *reinterpret_cast<const float4*>(p + k + 4);
-        v[0] = lo.x;
-        v[1] = lo.y;
-        v[2] = lo.z;
-        v[3] = lo.w;
-        v[4] = hi.x;
-        v[5] = hi.y;
-        v[6] = hi.z;
-        v[7] = hi.w;
+        r.v[0] = lo.x;
+        r.v[1] = lo.y;
+        r.v[2] = lo.z;
+        r.v[3] = lo.w;
+        r.v[4] = hi.x;
+        r.v[5] = hi.y;
+        r.v[6] = hi.z;
+        r.v[7] = hi.w;
         return;
     }
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (ok && k + e < kend) ? p[(long)(k + e) * sk] : 0.f;
+    for (int e = 0; e < 8; ++e) r.v[e] = (ok && k + e < kend) ? p[(long)(k + e) * sk] : 0.f;
+}
+
+__device__ __forceinline__ void load8(const __bf16* p, long sk, int k, int kend, bool ok, Raw<__bf16>& r) {
+    if (ok && sk == 1 && k + 8 <= kend && ((reinterpret_cast<unsigned long long>(p + k) & 15ull) == 0)) {
+        r.v = *reinterpret_cast<const bf16x8*>(p + k);
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r.v[e] = (ok && k + e < kend) ? p[(long)(k + e) * sk] : (__bf16)0.f;
+}
+
+// MFMA operand: FP32 values round to BF16 (nearest even) here, exactly as a BF16 copy written by
+// another kernel with the same conversion, so the product does not depend on the storage type
+__device__ __forceinline__ bf16x8 operand(const Raw<float>& r) {
+    bf16x8 a;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[e] = (__bf16)r.v[e];
+    return a;
+}
+__device__ __forceinline__ bf16x8 operand(const Raw<__bf16>& r) { return r.v; }
+
+// acc0/acc1 += the K range [kbeg, kend) of row Ar of A times columns B0, B1 of B (one wave; lane
+// group kg feeds K values 8 kg .. +7 of each 32-wide step). The loads of the next K step are issued
+// before the MFMAs of the current one (two register sets), so the loop waits for one load round at
+// its start instead of one per step.
+template <typename TA, typename TB>
+__device__ __forceinline__ void mma_loop(const TA* Ar, long sAk, const TB* B0, const TB* B1, long sBk, int kbeg,
+                                         int kend, int kg, bool aok, bool b0ok, bool b1ok, f32x4& acc0, f32x4& acc1) {
+    Raw<TA> ra;
+    Raw<TB> r0, r1;
+    load8(Ar, sAk, kbeg + 8 * kg, kend, aok, ra);
+    load8(B0, sBk, kbeg + 8 * kg, kend, b0ok, r0);
+    load8(B1, sBk, kbeg + 8 * kg, kend, b1ok, r1);
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
+        Raw<TA> na;
+        Raw<TB> n0r, n1r;
+        const int kn = k0 + 32 + 8 * kg;
+        load8(Ar, sAk, kn, kend, aok, na);
+        load8(B0, sBk, kn, kend, b0ok, n0r);
+        load8(B1, sBk, kn, kend, b1ok, n1r);
+        const bf16x8 a = operand(ra);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, operand(r0), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, operand(r1), acc1, 0, 0, 0);
+        ra = na;
+        r0 = n0r;
+        r1 = n1r;
+    }
 }
 
 // Wave w of the block owns rows m0 + 16 w .. +15 and both 16-column halves of the block's 32
 // columns: two 16x16x32 MFMAs per 32-wide K step. Lane l feeds row/column (l mod 16) with the 8
 // K values 8 (l / 16) .. +7 (the same K assignment for A and B), and receives rows
 // 4 (l / 16) .. +3 of column (l mod 16) of each 16x16 result.
+template <typename TA, typename TB>
 __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.z;
@@ -79,43 +153,29 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
     const int n0 = blockIdx.y * 32;
     const int r = lane & 15, kg = lane >> 4;
     int kbeg = 0, kend = g.K;
-    const float* A = g.A + (long)b * g.sAb;
-    const float* B = g.B + (long)b * g.sBb;
+    const TA* A;
+    const TB* B;
     if (g.nslice > 0) {
         const int pb = b / g.nslice, sl = b - pb * g.nslice;
         kbeg = sl * g.kchunk;
         kend = min(g.K, kbeg + g.kchunk);
-        A = g.A + (long)pb * g.sAb;
-        B = g.B + (long)pb * g.sBb;
+        A = (const TA*)g.A + boff(pb, g.nb0, g.sAb, g.sAb1);
+        B = (const TB*)g.B + boff(pb, g.nb0, g.sBb, g.sBb1);
     } else if (g.kchunk > 0) {
         kbeg = b * g.kchunk;
         kend = min(g.K, kbeg + g.kchunk);
-        A = g.A;
-        B = g.B;
+        A = (const TA*)g.A;
+        B = (const TB*)g.B;
+    } else {
+        A = (const TA*)g.A + boff(b, g.nb0, g.sAb, g.sAb1);
+        B = (const TB*)g.B + boff(b, g.nb0, g.sBb, g.sBb1);
     }
-    float* C = g.C + (long)b * g.sCb;
+    float* C = g.C + ((g.nslice > 0 || g.kchunk > 0) ? (long)b * g.sCb : boff(b, g.nb0, g.sCb, g.sCb1));
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     const int am = m0 + r, bn0 = n0 + r, bn1 = n0 + 16 + r;
     const bool aok = am < g.M, b0ok = bn0 < g.N, b1ok = bn1 < g.N;
-    const float* Ar = A + (long)am * g.sAm;
-    const float* B0 = B + (long)bn0 * g.sBn;
-    const float* B1 = B + (long)bn1 * g.sBn;
-    for (int k0 = kbeg; k0 < kend; k0 += 32) {
-        bf16x8 a, x0, x1;
-        float av[8], v0[8], v1[8];
-        const int kk = k0 + 8 * kg;
-        load8(Ar, g.sAk, kk, kend, aok, av);
-        load8(B0, g.sBk, kk, kend, b0ok, v0);
-        load8(B1, g.sBk, kk, kend, b1ok, v1);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            a[e] = (__bf16)av[e];
-            x0[e] = (__bf16)v0[e];
-            x1[e] = (__bf16)v1[e];
-        }
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x1, acc1, 0, 0, 0);
-    }
+    mma_loop(A + (long)am * g.sAm, g.sAk, B + (long)bn0 * g.sBn, B + (long)bn1 * g.sBn, g.sBk, kbeg, kend, kg, aok,
+             b0ok, b1ok, acc0, acc1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int row = m0 + 4 * kg + i;
@@ -149,11 +209,12 @@ __global__ void __launch_bounds__(256) k_splitk_reduce(const float* __restrict__
     }
 }
 
-// dst[p](m, n) = beta dst[p](m, n) + sum_s part[p][s][m][n] for products p: one thread per output
-// (consecutive threads read consecutive partials), slices summed in order (deterministic)
+// dst[p](m, n) = beta dst[p](m, n) + sum_s part[p][s][m][n] for products p, dst[p] at
+// dst + (p mod nb0) sp + (p / nb0) sp1: one thread per output (consecutive threads read consecutive
+// partials), slices summed in order (deterministic)
 __global__ void __launch_bounds__(256) k_splitk_reduce_batched(const float* __restrict__ part, int nslice, int M,
-                                                               int N, int nprod, float* dst, long sp, long sm,
-                                                               long sn, float beta) {
+                                                               int N, int nprod, float* dst, long sp, int nb0,
+                                                               long sp1, long sm, long sn, float beta) {
     const long MN = (long)M * N;
     for (long out = (long)blockIdx.x * blockDim.x + threadIdx.x; out < MN * nprod;
          out += (long)gridDim.x * blockDim.x) {
@@ -162,7 +223,7 @@ __global__ void __launch_bounds__(256) k_splitk_reduce_batched(const float* __re
         float s = 0.f;
         for (int sl = 0; sl < nslice; ++sl) s += pp[(long)sl * MN];
         const int m = (int)(mn / N), n = (int)(mn - (long)m * N);
-        float* d = dst + pb * sp + (long)m * sm + (long)n * sn;
+        float* d = dst + boff((int)pb, nb0, sp, sp1) + (long)m * sm + (long)n * sn;
         *d = (beta != 0.f) ? beta * *d + s : s;
     }
 }
@@ -170,6 +231,122 @@ __global__ void __launch_bounds__(256) k_splitk_reduce_batched(const float* __re
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_d(float z) {
     return 0.5f * (1.f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// A layer's product and its elementwise step in one kernel. For product b (a row of a sample, or
+// all cells at once):
+//   acc = A[b] B[b] (+ A2[b] B2[b]: a second K segment accumulated in the same MFMA registers)
+// then, at out + b sOb + m N + n,
+//   EPI_BIAS_ACT: z = acc + bias[n]; zout = z (if zout); vout = BF16(act ? gelu(z) : z)
+//   EPI_ACT_BACK: vout = BF16(zin ? acc gelu'(zin) : acc)
+// A is FP32 or BF16 (TA); B, A2, B2 are FP32. Tile mapping as in k_gemm.
+enum { EPI_BIAS_ACT = 0, EPI_ACT_BACK = 1 };
+struct Gemm2 {
+    const void* A;
+    const float* B;
+    long sAb, sAm, sAk, sBb, sBk, sBn;
+    int K;
+    const float* A2;
+    const float* B2;
+    long sA2b, sA2m, sA2k, sB2b, sB2k, sB2n;
+    int K2;
+    int M, N;
+    long sOb;
+    int epi, act;
+    const float* bias;
+    float* zout;
+    __bf16* vout;
+    const float* zin;
+};
+
+// B tile of one segment into LDS as BF16, [column][k] with k contiguous (pitch kr + 8, kr = K
+// rounded up to 32, zeros past K), loaded by the whole workgroup; consecutive threads take
+// consecutive addresses of whichever index is contiguous in memory.
+__device__ __forceinline__ void stage_b(__bf16* sb, const float* B, long sBk, long sBn, int K, int kr, int n0, int N) {
+    const int pitch = kr + 8;
+    const bool nfast = sBn == 1;
+    for (int e = threadIdx.x; e < 32 * kr; e += GEMM_THREADS) {
+        const int n = nfast ? (e & 31) : e / kr;
+        const int k = nfast ? (e >> 5) : e - (e / kr) * kr;
+        const float v = (k < K && n0 + n < N) ? B[(long)k * sBk + (long)(n0 + n) * sBn] : 0.f;
+        sb[n * pitch + k] = (__bf16)v;
+    }
+}
+
+__device__ __forceinline__ bf16x8 lds8(const __bf16* sb, int pitch, int n, int k) {
+    return *reinterpret_cast<const bf16x8*>(sb + n * pitch + k);
+}
+
+// Each workgroup stages its 32 columns of B (and of B2) in LDS once, so the four waves share them;
+// the A rows of both segments and the epilogue's z values are loaded before the first MFMA, so a
+// workgroup of the layer products (K = width, K2 = 2 modes_x: one step each) waits for one load
+// round. Dynamic LDS: 32 (kr + 8) + 32 (kr2 + 8) BF16.
+template <typename TA>
+__global__ void __launch_bounds__(GEMM_THREADS) k_gemm2(const Gemm2 g) {
+    extern __shared__ __align__(16) __bf16 sbuf[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long b = blockIdx.z;
+    const int m0 = blockIdx.x * 64 + wave * 16;
+    const int n0 = blockIdx.y * 32;
+    const int r = lane & 15, kg = lane >> 4;
+    const int kr = (g.K + 31) & ~31, kr2 = (g.K2 + 31) & ~31;
+    __bf16* sb1 = sbuf;
+    __bf16* sb2 = sbuf + 32 * (kr + 8);
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int am = m0 + r;
+    const bool aok = am < g.M, b0ok = n0 + r < g.N, b1ok = n0 + 16 + r < g.N;
+    const TA* Ar = (const TA*)g.A + b * g.sAb + (long)am * g.sAm;
+    const bool two = g.K2 > 0;
+    const float* A2r = two ? g.A2 + b * g.sA2b + (long)am * g.sA2m : nullptr;
+    Raw<TA> ra;
+    Raw<float> qa;
+    load8(Ar, g.sAk, 8 * kg, g.K, aok, ra);
+    if (two) load8(A2r, g.sA2k, 8 * kg, g.K2, aok, qa);
+    float zv[4][2];
+    const bool zl = g.epi == EPI_ACT_BACK && g.zin;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = m0 + 4 * kg + i;
+        const long o = b * g.sOb + (long)row * g.N;
+        zv[i][0] = (zl && row < g.M && b0ok) ? g.zin[o + n0 + r] : 0.f;
+        zv[i][1] = (zl && row < g.M && b1ok) ? g.zin[o + n0 + 16 + r] : 0.f;
+    }
+    stage_b(sb1, g.B + b * g.sBb, g.sBk, g.sBn, g.K, kr, n0, g.N);
+    if (two) stage_b(sb2, g.B2 + b * g.sB2b, g.sB2k, g.sB2n, g.K2, kr2, n0, g.N);
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+        if (k0 > 0) load8(Ar, g.sAk, k0 + 8 * kg, g.K, aok, ra);
+        const bf16x8 a = operand(ra);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, lds8(sb1, kr + 8, r, k0 + 8 * kg), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, lds8(sb1, kr + 8, 16 + r, k0 + 8 * kg), acc1, 0, 0, 0);
+    }
+    if (two) {
+        for (int k0 = 0; k0 < g.K2; k0 += 32) {
+            if (k0 > 0) load8(A2r, g.sA2k, k0 + 8 * kg, g.K2, aok, qa);
+            const bf16x8 a = operand(qa);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, lds8(sb2, kr2 + 8, r, k0 + 8 * kg), acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, lds8(sb2, kr2 + 8, 16 + r, k0 + 8 * kg), acc1, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = m0 + 4 * kg + i;
+        if (row >= g.M) continue;
+        const long o = b * g.sOb + (long)row * g.N;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int n = n0 + 16 * h + r;
+            if (!(h ? b1ok : b0ok)) continue;
+            const float a = h ? acc1[i] : acc0[i];
+            if (g.epi == EPI_BIAS_ACT) {
+                const float z = a + g.bias[n];
+                if (g.zout) g.zout[o + n] = z;
+                g.vout[o + n] = (__bf16)(g.act ? gelu(z) : z);
+            } else {
+                g.vout[o + n] = (__bf16)(zl ? a * gelu_d(zv[i][h]) : a);
+            }
+        }
+    }
 }
 
 // z += bias (per channel); out = gelu(z) or z
@@ -182,19 +359,15 @@ __global__ void k_bias_act(float* z, const float* __restrict__ bias, long P, int
     }
 }
 
-// d *= gelu'(z)
-__global__ void k_gelu_back(float* d, const float* __restrict__ z, long n) {
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x)
-        d[idx] *= gelu_d(z[idx]);
-}
-
-// H[kx][r][o] = complex sum_c G[kx][r][c] R[kx][m][c][o]   (r = 2 m + re/im)
-__global__ void k_mix_fwd(const float* __restrict__ G, const float* __restrict__ R, float* H, int nmode, int C) {
+// H[kx][r][o] = complex sum_c G[kx][r][c] R[kx][m][c][o]   (r = 2 m + re/im), for the nmode
+// modes of each of nb samples (G, H: [nb][nmode][2][C])
+__global__ void k_mix_fwd(const float* __restrict__ G, const float* __restrict__ R, float* H, int nmode, int C,
+                          int nb) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nmode * C) return;
+    if (idx >= nb * nmode * C) return;
     const int mode = idx / C, o = idx - mode * C;
     const float* g = G + (long)mode * 2 * C;
-    const float* w = R + (long)mode * C * C * 2;
+    const float* w = R + (long)(mode % nmode) * C * C * 2;
     float hr = 0.f, hi = 0.f;
     for (int c = 0; c < C; ++c) {
         const float gr = g[c], gi = g[C + c];
@@ -207,12 +380,13 @@ __global__ void k_mix_fwd(const float* __restrict__ G, const float* __restrict__
 }
 
 // dG[kx][r][c] = sum_o dH R^*-style terms (real form of the complex product's adjoint)
-__global__ void k_mix_back_g(const float* __restrict__ dH, const float* __restrict__ R, float* dG, int nmode, int C) {
+__global__ void k_mix_back_g(const float* __restrict__ dH, const float* __restrict__ R, float* dG, int nmode, int C,
+                             int nb) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nmode * C) return;
+    if (idx >= nb * nmode * C) return;
     const int mode = idx / C, c = idx - mode * C;
     const float* d = dH + (long)mode * 2 * C;
-    const float* w = R + (long)mode * C * C * 2 + (long)c * C * 2;
+    const float* w = R + (long)(mode % nmode) * C * C * 2 + (long)c * C * 2;
     float gr = 0.f, gi = 0.f;
     for (int o = 0; o < C; ++o) {
         const float dr = d[o], di = d[C + o];
@@ -224,26 +398,34 @@ __global__ void k_mix_back_g(const float* __restrict__ dH, const float* __restri
     dG[(long)mode * 2 * C + C + c] = gi;
 }
 
-// dR[kx][m][c][o] += (dHr Gr + dHi Gi, -dHr Gi + dHi Gr)
-__global__ void k_mix_back_r(const float* __restrict__ dH, const float* __restrict__ G, float* dR, int nmode, int C) {
+// dR[kx][m][c][o] += (dHr Gr + dHi Gi, -dHr Gi + dHi Gr), summed over the nb samples in order
+__global__ void k_mix_back_r(const float* __restrict__ dH, const float* __restrict__ G, float* dR, int nmode, int C,
+                             int nb) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)nmode * C * C) return;
     const int mode = (int)(idx / (C * C));
     const int rem = (int)(idx - (long)mode * C * C);
     const int c = rem / C, o = rem - c * C;
-    const float* d = dH + (long)mode * 2 * C;
-    const float* g = G + (long)mode * 2 * C;
-    const float dr = d[o], di = d[C + o], gr = g[c], gi = g[C + c];
     float* w = dR + idx * 2;
-    w[0] += dr * gr + di * gi;
-    w[1] += -dr * gi + di * gr;
+    float w0 = w[0], w1 = w[1];
+    for (int sm = 0; sm < nb; ++sm) {
+        const float* d = dH + ((long)sm * nmode + mode) * 2 * C;
+        const float* g = G + ((long)sm * nmode + mode) * 2 * C;
+        const float dr = d[o], di = d[C + o], gr = g[c], gi = g[C + c];
+        w0 += dr * gr + di * gi;
+        w1 += -dr * gi + di * gr;
+    }
+    w[0] = w0;
+    w[1] = w1;
 }
 
-// dy = 2/P (y - t); loss partial sums per block
-__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, long P, float* dy, double* part) {
+// dy = 2/P (y - t) over n = nb P cells of nb samples; loss partial sums per block (the sum of the nb
+// samples' mean squared errors)
+__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, long P, long n, float* dy,
+                      double* part) {
     __shared__ double red[256];
     double s = 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (long)gridDim.x * blockDim.x) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const float d = y[i] - t[i];
         dy[i] = 2.f * d / (float)P;
         s += (double)d * d;
@@ -262,6 +444,25 @@ __global__ void k_sum_parts(const double* __restrict__ part, int n, double* out)
         double s = 0.0;
         for (int i = 0; i < n; ++i) s += part[i];
         *out = s;
+    }
+}
+
+// the inputs and targets of the training samples idx[0..nb-1], side by side
+struct Picks {
+    int k[64];
+};
+__global__ void k_gather(const float* __restrict__ xs, const float* __restrict__ ys, long nxs, long nys,
+                         const Picks pk, int nb, float* x, float* y) {
+    const long n = (long)nb * (nxs + nys);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        if (i < nb * nxs) {
+            const int sm = (int)(i / nxs);
+            x[i] = xs[(long)pk.k[sm] * nxs + (i - sm * nxs)];
+        } else {
+            const long j = i - nb * nxs;
+            const int sm = (int)(j / nys);
+            y[j] = ys[(long)pk.k[sm] * nys + (j - sm * nys)];
+        }
     }
 }
 
@@ -326,12 +527,18 @@ struct fi_surrogate {
     // grid
     int nx = 0, ny = 0;
     long P = 0;
-    DB<float> Ex, Ar, Ai, Bx;
-    DB<float> x, y, dy, tgt;                // current sample
-    std::vector<DB<float>> v, z;            // v[0..L], z[0..L-1]
-    DB<float> Fx, G, H, Fy, dFx, dG, dH, dFy;
+    DB<float> Ex, Ar, Ai, Bx, ExT;   // ExT: Ex transposed, [nx][2 modes_x]
+    // activations of up to cap samples side by side ([sample][cell][channel]); the layer outputs v,
+    // the hidden projection q and the backward products dq, dz feed only MFMA products and are
+    // stored as BF16, the pre-activations z and q_pre as FP32
+    int cap = 0;
+    DB<float> x, y, dy, tgt;
+    std::vector<DB<__bf16>> v;              // v[0..L]
+    std::vector<DB<float>> z;               // z[0..L]
+    DB<float> Fx, H, Fy, dFx, dG, dH, dFy;
     std::vector<DB<float>> Gs;              // G per layer (kept for the backward pass)
-    DB<float> q_pre, q, dq, dv, dz, part;
+    DB<float> q_pre, part;
+    DB<__bf16> q, dqb, dzb[2];
     DB<double> lpart, lsum;
     long part_cap = 0;
     // training set
@@ -355,12 +562,35 @@ int sfail(fi_surrogate* s, int code, const std::string& m) {
 
 constexpr int S_ERR_INVALID = -1, S_ERR_STATE = -6;
 
+// an FP32 or BF16 operand
+struct Src {
+    const void* p;
+    bool bf;
+    Src(const float* q) : p(q), bf(false) {}
+    Src(const __bf16* q) : p(q), bf(true) {}
+};
+
+void launch_gemm(fi_surrogate* s, const Gemm& g, dim3 grid, bool abf, bool bbf) {
+    if (abf && bbf)
+        hipLaunchKernelGGL((k_gemm<__bf16, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+    else if (abf)
+        hipLaunchKernelGGL((k_gemm<__bf16, float>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+    else if (bbf)
+        hipLaunchKernelGGL((k_gemm<float, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+    else
+        hipLaunchKernelGGL((k_gemm<float, float>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+}
+
+constexpr int NB0_NONE = 1 << 30;
+
 // plain product. A batch of few output tiles over a long K (the y transforms per retained kx: 16
-// products of one 64 x 32 tile over K = ny; the x transforms per row: ny products over K = nx) is
-// split along K so that the launch has about 1024 workgroups (the K loop is load-latency bound),
-// then reduced in a fixed order.
-int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, long sBk, long sBn, float* C, long sCm,
-         long sCn, int M, int N, int K, float beta, int batch = 1, long sAb = 0, long sBb = 0, long sCb = 0) {
+// products of one 64 x 32 tile over K = ny per sample; the x transforms per row: ny products over
+// K = nx per sample) is split along K so that the launch has about 1024 workgroups (the K loop is
+// load-latency bound), then reduced in a fixed order. Product b's operands sit at
+// X + (b mod nb0) sXb + (b / nb0) sXb1.
+int gemm(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, float* C, long sCm, long sCn, int M,
+         int N, int K, float beta, int batch = 1, long sAb = 0, long sBb = 0, long sCb = 0, int nb0 = NB0_NONE,
+         long sAb1 = 0, long sBb1 = 0, long sCb1 = 0) {
     const long tiles = (long)((M + 63) / 64) * ((N + 31) / 32);
     if (batch > 1 && tiles * batch < 1024 && K >= 128) {
         const int steps = (K + 31) / 32;
@@ -373,36 +603,36 @@ int gemm(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, lo
             SCHECK(s, s->part.alloc(need));
             s->part_cap = need;
         }
-        Gemm g{A, B, s->part.p, sAb, sAm, sAk, sBb, sBk, sBn, MN, N, 1, M, N, K, kchunk, 1.f, 0.f, nslice};
-        dim3 grid((M + 63) / 64, (N + 31) / 32, batch * nslice);
-        hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        Gemm g{A.p, B.p, s->part.p, sAb, sAm, sAk, sBb, sBk, sBn, MN, N, 1, M, N, K, kchunk, 1.f, 0.f, nslice,
+               nb0, sAb1, sBb1, 0};
+        launch_gemm(s, g, dim3((M + 63) / 64, (N + 31) / 32, batch * nslice), A.bf, B.bf);
         SCHECK(s, hipGetLastError());
         const long outs = MN * batch;
         hipLaunchKernelGGL(k_splitk_reduce_batched, dim3((unsigned)std::min<long>((outs + 255) / 256, 4096)), dim3(256),
-                           0, s->stream, s->part.p, nslice, M, N, batch, C, sCb, sCm, sCn, beta);
+                           0, s->stream, s->part.p, nslice, M, N, batch, C, sCb, nb0, sCb1, sCm, sCn, beta);
         SCHECK(s, hipGetLastError());
         return 0;
     }
-    Gemm g{A, B, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta, 0};
-    dim3 grid((M + 63) / 64, (N + 31) / 32, batch);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
+    Gemm g{A.p, B.p, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta, 0, nb0, sAb1, sBb1, sCb1};
+    launch_gemm(s, g, dim3((M + 63) / 64, (N + 31) / 32, batch), A.bf, B.bf);
     SCHECK(s, hipGetLastError());
     return 0;
 }
 
-// product with a long K (parameter gradients): split-K partials, then a fixed-order reduction
-int gemm_splitk(fi_surrogate* s, const float* A, long sAm, long sAk, const float* B, long sBk, long sBn, float* C,
-                long sCm, long sCn, int M, int N, int K, float beta) {
-    const int kchunk = 128;   // ~960 slices over the 122 500 cells of an N=350 grid
+// product with a long K (parameter gradients, K = the cells of the samples): split-K partials over
+// about 2048 slices of at least 128, then a fixed-order reduction
+int gemm_splitk(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, float* C, long sCm, long sCn,
+                int M, int N, int K, float beta) {
+    const int kchunk = std::max(128, 32 * (int)(((long)K + 2048L * 32 - 1) / (2048L * 32)));
     const int nsplit = (K + kchunk - 1) / kchunk;
     const long need = (long)nsplit * M * N;
     if (need > s->part_cap) {
         SCHECK(s, s->part.alloc(need));
         s->part_cap = need;
     }
-    Gemm g{A, B, s->part.p, 0, sAm, sAk, 0, sBk, sBn, (long)M * N, N, 1, M, N, K, kchunk, 1.f, 0.f, 0};
-    dim3 grid((M + 63) / 64, (N + 31) / 32, nsplit);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(GEMM_THREADS), 0, s->stream, g);
+    Gemm g{A.p, B.p, s->part.p, 0, sAm, sAk, 0, sBk, sBn, (long)M * N, N, 1, M, N, K, kchunk, 1.f, 0.f, 0,
+           NB0_NONE, 0, 0, 0};
+    launch_gemm(s, g, dim3((M + 63) / 64, (N + 31) / 32, nsplit), A.bf, B.bf);
     SCHECK(s, hipGetLastError());
     hipLaunchKernelGGL(k_splitk_reduce, dim3((M * N + 3) / 4), dim3(256), 0, s->stream, s->part.p, nsplit, M, N, C,
                        sCm, sCn, beta);
@@ -410,129 +640,298 @@ int gemm_splitk(fi_surrogate* s, const float* A, long sAm, long sAk, const float
     return 0;
 }
 
-int ew_grid(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
-
-int forward(fi_surrogate* s, const float* x) {
-    const fi_fno_config& c = s->cfg;
-    const int C = c.width, L = c.layers, X2 = 2 * c.modes_x, R4 = 4 * c.modes_y, mx = c.modes_x, Hh = c.proj_hidden;
-    const int nx = s->nx, ny = s->ny;
-    const long P = s->P;
-    float* prm = s->par.p;
-    int st;
-    // lift
-    if ((st = gemm(s, x, c.in_channels, 1, prm + s->oP, C, 1, s->z[0].p, C, 1, (int)P, C, c.in_channels, 0.f))) return st;
-    hipLaunchKernelGGL(k_bias_act, dim3(ew_grid(P * C)), dim3(256), 0, s->stream, s->z[0].p, prm + s->op, P, C, 0,
-                       s->v[0].p);
-    for (int l = 0; l < L; ++l) {
-        const float* vl = s->v[l].p;
-        float* zl = s->z[l + 1].p;
-        float* G = s->Gs[l].p;
-        // x-DFT per row j
-        if ((st = gemm(s, s->Ex.p, nx, 1, vl, C, 1, s->Fx.p, C, 1, X2, C, nx, 0.f, ny, 0, (long)nx * C, (long)X2 * C)))
-            return st;
-        // y-DFT per kx: G = Ar Re(Fx) + Ai Im(Fx)
-        if ((st = gemm(s, s->Ar.p, ny, 1, s->Fx.p, (long)X2 * C, 1, G, C, 1, R4, C, ny, 0.f, mx, 0, 2L * C, (long)R4 * C)))
-            return st;
-        if ((st = gemm(s, s->Ai.p, ny, 1, s->Fx.p + C, (long)X2 * C, 1, G, C, 1, R4, C, ny, 1.f, mx, 0, 2L * C,
-                       (long)R4 * C)))
-            return st;
-        hipLaunchKernelGGL(k_mix_fwd, dim3((mx * 2 * c.modes_y * C + 255) / 256), dim3(256), 0, s->stream, G,
-                           prm + s->oR[l], s->H.p, mx * 2 * c.modes_y, C);
-        // y-iDFT per kx: Re F' = Ar^T H, Im F' = Ai^T H
-        if ((st = gemm(s, s->Ar.p, 1, ny, s->H.p, C, 1, s->Fy.p, (long)X2 * C, 1, ny, C, R4, 0.f, mx, 0, (long)R4 * C,
-                       2L * C)))
-            return st;
-        if ((st = gemm(s, s->Ai.p, 1, ny, s->H.p, C, 1, s->Fy.p + C, (long)X2 * C, 1, ny, C, R4, 0.f, mx, 0,
-                       (long)R4 * C, 2L * C)))
-            return st;
-        // pointwise W, then + x-iDFT per row j, then bias and activation
-        if ((st = gemm(s, vl, C, 1, prm + s->oW[l], C, 1, zl, C, 1, (int)P, C, C, 0.f))) return st;
-        if ((st = gemm(s, s->Bx.p, X2, 1, s->Fy.p, C, 1, zl, C, 1, nx, C, X2, 1.f, ny, 0, (long)X2 * C, (long)nx * C)))
-            return st;
-        hipLaunchKernelGGL(k_bias_act, dim3(ew_grid(P * C)), dim3(256), 0, s->stream, zl, prm + s->ob[l], P, C,
-                           l < L - 1 ? 1 : 0, s->v[l + 1].p);
-    }
-    // projection
-    if ((st = gemm(s, s->v[L].p, C, 1, prm + s->oQ1, Hh, 1, s->q_pre.p, Hh, 1, (int)P, Hh, C, 0.f))) return st;
-    hipLaunchKernelGGL(k_bias_act, dim3(ew_grid(P * Hh)), dim3(256), 0, s->stream, s->q_pre.p, prm + s->oq1, P, Hh, 1,
-                       s->q.p);
-    if ((st = gemm(s, s->q.p, Hh, 1, prm + s->oQ2, 1, 1, s->y.p, 1, 1, (int)P, 1, Hh, 0.f))) return st;
-    hipLaunchKernelGGL(k_bias_act, dim3(ew_grid(P)), dim3(256), 0, s->stream, s->y.p, prm + s->oq2, P, 1, 0, s->y.p);
+// fused layer product (k_gemm2) over batch products
+int gemm2(fi_surrogate* s, const Gemm2& g, bool abf, int batch) {
+    const dim3 grid((g.M + 63) / 64, (g.N + 31) / 32, batch);
+    const int kr = (g.K + 31) & ~31, kr2 = g.K2 > 0 ? (g.K2 + 31) & ~31 : 0;
+    const size_t lds = (size_t)32 * ((kr + 8) + (g.K2 > 0 ? kr2 + 8 : 0)) * sizeof(__bf16);
+    if (lds > 64 * 1024) return sfail(s, S_ERR_INVALID, "fused product: K too large");
+    if (abf)
+        hipLaunchKernelGGL(k_gemm2<__bf16>, grid, dim3(GEMM_THREADS), lds, s->stream, g);
+    else
+        hipLaunchKernelGGL(k_gemm2<float>, grid, dim3(GEMM_THREADS), lds, s->stream, g);
     SCHECK(s, hipGetLastError());
     return 0;
 }
 
-// gradient of the loss with respect to every parameter, accumulated into s->grad (after
-// forward(x); s->dy holds dL/dy)
-int backward(fi_surrogate* s, const float* x) {
+int ew_grid(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
+
+// forward pass of nb samples side by side (x: [nb][cell][in_channels], nb <= cap)
+int forward(fi_surrogate* s, const float* x, int nb) {
     const fi_fno_config& c = s->cfg;
     const int C = c.width, L = c.layers, X2 = 2 * c.modes_x, R4 = 4 * c.modes_y, mx = c.modes_x, Hh = c.proj_hidden;
-    const int nx = s->nx, ny = s->ny;
-    const long P = s->P;
+    const int nx = s->nx, ny = s->ny, nmode = mx * 2 * c.modes_y;
+    const long PB = s->P * nb;
+    const long sFs = (long)ny * X2 * C, sGs = (long)mx * R4 * C;   // per-sample strides of Fx/Fy and G/H
+    float* prm = s->par.p;
+    int st;
+    // lift: v_0 = x P + p
+    {
+        Gemm2 g{};
+        g.A = x;
+        g.sAm = c.in_channels;
+        g.sAk = 1;
+        g.B = prm + s->oP;
+        g.sBk = C;
+        g.sBn = 1;
+        g.K = c.in_channels;
+        g.M = (int)PB;
+        g.N = C;
+        g.epi = EPI_BIAS_ACT;
+        g.bias = prm + s->op;
+        g.vout = s->v[0].p;
+        if ((st = gemm2(s, g, false, 1))) return st;
+    }
+    for (int l = 0; l < L; ++l) {
+        const __bf16* vl = s->v[l].p;
+        float* zl = s->z[l + 1].p;
+        float* G = s->Gs[l].p;
+        // x-DFT per row j of each sample
+        if ((st = gemm(s, s->Ex.p, nx, 1, vl, C, 1, s->Fx.p, C, 1, X2, C, nx, 0.f, nb * ny, 0, (long)nx * C,
+                       (long)X2 * C)))
+            return st;
+        // y-DFT per (sample, kx): G = Ar Re(Fx) + Ai Im(Fx)
+        if ((st = gemm(s, s->Ar.p, ny, 1, s->Fx.p, (long)X2 * C, 1, G, C, 1, R4, C, ny, 0.f, nb * mx, 0, 2L * C,
+                       (long)R4 * C, mx, 0, sFs, sGs)))
+            return st;
+        if ((st = gemm(s, s->Ai.p, ny, 1, s->Fx.p + C, (long)X2 * C, 1, G, C, 1, R4, C, ny, 1.f, nb * mx, 0, 2L * C,
+                       (long)R4 * C, mx, 0, sFs, sGs)))
+            return st;
+        hipLaunchKernelGGL(k_mix_fwd, dim3((nb * nmode * C + 255) / 256), dim3(256), 0, s->stream, G, prm + s->oR[l],
+                           s->H.p, nmode, C, nb);
+        // y-iDFT per (sample, kx): Re F' = Ar^T H, Im F' = Ai^T H
+        if ((st = gemm(s, s->Ar.p, 1, ny, s->H.p, C, 1, s->Fy.p, (long)X2 * C, 1, ny, C, R4, 0.f, nb * mx, 0,
+                       (long)R4 * C, 2L * C, mx, 0, sGs, sFs)))
+            return st;
+        if ((st = gemm(s, s->Ai.p, 1, ny, s->H.p, C, 1, s->Fy.p + C, (long)X2 * C, 1, ny, C, R4, 0.f, nb * mx, 0,
+                       (long)R4 * C, 2L * C, mx, 0, sGs, sFs)))
+            return st;
+        // per row j of each sample: z = v_l W + x-iDFT(F') + b, v_{l+1} = gelu(z) (z for the last layer);
+        // z is kept where the backward pass needs gelu'(z)
+        Gemm2 g{};
+        g.A = vl;
+        g.sAb = (long)nx * C;
+        g.sAm = C;
+        g.sAk = 1;
+        g.B = prm + s->oW[l];
+        g.sBk = C;
+        g.sBn = 1;
+        g.K = C;
+        g.A2 = s->Bx.p;
+        g.sA2m = X2;
+        g.sA2k = 1;
+        g.B2 = s->Fy.p;
+        g.sB2b = (long)X2 * C;
+        g.sB2k = C;
+        g.sB2n = 1;
+        g.K2 = X2;
+        g.M = nx;
+        g.N = C;
+        g.sOb = (long)nx * C;
+        g.epi = EPI_BIAS_ACT;
+        g.act = l < L - 1;
+        g.bias = prm + s->ob[l];
+        g.zout = l < L - 1 ? zl : nullptr;
+        g.vout = s->v[l + 1].p;
+        if ((st = gemm2(s, g, true, nb * ny))) return st;
+    }
+    // projection: q = gelu(v_L Q1 + q1)
+    {
+        Gemm2 g{};
+        g.A = s->v[L].p;
+        g.sAm = C;
+        g.sAk = 1;
+        g.B = prm + s->oQ1;
+        g.sBk = Hh;
+        g.sBn = 1;
+        g.K = C;
+        g.M = (int)PB;
+        g.N = Hh;
+        g.epi = EPI_BIAS_ACT;
+        g.act = 1;
+        g.bias = prm + s->oq1;
+        g.zout = s->q_pre.p;
+        g.vout = s->q.p;
+        if ((st = gemm2(s, g, true, 1))) return st;
+    }
+    if ((st = gemm(s, s->q.p, Hh, 1, prm + s->oQ2, 1, 1, s->y.p, 1, 1, (int)PB, 1, Hh, 0.f))) return st;
+    hipLaunchKernelGGL(k_bias_act, dim3(ew_grid(PB)), dim3(256), 0, s->stream, s->y.p, prm + s->oq2, PB, 1, 0,
+                       s->y.p);
+    SCHECK(s, hipGetLastError());
+    return 0;
+}
+
+// gradient of the loss summed over the nb samples with respect to every parameter, accumulated
+// into s->grad (after forward(x, nb); s->dy holds dL/dy)
+int backward(fi_surrogate* s, const float* x, int nb) {
+    const fi_fno_config& c = s->cfg;
+    const int C = c.width, L = c.layers, X2 = 2 * c.modes_x, R4 = 4 * c.modes_y, mx = c.modes_x, Hh = c.proj_hidden;
+    const int nx = s->nx, ny = s->ny, nmode = mx * 2 * c.modes_y;
+    const long PB = s->P * nb;
+    const long sFs = (long)ny * X2 * C, sGs = (long)mx * R4 * C;
     float* prm = s->par.p;
     float* gr = s->grad.p;
     const float* one = s->ones.p;
     int st;
     // y = q Q2 + q2
-    if ((st = gemm_splitk(s, s->q.p, 1, Hh, s->dy.p, 1, 1, gr + s->oQ2, 1, 1, Hh, 1, (int)P, 1.f))) return st;
-    if ((st = gemm_splitk(s, one, 0, 1, s->dy.p, 1, 1, gr + s->oq2, 1, 1, 1, 1, (int)P, 1.f))) return st;
-    if ((st = gemm(s, s->dy.p, 1, 1, prm + s->oQ2, 1, 1, s->dq.p, Hh, 1, (int)P, Hh, 1, 0.f))) return st;
-    hipLaunchKernelGGL(k_gelu_back, dim3(ew_grid(P * Hh)), dim3(256), 0, s->stream, s->dq.p, s->q_pre.p, P * Hh);
+    if ((st = gemm_splitk(s, s->q.p, 1, Hh, s->dy.p, 1, 1, gr + s->oQ2, 1, 1, Hh, 1, (int)PB, 1.f))) return st;
+    if ((st = gemm_splitk(s, one, 0, 1, s->dy.p, 1, 1, gr + s->oq2, 1, 1, 1, 1, (int)PB, 1.f))) return st;
+    {   // dq = (dy Q2^T) gelu'(q_pre)
+        Gemm2 g{};
+        g.A = s->dy.p;
+        g.sAm = 1;
+        g.sAk = 1;
+        g.B = prm + s->oQ2;
+        g.sBk = 1;
+        g.sBn = 1;
+        g.K = 1;
+        g.M = (int)PB;
+        g.N = Hh;
+        g.epi = EPI_ACT_BACK;
+        g.zin = s->q_pre.p;
+        g.vout = s->dqb.p;
+        if ((st = gemm2(s, g, false, 1))) return st;
+    }
     // q_pre = v_L Q1 + q1
-    if ((st = gemm_splitk(s, s->v[L].p, 1, C, s->dq.p, Hh, 1, gr + s->oQ1, Hh, 1, C, Hh, (int)P, 1.f))) return st;
-    if ((st = gemm_splitk(s, one, 0, 1, s->dq.p, Hh, 1, gr + s->oq1, 1, 1, 1, Hh, (int)P, 1.f))) return st;
-    if ((st = gemm(s, s->dq.p, Hh, 1, prm + s->oQ1, 1, Hh, s->dv.p, C, 1, (int)P, C, Hh, 0.f))) return st;
+    if ((st = gemm_splitk(s, s->v[L].p, 1, C, s->dqb.p, Hh, 1, gr + s->oQ1, Hh, 1, C, Hh, (int)PB, 1.f))) return st;
+    if ((st = gemm_splitk(s, one, 0, 1, s->dqb.p, Hh, 1, gr + s->oq1, 1, 1, 1, Hh, (int)PB, 1.f))) return st;
+    int cur = 0;
+    {   // dz_L = dv_L = dq Q1^T (the last layer has no activation)
+        Gemm2 g{};
+        g.A = s->dqb.p;
+        g.sAm = Hh;
+        g.sAk = 1;
+        g.B = prm + s->oQ1;
+        g.sBk = 1;
+        g.sBn = Hh;
+        g.K = Hh;
+        g.M = (int)PB;
+        g.N = C;
+        g.epi = EPI_ACT_BACK;
+        g.vout = s->dzb[cur].p;
+        if ((st = gemm2(s, g, true, 1))) return st;
+    }
     for (int l = L - 1; l >= 0; --l) {
-        // dz = dv (last layer) or dv * gelu'(z)
-        SCHECK(s, hipMemcpyAsync(s->dz.p, s->dv.p, P * C * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
-        if (l < L - 1)
-            hipLaunchKernelGGL(k_gelu_back, dim3(ew_grid(P * C)), dim3(256), 0, s->stream, s->dz.p, s->z[l + 1].p, P * C);
-        const float* vl = s->v[l].p;
-        if ((st = gemm_splitk(s, one, 0, 1, s->dz.p, C, 1, gr + s->ob[l], 1, 1, 1, C, (int)P, 1.f))) return st;
-        if ((st = gemm_splitk(s, vl, 1, C, s->dz.p, C, 1, gr + s->oW[l], C, 1, C, C, (int)P, 1.f))) return st;
-        // dv_l = dz W^T
-        if ((st = gemm(s, s->dz.p, C, 1, prm + s->oW[l], 1, C, s->dv.p, C, 1, (int)P, C, C, 0.f))) return st;
-        // spectral path: dF' = Bx^T dz per row j
-        if ((st = gemm(s, s->Bx.p, 1, X2, s->dz.p, C, 1, s->dFy.p, C, 1, X2, C, nx, 0.f, ny, 0, (long)nx * C,
+        // dz: gradient with respect to z_{l+1}
+        const __bf16* vl = s->v[l].p;
+        const __bf16* dz = s->dzb[cur].p;
+        if ((st = gemm_splitk(s, one, 0, 1, dz, C, 1, gr + s->ob[l], 1, 1, 1, C, (int)PB, 1.f))) return st;
+        if ((st = gemm_splitk(s, vl, 1, C, dz, C, 1, gr + s->oW[l], C, 1, C, C, (int)PB, 1.f))) return st;
+        // spectral path: dF' = Bx^T dz per row j of each sample
+        if ((st = gemm(s, s->Bx.p, 1, X2, dz, C, 1, s->dFy.p, C, 1, X2, C, nx, 0.f, nb * ny, 0, (long)nx * C,
                        (long)X2 * C)))
             return st;
-        // dH = Ar dRe F' + Ai dIm F' per kx
-        if ((st = gemm(s, s->Ar.p, ny, 1, s->dFy.p, (long)X2 * C, 1, s->dH.p, C, 1, R4, C, ny, 0.f, mx, 0, 2L * C,
-                       (long)R4 * C)))
+        // dH = Ar dRe F' + Ai dIm F' per (sample, kx)
+        if ((st = gemm(s, s->Ar.p, ny, 1, s->dFy.p, (long)X2 * C, 1, s->dH.p, C, 1, R4, C, ny, 0.f, nb * mx, 0, 2L * C,
+                       (long)R4 * C, mx, 0, sFs, sGs)))
             return st;
-        if ((st = gemm(s, s->Ai.p, ny, 1, s->dFy.p + C, (long)X2 * C, 1, s->dH.p, C, 1, R4, C, ny, 1.f, mx, 0, 2L * C,
-                       (long)R4 * C)))
+        if ((st = gemm(s, s->Ai.p, ny, 1, s->dFy.p + C, (long)X2 * C, 1, s->dH.p, C, 1, R4, C, ny, 1.f, nb * mx, 0,
+                       2L * C, (long)R4 * C, mx, 0, sFs, sGs)))
             return st;
-        const int nmode = mx * 2 * c.modes_y;
         hipLaunchKernelGGL(k_mix_back_r, dim3((int)(((long)nmode * C * C + 255) / 256)), dim3(256), 0, s->stream,
-                           s->dH.p, s->Gs[l].p, gr + s->oR[l], nmode, C);
-        hipLaunchKernelGGL(k_mix_back_g, dim3((nmode * C + 255) / 256), dim3(256), 0, s->stream, s->dH.p,
-                           prm + s->oR[l], s->dG.p, nmode, C);
-        // dRe Fx = Ar^T dG, dIm Fx = Ai^T dG per kx
-        if ((st = gemm(s, s->Ar.p, 1, ny, s->dG.p, C, 1, s->dFx.p, (long)X2 * C, 1, ny, C, R4, 0.f, mx, 0,
-                       (long)R4 * C, 2L * C)))
+                           s->dH.p, s->Gs[l].p, gr + s->oR[l], nmode, C, nb);
+        hipLaunchKernelGGL(k_mix_back_g, dim3((nb * nmode * C + 255) / 256), dim3(256), 0, s->stream, s->dH.p,
+                           prm + s->oR[l], s->dG.p, nmode, C, nb);
+        // dRe Fx = Ar^T dG, dIm Fx = Ai^T dG per (sample, kx)
+        if ((st = gemm(s, s->Ar.p, 1, ny, s->dG.p, C, 1, s->dFx.p, (long)X2 * C, 1, ny, C, R4, 0.f, nb * mx, 0,
+                       (long)R4 * C, 2L * C, mx, 0, sGs, sFs)))
             return st;
-        if ((st = gemm(s, s->Ai.p, 1, ny, s->dG.p, C, 1, s->dFx.p + C, (long)X2 * C, 1, ny, C, R4, 0.f, mx, 0,
-                       (long)R4 * C, 2L * C)))
+        if ((st = gemm(s, s->Ai.p, 1, ny, s->dG.p, C, 1, s->dFx.p + C, (long)X2 * C, 1, ny, C, R4, 0.f, nb * mx, 0,
+                       (long)R4 * C, 2L * C, mx, 0, sGs, sFs)))
             return st;
-        // dv_l += Ex^T dFx per row j
-        if ((st = gemm(s, s->Ex.p, 1, nx, s->dFx.p, C, 1, s->dv.p, C, 1, nx, C, X2, 1.f, ny, 0, (long)X2 * C,
-                       (long)nx * C)))
-            return st;
+        // per row j of each sample: dv_l = dz W^T + Ex^T dFx, then the gradient with respect to z_l,
+        // dv_l gelu'(z_l) (layers 1 .. L-1 apply gelu; dv_0 as it is)
+        Gemm2 g{};
+        g.A = dz;
+        g.sAb = (long)nx * C;
+        g.sAm = C;
+        g.sAk = 1;
+        g.B = prm + s->oW[l];
+        g.sBk = 1;
+        g.sBn = C;
+        g.K = C;
+        g.A2 = s->ExT.p;
+        g.sA2m = X2;
+        g.sA2k = 1;
+        g.B2 = s->dFx.p;
+        g.sB2b = (long)X2 * C;
+        g.sB2k = C;
+        g.sB2n = 1;
+        g.K2 = X2;
+        g.M = nx;
+        g.N = C;
+        g.sOb = (long)nx * C;
+        g.epi = EPI_ACT_BACK;
+        g.zin = l >= 1 ? s->z[l].p : nullptr;
+        g.vout = s->dzb[cur ^ 1].p;
+        if ((st = gemm2(s, g, true, nb * ny))) return st;
+        cur ^= 1;
     }
-    // dv now holds dL/dv_0; v_0 = x P + p
-    if ((st = gemm_splitk(s, one, 0, 1, s->dv.p, C, 1, gr + s->op, 1, 1, 1, C, (int)P, 1.f))) return st;
-    if ((st = gemm_splitk(s, x, 1, c.in_channels, s->dv.p, C, 1, gr + s->oP, C, 1, c.in_channels, C, (int)P, 1.f)))
+    // dzb[cur] now holds dL/dv_0; v_0 = x P + p
+    const __bf16* dv0 = s->dzb[cur].p;
+    if ((st = gemm_splitk(s, one, 0, 1, dv0, C, 1, gr + s->op, 1, 1, 1, C, (int)PB, 1.f))) return st;
+    if ((st = gemm_splitk(s, x, 1, c.in_channels, dv0, C, 1, gr + s->oP, C, 1, c.in_channels, C, (int)PB, 1.f)))
         return st;
     SCHECK(s, hipGetLastError());
     return 0;
 }
 
-int loss_and_dy(fi_surrogate* s, const float* t, double* dev_out) {
-    const int nb = 256;
-    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s->stream, s->y.p, t, s->P, s->dy.p, s->lpart.p);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s->stream, s->lpart.p, nb, dev_out);
+// loss (sum of the nb samples' mean squared errors) into *dev_out and dL/dy
+int loss_and_dy(fi_surrogate* s, const float* t, int nb, double* dev_out) {
+    const int nblk = 256;
+    hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, s->stream, s->y.p, t, s->P, s->P * nb, s->dy.p, s->lpart.p);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s->stream, s->lpart.p, nblk, dev_out);
     SCHECK(s, hipGetLastError());
+    return 0;
+}
+
+// samples that go through the network side by side: up to 16, within about 4 GB of activations
+int chunk_for(const fi_surrogate* s, int batch) {
+    const fi_fno_config& c = s->cfg;
+    const double per_sample = (double)s->P * ((c.layers + 2) * c.width * 6.0 + c.proj_hidden * 12.0);
+    return std::max(1, std::min({batch, 16, (int)std::min(64.0, 4e9 / per_sample)}));
+}
+
+// per-sample buffers for nb samples side by side
+int ensure_cap(fi_surrogate* s, int nb) {
+    if (nb <= s->cap) return 0;
+    const fi_fno_config& c = s->cfg;
+    const long P = s->P * nb;
+    const int C = c.width, X2 = 2 * c.modes_x, R4 = 4 * c.modes_y, mx = c.modes_x, Hh = c.proj_hidden, L = c.layers;
+    const size_t nF = (size_t)nb * s->ny * X2 * C, nG = (size_t)nb * mx * R4 * C;
+    s->cap = 0;
+    s->v.clear();
+    s->z.clear();
+    s->Gs.clear();
+    s->v.resize(L + 1);
+    s->z.resize(L + 1);
+    s->Gs.resize(L);
+    for (int l = 0; l <= L; ++l) {
+        SCHECK(s, s->v[l].alloc(P * C));
+        SCHECK(s, s->z[l].alloc(l >= 1 && l < L ? P * C : 1));   // z_1 .. z_{L-1}: the activated layers
+    }
+    for (int l = 0; l < L; ++l) SCHECK(s, s->Gs[l].alloc(nG));
+    SCHECK(s, s->x.alloc(P * c.in_channels));
+    SCHECK(s, s->y.alloc(P));
+    SCHECK(s, s->dy.alloc(P));
+    SCHECK(s, s->tgt.alloc(P));
+    SCHECK(s, s->Fx.alloc(nF));
+    SCHECK(s, s->dFx.alloc(nF));
+    SCHECK(s, s->Fy.alloc(nF));
+    SCHECK(s, s->dFy.alloc(nF));
+    SCHECK(s, s->H.alloc(nG));
+    SCHECK(s, s->dG.alloc(nG));
+    SCHECK(s, s->dH.alloc(nG));
+    SCHECK(s, s->q_pre.alloc(P * Hh));
+    SCHECK(s, s->q.alloc(P * Hh));
+    SCHECK(s, s->dqb.alloc(P * Hh));
+    SCHECK(s, s->dzb[0].alloc(P * C));
+    SCHECK(s, s->dzb[1].alloc(P * C));
+    SCHECK(s, s->ones.alloc(P));
+    hipLaunchKernelGGL(k_fill, dim3(ew_grid(P)), dim3(256), 0, s->stream, s->ones.p, P, 1.f);
+    SCHECK(s, hipGetLastError());
+    SCHECK(s, hipStreamSynchronize(s->stream));
+    s->cap = nb;
     return 0;
 }
 
@@ -658,12 +1057,15 @@ int fi_surrogate_set_grid(fi_surrogate* s, int32_t nx, int32_t ny) {
     const long P = s->P;
     const int C = c.width, X2 = 2 * c.modes_x, R4 = 4 * c.modes_y, mx = c.modes_x, my = c.modes_y, Hh = c.proj_hidden;
     // DFT matrices (FP64 angles, FP32 storage)
-    std::vector<float> Ex((size_t)X2 * nx), Ar((size_t)R4 * ny), Ai((size_t)R4 * ny), Bx((size_t)nx * X2);
+    std::vector<float> Ex((size_t)X2 * nx), Ar((size_t)R4 * ny), Ai((size_t)R4 * ny), Bx((size_t)nx * X2),
+        ExT((size_t)nx * X2);
     for (int k = 0; k < mx; ++k)
         for (int i = 0; i < nx; ++i) {
             const double th = 2.0 * M_PI * (double)((long)k * i % nx) / nx;
             Ex[(size_t)(2 * k) * nx + i] = (float)std::cos(th);
             Ex[(size_t)(2 * k + 1) * nx + i] = (float)-std::sin(th);
+            ExT[(size_t)i * X2 + 2 * k] = Ex[(size_t)(2 * k) * nx + i];
+            ExT[(size_t)i * X2 + 2 * k + 1] = Ex[(size_t)(2 * k + 1) * nx + i];
             const double w = (k == 0 ? 1.0 : 2.0) / ((double)nx * ny);
             Bx[(size_t)i * X2 + 2 * k] = (float)(w * std::cos(th));
             Bx[(size_t)i * X2 + 2 * k + 1] = (float)(-w * std::sin(th));
@@ -687,39 +1089,10 @@ int fi_surrogate_set_grid(fi_surrogate* s, int32_t nx, int32_t ny) {
     SCHECK(s, up(s->Ar, Ar));
     SCHECK(s, up(s->Ai, Ai));
     SCHECK(s, up(s->Bx, Bx));
-    const int L = c.layers;
-    s->v.clear();
-    s->z.clear();
-    s->Gs.clear();
-    s->v.resize(L + 1);
-    s->z.resize(L + 1);
-    s->Gs.resize(L);
-    for (int l = 0; l <= L; ++l) {
-        SCHECK(s, s->v[l].alloc(P * C));
-        SCHECK(s, s->z[l].alloc(P * C));
-    }
-    for (int l = 0; l < L; ++l) SCHECK(s, s->Gs[l].alloc((size_t)mx * R4 * C));
-    SCHECK(s, s->x.alloc(P * c.in_channels));
-    SCHECK(s, s->y.alloc(P));
-    SCHECK(s, s->dy.alloc(P));
-    SCHECK(s, s->tgt.alloc(P));
-    SCHECK(s, s->Fx.alloc((size_t)ny * X2 * C));
-    SCHECK(s, s->dFx.alloc((size_t)ny * X2 * C));
-    SCHECK(s, s->Fy.alloc((size_t)ny * X2 * C));
-    SCHECK(s, s->dFy.alloc((size_t)ny * X2 * C));
-    SCHECK(s, s->G.alloc((size_t)mx * R4 * C));
-    SCHECK(s, s->H.alloc((size_t)mx * R4 * C));
-    SCHECK(s, s->dG.alloc((size_t)mx * R4 * C));
-    SCHECK(s, s->dH.alloc((size_t)mx * R4 * C));
-    SCHECK(s, s->q_pre.alloc(P * Hh));
-    SCHECK(s, s->q.alloc(P * Hh));
-    SCHECK(s, s->dq.alloc(P * Hh));
-    SCHECK(s, s->dv.alloc(P * C));
-    SCHECK(s, s->dz.alloc(P * C));
-    SCHECK(s, s->ones.alloc(P));
-    hipLaunchKernelGGL(k_fill, dim3(ew_grid(P)), dim3(256), 0, s->stream, s->ones.p, P, 1.f);
-    SCHECK(s, hipGetLastError());
-    SCHECK(s, hipStreamSynchronize(s->stream));
+    SCHECK(s, up(s->ExT, ExT));
+    s->cap = 0;
+    int st = ensure_cap(s, 1);
+    if (st) return st;
     s->nsamp = 0;   // a training set belongs to one grid
     return 0;
 }
@@ -730,7 +1103,7 @@ int fi_surrogate_forward(fi_surrogate* s, const float* x, float* y) {
     SCHECK(s, hipSetDevice(s->cfg.device));
     SCHECK(s, hipMemcpyAsync(s->x.p, x, s->P * s->cfg.in_channels * sizeof(float), hipMemcpyHostToDevice, s->stream));
     timed_begin(s);
-    int st = forward(s, s->x.p);
+    int st = forward(s, s->x.p, 1);
     if (st) return st;
     if ((st = timed_end(s))) return st;
     SCHECK(s, hipMemcpy(y, s->y.p, s->P * sizeof(float), hipMemcpyDeviceToHost));
@@ -746,10 +1119,46 @@ int fi_surrogate_loss_grad(fi_surrogate* s, const float* x, const float* t, doub
     hipLaunchKernelGGL(k_fill, dim3(ew_grid(s->np)), dim3(256), 0, s->stream, s->grad.p, s->np, 0.f);
     timed_begin(s);
     int st;
-    if ((st = forward(s, s->x.p)) || (st = loss_and_dy(s, s->tgt.p, s->lsum.p)) || (st = backward(s, s->x.p))) return st;
+    if ((st = forward(s, s->x.p, 1)) || (st = loss_and_dy(s, s->tgt.p, 1, s->lsum.p)) || (st = backward(s, s->x.p, 1)))
+        return st;
     if ((st = timed_end(s))) return st;
     SCHECK(s, hipMemcpy(grad, s->grad.p, s->np * sizeof(float), hipMemcpyDeviceToHost));
     if (loss) SCHECK(s, hipMemcpy(loss, s->lsum.p, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fi_surrogate_loss_grad_n(fi_surrogate* s, int32_t n, const float* x, const float* t, double* loss, float* grad) {
+    if (!s || n < 1 || !x || !t || !grad) return S_ERR_INVALID;
+    if (!s->P) return sfail(s, S_ERR_STATE, "fi_surrogate_set_grid first");
+    SCHECK(s, hipSetDevice(s->cfg.device));
+    const long P = s->P;
+    const int Ci = s->cfg.in_channels;
+    const int chunk = chunk_for(s, n), nchunk = (n + chunk - 1) / chunk;
+    int st = ensure_cap(s, chunk);
+    if (st) return st;
+    DB<double> ls;
+    SCHECK(s, ls.alloc(nchunk));
+    hipLaunchKernelGGL(k_fill, dim3(ew_grid(s->np)), dim3(256), 0, s->stream, s->grad.p, s->np, 0.f);
+    for (int ci = 0; ci < nchunk; ++ci) {
+        const int nb = std::min(chunk, n - ci * chunk);
+        const size_t s0 = (size_t)ci * chunk;
+        SCHECK(s, hipMemcpyAsync(s->x.p, x + s0 * P * Ci, (size_t)nb * P * Ci * sizeof(float), hipMemcpyHostToDevice,
+                                 s->stream));
+        SCHECK(s, hipMemcpyAsync(s->tgt.p, t + s0 * P, (size_t)nb * P * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        timed_begin(s);
+        if ((st = forward(s, s->x.p, nb)) || (st = loss_and_dy(s, s->tgt.p, nb, ls.p + ci)) ||
+            (st = backward(s, s->x.p, nb)))
+            return st;
+        if ((st = timed_end(s))) return st;
+    }
+    SCHECK(s, hipMemcpy(grad, s->grad.p, s->np * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<double> h(nchunk);
+    SCHECK(s, hipMemcpy(h.data(), ls.p, nchunk * sizeof(double), hipMemcpyDeviceToHost));
+    if (loss) {
+        double m = 0;
+        for (double v : h) m += v;
+        *loss = m;
+    }
     return 0;
 }
 
@@ -770,21 +1179,26 @@ int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, 
     if (!s || steps < 0 || batch < 1) return S_ERR_INVALID;
     if (s->nsamp < 1) return sfail(s, S_ERR_STATE, "no training set");
     SCHECK(s, hipSetDevice(s->cfg.device));
-    DB<double> losses;
-    SCHECK(s, losses.alloc((size_t)std::max(1, steps) * batch));
-    uint64_t rs = seed;
-    timed_begin(s);
     const long P = s->P;
     const int Ci = s->cfg.in_channels;
+    const int chunk = chunk_for(s, batch);
+    const int nchunk = (batch + chunk - 1) / chunk;
+    int st = ensure_cap(s, chunk);
+    if (st) return st;
+    DB<double> losses;
+    SCHECK(s, losses.alloc((size_t)std::max(1, steps) * nchunk));
+    uint64_t rs = seed;
+    timed_begin(s);
     for (int it = 0; it < steps; ++it) {
         hipLaunchKernelGGL(k_fill, dim3(ew_grid(s->np)), dim3(256), 0, s->stream, s->grad.p, s->np, 0.f);
-        for (int b = 0; b < batch; ++b) {
-            const int k = (int)(splitmix64(rs) % (uint64_t)s->nsamp);
-            const float* xk = s->xs.p + (size_t)k * P * Ci;
-            const float* yk = s->ys.p + (size_t)k * P;
-            int st;
-            if ((st = forward(s, xk)) || (st = loss_and_dy(s, yk, losses.p + (size_t)it * batch + b)) ||
-                (st = backward(s, xk)))
+        for (int ci = 0; ci < nchunk; ++ci) {
+            const int nb = std::min(chunk, batch - ci * chunk);
+            Picks pk{};
+            for (int b = 0; b < nb; ++b) pk.k[b] = (int)(splitmix64(rs) % (uint64_t)s->nsamp);
+            hipLaunchKernelGGL(k_gather, dim3(ew_grid((long)nb * P * (Ci + 1))), dim3(256), 0, s->stream, s->xs.p,
+                               s->ys.p, P * Ci, P, pk, nb, s->x.p, s->tgt.p);
+            if ((st = forward(s, s->x.p, nb)) || (st = loss_and_dy(s, s->tgt.p, nb, losses.p + (size_t)it * nchunk + ci)) ||
+                (st = backward(s, s->x.p, nb)))
                 return st;
         }
         // Adam step with cosine learning-rate decay over this call's steps
@@ -796,14 +1210,13 @@ int fi_surrogate_train(fi_surrogate* s, int32_t steps, int32_t batch, float lr, 
                            s->np, lr_t, 0.9f, 0.999f, 1e-8f, c1, c2, 1.f / (float)batch, wd);
         SCHECK(s, hipGetLastError());
     }
-    int st;
     if ((st = timed_end(s))) return st;
     if (loss_out && steps > 0) {
-        std::vector<double> h((size_t)steps * batch);
+        std::vector<double> h((size_t)steps * nchunk);
         SCHECK(s, hipMemcpy(h.data(), losses.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int it = 0; it < steps; ++it) {
             double m = 0;
-            for (int b = 0; b < batch; ++b) m += h[(size_t)it * batch + b];
+            for (int ci = 0; ci < nchunk; ++ci) m += h[(size_t)it * nchunk + ci];
             loss_out[it] = m / batch;
         }
     }

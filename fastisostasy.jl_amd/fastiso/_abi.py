@@ -215,6 +215,7 @@ def declare_surrogate(lib):
         "fi_surrogate_set_grid": (C.c_int, [P, C.c_int32, C.c_int32]),
         "fi_surrogate_forward": (C.c_int, [P, _fp, _fp]),
         "fi_surrogate_loss_grad": (C.c_int, [P, _fp, _fp, _dp, _fp]),
+        "fi_surrogate_loss_grad_n": (C.c_int, [P, C.c_int32, _fp, _fp, _dp, _fp]),
         "fi_surrogate_set_data": (C.c_int, [P, C.c_int32, _fp, _fp]),
         "fi_surrogate_train": (C.c_int, [P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_uint64, _dp]),
         "fi_surrogate_device_ms": (C.c_double, [P]),

@@ -130,6 +130,15 @@ class Surrogate:
         self._check(self.lib.fi_surrogate_loss_grad(self.h, _p(x), _p(t), C.byref(loss), _p(g)))
         return loss.value, g
 
+    def loss_grad_n(self, X, T):
+        """loss and gradient summed over the samples X: (samples, cells, channels), T: (samples, cells),
+        which go through the network side by side"""
+        X, T = _f32(X), _f32(T)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.fi_surrogate_loss_grad_n(self.h, int(X.shape[0]), _p(X), _p(T), C.byref(loss), _p(g)))
+        return loss.value, g
+
     def set_data(self, X, Y):
         """X: (samples, cells, channels), Y: (samples, cells)"""
         X, Y = _f32(X), _f32(Y)

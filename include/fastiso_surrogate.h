@@ -63,6 +63,11 @@ int fi_surrogate_forward(fi_surrogate* s, const float* x, float* y);
  * parameter (host arrays; grad has fi_surrogate_num_params entries). */
 int fi_surrogate_loss_grad(fi_surrogate* s, const float* x, const float* target, double* loss, float* grad);
 
+/* The same for n samples side by side (x: n * cells * in_channels, target: n * cells): loss and
+ * gradient summed over the samples, which go through the network together (up to 16 per pass). */
+int fi_surrogate_loss_grad_n(fi_surrogate* s, int32_t n, const float* x, const float* target, double* loss,
+                             float* grad);
+
 /* Training set on the device: nsamples samples of the current grid (x: nsamples * cells *
  * in_channels, y: nsamples * cells). Replaces any previous set. */
 int fi_surrogate_set_data(fi_surrogate* s, int32_t nsamples, const float* x, const float* y);

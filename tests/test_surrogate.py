@@ -83,3 +83,44 @@ def test_training_fits_a_smoothing_operator():
     losses = s.train(300, batch=4, lr=3e-3, lr_min=1e-4, seed=3, wd=0.0)
     assert np.all(np.isfinite(losses))
     assert losses[-20:].mean() < 0.1 * losses[:5].mean(), (losses[:5].mean(), losses[-20:].mean())
+
+
+def test_samples_side_by_side_match_one_at_a_time():
+    """fi_surrogate_loss_grad_n runs the samples through the network together (the batch strides of
+    every product, the per-sample spectral modes, the gradient sums over samples): its loss and
+    gradient equal the sums of the single-sample calls up to FP32 summation order, and the oracle's
+    sums within the BF16 bounds. 19 samples: one pass of 16 and one of 3."""
+    nx, ny = 40, 36
+    s = make(nx, ny)
+    cfg = s.config_dict()
+    th = F.init_params(cfg, 11)
+    s.set_params(th)
+    n = 19
+    X = np.stack([sample(nx, ny, cfg["in_channels"], 20 + k)[0] for k in range(n)])
+    T = np.stack([sample(nx, ny, cfg["in_channels"], 20 + k)[1] for k in range(n)])
+    Ln, gn = s.loss_grad_n(X, T)
+    L1, g1 = 0.0, np.zeros(s.n_params)
+    for k in range(n):
+        lk, gk = s.loss_grad(X[k], T[k])
+        L1 += lk
+        g1 += gk
+    assert abs(Ln - L1) <= 1e-9 * abs(L1), (Ln, L1)
+    lay, _ = F.param_layout(cfg)
+    for key, (o, shp) in lay.items():
+        m = int(np.prod(shp))
+        e = relerr(gn[o:o + m], g1[o:o + m])
+        assert e < 1e-5, (key, e)
+    # the oracle's sums over three of the samples, through the batched call on those three
+    th64 = th.astype(np.float32).astype(np.float64)
+    L3, g3 = s.loss_grad_n(X[:3], T[:3])
+    Lr, gr = 0.0, 0.0
+    for k in range(3):
+        lk, gk = F.loss_grad(th64, X[k].astype(np.float32).astype(np.float64),
+                             T[k].astype(np.float32).astype(np.float64), nx, ny, cfg)
+        Lr += lk
+        gr = gr + gk
+    assert abs(L3 - Lr) <= FWD_TOL * Lr
+    for key, (o, shp) in lay.items():
+        m = int(np.prod(shp))
+        e = relerr(g3[o:o + m], gr[o:o + m])
+        assert e < GRAD_TOL, (key, e)
