@@ -192,6 +192,110 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
     }
 }
 
+// k_gemm for operands strided in K (the per-row x-transform reads activations across cells; the
+// parameter gradients read both operands across cells): the workgroup loads each 32-wide K step of
+// its 64 x 32 A tile and 32 x 32 B tile cooperatively, consecutive threads on consecutive addresses
+// of whichever index is contiguous, converts to BF16 into LDS ([row][k], two buffers), and the
+// waves feed the MFMAs from LDS. The next step's loads are issued before the current step's MFMAs;
+// one barrier per step. Same MFMA operands and order as k_gemm, so the same results bit for bit.
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
+    constexpr int PS = 40;   // LDS pitch (BF16) of a row of 32 k: 80 bytes, 16-byte aligned rows
+    __shared__ __align__(16) __bf16 sa[2][64 * PS];
+    __shared__ __align__(16) __bf16 sb[2][32 * PS];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int b = blockIdx.z;
+    const int mb = blockIdx.x * 64, n0 = blockIdx.y * 32;
+    const int r = lane & 15, kg = lane >> 4;
+    int kbeg = 0, kend = g.K;
+    const TA* A;
+    const TB* B;
+    if (g.nslice > 0) {
+        const int pb = b / g.nslice, sl = b - pb * g.nslice;
+        kbeg = sl * g.kchunk;
+        kend = min(g.K, kbeg + g.kchunk);
+        A = (const TA*)g.A + boff(pb, g.nb0, g.sAb, g.sAb1);
+        B = (const TB*)g.B + boff(pb, g.nb0, g.sBb, g.sBb1);
+    } else if (g.kchunk > 0) {
+        kbeg = b * g.kchunk;
+        kend = min(g.K, kbeg + g.kchunk);
+        A = (const TA*)g.A;
+        B = (const TB*)g.B;
+    } else {
+        A = (const TA*)g.A + boff(b, g.nb0, g.sAb, g.sAb1);
+        B = (const TB*)g.B + boff(b, g.nb0, g.sBb, g.sBb1);
+    }
+    float* C = g.C + ((g.nslice > 0 || g.kchunk > 0) ? (long)b * g.sCb : boff(b, g.nb0, g.sCb, g.sCb1));
+    // this thread's elements of a K step: A e = tid + 256 j (j < 8), B e = tid + 256 j (j < 4)
+    const bool afk = g.sAk == 1, bfk = g.sBk == 1;
+    int arow[8], ak[8], bcol[4], bk[4];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int e = tid + GEMM_THREADS * j;
+        arow[j] = afk ? e >> 5 : e & 63;
+        ak[j] = afk ? e & 31 : e >> 6;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int e = tid + GEMM_THREADS * j;
+        bcol[j] = bfk ? e >> 5 : e & 31;
+        bk[j] = bfk ? e & 31 : e >> 5;
+    }
+    TA va[8];
+    TB vb[4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int m = mb + arow[j], k = k0 + ak[j];
+            va[j] = (m < g.M && k < kend) ? A[(long)m * g.sAm + (long)k * g.sAk] : (TA)0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + bcol[j], k = k0 + bk[j];
+            vb[j] = (n < g.N && k < kend) ? B[(long)k * g.sBk + (long)n * g.sBn] : (TB)0.f;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sa[buf][arow[j] * PS + ak[j]] = (__bf16)va[j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sb[buf][bcol[j] * PS + bk[j]] = (__bf16)vb[j];
+    };
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    gload(kbeg);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
+        const bool more = k0 + 32 < kend;
+        if (more) gload(k0 + 32);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sa[buf][(wave * 16 + r) * PS + 8 * kg]);
+        const bf16x8 x0 = *reinterpret_cast<const bf16x8*>(&sb[buf][r * PS + 8 * kg]);
+        const bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&sb[buf][(16 + r) * PS + 8 * kg]);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x1, acc1, 0, 0, 0);
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    const int m0 = mb + wave * 16, bn0 = n0 + r, bn1 = n0 + 16 + r;
+    const bool b0ok = bn0 < g.N, b1ok = bn1 < g.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = m0 + 4 * kg + i;
+        if (row >= g.M) continue;
+        float* cr = C + (long)row * g.sCm;
+        if (b0ok) {
+            float* c = cr + (long)bn0 * g.sCn;
+            *c = g.alpha * acc0[i] + (g.beta != 0.f ? g.beta * *c : 0.f);
+        }
+        if (b1ok) {
+            float* c = cr + (long)bn1 * g.sCn;
+            *c = g.alpha * acc1[i] + (g.beta != 0.f ? g.beta * *c : 0.f);
+        }
+    }
+}
+
 // dst(m, n) = beta dst(m, n) + sum_b part[b][m][n]: one wave per output, lane l sums the slices
 // b = l, l + 64, ... and the wave adds the lanes by a butterfly (fixed order: deterministic)
 __global__ void __launch_bounds__(256) k_splitk_reduce(const float* __restrict__ part, int nsplit, int M, int N,
@@ -571,6 +675,17 @@ struct Src {
 };
 
 void launch_gemm(fi_surrogate* s, const Gemm& g, dim3 grid, bool abf, bool bbf) {
+    if (g.sAk != 1 || g.sBk != 1) {   // an operand strided in K: stage the tiles in LDS
+        if (abf && bbf)
+            hipLaunchKernelGGL((k_gemm_s<__bf16, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        else if (abf)
+            hipLaunchKernelGGL((k_gemm_s<__bf16, float>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        else if (bbf)
+            hipLaunchKernelGGL((k_gemm_s<float, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        else
+            hipLaunchKernelGGL((k_gemm_s<float, float>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        return;
+    }
     if (abf && bbf)
         hipLaunchKernelGGL((k_gemm<__bf16, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
     else if (abf)
