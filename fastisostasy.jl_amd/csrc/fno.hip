@@ -54,6 +54,7 @@ struct Gemm {
     int nslice;
     int nb0;
     long sAb1, sBb1, sCb1;
+    int ones_row1;   // 1 + the row of A that reads 1.0 instead of memory (0: none; LDS-staged path)
 };
 
 __device__ __forceinline__ long boff(int b, int nb0, long s0, long s1) {
@@ -197,7 +198,9 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
 // its 64 x 32 A tile and 32 x 32 B tile cooperatively, consecutive threads on consecutive addresses
 // of whichever index is contiguous, converts to BF16 into LDS ([row][k], two buffers), and the
 // waves feed the MFMAs from LDS. The next step's loads are issued before the current step's MFMAs;
-// one barrier per step. Same MFMA operands and order as k_gemm, so the same results bit for bit.
+// one barrier per step. Same MFMA operands and order as k_gemm, so the same results bit for bit. A
+// row of ones in A (ones_row1) makes a bias gradient, the sum over K of B, one more row of its
+// weight gradient's product.
 template <typename TA, typename TB>
 __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
     constexpr int PS = 40;   // LDS pitch (BF16) of a row of 32 k: 80 bytes, 16-byte aligned rows
@@ -247,7 +250,8 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int m = mb + arow[j], k = k0 + ak[j];
-            va[j] = (m < g.M && k < kend) ? A[(long)m * g.sAm + (long)k * g.sAk] : (TA)0.f;
+            va[j] = (m < g.M && k < kend) ? (m + 1 == g.ones_row1 ? (TA)1.f : A[(long)m * g.sAm + (long)k * g.sAk])
+                                          : (TA)0.f;
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -627,7 +631,7 @@ struct fi_surrogate {
     // parameter offsets
     long oP, op, oQ1, oq1, oQ2, oq2, np;
     std::vector<long> oW, ob, oR;
-    DB<float> par, grad, am, av, ones;
+    DB<float> par, grad, am, av;
     // grid
     int nx = 0, ny = 0;
     long P = 0;
@@ -675,7 +679,7 @@ struct Src {
 };
 
 void launch_gemm(fi_surrogate* s, const Gemm& g, dim3 grid, bool abf, bool bbf) {
-    if (g.sAk != 1 || g.sBk != 1) {   // an operand strided in K: stage the tiles in LDS
+    if (g.sAk != 1 || g.sBk != 1 || g.ones_row1 > 0) {   // an operand strided in K: stage the tiles in LDS
         if (abf && bbf)
             hipLaunchKernelGGL((k_gemm_s<__bf16, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
         else if (abf)
@@ -735,9 +739,13 @@ int gemm(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, 
 }
 
 // product with a long K (parameter gradients, K = the cells of the samples): split-K partials over
-// about 2048 slices of at least 128, then a fixed-order reduction
+// about 2048 slices of at least 128, then a fixed-order reduction. With bias, A gets one more row
+// of ones (row M), whose product, the bias gradient, lands in row M of C: every bias follows its
+// weight matrix in the parameter block.
 int gemm_splitk(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, float* C, long sCm, long sCn,
-                int M, int N, int K, float beta) {
+                int M, int N, int K, float beta, bool bias = false) {
+    const int ones_row1 = bias ? M + 1 : 0;
+    if (bias) M += 1;
     const int kchunk = std::max(128, 32 * (int)(((long)K + 2048L * 32 - 1) / (2048L * 32)));
     const int nsplit = (K + kchunk - 1) / kchunk;
     const long need = (long)nsplit * M * N;
@@ -746,7 +754,7 @@ int gemm_splitk(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, lon
         s->part_cap = need;
     }
     Gemm g{A.p, B.p, s->part.p, 0, sAm, sAk, 0, sBk, sBn, (long)M * N, N, 1, M, N, K, kchunk, 1.f, 0.f, 0,
-           NB0_NONE, 0, 0, 0};
+           NB0_NONE, 0, 0, 0, ones_row1};
     launch_gemm(s, g, dim3((M + 63) / 64, (N + 31) / 32, nsplit), A.bf, B.bf);
     SCHECK(s, hipGetLastError());
     hipLaunchKernelGGL(k_splitk_reduce, dim3((M * N + 3) / 4), dim3(256), 0, s->stream, s->part.p, nsplit, M, N, C,
@@ -886,11 +894,9 @@ int backward(fi_surrogate* s, const float* x, int nb) {
     const long sFs = (long)ny * X2 * C, sGs = (long)mx * R4 * C;
     float* prm = s->par.p;
     float* gr = s->grad.p;
-    const float* one = s->ones.p;
     int st;
     // y = q Q2 + q2
-    if ((st = gemm_splitk(s, s->q.p, 1, Hh, s->dy.p, 1, 1, gr + s->oQ2, 1, 1, Hh, 1, (int)PB, 1.f))) return st;
-    if ((st = gemm_splitk(s, one, 0, 1, s->dy.p, 1, 1, gr + s->oq2, 1, 1, 1, 1, (int)PB, 1.f))) return st;
+    if ((st = gemm_splitk(s, s->q.p, 1, Hh, s->dy.p, 1, 1, gr + s->oQ2, 1, 1, Hh, 1, (int)PB, 1.f, true))) return st;
     {   // dq = (dy Q2^T) gelu'(q_pre)
         Gemm2 g{};
         g.A = s->dy.p;
@@ -908,8 +914,8 @@ int backward(fi_surrogate* s, const float* x, int nb) {
         if ((st = gemm2(s, g, false, 1))) return st;
     }
     // q_pre = v_L Q1 + q1
-    if ((st = gemm_splitk(s, s->v[L].p, 1, C, s->dqb.p, Hh, 1, gr + s->oQ1, Hh, 1, C, Hh, (int)PB, 1.f))) return st;
-    if ((st = gemm_splitk(s, one, 0, 1, s->dqb.p, Hh, 1, gr + s->oq1, 1, 1, 1, Hh, (int)PB, 1.f))) return st;
+    if ((st = gemm_splitk(s, s->v[L].p, 1, C, s->dqb.p, Hh, 1, gr + s->oQ1, Hh, 1, C, Hh, (int)PB, 1.f, true)))
+        return st;
     int cur = 0;
     {   // dz_L = dv_L = dq Q1^T (the last layer has no activation)
         Gemm2 g{};
@@ -930,8 +936,7 @@ int backward(fi_surrogate* s, const float* x, int nb) {
         // dz: gradient with respect to z_{l+1}
         const __bf16* vl = s->v[l].p;
         const __bf16* dz = s->dzb[cur].p;
-        if ((st = gemm_splitk(s, one, 0, 1, dz, C, 1, gr + s->ob[l], 1, 1, 1, C, (int)PB, 1.f))) return st;
-        if ((st = gemm_splitk(s, vl, 1, C, dz, C, 1, gr + s->oW[l], C, 1, C, C, (int)PB, 1.f))) return st;
+        if ((st = gemm_splitk(s, vl, 1, C, dz, C, 1, gr + s->oW[l], C, 1, C, C, (int)PB, 1.f, true))) return st;
         // spectral path: dF' = Bx^T dz per row j of each sample
         if ((st = gemm(s, s->Bx.p, 1, X2, dz, C, 1, s->dFy.p, C, 1, X2, C, nx, 0.f, nb * ny, 0, (long)nx * C,
                        (long)X2 * C)))
@@ -984,8 +989,7 @@ int backward(fi_surrogate* s, const float* x, int nb) {
     }
     // dzb[cur] now holds dL/dv_0; v_0 = x P + p
     const __bf16* dv0 = s->dzb[cur].p;
-    if ((st = gemm_splitk(s, one, 0, 1, dv0, C, 1, gr + s->op, 1, 1, 1, C, (int)PB, 1.f))) return st;
-    if ((st = gemm_splitk(s, x, 1, c.in_channels, dv0, C, 1, gr + s->oP, C, 1, c.in_channels, C, (int)PB, 1.f)))
+    if ((st = gemm_splitk(s, x, 1, c.in_channels, dv0, C, 1, gr + s->oP, C, 1, c.in_channels, C, (int)PB, 1.f, true)))
         return st;
     SCHECK(s, hipGetLastError());
     return 0;
@@ -1042,9 +1046,6 @@ int ensure_cap(fi_surrogate* s, int nb) {
     SCHECK(s, s->dqb.alloc(P * Hh));
     SCHECK(s, s->dzb[0].alloc(P * C));
     SCHECK(s, s->dzb[1].alloc(P * C));
-    SCHECK(s, s->ones.alloc(P));
-    hipLaunchKernelGGL(k_fill, dim3(ew_grid(P)), dim3(256), 0, s->stream, s->ones.p, P, 1.f);
-    SCHECK(s, hipGetLastError());
     SCHECK(s, hipStreamSynchronize(s->stream));
     s->cap = nb;
     return 0;
