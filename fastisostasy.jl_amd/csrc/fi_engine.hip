@@ -141,17 +141,17 @@ __global__ void __launch_bounds__(NT) fi_persistent_implicit(const KP* __restric
 }
 
 
-// kernel instances: FP64 or FP32 moments; with or without per-cell grid steps (map distortion,
-// FP64-moment path only)
+// kernel instances: FP64 or FP32 moments; with or without per-cell grid steps (map distortion)
 inline const void* persistent_kernel(bool mt_double, bool dist, bool elra, bool implicit = false) {
     if (implicit) return (const void*)fi_persistent_implicit;
     if (elra) return (const void*)fi_persistent_elra;
-    if (!mt_double) return (const void*)fi_persistent<float, false>;
+    if (!mt_double) return dist ? (const void*)fi_persistent<float, true> : (const void*)fi_persistent<float, false>;
     return dist ? (const void*)fi_persistent<double, true> : (const void*)fi_persistent<double, false>;
 }
 inline const void* batch_kernel(bool mt_double, bool dist, bool elra = false) {
     if (elra) return (const void*)fi_batch_persistent_elra;
-    if (!mt_double) return (const void*)fi_batch_persistent<float, false>;
+    if (!mt_double)
+        return dist ? (const void*)fi_batch_persistent<float, true> : (const void*)fi_batch_persistent<float, false>;
     return dist ? (const void*)fi_batch_persistent<double, true> : (const void*)fi_batch_persistent<double, false>;
 }
 
@@ -1288,8 +1288,8 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     if (c->elra) c->lds = twb + other;   // ELRA runs only the padded-grid phases
     if (c->implicit) c->lds = twb + std::max({other, (size_t)32 * nx, (size_t)32 * ny});
     if (c->lds > 160 * 1024 || nx > 1024) return fail(c, FI_ERR_UNSUPPORTED, "grid too large for one workgroup's LDS");
-    if (!c->elra && !c->implicit && c->has_dist && (!c->mt_double || ny < 6 || nx < 3))
-        return fail(c, FI_ERR_UNSUPPORTED, "map distortion needs the FP64-moment rows phase (3 <= nx <= 512, ny >= 6)");
+    if (!c->elra && !c->implicit && c->has_dist && (ny < 6 || nx < 3))
+        return fail(c, FI_ERR_UNSUPPORTED, "map distortion needs a grid of at least 3 columns and 6 rows");
     const void* f = persistent_kernel(c->mt_double, c->has_dist, c->elra, c->implicit);
     HIPCHECK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds));
     int per_cu = 0, ncu = 0;
@@ -2777,8 +2777,6 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noa
         dist = dist || c->has_dist;
     }
     const bool elra = b->mem[0]->elra;
-    if (dist && !mt_double && !elra)
-        return bfail(b, FI_ERR_UNSUPPORTED, "map distortion needs every member on the FP64-moment path");
     // with a distorted member every member runs the distortion kernel (K = 1 where unset); ELRA
     // members read K^2 at run time and need the padded-grid phases' LDS only
     size_t lds = !mt_double ? lds_f : dist ? lds_dd : lds_d;

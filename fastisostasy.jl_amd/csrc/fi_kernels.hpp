@@ -498,7 +498,11 @@ struct CellF {
     float H, zo, hr, lt, lw, i2e, m;
 };
 
-template <typename MT, int QD, int QF>
+// Rows phase for grids wider than 512 cells (FP32 moments in LDS). DIST: per-cell grid steps K dx,
+// K dy (map distortion) as in phase_rows_lds; each thread keeps 1/K of its two columns' six window
+// rows in registers (K = 1 where p.Kinv is null). The general passes (grids under 6 rows or 3
+// columns) have no distortion; set-up refuses distorted grids of that shape.
+template <typename MT, int QD, int QF, bool DIST>
 __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const StageDesc& st, double uoff, const IceT it,
                            const float2* twx, char* smem, const KC4 kc) {
     const int nx = p.nx, ny = p.ny;
@@ -609,6 +613,15 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
             fr[q].i2e = p.inv2eta[c];
             fr[q].m = p.mask[c] ? 1.0f : 0.0f;
         }
+        // 1/K of window row r (u rows ju0 ..) at column threadIdx.x + h NT: kr[2 r + h]
+        float kr[DIST ? 12 : 1];
+        if constexpr (DIST) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                const int i = min((int)threadIdx.x + (q & 1) * NT, nx - 1), j = min(ju0 + (q >> 1), ju1 - 1);
+                kr[q] = p.Kinv ? p.Kinv[i + (long)j * nx] : 1.0f;
+            }
+        }
         __syncthreads();
         const bool spec_in = st.kin >= 0;
         // right-hand side of output cell (i, j0 + o) from the moment derivatives, its displacement
@@ -663,6 +676,11 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                         dd[r] = ci == 1 ? (x2 - x0) * (0.5 * p.inv_dx)
                                         : (ci == 0 ? (x1 - x0) * p.inv_dx : (x2 - x1) * p.inv_dx);
                         xx[r] = (x2 - 2.0 * x1 + x0) * p.inv_dx2;
+                        if constexpr (DIST) {   // Dx = K dx at this row's cell
+                            const double k = (double)kr[2 * r + h];
+                            dd[r] *= k;
+                            xx[r] *= k * k;
+                        }
                     }
 #pragma unroll
                     for (int o = 0; o < NO; ++o) uo[h][o] = cv[o - JU];
@@ -679,9 +697,14 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                         const int lo = first ? ur : ur - 1, hi = last ? ur : ur + 1;
                         const double D = (double)(h ? Dr[2 * m + 1] : Dr[2 * m]);
                         const double uxx = xx[ur];
-                        const double uyy = (cv[c + 1] - 2.0 * cv[c] + cv[c - 1]) * p.inv_dy2;
-                        const double uxy = (first || last) ? (dd[hi] - dd[lo]) * p.inv_dy
-                                                           : (dd[hi] - dd[lo]) * (0.5 * p.inv_dy);
+                        double uyy = (cv[c + 1] - 2.0 * cv[c] + cv[c - 1]) * p.inv_dy2;
+                        double uxy = (first || last) ? (dd[hi] - dd[lo]) * p.inv_dy
+                                                     : (dd[hi] - dd[lo]) * (0.5 * p.inv_dy);
+                        if constexpr (DIST) {   // Dy = K dy at the moment's cell
+                            const double k = (double)kr[2 * ur + h];
+                            uyy *= k * k;
+                            uxy *= k;
+                        }
                         sMxx[m * nx + i] = (MT)(-D * (nu * uyy + uxx));
                         sMyy[m * nx + i] = (MT)(-D * (nu * uxx + uyy));
                         sMxy[m * nx + i] = (MT)(-D * (1.0 - nu) * uxy);
@@ -697,9 +720,11 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                     const int ci = i - a;
                     auto mx = [&](const MT* M, int mm, int cc) { return (double)M[mm * nx + a + cc]; };
                     auto d1m = [&](int mm) {
-                        return ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
-                                       : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
-                                                  : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                        const double d = ci == 1 ? (mx(sMxy, mm, 2) - mx(sMxy, mm, 0)) * (0.5 * p.inv_dx)
+                                                 : (ci == 0 ? (mx(sMxy, mm, 1) - mx(sMxy, mm, 0)) * p.inv_dx
+                                                            : (mx(sMxy, mm, 2) - mx(sMxy, mm, 1)) * p.inv_dx);
+                        if constexpr (DIST) return d * (double)kr[2 * (JM + mm - JU) + h];
+                        return d;
                     };
                     float f[2] = {0.0f, 0.0f};
 #pragma unroll
@@ -708,11 +733,17 @@ __device__ __forceinline__ void phase_rows(const KP& p, const Team& tm, const St
                         const bool first = TOP && o == 0, last = o == LO;
                         const int c = first ? mr + 1 : (last ? mr - 1 : mr);
                         const int lo = first ? mr : mr - 1, hi = last ? mr : mr + 1;
-                        const double bxx = (mx(sMxx, mr, 2) - 2.0 * mx(sMxx, mr, 1) + mx(sMxx, mr, 0)) * p.inv_dx2;
-                        const double byy = ((double)sMyy[(c + 1) * nx + i] - 2.0 * (double)sMyy[c * nx + i] +
-                                            (double)sMyy[(c - 1) * nx + i]) * p.inv_dy2;
-                        const double bxy = (first || last) ? (d1m(hi) - d1m(lo)) * p.inv_dy
-                                                           : (d1m(hi) - d1m(lo)) * (0.5 * p.inv_dy);
+                        double bxx = (mx(sMxx, mr, 2) - 2.0 * mx(sMxx, mr, 1) + mx(sMxx, mr, 0)) * p.inv_dx2;
+                        double byy = ((double)sMyy[(c + 1) * nx + i] - 2.0 * (double)sMyy[c * nx + i] +
+                                      (double)sMyy[(c - 1) * nx + i]) * p.inv_dy2;
+                        double bxy = (first || last) ? (d1m(hi) - d1m(lo)) * p.inv_dy
+                                                     : (d1m(hi) - d1m(lo)) * (0.5 * p.inv_dy);
+                        if constexpr (DIST) {   // Dx, Dy at the output cell
+                            const double k = (double)kr[2 * (o - JU) + h];
+                            bxx *= k * k;
+                            byy *= k * k;
+                            bxy *= k;
+                        }
                         const CellF F = h ? fr[2 * o + 1] : fr[2 * o];
                         f[o] = rhs_cell(i, j0 + o, bxx, bxy, byy, uo[h][o], F);
                     }
@@ -2187,7 +2218,7 @@ __device__ __forceinline__ bool rhs_call(const KP& p, const Team& tm, Ctl& s, St
     if (p.ptime && tm.rank == 0 && threadIdx.x == 0) p.ptime[23] += wall_clock64() - tl;
 #endif
     if constexpr (sizeof(MT) == 8) phase_rows_lds<DIST>(p, tm, st, uoff, it, twx, smem, kc);
-    else phase_rows<MT, 8, 4>(p, tm, st, uoff, it, twx, smem, kc);
+    else phase_rows<MT, 8, 4, DIST>(p, tm, st, uoff, it, twx, smem, kc);
     if (!grid_sync(p, tm, gen, flag, TAG_ROWS, tl)) return false;
     phase_cols(p, tm, specbuf(p, st.sb_out), twy, smem);
     if (!grid_sync(p, tm, gen, flag, TAG_COLS, tl)) return false;

@@ -140,11 +140,37 @@ def test_gpu_distortion_rhs_vs_oracle():
 
 
 @pytest.mark.gpu
-def test_gpu_distortion_needs_the_fp64_moment_path():
-    w = lv_case(n=600, seed=3)
+def test_gpu_distortion_large_grid_rhs_vs_oracle():
+    """Grids wider than 512 cells keep the plate moments in FP32 and 1/K in registers: the same
+    distorted case at n = 600 against the oracle, to the bounds of the undistorted large grids."""
+    from oracle import pyref
+    w = distorted_case(n=600, seed=3)
+    sim = hip_sim(w)
+    ref = pyref.RefEngine.from_inputs(sim.inputs)
+    d = w["domain"]
+    rng = np.random.default_rng(8)
+    u = -60 * np.exp(-(d.R / 1e6) ** 2) + 0.01 * rng.standard_normal((d.nx, d.ny))
+    for t in (0.0, 5000.0, 5010.0):
+        dg, _ = sim.engine.rhs(t, u)
+        dr, _ = ref.rhs(t, u)
+        m = np.abs(dg - dr).max() / np.abs(dr).max()
+        l2 = np.linalg.norm(dg - dr) / np.linalg.norm(dr)
+        print(f"t={t} dudt max-rel {m:.3e} L2-rel {l2:.3e}")
+        assert m < 2e-6 and l2 < 1e-6, (t, m, l2)
+    for f in ("ue", "dz_ss"):
+        a, b = sim.engine.get_field(f), ref.get_field(f)
+        assert np.abs(a - b).max() <= 5e-6 * np.abs(b).max(), f
+    sim.engine.close()
+
+
+@pytest.mark.gpu
+def test_gpu_distortion_needs_six_rows():
+    w = lv_case(n=16, seed=3)
     d0 = w["domain"]
-    w["domain"] = fastiso.RegionalDomain(d0.Wx, d0.Wy, d0.nx, d0.ny, K=smooth_K(d0))
-    w["solidearth"] = fastiso.SolidEarth(w["domain"], layer_boundaries=[w["h"]],
-                                         layer_viscosities=[10.0 ** w["logeta"]])
+    d = fastiso.RegionalDomain(d0.Wx, d0.Wy * 5 / 16, d0.nx, 5, K=np.full((d0.nx, 5), 1.05))
+    w["domain"] = d
+    w["solidearth"] = fastiso.SolidEarth(d, layer_boundaries=[np.full((d.nx, d.ny), 88e3)],
+                                         layer_viscosities=[np.full((d.nx, d.ny), 1e21)])
+    w["bcs"] = fastiso.BoundaryConditions(d)
     with pytest.raises(RuntimeError, match="distortion"):
         hip_sim(w)
