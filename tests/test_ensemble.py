@@ -199,6 +199,35 @@ def test_ensemble_mixing_distorted_members_equals_single_contexts_bitwise():
         np.testing.assert_array_equal(res[m][0], e.get_field("u"), err_msg=f"member {m}")
 
 
+@pytest.mark.gpu
+def test_ensemble_wide_distorted_members_equal_single_contexts_bitwise():
+    """Members wider than 512 cells keep FP32 plate moments; with a distorted one the launch runs
+    that path's distortion instance (1/K in registers), K = 1 for the other member. Both equal
+    their single contexts bit for bit over a short window."""
+    ws = workloads.ensemble(nmem=2, sizes=(525,), t_end=1.0)
+    inputs = [sim_inputs(w).inputs for w in ws]
+    d = inputs[0]["cfg"]
+    x = np.linspace(-1.0, 1.0, d.nx)[:, None]
+    y = np.linspace(-1.0, 1.0, d.ny)[None, :]
+    inputs[0] = dict(inputs[0], fields=dict(inputs[0]["fields"], distortion=0.97 + 0.04 * (x ** 2 + y ** 2)))
+    tout = (0.2, 0.4)
+    ens = hip_ensemble(inputs, 0)
+    ens.integrator_init()
+    for t in tout:
+        ens.integrate(t)
+    res = [(ens[m].get_field("u"), ens[m].stats()) for m in range(len(inputs))]
+    if not DRYRUN:
+        ens.close()
+    for m, inp in enumerate(inputs):
+        e = hip_single(with_tout(inp, tout))
+        e.integrator_init()
+        e.integrate(tout[-1], lambda t, k: None)
+        np.testing.assert_array_equal(res[m][0], e.get_field("u"), err_msg=f"member {m}")
+        assert res[m][1]["n_accept"] == e.stats()["n_accept"] > 0
+        if not DRYRUN:
+            e.close()
+
+
 # ---- relaxed-mantle (ELRA) ensembles ------------------------------------------------------------
 def elra_ensemble(sizes=(32, 40, 48, 36), t_end=2000.0):
     import fastiso
