@@ -121,6 +121,8 @@ struct Team {
 struct KP {
     int nx, ny, nhx, nfx, nfy, nfhx, nrowitems, ncolitems, colnb;
     int c2rnb, nc2ritems;   // row pairs per item of the standalone inverse-transform phase, its item count
+    int erb, nerowitems;    // relaxed mantle: row pairs per item of its rows and inverse phases, item count
+    int ecb, necolitems;    // relaxed mantle: spectral columns per item of its columns phase, item count
     int i1, i2, j1, j2;  // 1-based crop indices (src/convolutions.jl:196-204)
     double dx, dy, A, inv_dx, inv_dy, inv_dx2, inv_dy2;
     double g, spy, rho_ice, rho_sw_ice, rho_ice_sw, rho_seawater, rho_water, rho_um, rho_litho, nu;

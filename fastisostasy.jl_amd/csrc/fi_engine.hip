@@ -1163,6 +1163,12 @@ int fi_init(fi_ctx* c, double t0, double t_end) {
     }
     kp.ncolitems = (c->nhx + kp.colnb - 1) / kp.colnb;
     kp.nc2ritems = (kp.nrowitems + kp.c2rnb - 1) / kp.c2rnb;
+    // relaxed mantle: one row pair per rows/inverse item, two columns per columns item (an
+    // ensemble batches more per item for its team size)
+    kp.erb = 1;
+    kp.nerowitems = kp.nrowitems;
+    kp.ecb = 2;
+    kp.necolitems = (c->nfhx + 1) / 2;
     kp.i1 = c->i1;
     kp.i2 = c->i2;
     kp.j1 = c->j1;
@@ -2450,6 +2456,20 @@ int batch_launch(fi_batch* b, const BatchScratch& sc, const void* f, size_t lds,
             kph[m].c2rnb = np2;
             kph[m].nc2ritems = (c->kp.nrowitems + np2 - 1) / np2;
         }
+        // relaxed-mantle members likewise: row pairs per rows/inverse item and columns per
+        // columns item so that a team covers them in one round where the LDS after the member's
+        // twiddles holds the batch (twiddles and two buffers of the batch); bitwise neutral
+        if (c->elra) {
+            const size_t avail = lds - (size_t)(c->nx + c->ny) * 8;
+            int nb = std::max(1, (c->kp.nrowitems + ts - 1) / ts);
+            while (nb > 1 && (size_t)(1 + 2 * nb) * c->nfx * 8 > avail) --nb;
+            kph[m].erb = nb;
+            kph[m].nerowitems = (c->kp.nrowitems + nb - 1) / nb;
+            int cb = std::max(2, (c->nfhx + ts - 1) / ts);
+            while (cb > 2 && (size_t)(1 + 2 * cb) * c->nfy * 8 > avail) --cb;
+            kph[m].ecb = cb;
+            kph[m].necolitems = (c->nfhx + cb - 1) / cb;
+        }
         kph[m].ctl = sc.ctl_all + m;
         ptrs[m] = sc.kp_all + m;
         b->ctl_h[m] = c->ctl;
@@ -2784,6 +2804,10 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noa
         lds = 0;
         for (int m = 0; m < nm; ++m) lds = std::max(lds, b->mem[m]->lds);
     }
+    // relaxed-mantle members batch row pairs and columns per item within the launch's LDS: offer
+    // 64 KB (two 256-thread workgroups still share a CU), unless that lowers the occupancy
+    const size_t lds_need = lds;
+    if (elra) lds = std::max(lds, (size_t)64 * 1024);
     b->elra = elra;
     if (lds > 160 * 1024) return bfail(b, FI_ERR_UNSUPPORTED, "members too large for one workgroup's LDS");
     b->dist = dist;
@@ -2800,10 +2824,18 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noa
     b->wg_threads = NT;
     const char* nt_env = getenv("FASTISO_BATCH_WG_THREADS");
     const bool allow256 = !(nt_env && atoi(nt_env) == NT);
+    auto occupancy = [&](const void* fk, int threads, size_t l, int* out) -> hipError_t {
+        hipError_t e = hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l);
+        if (e != hipSuccess) return e;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, fk, threads, l);
+    };
     if (allow256 && (elra || mt_double) && maxnx <= 256) {
         const void* f2 = fi256::batch_kernel_nt256(dist, elra);
-        BHIPCHECK(b, hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        BHIPCHECK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f2, 256, lds));
+        BHIPCHECK(b, occupancy(f2, 256, lds, &per_cu));
+        if (per_cu < 2 && lds > lds_need) {
+            lds = lds_need;
+            BHIPCHECK(b, occupancy(f2, 256, lds, &per_cu));
+        }
         if (per_cu >= 2) {
             f = f2;
             b->wg_threads = 256;
@@ -2811,8 +2843,15 @@ int batch_integrate(fi_batch* b, const double* t_targets, int32_t fsal, bool noa
     }
     if (!f) {
         f = batch_kernel(mt_double, dist, elra);
-        BHIPCHECK(b, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        BHIPCHECK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, lds));
+        BHIPCHECK(b, occupancy(f, NT, lds, &per_cu));
+        if (lds > lds_need) {
+            int per_cu_need = 0;
+            BHIPCHECK(b, occupancy(f, NT, lds_need, &per_cu_need));
+            if (per_cu_need > per_cu) {
+                lds = lds_need;
+                BHIPCHECK(b, occupancy(f, NT, lds, &per_cu));
+            }
+        }
     }
     const int gmax = per_cu * ncu;
     if (gmax < 1) return bfail(b, FI_ERR_UNSUPPORTED, "ensemble kernel does not fit on the device");

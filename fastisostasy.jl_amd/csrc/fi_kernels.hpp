@@ -1641,7 +1641,9 @@ __device__ __forceinline__ SsBracket ss_bracket(const KP& p, double t) {
     return b;
 }
 
-// pointwise sparse diagnostics with the updated barystatic sea level z_bsl
+// pointwise sparse diagnostics with the updated barystatic sea level z_bsl. Row pairs go to the
+// workgroups as in the relaxed-mantle rows phase (items of p.erb consecutive pairs; one pair per
+// item elsewhere), which reads these fields after a workgroup barrier only.
 __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, const StageDesc& st, double uoff,
                                                const IceT it, double z_bsl) {
     const G<float> ec = p.ext_corners;
@@ -1650,10 +1652,11 @@ __device__ __forceinline__ void phase_sp_point(const KP& p, const Team& tm, cons
     const double oe = ((0.25 * ec[0] + 0.25 * ec[1]) + (0.25 * ec[2] + 0.25 * ec[3])) - p.ebc_x;
     const double og = ((0.25 * ec[4] + 0.25 * ec[5]) + (0.25 * ec[6] + 0.25 * ec[7])) - p.sbc_x;
     const int nx = p.nx;
-    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
-      const int ncell = (min(2 * item + 2, p.ny) - 2 * item) * nx;
+    for (int item = tm.rank; item < p.nerowitems; item += tm.size) {
+      const int pr0 = item * p.erb, pr1 = min(pr0 + p.erb, p.nrowitems);
+      const int ncell = (min(2 * pr1, p.ny) - 2 * pr0) * nx;
       for (int q = threadIdx.x; q < ncell; q += NT) {
-        const long c = (long)2 * item * nx + q;
+        const long c = (long)2 * pr0 * nx + q;
         const double u = u_raw(p, st, c) - uoff;
         const double H = ice_at(p, it, c);
         const double ue_old = p.ue[c];
@@ -1742,17 +1745,23 @@ __device__ __forceinline__ bool sparse_phases(const KP& p, const Team& tm, Ctl& 
 // update_deformation_rhs!, which the reference calls first in this method, only fills scratch
 // buffers that the method does not read afterwards.
 // ------------------------------------------------------------------------------------------
+// An item of elra_rows and elra_c2r is p.erb consecutive row pairs, transformed as one batch (an
+// ensemble sets erb so that a team covers a member's rows in one round); an item of elra_cols is
+// p.ecb columns. Partial sums stay per row pair, so results do not depend on erb or ecb.
 __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, const StageDesc& st, double uoff,
                                                 const IceT it, char* smem) {
-    const int nx = p.nx, ny = p.ny, nfx = p.nfx;
+    const int nx = p.nx, ny = p.ny, nfx = p.nfx, erb = p.erb;
+    const float inv_nfx = 1.0f / (float)nfx, inv_nfhx = 1.0f / (float)p.nfhx;
     float2* tw = (float2*)smem;
     float2* sf = tw + nfx;
-    float2* sg = sf + nfx;
+    float2* sg = sf + erb * nfx;
     stage_twiddles(p, p.pfx, tw);
     const bool wsig = st.write_sigma != 0;
-    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
-        const int j0 = 2 * item;
-        for (int i = threadIdx.x; i < nfx; i += NT) {
+    for (int item = tm.rank; item < p.nerowitems; item += tm.size) {
+        const int pr0 = item * erb, npair = min(erb, p.nrowitems - pr0);
+        for (int idx = threadIdx.x; idx < npair * nfx; idx += NT) {
+            const int b = fdivi(idx, inv_nfx), i = idx - b * nfx;
+            const int j0 = 2 * (pr0 + b);
             float2 z = make_float2(0.f, 0.f);
             if (i < nx) {
 #pragma unroll
@@ -1783,12 +1792,15 @@ __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, con
                     }
                 }
             }
-            sf[i] = z;
+            sf[idx] = z;
         }
         __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, -1);
-        for (int kx = threadIdx.x; kx < p.nfhx; kx += NT) {
-            const float2 Z = res[kx], Zm = res[kx == 0 ? 0 : nfx - kx];
+        float2* res = fft_lds<float2>(sf, sg, npair, nfx, p.pfx, tw, -1);
+        for (int idx = threadIdx.x; idx < npair * p.nfhx; idx += NT) {
+            const int b = fdivi(idx, inv_nfhx), kx = idx - b * p.nfhx;
+            const int j0 = 2 * (pr0 + b);
+            const float2* rb = res + b * nfx;
+            const float2 Z = rb[kx], Zm = rb[kx == 0 ? 0 : nfx - kx];
             float2 A, B;
             A.x = 0.5f * (Z.x + Zm.x);
             A.y = 0.5f * (Z.y - Zm.y);
@@ -1802,16 +1814,16 @@ __device__ __forceinline__ void phase_elra_rows(const KP& p, const Team& tm, con
 }
 
 __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, char* smem) {
-    const int ny = p.ny, nfy = p.nfy, nfhx = p.nfhx;
+    const int ny = p.ny, nfy = p.nfy, nfhx = p.nfhx, ecb = p.ecb;
+    const float inv_nfy = 1.0f / (float)nfy, inv_ny = 1.0f / (float)ny;
     float2* tw = (float2*)smem;
     float2* sa = tw + nfy;
-    float2* sb = sa + 2 * nfy;
+    float2* sb = sa + ecb * nfy;
     stage_twiddles(p, p.pfy, tw);
-    const int nitems = (nfhx + 1) / 2;
-    for (int item = tm.rank; item < nitems; item += tm.size) {
-        const int kx0 = 2 * item, nb = min(2, nfhx - kx0);
+    for (int item = tm.rank; item < p.necolitems; item += tm.size) {
+        const int kx0 = ecb * item, nb = min(ecb, nfhx - kx0);
         for (int idx = threadIdx.x; idx < nb * nfy; idx += NT) {
-            const int s = idx >= nfy ? 1 : 0, j = idx - s * nfy;
+            const int s = fdivi(idx, inv_nfy), j = idx - s * nfy;
             float2 v = make_float2(0.f, 0.f);
             if (j < ny) v = p.pA[bidx(kx0 + s, j, ny)];
             sa[idx] = v;
@@ -1819,7 +1831,7 @@ __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, cha
         __syncthreads();
         float2* r = fft_lds<float2>(sa, sb, nb, nfy, p.pfy, tw, -1);
         for (int idx = threadIdx.x; idx < nb * nfy; idx += NT) {
-            const int s = idx >= nfy ? 1 : 0, ky = idx - s * nfy;
+            const int s = fdivi(idx, inv_nfy), ky = idx - s * nfy;
             const double2 G = p.gv_hat[(long)(kx0 + s) * nfy + ky];
             const double xr = r[idx].x, xi = r[idx].y;
             r[idx].x = (float)(xr * G.x - xi * G.y);
@@ -1829,7 +1841,7 @@ __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, cha
         float2* o = (r == sa) ? sb : sa;
         float2* r2 = fft_lds<float2>(r, o, nb, nfy, p.pfy, tw, +1);
         for (int idx = threadIdx.x; idx < nb * ny; idx += NT) {
-            const int s = idx >= ny ? 1 : 0, j = idx - s * ny;
+            const int s = fdivi(idx, inv_ny), j = idx - s * ny;
             st_shared(addr(p.oA, bidx(kx0 + s, j, ny)), r2[s * nfy + (p.j1 - 1) + j]);
         }
         __syncthreads();
@@ -1838,11 +1850,12 @@ __device__ __forceinline__ void phase_elra_cols(const KP& p, const Team& tm, cha
 
 __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, const StageDesc& st, double uoff,
                                                char* smem, double* red) {
-    const int nx = p.nx, ny = p.ny, nfx = p.nfx, nfhx = p.nfhx;
+    const int nx = p.nx, ny = p.ny, nfx = p.nfx, nfhx = p.nfhx, erb = p.erb;
+    const float inv_nfx = 1.0f / (float)nfx;
     const bool even = (nfx % 2) == 0;
     float2* tw = (float2*)smem;
     float2* sf = tw + nfx;
-    float2* sg = sf + nfx;
+    float2* sg = sf + erb * nfx;
     stage_twiddles(p, p.pfx, tw);
     // corners of the cropped u_eq before its BC: x(m, j) = sum_k w_k Re(X_k(j) e^{2 pi i k m / nfx})
     // at m = i1-1 and i1-1+nx-1 of the kept rows j = 0 and ny-1 (w = 1 for the DC and Nyquist
@@ -1874,10 +1887,12 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
     const double* v = vs[0];
     const double off = ((0.25 * v[0] + 0.25 * v[1]) + (0.25 * v[2] + 0.25 * v[3])) - p.vbc_x;
     G<float> K = p.Kb + (long)st.kout * p.Nl;
-    for (int item = tm.rank; item < p.nrowitems; item += tm.size) {
-        const int j0 = 2 * item;
-        const bool two = j0 + 1 < ny;
-        for (int k = threadIdx.x; k < nfx; k += NT) {
+    for (int item = tm.rank; item < p.nerowitems; item += tm.size) {
+        const int pr0 = item * erb, npair = min(erb, p.nrowitems - pr0);
+        for (int idx = threadIdx.x; idx < npair * nfx; idx += NT) {
+            const int b = fdivi(idx, inv_nfx), k = idx - b * nfx;
+            const int j0 = 2 * (pr0 + b);
+            const bool two = j0 + 1 < ny;
             const bool cj = k >= nfhx;
             const int kk = cj ? nfx - k : k;
             float2 A = p.oA[bidx(kk, j0, ny)];
@@ -1890,43 +1905,47 @@ __device__ __forceinline__ void phase_elra_c2r(const KP& p, const Team& tm, cons
                 A.y = -A.y;
                 B.y = -B.y;
             }
-            sf[k] = make_float2(A.x - B.y, A.y + B.x);
+            sf[idx] = make_float2(A.x - B.y, A.y + B.x);
         }
         __syncthreads();
-        float2* res = fft_lds<float2>(sf, sg, 1, nfx, p.pfx, tw, +1);
-        double accv[NV][3] = {};
-        for (int i0 = threadIdx.x; i0 < 2 * nx; i0 += VNT)
-#pragma unroll
-        for (int vv = 0; vv < NV; ++vv) {
-            const int idx = i0 + vv * NT;
-            if (idx >= 2 * nx) break;
-            double* acc = accv[vv];
-            const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
-            if (j >= ny) continue;
-            const long c = i + (long)j * nx;
-            const float2 z = res[m0 + i];
-            const double ueq = (double)(r ? z.y : z.x) - off;
-            // the stage displacement: stored by the rows phase when this call keeps it
-            const double u = st.write_u >= 0 ? ubuf(p, st.write_u)[c] : u_raw(p, st, c) - uoff;
-            const float dv = (float)((ueq - u) * (double)p.inv_tau[c]);
-            st_shared(addr(K, c), dv);
-            st_shared(addr(p.ueq, c), (float)ueq);
-            if (st.final_err) {
-                const double k1 = kslot(p, st.e1)[c], k2 = kslot(p, st.e2)[c], k3 = kslot(p, st.e3)[c];
-                const double un = ubuf(p, st.u_prev)[c];
-                const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)dv);
-                const double wgt = 1.0 / (p.abstol + fmax(fabs(un), fabs(u)) * p.reltol);
-                acc[0] += (a * wgt) * (a * wgt);
-                acc[1] += a * wgt * wgt;
-                acc[2] += wgt * wgt;
+        float2* resb = fft_lds<float2>(sf, sg, npair, nfx, p.pfx, tw, +1);
+        for (int b = 0; b < npair; ++b) {
+            const int pair = pr0 + b, j0 = 2 * pair;
+            const float2* res = resb + b * nfx;
+            double accv[NV][3] = {};
+            for (int i0 = threadIdx.x; i0 < 2 * nx; i0 += VNT)
+    #pragma unroll
+            for (int vv = 0; vv < NV; ++vv) {
+                const int idx = i0 + vv * NT;
+                if (idx >= 2 * nx) break;
+                double* acc = accv[vv];
+                const int r = idx >= nx ? 1 : 0, i = idx - r * nx, j = j0 + r;
+                if (j >= ny) continue;
+                const long c = i + (long)j * nx;
+                const float2 z = res[m0 + i];
+                const double ueq = (double)(r ? z.y : z.x) - off;
+                // the stage displacement: stored by the rows phase when this call keeps it
+                const double u = st.write_u >= 0 ? ubuf(p, st.write_u)[c] : u_raw(p, st, c) - uoff;
+                const float dv = (float)((ueq - u) * (double)p.inv_tau[c]);
+                st_shared(addr(K, c), dv);
+                st_shared(addr(p.ueq, c), (float)ueq);
+                if (st.final_err) {
+                    const double k1 = kslot(p, st.e1)[c], k2 = kslot(p, st.e2)[c], k3 = kslot(p, st.e3)[c];
+                    const double un = ubuf(p, st.u_prev)[c];
+                    const double a = st.dt * (BS3_BT1 * k1 + BS3_BT2 * k2 + BS3_BT3 * k3 + BS3_BT4 * (double)dv);
+                    const double wgt = 1.0 / (p.abstol + fmax(fabs(un), fabs(u)) * p.reltol);
+                    acc[0] += (a * wgt) * (a * wgt);
+                    acc[1] += a * wgt * wgt;
+                    acc[2] += wgt * wgt;
+                }
             }
-        }
-        if (st.final_err) {
-            block_sum<3>(red, accv);
-            if (threadIdx.x == 0) {
-                st_shared(addr(p.part, 4 * item + 0), accv[0][0]);
-                st_shared(addr(p.part, 4 * item + 1), accv[0][1]);
-                st_shared(addr(p.part, 4 * item + 2), accv[0][2]);
+            if (st.final_err) {
+                block_sum<3>(red, accv);
+                if (threadIdx.x == 0) {
+                    st_shared(addr(p.part, 4 * pair + 0), accv[0][0]);
+                    st_shared(addr(p.part, 4 * pair + 1), accv[0][1]);
+                    st_shared(addr(p.part, 4 * pair + 2), accv[0][2]);
+                }
             }
         }
         __syncthreads();

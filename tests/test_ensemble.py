@@ -352,6 +352,31 @@ def test_gpu_mixed_ensemble_members_equal_single_contexts_bitwise():
 
 
 @pytest.mark.gpu
+def test_gpu_elra_ensemble_repeatable_across_runs_and_team_sizes():
+    """Members of the ELRA benchmark ensemble (sides 64..256) batch several row pairs and columns per
+    item, as many as a team needs to cover them in one round, and the sparse update's pointwise
+    phase hands the row pairs out the same way (the rows phase after it reads those cells behind a
+    workgroup barrier only). Two runs at one team size and a run at another agree bit for bit."""
+    if DRYRUN:
+        pytest.skip("the team mapping is the HIP ensemble kernel's")
+    import fastiso
+    inputs = [sim_inputs(w).inputs for w in workloads.elra_ensemble(nmem=48)]
+    assert {inp["cfg"].nx for inp in inputs} >= {64, 256}
+    runs = []
+    for ts in (16, 16, 4):
+        ens = fastiso.Ensemble(inputs, team_size=ts)
+        ens.integrator_init()
+        for t in (1000.0, 2000.0):
+            ens.integrate(t, fsal=False)
+        runs.append([(ens[m].get_field("u"), ens[m].stats()["n_accept"]) for m in range(len(inputs))])
+        ens.close()
+    for r in runs[1:]:
+        for m in range(len(inputs)):
+            np.testing.assert_array_equal(runs[0][m][0], r[m][0], err_msg=f"member {m}")
+            assert runs[0][m][1] == r[m][1], m
+
+
+@pytest.mark.gpu
 def test_gpu_elra_ensemble_tuner_settles_on_short_windows():
     """The ELRA members take far fewer than the tuner's 64 attempts per 100-yr window, so its
     capped trials run into the window end: it must switch to timing one whole window per
