@@ -352,21 +352,25 @@ def test_gpu_mixed_ensemble_members_equal_single_contexts_bitwise():
 
 
 @pytest.mark.gpu
-def test_gpu_elra_ensemble_repeatable_across_runs_and_team_sizes():
-    """Members of the ELRA benchmark ensemble (sides 64..256) batch several row pairs and columns per
-    item, as many as a team needs to cover them in one round, and the sparse update's pointwise
-    phase hands the row pairs out the same way (the rows phase after it reads those cells behind a
-    workgroup barrier only). Two runs at one team size and a run at another agree bit for bit."""
+@pytest.mark.parametrize("kind", ["elra", "lv"])
+def test_gpu_ensemble_repeatable_across_runs_and_team_sizes(kind):
+    """Members of the benchmark ensembles (sides 64..256) batch several row pairs and columns per
+    item, as many as a team needs to cover them in one round; for ELRA members the sparse update's
+    pointwise phase hands the row pairs out the same way (the rows phase after it reads those cells
+    behind a workgroup barrier only). Two runs at one team size and a run at another agree bit for
+    bit."""
     if DRYRUN:
         pytest.skip("the team mapping is the HIP ensemble kernel's")
     import fastiso
-    inputs = [sim_inputs(w).inputs for w in workloads.elra_ensemble(nmem=48)]
+    ws = workloads.elra_ensemble(nmem=48) if kind == "elra" else workloads.ensemble(nmem=48, t_end=20e3)
+    inputs = [sim_inputs(w).inputs for w in ws]
     assert {inp["cfg"].nx for inp in inputs} >= {64, 256}
+    win = 1000.0 if kind == "elra" else 10.0
     runs = []
-    for ts in (16, 16, 4):
+    for ts in ((16, 16, 4) if kind == "elra" else (32, 32, 8)):
         ens = fastiso.Ensemble(inputs, team_size=ts)
         ens.integrator_init()
-        for t in (1000.0, 2000.0):
+        for t in (win, 2 * win):
             ens.integrate(t, fsal=False)
         runs.append([(ens[m].get_field("u"), ens[m].stats()["n_accept"]) for m in range(len(inputs))])
         ens.close()
