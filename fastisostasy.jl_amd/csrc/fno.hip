@@ -55,6 +55,7 @@ struct Gemm {
     int nb0;
     long sAb1, sBb1, sCb1;
     int ones_row1;   // 1 + the row of A that reads 1.0 instead of memory (0: none; LDS-staged path)
+    int batch;       // products in the launch (paired LDS-staged path)
 };
 
 __device__ __forceinline__ long boff(int b, int nb0, long s0, long s1) {
@@ -200,38 +201,56 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
 // waves feed the MFMAs from LDS. The next step's loads are issued before the current step's MFMAs;
 // one barrier per step. Same MFMA operands and order as k_gemm, so the same results bit for bit. A
 // row of ones in A (ones_row1) makes a bias gradient, the sum over K of B, one more row of its
-// weight gradient's product.
-template <typename TA, typename TB>
+// weight gradient's product. PAIR (products of at most 32 rows, no split): the tile's upper and
+// lower 32 rows are two consecutive products of the batch (waves 0-1 and 2-3), so no wave idles
+// on rows past M (the x-transforms per row have 2 modes_x = 32 rows).
+template <typename TA, typename TB, bool PAIR = false>
 __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
     constexpr int PS = 40;   // LDS pitch (BF16) of a row of 32 k: 80 bytes, 16-byte aligned rows
+    constexpr int NBL = PAIR ? 8 : 4;   // B elements per thread and step
     __shared__ __align__(16) __bf16 sa[2][64 * PS];
-    __shared__ __align__(16) __bf16 sb[2][32 * PS];
+    __shared__ __align__(16) __bf16 sb[2][(PAIR ? 64 : 32) * PS];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int b = blockIdx.z;
-    const int mb = blockIdx.x * 64, n0 = blockIdx.y * 32;
+    const int b = PAIR ? 2 * (int)blockIdx.z : (int)blockIdx.z;
+    const int mb = PAIR ? 0 : blockIdx.x * 64, n0 = blockIdx.y * 32;
     const int r = lane & 15, kg = lane >> 4;
     int kbeg = 0, kend = g.K;
     const TA* A;
     const TB* B;
-    if (g.nslice > 0) {
-        const int pb = b / g.nslice, sl = b - pb * g.nslice;
-        kbeg = sl * g.kchunk;
-        kend = min(g.K, kbeg + g.kchunk);
-        A = (const TA*)g.A + boff(pb, g.nb0, g.sAb, g.sAb1);
-        B = (const TB*)g.B + boff(pb, g.nb0, g.sBb, g.sBb1);
-    } else if (g.kchunk > 0) {
-        kbeg = b * g.kchunk;
-        kend = min(g.K, kbeg + g.kchunk);
-        A = (const TA*)g.A;
-        B = (const TB*)g.B;
-    } else {
+    const TA* A1 = nullptr;   // PAIR: the second product's operands and output
+    const TB* B1 = nullptr;
+    float* C1 = nullptr;
+    const bool have1 = PAIR && b + 1 < g.batch;
+    if constexpr (PAIR) {
         A = (const TA*)g.A + boff(b, g.nb0, g.sAb, g.sAb1);
         B = (const TB*)g.B + boff(b, g.nb0, g.sBb, g.sBb1);
+        if (have1) {
+            A1 = (const TA*)g.A + boff(b + 1, g.nb0, g.sAb, g.sAb1);
+            B1 = (const TB*)g.B + boff(b + 1, g.nb0, g.sBb, g.sBb1);
+            C1 = g.C + boff(b + 1, g.nb0, g.sCb, g.sCb1);
+        }
+    } else {
+        if (g.nslice > 0) {
+            const int pb = b / g.nslice, sl = b - pb * g.nslice;
+            kbeg = sl * g.kchunk;
+            kend = min(g.K, kbeg + g.kchunk);
+            A = (const TA*)g.A + boff(pb, g.nb0, g.sAb, g.sAb1);
+            B = (const TB*)g.B + boff(pb, g.nb0, g.sBb, g.sBb1);
+        } else if (g.kchunk > 0) {
+            kbeg = b * g.kchunk;
+            kend = min(g.K, kbeg + g.kchunk);
+            A = (const TA*)g.A;
+            B = (const TB*)g.B;
+        } else {
+            A = (const TA*)g.A + boff(b, g.nb0, g.sAb, g.sAb1);
+            B = (const TB*)g.B + boff(b, g.nb0, g.sBb, g.sBb1);
+        }
     }
-    float* C = g.C + ((g.nslice > 0 || g.kchunk > 0) ? (long)b * g.sCb : boff(b, g.nb0, g.sCb, g.sCb1));
-    // this thread's elements of a K step: A e = tid + 256 j (j < 8), B e = tid + 256 j (j < 4)
+    float* C = g.C + ((!PAIR && (g.nslice > 0 || g.kchunk > 0)) ? (long)b * g.sCb : boff(b, g.nb0, g.sCb, g.sCb1));
+    // this thread's elements of a K step: A e = tid + 256 j (j < 8), B e = tid + 256 j (j < NBL;
+    // with PAIR, e >= 1024 is the second product's)
     const bool afk = g.sAk == 1, bfk = g.sBk == 1;
-    int arow[8], ak[8], bcol[4], bk[4];
+    int arow[8], ak[8], bcol[NBL], bk[NBL];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int e = tid + GEMM_THREADS * j;
@@ -239,31 +258,50 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
         ak[j] = afk ? e & 31 : e >> 6;
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NBL; ++j) {
         const int e = tid + GEMM_THREADS * j;
-        bcol[j] = bfk ? e >> 5 : e & 31;
-        bk[j] = bfk ? e & 31 : e >> 5;
+        if constexpr (PAIR) {
+            const int ee = e & 1023, h = e >> 10;
+            bcol[j] = (bfk ? ee >> 5 : ee & 31) + 32 * h;   // LDS row: column of product h
+            bk[j] = bfk ? ee & 31 : ee >> 5;
+        } else {
+            bcol[j] = bfk ? e >> 5 : e & 31;
+            bk[j] = bfk ? e & 31 : e >> 5;
+        }
     }
     TA va[8];
-    TB vb[4];
+    TB vb[NBL];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int m = mb + arow[j], k = k0 + ak[j];
-            va[j] = (m < g.M && k < kend) ? (m + 1 == g.ones_row1 ? (TA)1.f : A[(long)m * g.sAm + (long)k * g.sAk])
-                                          : (TA)0.f;
+            if constexpr (PAIR) {
+                const int h = arow[j] >> 5, m = arow[j] & 31, k = k0 + ak[j];
+                const TA* Ah = h ? A1 : A;
+                va[j] = (m < g.M && k < kend && (h == 0 || have1)) ? Ah[(long)m * g.sAm + (long)k * g.sAk] : (TA)0.f;
+            } else {
+                const int m = mb + arow[j], k = k0 + ak[j];
+                va[j] = (m < g.M && k < kend)
+                            ? (m + 1 == g.ones_row1 ? (TA)1.f : A[(long)m * g.sAm + (long)k * g.sAk])
+                            : (TA)0.f;
+            }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = n0 + bcol[j], k = k0 + bk[j];
-            vb[j] = (n < g.N && k < kend) ? B[(long)k * g.sBk + (long)n * g.sBn] : (TB)0.f;
+        for (int j = 0; j < NBL; ++j) {
+            if constexpr (PAIR) {
+                const int h = bcol[j] >> 5, n = n0 + (bcol[j] & 31), k = k0 + bk[j];
+                const TB* Bh = h ? B1 : B;
+                vb[j] = (n < g.N && k < kend && (h == 0 || have1)) ? Bh[(long)k * g.sBk + (long)n * g.sBn] : (TB)0.f;
+            } else {
+                const int n = n0 + bcol[j], k = k0 + bk[j];
+                vb[j] = (n < g.N && k < kend) ? B[(long)k * g.sBk + (long)n * g.sBn] : (TB)0.f;
+            }
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) sa[buf][arow[j] * PS + ak[j]] = (__bf16)va[j];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sb[buf][bcol[j] * PS + bk[j]] = (__bf16)vb[j];
+        for (int j = 0; j < NBL; ++j) sb[buf][bcol[j] * PS + bk[j]] = (__bf16)vb[j];
     };
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     gload(kbeg);
@@ -273,16 +311,23 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
     for (int k0 = kbeg; k0 < kend; k0 += 32) {
         const bool more = k0 + 32 < kend;
         if (more) gload(k0 + 32);
+        const int bh = PAIR ? 32 * (wave >> 1) : 0;   // this wave's product's B rows in LDS
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sa[buf][(wave * 16 + r) * PS + 8 * kg]);
-        const bf16x8 x0 = *reinterpret_cast<const bf16x8*>(&sb[buf][r * PS + 8 * kg]);
-        const bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&sb[buf][(16 + r) * PS + 8 * kg]);
+        const bf16x8 x0 = *reinterpret_cast<const bf16x8*>(&sb[buf][(bh + r) * PS + 8 * kg]);
+        const bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&sb[buf][(bh + 16 + r) * PS + 8 * kg]);
         acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x0, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x1, acc1, 0, 0, 0);
         if (more) lstore(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
-    const int m0 = mb + wave * 16, bn0 = n0 + r, bn1 = n0 + 16 + r;
+    if constexpr (PAIR) {
+        if ((wave >> 1) == 1) {
+            if (!have1) return;
+            C = C1;
+        }
+    }
+    const int m0 = PAIR ? (wave & 1) * 16 : mb + wave * 16, bn0 = n0 + r, bn1 = n0 + 16 + r;
     const bool b0ok = bn0 < g.N, b1ok = bn1 < g.N;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -732,7 +777,21 @@ int gemm(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, 
         SCHECK(s, hipGetLastError());
         return 0;
     }
-    Gemm g{A.p, B.p, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta, 0, nb0, sAb1, sBb1, sCb1};
+    Gemm g{A.p, B.p, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta, 0, nb0, sAb1, sBb1, sCb1,
+           0, batch};
+    if (M <= 32 && batch >= 2 && (sAk != 1 || sBk != 1)) {   // two products per workgroup
+        const dim3 grid(1, (N + 31) / 32, (batch + 1) / 2);
+        if (A.bf && B.bf)
+            hipLaunchKernelGGL((k_gemm_s<__bf16, __bf16, true>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        else if (A.bf)
+            hipLaunchKernelGGL((k_gemm_s<__bf16, float, true>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        else if (B.bf)
+            hipLaunchKernelGGL((k_gemm_s<float, __bf16, true>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        else
+            hipLaunchKernelGGL((k_gemm_s<float, float, true>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        SCHECK(s, hipGetLastError());
+        return 0;
+    }
     launch_gemm(s, g, dim3((M + 63) / 64, (N + 31) / 32, batch), A.bf, B.bf);
     SCHECK(s, hipGetLastError());
     return 0;
