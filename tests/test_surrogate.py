@@ -124,3 +124,38 @@ def test_samples_side_by_side_match_one_at_a_time():
         m = int(np.prod(shp))
         e = relerr(g3[o:o + m], gr[o:o + m])
         assert e < GRAD_TOL, (key, e)
+
+
+def _splitmix64(state):
+    """the training loop's sample picker (splitmix64), for replaying its picks"""
+    m = (1 << 64) - 1
+    state = (state + 0x9E3779B97F4A7C15) & m
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return state, z ^ (z >> 31)
+
+
+def test_training_step_loss_over_chunked_batch():
+    """fi_surrogate_train gathers each step's samples on the device and runs them side by side in
+    passes of at most 16: the reported loss of a 20-sample step (passes of 16 and 4) is the mean of
+    the single-sample losses of the samples its seeded picker chose."""
+    nx, ny = 40, 36
+    s = make(nx, ny)
+    cfg = s.config_dict()
+    th = F.init_params(cfg, 13)
+    s.set_params(th)
+    n = 7
+    X = np.stack([sample(nx, ny, cfg["in_channels"], 40 + k)[0] for k in range(n)])
+    T = np.stack([sample(nx, ny, cfg["in_channels"], 40 + k)[1] for k in range(n)])
+    s.set_data(X, T)
+    seed, batch = 5, 20
+    st, picks = seed, []
+    for _ in range(batch):
+        st, z = _splitmix64(st)
+        picks.append(z % n)
+    assert len(set(picks)) > 1
+    single = [s.loss_grad(X[k], T[k])[0] for k in range(n)]
+    want = float(np.mean([single[k] for k in picks]))
+    losses = s.train(1, batch=batch, lr=0.0, lr_min=0.0, seed=seed, wd=0.0)
+    assert abs(losses[0] - want) <= 1e-9 * want, (losses[0], want)
