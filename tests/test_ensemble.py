@@ -280,6 +280,33 @@ def test_gpu_elra_ensemble_members_equal_single_contexts_bitwise(team_size):
         assert np.abs(a - b).max() <= 2e-3 * np.abs(b).max(), m
 
 
+@pytest.mark.gpu
+def test_gpu_elra_ensemble_with_distorted_member_equals_single_contexts_bitwise():
+    """An ELRA member with map distortion (its load enters the convolution times K^2, read per cell
+    in the batched rows phase) beside undistorted ones, on a team size that batches several row
+    pairs and columns per item: every member equals its single-context run bit for bit."""
+    inputs = elra_ensemble()
+    d = inputs[2]["cfg"]
+    x = np.linspace(-1.0, 1.0, d.nx)[:, None]
+    y = np.linspace(-1.0, 1.0, d.ny)[None, :]
+    inputs[2] = dict(inputs[2], fields=dict(inputs[2]["fields"], distortion=0.95 + 0.06 * (x ** 2 + y ** 2)))
+    ens = hip_ensemble(inputs, 2)
+    ens.integrator_init()
+    ens.integrate(300.0)
+    ens.integrate(600.0)
+    for m, inp in enumerate(inputs):
+        e = hip_single(with_tout(inp, [300.0, 600.0]))
+        e.integrator_init()
+        e.integrate(600.0, lambda t, k: None)
+        np.testing.assert_array_equal(ens[m].get_field("u"), e.get_field("u"), err_msg=f"member {m}")
+        np.testing.assert_array_equal(ens[m].get_field("u_eq"), e.get_field("u_eq"), err_msg=f"member {m}")
+    # the distortion is in effect: the same member without it ends elsewhere
+    e0 = hip_single(with_tout(elra_ensemble()[2], [300.0, 600.0]))
+    e0.integrator_init()
+    e0.integrate(600.0, lambda t, k: None)
+    assert np.abs(e0.get_field("u") - ens[2].get_field("u")).max() > 1e-4 * np.abs(e0.get_field("u")).max()
+
+
 # ---- ensembles mixing member kinds ------------------------------------------------------------
 def implicit_member(n=32):
     """The disc benchmark under a laterally constant lithosphere: the implicit step (Euler,
