@@ -56,6 +56,11 @@ struct Gemm {
     long sAb1, sBb1, sCb1;
     int ones_row1;   // 1 + the row of A that reads 1.0 instead of memory (0: none; LDS-staged path)
     int batch;       // products in the launch (paired LDS-staged path)
+    // second K segment (LDS-staged path): K values [K1, K) read A2, B2 at k - K1 with A's and B's
+    // strides and batch offsets, so one product sums two (A B + A2 B2)
+    const void* A2;
+    const void* B2;
+    int K1;
 };
 
 __device__ __forceinline__ long boff(int b, int nb0, long s0, long s1) {
@@ -204,7 +209,7 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm(const Gemm g) {
 // weight gradient's product. PAIR (products of at most 32 rows, no split): the tile's upper and
 // lower 32 rows are two consecutive products of the batch (waves 0-1 and 2-3), so no wave idles
 // on rows past M (the x-transforms per row have 2 modes_x = 32 rows).
-template <typename TA, typename TB, bool PAIR = false>
+template <typename TA, typename TB, bool PAIR = false, bool SEG2 = false>
 __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
     constexpr int PS = 40;   // LDS pitch (BF16) of a row of 32 k: 80 bytes, 16-byte aligned rows
     constexpr int NBL = PAIR ? 8 : 4;   // B elements per thread and step
@@ -247,6 +252,12 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
         }
     }
     float* C = g.C + ((!PAIR && (g.nslice > 0 || g.kchunk > 0)) ? (long)b * g.sCb : boff(b, g.nb0, g.sCb, g.sCb1));
+    const TA* As2 = nullptr;   // SEG2: the second segment's operands
+    const TB* Bs2 = nullptr;
+    if constexpr (SEG2) {
+        As2 = (const TA*)g.A2 + (A - (const TA*)g.A);
+        Bs2 = (const TB*)g.B2 + (B - (const TB*)g.B);
+    }
     // this thread's elements of a K step: A e = tid + 256 j (j < 8), B e = tid + 256 j (j < NBL;
     // with PAIR, e >= 1024 is the second product's)
     const bool afk = g.sAk == 1, bfk = g.sBk == 1;
@@ -278,6 +289,12 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
                 const int h = arow[j] >> 5, m = arow[j] & 31, k = k0 + ak[j];
                 const TA* Ah = h ? A1 : A;
                 va[j] = (m < g.M && k < kend && (h == 0 || have1)) ? Ah[(long)m * g.sAm + (long)k * g.sAk] : (TA)0.f;
+            } else if constexpr (SEG2) {
+                const int m = mb + arow[j], k = k0 + ak[j];
+                const bool s2 = k >= g.K1;
+                const TA* Ap = s2 ? As2 : A;
+                const int kk = s2 ? k - g.K1 : k;
+                va[j] = (m < g.M && k < kend) ? Ap[(long)m * g.sAm + (long)kk * g.sAk] : (TA)0.f;
             } else {
                 const int m = mb + arow[j], k = k0 + ak[j];
                 va[j] = (m < g.M && k < kend)
@@ -291,6 +308,12 @@ __global__ void __launch_bounds__(GEMM_THREADS) k_gemm_s(const Gemm g) {
                 const int h = bcol[j] >> 5, n = n0 + (bcol[j] & 31), k = k0 + bk[j];
                 const TB* Bh = h ? B1 : B;
                 vb[j] = (n < g.N && k < kend && (h == 0 || have1)) ? Bh[(long)k * g.sBk + (long)n * g.sBn] : (TB)0.f;
+            } else if constexpr (SEG2) {
+                const int n = n0 + bcol[j], k = k0 + bk[j];
+                const bool s2 = k >= g.K1;
+                const TB* Bp = s2 ? Bs2 : B;
+                const int kk = s2 ? k - g.K1 : k;
+                vb[j] = (n < g.N && k < kend) ? Bp[(long)kk * g.sBk + (long)n * g.sBn] : (TB)0.f;
             } else {
                 const int n = n0 + bcol[j], k = k0 + bk[j];
                 vb[j] = (n < g.N && k < kend) ? B[(long)k * g.sBk + (long)n * g.sBn] : (TB)0.f;
@@ -724,6 +747,10 @@ struct Src {
 };
 
 void launch_gemm(fi_surrogate* s, const Gemm& g, dim3 grid, bool abf, bool bbf) {
+    if (g.A2) {   // two K segments (FP32 operands: the y transforms of the real and imaginary parts)
+        hipLaunchKernelGGL((k_gemm_s<float, float, false, true>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
+        return;
+    }
     if (g.sAk != 1 || g.sBk != 1 || g.ones_row1 > 0) {   // an operand strided in K: stage the tiles in LDS
         if (abf && bbf)
             hipLaunchKernelGGL((k_gemm_s<__bf16, __bf16>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
@@ -752,9 +779,17 @@ constexpr int NB0_NONE = 1 << 30;
 // K = nx per sample) is split along K so that the launch has about 1024 workgroups (the K loop is
 // load-latency bound), then reduced in a fixed order. Product b's operands sit at
 // X + (b mod nb0) sXb + (b / nb0) sXb1.
+//
+// With A2, B2 (FP32, the strides and batch offsets of A and B) the product is A B + A2 B2 over a K
+// range of 2 K values, in one launch (and one reduction when split).
 int gemm(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, float* C, long sCm, long sCn, int M,
          int N, int K, float beta, int batch = 1, long sAb = 0, long sBb = 0, long sCb = 0, int nb0 = NB0_NONE,
-         long sAb1 = 0, long sBb1 = 0, long sCb1 = 0) {
+         long sAb1 = 0, long sBb1 = 0, long sCb1 = 0, const float* A2 = nullptr, const float* B2 = nullptr) {
+    const int K1 = K;
+    if (A2) {
+        if (A.bf || B.bf || !B2) return sfail(s, S_ERR_INVALID, "two-segment product: FP32 operands only");
+        K = 2 * K1;
+    }
     const long tiles = (long)((M + 63) / 64) * ((N + 31) / 32);
     if (batch > 1 && tiles * batch < 1024 && K >= 128) {
         const int steps = (K + 31) / 32;
@@ -768,7 +803,7 @@ int gemm(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, 
             s->part_cap = need;
         }
         Gemm g{A.p, B.p, s->part.p, sAb, sAm, sAk, sBb, sBk, sBn, MN, N, 1, M, N, K, kchunk, 1.f, 0.f, nslice,
-               nb0, sAb1, sBb1, 0};
+               nb0, sAb1, sBb1, 0, 0, 0, A2, B2, K1};
         launch_gemm(s, g, dim3((M + 63) / 64, (N + 31) / 32, batch * nslice), A.bf, B.bf);
         SCHECK(s, hipGetLastError());
         const long outs = MN * batch;
@@ -778,8 +813,8 @@ int gemm(fi_surrogate* s, Src A, long sAm, long sAk, Src B, long sBk, long sBn, 
         return 0;
     }
     Gemm g{A.p, B.p, C, sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn, M, N, K, 0, 1.f, beta, 0, nb0, sAb1, sBb1, sCb1,
-           0, batch};
-    if (M <= 32 && batch >= 2 && (sAk != 1 || sBk != 1)) {   // two products per workgroup
+           0, batch, A2, B2, K1};
+    if (!A2 && M <= 32 && batch >= 2 && (sAk != 1 || sBk != 1)) {   // two products per workgroup
         const dim3 grid(1, (N + 31) / 32, (batch + 1) / 2);
         if (A.bf && B.bf)
             hipLaunchKernelGGL((k_gemm_s<__bf16, __bf16, true>), grid, dim3(GEMM_THREADS), 0, s->stream, g);
@@ -874,10 +909,7 @@ int forward(fi_surrogate* s, const float* x, int nb) {
             return st;
         // y-DFT per (sample, kx): G = Ar Re(Fx) + Ai Im(Fx)
         if ((st = gemm(s, s->Ar.p, ny, 1, s->Fx.p, (long)X2 * C, 1, G, C, 1, R4, C, ny, 0.f, nb * mx, 0, 2L * C,
-                       (long)R4 * C, mx, 0, sFs, sGs)))
-            return st;
-        if ((st = gemm(s, s->Ai.p, ny, 1, s->Fx.p + C, (long)X2 * C, 1, G, C, 1, R4, C, ny, 1.f, nb * mx, 0, 2L * C,
-                       (long)R4 * C, mx, 0, sFs, sGs)))
+                       (long)R4 * C, mx, 0, sFs, sGs, s->Ai.p, s->Fx.p + C)))
             return st;
         hipLaunchKernelGGL(k_mix_fwd, dim3((nb * nmode * C + 255) / 256), dim3(256), 0, s->stream, G, prm + s->oR[l],
                            s->H.p, nmode, C, nb);
@@ -1002,10 +1034,7 @@ int backward(fi_surrogate* s, const float* x, int nb) {
             return st;
         // dH = Ar dRe F' + Ai dIm F' per (sample, kx)
         if ((st = gemm(s, s->Ar.p, ny, 1, s->dFy.p, (long)X2 * C, 1, s->dH.p, C, 1, R4, C, ny, 0.f, nb * mx, 0, 2L * C,
-                       (long)R4 * C, mx, 0, sFs, sGs)))
-            return st;
-        if ((st = gemm(s, s->Ai.p, ny, 1, s->dFy.p + C, (long)X2 * C, 1, s->dH.p, C, 1, R4, C, ny, 1.f, nb * mx, 0,
-                       2L * C, (long)R4 * C, mx, 0, sFs, sGs)))
+                       (long)R4 * C, mx, 0, sFs, sGs, s->Ai.p, s->dFy.p + C)))
             return st;
         hipLaunchKernelGGL(k_mix_back_r, dim3((int)(((long)nmode * C * C + 255) / 256)), dim3(256), 0, s->stream,
                            s->dH.p, s->Gs[l].p, gr + s->oR[l], nmode, C, nb);
