@@ -545,6 +545,7 @@ __global__ void k_mix_fwd(const float* __restrict__ G, const float* __restrict__
     const float* g = G + (long)mode * 2 * C;
     const float* w = R + (long)(mode % nmode) * C * C * 2;
     float hr = 0.f, hi = 0.f;
+#pragma unroll 8
     for (int c = 0; c < C; ++c) {
         const float gr = g[c], gi = g[C + c];
         const float wr = w[(c * C + o) * 2], wi = w[(c * C + o) * 2 + 1];
@@ -564,6 +565,7 @@ __global__ void k_mix_back_g(const float* __restrict__ dH, const float* __restri
     const float* d = dH + (long)mode * 2 * C;
     const float* w = R + (long)(mode % nmode) * C * C * 2 + (long)c * C * 2;
     float gr = 0.f, gi = 0.f;
+#pragma unroll 8
     for (int o = 0; o < C; ++o) {
         const float dr = d[o], di = d[C + o];
         const float wr = w[2 * o], wi = w[2 * o + 1];
