@@ -620,6 +620,7 @@ __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, 
 __global__ void k_sum_parts(const double* __restrict__ part, int n, double* out) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double s = 0.0;
+#pragma unroll 16
         for (int i = 0; i < n; ++i) s += part[i];
         *out = s;
     }
